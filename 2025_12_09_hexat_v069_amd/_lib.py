@@ -1,0 +1,362 @@
+"""ctypes binding of libhexat.so (the C ABI declared in include/hexat.h).
+
+The library is the product: every numeric call below runs hand-written gfx950
+HIP kernels.  If the library is missing or no HIP device is visible the calls
+raise -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhexat.so")
+
+HX_OK = 0
+HX_ERR_NO_DEVICE = 1
+HX_ERR_BAD_ARG = 2
+HX_ERR_NOT_CONVERGED = 3
+HX_ERR_BREAKDOWN = 4
+HX_ERR_DIVERGED = 5
+HX_ERR_NO_ACTIVE = 6
+HX_ERR_HIP = 7
+HX_ERR_STATE = 8
+HX_ERR_OOM = 9
+HX_ZERO_RHS = 10
+
+HX_FP64, HX_FP32_IR = 0, 1
+HX_PC_JACOBI, HX_PC_LEARNED = 0, 1
+HX_MV_AUTO, HX_MV_PARITY, HX_MV_DENSE, HX_MV_GATHER = 0, 1, 2, 3
+HX_K_MATVEC, HX_K_UPDATE, HX_K_PUPDATE, HX_K_HELM, HX_K_ELEM, HX_K_MATVEC32 = range(6)
+HX_FIELD_X, HX_FIELD_XPHYS, HX_FIELD_U, HX_FIELD_VM, HX_FIELD_DC = range(5)
+
+STATUS_NAMES = {0: "ok", 1: "no device", 2: "bad argument", 3: "not converged", 4: "breakdown",
+                5: "diverged", 6: "no active element", 7: "HIP error", 8: "bad state", 9: "out of memory",
+                10: "zero rhs"}
+
+#: every symbol declared in include/hexat.h
+EXPORTS = [
+    "hx_version", "hx_device_count", "hx_ctx_create", "hx_ctx_destroy", "hx_last_error", "hx_grid_set",
+    "hx_set_fixed", "hx_element_stiffness", "hx_scalar_matrices", "hx_matvec", "hx_diagonal", "hx_pcg_solve",
+    "hx_block_pcg_solve", "hx_helmholtz_filter", "hx_element_stress", "hx_compliance_sens", "hx_oc_update",
+    "hx_to_init", "hx_to_step", "hx_to_read", "hx_learned_train", "hx_learned_weights", "hx_timing_enable",
+    "hx_timing_read", "hx_device_info",
+]
+
+
+class HexatError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class PcgOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("max_iter", C.c_int), ("shift", C.c_double), ("precision", C.c_int),
+                ("precond", C.c_int), ("warm_start", C.c_int), ("matvec", C.c_int), ("batch", C.c_int)]
+
+
+class PcgStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("status", C.c_int), ("rel_res", C.c_double), ("seconds", C.c_double),
+                ("refinements", C.c_int), ("fallbacks", C.c_int)]
+
+
+class FilterOpts(C.Structure):
+    _fields_ = [("quirk", C.c_int), ("tol", C.c_double), ("max_iter", C.c_int), ("check_every", C.c_int)]
+
+
+class FilterStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("status", C.c_int), ("rel_res", C.c_double), ("seconds", C.c_double)]
+
+
+class TOParams(C.Structure):
+    _fields_ = [("E0", C.c_double), ("Emin", C.c_double), ("nu", C.c_double), ("penal", C.c_double),
+                ("volfrac", C.c_double), ("move", C.c_double), ("xmin", C.c_double), ("rmin", C.c_double),
+                ("pcg_tol", C.c_double), ("pcg_max_iter", C.c_int), ("precision", C.c_int), ("precond", C.c_int),
+                ("filter_tol", C.c_double), ("filter_max_iter", C.c_int), ("oc_rel_tol", C.c_double),
+                ("with_stress", C.c_int)]
+
+
+class TOStats(C.Structure):
+    _fields_ = [("iter", C.c_int), ("compliance", C.c_double), ("change", C.c_double), ("volume", C.c_double),
+                ("pcg_iterations", C.c_int), ("pcg_rel_res", C.c_double), ("pcg_status", C.c_int),
+                ("filter_iterations", C.c_int), ("max_vm", C.c_double), ("seconds", C.c_double)]
+
+
+class LearnStats(C.Structure):
+    _fields_ = [("samples", C.c_int), ("epochs", C.c_int), ("lr", C.c_double), ("seed", C.c_int),
+                ("final_loss", C.c_double), ("seconds", C.c_double)]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """Load libhexat.so (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python {_HERE}/build.py`")
+        L = C.CDLL(LIB_PATH)
+        P, d, i = C.POINTER, C.c_double, C.c_int
+        pd = P(C.c_double)
+        vp = C.c_void_p
+        L.hx_version.restype = i
+        L.hx_device_count.argtypes = [P(C.c_int)]
+        L.hx_ctx_create.argtypes = [i, P(vp)]
+        L.hx_ctx_destroy.argtypes = [vp]
+        L.hx_ctx_destroy.restype = None
+        L.hx_last_error.argtypes = [vp]
+        L.hx_last_error.restype = C.c_char_p
+        L.hx_grid_set.argtypes = [vp, i, i, i, d, d, d, d]
+        L.hx_set_fixed.argtypes = [vp, P(C.c_uint8)]
+        L.hx_element_stiffness.argtypes = [d, d, d, d, d, pd]
+        L.hx_scalar_matrices.argtypes = [d, d, d, i, pd, pd]
+        L.hx_matvec.argtypes = [vp, pd, pd, pd, i]
+        L.hx_diagonal.argtypes = [vp, pd, pd]
+        L.hx_pcg_solve.argtypes = [vp, pd, pd, pd, P(PcgOpts), P(PcgStats)]
+        L.hx_block_pcg_solve.argtypes = [vp, i, pd, pd, pd, P(PcgOpts), P(PcgStats)]
+        L.hx_helmholtz_filter.argtypes = [vp, d, P(FilterOpts), pd, pd, P(FilterStats)]
+        L.hx_element_stress.argtypes = [vp, pd, pd, pd, pd, pd, pd, pd]
+        L.hx_compliance_sens.argtypes = [vp, pd, d, d, d, pd, pd, pd, pd, pd]
+        L.hx_oc_update.argtypes = [vp, pd, pd, pd, d, d, d, d, pd, pd, P(C.c_int), pd]
+        L.hx_to_init.argtypes = [vp, P(TOParams), pd, pd]
+        L.hx_to_step.argtypes = [vp, P(TOStats)]
+        L.hx_to_read.argtypes = [vp, i, pd]
+        L.hx_learned_train.argtypes = [vp, pd, pd, P(LearnStats)]
+        L.hx_learned_weights.argtypes = [vp, P(C.c_uint16), P(C.c_int)]
+        L.hx_timing_enable.argtypes = [vp, i]
+        L.hx_timing_read.argtypes = [vp, i, pd, P(C.c_long)]
+        L.hx_device_info.argtypes = [vp, P(C.c_int), P(C.c_int), pd]
+        for name in EXPORTS:
+            f = getattr(L, name)
+            if name not in ("hx_ctx_destroy", "hx_last_error"):
+                f.restype = C.c_int
+        _lib = L
+        return L
+
+
+def dptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    load().hx_device_count(C.byref(n))
+    return n.value
+
+
+def element_stiffness(dx, dy, dz, nu, E=1.0):
+    K = np.empty(576)
+    rc = load().hx_element_stiffness(float(dx), float(dy), float(dz), float(nu), float(E), dptr(K))
+    if rc:
+        raise HexatError(rc, "hx_element_stiffness")
+    return K.reshape(24, 24)
+
+
+def scalar_matrices(dx, dy, dz, quirk):
+    Ks, Ms = np.empty(64), np.empty(64)
+    rc = load().hx_scalar_matrices(float(dx), float(dy), float(dz), int(quirk), dptr(Ks), dptr(Ms))
+    if rc:
+        raise HexatError(rc, "hx_scalar_matrices")
+    return Ks.reshape(8, 8), Ms.reshape(8, 8)
+
+
+class Context:
+    """One HIP device context with a structured grid (libhexat hx_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.L = load()
+        h = C.c_void_p()
+        rc = self.L.hx_ctx_create(int(device), C.byref(h))
+        if rc:
+            raise HexatError(rc, "hx_ctx_create: no usable HIP device" if rc == HX_ERR_NO_DEVICE else "hx_ctx_create")
+        self.h = h
+        self.grid = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.hx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def check(self, rc, what, ok=(HX_OK,)):
+        if rc not in ok:
+            msg = self.L.hx_last_error(self.h)
+            raise HexatError(rc, f"{what}: {msg.decode() if msg else ''}")
+        return rc
+
+    # ---- grid -------------------------------------------------------------
+    def set_grid(self, nx, ny, nz, dx=1.0, dy=1.0, dz=1.0, nu=0.3):
+        self.check(self.L.hx_grid_set(self.h, int(nx), int(ny), int(nz), float(dx), float(dy), float(dz),
+                                      float(nu)), "hx_grid_set")
+        self.grid = (int(nx), int(ny), int(nz), float(dx), float(dy), float(dz), float(nu))
+        return self
+
+    @property
+    def ne(self):
+        nx, ny, nz = self.grid[:3]
+        return nx * ny * nz
+
+    @property
+    def nn(self):
+        nx, ny, nz = self.grid[:3]
+        return (nx + 1) * (ny + 1) * (nz + 1)
+
+    @property
+    def ndof(self):
+        return 3 * self.nn
+
+    def set_fixed(self, fixed):
+        if fixed is None:
+            self.check(self.L.hx_set_fixed(self.h, None), "hx_set_fixed")
+            return
+        m = np.ascontiguousarray(fixed, dtype=np.uint8).ravel()
+        if m.size != self.ndof:
+            raise ValueError("fixed mask must have 3*nNodes entries")
+        self._fixed = m
+        self.check(self.L.hx_set_fixed(self.h, m.ctypes.data_as(C.POINTER(C.c_uint8))), "hx_set_fixed")
+
+    # ---- elasticity ---------------------------------------------------------
+    def matvec(self, Ee, x, variant=HX_MV_AUTO):
+        Ee, x = f64(Ee, self.ne), f64(x, self.ndof)
+        y = np.empty(self.ndof)
+        self.check(self.L.hx_matvec(self.h, dptr(Ee), dptr(x), dptr(y), int(variant)), "hx_matvec")
+        return y
+
+    def diagonal(self, Ee):
+        Ee = f64(Ee, self.ne)
+        d = np.empty(self.ndof)
+        self.check(self.L.hx_diagonal(self.h, dptr(Ee), dptr(d)), "hx_diagonal")
+        return d
+
+    def pcg(self, Ee, f, u0=None, tol=1e-8, max_iter=100000, shift=0.0, precision=HX_FP64,
+            precond=HX_PC_JACOBI, matvec=HX_MV_AUTO, batch=0, raise_on_fail=False):
+        Ee, f = f64(Ee, self.ne), f64(f, self.ndof)
+        u = np.zeros(self.ndof) if u0 is None else f64(u0, self.ndof).copy()
+        o = PcgOpts(float(tol), int(max_iter), float(shift), int(precision), int(precond),
+                    0 if u0 is None else 1, int(matvec), int(batch))
+        s = PcgStats()
+        rc = self.L.hx_pcg_solve(self.h, dptr(Ee), dptr(f), dptr(u), C.byref(o), C.byref(s))
+        ok = (HX_OK,) if raise_on_fail else (HX_OK, HX_ERR_NOT_CONVERGED, HX_ERR_BREAKDOWN, HX_ERR_DIVERGED)
+        self.check(rc, "hx_pcg_solve", ok)
+        return u, s
+
+    def block_pcg(self, Ee, F, tol=1e-8, max_iter=100000, shift=0.0, precision=HX_FP64, matvec=HX_MV_AUTO):
+        Ee = f64(Ee, self.ne)
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        nrhs = F.shape[0]
+        if F.shape != (nrhs, self.ndof):
+            raise ValueError("F must be nrhs x ndof")
+        U = np.zeros_like(F)
+        o = PcgOpts(float(tol), int(max_iter), float(shift), int(precision), HX_PC_JACOBI, 0, int(matvec), 0)
+        S = (PcgStats * nrhs)()
+        rc = self.L.hx_block_pcg_solve(self.h, nrhs, dptr(Ee), dptr(F), dptr(U), C.byref(o), S)
+        self.check(rc, "hx_block_pcg_solve", (HX_OK, HX_ERR_NOT_CONVERGED))
+        return U, list(S)
+
+    # ---- filter ---------------------------------------------------------------
+    def helmholtz(self, R, rho, quirk=0, tol=1e-5, max_iter=200, check_every=10):
+        rho = f64(rho, self.ne)
+        out = np.empty(self.ne)
+        o = FilterOpts(int(quirk), float(tol), int(max_iter), int(check_every))
+        s = FilterStats()
+        self.check(self.L.hx_helmholtz_filter(self.h, float(R), C.byref(o), dptr(rho), dptr(out), C.byref(s)),
+                   "hx_helmholtz_filter")
+        return out, s
+
+    # ---- element post ----------------------------------------------------------
+    def element_stress(self, Ee, u, principal=True, vm=True, voigt=True, l1=True, direction=True):
+        Ee, u = f64(Ee, self.ne), f64(u, self.ndof)
+        ne = self.ne
+        out = {"principal": np.empty((ne, 3)) if principal else None, "vm": np.empty(ne) if vm else None,
+               "voigt": np.empty((ne, 6)) if voigt else None, "l1": np.empty(ne) if l1 else None,
+               "dir": np.empty((ne, 3)) if direction else None}
+        self.check(self.L.hx_element_stress(self.h, dptr(Ee), dptr(u), dptr(out["principal"]), dptr(out["vm"]),
+                                            dptr(out["voigt"]), dptr(out["l1"]), dptr(out["dir"])),
+                   "hx_element_stress")
+        return out
+
+    def compliance_sens(self, u, xphys=None, penal=3.0, E0=1.0, Emin=1e-9, f=None):
+        u = f64(u, self.ndof)
+        xp = None if xphys is None else f64(xphys, self.ne)
+        ff = None if f is None else f64(f, self.ndof)
+        ce = np.empty(self.ne)
+        dc = np.empty(self.ne) if xp is not None else None
+        comp = np.zeros(1) if ff is not None else None
+        self.check(self.L.hx_compliance_sens(self.h, dptr(xp), float(penal), float(E0), float(Emin), dptr(u),
+                                             dptr(ff), dptr(ce), dptr(dc), dptr(comp)), "hx_compliance_sens")
+        return ce, dc, (None if comp is None else float(comp[0]))
+
+    def oc_update(self, x, dc, dv, volfrac, move=0.2, xmin=0.0, rel_tol=1e-3):
+        x, dc, dv = f64(x, self.ne), f64(dc, self.ne), f64(dv, self.ne)
+        xn = np.empty(self.ne)
+        lm, ch = C.c_double(0), C.c_double(0)
+        st = C.c_int(0)
+        self.check(self.L.hx_oc_update(self.h, dptr(x), dptr(dc), dptr(dv), float(volfrac), float(move),
+                                       float(xmin), float(rel_tol), dptr(xn), C.byref(lm), C.byref(st),
+                                       C.byref(ch)), "hx_oc_update")
+        return xn, lm.value, st.value, ch.value
+
+    # ---- device-resident SIMP loop ------------------------------------------------
+    def to_init(self, params: TOParams, f, x0):
+        f, x0 = f64(f, self.ndof), f64(x0, self.ne)
+        self.check(self.L.hx_to_init(self.h, C.byref(params), dptr(f), dptr(x0)), "hx_to_init")
+
+    def to_step(self) -> TOStats:
+        s = TOStats()
+        self.check(self.L.hx_to_step(self.h, C.byref(s)), "hx_to_step")
+        return s
+
+    def to_read(self, field):
+        n = self.ndof if field == HX_FIELD_U else self.ne
+        out = np.empty(n)
+        self.check(self.L.hx_to_read(self.h, int(field), dptr(out)), "hx_to_read")
+        return out
+
+    # ---- learned preconditioner -----------------------------------------------------
+    def learned_train(self, Ee, f, samples=64, epochs=4, lr=1e-3, seed=0):
+        Ee, f = f64(Ee, self.ne), f64(f, self.ndof)
+        s = LearnStats(int(samples), int(epochs), float(lr), int(seed), 0.0, 0.0)
+        self.check(self.L.hx_learned_train(self.h, dptr(Ee), dptr(f), C.byref(s)), "hx_learned_train")
+        return s
+
+    # ---- timing -----------------------------------------------------------------------
+    def timing(self, on: bool):
+        self.check(self.L.hx_timing_enable(self.h, 1 if on else 0), "hx_timing_enable")
+
+    def timing_read(self, kernel):
+        ms, cnt = C.c_double(0), C.c_long(0)
+        self.check(self.L.hx_timing_read(self.h, int(kernel), C.byref(ms), C.byref(cnt)), "hx_timing_read")
+        return ms.value, cnt.value
+
+    def device_info(self):
+        cu, clk = C.c_int(0), C.c_int(0)
+        mem = C.c_double(0)
+        self.check(self.L.hx_device_info(self.h, C.byref(cu), C.byref(clk), C.byref(mem)), "hx_device_info")
+        return {"cu": cu.value, "clock_khz": clk.value, "mem_gb": mem.value}

@@ -1,0 +1,949 @@
+// hx_api.hip -- C ABI of libhexat.so: context, device workspace, solve
+// drivers (FP64 Jacobi-PCG, FP32 PCG with FP64 reliable updates, Helmholtz
+// filter), element post-processing, OC and the device-resident SIMP loop.
+//
+// The drivers keep every vector and scalar of a solve in HBM.  The host
+// launches iterations in batches and reads the 128-byte PcgState back once
+// per batch to decide whether to stop; kernels of an already finished solve
+// return at once, so a batch may overshoot by a few empty launches only.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hexat.h"
+#include "hx_launch.h"
+
+namespace hx {
+void element_stiffness(double dx, double dy, double dz, double nu, double E, double* Ke);
+void scalar_matrices(double dx, double dy, double dz, int quirk, double* Ks, double* Ms);
+}  // namespace hx
+
+using namespace hx;
+
+namespace {
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t m) {
+        if (m <= n && p) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc((void**)&p, std::max<size_t>(m, 1) * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            n = 0;
+            return e;
+        }
+        n = m;
+        return hipSuccess;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct TimerPair {
+    hipEvent_t a, b;
+    int cls;
+};
+
+constexpr double kIrDelta = 0.05;  // reliable update when ||r|| drops by this factor
+
+}  // namespace
+
+struct hx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool has_grid = false;
+    GridArgs g{};
+    double nu = 0.3;
+    double Ke[576];
+    double Kbin[576];
+    ParityK<double> kp64{};
+    ParityK<float> kp32{};
+    double kd[3] = {0, 0, 0};
+    bool has_fixed = false;
+    int mv_variant = HX_MV_PARITY;
+    int tile = 0;
+    int kchunk = 16;
+
+    DevBuf<uint8_t> fixed;
+    DevBuf<double> E, f, x, r, p, q, minv, Kbin_d;
+    DevBuf<float> Kbin32_d, E32, x32, r32, p32, q32, minv32;
+    DevBuf<double> partials;
+    PcgState* st = nullptr;
+    PcgState* st_host = nullptr;
+    DevBuf<double> h_b, h_x, h_r, h_p, h_q, h_minv, h_in, h_out;
+    DevBuf<double> e_pr, e_vm, e_sv, e_l1, e_dir, e_ce, e_dc, e_aux, small;
+    // SIMP session
+    bool to_ready = false;
+    hx_to_params tp{};
+    int to_iter = 0;
+    DevBuf<double> t_x, t_xphys, t_xnew, t_dc, t_dcf, t_dv, t_vm, t_ones;
+    // timing
+    bool timing = false;
+    std::vector<TimerPair> pending;
+    std::vector<hipEvent_t> free_events;
+    double acc_ms[HX_K_COUNT] = {0};
+    long cnt[HX_K_COUNT] = {0};
+};
+
+namespace {
+
+int fail(hx_ctx* c, int code, const std::string& m) {
+    if (c) c->err = m;
+    return code;
+}
+
+#define HXC(call)                                                                                     \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess) return fail(c, HX_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define HXA(buf, n)                                                                                   \
+    do {                                                                                              \
+        hipError_t e_ = (buf).ensure(n);                                                              \
+        if (e_ != hipSuccess) return fail(c, HX_ERR_OOM, std::string("allocating " #buf ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+#define HXK()                                                                                         \
+    do {                                                                                              \
+        hipError_t e_ = hipGetLastError();                                                            \
+        if (e_ != hipSuccess) return fail(c, HX_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+hipEvent_t get_event(hx_ctx* c) {
+    if (!c->free_events.empty()) {
+        hipEvent_t e = c->free_events.back();
+        c->free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+// Brackets one launch with events when timing is on.
+struct TimeScope {
+    hx_ctx* c;
+    int cls;
+    hipEvent_t a = nullptr;
+    TimeScope(hx_ctx* c_, int cls_) : c(c_), cls(cls_) {
+        if (c->timing) {
+            a = get_event(c);
+            (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~TimeScope() {
+        if (a) {
+            hipEvent_t b = get_event(c);
+            (void)hipEventRecord(b, c->stream);
+            c->pending.push_back({a, b, cls});
+        }
+    }
+};
+
+void flush_timing(hx_ctx* c) {
+    for (auto& t : c->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            c->acc_ms[t.cls] += ms;
+            c->cnt[t.cls] += 1;
+        }
+        c->free_events.push_back(t.a);
+        c->free_events.push_back(t.b);
+    }
+    c->pending.clear();
+}
+
+int sync_state(hx_ctx* c) {
+    HXC(hipMemcpyAsync(c->st_host, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, c->stream));
+    HXC(hipStreamSynchronize(c->stream));
+    flush_timing(c);
+    return HX_OK;
+}
+
+int status_to_code(int s) {
+    switch (s) {
+        case 0: return HX_OK;
+        case 3: return HX_ERR_NOT_CONVERGED;
+        case 4: return HX_ERR_BREAKDOWN;
+        case 5: return HX_ERR_DIVERGED;
+        case 10: return HX_OK;
+        default: return HX_ERR_STATE;
+    }
+}
+
+long ndof(const hx_ctx* c) { return 3 * c->g.nn; }
+
+// y = K(E) x (+ shift x); MV_PCG mode also reduces p'q into the state.
+int mv64(hx_ctx* c, const double* x, double* y, const double* E, double shift, int mode, int variant) {
+    MvArgs<double> a{x, y, E, shift, mode, c->st, c->partials.p};
+    const int v = variant == HX_MV_AUTO ? c->mv_variant : variant;
+    TimeScope ts(c, HX_K_MATVEC);
+    if (v == HX_MV_DENSE) launch_matvec_dense<double>(c->g, c->Kbin_d.p, a, c->kchunk, c->stream);
+    else if (v == HX_MV_GATHER) launch_matvec_gather(c->g, c->Kbin_d.p, a, c->stream);
+    else launch_matvec_parity<double>(c->g, c->kp64, a, c->tile, c->kchunk, c->stream);
+    return HX_OK;
+}
+
+int mv32(hx_ctx* c, const float* x, float* y, const float* E, double shift, int mode, int variant) {
+    MvArgs<float> a{x, y, E, (float)shift, mode, c->st, c->partials.p};
+    TimeScope ts(c, HX_K_MATVEC32);
+    if (variant == HX_MV_DENSE) launch_matvec_dense<float>(c->g, c->Kbin32_d.p, a, c->kchunk, c->stream);
+    else launch_matvec_parity<float>(c->g, c->kp32, a, c->tile, c->kchunk, c->stream);
+    return HX_OK;
+}
+
+int ensure_grid(hx_ctx* c) {
+    if (!c->has_grid) return fail(c, HX_ERR_STATE, "no grid set (call hx_grid_set first)");
+    return HX_OK;
+}
+
+// ---------------------------------------------------------------- FP64 PCG
+int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
+    const long n = ndof(c);
+    const int batch = o.batch > 0 ? o.batch : (n < 3000000 ? 64 : 16);
+    launch_elastic_minv<double>(c->g, c->kd, c->E.p, c->has_fixed ? c->fixed.p : nullptr, o.shift, c->minv.p,
+                                c->stream);
+    launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream);
+    if (o.warm_start) launch_mask<double>(n, c->minv.p, c->x.p, c->stream);
+    else launch_fill(n, 0.0, c->x.p, c->stream);
+    mv64(c, c->x.p, c->q.p, c->E.p, 0.0, MV_PLAIN, o.matvec);
+    launch_pcg_init<double>(n, c->f.p, c->x.p, c->q.p, c->minv.p, c->r.p, c->p.p, o.shift, c->st, c->partials.p,
+                            c->stream);
+    HXK();
+    int rc = sync_state(c);
+    if (rc) return rc;
+    while (!c->st_host->done) {
+        for (int b = 0; b < batch; ++b) {
+            mv64(c, c->p.p, c->q.p, c->E.p, o.shift, MV_PCG, o.matvec);
+            {
+                TimeScope ts(c, HX_K_UPDATE);
+                launch_pcg_update<double>(n, c->x.p, c->p.p, c->r.p, c->q.p, c->minv.p, c->st, c->partials.p,
+                                          c->stream);
+            }
+            {
+                TimeScope ts(c, HX_K_PUPDATE);
+                launch_pcg_pupdate<double>(n, c->r.p, c->minv.p, c->p.p, c->st, c->stream);
+            }
+        }
+        HXK();
+        if ((rc = sync_state(c))) return rc;
+    }
+    const int status = c->st_host->status, iters = c->st_host->iter;
+    // true residual of the returned iterate (FP64)
+    mv64(c, c->x.p, c->q.p, c->E.p, 0.0, MV_PLAIN, o.matvec);
+    launch_true_residual(n, c->f.p, c->q.p, c->x.p, c->minv.p, o.shift, nullptr, nullptr, c->st, c->partials.p, 1,
+                         c->stream);
+    HXK();
+    if ((rc = sync_state(c))) return rc;
+    if (s) {
+        s->iterations = iters;
+        s->status = status == 10 ? HX_ZERO_RHS : status_to_code(status);
+        s->rel_res = c->st_host->rel;
+        s->refinements = 0;
+        s->fallbacks = 0;
+    }
+    return status_to_code(status);
+}
+
+// ------------------------------------------- FP32 PCG + FP64 reliable updates
+int ensure_fp32(hx_ctx* c) {
+    const long n = ndof(c);
+    HXA(c->E32, c->g.ne);
+    HXA(c->x32, n);
+    HXA(c->r32, n);
+    HXA(c->p32, n);
+    HXA(c->q32, n);
+    HXA(c->minv32, n);
+    return HX_OK;
+}
+
+int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
+    const long n = ndof(c);
+    int rc = ensure_fp32(c);
+    if (rc) return rc;
+    const int batch = o.batch > 0 ? o.batch : (n < 3000000 ? 64 : 16);
+    const int v32 = (o.matvec == HX_MV_DENSE) ? HX_MV_DENSE : HX_MV_PARITY;
+    launch_convert<double, float>(c->g.ne, c->E.p, c->E32.p, c->stream);
+    launch_elastic_minv<double>(c->g, c->kd, c->E.p, c->has_fixed ? c->fixed.p : nullptr, o.shift, c->minv.p,
+                                c->stream);
+    launch_convert<double, float>(n, c->minv.p, c->minv32.p, c->stream);
+    launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream);
+    if (o.warm_start) launch_mask<double>(n, c->minv.p, c->x.p, c->stream);
+    else launch_fill(n, 0.0, c->x.p, c->stream);
+    launch_fill32(n, 0.0f, c->x32.p, c->stream);
+    launch_fill32(n, 0.0f, c->p32.p, c->stream);
+    mv64(c, c->x.p, c->q.p, c->E.p, 0.0, MV_PLAIN, HX_MV_PARITY);
+    launch_true_residual(n, c->f.p, c->q.p, c->x.p, c->minv.p, o.shift, nullptr, c->r32.p, c->st, c->partials.p, 3,
+                         c->stream);
+    HXK();
+    if ((rc = sync_state(c))) return rc;
+    double rel_true = c->st_host->rel;
+    int refinements = 0, status = 0;
+    const bool zero_rhs = c->st_host->normb == 0.0;
+    if (!zero_rhs && !(rel_true < o.tol)) {
+        launch_ir_resume(c->st, std::max(o.tol, kIrDelta * rel_true), 1, c->stream);
+        launch_pcg_pupdate<float>(n, c->r32.p, c->minv32.p, c->p32.p, c->st, c->stream);
+        HXK();
+        if ((rc = sync_state(c))) return rc;
+        for (;;) {
+            while (!c->st_host->done) {
+                for (int b = 0; b < batch; ++b) {
+                    mv32(c, c->p32.p, c->q32.p, c->E32.p, o.shift, MV_PCG, v32);
+                    {
+                        TimeScope ts(c, HX_K_UPDATE);
+                        launch_pcg_update<float>(n, c->x32.p, c->p32.p, c->r32.p, c->q32.p, c->minv32.p, c->st,
+                                                 c->partials.p, c->stream);
+                    }
+                    {
+                        TimeScope ts(c, HX_K_PUPDATE);
+                        launch_pcg_pupdate<float>(n, c->r32.p, c->minv32.p, c->p32.p, c->st, c->stream);
+                    }
+                }
+                HXK();
+                if ((rc = sync_state(c))) return rc;
+            }
+            status = c->st_host->status;
+            // reliable update: fold the FP32 correction into x and replace r
+            launch_accumulate(n, c->x.p, c->x32.p, c->stream);
+            mv64(c, c->x.p, c->q.p, c->E.p, 0.0, MV_PLAIN, HX_MV_PARITY);
+            launch_true_residual(n, c->f.p, c->q.p, c->x.p, c->minv.p, o.shift, nullptr, c->r32.p, c->st,
+                                 c->partials.p, 2, c->stream);
+            HXK();
+            if ((rc = sync_state(c))) return rc;
+            ++refinements;
+            rel_true = c->st_host->rel;
+            if (rel_true < o.tol) {
+                status = 0;
+                break;
+            }
+            if (status == 4 || status == 5) break;
+            if (c->st_host->iter >= o.max_iter) {
+                status = 3;
+                break;
+            }
+            launch_ir_resume(c->st, std::max(o.tol, kIrDelta * rel_true), 0, c->stream);
+            launch_pcg_pupdate<float>(n, c->r32.p, c->minv32.p, c->p32.p, c->st, c->stream);
+            HXK();
+            if ((rc = sync_state(c))) return rc;
+        }
+    }
+    if (s) {
+        s->iterations = c->st_host->iter;
+        s->status = zero_rhs ? HX_ZERO_RHS : status_to_code(status);
+        s->rel_res = rel_true;
+        s->refinements = refinements;
+        s->fallbacks = 0;
+    }
+    return zero_rhs ? HX_OK : status_to_code(status);
+}
+
+// ------------------------------------------------------------- Helmholtz
+int helm_solve(hx_ctx* c, double R, const hx_filter_opts& o, const double* in_dev, double* out_dev,
+               hx_filter_stats* s) {
+    const long nn = c->g.nn;
+    HXA(c->h_b, nn);
+    HXA(c->h_x, nn);
+    HXA(c->h_r, nn);
+    HXA(c->h_p, nn);
+    HXA(c->h_q, nn);
+    HXA(c->h_minv, nn);
+    double Ks[64], Ms[64];
+    scalar_matrices(c->g.dx, c->g.dy, c->g.dz, o.quirk, Ks, Ms);
+    HelmArgs A;
+    for (int t = 0; t < 64; ++t) A.Ae[t] = R * R * Ks[t] + Ms[t];  // GPUHelmholtz.jl:128-129
+    const double vol8 = c->g.dx * c->g.dy * c->g.dz / 8.0;
+    launch_helm_rhs(c->g, in_dev, vol8, c->h_b.p, c->stream);
+    launch_helm_minv(c->g, A, c->h_minv.p, c->stream);
+    launch_pcg_reset(c->st, o.tol, 0.0, o.max_iter > 0 ? o.max_iter : 200, o.check_every, c->stream);
+    launch_fill(nn, 0.0, c->h_x.p, c->stream);
+    launch_fill(nn, 0.0, c->h_q.p, c->stream);
+    launch_pcg_init<double>(nn, c->h_b.p, c->h_x.p, c->h_q.p, c->h_minv.p, c->h_r.p, c->h_p.p, 0.0, c->st,
+                            c->partials.p, c->stream);
+    HXK();
+    int rc = sync_state(c);
+    if (rc) return rc;
+    if (c->st_host->status == 10) {  // GPUHelmholtz.jl:179-182: zero field returned as is
+        HXC(hipMemcpyAsync(out_dev, in_dev, c->g.ne * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        if (s) {
+            s->iterations = 0;
+            s->status = HX_OK;
+            s->rel_res = 0.0;
+        }
+        return HX_OK;
+    }
+    const int batch = 16;
+    while (!c->st_host->done) {
+        for (int b = 0; b < batch; ++b) {
+            {
+                TimeScope ts(c, HX_K_HELM);
+                launch_helm_matvec(c->g, A, c->h_p.p, c->h_q.p, MV_PCG, c->st, c->partials.p, c->stream);
+            }
+            launch_pcg_update<double>(nn, c->h_x.p, c->h_p.p, c->h_r.p, c->h_q.p, c->h_minv.p, c->st, c->partials.p,
+                                      c->stream);
+            launch_pcg_pupdate<double>(nn, c->h_r.p, c->h_minv.p, c->h_p.p, c->st, c->stream);
+        }
+        HXK();
+        if ((rc = sync_state(c))) return rc;
+    }
+    launch_helm_extract(c->g, c->h_x.p, out_dev, c->stream);
+    HXK();
+    if ((rc = sync_state(c))) return rc;
+    const int status = c->st_host->status;
+    if (s) {
+        s->iterations = c->st_host->iter;
+        s->status = status == 3 ? HX_OK : status_to_code(status);  // the reference stops at max_iter silently
+        s->rel_res = c->st_host->rel;
+    }
+    return (status == 4 || status == 5) ? status_to_code(status) : HX_OK;
+}
+
+int elem_post(hx_ctx* c, const ElemArgs& a) {
+    ParityK<double> k = c->kp64;
+    TimeScope ts(c, HX_K_ELEM);
+    launch_element_post(c->g, k, a, c->stream);
+    return HX_OK;
+}
+
+void material_consts(double nu, double* l1, double* l2, double* mu) {
+    const double f = 1.0 / ((1.0 + nu) * (1.0 - 2.0 * nu));  // Element.jl:45-62 with E = 1
+    *l1 = (1.0 - nu) * f;
+    *l2 = nu * f;
+    *mu = 0.5 * (1.0 - 2.0 * nu) * f;
+}
+
+int oc_run(hx_ctx* c, long ne, const double* x, const double* dc, const double* dv, double volfrac, double move,
+           double xmin, double rel_tol, double* xnew) {
+    launch_oc_init(c->st, c->stream);
+    int rc;
+    for (int chunk = 0; chunk < 13; ++chunk) {
+        for (int k = 0; k < 16; ++k)
+            launch_oc_eval(ne, x, dc, dv, volfrac, move, xmin, rel_tol, c->st, c->partials.p, c->stream);
+        HXK();
+        if ((rc = sync_state(c))) return rc;
+        if (c->st_host->done) break;
+    }
+    launch_oc_apply(ne, x, dc, dv, move, xmin, xnew, c->st, c->partials.p, c->stream);
+    HXK();
+    return sync_state(c);
+}
+
+int upload(hx_ctx* c, void* dst, const void* src, size_t bytes) {
+    HXC(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return HX_OK;
+}
+int download(hx_ctx* c, void* dst, const void* src, size_t bytes) {
+    HXC(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HXC(hipStreamSynchronize(c->stream));
+    return HX_OK;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+int hx_version(void) { return 10; }
+
+int hx_device_count(int* count) {
+    if (!count) return HX_ERR_BAD_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return HX_OK;
+}
+
+int hx_ctx_create(int device, hx_ctx** out) {
+    if (!out) return HX_ERR_BAD_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return HX_ERR_NO_DEVICE;
+    if (device < 0 || device >= n) return HX_ERR_BAD_ARG;
+    if (hipSetDevice(device) != hipSuccess) return HX_ERR_HIP;
+    hx_ctx* c = new hx_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void**)&c->st, sizeof(PcgState)) != hipSuccess ||
+        hipHostMalloc((void**)&c->st_host, sizeof(PcgState), hipHostMallocDefault) != hipSuccess) {
+        hx_ctx_destroy(c);
+        return HX_ERR_HIP;
+    }
+    std::memset(c->st_host, 0, sizeof(PcgState));
+    *out = c;
+    return HX_OK;
+}
+
+void hx_ctx_destroy(hx_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    flush_timing(c);
+    for (auto e : c->free_events) (void)hipEventDestroy(e);
+    DevBuf<double>* d[] = {&c->E, &c->f, &c->x, &c->r, &c->p, &c->q, &c->minv, &c->Kbin_d, &c->partials,
+                           &c->h_b, &c->h_x, &c->h_r, &c->h_p, &c->h_q, &c->h_minv, &c->h_in, &c->h_out,
+                           &c->e_pr, &c->e_vm, &c->e_sv, &c->e_l1, &c->e_dir, &c->e_ce, &c->e_dc, &c->e_aux,
+                           &c->small, &c->t_x, &c->t_xphys, &c->t_xnew, &c->t_dc, &c->t_dcf, &c->t_dv, &c->t_vm,
+                           &c->t_ones};
+    for (auto b : d) b->release();
+    DevBuf<float>* fl[] = {&c->Kbin32_d, &c->E32, &c->x32, &c->r32, &c->p32, &c->q32, &c->minv32};
+    for (auto b : fl) b->release();
+    c->fixed.release();
+    if (c->st) (void)hipFree(c->st);
+    if (c->st_host) (void)hipHostFree(c->st_host);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* hx_last_error(const hx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int hx_element_stiffness(double dx, double dy, double dz, double nu, double E, double* Ke) {
+    if (!Ke || !(dx > 0) || !(dy > 0) || !(dz > 0) || !(nu > -1.0 && nu < 0.5)) return HX_ERR_BAD_ARG;
+    element_stiffness(dx, dy, dz, nu, E, Ke);
+    return HX_OK;
+}
+
+int hx_scalar_matrices(double dx, double dy, double dz, int quirk, double* Ks, double* Ms) {
+    if (!Ks || !Ms || !(dx > 0) || !(dy > 0) || !(dz > 0)) return HX_ERR_BAD_ARG;
+    scalar_matrices(dx, dy, dz, quirk, Ks, Ms);
+    return HX_OK;
+}
+
+int hx_grid_set(hx_ctx* c, int nx, int ny, int nz, double dx, double dy, double dz, double nu) {
+    if (!c) return HX_ERR_BAD_ARG;
+    if (nx < 1 || ny < 1 || nz < 1 || !(dx > 0) || !(dy > 0) || !(dz > 0) || !(nu > -1.0 && nu < 0.5))
+        return fail(c, HX_ERR_BAD_ARG, "invalid grid");
+    const long NX = nx + 1, NY = ny + 1, NZ = nz + 1;
+    if (3 * NX * NY * NZ > (long)INT32_MAX) return fail(c, HX_ERR_BAD_ARG, "grid too large");
+    (void)hipSetDevice(c->device);
+    GridArgs g{nx, ny, nz, (int)NX, (int)NY, (int)NZ, NX * NY * NZ, (long)nx * ny * nz, dx, dy, dz};
+    c->g = g;
+    c->nu = nu;
+    element_stiffness(dx, dy, dz, nu, 1.0, c->Ke);
+    // binary local order and parity-basis image
+    for (int a = 0; a < 8; ++a)
+        for (int b = 0; b < 8; ++b)
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    c->Kbin[24 * (3 * kRefToBin[a] + i) + 3 * kRefToBin[b] + j] = c->Ke[24 * (3 * a + i) + 3 * b + j];
+    auto V = [](int L, int s) {  // prod over axes in s of xi(L)
+        double v = 1.0;
+        for (int ax = 0; ax < 3; ++ax)
+            if (s & (1 << ax)) v *= ((L >> ax) & 1) ? 1.0 : -1.0;
+        return v;
+    };
+    double Kp[24][24];
+    double kmax = 0.0;
+    for (int s = 0; s < 8; ++s)
+        for (int ci = 0; ci < 3; ++ci)
+            for (int t = 0; t < 8; ++t)
+                for (int di = 0; di < 3; ++di) {
+                    double acc = 0.0;
+                    for (int L = 0; L < 8; ++L)
+                        for (int M = 0; M < 8; ++M)
+                            acc += V(L, s) * V(M, t) * c->Kbin[24 * (3 * L + ci) + 3 * M + di];
+                    Kp[3 * s + ci][3 * t + di] = acc / 64.0;
+                    kmax = std::max(kmax, std::fabs(acc / 64.0));
+                }
+    bool inpat[24][24] = {};
+    for (int t = 0; t < kParityNnz; ++t) {
+        inpat[kParityPattern[t].r][kParityPattern[t].c] = true;
+        c->kp64.v[t] = Kp[kParityPattern[t].r][kParityPattern[t].c];
+        c->kp32.v[t] = (float)c->kp64.v[t];
+    }
+    for (int i = 0; i < 24; ++i)
+        for (int j = 0; j < 24; ++j)
+            if (!inpat[i][j] && std::fabs(Kp[i][j]) > 1e-12 * kmax)
+                return fail(c, HX_ERR_STATE, "element stiffness outside the parity pattern");
+    for (int d = 0; d < 3; ++d) c->kd[d] = c->Ke[24 * d + d];
+    for (int a = 1; a < 8; ++a)
+        for (int d = 0; d < 3; ++d)
+            if (std::fabs(c->Ke[24 * (3 * a + d) + 3 * a + d] - c->kd[d]) > 1e-12 * std::fabs(c->kd[d]))
+                return fail(c, HX_ERR_STATE, "unequal element diagonal");
+    const long n = 3 * g.nn;
+    HXA(c->E, g.ne);
+    HXA(c->f, n);
+    HXA(c->x, n);
+    HXA(c->r, n);
+    HXA(c->p, n);
+    HXA(c->q, n);
+    HXA(c->minv, n);
+    HXA(c->Kbin_d, 576);
+    HXA(c->Kbin32_d, 576);
+    const long np = std::max<long>({3L * kVecBlocks + 8, (long)matvec_parity_blocks(g, 0, 4) + 8,
+                                    (long)matvec_parity_blocks(g, 1, 4) + 8, (long)matvec_gather_blocks(g) + 8,
+                                    (long)helm_matvec_blocks(g) + 8, (long)element_post_blocks(g) + 8});
+    HXA(c->partials, np);
+    HXA(c->small, 64);
+    float kb32[576];
+    for (int t = 0; t < 576; ++t) kb32[t] = (float)c->Kbin[t];
+    HXC(hipMemcpy(c->Kbin_d.p, c->Kbin, 576 * sizeof(double), hipMemcpyHostToDevice));
+    HXC(hipMemcpy(c->Kbin32_d.p, kb32, 576 * sizeof(float), hipMemcpyHostToDevice));
+    // default: everything free
+    HXA(c->fixed, n);
+    HXC(hipMemsetAsync(c->fixed.p, 0, n, c->stream));
+    launch_fill(n, 0.0, c->x.p, c->stream);
+    launch_fill(n, 0.0, c->p.p, c->stream);
+    HXC(hipStreamSynchronize(c->stream));
+    c->has_fixed = false;
+    c->has_grid = true;
+    c->to_ready = false;
+    return HX_OK;
+}
+
+int hx_set_fixed(hx_ctx* c, const uint8_t* fixed) {
+    if (!c) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    const long n = ndof(c);
+    if (!fixed) {
+        HXC(hipMemsetAsync(c->fixed.p, 0, n, c->stream));
+        c->has_fixed = false;
+    } else {
+        HXC(hipMemcpyAsync(c->fixed.p, fixed, n, hipMemcpyHostToDevice, c->stream));
+        c->has_fixed = true;
+    }
+    HXC(hipStreamSynchronize(c->stream));
+    return HX_OK;
+}
+
+int hx_matvec(hx_ctx* c, const double* Ee, const double* x, double* y, int variant) {
+    if (!c || !Ee || !x || !y) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    if (variant < HX_MV_AUTO || variant > HX_MV_GATHER) return fail(c, HX_ERR_BAD_ARG, "matvec variant");
+    const long n = ndof(c);
+    int rc;
+    if ((rc = upload(c, c->E.p, Ee, c->g.ne * sizeof(double))) || (rc = upload(c, c->p.p, x, n * sizeof(double))))
+        return rc;
+    mv64(c, c->p.p, c->q.p, c->E.p, 0.0, MV_PLAIN, variant);
+    HXK();
+    return download(c, y, c->q.p, n * sizeof(double));
+}
+
+int hx_diagonal(hx_ctx* c, const double* Ee, double* diag) {
+    if (!c || !Ee || !diag) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    int rc = upload(c, c->E.p, Ee, c->g.ne * sizeof(double));
+    if (rc) return rc;
+    launch_elastic_diag(c->g, c->kd, c->E.p, c->q.p, c->stream);
+    HXK();
+    return download(c, diag, c->q.p, ndof(c) * sizeof(double));
+}
+
+static int pcg_dispatch(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
+    hipEvent_t a = get_event(c), b = get_event(c);
+    (void)hipEventRecord(a, c->stream);
+    const int rc = (o.precision == HX_FP32_IR) ? pcg_fp32ir(c, o, s) : pcg_fp64(c, o, s);
+    (void)hipEventRecord(b, c->stream);
+    (void)hipEventSynchronize(b);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    c->free_events.push_back(a);
+    c->free_events.push_back(b);
+    if (s) s->seconds = ms * 1e-3;
+    return rc;
+}
+
+static int check_opts(hx_ctx* c, const hx_pcg_opts* o) {
+    if (!o) return fail(c, HX_ERR_BAD_ARG, "null options");
+    if (!(o->tol > 0) || o->max_iter < 1 || !(o->shift >= 0)) return fail(c, HX_ERR_BAD_ARG, "invalid PCG options");
+    if (o->precision != HX_FP64 && o->precision != HX_FP32_IR) return fail(c, HX_ERR_BAD_ARG, "precision");
+    if (o->precond != HX_PC_JACOBI) return fail(c, HX_ERR_BAD_ARG, "preconditioner not available in this call");
+    if (o->matvec < HX_MV_AUTO || o->matvec > HX_MV_GATHER) return fail(c, HX_ERR_BAD_ARG, "matvec variant");
+    return HX_OK;
+}
+
+int hx_pcg_solve(hx_ctx* c, const double* Ee, const double* f, double* u, const hx_pcg_opts* o, hx_pcg_stats* s) {
+    if (!c || !Ee || !f || !u) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    if (int rc = check_opts(c, o)) return rc;
+    const long n = ndof(c);
+    int rc;
+    if ((rc = upload(c, c->E.p, Ee, c->g.ne * sizeof(double))) || (rc = upload(c, c->f.p, f, n * sizeof(double))))
+        return rc;
+    if (o->warm_start && (rc = upload(c, c->x.p, u, n * sizeof(double)))) return rc;
+    const int prc = pcg_dispatch(c, *o, s);
+    if ((rc = download(c, u, c->x.p, n * sizeof(double)))) return rc;
+    return prc;
+}
+
+int hx_block_pcg_solve(hx_ctx* c, int nrhs, const double* Ee, const double* F, double* U, const hx_pcg_opts* o,
+                       hx_pcg_stats* s) {
+    (void)nrhs; (void)Ee; (void)F; (void)U; (void)o; (void)s;
+    return fail(c, HX_ERR_STATE, "block solve not available in this build");
+}
+
+int hx_helmholtz_filter(hx_ctx* c, double R, const hx_filter_opts* o, const double* in_e, double* out_e,
+                        hx_filter_stats* s) {
+    if (!c || !o || !in_e || !out_e || !(R >= 0) || !(o->tol > 0)) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    const long ne = c->g.ne;
+    HXA(c->h_in, ne);
+    HXA(c->h_out, ne);
+    int rc;
+    if ((rc = upload(c, c->h_in.p, in_e, ne * sizeof(double)))) return rc;
+    hipEvent_t a = get_event(c), b = get_event(c);
+    (void)hipEventRecord(a, c->stream);
+    const int hrc = helm_solve(c, R, *o, c->h_in.p, c->h_out.p, s);
+    (void)hipEventRecord(b, c->stream);
+    (void)hipEventSynchronize(b);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    c->free_events.push_back(a);
+    c->free_events.push_back(b);
+    if (s) s->seconds = ms * 1e-3;
+    if ((rc = download(c, out_e, c->h_out.p, ne * sizeof(double)))) return rc;
+    return hrc;
+}
+
+int hx_element_stress(hx_ctx* c, const double* Ee, const double* u, double* principal, double* vm, double* voigt,
+                      double* l1, double* dir) {
+    if (!c || !Ee || !u) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    const long ne = c->g.ne;
+    HXA(c->e_pr, 3 * ne);
+    HXA(c->e_vm, ne);
+    HXA(c->e_sv, 6 * ne);
+    HXA(c->e_l1, ne);
+    HXA(c->e_dir, 3 * ne);
+    int rc;
+    if ((rc = upload(c, c->E.p, Ee, ne * sizeof(double))) || (rc = upload(c, c->x.p, u, ndof(c) * sizeof(double))))
+        return rc;
+    ElemArgs a{};
+    a.u = c->x.p;
+    a.Emod = c->E.p;
+    material_consts(c->nu, &a.lam1, &a.lam2, &a.mu);
+    a.principal = principal ? c->e_pr.p : nullptr;
+    a.vm = vm ? c->e_vm.p : nullptr;
+    a.voigt = voigt ? c->e_sv.p : nullptr;
+    a.l1 = l1 ? c->e_l1.p : nullptr;
+    a.dir = dir ? c->e_dir.p : nullptr;
+    elem_post(c, a);
+    HXK();
+    if (principal && (rc = download(c, principal, c->e_pr.p, 3 * ne * sizeof(double)))) return rc;
+    if (vm && (rc = download(c, vm, c->e_vm.p, ne * sizeof(double)))) return rc;
+    if (voigt && (rc = download(c, voigt, c->e_sv.p, 6 * ne * sizeof(double)))) return rc;
+    if (l1 && (rc = download(c, l1, c->e_l1.p, ne * sizeof(double)))) return rc;
+    if (dir && (rc = download(c, dir, c->e_dir.p, 3 * ne * sizeof(double)))) return rc;
+    HXC(hipStreamSynchronize(c->stream));
+    return HX_OK;
+}
+
+int hx_compliance_sens(hx_ctx* c, const double* xphys, double penal, double E0, double Emin, const double* u,
+                       const double* f, double* ce, double* dc, double* compliance) {
+    if (!c || !u || (dc && !xphys) || (compliance && !f)) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    const long ne = c->g.ne, n = ndof(c);
+    HXA(c->e_ce, ne);
+    HXA(c->e_dc, ne);
+    HXA(c->e_aux, ne);
+    int rc;
+    if ((rc = upload(c, c->x.p, u, n * sizeof(double)))) return rc;
+    if (xphys && (rc = upload(c, c->e_aux.p, xphys, ne * sizeof(double)))) return rc;
+    if (f && (rc = upload(c, c->f.p, f, n * sizeof(double)))) return rc;
+    ElemArgs a{};
+    a.u = c->x.p;
+    a.xphys = xphys ? c->e_aux.p : nullptr;
+    a.penal = penal;
+    a.E0 = E0;
+    a.Emin = Emin;
+    a.ce = c->e_ce.p;
+    a.dc = dc ? c->e_dc.p : nullptr;
+    elem_post(c, a);
+    if (compliance) launch_dot(n, c->f.p, c->x.p, c->small.p, &c->st->ticket[0], c->partials.p, c->stream);
+    HXK();
+    if (ce && (rc = download(c, ce, c->e_ce.p, ne * sizeof(double)))) return rc;
+    if (dc && (rc = download(c, dc, c->e_dc.p, ne * sizeof(double)))) return rc;
+    if (compliance && (rc = download(c, compliance, c->small.p, sizeof(double)))) return rc;
+    return HX_OK;
+}
+
+int hx_oc_update(hx_ctx* c, const double* x, const double* dc, const double* dv, double volfrac, double move,
+                 double xmin, double rel_tol, double* xnew, double* lmid, int* steps, double* change) {
+    if (!c || !x || !dc || !dv || !xnew || !(rel_tol > 0) || !(move > 0)) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    const long ne = c->g.ne;
+    HXA(c->e_ce, ne);
+    HXA(c->e_dc, ne);
+    HXA(c->e_aux, ne);
+    HXA(c->e_vm, ne);
+    int rc;
+    if ((rc = upload(c, c->e_aux.p, x, ne * sizeof(double))) || (rc = upload(c, c->e_dc.p, dc, ne * sizeof(double))) ||
+        (rc = upload(c, c->e_ce.p, dv, ne * sizeof(double))))
+        return rc;
+    if ((rc = oc_run(c, ne, c->e_aux.p, c->e_dc.p, c->e_ce.p, volfrac, move, xmin, rel_tol, c->e_vm.p))) return rc;
+    if (lmid) *lmid = c->st_host->scal[2];
+    if (steps) *steps = c->st_host->iter;
+    if (change) *change = c->st_host->scal[3];
+    return download(c, xnew, c->e_vm.p, ne * sizeof(double));
+}
+
+// ------------------------------------------------------- SIMP/OC session
+int hx_to_init(hx_ctx* c, const hx_to_params* p, const double* f, const double* x0) {
+    if (!c || !p || !f || !x0) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    if (!(p->penal >= 1) || !(p->volfrac > 0 && p->volfrac < 1) || !(p->rmin >= 0) || !(p->pcg_tol > 0) ||
+        p->pcg_max_iter < 1 || !(p->filter_tol > 0) || !(p->oc_rel_tol > 0) || !(p->move > 0))
+        return fail(c, HX_ERR_BAD_ARG, "invalid TO parameters");
+    const long ne = c->g.ne, n = ndof(c);
+    HXA(c->t_x, ne);
+    HXA(c->t_xphys, ne);
+    HXA(c->t_xnew, ne);
+    HXA(c->t_dc, ne);
+    HXA(c->t_dcf, ne);
+    HXA(c->t_dv, ne);
+    HXA(c->t_vm, ne);
+    HXA(c->t_ones, ne);
+    int rc;
+    if ((rc = upload(c, c->f.p, f, n * sizeof(double))) || (rc = upload(c, c->t_x.p, x0, ne * sizeof(double))))
+        return rc;
+    c->tp = *p;
+    launch_fill(ne, 1.0, c->t_ones.p, c->stream);
+    launch_fill(n, 0.0, c->x.p, c->stream);
+    const double R = p->rmin / (2.0 * std::sqrt(3.0));  // Lazarov & Sigmund 2011
+    hx_filter_opts fo{0, p->filter_tol, p->filter_max_iter > 0 ? p->filter_max_iter : 1000, 1};
+    if ((rc = helm_solve(c, R, fo, c->t_x.p, c->t_xphys.p, nullptr))) return rc;
+    if ((rc = helm_solve(c, R, fo, c->t_ones.p, c->t_dv.p, nullptr))) return rc;  // dv = H 1
+    c->to_iter = 0;
+    c->to_ready = true;
+    return HX_OK;
+}
+
+int hx_to_step(hx_ctx* c, hx_to_stats* s) {
+    if (!c) return HX_ERR_BAD_ARG;
+    if (!c->to_ready) return fail(c, HX_ERR_STATE, "hx_to_init first");
+    const hx_to_params& p = c->tp;
+    const long ne = c->g.ne, n = ndof(c);
+    hipEvent_t ev0 = get_event(c), ev1 = get_event(c);
+    (void)hipEventRecord(ev0, c->stream);
+    int rc;
+    // 1. SIMP moduli of the physical densities
+    launch_simp_modulus(ne, c->t_xphys.p, p.penal, p.E0, p.Emin, c->E.p, c->stream);
+    // 2. elasticity solve, warm-started from the previous displacement
+    hx_pcg_opts po{p.pcg_tol, p.pcg_max_iter, 0.0, p.precision, p.precond, 1, HX_MV_AUTO, 0};
+    hx_pcg_stats ps{};
+    const int prc = (p.precision == HX_FP32_IR) ? pcg_fp32ir(c, po, &ps) : pcg_fp64(c, po, &ps);
+    if (prc == HX_ERR_HIP || prc == HX_ERR_OOM) return prc;
+    // 3. compliance, sensitivities, von Mises
+    launch_dot(n, c->f.p, c->x.p, c->small.p, &c->st->ticket[0], c->partials.p, c->stream);
+    ElemArgs a{};
+    a.u = c->x.p;
+    a.xphys = c->t_xphys.p;
+    a.penal = p.penal;
+    a.E0 = p.E0;
+    a.Emin = p.Emin;
+    a.dc = c->t_dc.p;
+    if (p.with_stress) {
+        a.Emod = c->E.p;
+        a.vm = c->t_vm.p;
+        a.vm_max_partials = c->e_aux.p;
+        HXA(c->e_aux, std::max<long>(ne, element_post_blocks(c->g)));
+        a.vm_max_partials = c->e_aux.p;
+        material_consts(c->nu, &a.lam1, &a.lam2, &a.mu);
+    }
+    elem_post(c, a);
+    if (p.with_stress) launch_max_reduce(ne, c->e_aux.p, element_post_blocks(c->g), c->small.p + 1, c->stream);
+    HXK();
+    // 4. sensitivity filter (the PDE filter is symmetric: dc_x = H dc_xphys)
+    const double R = p.rmin / (2.0 * std::sqrt(3.0));
+    hx_filter_opts fo{0, p.filter_tol, p.filter_max_iter > 0 ? p.filter_max_iter : 1000, 1};
+    hx_filter_stats f1{}, f2{};
+    if ((rc = helm_solve(c, R, fo, c->t_dc.p, c->t_dcf.p, &f1))) return rc;
+    // 5. OC (volume of the mass-preserving filter equals that of x_new)
+    if ((rc = oc_run(c, ne, c->t_x.p, c->t_dcf.p, c->t_dv.p, p.volfrac, p.move, p.xmin, p.oc_rel_tol, c->t_xnew.p)))
+        return rc;
+    const double change = c->st_host->scal[3];
+    // 6. density filter
+    if ((rc = helm_solve(c, R, fo, c->t_xnew.p, c->t_xphys.p, &f2))) return rc;
+    std::swap(c->t_x, c->t_xnew);
+    launch_dot(ne, c->t_xphys.p, c->t_ones.p, c->small.p + 2, &c->st->ticket[0], c->partials.p, c->stream);
+    HXK();
+    (void)hipEventRecord(ev1, c->stream);
+    double sm[3] = {0, 0, 0};
+    if ((rc = download(c, sm, c->small.p, 3 * sizeof(double)))) return rc;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev0, ev1);
+    c->free_events.push_back(ev0);
+    c->free_events.push_back(ev1);
+    c->to_iter += 1;
+    if (s) {
+        s->iter = c->to_iter;
+        s->compliance = sm[0];
+        s->change = change;
+        s->volume = sm[2] / (double)ne;
+        s->pcg_iterations = ps.iterations;
+        s->pcg_rel_res = ps.rel_res;
+        s->pcg_status = ps.status;
+        s->filter_iterations = f1.iterations + f2.iterations;
+        s->max_vm = p.with_stress ? sm[1] : 0.0;
+        s->seconds = ms * 1e-3;
+    }
+    return HX_OK;
+}
+
+int hx_to_read(hx_ctx* c, int field, double* out) {
+    if (!c || !out) return HX_ERR_BAD_ARG;
+    if (!c->to_ready) return fail(c, HX_ERR_STATE, "hx_to_init first");
+    const long ne = c->g.ne;
+    switch (field) {
+        case HX_FIELD_X: return download(c, out, c->t_x.p, ne * sizeof(double));
+        case HX_FIELD_XPHYS: return download(c, out, c->t_xphys.p, ne * sizeof(double));
+        case HX_FIELD_U: return download(c, out, c->x.p, ndof(c) * sizeof(double));
+        case HX_FIELD_VM: return download(c, out, c->t_vm.p, ne * sizeof(double));
+        case HX_FIELD_DC: return download(c, out, c->t_dcf.p, ne * sizeof(double));
+        default: return fail(c, HX_ERR_BAD_ARG, "unknown field");
+    }
+}
+
+int hx_learned_train(hx_ctx* c, const double* Ee, const double* f, hx_learn_stats* s) {
+    (void)Ee; (void)f; (void)s;
+    return fail(c, HX_ERR_STATE, "learned preconditioner not available in this build");
+}
+
+int hx_learned_weights(hx_ctx* c, uint16_t* out, int* count) {
+    (void)out; (void)count;
+    return fail(c, HX_ERR_STATE, "learned preconditioner not available in this build");
+}
+
+int hx_timing_enable(hx_ctx* c, int on) {
+    if (!c) return HX_ERR_BAD_ARG;
+    (void)hipStreamSynchronize(c->stream);
+    flush_timing(c);
+    c->timing = on != 0;
+    for (int k = 0; k < HX_K_COUNT; ++k) {
+        c->acc_ms[k] = 0.0;
+        c->cnt[k] = 0;
+    }
+    return HX_OK;
+}
+
+int hx_timing_read(hx_ctx* c, int kernel, double* avg_ms, long* count) {
+    if (!c || kernel < 0 || kernel >= HX_K_COUNT) return HX_ERR_BAD_ARG;
+    (void)hipStreamSynchronize(c->stream);
+    flush_timing(c);
+    if (avg_ms) *avg_ms = c->cnt[kernel] ? c->acc_ms[kernel] / c->cnt[kernel] : 0.0;
+    if (count) *count = c->cnt[kernel];
+    return HX_OK;
+}
+
+int hx_device_info(hx_ctx* c, int* cu_count, int* clock_khz, double* mem_gb) {
+    if (!c) return HX_ERR_BAD_ARG;
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, c->device) != hipSuccess) return HX_ERR_HIP;
+    if (cu_count) *cu_count = pr.multiProcessorCount;
+    if (clock_khz) *clock_khz = pr.clockRate;
+    if (mem_gb) *mem_gb = pr.totalGlobalMem / 1073741824.0;
+    return HX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,349 @@
+// hx_elastic.hip -- matrix-free elasticity operator y = sum_e E_e Ke x_e on a
+// structured hex grid, owner-computes (no atomics, deterministic).
+//
+// Replaces the reference's element-parallel kernel with 24 atomic adds per
+// element (GPUSolver.jl:37-79) by a z-marching tile: a 256-thread workgroup
+// owns a (TX-1)x(TY-1) column of nodes over a chunk of z planes.  Per element
+// layer it stages one node plane of x in LDS (ring of two planes), every
+// thread applies one element (parity basis, 234 flops) and parks its 24
+// outputs in LDS, then every owned node sums the 4 elements below-and-above
+// it in a fixed order and carries the upper half to the next layer.  Each
+// node value of y is written exactly once; x and E are read ~1.2x (halo).
+#include "hx_launch.h"
+
+namespace hx {
+
+namespace {
+
+template <typename T>
+struct DenseK {
+    const T* k;  // 24x24 binary-order, row-major, unit E
+};
+
+template <typename T, int TX, int TY, bool PARITY>
+__global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T> K, DenseK<T> Kd, MvArgs<T> a,
+                                                          int kchunk, int ntx, int nty, int ntiles) {
+    constexpr int NT = TX * TY;
+    constexpr int PX = TX + 1, PY = TY + 1;
+    constexpr int PLANE = 3 * PX * PY;
+    constexpr int NL = (PLANE + NT - 1) / NT;
+    __shared__ T xs[2][3][PY][PX];
+    __shared__ T ys[24][TY][TX];
+    __shared__ double red[NT / 64];
+
+    if (a.mode == MV_PCG && a.st->done) return;
+
+    // XCD-aware tile order: the 8 XCDs take blocks round-robin, so give each
+    // XCD a contiguous range of tiles (neighbouring tiles share halo planes).
+    const unsigned nb = gridDim.x;  // multiple of 8
+    const unsigned tile = (blockIdx.x & 7u) * (nb >> 3) + (blockIdx.x >> 3);
+    const bool active = tile < (unsigned)ntiles;
+
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    double pq = 0.0;
+
+    if (active) {
+        const int cx = tile % ntx, cy = (tile / ntx) % nty, cz = tile / (ntx * nty);
+        const int i0 = cx * (TX - 1), j0 = cy * (TY - 1);
+        const int k0 = cz * kchunk, k1 = min(k0 + kchunk, g.NZ);
+        const int ie = i0 - 1 + tx, je = j0 - 1 + ty;
+        const bool ecol = ie >= 0 && ie < g.nx && je >= 0 && je < g.ny;
+        const bool nthr = threadIdx.x < (TX - 1) * (TY - 1);
+        const int ox = threadIdx.x % (TX - 1), oy = threadIdx.x / (TX - 1);
+        const int ni = i0 + ox, nj = j0 + oy;
+        const bool nown = nthr && ni < g.NX && nj < g.NY;
+        const long plane_stride = (long)g.NX * g.NY;
+
+        T regs[NL];
+        auto plane_load = [&](int k) {
+#pragma unroll
+            for (int m = 0; m < NL; ++m) {
+                const int idx = threadIdx.x + m * NT;
+                T v = T(0);
+                if (idx < PLANE && k >= 0 && k < g.NZ) {
+                    const int row = idx / (3 * PX), col3 = idx - row * 3 * PX;
+                    const int ii = i0 - 1 + col3 / 3, jj = j0 - 1 + row;
+                    if (ii >= 0 && ii < g.NX && jj >= 0 && jj < g.NY)
+                        v = a.x[3 * ((long)k * plane_stride + (long)jj * g.NX + (i0 - 1)) + col3];
+                }
+                regs[m] = v;
+            }
+        };
+        auto plane_store = [&](int k) {
+            const int sl = k & 1;
+#pragma unroll
+            for (int m = 0; m < NL; ++m) {
+                const int idx = threadIdx.x + m * NT;
+                if (idx < PLANE) {
+                    const int row = idx / (3 * PX), col3 = idx - row * 3 * PX;
+                    xs[sl][col3 % 3][row][col3 / 3] = regs[m];
+                }
+            }
+        };
+        auto elem_E = [&](int ke) -> T {
+            return (ecol && ke >= 0 && ke < g.nz) ? a.E[(long)ke * g.nx * g.ny + (long)je * g.nx + ie] : T(0);
+        };
+
+        plane_load(k0 - 1);
+        plane_store(k0 - 1);
+        plane_load(k0);
+        T ee = elem_E(k0 - 1);
+        T carry[3] = {T(0), T(0), T(0)};
+
+        for (int ke = k0 - 1; ke < k1; ++ke) {
+            plane_store(ke + 1);
+            __syncthreads();
+            if (ke + 2 <= k1) plane_load(ke + 2);
+            const T ee_next = elem_E(ke + 1);
+
+            // ---- element phase: one element per thread
+            T y[24];
+            if (ee != T(0)) {
+                T u[24];
+#pragma unroll
+                for (int L = 0; L < 8; ++L) {
+                    const int bx = L & 1, by = (L >> 1) & 1, bz = L >> 2;
+                    const int sl = (ke + bz) & 1;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) u[3 * L + c] = xs[sl][c][ty + by][tx + bx];
+                }
+                if constexpr (PARITY) {
+                    element_apply_parity(u, y, K, ee);
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 24; ++r) {
+                        T s = T(0);
+#pragma unroll
+                        for (int c = 0; c < 24; ++c) s += Kd.k[24 * r + c] * u[c];
+                        y[r] = ee * s;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 24; ++r) y[r] = T(0);
+            }
+#pragma unroll
+            for (int r = 0; r < 24; ++r) ys[r][ty][tx] = y[r];
+            __syncthreads();
+
+            // ---- node phase: gather the 4 elements around each owned node
+            if (nthr) {
+                T bot[3] = {T(0), T(0), T(0)}, top[3] = {T(0), T(0), T(0)};
+#pragma unroll
+                for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+                    for (int dx = 0; dx < 2; ++dx) {
+                        const int L = (1 - dx) + 2 * (1 - dy);
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            bot[c] += ys[3 * L + c][oy + dy][ox + dx];
+                            top[c] += ys[3 * (L + 4) + c][oy + dy][ox + dx];
+                        }
+                    }
+                if (ke >= k0 && nown) {
+                    const int sl = ke & 1;
+                    const long base = 3 * ((long)ke * plane_stride + (long)nj * g.NX + ni);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const T xv = xs[sl][c][oy + 1][ox + 1];
+                        const T yv = carry[c] + bot[c] + a.shift * xv;
+                        a.y[base + c] = yv;
+                        pq += (double)xv * (double)yv;
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) carry[c] = top[c];
+            }
+            ee = ee_next;
+            __syncthreads();
+        }
+    }
+
+    if (a.mode == MV_PCG) {
+        const double v = block_sum<NT>(pq, red);
+        if (threadIdx.x == 0) a.partials[blockIdx.x] = v;
+        if (publish_and_check_last(&a.st->ticket[0], gridDim.x)) {
+            const double s = reduce_partials<NT>(a.partials, 1, 0, gridDim.x, red);
+            if (threadIdx.x == 0) {
+                a.st->pq = s;
+                if (!(fabs(s) >= 1e-300)) {  // GPUSolver.jl:462-466 (also catches NaN)
+                    a.st->done = 1;
+                    a.st->status = 4;
+                } else {
+                    a.st->alpha = a.st->rz / s;
+                }
+                a.st->ticket[0] = 0;
+            }
+        }
+    }
+}
+
+// Reference-simple variant: one thread per node gathers its 8 elements with
+// the dense Ke (x from global memory / cache).  Used to cross-check.
+__global__ __launch_bounds__(256) void k_matvec_gather(GridArgs g, const double* __restrict__ Kb,
+                                                       MvArgs<double> a) {
+    __shared__ double red[4];
+    if (a.mode == MV_PCG && a.st->done) return;
+    const long n = (long)blockIdx.x * 256 + threadIdx.x;
+    double pq = 0.0;
+    if (n < g.nn) {
+        const int i = (int)(n % g.NX), j = (int)((n / g.NX) % g.NY), k = (int)(n / ((long)g.NX * g.NY));
+        double acc[3] = {0, 0, 0};
+        for (int dz = 0; dz < 2; ++dz)
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx) {
+                    const int ie = i - 1 + dx, je = j - 1 + dy, ke = k - 1 + dz;
+                    if (ie < 0 || je < 0 || ke < 0 || ie >= g.nx || je >= g.ny || ke >= g.nz) continue;
+                    const double E = a.E[(long)ke * g.nx * g.ny + (long)je * g.nx + ie];
+                    if (E == 0.0) continue;
+                    const int Ln = (1 - dx) + 2 * (1 - dy) + 4 * (1 - dz);  // this node inside the element
+                    for (int L = 0; L < 8; ++L) {
+                        const long m = (long)(ie + (L & 1)) + (long)g.NX * ((je + ((L >> 1) & 1)) +
+                                                                             (long)g.NY * (ke + (L >> 2)));
+                        for (int d = 0; d < 3; ++d) {
+                            const double xv = a.x[3 * m + d];
+                            for (int c = 0; c < 3; ++c) acc[c] += E * Kb[24 * (3 * Ln + c) + 3 * L + d] * xv;
+                        }
+                    }
+                }
+        for (int c = 0; c < 3; ++c) {
+            const double xv = a.x[3 * n + c];
+            const double yv = acc[c] + a.shift * xv;
+            a.y[3 * n + c] = yv;
+            pq += xv * yv;
+        }
+    }
+    if (a.mode == MV_PCG) {
+        const double v = block_sum<256>(pq, red);
+        if (threadIdx.x == 0) a.partials[blockIdx.x] = v;
+        if (publish_and_check_last(&a.st->ticket[0], gridDim.x)) {
+            const double s = reduce_partials<256>(a.partials, 1, 0, gridDim.x, red);
+            if (threadIdx.x == 0) {
+                a.st->pq = s;
+                if (!(fabs(s) >= 1e-300)) {
+                    a.st->done = 1;
+                    a.st->status = 4;
+                } else {
+                    a.st->alpha = a.st->rz / s;
+                }
+                a.st->ticket[0] = 0;
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_elastic_minv(GridArgs g, double kd0, double kd1, double kd2,
+                                                      const T* __restrict__ E, const uint8_t* __restrict__ fixed,
+                                                      double shift, T* __restrict__ minv) {
+    const long n = (long)blockIdx.x * 256 + threadIdx.x;
+    if (n >= g.nn) return;
+    const int i = (int)(n % g.NX), j = (int)((n / g.NX) % g.NY), k = (int)(n / ((long)g.NX * g.NY));
+    double S = 0.0;
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                const int ie = i - 1 + dx, je = j - 1 + dy, ke = k - 1 + dz;
+                if (ie < 0 || je < 0 || ke < 0 || ie >= g.nx || je >= g.ny || ke >= g.nz) continue;
+                S += (double)E[(long)ke * g.nx * g.ny + (long)je * g.nx + ie];
+            }
+    const double kd[3] = {kd0, kd1, kd2};
+    for (int c = 0; c < 3; ++c) {
+        const bool fr = (fixed == nullptr || fixed[3 * n + c] == 0) && S > 0.0;
+        minv[3 * n + c] = fr ? T(1.0 / (kd[c] * S + shift)) : T(0);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_elastic_diag(GridArgs g, double kd0, double kd1, double kd2,
+                                                      const double* __restrict__ E, double* __restrict__ diag) {
+    const long n = (long)blockIdx.x * 256 + threadIdx.x;
+    if (n >= g.nn) return;
+    const int i = (int)(n % g.NX), j = (int)((n / g.NX) % g.NY), k = (int)(n / ((long)g.NX * g.NY));
+    double S = 0.0;
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                const int ie = i - 1 + dx, je = j - 1 + dy, ke = k - 1 + dz;
+                if (ie < 0 || je < 0 || ke < 0 || ie >= g.nx || je >= g.ny || ke >= g.nz) continue;
+                S += E[(long)ke * g.nx * g.ny + (long)je * g.nx + ie];
+            }
+    diag[3 * n + 0] = kd0 * S;
+    diag[3 * n + 1] = kd1 * S;
+    diag[3 * n + 2] = kd2 * S;
+}
+
+inline int round8(int n) { return (n + 7) & ~7; }
+
+template <int TX, int TY>
+int zmarch_tiles(const GridArgs& g, int kchunk, int* ntx, int* nty) {
+    *ntx = (g.NX + TX - 2) / (TX - 1);
+    *nty = (g.NY + TY - 2) / (TY - 1);
+    const int nz = (g.NZ + kchunk - 1) / kchunk;
+    return *ntx * *nty * nz;
+}
+
+}  // namespace
+
+int matvec_parity_blocks(const GridArgs& g, int tile, int kchunk) {
+    int a, b;
+    return round8(tile == 1 ? zmarch_tiles<32, 8>(g, kchunk, &a, &b) : zmarch_tiles<16, 16>(g, kchunk, &a, &b));
+}
+int matvec_dense_blocks(const GridArgs& g, int kchunk) {
+    int a, b;
+    return round8(zmarch_tiles<16, 16>(g, kchunk, &a, &b));
+}
+int matvec_gather_blocks(const GridArgs& g) { return (int)((g.nn + 255) / 256); }
+
+template <typename T>
+void launch_matvec_parity(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, int tile, int kchunk,
+                          hipStream_t s) {
+    int ntx, nty;
+    DenseK<T> none{nullptr};
+    if (tile == 1) {
+        const int nt = zmarch_tiles<32, 8>(g, kchunk, &ntx, &nty);
+        hipLaunchKernelGGL((k_matvec_zmarch<T, 32, 8, true>), dim3(round8(nt)), dim3(256), 0, s, g, K, none, a,
+                           kchunk, ntx, nty, nt);
+    } else {
+        const int nt = zmarch_tiles<16, 16>(g, kchunk, &ntx, &nty);
+        hipLaunchKernelGGL((k_matvec_zmarch<T, 16, 16, true>), dim3(round8(nt)), dim3(256), 0, s, g, K, none, a,
+                           kchunk, ntx, nty, nt);
+    }
+}
+
+template <typename T>
+void launch_matvec_dense(const GridArgs& g, const T* Kbin, const MvArgs<T>& a, int kchunk, hipStream_t s) {
+    int ntx, nty;
+    ParityK<T> none{};
+    const int nt = zmarch_tiles<16, 16>(g, kchunk, &ntx, &nty);
+    hipLaunchKernelGGL((k_matvec_zmarch<T, 16, 16, false>), dim3(round8(nt)), dim3(256), 0, s, g, none,
+                       DenseK<T>{Kbin}, a, kchunk, ntx, nty, nt);
+}
+
+void launch_matvec_gather(const GridArgs& g, const double* Kbin, const MvArgs<double>& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_matvec_gather, dim3(matvec_gather_blocks(g)), dim3(256), 0, s, g, Kbin, a);
+}
+
+template <typename T>
+void launch_elastic_minv(const GridArgs& g, const double kd[3], const T* E, const uint8_t* fixed, double shift,
+                         T* minv, hipStream_t s) {
+    hipLaunchKernelGGL((k_elastic_minv<T>), dim3((unsigned)((g.nn + 255) / 256)), dim3(256), 0, s, g, kd[0], kd[1],
+                       kd[2], E, fixed, shift, minv);
+}
+
+void launch_elastic_diag(const GridArgs& g, const double kd[3], const double* E, double* diag, hipStream_t s) {
+    hipLaunchKernelGGL(k_elastic_diag, dim3((unsigned)((g.nn + 255) / 256)), dim3(256), 0, s, g, kd[0], kd[1], kd[2],
+                       E, diag);
+}
+
+template void launch_matvec_parity<double>(const GridArgs&, const ParityK<double>&, const MvArgs<double>&, int, int,
+                                           hipStream_t);
+template void launch_matvec_parity<float>(const GridArgs&, const ParityK<float>&, const MvArgs<float>&, int, int,
+                                          hipStream_t);
+template void launch_matvec_dense<double>(const GridArgs&, const double*, const MvArgs<double>&, int, hipStream_t);
+template void launch_matvec_dense<float>(const GridArgs&, const float*, const MvArgs<float>&, int, hipStream_t);
+template void launch_elastic_minv<double>(const GridArgs&, const double*, const double*, const uint8_t*, double,
+                                          double*, hipStream_t);
+template void launch_elastic_minv<float>(const GridArgs&, const double*, const float*, const uint8_t*, double,
+                                         float*, hipStream_t);
+
+}  // namespace hx

@@ -1,0 +1,211 @@
+// hx_internal.h -- shared definitions of libhexat's HIP kernels and host glue.
+//
+// Element operator in the per-axis parity ("monomial") basis.  With the
+// element's nodes in binary order L = bx + 2*by + 4*bz and the transform
+//   xh[s] = sum_L prod_{axis a in s} xi_a(L) * x[L]      (xi = -1 / +1)
+// every box-element stiffness (2x2x2 Gauss, any dx,dy,dz,nu) has exactly the
+// 45 non-zeros listed in kParityPattern; the scalar Laplacian and mass are
+// diagonal.  y = Ke x becomes  xh = V^T x,  yh = Kp xh,  y = V yh  with
+// Kp = V^T Ke V / 64: 72 + 90 + 72 flops instead of 1152 for dense Ke.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+
+namespace hx {
+
+// (row, col) of the 45 non-zeros of Kp; row/col = 3*s + component.
+struct RC {
+    int8_t r, c;
+};
+constexpr int kParityNnz = 45;
+constexpr RC kParityPattern[kParityNnz] = {
+    // linear modes s in {1,2,4}: constant-strain block
+    {3, 3}, {3, 7}, {3, 14},
+    {4, 4}, {4, 6},
+    {5, 5}, {5, 12},
+    {6, 4}, {6, 6},
+    {7, 3}, {7, 7}, {7, 14},
+    {8, 8}, {8, 13},
+    {12, 5}, {12, 12},
+    {13, 8}, {13, 13},
+    {14, 3}, {14, 7}, {14, 14},
+    // bilinear modes s in {3,5,6}
+    {9, 9}, {9, 20},
+    {10, 10}, {10, 17},
+    {11, 11}, {11, 16}, {11, 18},
+    {15, 15}, {15, 19},
+    {16, 11}, {16, 16}, {16, 18},
+    {17, 10}, {17, 17},
+    {18, 11}, {18, 16}, {18, 18},
+    {19, 15}, {19, 19},
+    {20, 9}, {20, 20},
+    // trilinear mode s = 7
+    {21, 21}, {22, 22}, {23, 23}};
+
+template <typename T>
+struct ParityK {
+    T v[kParityNnz];
+};
+
+// reference local node n1..n8 -> binary local index L
+constexpr int kRefToBin[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+
+// In-register transforms on 8 values of one component.
+template <typename T>
+__device__ __forceinline__ void fwd8(T* a) {  // (lo,hi) -> (lo+hi, hi-lo)
+#pragma unroll
+    for (int b = 1; b < 8; b <<= 1)
+#pragma unroll
+        for (int L = 0; L < 8; ++L)
+            if (!(L & b)) {
+                T lo = a[L], hi = a[L | b];
+                a[L] = lo + hi;
+                a[L | b] = hi - lo;
+            }
+}
+template <typename T>
+__device__ __forceinline__ void inv8(T* a) {  // (lo,hi) -> (lo-hi, lo+hi)
+#pragma unroll
+    for (int b = 1; b < 8; b <<= 1)
+#pragma unroll
+        for (int L = 0; L < 8; ++L)
+            if (!(L & b)) {
+                T lo = a[L], hi = a[L | b];
+                a[L] = lo - hi;
+                a[L | b] = lo + hi;
+            }
+}
+
+// y (24, index 3L+c) = E * Ke x for one element, parity basis.
+template <typename T>
+__device__ __forceinline__ void element_apply_parity(const T* __restrict__ x, T* __restrict__ y,
+                                                     const ParityK<T>& K, T E) {
+    T xh[24], yh[24];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        T t[8];
+#pragma unroll
+        for (int L = 0; L < 8; ++L) t[L] = x[3 * L + c];
+        fwd8(t);
+#pragma unroll
+        for (int s = 0; s < 8; ++s) xh[3 * s + c] = t[s];
+    }
+#pragma unroll
+    for (int i = 0; i < 24; ++i) yh[i] = T(0);
+#pragma unroll
+    for (int n = 0; n < kParityNnz; ++n) yh[kParityPattern[n].r] += K.v[n] * xh[kParityPattern[n].c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        T t[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) t[s] = E * yh[3 * s + c];
+        inv8(t);
+#pragma unroll
+        for (int L = 0; L < 8; ++L) y[3 * L + c] = t[L];
+    }
+}
+
+// Device-side state of one PCG solve (all scalars stay on the device).
+struct PcgState {
+    double rz;       // r'z of the current iterate
+    double pq;       // p'Ap
+    double alpha, beta;
+    double rr;       // r'r
+    double normb;    // ||b||
+    double rel;      // ||r||/||b|| (recursive)
+    double best_rel;
+    double tol;
+    double shift;
+    double scal[6];  // spare scalar slots (compliance, OC bracket, change ...)
+    int iter;
+    int max_iter;
+    int done;        // 0 running, 1 finished
+    int status;      // hx_status
+    int check_every;
+    int pad;
+    unsigned ticket[4];
+};
+
+// Grid description passed by value to every kernel.
+struct GridArgs {
+    int nx, ny, nz;
+    int NX, NY, NZ;
+    long nn, ne;
+    double dx, dy, dz;
+};
+
+constexpr int kRedThreads = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Deterministic block sum (fixed tree).  Result valid in thread 0.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x < 64) {
+        r = (threadIdx.x < NT / 64) ? sh[threadIdx.x] : 0.0;
+        r = wave_sum(r);
+    }
+    return r;
+}
+template <int NT>
+__device__ __forceinline__ double block_max(double v, double* sh) {
+    v = wave_max(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = -INFINITY;
+    if (threadIdx.x < 64) {
+        r = (threadIdx.x < NT / 64) ? sh[threadIdx.x] : -INFINITY;
+        r = wave_max(r);
+    }
+    return r;
+}
+
+__device__ __forceinline__ double load_coherent(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Publish this block's partials and find out whether it is the last block to
+// finish.  The last block then reduces all partials in a fixed order.
+__device__ __forceinline__ bool publish_and_check_last(unsigned* ticket, unsigned nblocks) {
+    __shared__ bool s_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned t = atomicAdd(ticket, 1u);
+        s_last = (t == nblocks - 1);
+    }
+    __syncthreads();
+    if (s_last) __threadfence();
+    return s_last;
+}
+
+// Sum partials[j + nv*b] over b in fixed order; result valid in thread 0.
+template <int NT>
+__device__ __forceinline__ double reduce_partials(const double* partials, int nv, int j, unsigned nblocks,
+                                                  double* sh) {
+    double s = 0.0;
+    for (unsigned b = threadIdx.x; b < nblocks; b += NT) s += load_coherent(partials + (size_t)nv * b + j);
+    return block_sum<NT>(s, sh);
+}
+
+}  // namespace hx

@@ -1,0 +1,676 @@
+// hx_kernels.hip -- streaming kernels of the PCG recurrences, the Helmholtz
+// filter, the element post-processing (stress, energies, sensitivities) and
+// the OC update.  All reductions are two-stage and deterministic: per-block
+// partials in a fixed tree, then the last block (ticket counter) sums the
+// partials in block order and updates the device-side PcgState, so no scalar
+// ever returns to the host inside a solve (the reference pays three
+// blocking CUBLAS dots per iteration, GPUSolver.jl:460,472,522).
+#include "hx_launch.h"
+
+namespace hx {
+
+namespace {
+
+constexpr int NT = kRedThreads;
+
+__device__ __forceinline__ long gtid() { return (long)blockIdx.x * NT + threadIdx.x; }
+__device__ __forceinline__ long gstride() { return (long)gridDim.x * NT; }
+
+__global__ void k_pcg_reset(PcgState* st, double tol, double shift, int max_iter, int check_every) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st->rz = st->pq = st->alpha = st->beta = st->rr = st->normb = st->rel = 0.0;
+        st->best_rel = INFINITY;
+        st->tol = tol;
+        st->shift = shift;
+        for (int i = 0; i < 6; ++i) st->scal[i] = 0.0;
+        st->iter = 0;
+        st->max_iter = max_iter;
+        st->done = 0;
+        st->status = 0;
+        st->check_every = check_every > 0 ? check_every : 1;
+        for (int i = 0; i < 4; ++i) st->ticket[i] = 0;
+    }
+}
+
+// x = free ? x : 0 ; r = free ? b - (q + shift x) : 0 ; p = minv r
+template <typename T>
+__global__ __launch_bounds__(NT) void k_pcg_init(long n, const T* __restrict__ b, T* __restrict__ x,
+                                                 const T* __restrict__ q, const T* __restrict__ minv,
+                                                 T* __restrict__ r, T* __restrict__ p, double shift, PcgState* st,
+                                                 double* partials) {
+    __shared__ double red[NT / 64];
+    double rz = 0.0, bb = 0.0, rr = 0.0;
+    for (long i = gtid(); i < n; i += gstride()) {
+        const double m = (double)minv[i];
+        const bool fr = m != 0.0;
+        const double bi = (double)b[i];
+        const double xi = fr ? (double)x[i] : 0.0;
+        const T rt = fr ? T(bi - ((double)q[i] + shift * xi)) : T(0);
+        const T zt = T(m * (double)rt);
+        x[i] = T(xi);
+        r[i] = rt;
+        p[i] = zt;
+        rz += (double)rt * (double)zt;
+        rr += (double)rt * (double)rt;
+        bb += fr ? bi * bi : 0.0;
+    }
+    const double v0 = block_sum<NT>(rz, red);
+    const double v1 = block_sum<NT>(bb, red);
+    const double v2 = block_sum<NT>(rr, red);
+    if (threadIdx.x == 0) {
+        partials[3 * blockIdx.x + 0] = v0;
+        partials[3 * blockIdx.x + 1] = v1;
+        partials[3 * blockIdx.x + 2] = v2;
+    }
+    if (publish_and_check_last(&st->ticket[1], gridDim.x)) {
+        const double s0 = reduce_partials<NT>(partials, 3, 0, gridDim.x, red);
+        const double s1 = reduce_partials<NT>(partials, 3, 1, gridDim.x, red);
+        const double s2 = reduce_partials<NT>(partials, 3, 2, gridDim.x, red);
+        if (threadIdx.x == 0) {
+            st->rz = s0;
+            st->normb = sqrt(s1);
+            st->rr = s2;
+            st->rel = s1 > 0.0 ? sqrt(s2 / s1) : 0.0;
+            if (s1 == 0.0) {  // GPUSolver.jl:373
+                st->done = 1;
+                st->status = 10;
+            }
+            st->ticket[1] = 0;
+        }
+    }
+}
+
+// Convergence bookkeeping after the r update (GPUSolver.jl:472-511).
+__device__ void pcg_after_update(PcgState* st, double rz_new, double rr) {
+    const int it = st->iter + 1;
+    st->iter = it;
+    const double rel = sqrt(rr) / st->normb;
+    st->rr = rr;
+    st->rel = rel;
+    if (it > 50 && rel > st->best_rel * 100.0) {  // instability (:480-484)
+        st->done = 1;
+        st->status = 5;
+    } else if (!(rel == rel) || isinf(rel) || rel > 1000.0) {  // explosion (:486-490)
+        st->done = 1;
+        st->status = 5;
+    } else {
+        if (rel < st->best_rel) st->best_rel = rel;
+        if (rel < st->tol && (it % st->check_every) == 0) {
+            st->done = 1;
+            st->status = 0;
+        } else if (it >= st->max_iter) {
+            st->done = 1;
+            st->status = 3;
+        }
+    }
+    st->beta = rz_new / st->rz;
+    st->rz = rz_new;
+}
+
+// x += alpha p ; r = free ? r - alpha q : 0 ; rz = r'(minv r), rr = r'r
+template <typename T>
+__global__ __launch_bounds__(NT) void k_pcg_update(long n, T* __restrict__ x, const T* __restrict__ p,
+                                                   T* __restrict__ r, const T* __restrict__ q,
+                                                   const T* __restrict__ minv, PcgState* st, double* partials) {
+    __shared__ double red[NT / 64];
+    if (st->done) return;
+    const double alpha = st->alpha;
+    double rz = 0.0, rr = 0.0;
+    for (long i = gtid(); i < n; i += gstride()) {
+        const double m = (double)minv[i];
+        const T xt = T((double)x[i] + alpha * (double)p[i]);
+        const T rt = m != 0.0 ? T((double)r[i] - alpha * (double)q[i]) : T(0);
+        x[i] = xt;
+        r[i] = rt;
+        const double rd = (double)rt;
+        rz += rd * (m * rd);
+        rr += rd * rd;
+    }
+    const double v0 = block_sum<NT>(rz, red);
+    const double v1 = block_sum<NT>(rr, red);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x + 0] = v0;
+        partials[2 * blockIdx.x + 1] = v1;
+    }
+    if (publish_and_check_last(&st->ticket[1], gridDim.x)) {
+        const double s0 = reduce_partials<NT>(partials, 2, 0, gridDim.x, red);
+        const double s1 = reduce_partials<NT>(partials, 2, 1, gridDim.x, red);
+        if (threadIdx.x == 0) {
+            pcg_after_update(st, s0, s1);
+            st->ticket[1] = 0;
+        }
+    }
+}
+
+// p = minv r + beta p
+template <typename T>
+__global__ __launch_bounds__(NT) void k_pcg_pupdate(long n, const T* __restrict__ r, const T* __restrict__ minv,
+                                                    T* __restrict__ p, const PcgState* st) {
+    if (st->done) return;
+    const double beta = st->beta;
+    for (long i = gtid(); i < n; i += gstride())
+        p[i] = T((double)minv[i] * (double)r[i] + beta * (double)p[i]);
+}
+
+__global__ __launch_bounds__(NT) void k_dot(long n, const double* __restrict__ a, const double* __restrict__ b,
+                                            double* out, unsigned* ticket, double* partials) {
+    __shared__ double red[NT / 64];
+    double s = 0.0;
+    for (long i = gtid(); i < n; i += gstride()) s += a[i] * b[i];
+    const double v = block_sum<NT>(s, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = v;
+    if (publish_and_check_last(ticket, gridDim.x)) {
+        const double t = reduce_partials<NT>(partials, 1, 0, gridDim.x, red);
+        if (threadIdx.x == 0) {
+            *out = t;
+            *ticket = 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_fill(long n, double v, double* a) {
+    for (long i = gtid(); i < n; i += gstride()) a[i] = v;
+}
+
+__global__ __launch_bounds__(NT) void k_fill32(long n, float v, float* a) {
+    for (long i = gtid(); i < n; i += gstride()) a[i] = v;
+}
+
+template <typename Tin, typename Tout>
+__global__ __launch_bounds__(NT) void k_convert(long n, const Tin* __restrict__ a, Tout* __restrict__ b) {
+    for (long i = gtid(); i < n; i += gstride()) b[i] = Tout(a[i]);
+}
+
+// r = free ? b - (q + shift x) : 0 in FP64; r32 copy; rz (with minv rounded
+// to FP32), rr and optionally bb.  mode bit0: set normb = sqrt(bb);
+// mode bit1: park rz in scal[5] (reliable update) instead of st->rz.
+__global__ __launch_bounds__(NT) void k_true_residual(long n, const double* __restrict__ b,
+                                                      const double* __restrict__ q, const double* __restrict__ x,
+                                                      const double* __restrict__ minv, double shift,
+                                                      double* __restrict__ r64, float* __restrict__ r32,
+                                                      PcgState* st, double* partials, int mode) {
+    __shared__ double red[NT / 64];
+    double rz = 0.0, rr = 0.0, bb = 0.0;
+    for (long i = gtid(); i < n; i += gstride()) {
+        const double m = minv[i];
+        const bool fr = m != 0.0;
+        const double bi = b[i];
+        const double ri = fr ? bi - (q[i] + shift * x[i]) : 0.0;
+        if (r64) r64[i] = ri;
+        const float rf = (float)ri;
+        if (r32) r32[i] = rf;
+        rz += (double)rf * ((double)(float)m * (double)rf);
+        rr += ri * ri;
+        bb += fr ? bi * bi : 0.0;
+    }
+    const double v0 = block_sum<NT>(rz, red);
+    const double v1 = block_sum<NT>(rr, red);
+    const double v2 = block_sum<NT>(bb, red);
+    if (threadIdx.x == 0) {
+        partials[3 * blockIdx.x + 0] = v0;
+        partials[3 * blockIdx.x + 1] = v1;
+        partials[3 * blockIdx.x + 2] = v2;
+    }
+    if (publish_and_check_last(&st->ticket[2], gridDim.x)) {
+        const double s0 = reduce_partials<NT>(partials, 3, 0, gridDim.x, red);
+        const double s1 = reduce_partials<NT>(partials, 3, 1, gridDim.x, red);
+        const double s2 = reduce_partials<NT>(partials, 3, 2, gridDim.x, red);
+        if (threadIdx.x == 0) {
+            if (mode & 1) st->normb = sqrt(s2);
+            if (mode & 2) st->scal[5] = s0;
+            else st->rz = s0;
+            st->scal[4] = s1;  // true ||r||^2
+            st->rel = st->normb > 0.0 ? sqrt(s1) / st->normb : 0.0;
+            st->ticket[2] = 0;
+        }
+    }
+}
+
+// Resume the FP32 recurrence after a reliable update: the true residual's
+// r'z (scal[5]) replaces the recursive one and rescales beta accordingly.
+__global__ void k_ir_resume(PcgState* st, double tol_inner, int first) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const double rzt = st->scal[5];
+        st->beta = first ? 0.0 : st->beta * (rzt / st->rz);
+        st->rz = rzt;
+        st->tol = tol_inner;
+        st->done = 0;
+        st->status = 0;
+        if (st->normb == 0.0) {
+            st->done = 1;
+            st->status = 10;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_mask(long n, const T* __restrict__ minv, T* __restrict__ x) {
+    for (long i = gtid(); i < n; i += gstride())
+        if (minv[i] == T(0)) x[i] = T(0);
+}
+
+__global__ __launch_bounds__(NT) void k_accumulate(long n, double* __restrict__ x64, float* __restrict__ x32) {
+    for (long i = gtid(); i < n; i += gstride()) {
+        x64[i] += (double)x32[i];
+        x32[i] = 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------- Helmholtz
+constexpr int kBinToRef[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+
+__device__ __forceinline__ void node_ijk(const GridArgs& g, long n, int& i, int& j, int& k) {
+    i = (int)(n % g.NX);
+    j = (int)((n / g.NX) % g.NY);
+    k = (int)(n / ((long)g.NX * g.NY));
+}
+
+// GPUHelmholtz.jl:39-50 as a node gather: b_n = vol/8 * sum_{e ni n} in_e
+__global__ __launch_bounds__(NT) void k_helm_rhs(GridArgs g, const double* __restrict__ in_e, double vol8,
+                                                 double* __restrict__ b) {
+    const long n = gtid();
+    if (n >= g.nn) return;
+    int i, j, k;
+    node_ijk(g, n, i, j, k);
+    double s = 0.0;
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                const int ie = i - 1 + dx, je = j - 1 + dy, ke = k - 1 + dz;
+                if (ie < 0 || je < 0 || ke < 0 || ie >= g.nx || je >= g.ny || ke >= g.nz) continue;
+                s += in_e[(long)ke * g.nx * g.ny + (long)je * g.nx + ie];
+            }
+    b[n] = s * vol8;
+}
+
+// GPUHelmholtz.jl:91-102,144: inv of the assembled diagonal
+__global__ __launch_bounds__(NT) void k_helm_minv(GridArgs g, HelmArgs A, double* __restrict__ minv) {
+    const long n = gtid();
+    if (n >= g.nn) return;
+    int i, j, k;
+    node_ijk(g, n, i, j, k);
+    double s = 0.0;
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                const int ie = i - 1 + dx, je = j - 1 + dy, ke = k - 1 + dz;
+                if (ie < 0 || je < 0 || ke < 0 || ie >= g.nx || je >= g.ny || ke >= g.nz) continue;
+                const int a = kBinToRef[(1 - dx) + 2 * (1 - dy) + 4 * (1 - dz)];
+                s += A.Ae[9 * a];
+            }
+    minv[n] = 1.0 / s;
+}
+
+// GPUHelmholtz.jl:52-76 as a node gather (64 MACs per node, x via cache)
+__global__ __launch_bounds__(NT) void k_helm_matvec(GridArgs g, HelmArgs A, const double* __restrict__ x,
+                                                    double* __restrict__ y, int mode, PcgState* st,
+                                                    double* partials) {
+    __shared__ double red[NT / 64];
+    if (mode == MV_PCG && st->done) return;
+    const long n = gtid();
+    double pq = 0.0;
+    if (n < g.nn) {
+        int i, j, k;
+        node_ijk(g, n, i, j, k);
+        double acc = 0.0;
+        for (int dz = 0; dz < 2; ++dz)
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx) {
+                    const int ie = i - 1 + dx, je = j - 1 + dy, ke = k - 1 + dz;
+                    if (ie < 0 || je < 0 || ke < 0 || ie >= g.nx || je >= g.ny || ke >= g.nz) continue;
+                    const int a = kBinToRef[(1 - dx) + 2 * (1 - dy) + 4 * (1 - dz)];
+#pragma unroll
+                    for (int L = 0; L < 8; ++L) {
+                        const long m = (long)(ie + (L & 1)) +
+                                       (long)g.NX * ((je + ((L >> 1) & 1)) + (long)g.NY * (ke + (L >> 2)));
+                        acc += A.Ae[8 * a + kBinToRef[L]] * x[m];
+                    }
+                }
+        y[n] = acc;
+        pq = x[n] * acc;
+    }
+    if (mode == MV_PCG) {
+        const double v = block_sum<NT>(pq, red);
+        if (threadIdx.x == 0) partials[blockIdx.x] = v;
+        if (publish_and_check_last(&st->ticket[0], gridDim.x)) {
+            const double s = reduce_partials<NT>(partials, 1, 0, gridDim.x, red);
+            if (threadIdx.x == 0) {
+                st->pq = s;
+                if (!(fabs(s) >= 1e-300)) {
+                    st->done = 1;
+                    st->status = 4;
+                } else {
+                    st->alpha = st->rz / s;
+                }
+                st->ticket[0] = 0;
+            }
+        }
+    }
+}
+
+// GPUHelmholtz.jl:78-89
+__global__ __launch_bounds__(NT) void k_helm_extract(GridArgs g, const double* __restrict__ xn,
+                                                     double* __restrict__ out) {
+    const long e = gtid();
+    if (e >= g.ne) return;
+    const int i = (int)(e % g.nx), j = (int)((e / g.nx) % g.ny), k = (int)(e / ((long)g.nx * g.ny));
+    double s = 0.0;
+#pragma unroll
+    for (int L = 0; L < 8; ++L)
+        s += xn[(long)(i + (L & 1)) + (long)g.NX * ((j + ((L >> 1) & 1)) + (long)g.NY * (k + (L >> 2)))];
+    out[e] = s / 8.0;
+}
+
+// ------------------------------------------------------------ element post
+// cyclic Jacobi for a symmetric 3x3 (A row-major); vals descending, V columns
+__device__ void eig3_jacobi(double A[3][3], double vals[3], double vec[3]) {
+    double V[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        const double off = fabs(A[0][1]) + fabs(A[0][2]) + fabs(A[1][2]);
+        const double scale = fabs(A[0][0]) + fabs(A[1][1]) + fabs(A[2][2]) + off;
+        if (off <= 1e-300 || off <= 1e-17 * scale) break;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                if (A[p][q] == 0.0) continue;
+                const double th = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < 3; ++k) {
+                    const double akp = A[k][p], akq = A[k][q];
+                    A[k][p] = c * akp - s * akq;
+                    A[k][q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    const double apk = A[p][k], aqk = A[q][k];
+                    A[p][k] = c * apk - s * aqk;
+                    A[q][k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    const double vkp = V[k][p], vkq = V[k][q];
+                    V[k][p] = c * vkp - s * vkq;
+                    V[k][q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    double ev[3] = {A[0][0], A[1][1], A[2][2]};
+    int idx[3] = {0, 1, 2};
+    for (int a = 0; a < 3; ++a)
+        for (int b = a + 1; b < 3; ++b)
+            if (ev[idx[b]] > ev[idx[a]]) {
+                const int t = idx[a];
+                idx[a] = idx[b];
+                idx[b] = t;
+            }
+    for (int a = 0; a < 3; ++a) vals[a] = ev[idx[a]];
+    for (int k = 0; k < 3; ++k) vec[k] = V[k][idx[0]];
+}
+
+// Stress.jl:10-100 (centroid stress, von Mises, principal, L1, direction) and
+// the SIMP energy / sensitivity, all from one read of the element's 24 DOFs.
+__global__ __launch_bounds__(NT) void k_element_post(GridArgs g, ParityK<double> K, ElemArgs a) {
+    __shared__ double red[NT / 64];
+    const long e = gtid();
+    double vmv = 0.0;
+    if (e < g.ne) {
+        const int i = (int)(e % g.nx), j = (int)((e / g.nx) % g.ny), k = (int)(e / ((long)g.nx * g.ny));
+        double xh[24];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double t[8];
+#pragma unroll
+            for (int L = 0; L < 8; ++L) {
+                const long m = (long)(i + (L & 1)) + (long)g.NX * ((j + ((L >> 1) & 1)) + (long)g.NY * (k + (L >> 2)));
+                t[L] = a.u[3 * m + c];
+            }
+            fwd8(t);
+#pragma unroll
+            for (int s = 0; s < 8; ++s) xh[3 * s + c] = t[s];
+        }
+        if (a.ce || a.dc) {
+            double yh[24];
+#pragma unroll
+            for (int r = 0; r < 24; ++r) yh[r] = 0.0;
+#pragma unroll
+            for (int t = 0; t < kParityNnz; ++t) yh[kParityPattern[t].r] += K.v[t] * xh[kParityPattern[t].c];
+            double ce = 0.0;
+#pragma unroll
+            for (int r = 3; r < 24; ++r) ce += xh[r] * yh[r];
+            if (a.ce) a.ce[e] = ce;
+            if (a.dc) {
+                const double xp = a.xphys[e];
+                a.dc[e] = -a.penal * pow(xp, a.penal - 1.0) * (a.E0 - a.Emin) * ce;
+            }
+        }
+        if (a.Emod) {
+            const double Em = a.Emod[e];
+            double sv[6] = {0, 0, 0, 0, 0, 0}, pr[3] = {0, 0, 0}, dv[3] = {0, 0, 0}, l1 = 0.0;
+            if (Em != 0.0) {
+                const double h[3] = {g.dx, g.dy, g.dz};
+                double gr[3][3];  // gr[a][c] = d u_c / d x_a at the centroid
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) gr[ax][c] = xh[3 * (1 << ax) + c] / (4.0 * h[ax]);
+                const double ex = gr[0][0], ey = gr[1][1], ez = gr[2][2];
+                const double gxy = gr[1][0] + gr[0][1], gyz = gr[2][1] + gr[1][2], gxz = gr[2][0] + gr[0][2];
+                sv[0] = Em * (a.lam1 * ex + a.lam2 * (ey + ez));
+                sv[1] = Em * (a.lam1 * ey + a.lam2 * (ex + ez));
+                sv[2] = Em * (a.lam1 * ez + a.lam2 * (ex + ey));
+                sv[3] = Em * a.mu * gxy;
+                sv[4] = Em * a.mu * gyz;
+                sv[5] = Em * a.mu * gxz;
+                vmv = sqrt(0.5 * ((sv[0] - sv[1]) * (sv[0] - sv[1]) + (sv[1] - sv[2]) * (sv[1] - sv[2]) +
+                                  (sv[2] - sv[0]) * (sv[2] - sv[0])) +
+                           3.0 * (sv[3] * sv[3] + sv[4] * sv[4] + sv[5] * sv[5]));
+                if (a.principal || a.l1 || a.dir) {
+                    double S[3][3] = {{sv[0], sv[3], sv[5]}, {sv[3], sv[1], sv[4]}, {sv[5], sv[4], sv[2]}};
+                    eig3_jacobi(S, pr, dv);
+                    l1 = fabs(pr[0]) + fabs(pr[1]) + fabs(pr[2]);
+                }
+            }
+            if (a.vm) a.vm[e] = vmv;
+            if (a.voigt)
+                for (int d = 0; d < 6; ++d) a.voigt[6 * e + d] = sv[d];
+            if (a.principal)
+                for (int d = 0; d < 3; ++d) a.principal[3 * e + d] = pr[d];
+            if (a.dir)
+                for (int d = 0; d < 3; ++d) a.dir[3 * e + d] = dv[d];
+            if (a.l1) a.l1[e] = l1;
+        }
+    }
+    if (a.vm_max_partials) {
+        const double m = block_max<NT>(vmv, red);
+        if (threadIdx.x == 0) a.vm_max_partials[blockIdx.x] = m;
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_max_reduce(const double* __restrict__ in, int nin, double* out) {
+    __shared__ double red[NT / 64];
+    double m = -INFINITY;
+    for (int i = threadIdx.x; i < nin; i += NT) m = fmax(m, in[i]);
+    m = block_max<NT>(m, red);
+    if (threadIdx.x == 0) *out = m;
+}
+
+__global__ __launch_bounds__(NT) void k_simp_modulus(long ne, const double* __restrict__ x, double penal, double E0,
+                                                     double Emin, double* __restrict__ E) {
+    for (long e = gtid(); e < ne; e += gstride()) E[e] = Emin + pow(x[e], penal) * (E0 - Emin);
+}
+
+__device__ __forceinline__ double oc_candidate(double x, double dc, double dv, double lmid, double move,
+                                               double xmin) {
+    const double be = -dc / (dv * lmid);
+    const double cand = x * sqrt(be > 0.0 ? be : 0.0);
+    const double lo = fmax(xmin, x - move), hi = fmin(1.0, x + move);
+    return fmin(fmax(cand, lo), hi);
+}
+
+__global__ void k_oc_init(PcgState* st) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st->scal[0] = 0.0;  // l1
+        st->scal[1] = 1e9;  // l2
+        st->scal[2] = 0.0;  // lmid
+        st->scal[3] = 0.0;  // change
+        st->iter = 0;
+        st->done = 0;
+        for (int i = 0; i < 4; ++i) st->ticket[i] = 0;
+    }
+}
+
+// one bisection step of top88's OC (lines 55-64)
+__global__ __launch_bounds__(NT) void k_oc_eval(long ne, const double* __restrict__ x, const double* __restrict__ dc,
+                                                const double* __restrict__ dv, double volfrac, double move,
+                                                double xmin, double rel_tol, PcgState* st, double* partials) {
+    __shared__ double red[NT / 64];
+    if (st->done) return;
+    const double lmid = 0.5 * (st->scal[0] + st->scal[1]);
+    double s = 0.0;
+    for (long e = gtid(); e < ne; e += gstride()) s += oc_candidate(x[e], dc[e], dv[e], lmid, move, xmin);
+    const double v = block_sum<NT>(s, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = v;
+    if (publish_and_check_last(&st->ticket[3], gridDim.x)) {
+        const double t = reduce_partials<NT>(partials, 1, 0, gridDim.x, red);
+        if (threadIdx.x == 0) {
+            if (t > volfrac * (double)ne) st->scal[0] = lmid;
+            else st->scal[1] = lmid;
+            st->scal[2] = lmid;
+            st->iter += 1;
+            const double l1 = st->scal[0], l2 = st->scal[1];
+            if ((l2 - l1) / (l1 + l2) <= rel_tol || st->iter >= 200) st->done = 1;
+            st->ticket[3] = 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_oc_apply(long ne, const double* __restrict__ x,
+                                                 const double* __restrict__ dc, const double* __restrict__ dv,
+                                                 double move, double xmin, double* __restrict__ xnew, PcgState* st,
+                                                 double* partials) {
+    __shared__ double red[NT / 64];
+    const double lmid = st->scal[2];
+    double ch = 0.0;
+    for (long e = gtid(); e < ne; e += gstride()) {
+        const double v = oc_candidate(x[e], dc[e], dv[e], lmid, move, xmin);
+        ch = fmax(ch, fabs(v - x[e]));
+        xnew[e] = v;
+    }
+    const double v = block_max<NT>(ch, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = v;
+    if (publish_and_check_last(&st->ticket[3], gridDim.x)) {
+        double m = 0.0;
+        for (unsigned b = threadIdx.x; b < gridDim.x; b += NT) m = fmax(m, load_coherent(partials + b));
+        m = block_max<NT>(m, red);
+        if (threadIdx.x == 0) {
+            st->scal[3] = m;
+            st->ticket[3] = 0;
+        }
+    }
+}
+
+inline unsigned blocks_for(long n) { return (unsigned)((n + NT - 1) / NT); }
+
+}  // namespace
+
+// ------------------------------------------------------------- launchers
+void launch_pcg_reset(PcgState* st, double tol, double shift, int max_iter, int check_every, hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_reset, dim3(1), dim3(64), 0, s, st, tol, shift, max_iter, check_every);
+}
+template <typename T>
+void launch_pcg_init(long n, const T* b, T* x, const T* q, const T* minv, T* r, T* p, double shift, PcgState* st,
+                     double* partials, hipStream_t s) {
+    hipLaunchKernelGGL((k_pcg_init<T>), dim3(kVecBlocks), dim3(NT), 0, s, n, b, x, q, minv, r, p, shift, st,
+                       partials);
+}
+template <typename T>
+void launch_pcg_update(long n, T* x, const T* p, T* r, const T* q, const T* minv, PcgState* st, double* partials,
+                       hipStream_t s) {
+    hipLaunchKernelGGL((k_pcg_update<T>), dim3(kVecBlocks), dim3(NT), 0, s, n, x, p, r, q, minv, st, partials);
+}
+template <typename T>
+void launch_pcg_pupdate(long n, const T* r, const T* minv, T* p, const PcgState* st, hipStream_t s) {
+    hipLaunchKernelGGL((k_pcg_pupdate<T>), dim3(kVecBlocks), dim3(NT), 0, s, n, r, minv, p, st);
+}
+void launch_dot(long n, const double* a, const double* b, double* out, unsigned* ticket, double* partials,
+                hipStream_t s) {
+    hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(NT), 0, s, n, a, b, out, ticket, partials);
+}
+void launch_fill(long n, double v, double* a, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill, dim3(kVecBlocks), dim3(NT), 0, s, n, v, a);
+}
+void launch_fill32(long n, float v, float* a, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill32, dim3(kVecBlocks), dim3(NT), 0, s, n, v, a);
+}
+template <typename Tin, typename Tout>
+void launch_convert(long n, const Tin* a, Tout* b, hipStream_t s) {
+    hipLaunchKernelGGL((k_convert<Tin, Tout>), dim3(kVecBlocks), dim3(NT), 0, s, n, a, b);
+}
+void launch_true_residual(long n, const double* b, const double* q, const double* x, const double* minv64,
+                          double shift, double* r64, float* r32, PcgState* st, double* partials, int mode,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_true_residual, dim3(kVecBlocks), dim3(NT), 0, s, n, b, q, x, minv64, shift, r64, r32, st,
+                       partials, mode);
+}
+void launch_ir_resume(PcgState* st, double tol_inner, int first, hipStream_t s) {
+    hipLaunchKernelGGL(k_ir_resume, dim3(1), dim3(64), 0, s, st, tol_inner, first);
+}
+template <typename T>
+void launch_mask(long n, const T* minv, T* x, hipStream_t s) {
+    hipLaunchKernelGGL((k_mask<T>), dim3(kVecBlocks), dim3(NT), 0, s, n, minv, x);
+}
+template void launch_mask<double>(long, const double*, double*, hipStream_t);
+template void launch_mask<float>(long, const float*, float*, hipStream_t);
+void launch_accumulate(long n, double* x64, float* x32, hipStream_t s) {
+    hipLaunchKernelGGL(k_accumulate, dim3(kVecBlocks), dim3(NT), 0, s, n, x64, x32);
+}
+
+void launch_helm_rhs(const GridArgs& g, const double* in_e, double vol8, double* b, hipStream_t s) {
+    hipLaunchKernelGGL(k_helm_rhs, dim3(blocks_for(g.nn)), dim3(NT), 0, s, g, in_e, vol8, b);
+}
+void launch_helm_minv(const GridArgs& g, const HelmArgs& A, double* minv, hipStream_t s) {
+    hipLaunchKernelGGL(k_helm_minv, dim3(blocks_for(g.nn)), dim3(NT), 0, s, g, A, minv);
+}
+int helm_matvec_blocks(const GridArgs& g) { return (int)blocks_for(g.nn); }
+void launch_helm_matvec(const GridArgs& g, const HelmArgs& A, const double* x, double* y, int mode, PcgState* st,
+                        double* partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_helm_matvec, dim3(blocks_for(g.nn)), dim3(NT), 0, s, g, A, x, y, mode, st, partials);
+}
+void launch_helm_extract(const GridArgs& g, const double* xn, double* out_e, hipStream_t s) {
+    hipLaunchKernelGGL(k_helm_extract, dim3(blocks_for(g.ne)), dim3(NT), 0, s, g, xn, out_e);
+}
+int element_post_blocks(const GridArgs& g) { return (int)blocks_for(g.ne); }
+void launch_element_post(const GridArgs& g, const ParityK<double>& K, const ElemArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_element_post, dim3(blocks_for(g.ne)), dim3(NT), 0, s, g, K, a);
+}
+void launch_max_reduce(long n, const double* partials_in, int nin, double* out, hipStream_t s) {
+    (void)n;
+    hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(NT), 0, s, partials_in, nin, out);
+}
+void launch_simp_modulus(long ne, const double* xphys, double penal, double E0, double Emin, double* E,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_simp_modulus, dim3(kVecBlocks), dim3(NT), 0, s, ne, xphys, penal, E0, Emin, E);
+}
+void launch_oc_init(PcgState* st, hipStream_t s) { hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(64), 0, s, st); }
+void launch_oc_eval(long ne, const double* x, const double* dc, const double* dv, double volfrac, double move,
+                    double xmin, double rel_tol, PcgState* st, double* partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_oc_eval, dim3(kVecBlocks), dim3(NT), 0, s, ne, x, dc, dv, volfrac, move, xmin, rel_tol, st,
+                       partials);
+}
+void launch_oc_apply(long ne, const double* x, const double* dc, const double* dv, double move, double xmin,
+                     double* xnew, PcgState* st, double* partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_oc_apply, dim3(kVecBlocks), dim3(NT), 0, s, ne, x, dc, dv, move, xmin, xnew, st, partials);
+}
+
+template void launch_pcg_init<double>(long, const double*, double*, const double*, const double*, double*, double*,
+                                      double, PcgState*, double*, hipStream_t);
+template void launch_pcg_init<float>(long, const float*, float*, const float*, const float*, float*, float*, double,
+                                     PcgState*, double*, hipStream_t);
+template void launch_pcg_update<double>(long, double*, const double*, double*, const double*, const double*,
+                                        PcgState*, double*, hipStream_t);
+template void launch_pcg_update<float>(long, float*, const float*, float*, const float*, const float*, PcgState*,
+                                       double*, hipStream_t);
+template void launch_pcg_pupdate<double>(long, const double*, const double*, double*, const PcgState*, hipStream_t);
+template void launch_pcg_pupdate<float>(long, const float*, const float*, float*, const PcgState*, hipStream_t);
+template void launch_convert<double, float>(long, const double*, float*, hipStream_t);
+template void launch_convert<float, double>(long, const float*, double*, hipStream_t);
+
+}  // namespace hx

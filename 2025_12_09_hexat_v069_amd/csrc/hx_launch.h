@@ -1,0 +1,107 @@
+// hx_launch.h -- host-side launch wrappers of libhexat's kernels.
+#pragma once
+
+#include "hx_internal.h"
+
+namespace hx {
+
+// ---- elasticity operator -------------------------------------------------
+enum MvMode { MV_PLAIN = 0, MV_PCG = 1 };
+
+template <typename T>
+struct MvArgs {
+    const T* x;        // input vector (3*nn)
+    T* y;              // output vector (3*nn)
+    const T* E;        // element moduli (ne)
+    T shift;           // y += shift * x (only where the node is owned)
+    int mode;          // MvMode
+    PcgState* st;      // MV_PCG: pq -> alpha, breakdown test
+    double* partials;  // MV_PCG: one double per block
+};
+
+// tile: 0 -> 16x16 element columns (default), 1 -> 32x8
+template <typename T>
+void launch_matvec_parity(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, int tile, int kchunk,
+                          hipStream_t s);
+template <typename T>
+void launch_matvec_dense(const GridArgs& g, const T* Kbin, const MvArgs<T>& a, int kchunk, hipStream_t s);
+void launch_matvec_gather(const GridArgs& g, const double* Kbin, const MvArgs<double>& a, hipStream_t s);
+int matvec_parity_blocks(const GridArgs& g, int tile, int kchunk);
+int matvec_dense_blocks(const GridArgs& g, int kchunk);
+int matvec_gather_blocks(const GridArgs& g);
+
+
+// Jacobi inverse diagonal: minv = free ? 1/(kd[c] * sum_{e ni n} E_e + shift) : 0
+template <typename T>
+void launch_elastic_minv(const GridArgs& g, const double kd[3], const T* E, const uint8_t* fixed, double shift,
+                         T* minv, hipStream_t s);
+void launch_elastic_diag(const GridArgs& g, const double kd[3], const double* E, double* diag, hipStream_t s);
+
+// ---- PCG vector kernels ----------------------------------------------------
+constexpr int kVecBlocks = 1024;  // blocks of the streaming kernels (multiple of 8)
+template <typename T>
+void launch_pcg_init(long n, const T* b, T* x, const T* q, const T* minv, T* r, T* p, double shift,
+                     PcgState* st, double* partials, hipStream_t s);
+template <typename T>
+void launch_pcg_update(long n, T* x, const T* p, T* r, const T* q, const T* minv, PcgState* st,
+                       double* partials, hipStream_t s);
+template <typename T>
+void launch_pcg_pupdate(long n, const T* r, const T* minv, T* p, const PcgState* st, hipStream_t s);
+void launch_pcg_reset(PcgState* st, double tol, double shift, int max_iter, int check_every, hipStream_t s);
+
+// generic reductions / helpers
+void launch_dot(long n, const double* a, const double* b, double* out, unsigned* ticket, double* partials,
+                hipStream_t s);
+void launch_fill(long n, double v, double* a, hipStream_t s);
+void launch_fill32(long n, float v, float* a, hipStream_t s);
+template <typename Tin, typename Tout>
+void launch_convert(long n, const Tin* a, Tout* b, hipStream_t s);
+// r = mask(b - q - shift*x) in FP64 with FP32 copy; rz, ||r||^2 (scal[4]) and
+// rel; mode bit0 also sets normb, bit1 parks rz in scal[5]
+void launch_true_residual(long n, const double* b, const double* q, const double* x, const double* minv64,
+                          double shift, double* r64, float* r32, PcgState* st, double* partials, int mode,
+                          hipStream_t s);
+void launch_ir_resume(PcgState* st, double tol_inner, int first, hipStream_t s);
+template <typename T>
+void launch_mask(long n, const T* minv, T* x, hipStream_t s);
+// x64 += x32; x32 = 0
+void launch_accumulate(long n, double* x64, float* x32, hipStream_t s);
+
+// ---- Helmholtz filter --------------------------------------------------------
+struct HelmArgs {
+    double Ae[64];  // reference node order (R^2 Ks + Ms)
+};
+void launch_helm_rhs(const GridArgs& g, const double* in_e, double vol8, double* b, hipStream_t s);
+void launch_helm_minv(const GridArgs& g, const HelmArgs& A, double* minv, hipStream_t s);
+void launch_helm_matvec(const GridArgs& g, const HelmArgs& A, const double* x, double* y, int mode, PcgState* st,
+                        double* partials, hipStream_t s);
+int helm_matvec_blocks(const GridArgs& g);
+void launch_helm_extract(const GridArgs& g, const double* xn, double* out_e, hipStream_t s);
+
+// ---- element kernels -----------------------------------------------------------
+struct ElemArgs {
+    const double* u;
+    const double* Emod;      // stress modulus per element (nullable -> skip stress)
+    const double* xphys;     // for dc (nullable -> skip dc)
+    double penal, E0, Emin;
+    double lam1, lam2, mu;   // unit-E material: c1 = (1-nu)f, c2 = nu f, c3 = (1-2nu)/2 f
+    double* principal;       // nullable outputs
+    double* vm;
+    double* voigt;
+    double* l1;
+    double* dir;
+    double* ce;
+    double* dc;
+    double* vm_max_partials; // nullable: per-block max vm
+};
+void launch_element_post(const GridArgs& g, const ParityK<double>& Kunit, const ElemArgs& a, hipStream_t s);
+int element_post_blocks(const GridArgs& g);
+void launch_simp_modulus(long ne, const double* xphys, double penal, double E0, double Emin, double* E, hipStream_t s);
+void launch_oc_init(PcgState* st, hipStream_t s);
+void launch_oc_eval(long ne, const double* x, const double* dc, const double* dv, double volfrac, double move,
+                    double xmin, double rel_tol, PcgState* st, double* partials, hipStream_t s);
+void launch_oc_apply(long ne, const double* x, const double* dc, const double* dv, double move, double xmin,
+                     double* xnew, PcgState* st, double* partials, hipStream_t s);
+void launch_max_reduce(long n, const double* partials_in, int nin, double* out, hipStream_t s);
+
+}  // namespace hx

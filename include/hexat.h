@@ -1,0 +1,213 @@
+/*
+ * hexat.h -- C ABI of libhexat.so, the MI355X (gfx950) hot path of the HEXAT
+ * topology-optimisation loop.  Plain pointers and sizes only; every array is
+ * caller-owned host memory unless a function says otherwise.  One context per
+ * host thread; a context is not thread-safe.  All functions return an
+ * hx_status (0 = ok); hx_last_error() gives the message of the last failure.
+ *
+ * Structured grid conventions (0-based; the reference is 1-based Julia):
+ *   nx*ny*nz box elements of size dx*dy*dz, nodes NX=nx+1, NY, NZ,
+ *   node(i,j,k) = i + NX*(j + NY*k), elem(i,j,k) = i + nx*(j + ny*k),
+ *   DOF = 3*node + d (node-major, d = x,y,z).
+ * This is exactly the layout of Mesh.generate_mesh (Mesh.jl:23-60) and of the
+ * DOF numbering of GPUSolver.jl:50-57, so a Julia caller passes its arrays
+ * unchanged (Float64 copies) and drops the explicit connectivity, which the
+ * kernels compute from (i,j,k).
+ */
+#ifndef HEXAT_H
+#define HEXAT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    HX_OK = 0,
+    HX_ERR_NO_DEVICE = 1,     /* GPUSolver.jl:620 error("CUDA not found.") */
+    HX_ERR_BAD_ARG = 2,       /* invalid size / pointer / option */
+    HX_ERR_NOT_CONVERGED = 3, /* max_iter reached; best iterate returned (GPUSolver.jl:603-606) */
+    HX_ERR_BREAKDOWN = 4,     /* |p'Ap| tiny or NaN (GPUSolver.jl:462-466) */
+    HX_ERR_DIVERGED = 5,      /* NaN/Inf, > 1000*|b|, or > 100x best after 50 its (:480-490) */
+    HX_ERR_NO_ACTIVE = 6,     /* no element with non-zero stiffness (GPUSolver.jl:132-134) */
+    HX_ERR_HIP = 7,           /* HIP runtime failure */
+    HX_ERR_STATE = 8,         /* call order (e.g. no grid set) */
+    HX_ERR_OOM = 9,           /* device allocation failed */
+    HX_ZERO_RHS = 10          /* informational: zero right-hand side, zero solution (GPUSolver.jl:373) */
+} hx_status;
+
+typedef struct hx_ctx hx_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+int hx_version(void);
+/* number of visible HIP devices (0 when none) */
+int hx_device_count(int* count);
+/* create a context on a device; HX_ERR_NO_DEVICE when there is none */
+int hx_ctx_create(int device, hx_ctx** out);
+void hx_ctx_destroy(hx_ctx* ctx);
+const char* hx_last_error(const hx_ctx* ctx);
+
+/* Set the structured grid and Poisson ratio.  Builds Ke (replaces
+ * GPUSolver.setup_matrix_free_operator, GPUSolver.jl:124-150, whose dx,dy,dz
+ * come from element 1) and (re)allocates the device workspace. */
+int hx_grid_set(hx_ctx* ctx, int nx, int ny, int nz, double dx, double dy, double dz, double nu);
+/* fixed-DOF mask, 3*NX*NY*NZ bytes, non-zero = fixed.  Replaces
+ * GPUSolver.get_free_dofs (GPUSolver.jl:112-122) + free<->full maps (:81-97). */
+int hx_set_fixed(hx_ctx* ctx, const uint8_t* fixed);
+
+/* ---- element matrices (host, FP64) ---------------------------------------- */
+/* 24x24 row-major H8 stiffness of a dx*dy*dz box (Element.jl:64-102) */
+int hx_element_stiffness(double dx, double dy, double dz, double nu, double E, double* Ke);
+/* 8x8 scalar Laplacian / mass (Element.jl:104-125); quirk=1 keeps the
+ * reference's top face at z=+dz */
+int hx_scalar_matrices(double dx, double dy, double dz, int quirk, double* Ks, double* Ms);
+
+/* ---- elasticity ------------------------------------------------------------ */
+enum { HX_FP64 = 0, HX_FP32_IR = 1 };           /* precision form */
+enum { HX_PC_JACOBI = 0, HX_PC_LEARNED = 1 };   /* preconditioner */
+enum { HX_MV_AUTO = 0, HX_MV_PARITY = 1, HX_MV_DENSE = 2, HX_MV_GATHER = 3, HX_MV_MFMA = 4 };
+
+typedef struct {
+    double tol;       /* relative residual ||r||/||b|| target */
+    int max_iter;     /* PCG iteration cap */
+    double shift;     /* absolute diagonal shift s (K_ff + s I), GPUSolver.jl:391 */
+    int precision;    /* HX_FP64 | HX_FP32_IR */
+    int precond;      /* HX_PC_JACOBI | HX_PC_LEARNED */
+    int warm_start;   /* 1: u_inout holds the start vector (GPUSolver.jl:407-412) */
+    int matvec;       /* HX_MV_* kernel variant (HX_MV_AUTO = tuned choice) */
+    int batch;        /* iterations per host check (0 = default) */
+} hx_pcg_opts;
+
+typedef struct {
+    int iterations;   /* PCG iterations (inner iterations for HX_FP32_IR) */
+    int status;       /* hx_status of the solve */
+    double rel_res;   /* final ||b - K u||/||b||, recomputed in FP64 */
+    double seconds;   /* device time of the solve */
+    int refinements;  /* FP64 residual replacements (HX_FP32_IR) */
+    int fallbacks;    /* learned -> Jacobi fallbacks */
+} hx_pcg_stats;
+
+/* y = sum_e Ee[e] Ke x_e over all DOFs (no mask); Ee[e] = 0 drops the element.
+ * Replaces GPUSolver.apply_matrix_free_operator! (GPUSolver.jl:152-162). */
+int hx_matvec(hx_ctx* ctx, const double* Ee, const double* x, double* y, int variant);
+/* Jacobi diagonal sum_e Ee[e] diag(Ke) (GPUSolver.jl:301-317) */
+int hx_diagonal(hx_ctx* ctx, const double* Ee, double* diag);
+/* Solve (K_ff + s I) u = f on free DOFs; fixed DOFs of u are 0 on return.
+ * Replaces GPUSolver.gpu_matrix_free_cg_solve (GPUSolver.jl:274-609). */
+int hx_pcg_solve(hx_ctx* ctx, const double* Ee, const double* f, double* u_inout,
+                 const hx_pcg_opts* opts, hx_pcg_stats* stats);
+/* nrhs independent systems with one shared operator, solved together
+ * (fused multi-RHS kernels).  F, U are nrhs x ndof row-major (RHS-major).
+ * stats: nrhs entries.  No reference counterpart (the reference solves one
+ * load case per call, Main.jl:414-418). */
+int hx_block_pcg_solve(hx_ctx* ctx, int nrhs, const double* Ee, const double* F, double* U,
+                       const hx_pcg_opts* opts, hx_pcg_stats* stats);
+
+/* ---- Helmholtz PDE filter ------------------------------------------------- */
+typedef struct {
+    int quirk;        /* 1: reference element (top at +dz, constant -> x2/3) */
+    double tol;       /* relative residual target (reference filter_tolerance 1e-5) */
+    int max_iter;     /* reference: 200 (GPUHelmholtz.jl:191) */
+    int check_every;  /* residual test period (reference: 10, GPUHelmholtz.jl:208) */
+} hx_filter_opts;
+
+typedef struct {
+    int iterations;
+    int status;
+    double rel_res;
+    double seconds;
+} hx_filter_stats;
+
+/* out_e = mean over the element's nodes of xn, (R^2 Ks + Ms) xn = sum_e in_e vol/8.
+ * Replaces GPUHelmholtz.apply_gpu_filter! (GPUHelmholtz.jl:344-376) and
+ * solve_helmholtz_on_gpu (:165-229). */
+int hx_helmholtz_filter(hx_ctx* ctx, double R, const hx_filter_opts* opts, const double* in_e,
+                        double* out_e, hx_filter_stats* stats);
+
+/* ---- element post-processing ---------------------------------------------- */
+/* Centroid stress of material with modulus Ee[e] (Ee = 0 -> zeros).  Any
+ * output may be NULL.  principal: Ne x 3 descending; voigt: Ne x 6
+ * (xx,yy,zz,xy,yz,xz); dir: Ne x 3 unit eigenvector of the largest principal
+ * stress (sign arbitrary).  Replaces Stress.compute_stress_field (Stress.jl:57-100). */
+int hx_element_stress(hx_ctx* ctx, const double* Ee, const double* u, double* principal,
+                      double* vm, double* voigt, double* l1, double* dir);
+/* ce_e = u_e^T Ke(E=1) u_e and SIMP sensitivity
+ * dc_e = -penal xphys^(penal-1) (E0-Emin) ce_e; compliance = f^T u.
+ * No reference counterpart (the reference has no sensitivities). */
+int hx_compliance_sens(hx_ctx* ctx, const double* xphys, double penal, double E0, double Emin,
+                       const double* u, const double* f, double* ce, double* dc, double* compliance);
+/* OC bisection (top88): xnew = clamp(x sqrt(-dc/(dv lmid)), max(xmin,x-move), min(1,x+move)),
+ * volume tested on mean(xnew); stops at (l2-l1)/(l1+l2) <= rel_tol.
+ * No reference counterpart. */
+int hx_oc_update(hx_ctx* ctx, const double* x, const double* dc, const double* dv, double volfrac,
+                 double move, double xmin, double rel_tol, double* xnew, double* lmid, int* steps,
+                 double* change);
+
+/* ---- device-resident SIMP/OC topology-optimisation loop -------------------- */
+typedef struct {
+    double E0, Emin, nu, penal, volfrac, move, xmin;
+    double rmin;          /* filter radius r; the PDE filter uses R = r / (2 sqrt 3) */
+    double pcg_tol;       /* elasticity relative residual target */
+    int pcg_max_iter;
+    int precision;        /* HX_FP64 | HX_FP32_IR */
+    int precond;          /* HX_PC_JACOBI | HX_PC_LEARNED */
+    double filter_tol;    /* Helmholtz filter relative residual target */
+    int filter_max_iter;
+    double oc_rel_tol;    /* OC bisection stop */
+    int with_stress;      /* 1: von Mises at centroids every iteration */
+} hx_to_params;
+
+typedef struct {
+    int iter;
+    double compliance;    /* f^T u */
+    double change;        /* max |x_new - x| */
+    double volume;        /* mean(xphys) after the update */
+    int pcg_iterations;
+    double pcg_rel_res;
+    int pcg_status;
+    int filter_iterations;/* sum over the two filter solves */
+    double max_vm;        /* max von Mises stress (with_stress) */
+    double seconds;       /* device time of the iteration */
+} hx_to_stats;
+
+enum { HX_FIELD_X = 0, HX_FIELD_XPHYS = 1, HX_FIELD_U = 2, HX_FIELD_VM = 3, HX_FIELD_DC = 4 };
+
+/* Upload f (ndof) and x0 (Ne); xphys = filter(x0). */
+int hx_to_init(hx_ctx* ctx, const hx_to_params* params, const double* f, const double* x0);
+/* One TO iteration on the device: E(xphys), PCG (warm start), compliance,
+ * sensitivities (+ von Mises), sensitivity filter, OC, density filter. */
+int hx_to_step(hx_ctx* ctx, hx_to_stats* stats);
+/* copy a device field (HX_FIELD_*) to host (Ne or ndof doubles) */
+int hx_to_read(hx_ctx* ctx, int field, double* out);
+
+/* ---- learned preconditioner (BF16 3-D conv U-Net) --------------------------- */
+/* Train the network on (density, residual, error) triples that the Jacobi-PCG
+ * of the current grid generates; loss, epochs and timings in the stats. */
+typedef struct {
+    int samples;      /* (rho, r, e) triples to harvest */
+    int epochs;
+    double lr;
+    int seed;
+    double final_loss;
+    double seconds;
+} hx_learn_stats;
+int hx_learned_train(hx_ctx* ctx, const double* Ee, const double* f, hx_learn_stats* stats);
+/* copy of the current weights (BF16 bit patterns) and their count */
+int hx_learned_weights(hx_ctx* ctx, uint16_t* out, int* count);
+
+/* ---- kernel timing ---------------------------------------------------------- */
+enum { HX_K_MATVEC = 0, HX_K_UPDATE = 1, HX_K_PUPDATE = 2, HX_K_HELM = 3, HX_K_ELEM = 4,
+       HX_K_MATVEC32 = 5, HX_K_COUNT = 6 };
+/* 1: bracket each hot-kernel launch with HIP events on the context stream */
+int hx_timing_enable(hx_ctx* ctx, int on);
+/* average launch duration (ms) and launch count of kernel class `kernel`
+ * since the last hx_timing_enable(ctx, 1) */
+int hx_timing_read(hx_ctx* ctx, int kernel, double* avg_ms, long* count);
+/* hardware facts used for roofline reporting */
+int hx_device_info(hx_ctx* ctx, int* cu_count, int* clock_khz, double* mem_gb);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HEXAT_H */

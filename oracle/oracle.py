@@ -1,0 +1,376 @@
+"""CPU oracle for the HEXAT topology-optimisation hot path.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker that the HIP path is
+compared against; it may be imported only by ``tests/``, by
+``__graft_entry__.smoke()`` and by the ``cpu_baseline`` leg of ``bench.py``.
+The product package never imports it.
+
+Numeric kernels live in ``hexat_oracle.c`` (FP64 restatement of the
+reference's Julia algorithms, built by ``oracle/Makefile`` into
+``liboracle.so``); this file wraps them with ctypes and restates the host-side
+pieces in numpy:
+
+* ``parity_update_density`` -- TopOpt.jl:139-405 (stress-ratio update with the
+  radius schedule, Helmholtz filter, clamp, culling and protected restore).
+* ``bc_indicator`` / ``external_forces`` -- Boundary.jl:56-291 location
+  grammar on a structured grid.
+* ``simp_*`` -- textbook SIMP/OC (Andreassen et al. 2011 "top88", Liu & Tovar
+  2014 "top3d", Lazarov & Sigmund 2011 PDE filter); no reference code exists.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+class OxGrid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        P = C.POINTER
+        d, i = C.c_double, C.c_int
+        pd = P(C.c_double)
+        pu8 = P(C.c_uint8)
+        g = P(OxGrid)
+        L.ox_material_matrix.argtypes = [d, d, pd]
+        L.ox_ke_hex.argtypes = [d, d, d, d, d, pd]
+        L.ox_scalar_ke_me.argtypes = [d, d, d, i, pd, pd]
+        L.ox_matvec.argtypes = [g, pd, pd, pd, pd]
+        L.ox_diag.argtypes = [g, pd, pd, pd]
+        L.ox_pcg.argtypes = [g, pd, pd, pu8, pd, pd, d, d, i, pd, P(C.c_int)]
+        L.ox_pcg.restype = C.c_int
+        L.ox_helmholtz.argtypes = [g, d, i, pd, pd, d, i, i, pd, P(C.c_int)]
+        L.ox_helmholtz.restype = C.c_int
+        L.ox_helm_matvec.argtypes = [g, pd, pd, pd]
+        L.ox_stress.argtypes = [g, d, d, pd, pd, pd, pd, pd, pd, pd]
+        L.ox_element_energy.argtypes = [g, pd, pd, pd]
+        L.ox_eig3.argtypes = [pd, pd, pd]
+        L.ox_oc.argtypes = [i, pd, pd, pd, d, d, d, d, pd, pd]
+        L.ox_oc.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Grid:
+    """Structured grid of nx*ny*nz box elements (Mesh.jl:23-60)."""
+
+    def __init__(self, nx, ny, nz, dx=1.0, dy=1.0, dz=1.0):
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.dx, self.dy, self.dz = float(dx), float(dy), float(dz)
+        self.c = OxGrid(self.nx, self.ny, self.nz, self.dx, self.dy, self.dz)
+
+    @property
+    def ne(self):
+        return self.nx * self.ny * self.nz
+
+    @property
+    def nn(self):
+        return (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
+
+    @property
+    def ndof(self):
+        return 3 * self.nn
+
+    def node_coords(self):
+        NX, NY, NZ = self.nx + 1, self.ny + 1, self.nz + 1
+        k, j, i = np.meshgrid(np.arange(NZ), np.arange(NY), np.arange(NX), indexing="ij")
+        return np.stack([i.ravel() * self.dx, j.ravel() * self.dy, k.ravel() * self.dz], axis=1)
+
+    def elements(self):
+        """0-based Ne x 8 connectivity in reference node order n1..n8."""
+        NX, NY = self.nx + 1, self.ny + 1
+        k, j, i = np.meshgrid(np.arange(self.nz), np.arange(self.ny), np.arange(self.nx), indexing="ij")
+        i, j, k = i.ravel(), j.ravel(), k.ravel()
+        nid = lambda a, b, c: a + NX * (b + NY * c)
+        return np.stack([nid(i, j, k), nid(i + 1, j, k), nid(i + 1, j + 1, k), nid(i, j + 1, k),
+                         nid(i, j, k + 1), nid(i + 1, j, k + 1), nid(i + 1, j + 1, k + 1),
+                         nid(i, j + 1, k + 1)], axis=1)
+
+
+def material_matrix(E, nu):
+    D = np.zeros(36)
+    lib().ox_material_matrix(E, nu, _p(D))
+    return D.reshape(6, 6)
+
+
+def ke_hex(dx, dy, dz, nu, E=1.0):
+    K = np.zeros(576)
+    lib().ox_ke_hex(dx, dy, dz, nu, E, _p(K))
+    return K.reshape(24, 24)
+
+
+def scalar_ke_me(dx, dy, dz, quirk):
+    Ks, Ms = np.zeros(64), np.zeros(64)
+    lib().ox_scalar_ke_me(dx, dy, dz, int(quirk), _p(Ks), _p(Ms))
+    return Ks.reshape(8, 8), Ms.reshape(8, 8)
+
+
+def matvec(g: Grid, Ke, Ee, x):
+    Ke, Ee, x = _f64(Ke), _f64(Ee), _f64(x)
+    y = np.empty(g.ndof)
+    lib().ox_matvec(C.byref(g.c), _p(Ke), _p(Ee), _p(x), _p(y))
+    return y
+
+
+def diag(g: Grid, Ke, Ee):
+    Ke, Ee = _f64(Ke), _f64(Ee)
+    y = np.empty(g.ndof)
+    lib().ox_diag(C.byref(g.c), _p(Ke), _p(Ee), _p(y))
+    return y
+
+
+def pcg(g: Grid, Ke, Ee, fixed, b, x0=None, shift=0.0, tol=1e-8, max_iter=100000):
+    Ke, Ee, b = _f64(Ke), _f64(Ee), _f64(b)
+    fixed = np.ascontiguousarray(fixed, dtype=np.uint8)
+    x = np.zeros(g.ndof) if x0 is None else _f64(x0).copy()
+    rel = C.c_double(0.0)
+    it = C.c_int(0)
+    st = lib().ox_pcg(C.byref(g.c), _p(Ke), _p(Ee), fixed.ctypes.data_as(C.POINTER(C.c_uint8)),
+                      _p(b), _p(x), shift, tol, int(max_iter), C.byref(rel), C.byref(it))
+    return x, st, rel.value, it.value
+
+
+def helmholtz(g: Grid, R, quirk, rho, tol=1e-5, max_iter=200, check_every=10):
+    rho = _f64(rho)
+    out = np.empty(g.ne)
+    rel = C.c_double(0.0)
+    it = C.c_int(0)
+    st = lib().ox_helmholtz(C.byref(g.c), R, int(quirk), _p(rho), _p(out), tol, int(max_iter),
+                            int(check_every), C.byref(rel), C.byref(it))
+    return out, st, rel.value, it.value
+
+
+def helm_matvec(g: Grid, Ae, x):
+    Ae, x = _f64(Ae), _f64(x)
+    y = np.empty(g.nn)
+    lib().ox_helm_matvec(C.byref(g.c), _p(Ae), _p(x), _p(y))
+    return y
+
+
+def stress(g: Grid, nu, E, rho, u):
+    rho, u = _f64(rho), _f64(u)
+    ne = g.ne
+    pr, vm, sv, l1, dr = np.empty(3 * ne), np.empty(ne), np.empty(6 * ne), np.empty(ne), np.empty(3 * ne)
+    lib().ox_stress(C.byref(g.c), nu, E, _p(rho), _p(u), _p(pr), _p(vm), _p(sv), _p(l1), _p(dr))
+    return pr.reshape(ne, 3), vm, sv.reshape(ne, 6), l1, dr.reshape(ne, 3)
+
+
+def element_energy(g: Grid, Ke, u):
+    Ke, u = _f64(Ke), _f64(u)
+    ce = np.empty(g.ne)
+    lib().ox_element_energy(C.byref(g.c), _p(Ke), _p(u), _p(ce))
+    return ce
+
+
+def eig3(A):
+    A = _f64(A).reshape(9)
+    v, V = np.empty(3), np.empty(9)
+    lib().ox_eig3(_p(A), _p(v), _p(V))
+    return v, V.reshape(3, 3)
+
+
+def oc(x, dc, dv, volfrac, move=0.2, xmin=0.0, rel_tol=1e-3):
+    x, dc, dv = _f64(x), _f64(dc), _f64(dv)
+    xn = np.empty_like(x)
+    lm = C.c_double(0.0)
+    steps = lib().ox_oc(x.size, _p(x), _p(dc), _p(dv), volfrac, move, xmin, rel_tol, _p(xn), C.byref(lm))
+    return xn, lm.value, steps
+
+
+# ---------------------------------------------------------------------------
+# Boundary grammar (Boundary.jl:56-291) on a structured grid
+# ---------------------------------------------------------------------------
+
+def _resolve(spec, cmin, cmax):
+    """Boundary.jl:117-130"""
+    if isinstance(spec, (int, float)) and not isinstance(spec, bool):
+        if 0.0 <= spec <= 1.0:
+            return cmin + spec * (cmax - cmin)
+        return float(spec)
+    if isinstance(spec, str) and spec.endswith("%"):
+        frac = min(max(float(spec[:-1]) / 100.0, 0.0), 1.0)
+        return cmin + frac * (cmax - cmin)
+    return cmin
+
+
+def _mask(spec, coords, cmin, cmax):
+    """Boundary.jl:79-115"""
+    tol = 1e-4 * max(1.0, abs(cmax - cmin))
+    if spec == ":":
+        return np.ones(coords.shape[0], bool)
+    return np.abs(coords - _resolve(spec, cmin, cmax)) <= tol
+
+
+def nodes_by_location(loc, X):
+    m = np.ones(X.shape[0], bool)
+    for d in range(3):
+        m &= _mask(loc[d], X[:, d], X[:, d].min(), X[:, d].max())
+    return np.nonzero(m)[0]
+
+
+def bc_indicator(X, bc_data):
+    """Boundary.jl:135-160 -> (nNodes, 3) float mask"""
+    ind = np.zeros((X.shape[0], 3))
+    for bc in bc_data:
+        nodes = nodes_by_location(bc["location"], X)
+        for d in bc["DoFs"]:
+            ind[nodes, d - 1] = 1.0
+    return ind
+
+
+def nearest_node(target, X):
+    """Boundary.jl:195-224 (ties -> larger x, then y, then z)"""
+    best, best_d = -1, math.inf
+    for i in range(X.shape[0]):
+        d2 = float(np.sum((X[i] - target) ** 2))
+        if d2 < best_d - 1e-9:
+            best, best_d = i, d2
+        elif abs(d2 - best_d) <= 1e-9 and best >= 0:
+            if tuple(X[i]) > tuple(X[best]):
+                best = i
+    return best
+
+
+def external_forces(X, forces):
+    """Boundary.jl:232-291"""
+    F = np.zeros(3 * X.shape[0])
+    for f in forces:
+        nodes = nodes_by_location(f["location"], X)
+        if nodes.size == 0 and all(c != ":" for c in f["location"]):
+            tgt = np.array([_resolve(f["location"][d], X[:, d].min(), X[:, d].max()) for d in range(3)])
+            nodes = np.array([nearest_node(tgt, X)])
+        if nodes.size == 0:
+            continue
+        fv = np.zeros(3)
+        fr = list(f["F"])[:3]
+        fv[:len(fr)] = fr
+        s = 1.0 / nodes.size
+        for d in range(3):
+            F[3 * nodes + d] += s * fv[d]
+    return F
+
+
+# ---------------------------------------------------------------------------
+# Parity-mode density update (TopOpt.jl:139-405), numpy restatement
+# ---------------------------------------------------------------------------
+
+def radius_schedule(iter_, n_iter, R_init_perc, R_interm_perc, R_final_perc, interm_iter_perc,
+                    max_dim, avg_h):
+    """TopOpt.jl:216-255 -> (R_length, R_effective)"""
+    Ri, Rm, Rf = (R_init_perc / 100 * max_dim, R_interm_perc / 100 * max_dim, R_final_perc / 100 * max_dim)
+    it_m = max(1, int(round(interm_iter_perc / 100 * n_iter)))
+    c = min(iter_, n_iter)
+    if c <= it_m:
+        t = (c - 1) / (it_m - 1) if it_m > 1 else 0.0
+        R = Ri * (1 - t) + Rm * t
+    else:
+        t = (c - it_m) / (n_iter - it_m) if n_iter > it_m else 0.0
+        R = Rm * (1 - t) + Rf * t
+    R = max(R, 1.2 * avg_h)
+    return R, R / 2.5
+
+
+def parity_update_density(g: Grid, density, l1, protected, E, l1_allowable, iter_, n_iter,
+                          original, min_density, max_density, opt, max_dim, filter_tol=1e-5):
+    """TopOpt.jl:139-405.  Mutates density and l1 in place; returns
+    (max_change, R_length, threshold)."""
+    avg_h = (g.dx + g.dy + g.dz) / 3.0
+    ratio = float(opt.get("stress_regularization_ratio", 0.0))
+    if ratio > 1e-6:  # :176-211
+        fs, _, _, _ = helmholtz(g, ratio * avg_h, 1, l1, tol=filter_tol)
+        l1[:] = fs
+    R_len, R_eff = radius_schedule(iter_, n_iter, float(opt.get("filter_R_init_perc", 0.0)),
+                                   float(opt.get("filter_R_interm_perc", 0.0)),
+                                   float(opt.get("filter_R_final_perc", 0.0)),
+                                   float(opt.get("filter_R_interm_iter_perc", 50.0)), max_dim, avg_h)
+    prop = np.where(protected, original, np.clip(l1 / l1_allowable / E, min_density, max_density))
+    filt = prop
+    if R_eff > 1e-4:  # :281-322
+        filt, _, _, _ = helmholtz(g, R_eff, 1, prop, tol=filter_tol)
+    new = np.clip(filt, min_density, max_density)
+    diff = np.where(protected, 0.0, np.abs(new - density))
+    max_change = float(diff.max()) if diff.size else 0.0
+    density[:] = np.where(protected, density, new)
+    fin = float(opt.get("final_density_threshold", 0.95))
+    mcr = float(opt.get("max_culling_ratio", 0.1))
+    thr = fin if iter_ > n_iter else fin * (iter_ / n_iter)  # :354-359
+    act = np.where(protected, original > min_density, density > min_density)
+    cand = np.nonzero((~protected) & (density > min_density) & (density < thr))[0]
+    max_cull = int(math.floor(int(act.sum()) * mcr))
+    if cand.size > max_cull:  # :383-395 (stable sort by density)
+        order = np.argsort(density[cand], kind="stable")
+        density[cand[order[:max_cull]]] = min_density
+    else:
+        density[cand] = min_density
+    density[protected] = original[protected]
+    return max_change, R_len, thr
+
+
+# ---------------------------------------------------------------------------
+# Textbook SIMP pieces
+# ---------------------------------------------------------------------------
+
+def simp_modulus(xphys, penal, E0, Emin):
+    return Emin + xphys ** penal * (E0 - Emin)
+
+
+def simp_dc(xphys, ce, penal, E0, Emin):
+    return -penal * xphys ** (penal - 1) * (E0 - Emin) * ce
+
+
+def simp_to_loop(g: Grid, f, fixed, n_iter, nu=0.3, E0=1.0, Emin=1e-9, penal=3.0, volfrac=0.3, rmin=1.5,
+                 move=0.2, xmin=0.0, pcg_tol=1e-8, filter_tol=1e-10, oc_rel_tol=1e-9, x0=None,
+                 pcg_max_iter=200000, filter_max_iter=5000):
+    """Textbook SIMP/OC loop (top88/top3d order, PDE filter of Lazarov & Sigmund
+    with R = rmin/(2 sqrt 3)) on the oracle kernels.  Returns a list of per-
+    iteration dicts and the final (x, xphys, u)."""
+    Ke = ke_hex(g.dx, g.dy, g.dz, nu)
+    R = rmin / (2.0 * math.sqrt(3.0))
+    x = np.full(g.ne, volfrac) if x0 is None else np.array(x0, dtype=np.float64)
+    filt = lambda v: helmholtz(g, R, 0, v, tol=filter_tol, max_iter=filter_max_iter, check_every=1)[0]
+    xphys = filt(x)
+    dv = filt(np.ones(g.ne))
+    u = np.zeros(g.ndof)
+    hist = []
+    for it in range(1, n_iter + 1):
+        E = simp_modulus(xphys, penal, E0, Emin)
+        u, st, rel, its = pcg(g, Ke, E, fixed, f, x0=u, tol=pcg_tol, max_iter=pcg_max_iter)
+        c = float(f @ u)
+        ce = element_energy(g, Ke, u)
+        dcf = filt(simp_dc(xphys, ce, penal, E0, Emin))
+        xnew, lmid, steps = oc(x, dcf, dv, volfrac, move, xmin, oc_rel_tol)
+        change = float(np.abs(xnew - x).max())
+        xphys = filt(xnew)
+        x = xnew
+        hist.append({"iter": it, "compliance": c, "change": change, "pcg_iterations": its, "pcg_rel": rel,
+                     "volume": float(xphys.mean()), "lmid": lmid})
+    return hist, x, xphys, u
