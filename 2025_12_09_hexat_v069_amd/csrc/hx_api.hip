@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -56,6 +57,7 @@ struct TimerPair {
 };
 
 constexpr double kIrDelta = 0.05;  // reliable update when ||r|| drops by this factor
+constexpr int kMaxRhs = 8;         // columns of a block solve
 
 }  // namespace
 
@@ -80,8 +82,11 @@ struct hx_ctx {
     DevBuf<double> E, f, x, r, p, q, minv, Kbin_d;
     DevBuf<float> Kbin32_d, E32, x32, r32, p32, q32, minv32;
     DevBuf<double> partials;
-    PcgState* st = nullptr;
-    PcgState* st_host = nullptr;
+    PcgState* st = nullptr;       // kMaxRhs device states
+    PcgState* st_host = nullptr;  // pinned mirror
+    // block (multi-RHS) buffers, nrhs * ndof each
+    DevBuf<double> b_f, b_x, b_r, b_p, b_q;
+    DevBuf<float> b_x32, b_r32, b_p32, b_q32;
     DevBuf<double> h_b, h_x, h_r, h_p, h_q, h_minv, h_in, h_out;
     DevBuf<double> e_pr, e_vm, e_sv, e_l1, e_dir, e_ce, e_dc, e_aux, small;
     // SIMP session
@@ -89,6 +94,9 @@ struct hx_ctx {
     hx_to_params tp{};
     int to_iter = 0;
     DevBuf<double> t_x, t_xphys, t_xnew, t_dc, t_dcf, t_dv, t_vm, t_ones;
+    // captured iteration batches (hipGraph), keyed by kernel arguments
+    bool graphs_on = true;
+    std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
     // timing
     bool timing = false;
     std::vector<TimerPair> pending;
@@ -166,8 +174,53 @@ void flush_timing(hx_ctx* c) {
     c->pending.clear();
 }
 
-int sync_state(hx_ctx* c) {
-    HXC(hipMemcpyAsync(c->st_host, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, c->stream));
+void clear_graphs(hx_ctx* c) {
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
+    c->graphs.clear();
+}
+
+// Launch one batch of iterations.  Without timing, the batch is captured
+// once into a hipGraph (kernel nodes only) and replayed; the key holds every
+// kernel argument that can change between calls.
+template <typename F>
+int run_batch(hx_ctx* c, const std::string& key, F&& launch_batch) {
+    if (c->timing || !c->graphs_on) {
+        launch_batch();
+        HXK();
+        return HX_OK;
+    }
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : c->graphs)
+        if (g.first == key) exec = g.second;
+    if (!exec) {
+        hipGraph_t graph = nullptr;
+        HXC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        launch_batch();
+        const hipError_t le = hipGetLastError();
+        HXC(hipStreamEndCapture(c->stream, &graph));
+        if (le != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return fail(c, HX_ERR_HIP, std::string("capture: ") + hipGetErrorString(le));
+        }
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) return fail(c, HX_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+        if (c->graphs.size() > 64) clear_graphs(c);
+        c->graphs.emplace_back(key, exec);
+    }
+    HXC(hipGraphLaunch(exec, c->stream));
+    return HX_OK;
+}
+
+std::string batch_key(const char* kind, int nr, int batch, int variant, double shift, const void* a, const void* b,
+                      const void* d, const void* e, const void* f) {
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%s|%d|%d|%d|%a|%p|%p|%p|%p|%p", kind, nr, batch, variant, shift, a, b, d, e, f);
+    return buf;
+}
+
+int sync_state(hx_ctx* c, int nrhs = 1) {
+    HXC(hipMemcpyAsync(c->st_host, c->st, nrhs * sizeof(PcgState), hipMemcpyDeviceToHost, c->stream));
     HXC(hipStreamSynchronize(c->stream));
     flush_timing(c);
     return HX_OK;
@@ -187,21 +240,24 @@ int status_to_code(int s) {
 long ndof(const hx_ctx* c) { return 3 * c->g.nn; }
 
 // y = K(E) x (+ shift x); MV_PCG mode also reduces p'q into the state.
-int mv64(hx_ctx* c, const double* x, double* y, const double* E, double shift, int mode, int variant) {
-    MvArgs<double> a{x, y, E, shift, mode, c->st, c->partials.p};
+long ndof(const hx_ctx* c);
+
+int mv64(hx_ctx* c, const double* x, double* y, const double* E, double shift, int mode, int variant,
+         int nrhs = 1) {
+    MvArgs<double> a{x, y, E, shift, mode, c->st, c->partials.p, ndof(c)};
     const int v = variant == HX_MV_AUTO ? c->mv_variant : variant;
     TimeScope ts(c, HX_K_MATVEC);
-    if (v == HX_MV_DENSE) launch_matvec_dense<double>(c->g, c->Kbin_d.p, a, c->kchunk, c->stream);
-    else if (v == HX_MV_GATHER) launch_matvec_gather(c->g, c->Kbin_d.p, a, c->stream);
-    else launch_matvec_parity<double>(c->g, c->kp64, a, c->tile, c->kchunk, c->stream);
+    if (v == HX_MV_DENSE) launch_matvec_dense<double>(c->g, c->Kbin_d.p, a, c->kchunk, c->stream, nrhs);
+    else if (v == HX_MV_GATHER) launch_matvec_gather(c->g, c->Kbin_d.p, a, c->stream, nrhs);
+    else launch_matvec_parity<double>(c->g, c->kp64, a, c->tile, c->kchunk, c->stream, nrhs);
     return HX_OK;
 }
 
-int mv32(hx_ctx* c, const float* x, float* y, const float* E, double shift, int mode, int variant) {
-    MvArgs<float> a{x, y, E, (float)shift, mode, c->st, c->partials.p};
+int mv32(hx_ctx* c, const float* x, float* y, const float* E, double shift, int mode, int variant, int nrhs = 1) {
+    MvArgs<float> a{x, y, E, (float)shift, mode, c->st, c->partials.p, ndof(c)};
     TimeScope ts(c, HX_K_MATVEC32);
-    if (variant == HX_MV_DENSE) launch_matvec_dense<float>(c->g, c->Kbin32_d.p, a, c->kchunk, c->stream);
-    else launch_matvec_parity<float>(c->g, c->kp32, a, c->tile, c->kchunk, c->stream);
+    if (variant == HX_MV_DENSE) launch_matvec_dense<float>(c->g, c->Kbin32_d.p, a, c->kchunk, c->stream, nrhs);
+    else launch_matvec_parity<float>(c->g, c->kp32, a, c->tile, c->kchunk, c->stream, nrhs);
     return HX_OK;
 }
 
@@ -210,52 +266,89 @@ int ensure_grid(hx_ctx* c) {
     return HX_OK;
 }
 
+// Vectors of one solve: nrhs columns of ndof values each.
+struct SolveVecs {
+    int nrhs;
+    double *f, *x, *r, *p, *q;
+    float *x32, *r32, *p32, *q32;
+};
+
+bool all_done(const hx_ctx* c, int nrhs) {
+    for (int k = 0; k < nrhs; ++k)
+        if (!c->st_host[k].done) return false;
+    return true;
+}
+
+void fill_stats(const hx_ctx* c, int nrhs, int refinements, hx_pcg_stats* s) {
+    if (!s) return;
+    for (int k = 0; k < nrhs; ++k) {
+        const PcgState& h = c->st_host[k];
+        s[k].iterations = h.iter;
+        s[k].status = h.status == 10 ? HX_ZERO_RHS : status_to_code(h.status);
+        s[k].rel_res = h.status == 10 ? 0.0 : h.rel;
+        s[k].refinements = refinements;
+        s[k].fallbacks = 0;
+    }
+}
+
+int worst_code(const hx_ctx* c, int nrhs) {
+    int rc = HX_OK;
+    for (int k = 0; k < nrhs; ++k) {
+        const int cd = status_to_code(c->st_host[k].status);
+        if (cd != HX_OK) rc = cd;
+    }
+    return rc;
+}
+
 // ---------------------------------------------------------------- FP64 PCG
-int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
+int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& v) {
     const long n = ndof(c);
-    const int batch = o.batch > 0 ? o.batch : (n < 3000000 ? 64 : 16);
+    const int nr = v.nrhs;
+    const int batch = o.batch > 0 ? o.batch : (n * nr < 3000000 ? 64 : 16);
     launch_elastic_minv<double>(c->g, c->kd, c->E.p, c->has_fixed ? c->fixed.p : nullptr, o.shift, c->minv.p,
                                 c->stream);
-    launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream);
-    if (o.warm_start) launch_mask<double>(n, c->minv.p, c->x.p, c->stream);
-    else launch_fill(n, 0.0, c->x.p, c->stream);
-    mv64(c, c->x.p, c->q.p, c->E.p, 0.0, MV_PLAIN, o.matvec);
-    launch_pcg_init<double>(n, c->f.p, c->x.p, c->q.p, c->minv.p, c->r.p, c->p.p, o.shift, c->st, c->partials.p,
-                            c->stream);
+    launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream, nr);
+    if (o.warm_start) launch_mask<double>(n, c->minv.p, v.x, c->stream, nr);
+    else launch_fill(n * nr, 0.0, v.x, c->stream);
+    mv64(c, v.x, v.q, c->E.p, 0.0, MV_PLAIN, o.matvec, nr);
+    launch_pcg_init<double>(n, v.f, v.x, v.q, c->minv.p, v.r, v.p, o.shift, c->st, c->partials.p, c->stream, nr);
     HXK();
-    int rc = sync_state(c);
+    int rc = sync_state(c, nr);
     if (rc) return rc;
-    while (!c->st_host->done) {
-        for (int b = 0; b < batch; ++b) {
-            mv64(c, c->p.p, c->q.p, c->E.p, o.shift, MV_PCG, o.matvec);
-            {
-                TimeScope ts(c, HX_K_UPDATE);
-                launch_pcg_update<double>(n, c->x.p, c->p.p, c->r.p, c->q.p, c->minv.p, c->st, c->partials.p,
-                                          c->stream);
+    const std::string key = batch_key("f64", nr, batch, o.matvec == HX_MV_AUTO ? c->mv_variant : o.matvec, o.shift,
+                                      v.x, v.p, v.q, v.r, c->E.p);
+    while (!all_done(c, nr)) {
+        rc = run_batch(c, key, [&] {
+            for (int b = 0; b < batch; ++b) {
+                mv64(c, v.p, v.q, c->E.p, o.shift, MV_PCG, o.matvec, nr);
+                {
+                    TimeScope ts(c, HX_K_UPDATE);
+                    launch_pcg_update<double>(n, v.x, v.p, v.r, v.q, c->minv.p, c->st, c->partials.p, c->stream,
+                                              nr);
+                }
+                {
+                    TimeScope ts(c, HX_K_PUPDATE);
+                    launch_pcg_pupdate<double>(n, v.r, c->minv.p, v.p, c->st, c->stream, nr);
+                }
             }
-            {
-                TimeScope ts(c, HX_K_PUPDATE);
-                launch_pcg_pupdate<double>(n, c->r.p, c->minv.p, c->p.p, c->st, c->stream);
-            }
-        }
-        HXK();
-        if ((rc = sync_state(c))) return rc;
+        });
+        if (rc) return rc;
+        if ((rc = sync_state(c, nr))) return rc;
     }
-    const int status = c->st_host->status, iters = c->st_host->iter;
+    std::vector<PcgState> fin(c->st_host, c->st_host + nr);
     // true residual of the returned iterate (FP64)
-    mv64(c, c->x.p, c->q.p, c->E.p, 0.0, MV_PLAIN, o.matvec);
-    launch_true_residual(n, c->f.p, c->q.p, c->x.p, c->minv.p, o.shift, nullptr, nullptr, c->st, c->partials.p, 1,
-                         c->stream);
+    mv64(c, v.x, v.q, c->E.p, 0.0, MV_PLAIN, o.matvec, nr);
+    launch_true_residual(n, v.f, v.q, v.x, c->minv.p, o.shift, nullptr, nullptr, c->st, c->partials.p, 1, c->stream,
+                         nr);
     HXK();
-    if ((rc = sync_state(c))) return rc;
-    if (s) {
-        s->iterations = iters;
-        s->status = status == 10 ? HX_ZERO_RHS : status_to_code(status);
-        s->rel_res = c->st_host->rel;
-        s->refinements = 0;
-        s->fallbacks = 0;
+    if ((rc = sync_state(c, nr))) return rc;
+    for (int k = 0; k < nr; ++k) {  // keep the solve's status/iterations, report the true residual
+        const double rel = c->st_host[k].rel;
+        c->st_host[k] = fin[k];
+        c->st_host[k].rel = rel;
     }
-    return status_to_code(status);
+    fill_stats(c, nr, 0, s);
+    return worst_code(c, nr);
 }
 
 // ------------------------------------------- FP32 PCG + FP64 reliable updates
@@ -270,84 +363,72 @@ int ensure_fp32(hx_ctx* c) {
     return HX_OK;
 }
 
-int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
+// FP32 recurrences (element contraction, vectors) with FP64 dot products and
+// alpha/beta; every time a column's recursive residual falls by kIrDelta, its
+// FP32 correction is folded into the FP64 solution and the residual is
+// replaced by the true FP64 residual b - K x (reliable update).  Converged
+// when the true residual is below tol.
+int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& v) {
     const long n = ndof(c);
+    const int nr = v.nrhs;
     int rc = ensure_fp32(c);
     if (rc) return rc;
-    const int batch = o.batch > 0 ? o.batch : (n < 3000000 ? 64 : 16);
+    const int batch = o.batch > 0 ? o.batch : (n * nr < 3000000 ? 64 : 16);
     const int v32 = (o.matvec == HX_MV_DENSE) ? HX_MV_DENSE : HX_MV_PARITY;
     launch_convert<double, float>(c->g.ne, c->E.p, c->E32.p, c->stream);
     launch_elastic_minv<double>(c->g, c->kd, c->E.p, c->has_fixed ? c->fixed.p : nullptr, o.shift, c->minv.p,
                                 c->stream);
     launch_convert<double, float>(n, c->minv.p, c->minv32.p, c->stream);
-    launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream);
-    if (o.warm_start) launch_mask<double>(n, c->minv.p, c->x.p, c->stream);
-    else launch_fill(n, 0.0, c->x.p, c->stream);
-    launch_fill32(n, 0.0f, c->x32.p, c->stream);
-    launch_fill32(n, 0.0f, c->p32.p, c->stream);
-    mv64(c, c->x.p, c->q.p, c->E.p, 0.0, MV_PLAIN, HX_MV_PARITY);
-    launch_true_residual(n, c->f.p, c->q.p, c->x.p, c->minv.p, o.shift, nullptr, c->r32.p, c->st, c->partials.p, 3,
-                         c->stream);
+    launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream, nr);
+    if (o.warm_start) launch_mask<double>(n, c->minv.p, v.x, c->stream, nr);
+    else launch_fill(n * nr, 0.0, v.x, c->stream);
+    launch_fill32(n * nr, 0.0f, v.x32, c->stream);
+    launch_fill32(n * nr, 0.0f, v.p32, c->stream);
+    mv64(c, v.x, v.q, c->E.p, 0.0, MV_PLAIN, HX_MV_PARITY, nr);
+    launch_true_residual(n, v.f, v.q, v.x, c->minv.p, o.shift, nullptr, v.r32, c->st, c->partials.p, 3, c->stream,
+                         nr);
+    launch_ir_resume(c->st, o.tol, kIrDelta, 1, c->stream, nr);
+    launch_pcg_pupdate<float>(n, v.r32, c->minv32.p, v.p32, c->st, c->stream, nr);
     HXK();
-    if ((rc = sync_state(c))) return rc;
-    double rel_true = c->st_host->rel;
-    int refinements = 0, status = 0;
-    const bool zero_rhs = c->st_host->normb == 0.0;
-    if (!zero_rhs && !(rel_true < o.tol)) {
-        launch_ir_resume(c->st, std::max(o.tol, kIrDelta * rel_true), 1, c->stream);
-        launch_pcg_pupdate<float>(n, c->r32.p, c->minv32.p, c->p32.p, c->st, c->stream);
-        HXK();
-        if ((rc = sync_state(c))) return rc;
-        for (;;) {
-            while (!c->st_host->done) {
+    if ((rc = sync_state(c, nr))) return rc;
+    int refinements = 0;
+    const std::string key = batch_key("f32", nr, batch, v32, o.shift, v.x32, v.p32, v.q32, v.r32, c->E32.p);
+    while (!all_done(c, nr)) {
+        while (!all_done(c, nr)) {
+            rc = run_batch(c, key, [&] {
                 for (int b = 0; b < batch; ++b) {
-                    mv32(c, c->p32.p, c->q32.p, c->E32.p, o.shift, MV_PCG, v32);
+                    mv32(c, v.p32, v.q32, c->E32.p, o.shift, MV_PCG, v32, nr);
                     {
                         TimeScope ts(c, HX_K_UPDATE);
-                        launch_pcg_update<float>(n, c->x32.p, c->p32.p, c->r32.p, c->q32.p, c->minv32.p, c->st,
-                                                 c->partials.p, c->stream);
+                        launch_pcg_update<float>(n, v.x32, v.p32, v.r32, v.q32, c->minv32.p, c->st, c->partials.p,
+                                                 c->stream, nr);
                     }
                     {
                         TimeScope ts(c, HX_K_PUPDATE);
-                        launch_pcg_pupdate<float>(n, c->r32.p, c->minv32.p, c->p32.p, c->st, c->stream);
+                        launch_pcg_pupdate<float>(n, v.r32, c->minv32.p, v.p32, c->st, c->stream, nr);
                     }
                 }
-                HXK();
-                if ((rc = sync_state(c))) return rc;
-            }
-            status = c->st_host->status;
-            // reliable update: fold the FP32 correction into x and replace r
-            launch_accumulate(n, c->x.p, c->x32.p, c->stream);
-            mv64(c, c->x.p, c->q.p, c->E.p, 0.0, MV_PLAIN, HX_MV_PARITY);
-            launch_true_residual(n, c->f.p, c->q.p, c->x.p, c->minv.p, o.shift, nullptr, c->r32.p, c->st,
-                                 c->partials.p, 2, c->stream);
-            HXK();
-            if ((rc = sync_state(c))) return rc;
-            ++refinements;
-            rel_true = c->st_host->rel;
-            if (rel_true < o.tol) {
-                status = 0;
-                break;
-            }
-            if (status == 4 || status == 5) break;
-            if (c->st_host->iter >= o.max_iter) {
-                status = 3;
-                break;
-            }
-            launch_ir_resume(c->st, std::max(o.tol, kIrDelta * rel_true), 0, c->stream);
-            launch_pcg_pupdate<float>(n, c->r32.p, c->minv32.p, c->p32.p, c->st, c->stream);
-            HXK();
-            if ((rc = sync_state(c))) return rc;
+            });
+            if (rc) return rc;
+            if ((rc = sync_state(c, nr))) return rc;
         }
+        // reliable update of every column: fold the FP32 correction, replace r
+        launch_accumulate(n, v.x, v.x32, c->stream, nr);
+        mv64(c, v.x, v.q, c->E.p, 0.0, MV_PLAIN, HX_MV_PARITY, nr);
+        launch_true_residual(n, v.f, v.q, v.x, c->minv.p, o.shift, nullptr, v.r32, c->st, c->partials.p, 2,
+                             c->stream, nr);
+        launch_ir_resume(c->st, o.tol, kIrDelta, 0, c->stream, nr);
+        launch_pcg_pupdate<float>(n, v.r32, c->minv32.p, v.p32, c->st, c->stream, nr);
+        HXK();
+        ++refinements;
+        if ((rc = sync_state(c, nr))) return rc;
     }
-    if (s) {
-        s->iterations = c->st_host->iter;
-        s->status = zero_rhs ? HX_ZERO_RHS : status_to_code(status);
-        s->rel_res = rel_true;
-        s->refinements = refinements;
-        s->fallbacks = 0;
-    }
-    return zero_rhs ? HX_OK : status_to_code(status);
+    fill_stats(c, nr, refinements, s);
+    return worst_code(c, nr);
+}
+
+SolveVecs single_vecs(hx_ctx* c) {
+    return SolveVecs{1, c->f.p, c->x.p, c->r.p, c->p.p, c->q.p, c->x32.p, c->r32.p, c->p32.p, c->q32.p};
 }
 
 // ------------------------------------------------------------- Helmholtz
@@ -474,13 +555,14 @@ int hx_ctx_create(int device, hx_ctx** out) {
     if (hipSetDevice(device) != hipSuccess) return HX_ERR_HIP;
     hx_ctx* c = new hx_ctx();
     c->device = device;
+    if (const char* ev = std::getenv("HEXAT_NO_GRAPH")) c->graphs_on = ev[0] == '0';
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc((void**)&c->st, sizeof(PcgState)) != hipSuccess ||
-        hipHostMalloc((void**)&c->st_host, sizeof(PcgState), hipHostMallocDefault) != hipSuccess) {
+        hipMalloc((void**)&c->st, kMaxRhs * sizeof(PcgState)) != hipSuccess ||
+        hipHostMalloc((void**)&c->st_host, kMaxRhs * sizeof(PcgState), hipHostMallocDefault) != hipSuccess) {
         hx_ctx_destroy(c);
         return HX_ERR_HIP;
     }
-    std::memset(c->st_host, 0, sizeof(PcgState));
+    std::memset(c->st_host, 0, kMaxRhs * sizeof(PcgState));
     *out = c;
     return HX_OK;
 }
@@ -490,14 +572,16 @@ void hx_ctx_destroy(hx_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     flush_timing(c);
+    clear_graphs(c);
     for (auto e : c->free_events) (void)hipEventDestroy(e);
     DevBuf<double>* d[] = {&c->E, &c->f, &c->x, &c->r, &c->p, &c->q, &c->minv, &c->Kbin_d, &c->partials,
                            &c->h_b, &c->h_x, &c->h_r, &c->h_p, &c->h_q, &c->h_minv, &c->h_in, &c->h_out,
                            &c->e_pr, &c->e_vm, &c->e_sv, &c->e_l1, &c->e_dir, &c->e_ce, &c->e_dc, &c->e_aux,
                            &c->small, &c->t_x, &c->t_xphys, &c->t_xnew, &c->t_dc, &c->t_dcf, &c->t_dv, &c->t_vm,
-                           &c->t_ones};
+                           &c->t_ones, &c->b_f, &c->b_x, &c->b_r, &c->b_p, &c->b_q};
     for (auto b : d) b->release();
-    DevBuf<float>* fl[] = {&c->Kbin32_d, &c->E32, &c->x32, &c->r32, &c->p32, &c->q32, &c->minv32};
+    DevBuf<float>* fl[] = {&c->Kbin32_d, &c->E32, &c->x32, &c->r32, &c->p32, &c->q32, &c->minv32,
+                           &c->b_x32, &c->b_r32, &c->b_p32, &c->b_q32};
     for (auto b : fl) b->release();
     c->fixed.release();
     if (c->st) (void)hipFree(c->st);
@@ -527,6 +611,7 @@ int hx_grid_set(hx_ctx* c, int nx, int ny, int nz, double dx, double dy, double 
     const long NX = nx + 1, NY = ny + 1, NZ = nz + 1;
     if (3 * NX * NY * NZ > (long)INT32_MAX) return fail(c, HX_ERR_BAD_ARG, "grid too large");
     (void)hipSetDevice(c->device);
+    clear_graphs(c);
     GridArgs g{nx, ny, nz, (int)NX, (int)NY, (int)NZ, NX * NY * NZ, (long)nx * ny * nz, dx, dy, dz};
     c->g = g;
     c->nu = nu;
@@ -584,7 +669,7 @@ int hx_grid_set(hx_ctx* c, int nx, int ny, int nz, double dx, double dy, double 
     const long np = std::max<long>({3L * kVecBlocks + 8, (long)matvec_parity_blocks(g, 0, 4) + 8,
                                     (long)matvec_parity_blocks(g, 1, 4) + 8, (long)matvec_gather_blocks(g) + 8,
                                     (long)helm_matvec_blocks(g) + 8, (long)element_post_blocks(g) + 8});
-    HXA(c->partials, np);
+    HXA(c->partials, np * kMaxRhs);
     HXA(c->small, 64);
     float kb32[576];
     for (int t = 0; t < 576; ++t) kb32[t] = (float)c->Kbin[t];
@@ -640,10 +725,16 @@ int hx_diagonal(hx_ctx* c, const double* Ee, double* diag) {
     return download(c, diag, c->q.p, ndof(c) * sizeof(double));
 }
 
+static int run_pcg(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& v) {
+    return (o.precision == HX_FP32_IR) ? pcg_fp32ir(c, o, s, v) : pcg_fp64(c, o, s, v);
+}
+
 static int pcg_dispatch(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
+    if (o.precision == HX_FP32_IR)
+        if (int rc = ensure_fp32(c)) return rc;
     hipEvent_t a = get_event(c), b = get_event(c);
     (void)hipEventRecord(a, c->stream);
-    const int rc = (o.precision == HX_FP32_IR) ? pcg_fp32ir(c, o, s) : pcg_fp64(c, o, s);
+    const int rc = run_pcg(c, o, s, single_vecs(c));
     (void)hipEventRecord(b, c->stream);
     (void)hipEventSynchronize(b);
     float ms = 0.f;
@@ -679,8 +770,42 @@ int hx_pcg_solve(hx_ctx* c, const double* Ee, const double* f, double* u, const 
 
 int hx_block_pcg_solve(hx_ctx* c, int nrhs, const double* Ee, const double* F, double* U, const hx_pcg_opts* o,
                        hx_pcg_stats* s) {
-    (void)nrhs; (void)Ee; (void)F; (void)U; (void)o; (void)s;
-    return fail(c, HX_ERR_STATE, "block solve not available in this build");
+    if (!c || !Ee || !F || !U) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    if (nrhs < 1 || nrhs > kMaxRhs) return fail(c, HX_ERR_BAD_ARG, "nrhs must be 1..8");
+    if (int rc = check_opts(c, o)) return rc;
+    const long n = ndof(c), nt = n * nrhs;
+    HXA(c->b_f, nt);
+    HXA(c->b_x, nt);
+    HXA(c->b_r, nt);
+    HXA(c->b_p, nt);
+    HXA(c->b_q, nt);
+    if (o->precision == HX_FP32_IR) {
+        if (int rc = ensure_fp32(c)) return rc;
+        HXA(c->b_x32, nt);
+        HXA(c->b_r32, nt);
+        HXA(c->b_p32, nt);
+        HXA(c->b_q32, nt);
+    }
+    int rc;
+    if ((rc = upload(c, c->E.p, Ee, c->g.ne * sizeof(double))) || (rc = upload(c, c->b_f.p, F, nt * sizeof(double))))
+        return rc;
+    if (o->warm_start && (rc = upload(c, c->b_x.p, U, nt * sizeof(double)))) return rc;
+    SolveVecs v{nrhs, c->b_f.p, c->b_x.p, c->b_r.p, c->b_p.p, c->b_q.p, c->b_x32.p, c->b_r32.p, c->b_p32.p,
+                c->b_q32.p};
+    hipEvent_t a = get_event(c), b = get_event(c);
+    (void)hipEventRecord(a, c->stream);
+    const int prc = run_pcg(c, *o, s, v);
+    (void)hipEventRecord(b, c->stream);
+    (void)hipEventSynchronize(b);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    c->free_events.push_back(a);
+    c->free_events.push_back(b);
+    if (s)
+        for (int k = 0; k < nrhs; ++k) s[k].seconds = ms * 1e-3;
+    if ((rc = download(c, U, c->b_x.p, nt * sizeof(double)))) return rc;
+    return prc;
 }
 
 int hx_helmholtz_filter(hx_ctx* c, double R, const hx_filter_opts* o, const double* in_e, double* out_e,
@@ -832,7 +957,9 @@ int hx_to_step(hx_ctx* c, hx_to_stats* s) {
     // 2. elasticity solve, warm-started from the previous displacement
     hx_pcg_opts po{p.pcg_tol, p.pcg_max_iter, 0.0, p.precision, p.precond, 1, HX_MV_AUTO, 0};
     hx_pcg_stats ps{};
-    const int prc = (p.precision == HX_FP32_IR) ? pcg_fp32ir(c, po, &ps) : pcg_fp64(c, po, &ps);
+    if (p.precision == HX_FP32_IR)
+        if ((rc = ensure_fp32(c))) return rc;
+    const int prc = run_pcg(c, po, &ps, single_vecs(c));
     if (prc == HX_ERR_HIP || prc == HX_ERR_OOM) return prc;
     // 3. compliance, sensitivities, von Mises
     launch_dot(n, c->f.p, c->x.p, c->small.p, &c->st->ticket[0], c->partials.p, c->stream);

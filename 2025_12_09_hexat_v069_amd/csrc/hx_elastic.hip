@@ -31,6 +31,13 @@ __global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T>
     __shared__ T ys[24][TY][TX];
     __shared__ double red[NT / 64];
 
+    // multi-RHS: blockIdx.y selects the column (own x, y, state, partials)
+    a.x += (long)blockIdx.y * a.cs;
+    a.y += (long)blockIdx.y * a.cs;
+    if (a.mode == MV_PCG) {
+        a.st += blockIdx.y;
+        a.partials += (long)blockIdx.y * gridDim.x;
+    }
     if (a.mode == MV_PCG && a.st->done) return;
 
     // XCD-aware tile order: the 8 XCDs take blocks round-robin, so give each
@@ -183,6 +190,12 @@ __global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T>
 __global__ __launch_bounds__(256) void k_matvec_gather(GridArgs g, const double* __restrict__ Kb,
                                                        MvArgs<double> a) {
     __shared__ double red[4];
+    a.x += (long)blockIdx.y * a.cs;
+    a.y += (long)blockIdx.y * a.cs;
+    if (a.mode == MV_PCG) {
+        a.st += blockIdx.y;
+        a.partials += (long)blockIdx.y * gridDim.x;
+    }
     if (a.mode == MV_PCG && a.st->done) return;
     const long n = (long)blockIdx.x * 256 + threadIdx.x;
     double pq = 0.0;
@@ -296,31 +309,33 @@ int matvec_gather_blocks(const GridArgs& g) { return (int)((g.nn + 255) / 256); 
 
 template <typename T>
 void launch_matvec_parity(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, int tile, int kchunk,
-                          hipStream_t s) {
+                          hipStream_t s, int nrhs) {
     int ntx, nty;
     DenseK<T> none{nullptr};
     if (tile == 1) {
         const int nt = zmarch_tiles<32, 8>(g, kchunk, &ntx, &nty);
-        hipLaunchKernelGGL((k_matvec_zmarch<T, 32, 8, true>), dim3(round8(nt)), dim3(256), 0, s, g, K, none, a,
+        hipLaunchKernelGGL((k_matvec_zmarch<T, 32, 8, true>), dim3(round8(nt), nrhs), dim3(256), 0, s, g, K, none, a,
                            kchunk, ntx, nty, nt);
     } else {
         const int nt = zmarch_tiles<16, 16>(g, kchunk, &ntx, &nty);
-        hipLaunchKernelGGL((k_matvec_zmarch<T, 16, 16, true>), dim3(round8(nt)), dim3(256), 0, s, g, K, none, a,
+        hipLaunchKernelGGL((k_matvec_zmarch<T, 16, 16, true>), dim3(round8(nt), nrhs), dim3(256), 0, s, g, K, none, a,
                            kchunk, ntx, nty, nt);
     }
 }
 
 template <typename T>
-void launch_matvec_dense(const GridArgs& g, const T* Kbin, const MvArgs<T>& a, int kchunk, hipStream_t s) {
+void launch_matvec_dense(const GridArgs& g, const T* Kbin, const MvArgs<T>& a, int kchunk, hipStream_t s,
+                         int nrhs) {
     int ntx, nty;
     ParityK<T> none{};
     const int nt = zmarch_tiles<16, 16>(g, kchunk, &ntx, &nty);
-    hipLaunchKernelGGL((k_matvec_zmarch<T, 16, 16, false>), dim3(round8(nt)), dim3(256), 0, s, g, none,
+    hipLaunchKernelGGL((k_matvec_zmarch<T, 16, 16, false>), dim3(round8(nt), nrhs), dim3(256), 0, s, g, none,
                        DenseK<T>{Kbin}, a, kchunk, ntx, nty, nt);
 }
 
-void launch_matvec_gather(const GridArgs& g, const double* Kbin, const MvArgs<double>& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_matvec_gather, dim3(matvec_gather_blocks(g)), dim3(256), 0, s, g, Kbin, a);
+void launch_matvec_gather(const GridArgs& g, const double* Kbin, const MvArgs<double>& a, hipStream_t s,
+                          int nrhs) {
+    hipLaunchKernelGGL(k_matvec_gather, dim3(matvec_gather_blocks(g), nrhs), dim3(256), 0, s, g, Kbin, a);
 }
 
 template <typename T>
@@ -336,11 +351,12 @@ void launch_elastic_diag(const GridArgs& g, const double kd[3], const double* E,
 }
 
 template void launch_matvec_parity<double>(const GridArgs&, const ParityK<double>&, const MvArgs<double>&, int, int,
-                                           hipStream_t);
+                                           hipStream_t, int);
 template void launch_matvec_parity<float>(const GridArgs&, const ParityK<float>&, const MvArgs<float>&, int, int,
-                                          hipStream_t);
-template void launch_matvec_dense<double>(const GridArgs&, const double*, const MvArgs<double>&, int, hipStream_t);
-template void launch_matvec_dense<float>(const GridArgs&, const float*, const MvArgs<float>&, int, hipStream_t);
+                                          hipStream_t, int);
+template void launch_matvec_dense<double>(const GridArgs&, const double*, const MvArgs<double>&, int, hipStream_t,
+                                          int);
+template void launch_matvec_dense<float>(const GridArgs&, const float*, const MvArgs<float>&, int, hipStream_t, int);
 template void launch_elastic_minv<double>(const GridArgs&, const double*, const double*, const uint8_t*, double,
                                           double*, hipStream_t);
 template void launch_elastic_minv<float>(const GridArgs&, const double*, const float*, const uint8_t*, double,

@@ -16,8 +16,9 @@ constexpr int NT = kRedThreads;
 __device__ __forceinline__ long gtid() { return (long)blockIdx.x * NT + threadIdx.x; }
 __device__ __forceinline__ long gstride() { return (long)gridDim.x * NT; }
 
-__global__ void k_pcg_reset(PcgState* st, double tol, double shift, int max_iter, int check_every) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
+__global__ void k_pcg_reset(PcgState* st0, double tol, double shift, int max_iter, int check_every, int nrhs) {
+    if (blockIdx.x == 0 && (int)threadIdx.x < nrhs) {
+        PcgState* st = st0 + threadIdx.x;
         st->rz = st->pq = st->alpha = st->beta = st->rr = st->normb = st->rel = 0.0;
         st->best_rel = INFINITY;
         st->tol = tol;
@@ -39,6 +40,12 @@ __global__ __launch_bounds__(NT) void k_pcg_init(long n, const T* __restrict__ b
                                                  T* __restrict__ r, T* __restrict__ p, double shift, PcgState* st,
                                                  double* partials) {
     __shared__ double red[NT / 64];
+    {
+        const long off = (long)blockIdx.y * n;
+        b += off; x += off; q += off; r += off; p += off;
+        st += blockIdx.y;
+        partials += (long)blockIdx.y * gridDim.x * 3;
+    }
     double rz = 0.0, bb = 0.0, rr = 0.0;
     for (long i = gtid(); i < n; i += gstride()) {
         const double m = (double)minv[i];
@@ -113,6 +120,12 @@ __global__ __launch_bounds__(NT) void k_pcg_update(long n, T* __restrict__ x, co
                                                    T* __restrict__ r, const T* __restrict__ q,
                                                    const T* __restrict__ minv, PcgState* st, double* partials) {
     __shared__ double red[NT / 64];
+    {
+        const long off = (long)blockIdx.y * n;
+        x += off; p += off; r += off; q += off;
+        st += blockIdx.y;
+        partials += (long)blockIdx.y * gridDim.x * 2;
+    }
     if (st->done) return;
     const double alpha = st->alpha;
     double rz = 0.0, rr = 0.0;
@@ -146,6 +159,11 @@ __global__ __launch_bounds__(NT) void k_pcg_update(long n, T* __restrict__ x, co
 template <typename T>
 __global__ __launch_bounds__(NT) void k_pcg_pupdate(long n, const T* __restrict__ r, const T* __restrict__ minv,
                                                     T* __restrict__ p, const PcgState* st) {
+    {
+        const long off = (long)blockIdx.y * n;
+        r += off; p += off;
+        st += blockIdx.y;
+    }
     if (st->done) return;
     const double beta = st->beta;
     for (long i = gtid(); i < n; i += gstride())
@@ -190,6 +208,14 @@ __global__ __launch_bounds__(NT) void k_true_residual(long n, const double* __re
                                                       double* __restrict__ r64, float* __restrict__ r32,
                                                       PcgState* st, double* partials, int mode) {
     __shared__ double red[NT / 64];
+    {
+        const long off = (long)blockIdx.y * n;
+        b += off; q += off; x += off;
+        if (r64) r64 += off;
+        if (r32) r32 += off;
+        st += blockIdx.y;
+        partials += (long)blockIdx.y * gridDim.x * 3;
+    }
     double rz = 0.0, rr = 0.0, bb = 0.0;
     for (long i = gtid(); i < n; i += gstride()) {
         const double m = minv[i];
@@ -226,30 +252,50 @@ __global__ __launch_bounds__(NT) void k_true_residual(long n, const double* __re
     }
 }
 
-// Resume the FP32 recurrence after a reliable update: the true residual's
-// r'z (scal[5]) replaces the recursive one and rescales beta accordingly.
-__global__ void k_ir_resume(PcgState* st, double tol_inner, int first) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        const double rzt = st->scal[5];
-        st->beta = first ? 0.0 : st->beta * (rzt / st->rz);
-        st->rz = rzt;
-        st->tol = tol_inner;
-        st->done = 0;
-        st->status = 0;
-        if (st->normb == 0.0) {
-            st->done = 1;
-            st->status = 10;
-        }
+// Resume the FP32 recurrence of each column after a reliable update: the
+// true residual's r'z (scal[5]) replaces the recursive one and rescales beta.
+// A column whose true residual is below tol_final, that failed, or that used
+// up its iterations stays finished.
+__global__ void k_ir_resume(PcgState* st0, double tol_final, double delta, int first, int nrhs) {
+    if (blockIdx.x != 0 || (int)threadIdx.x >= nrhs) return;
+    PcgState* st = st0 + threadIdx.x;
+    if (st->normb == 0.0) {
+        st->done = 1;
+        st->status = 10;
+        return;
     }
+    if (st->status == 4 || st->status == 5) {
+        st->done = 1;
+        return;
+    }
+    if (st->rel < tol_final) {
+        st->done = 1;
+        st->status = 0;
+        return;
+    }
+    if (!first && st->iter >= st->max_iter) {
+        st->done = 1;
+        st->status = 3;
+        return;
+    }
+    const double rzt = st->scal[5];
+    st->beta = first ? 0.0 : st->beta * (rzt / st->rz);
+    st->rz = rzt;
+    st->tol = fmax(tol_final, delta * st->rel);
+    st->done = 0;
+    st->status = 0;
 }
 
 template <typename T>
 __global__ __launch_bounds__(NT) void k_mask(long n, const T* __restrict__ minv, T* __restrict__ x) {
+    x += (long)blockIdx.y * n;
     for (long i = gtid(); i < n; i += gstride())
         if (minv[i] == T(0)) x[i] = T(0);
 }
 
 __global__ __launch_bounds__(NT) void k_accumulate(long n, double* __restrict__ x64, float* __restrict__ x32) {
+    x64 += (long)blockIdx.y * n;
+    x32 += (long)blockIdx.y * n;
     for (long i = gtid(); i < n; i += gstride()) {
         x64[i] += (double)x32[i];
         x32[i] = 0.0f;
@@ -572,23 +618,24 @@ inline unsigned blocks_for(long n) { return (unsigned)((n + NT - 1) / NT); }
 }  // namespace
 
 // ------------------------------------------------------------- launchers
-void launch_pcg_reset(PcgState* st, double tol, double shift, int max_iter, int check_every, hipStream_t s) {
-    hipLaunchKernelGGL(k_pcg_reset, dim3(1), dim3(64), 0, s, st, tol, shift, max_iter, check_every);
+void launch_pcg_reset(PcgState* st, double tol, double shift, int max_iter, int check_every, hipStream_t s,
+                      int nrhs) {
+    hipLaunchKernelGGL(k_pcg_reset, dim3(1), dim3(64), 0, s, st, tol, shift, max_iter, check_every, nrhs);
 }
 template <typename T>
 void launch_pcg_init(long n, const T* b, T* x, const T* q, const T* minv, T* r, T* p, double shift, PcgState* st,
-                     double* partials, hipStream_t s) {
-    hipLaunchKernelGGL((k_pcg_init<T>), dim3(kVecBlocks), dim3(NT), 0, s, n, b, x, q, minv, r, p, shift, st,
+                     double* partials, hipStream_t s, int nrhs) {
+    hipLaunchKernelGGL((k_pcg_init<T>), dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, b, x, q, minv, r, p, shift, st,
                        partials);
 }
 template <typename T>
 void launch_pcg_update(long n, T* x, const T* p, T* r, const T* q, const T* minv, PcgState* st, double* partials,
-                       hipStream_t s) {
-    hipLaunchKernelGGL((k_pcg_update<T>), dim3(kVecBlocks), dim3(NT), 0, s, n, x, p, r, q, minv, st, partials);
+                       hipStream_t s, int nrhs) {
+    hipLaunchKernelGGL((k_pcg_update<T>), dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, x, p, r, q, minv, st, partials);
 }
 template <typename T>
-void launch_pcg_pupdate(long n, const T* r, const T* minv, T* p, const PcgState* st, hipStream_t s) {
-    hipLaunchKernelGGL((k_pcg_pupdate<T>), dim3(kVecBlocks), dim3(NT), 0, s, n, r, minv, p, st);
+void launch_pcg_pupdate(long n, const T* r, const T* minv, T* p, const PcgState* st, hipStream_t s, int nrhs) {
+    hipLaunchKernelGGL((k_pcg_pupdate<T>), dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, r, minv, p, st);
 }
 void launch_dot(long n, const double* a, const double* b, double* out, unsigned* ticket, double* partials,
                 hipStream_t s) {
@@ -606,21 +653,21 @@ void launch_convert(long n, const Tin* a, Tout* b, hipStream_t s) {
 }
 void launch_true_residual(long n, const double* b, const double* q, const double* x, const double* minv64,
                           double shift, double* r64, float* r32, PcgState* st, double* partials, int mode,
-                          hipStream_t s) {
-    hipLaunchKernelGGL(k_true_residual, dim3(kVecBlocks), dim3(NT), 0, s, n, b, q, x, minv64, shift, r64, r32, st,
+                          hipStream_t s, int nrhs) {
+    hipLaunchKernelGGL(k_true_residual, dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, b, q, x, minv64, shift, r64, r32, st,
                        partials, mode);
 }
-void launch_ir_resume(PcgState* st, double tol_inner, int first, hipStream_t s) {
-    hipLaunchKernelGGL(k_ir_resume, dim3(1), dim3(64), 0, s, st, tol_inner, first);
+void launch_ir_resume(PcgState* st, double tol_final, double delta, int first, hipStream_t s, int nrhs) {
+    hipLaunchKernelGGL(k_ir_resume, dim3(1), dim3(64), 0, s, st, tol_final, delta, first, nrhs);
 }
 template <typename T>
-void launch_mask(long n, const T* minv, T* x, hipStream_t s) {
-    hipLaunchKernelGGL((k_mask<T>), dim3(kVecBlocks), dim3(NT), 0, s, n, minv, x);
+void launch_mask(long n, const T* minv, T* x, hipStream_t s, int nrhs) {
+    hipLaunchKernelGGL((k_mask<T>), dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, minv, x);
 }
-template void launch_mask<double>(long, const double*, double*, hipStream_t);
-template void launch_mask<float>(long, const float*, float*, hipStream_t);
-void launch_accumulate(long n, double* x64, float* x32, hipStream_t s) {
-    hipLaunchKernelGGL(k_accumulate, dim3(kVecBlocks), dim3(NT), 0, s, n, x64, x32);
+template void launch_mask<double>(long, const double*, double*, hipStream_t, int);
+template void launch_mask<float>(long, const float*, float*, hipStream_t, int);
+void launch_accumulate(long n, double* x64, float* x32, hipStream_t s, int nrhs) {
+    hipLaunchKernelGGL(k_accumulate, dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, x64, x32);
 }
 
 void launch_helm_rhs(const GridArgs& g, const double* in_e, double vol8, double* b, hipStream_t s) {
@@ -661,15 +708,16 @@ void launch_oc_apply(long ne, const double* x, const double* dc, const double* d
 }
 
 template void launch_pcg_init<double>(long, const double*, double*, const double*, const double*, double*, double*,
-                                      double, PcgState*, double*, hipStream_t);
+                                      double, PcgState*, double*, hipStream_t, int);
 template void launch_pcg_init<float>(long, const float*, float*, const float*, const float*, float*, float*, double,
-                                     PcgState*, double*, hipStream_t);
+                                     PcgState*, double*, hipStream_t, int);
 template void launch_pcg_update<double>(long, double*, const double*, double*, const double*, const double*,
-                                        PcgState*, double*, hipStream_t);
+                                        PcgState*, double*, hipStream_t, int);
 template void launch_pcg_update<float>(long, float*, const float*, float*, const float*, const float*, PcgState*,
-                                       double*, hipStream_t);
-template void launch_pcg_pupdate<double>(long, const double*, const double*, double*, const PcgState*, hipStream_t);
-template void launch_pcg_pupdate<float>(long, const float*, const float*, float*, const PcgState*, hipStream_t);
+                                       double*, hipStream_t, int);
+template void launch_pcg_pupdate<double>(long, const double*, const double*, double*, const PcgState*, hipStream_t,
+                                         int);
+template void launch_pcg_pupdate<float>(long, const float*, const float*, float*, const PcgState*, hipStream_t, int);
 template void launch_convert<double, float>(long, const double*, float*, hipStream_t);
 template void launch_convert<float, double>(long, const float*, double*, hipStream_t);
 

@@ -15,17 +15,20 @@ struct MvArgs {
     const T* E;        // element moduli (ne)
     T shift;           // y += shift * x (only where the node is owned)
     int mode;          // MvMode
-    PcgState* st;      // MV_PCG: pq -> alpha, breakdown test
-    double* partials;  // MV_PCG: one double per block
+    PcgState* st;      // MV_PCG: pq -> alpha, breakdown test (one state per column)
+    double* partials;  // MV_PCG: one double per block and column
+    long cs;           // column stride of x / y (multi-RHS: blockIdx.y = column)
 };
 
 // tile: 0 -> 16x16 element columns (default), 1 -> 32x8
 template <typename T>
 void launch_matvec_parity(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, int tile, int kchunk,
-                          hipStream_t s);
+                          hipStream_t s, int nrhs = 1);
 template <typename T>
-void launch_matvec_dense(const GridArgs& g, const T* Kbin, const MvArgs<T>& a, int kchunk, hipStream_t s);
-void launch_matvec_gather(const GridArgs& g, const double* Kbin, const MvArgs<double>& a, hipStream_t s);
+void launch_matvec_dense(const GridArgs& g, const T* Kbin, const MvArgs<T>& a, int kchunk, hipStream_t s,
+                         int nrhs = 1);
+void launch_matvec_gather(const GridArgs& g, const double* Kbin, const MvArgs<double>& a, hipStream_t s,
+                          int nrhs = 1);
 int matvec_parity_blocks(const GridArgs& g, int tile, int kchunk);
 int matvec_dense_blocks(const GridArgs& g, int kchunk);
 int matvec_gather_blocks(const GridArgs& g);
@@ -38,16 +41,20 @@ void launch_elastic_minv(const GridArgs& g, const double kd[3], const T* E, cons
 void launch_elastic_diag(const GridArgs& g, const double kd[3], const double* E, double* diag, hipStream_t s);
 
 // ---- PCG vector kernels ----------------------------------------------------
+// All of them take nrhs columns (blockIdx.y) of length n at stride n; the
+// Jacobi inverse diagonal is shared; each column has its own PcgState and
+// partials block.
 constexpr int kVecBlocks = 1024;  // blocks of the streaming kernels (multiple of 8)
 template <typename T>
 void launch_pcg_init(long n, const T* b, T* x, const T* q, const T* minv, T* r, T* p, double shift,
-                     PcgState* st, double* partials, hipStream_t s);
+                     PcgState* st, double* partials, hipStream_t s, int nrhs = 1);
 template <typename T>
 void launch_pcg_update(long n, T* x, const T* p, T* r, const T* q, const T* minv, PcgState* st,
-                       double* partials, hipStream_t s);
+                       double* partials, hipStream_t s, int nrhs = 1);
 template <typename T>
-void launch_pcg_pupdate(long n, const T* r, const T* minv, T* p, const PcgState* st, hipStream_t s);
-void launch_pcg_reset(PcgState* st, double tol, double shift, int max_iter, int check_every, hipStream_t s);
+void launch_pcg_pupdate(long n, const T* r, const T* minv, T* p, const PcgState* st, hipStream_t s, int nrhs = 1);
+void launch_pcg_reset(PcgState* st, double tol, double shift, int max_iter, int check_every, hipStream_t s,
+                      int nrhs = 1);
 
 // generic reductions / helpers
 void launch_dot(long n, const double* a, const double* b, double* out, unsigned* ticket, double* partials,
@@ -60,12 +67,13 @@ void launch_convert(long n, const Tin* a, Tout* b, hipStream_t s);
 // rel; mode bit0 also sets normb, bit1 parks rz in scal[5]
 void launch_true_residual(long n, const double* b, const double* q, const double* x, const double* minv64,
                           double shift, double* r64, float* r32, PcgState* st, double* partials, int mode,
-                          hipStream_t s);
-void launch_ir_resume(PcgState* st, double tol_inner, int first, hipStream_t s);
+                          hipStream_t s, int nrhs = 1);
+// per column: resume unless the column has converged (rel < tol_final) or failed
+void launch_ir_resume(PcgState* st, double tol_final, double delta, int first, hipStream_t s, int nrhs = 1);
 template <typename T>
-void launch_mask(long n, const T* minv, T* x, hipStream_t s);
+void launch_mask(long n, const T* minv, T* x, hipStream_t s, int nrhs = 1);
 // x64 += x32; x32 = 0
-void launch_accumulate(long n, double* x64, float* x32, hipStream_t s);
+void launch_accumulate(long n, double* x64, float* x32, hipStream_t s, int nrhs = 1);
 
 // ---- Helmholtz filter --------------------------------------------------------
 struct HelmArgs {

@@ -1,0 +1,145 @@
+"""YAML-driven TO loop (mirror of Main._run_safe, Main.jl:175-476, and the
+iteration log of Diagnostics.jl:11-83).
+
+    python -m <package>.driver config.yaml [--iters N]
+
+``optimization_parameters.method: stress_ratio`` (default) runs the
+reference's loop: solve -> compliance f'u -> stress field -> stress-ratio
+update.  ``simp_oc`` runs the device-resident SIMP/OC loop.  Mesh refinement,
+gravity/thermal loads, checkpoints and VTK/STL export are outside this
+build's hot path and are not performed (a config that asks for gravity or
+thermal load is rejected rather than silently ignored).
+"""
+from __future__ import annotations
+
+import argparse
+import datetime
+import time
+
+import numpy as np
+
+from . import boundary, configuration, mesh, stress, topopt
+from .solver import solve_system
+
+LOG_HEADER = ("| Iter | Mesh Size | Total El | Active El | Radius | Cutoff | Compliance | Strain Energy | "
+              "Avg L1 Stress | Vol Frac | Delta Rho | Refine? | Lin Res  | Prec   | Time (s) | Wall Time | VRAM |\n"
+              "|------|-----------|----------|-----------|--------|--------|------------|---------------|"
+              "---------------|----------|-----------|---------|----------|--------|----------|-----------|------|")
+
+
+def format_log_line(it, dims, n_total, n_active, R, thr, comp, se, avg_l1, vf, drho, status, t, res, prec):
+    """Diagnostics.jl:52-83 line format."""
+    return ("| %4d | %9s | %8d | %9d | %6.3f | %6.3f | %10.3e | %13.3e | %13.3e | %8.4f | %8.2f%% | %7s | %8.1e "
+            "| %-6s | %8.2f | %9s | %4s |" % (it, dims, n_total, n_active, R, thr, comp, se, avg_l1, vf, drho * 100,
+                                              status, res, prec, t, datetime.datetime.now().strftime("%H:%M:%S"),
+                                              "HBM"))
+
+
+def build_problem(config):
+    geo = configuration.setup_geometry(config)
+    nodes, elements, dims = mesh.generate_mesh(geo["nElem_x"], geo["nElem_y"], geo["nElem_z"], geo["dx"], geo["dy"],
+                                               geo["dz"])
+    g = config["geometry"]
+    g.update(nElem_x_computed=geo["nElem_x"], nElem_y_computed=geo["nElem_y"], nElem_z_computed=geo["nElem_z"],
+             dx_computed=float(geo["dx"]), dy_computed=float(geo["dy"]), dz_computed=float(geo["dz"]),
+             max_domain_dim=float(geo["max_domain_dim"]))
+    bc = boundary.get_bc_indicator(nodes.shape[0], nodes, list(config["boundary_conditions"]))
+    F = np.zeros(3 * nodes.shape[0], dtype=np.float64)
+    boundary.apply_external_forces(F, list(config["external_forces"]), nodes, elements)
+    mat = config["material"]
+    if float(mat.get("gravity_acceleration", 0.0)) != 0.0 and float(mat.get("material_density", 0.0)) > 1e-9:
+        raise NotImplementedError("self-weight loads are outside this build")
+    if abs(float(mat.get("delta_temperature", 0.0))) > 1e-6:
+        raise NotImplementedError("thermal loads are outside this build")
+    return geo, nodes, elements, bc, F
+
+
+def run_stress_ratio(config, n_iter=None, log_path=None, verbose=True):
+    """Main.jl:331-474 without refinement/export."""
+    geo, nodes, elements, bc, F = build_problem(config)
+    E = np.float32(config["material"]["E"])
+    nu = np.float32(config["material"]["nu"])
+    opt = config["optimization_parameters"]
+    min_density = np.float32(opt.get("min_density", 1e-3))
+    max_density = np.float32(opt.get("density_clamp_max", 1.0))
+    density, original, protected, _ = configuration.initialize_density_field(nodes, elements, geo["shapes"], config)
+    allow = np.float32(config.get("l1_stress_allowable", 1.0)) or np.float32(1.0)
+    nominal = int(config.get("number_of_iterations", 30))
+    n_iter = n_iter or nominal
+    U = np.zeros(3 * nodes.shape[0])
+    dims = f"{geo['nElem_x']}x{geo['nElem_y']}x{geo['nElem_z']}"
+    hist = []
+    if log_path:
+        with open(log_path, "w") as fh:
+            fh.write("HEXA FEM TOPOLOGY OPTIMIZATION LOG\nStart Date: %s\n%s\n%s\n" % (
+                datetime.datetime.now().isoformat(), "=" * 200, LOG_HEADER))
+    for it in range(1, n_iter + 1):
+        t0 = time.perf_counter()
+        config["current_outer_iter"] = it
+        U, res, prec = solve_system(nodes, elements, E, nu, bc, F, density=density, config=config,
+                                    min_stiffness_threshold=min_density, prune_voids=True, u_prev=U)
+        comp = float(F @ U)
+        _, _, _, l1, _ = stress.compute_stress_field(nodes, elements, U, E, nu, density)
+        l1 = l1.astype(np.float32)
+        act = density > np.float32(0.1)
+        avg_l1 = float(l1[act].mean()) if act.any() else 0.0
+        vf = float(density.astype(np.float64).sum() / density.size)
+        n_active = int((density > min_density).sum())
+        mc, R, thr, *_ = topopt.update_density(density, l1, protected, E, allow, it, nominal + 10, original,
+                                               min_density, max_density, config, elements, False)
+        dt = time.perf_counter() - t0
+        rec = dict(iter=it, compliance=comp, strain_energy=0.5 * comp, avg_l1=avg_l1, vol_frac=vf, delta_rho=mc,
+                   radius=R, cutoff=thr, lin_res=res, prec=prec, time=dt, active=n_active)
+        hist.append(rec)
+        line = format_log_line(it, dims, density.size, n_active, R, thr, comp, 0.5 * comp, avg_l1, vf, mc,
+                               "Nominal", dt, res, prec)
+        if verbose:
+            print(line, flush=True)
+        if log_path:
+            with open(log_path, "a") as fh:
+                fh.write(line + "\n")
+    return hist, density, U
+
+
+def run_simp(config, n_iter=None, verbose=True):
+    geo, nodes, elements, bc, F = build_problem(config)
+    opt = config["optimization_parameters"]
+    sp = config.get("solver_parameters", {})
+    fixed = boundary.fixed_dof_mask(bc)
+    o = topopt.SimpOptimizer(geo["nElem_x"], geo["nElem_y"], geo["nElem_z"], F, fixed, dx=float(geo["dx"]),
+                             dy=float(geo["dy"]), dz=float(geo["dz"]), nu=float(config["material"]["nu"]),
+                             E0=float(config["material"]["E"]),
+                             Emin=float(opt.get("E_min", 1e-9)) * float(config["material"]["E"]),
+                             penal=float(opt.get("penal", 3.0)), volfrac=float(opt.get("volfrac", 0.3)),
+                             rmin=float(opt.get("filter_radius", 1.5)), move=float(opt.get("move", 0.2)),
+                             pcg_tol=float(sp.get("tolerance", 1e-8)),
+                             precision=sp.get("precision", "fp64"), precond=sp.get("preconditioner", "jacobi"))
+    n_iter = n_iter or int(config.get("number_of_iterations", 20))
+    for _ in range(n_iter):
+        r = o.step()
+        if verbose:
+            print("iter %3d  c=%.6e  change=%.4f  vol=%.4f  pcg=%d (%.1e)  vm_max=%.3e  %.3fs" % (
+                r["iter"], r["compliance"], r["change"], r["volume"], r["pcg_iterations"], r["pcg_rel_res"],
+                r["max_vm"], r["device_s"]), flush=True)
+    return o.history, o
+
+
+def run_main(config_file, n_iter=None, log_path=None, verbose=True):
+    config = configuration.apply_hardware_profile(configuration.load_configuration(config_file))
+    method = str(config.get("optimization_parameters", {}).get("method", "stress_ratio")).lower()
+    if method == "simp_oc":
+        return run_simp(config, n_iter, verbose)
+    return run_stress_ratio(config, n_iter, log_path, verbose)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("config")
+    ap.add_argument("--iters", type=int, default=None)
+    ap.add_argument("--log", default=None)
+    a = ap.parse_args(argv)
+    run_main(a.config, a.iters, a.log)
+
+
+if __name__ == "__main__":
+    main()

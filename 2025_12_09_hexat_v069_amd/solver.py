@@ -1,0 +1,153 @@
+"""Elasticity solve entry points (mirror of Solver.jl:38-164,
+IterativeSolver.jl:20-43 and GPUSolver.jl:274-627) on libhexat's HIP PCG.
+
+Semantics kept from the reference:
+* active elements: rho >= min_stiffness_threshold, modulus E*rho (linear,
+  GPUSolver.jl:128-147); with ``prune_voids`` and fewer than 99 % active
+  elements (rho > 1.01*threshold), voids and the nodes they orphan leave the
+  system (Solver.jl:60-91): here their moduli are 0 and orphan DOFs are masked,
+  which is the same reduced system without re-indexing;
+* shift s = shift_factor * max(diag over free DOFs) (GPUSolver.jl:391);
+* adaptive tolerance 10^((1-t)(-5) + t log10(tol)) with the GPU floor 5e-7
+  (Solver.jl:99-120) in ``parity`` mode; ``northstar`` mode uses the
+  configured tolerance as is;
+* retries with the shift multiplied by 10 (hard failure) or 5 (stagnation),
+  acceptance of a stagnated result below 5e-2 (GPUSolver.jl:531-593); no AMG
+  (the CPU AMG branch is out of scope; "jacobi_amg" runs as Jacobi);
+* zero RHS returns zeros, 0.0, "None" (GPUSolver.jl:373).
+
+Results are float64 (the reference returns Float32).
+"""
+from __future__ import annotations
+
+import math
+import threading
+import warnings
+
+import numpy as np
+
+from . import _lib
+from . import boundary as _bnd
+from . import mesh as _mesh
+
+_ctx_lock = threading.Lock()
+_ctx = None
+
+
+def get_context(device: int = 0) -> _lib.Context:
+    """Process-wide HIP context (one per process, device 0 like Main.jl:179)."""
+    global _ctx
+    with _ctx_lock:
+        if _ctx is None:
+            _ctx = _lib.Context(device)
+        return _ctx
+
+
+def _bind_grid(ctx, nodes, elements, nu):
+    nx, ny, nz, dx, dy, dz = _mesh.grid_of(nodes, elements)
+    if ctx.grid != (nx, ny, nz, dx, dy, dz, float(nu)):
+        ctx.set_grid(nx, ny, nz, dx, dy, dz, nu)
+    return nx, ny, nz, dx, dy, dz
+
+
+def _precision_of(config):
+    sp = (config or {}).get("solver_parameters", {})
+    p = str(sp.get("precision", "fp64")).lower()
+    return _lib.HX_FP32_IR if p in ("fp32_mfma_ir", "fp32_ir", "fp32") else _lib.HX_FP64
+
+
+def solve_system_gpu(nodes, elements, E, nu, bc_indicator, f, density, *, max_iter=40000, tol=1e-6,
+                     method="native", solver="cg", use_precond=True, shift_factor=1.0e-6,
+                     min_stiffness_threshold=1.0e-3, u_guess=None, config=None, prune=None):
+    """GPUSolver.jl:611-627 -> (x_full, final_rel_res, prec_name)."""
+    ctx = get_context()
+    _bind_grid(ctx, nodes, elements, nu)
+    density = np.asarray(density, dtype=np.float64)
+    active = density >= min_stiffness_threshold
+    if prune is not None:
+        active &= prune
+    if not active.any():
+        raise RuntimeError(f"No active elements found (Threshold: {min_stiffness_threshold}).")
+    Ee = np.where(active, float(E) * density, 0.0)
+    fixed = _bnd.fixed_dof_mask(bc_indicator)
+    ctx.set_fixed(fixed)
+    f = np.asarray(f, dtype=np.float64)
+    free = (fixed == 0)
+    b = np.where(free, f, 0.0)
+    if not np.any(b):  # GPUSolver.jl:372-373
+        return np.zeros(ctx.ndof), 0.0, "None"
+    diag = ctx.diagonal(Ee)
+    live = free & (diag > 0)
+    max_diag = float(np.abs(diag[live]).max()) if live.any() else 0.0
+    prec = _precision_of(config)
+    u0 = None
+    if u_guess is not None and len(u_guess) == ctx.ndof:
+        u0 = np.asarray(u_guess, dtype=np.float64)
+    shift_f = float(shift_factor)
+    best_u, best_rel = None, math.inf
+    last_rel = 0.0
+    for attempt in range(1, 4):  # GPUSolver.jl:390-594
+        shift = shift_f * max_diag
+        u, st = ctx.pcg(Ee, f, u0=u0 if attempt == 1 else None, tol=float(tol), max_iter=int(max_iter),
+                        shift=shift, precision=prec)
+        last_rel = st.rel_res
+        if st.status == _lib.HX_OK:
+            return u, st.rel_res, "Jacobi"
+        if st.rel_res < best_rel:
+            best_u, best_rel = u, st.rel_res
+        diverged = st.status in (_lib.HX_ERR_DIVERGED, _lib.HX_ERR_BREAKDOWN)
+        failed_badly = diverged or best_rel > 0.1
+        if not failed_badly and best_rel < 5e-2:  # acceptable stagnation (:549-557)
+            return best_u, best_rel, "Jacobi"
+        shift_f *= 10.0 if failed_badly else 5.0
+    warnings.warn("All solver attempts failed. Returning best stagnated result.")
+    return (best_u if best_u is not None else np.zeros(ctx.ndof)), last_rel, "Jacobi"
+
+
+def adaptive_tolerance(config, solver_type="gpu"):
+    """Solver.jl:99-120"""
+    sp = config["solver_parameters"]
+    base = float(sp.get("tolerance", 1.0e-6))
+    it = config.get("current_outer_iter", 1)
+    n = config.get("number_of_iterations", 30)
+    t = min(max(float(it) / float(n), 0.0), 1.0)
+    tol = 10.0 ** ((1.0 - t) * -5.0 + t * math.log10(base))
+    tol = float(np.float32(tol))
+    if solver_type == "gpu" and tol < 5.0e-7:
+        tol = 5.0e-7
+    return tol
+
+
+def choose_solver(nNodes, nElem, config):
+    """Solver.jl:13-36 (only the GPU path exists in this build)."""
+    t = str(config["solver_parameters"].get("solver_type", "direct")).lower()
+    if t == "gpu":
+        if _lib.device_count() > 0:
+            return "gpu"
+        raise RuntimeError("CUDA not found.")  # GPUSolver.jl:620 -- no CPU fallback here
+    raise NotImplementedError(f"solver_type '{t}': only the GPU path is implemented (CPU solvers are out of scope)")
+
+
+def solve_system(nodes, elements, E, nu, bc_indicator, F, *, density=None, config=None,
+                 min_stiffness_threshold=1.0e-3, prune_voids=True, u_prev=None):
+    """Solver.jl:38-164 -> (U_full, res, prec)."""
+    config = config or {"solver_parameters": {"solver_type": "gpu"}}
+    ne = elements.shape[0]
+    if density is None:
+        density = np.ones(ne, dtype=np.float32)
+    density = np.asarray(density)
+    prune = None
+    if prune_voids:
+        thr = np.float32(min_stiffness_threshold) * np.float32(1.01)
+        keep = density > thr
+        if keep.sum() < ne * 0.99:  # Solver.jl:65
+            prune = keep
+    sp = config["solver_parameters"]
+    stype = choose_solver(nodes.shape[0], ne, config)
+    mode = str(sp.get("mode", "parity")).lower()
+    tol = adaptive_tolerance(config, stype) if mode == "parity" else float(sp.get("tolerance", 1e-8))
+    return solve_system_gpu(nodes, elements, np.float32(E), np.float32(nu), bc_indicator, F, density,
+                            max_iter=int(sp.get("max_iterations", 1000)), tol=tol,
+                            shift_factor=float(sp.get("diagonal_shift_factor", 1.0e-6)),
+                            min_stiffness_threshold=float(min_stiffness_threshold),
+                            u_guess=u_prev, config=config, prune=prune)

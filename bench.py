@@ -1,0 +1,206 @@
+"""Benchmark: seconds per topology-optimisation iteration at 12.6M DOFs.
+
+Workload (one GPU): cantilever on a 256x128x128 grid of unit hex elements
+(4,194,304 elements, 12,830,211 DOFs), clamped x=0 face, unit downward line
+load split over the bottom edge at x=256, SIMP p=3, E0=1, Emin=1e-9, nu=0.3,
+volfrac 0.15, PDE filter r=2.5, PCG to relative residual 1e-8, von Mises
+stress at every element centroid each iteration.  A "step" is one full TO
+iteration on the device: SIMP moduli, elasticity PCG (warm start),
+compliance, sensitivities + von Mises, sensitivity filter, OC, density filter.
+
+Two forms run from the same start: the plain FP64 Jacobi-PCG and the
+precision form (FP32 element contraction and vectors with FP64 residual
+replacement, dot products and refinement).  ``value`` is the precision form's
+s/iteration (lower is better); the plain form's number and the speed-up are
+reported beside it.  All inputs are resident in HBM during the timed region.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 256x128x128]
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "s/TO-iteration at 12.6M DOFs: plain FP64 PCG vs FP32-MFMA+learned-PC"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cantilever(nx, ny, nz):
+    """Clamp x=0 (DoFs 1-3); total load (0,-1,0) split over the nodes of the
+    edge x=nx, y=0 (location [100%, 0, ':'] in the reference grammar)."""
+    NX, NY, NZ = nx + 1, ny + 1, nz + 1
+    nn = NX * NY * NZ
+    k, j, i = np.meshgrid(np.arange(NZ), np.arange(NY), np.arange(NX), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    fixed = np.repeat((i == 0)[:, None], 3, axis=1).ravel().astype(np.uint8)
+    f = np.zeros(3 * nn)
+    edge = np.nonzero((i == nx) & (j == 0))[0]
+    f[3 * edge + 1] = -1.0 / edge.size
+    return fixed, f
+
+
+def run_form(hx, L, ctx, nx, ny, nz, f, fixed, precision, steps, warmup, args):
+    ctx.set_grid(nx, ny, nz, 1.0, 1.0, 1.0, 0.3)
+    ctx.set_fixed(fixed)
+    p = L.TOParams(E0=1.0, Emin=1e-9, nu=0.3, penal=3.0, volfrac=args.volfrac, move=0.2, xmin=0.0,
+                   rmin=args.rmin, pcg_tol=args.tol, pcg_max_iter=args.max_iter, precision=precision, precond=0,
+                   filter_tol=1e-10, filter_max_iter=2000, oc_rel_tol=1e-9, with_stress=1)
+    ctx.to_init(p, f, np.full(nx * ny * nz, args.volfrac))
+    for w in range(warmup):
+        s = ctx.to_step()
+        log(f"  warmup {w + 1}: c={s.compliance:.6e} pcg={s.pcg_iterations} t={s.seconds:.3f}s")
+    ctx.timing(True)
+    ctx.L.hx_device_count  # noqa: B018 (library loaded)
+    t0 = time.perf_counter()
+    recs = []
+    for k in range(steps):
+        s = ctx.to_step()
+        recs.append(s)
+        log(f"  step {k + 1}: c={s.compliance:.6e} change={s.change:.4f} pcg={s.pcg_iterations} "
+            f"rel={s.pcg_rel_res:.2e} filt={s.filter_iterations} vm_max={s.max_vm:.3e} t={s.seconds:.3f}s")
+    wall = time.perf_counter() - t0  # to_step synchronises the stream at its end
+    kern = {}
+    names = {L.HX_K_MATVEC: "matvec_fp64", L.HX_K_MATVEC32: "matvec_fp32", L.HX_K_UPDATE: "pcg_update",
+             L.HX_K_PUPDATE: "pcg_pupdate", L.HX_K_HELM: "helm_matvec", L.HX_K_ELEM: "element_post"}
+    for kid, nm in names.items():
+        ms, cnt = ctx.timing_read(kid)
+        kern[nm] = {"avg_ms": ms, "count": cnt, "total_ms": ms * cnt}
+    ctx.timing(False)
+    return wall / steps, recs, kern
+
+
+def cpu_baseline(nx, ny, nz, f, fixed, pcg_iters_per_step, filt_iters_per_step, args):
+    """Oracle (FP64 C/OpenMP restatement of the reference CPU path) on a
+    bounded sample of the same workload, extrapolated per TO iteration."""
+    from oracle import oracle as ox
+    g = ox.Grid(nx, ny, nz)
+    Ke = ox.ke_hex(1, 1, 1, 0.3)
+    rng = np.random.default_rng(0)
+    xphys = rng.uniform(0.05, 1.0, g.ne)
+    Ee = 1e-9 + xphys ** 3
+    n_pcg, n_filt = args.cpu_pcg_iters, args.cpu_filter_iters
+    t0 = time.perf_counter()
+    ox.pcg(g, Ke, Ee, fixed, f, tol=1e-30, max_iter=n_pcg)
+    t_pcg = (time.perf_counter() - t0) / n_pcg
+    t0 = time.perf_counter()
+    ox.helmholtz(g, args.rmin / (2 * math.sqrt(3)), 0, xphys, tol=1e-30, max_iter=n_filt, check_every=1)
+    t_filt = (time.perf_counter() - t0) / n_filt
+    u = rng.standard_normal(g.ndof)
+    t0 = time.perf_counter()
+    ox.element_energy(g, Ke, u)
+    ox.stress(g, 0.3, 1.0, xphys, u)
+    t_elem = time.perf_counter() - t0
+    per_step = t_pcg * pcg_iters_per_step + t_filt * filt_iters_per_step + t_elem
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": per_step, "unit": "s/TO-iteration", "cores": cores, "kind": "port",
+            "sample": (f"{n_pcg} FP64 Jacobi-PCG iterations ({t_pcg:.3f} s each), {n_filt} Helmholtz PCG iterations "
+                       f"({t_filt:.4f} s each) and one sensitivity+stress pass ({t_elem:.3f} s) on the "
+                       f"{nx}x{ny}x{nz} grid with the oracle, extrapolated with the GPU run's mean "
+                       f"{pcg_iters_per_step:.0f} PCG and {filt_iters_per_step:.0f} filter iterations per TO "
+                       f"iteration")}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", default="256x128x128")
+    ap.add_argument("--volfrac", type=float, default=0.15)
+    ap.add_argument("--rmin", type=float, default=2.5)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--max-iter", type=int, default=100000)
+    ap.add_argument("--forms", default="plain,precision")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-pcg-iters", type=int, default=20)
+    ap.add_argument("--cpu-filter-iters", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus N>1 is not supported (the path is designed "
+                                   "to shard by x-slabs with halo exchange, see DESIGN.md)"}))
+        sys.exit(2)
+    nx, ny, nz = (int(v) for v in args.grid.lower().split("x"))
+    hx = importlib.import_module("2025_12_09_hexat_v069_amd")
+    L = hx._lib
+    ctx = hx.Context(0)
+    info = ctx.device_info()
+    log(f"device: {info}")
+    fixed, f = cantilever(nx, ny, nz)
+    ne, ndof = nx * ny * nz, 3 * (nx + 1) * (ny + 1) * (nz + 1)
+    forms = [s.strip() for s in args.forms.split(",") if s.strip()]
+    res = {}
+    for form in forms:
+        prec = L.HX_FP64 if form == "plain" else L.HX_FP32_IR
+        log(f"form {form}:")
+        sps, recs, kern = run_form(hx, L, ctx, nx, ny, nz, f, fixed, prec, args.steps, args.warmup, args)
+        res[form] = {"s_per_iter": sps, "recs": recs, "kern": kern}
+    main_form = "precision" if "precision" in res else forms[0]
+    r = res[main_form]
+    # roofline of the dominant kernel of the main form
+    mv_bytes64 = 16.0 * ndof + 8.0 * ne  # read x, write y (8 B each per DOF), read E (8 B per element)
+    mv_bytes32 = 8.0 * ndof + 4.0 * ne
+    upd_bytes = {"plain": 56.0 * ndof, "precision": 28.0 * ndof}[main_form] if main_form in ("plain", "precision") \
+        else 56.0 * ndof
+    pup_bytes = {"plain": 32.0 * ndof, "precision": 16.0 * ndof}.get(main_form, 32.0 * ndof)
+    cand = []
+    for nm, by in (("matvec_fp64", mv_bytes64), ("matvec_fp32", mv_bytes32), ("pcg_update", upd_bytes),
+                   ("pcg_pupdate", pup_bytes)):
+        k = r["kern"][nm]
+        if k["count"]:
+            cand.append((k["total_ms"], nm, by, k["avg_ms"]))
+    cand.sort(reverse=True)
+    roof = None
+    if cand:
+        _, nm, by, avg = cand[0]
+        ach = by / (avg * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": nm, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": by,
+                "avg_launch_ms": avg}
+    pcg_mean = float(np.mean([s.pcg_iterations for s in res.get("plain", r)["recs"]]))
+    filt_mean = float(np.mean([s.filter_iterations for s in res.get("plain", r)["recs"]]))
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (oracle) ...")
+        cpu = cpu_baseline(nx, ny, nz, f, fixed, pcg_mean, filt_mean, args)
+    out = {
+        "metric": METRIC, "value": round(r["s_per_iter"], 4), "unit": "s/TO-iteration", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * r["s_per_iter"], 2),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if main_form == "precision" else "f64", "data": "synthetic",
+        "config": {"workload": f"cantilever {nx}x{ny}x{nz} ({ne} elements, {ndof} DOFs), SIMP p=3, volfrac "
+                               f"{args.volfrac}, PDE filter r={args.rmin}, PCG rel. residual {args.tol}, von Mises "
+                               f"each iteration", "form": main_form, "preconditioner": "jacobi",
+                   "iterations_timed": f"TO iterations {args.warmup + 1}..{args.warmup + args.steps} from uniform "
+                                       f"rho={args.volfrac}"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "forms": {k: {"s_per_iter": round(v["s_per_iter"], 4),
+                      "pcg_iterations": [s.pcg_iterations for s in v["recs"]],
+                      "compliance": [s.compliance for s in v["recs"]],
+                      "pcg_rel_res": [s.pcg_rel_res for s in v["recs"]],
+                      "kernels": {n: {"avg_ms": round(x["avg_ms"], 5), "count": x["count"]}
+                                  for n, x in v["kern"].items() if x["count"]}} for k, v in res.items()},
+    }
+    if "plain" in res and "precision" in res:
+        out["speedup_vs_plain_fp64"] = round(res["plain"]["s_per_iter"] / res["precision"]["s_per_iter"], 3)
+        cp, cq = res["plain"]["recs"][-1].compliance, res["precision"]["recs"][-1].compliance
+        out["compliance_rel_diff_vs_fp64"] = abs(cq - cp) / abs(cp)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

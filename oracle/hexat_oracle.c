@@ -525,3 +525,105 @@ int ox_oc(int n, const double* x, const double* dc, const double* dv,
     if (lmid_out) *lmid_out = lmid;
     return steps;
 }
+
+/* ---- FP32 mirror of the reference's GPU solver arithmetic ----------------
+ * GPUSolver.jl runs everything in Float32: Ke, factors, the EbE matvec
+ * (GPUSolver.jl:60-75, four products per statement), CUBLAS Sdot for the dot
+ * products and Float32 Jacobi.  This mirror keeps Float32 storage and
+ * arithmetic (dot products as per-thread Float32 partials), so it follows
+ * the reference's rounding behaviour rather than the exact FP64 answer. */
+static float dotf(long n, const float* a, const float* b) {
+    float s = 0.0f;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (long i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+
+void ox_matvec_f32(const ox_grid* g, const float* Ke, const float* Ee, const float* x, float* y) {
+    const long nn = (long)(g->nx + 1) * (g->ny + 1) * (g->nz + 1);
+    memset(y, 0, 3 * nn * sizeof(float));
+    FOR_COLOUR_BEGIN(g)
+    const float f = Ee[e];
+    if (f != 0.0f) {
+        long n[8];
+        float xe[24];
+        elem_nodes(g, i, j, k, n);
+        for (int a = 0; a < 8; ++a)
+            for (int d = 0; d < 3; ++d) xe[3 * a + d] = x[3 * n[a] + d];
+        for (int r = 0; r < 24; ++r) {
+            float v = 0.0f;
+            for (int c = 0; c < 24; c += 4)
+                v += Ke[24 * r + c] * xe[c] + Ke[24 * r + c + 1] * xe[c + 1] + Ke[24 * r + c + 2] * xe[c + 2] +
+                     Ke[24 * r + c + 3] * xe[c + 3];
+            y[3 * n[r / 3] + r % 3] += f * v;
+        }
+    }
+    FOR_COLOUR_END
+}
+
+int ox_pcg_f32(const ox_grid* g, const float* Ke, const float* Ee, const uint8_t* fixed, const float* b,
+               float* x, float shift, double tol, int max_iter, double* rel_out, int* iters_out) {
+    const long nn = (long)(g->nx + 1) * (g->ny + 1) * (g->nz + 1), nd = 3 * nn;
+    float* minv = (float*)malloc(nd * sizeof(float));
+    float* r = (float*)malloc(nd * sizeof(float));
+    float* p = (float*)malloc(nd * sizeof(float));
+    float* q = (float*)malloc(nd * sizeof(float));
+    float* z = (float*)malloc(nd * sizeof(float));
+    float* bf = (float*)malloc(nd * sizeof(float));
+    int status = 1, it = 0;
+    double rel = 0.0;
+    memset(minv, 0, nd * sizeof(float));
+    FOR_COLOUR_BEGIN(g)
+    if (Ee[e] != 0.0f) {
+        long n[8];
+        elem_nodes(g, i, j, k, n);
+        for (int a = 0; a < 8; ++a)
+            for (int d = 0; d < 3; ++d) minv[3 * n[a] + d] += Ke[24 * (3 * a + d) + 3 * a + d] * Ee[e];
+    }
+    FOR_COLOUR_END
+    for (long d = 0; d < nd; ++d) {
+        minv[d] = fixed[d] ? 0.0f : 1.0f / (minv[d] + shift);
+        bf[d] = fixed[d] ? 0.0f : b[d];
+        if (fixed[d]) x[d] = 0.0f;
+    }
+    const float normb = sqrtf(dotf(nd, bf, bf));
+    if (normb == 0.0f) {
+        memset(x, 0, nd * sizeof(float));
+        status = 3;
+        goto done;
+    }
+    ox_matvec_f32(g, Ke, Ee, x, q);
+    for (long d = 0; d < nd; ++d) r[d] = fixed[d] ? 0.0f : bf[d] - (q[d] + shift * x[d]);
+    for (long d = 0; d < nd; ++d) p[d] = minv[d] * r[d];
+    float rz = dotf(nd, r, p);
+    for (it = 1; it <= max_iter; ++it) {
+        ox_matvec_f32(g, Ke, Ee, p, q);
+        for (long d = 0; d < nd; ++d) q[d] = fixed[d] ? 0.0f : q[d] + shift * p[d];
+        const float pq = dotf(nd, p, q);
+        if (fabsf(pq) < 1e-20f || isnan(pq)) {
+            status = 2;
+            break;
+        }
+        const float alpha = rz / pq;
+        for (long d = 0; d < nd; ++d) {
+            x[d] += alpha * p[d];
+            r[d] -= alpha * q[d];
+        }
+        rel = sqrtf(dotf(nd, r, r)) / normb;
+        if (rel < tol) {
+            status = 0;
+            break;
+        }
+        for (long d = 0; d < nd; ++d) z[d] = r[d] * minv[d];
+        const float rz_new = dotf(nd, r, z);
+        const float beta = rz_new / rz;
+        for (long d = 0; d < nd; ++d) p[d] = z[d] + beta * p[d];
+        rz = rz_new;
+    }
+    if (it > max_iter) it = max_iter;
+done:
+    if (rel_out) *rel_out = rel;
+    if (iters_out) *iters_out = it;
+    free(minv); free(r); free(p); free(q); free(z); free(bf);
+    return status;
+}

@@ -84,6 +84,12 @@ int ox_oc(int n, const double* x, const double* dc, const double* dv,
           double volfrac, double move, double xmin, double rel_tol,
           double* xnew, double* lmid_out);
 
+/* FP32 mirror of the reference's GPU solver arithmetic (GPUSolver.jl runs in
+ * Float32): Float32 storage, EbE products and dot products. */
+void ox_matvec_f32(const ox_grid* g, const float* Ke, const float* Ee, const float* x, float* y);
+int ox_pcg_f32(const ox_grid* g, const float* Ke, const float* Ee, const uint8_t* fixed, const float* b,
+               float* x, float shift, double tol, int max_iter, double* rel_out, int* iters_out);
+
 #ifdef __cplusplus
 }
 #endif

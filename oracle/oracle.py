@@ -66,6 +66,10 @@ def lib():
         L.ox_eig3.argtypes = [pd, pd, pd]
         L.ox_oc.argtypes = [i, pd, pd, pd, d, d, d, d, pd, pd]
         L.ox_oc.restype = C.c_int
+        pf = P(C.c_float)
+        L.ox_matvec_f32.argtypes = [g, pf, pf, pf, pf]
+        L.ox_pcg_f32.argtypes = [g, pf, pf, pu8, pf, pf, C.c_float, d, i, pd, P(C.c_int)]
+        L.ox_pcg_f32.restype = C.c_int
         _lib = L
     return _lib
 
@@ -156,6 +160,20 @@ def pcg(g: Grid, Ke, Ee, fixed, b, x0=None, shift=0.0, tol=1e-8, max_iter=100000
     it = C.c_int(0)
     st = lib().ox_pcg(C.byref(g.c), _p(Ke), _p(Ee), fixed.ctypes.data_as(C.POINTER(C.c_uint8)),
                       _p(b), _p(x), shift, tol, int(max_iter), C.byref(rel), C.byref(it))
+    return x, st, rel.value, it.value
+
+
+def pcg_f32(g: Grid, Ke, Ee, fixed, b, x0=None, shift=0.0, tol=1e-6, max_iter=40000):
+    """FP32 mirror of the reference's GPU Jacobi-PCG (GPUSolver.jl, Float32)."""
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    pf = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    Ke, Ee, b = f32(Ke), f32(Ee), f32(b)
+    fixed = np.ascontiguousarray(fixed, dtype=np.uint8)
+    x = np.zeros(g.ndof, np.float32) if x0 is None else f32(x0).copy()
+    rel = C.c_double(0.0)
+    it = C.c_int(0)
+    st = lib().ox_pcg_f32(C.byref(g.c), pf(Ke), pf(Ee), fixed.ctypes.data_as(C.POINTER(C.c_uint8)), pf(b), pf(x),
+                          float(shift), float(tol), int(max_iter), C.byref(rel), C.byref(it))
     return x, st, rel.value, it.value
 
 

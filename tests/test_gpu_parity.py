@@ -217,3 +217,50 @@ def test_simp_loop_matches_oracle(ox, hx, ctx, dims, precision, n_iter):
     x = ctx.to_read(L.HX_FIELD_X)
     assert np.allclose(x, xo, rtol=1e-6, atol=1e-7)
     assert np.allclose(xp, xpo, rtol=1e-6, atol=1e-7)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_block_pcg_columns_equal_single_solves(ox, hx, ctx, precision):
+    """Eight load cases in one fused solve: every column is bit-identical to
+    the single-RHS solve (same kernels, same reduction order per column)."""
+    _say("block pcg", precision)
+    g = _grid_setup(ox, ctx, (16, 8, 8))
+    fixed, _ = _cantilever(ox, g)
+    ctx.set_fixed(fixed)
+    rng = np.random.default_rng(11)
+    X = g.node_coords()
+    face = np.nonzero(X[:, 0] == g.nx)[0]
+    F = np.zeros((8, g.ndof))
+    for k in range(8):
+        d = rng.standard_normal(3)
+        F[k, 3 * face[rng.integers(face.size)] + np.arange(3)] = d / np.linalg.norm(d)
+    Ee = 1e-9 + rng.uniform(0.05, 1.0, g.ne) ** 3
+    U, S = ctx.block_pcg(Ee, F, tol=1e-9, precision=precision)
+    Ke = ox.ke_hex(1, 1, 1, 0.3)
+    for k in range(8):
+        u, s = ctx.pcg(Ee, F[k], tol=1e-9, precision=precision)
+        assert S[k].status == 0 and S[k].iterations == s.iterations
+        assert np.array_equal(U[k], u)
+        assert _true_rel(ox, g, Ke, Ee, fixed, F[k], U[k]) < 1.05e-9
+
+
+def test_graph_replay_matches_plain_launches(ox, hx, ctx, monkeypatch):
+    """Captured iteration batches (hipGraph, replayed many times) give the
+    same iterates as plain launches."""
+    g = _grid_setup(ox, ctx, (20, 6, 6))
+    fixed, f = _cantilever(ox, g)
+    ctx.set_fixed(fixed)
+    Ee = 1e-9 + np.random.default_rng(12).uniform(0.05, 1.0, g.ne) ** 3
+    u1, s1 = ctx.pcg(Ee, f, tol=1e-10, batch=4)
+    monkeypatch.setenv("HEXAT_NO_GRAPH", "1")
+    c2 = hx.Context(0)
+    try:
+        c2.set_grid(20, 6, 6)
+        c2.set_fixed(fixed)
+        u2, s2 = c2.pcg(Ee, f, tol=1e-10, batch=4)
+        u3, s3 = c2.pcg(Ee, f, tol=1e-10, batch=4, precision=1)
+    finally:
+        c2.close()
+    u4, s4 = ctx.pcg(Ee, f, tol=1e-10, batch=4, precision=1)
+    assert s1.iterations == s2.iterations and np.array_equal(u1, u2)
+    assert s3.iterations == s4.iterations and np.array_equal(u3, u4)
