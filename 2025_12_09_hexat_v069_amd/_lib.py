@@ -29,7 +29,7 @@ HX_ZERO_RHS = 10
 
 HX_FP64, HX_FP32_IR = 0, 1
 HX_PC_JACOBI, HX_PC_LEARNED = 0, 1
-HX_MV_AUTO, HX_MV_PARITY, HX_MV_DENSE, HX_MV_GATHER = 0, 1, 2, 3
+HX_MV_AUTO, HX_MV_PARITY, HX_MV_DENSE, HX_MV_GATHER, HX_MV_MFMA = 0, 1, 2, 3, 4
 HX_K_MATVEC, HX_K_UPDATE, HX_K_PUPDATE, HX_K_HELM, HX_K_ELEM, HX_K_MATVEC32 = range(6)
 HX_FIELD_X, HX_FIELD_XPHYS, HX_FIELD_U, HX_FIELD_VM, HX_FIELD_DC = range(5)
 
@@ -45,6 +45,8 @@ EXPORTS = [
     "hx_to_init", "hx_to_step", "hx_to_read", "hx_learned_train", "hx_learned_weights", "hx_timing_enable",
     "hx_timing_read", "hx_device_info",
 ]
+#: test hooks declared in include/hexat_testing.h
+TEST_EXPORTS = ["hx_test_unet_nparam", "hx_test_unet_init", "hx_test_unet_forward", "hx_test_unet_backward"]
 
 
 class HexatError(RuntimeError):
@@ -133,7 +135,12 @@ def load():
         L.hx_timing_enable.argtypes = [vp, i]
         L.hx_timing_read.argtypes = [vp, i, pd, P(C.c_long)]
         L.hx_device_info.argtypes = [vp, P(C.c_int), P(C.c_int), pd]
-        for name in EXPORTS:
+        pf = P(C.c_float)
+        L.hx_test_unet_nparam.argtypes = [vp, P(C.c_long)]
+        L.hx_test_unet_init.argtypes = [vp, i, pf]
+        L.hx_test_unet_forward.argtypes = [vp, P(C.c_uint16), pf]
+        L.hx_test_unet_backward.argtypes = [vp, pf, pf]
+        for name in EXPORTS + TEST_EXPORTS:
             f = getattr(L, name)
             if name not in ("hx_ctx_destroy", "hx_last_error"):
                 f.restype = C.c_int
@@ -340,11 +347,48 @@ class Context:
         return out
 
     # ---- learned preconditioner -----------------------------------------------------
-    def learned_train(self, Ee, f, samples=64, epochs=4, lr=1e-3, seed=0):
-        Ee, f = f64(Ee, self.ne), f64(f, self.ndof)
+    def learned_train(self, Ee=None, f=None, samples=64, epochs=4, lr=1e-3, seed=0):
+        Ee = None if Ee is None else f64(Ee, self.ne)
+        f = None if f is None else f64(f, self.ndof)
         s = LearnStats(int(samples), int(epochs), float(lr), int(seed), 0.0, 0.0)
         self.check(self.L.hx_learned_train(self.h, dptr(Ee), dptr(f), C.byref(s)), "hx_learned_train")
         return s
+
+    def learned_weights(self):
+        n = C.c_int(0)
+        self.check(self.L.hx_learned_weights(self.h, None, C.byref(n)), "hx_learned_weights")
+        w = np.empty(n.value, dtype=np.uint16)
+        self.check(self.L.hx_learned_weights(self.h, w.ctypes.data_as(C.POINTER(C.c_uint16)), C.byref(n)),
+                   "hx_learned_weights")
+        return w
+
+    # ---- network test hooks (include/hexat_testing.h) ---------------------------
+    def unet_nparam(self):
+        n = C.c_long(0)
+        self.check(self.L.hx_test_unet_nparam(self.h, C.byref(n)), "hx_test_unet_nparam")
+        return n.value
+
+    def unet_init(self, seed=0):
+        p = np.empty(self.unet_nparam(), dtype=np.float32)
+        self.check(self.L.hx_test_unet_init(self.h, int(seed), p.ctypes.data_as(C.POINTER(C.c_float))),
+                   "hx_test_unet_init")
+        return p
+
+    def unet_forward(self, x0_bf16):
+        x0 = np.ascontiguousarray(x0_bf16, dtype=np.uint16)
+        if x0.size != self.nn * 8:
+            raise ValueError("x0 must be nNodes x 8 BF16")
+        y = np.empty((self.nn, 3), dtype=np.float32)
+        self.check(self.L.hx_test_unet_forward(self.h, x0.ctypes.data_as(C.POINTER(C.c_uint16)),
+                                               y.ctypes.data_as(C.POINTER(C.c_float))), "hx_test_unet_forward")
+        return y
+
+    def unet_backward(self, gy):
+        gy = np.ascontiguousarray(gy, dtype=np.float32)
+        g = np.empty(self.unet_nparam(), dtype=np.float32)
+        self.check(self.L.hx_test_unet_backward(self.h, gy.ctypes.data_as(C.POINTER(C.c_float)),
+                                                g.ctypes.data_as(C.POINTER(C.c_float))), "hx_test_unet_backward")
+        return g
 
     # ---- timing -----------------------------------------------------------------------
     def timing(self, on: bool):

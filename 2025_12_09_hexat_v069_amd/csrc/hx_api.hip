@@ -16,12 +16,31 @@
 #include <string>
 #include <vector>
 
+#include <random>
+
 #include "../../include/hexat.h"
+#include "../../include/hexat_testing.h"
 #include "hx_launch.h"
+#include "hx_unet.h"
 
 namespace hx {
 void element_stiffness(double dx, double dy, double dz, double nu, double E, double* Ke);
 void scalar_matrices(double dx, double dy, double dz, int quirk, double* Ks, double* Ms);
+struct UNet;
+UNet* unet_create();
+void unet_destroy(UNet* u);
+long unet_nparam(const UNet* u);
+bool unet_ready(const UNet* u);
+bool unet_bind(UNet* u, const GridArgs& g, unsigned seed, hipStream_t s);
+void unet_apply(UNet* u, const double* r, const double* E, const double* minv, double rms, double* z, hipStream_t s);
+double unet_train_step(UNet* u, const double* r, const double* e, const double* E, const double* minv, float lr,
+                       hipStream_t s);
+double unet_eval(UNet* u, const double* r, const double* e, const double* E, const double* minv, hipStream_t s);
+int unet_weights_bf16(UNet* u, uint16_t* out, hipStream_t s);
+void unet_init(UNet* u, unsigned seed, hipStream_t s);
+void unet_params_f32(UNet* u, float* out, hipStream_t s);
+void unet_forward_packed(UNet* u, const uint16_t* x0_host, float* y_host, hipStream_t s);
+void unet_backward_host(UNet* u, const float* gy_host, float* grad_host, hipStream_t s);
 }  // namespace hx
 
 using namespace hx;
@@ -75,6 +94,7 @@ struct hx_ctx {
     double kd[3] = {0, 0, 0};
     bool has_fixed = false;
     int mv_variant = HX_MV_PARITY;
+    int mv32_variant = HX_MV_PARITY;
     int tile = 0;
     int kchunk = 16;
 
@@ -94,6 +114,12 @@ struct hx_ctx {
     hx_to_params tp{};
     int to_iter = 0;
     DevBuf<double> t_x, t_xphys, t_xnew, t_dc, t_dcf, t_dv, t_vm, t_ones;
+    // learned preconditioner
+    UNet* unet = nullptr;
+    bool unet_trained = false;
+    DevBuf<double> l_z, l_zold, l_ustar, l_samples, l_rho;
+    int l_nsamples = 0;
+    long l_sample_stride = 0;
     // captured iteration batches (hipGraph), keyed by kernel arguments
     bool graphs_on = true;
     std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
@@ -257,6 +283,7 @@ int mv32(hx_ctx* c, const float* x, float* y, const float* E, double shift, int 
     MvArgs<float> a{x, y, E, (float)shift, mode, c->st, c->partials.p, ndof(c)};
     TimeScope ts(c, HX_K_MATVEC32);
     if (variant == HX_MV_DENSE) launch_matvec_dense<float>(c->g, c->Kbin32_d.p, a, c->kchunk, c->stream, nrhs);
+    else if (variant == HX_MV_MFMA) launch_matvec_mfma(c->g, c->Kbin32_d.p, a, c->kchunk, c->stream, nrhs);
     else launch_matvec_parity<float>(c->g, c->kp32, a, c->tile, c->kchunk, c->stream, nrhs);
     return HX_OK;
 }
@@ -374,7 +401,8 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
     int rc = ensure_fp32(c);
     if (rc) return rc;
     const int batch = o.batch > 0 ? o.batch : (n * nr < 3000000 ? 64 : 16);
-    const int v32 = (o.matvec == HX_MV_DENSE) ? HX_MV_DENSE : HX_MV_PARITY;
+    const int v32 = (o.matvec == HX_MV_DENSE || o.matvec == HX_MV_MFMA) ? o.matvec
+                    : (o.matvec == HX_MV_AUTO ? c->mv32_variant : HX_MV_PARITY);
     launch_convert<double, float>(c->g.ne, c->E.p, c->E32.p, c->stream);
     launch_elastic_minv<double>(c->g, c->kd, c->E.p, c->has_fixed ? c->fixed.p : nullptr, o.shift, c->minv.p,
                                 c->stream);
@@ -425,6 +453,76 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
     }
     fill_stats(c, nr, refinements, s);
     return worst_code(c, nr);
+}
+
+// ------------------------------------------ flexible PCG, learned preconditioner
+// z = D^-1 r + |r|_rms net(r, E); Polak-Ribiere beta.  If it does not reach
+// tol within max_iter (or breaks down), Jacobi-PCG takes over from its iterate.
+int pcg_learned(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
+    if (!c->unet || !c->unet_trained) return fail(c, HX_ERR_STATE, "learned preconditioner not trained");
+    if (!unet_bind(c->unet, c->g, 1234u, c->stream)) return fail(c, HX_ERR_OOM, "U-Net activations");
+    const long n = ndof(c);
+    HXA(c->l_z, n);
+    HXA(c->l_zold, n);
+    launch_elastic_minv<double>(c->g, c->kd, c->E.p, c->has_fixed ? c->fixed.p : nullptr, o.shift, c->minv.p,
+                                c->stream);
+    launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream, 1);
+    if (o.warm_start) launch_mask<double>(n, c->minv.p, c->x.p, c->stream, 1);
+    else launch_fill(n, 0.0, c->x.p, c->stream);
+    mv64(c, c->x.p, c->q.p, c->E.p, 0.0, MV_PLAIN, HX_MV_AUTO, 1);
+    launch_pcg_init<double>(n, c->f.p, c->x.p, c->q.p, c->minv.p, c->r.p, c->p.p, o.shift, c->st, c->partials.p,
+                            c->stream, 1);
+    HXK();
+    int rc = sync_state(c, 1);
+    if (rc) return rc;
+    int fallbacks = 0;
+    if (!c->st_host->done) {
+        unet_apply(c->unet, c->r.p, c->E.p, c->minv.p, std::sqrt(c->st_host->rr / (double)n), c->l_z.p, c->stream);
+        launch_fpcg_beta(n, c->r.p, c->l_z.p, c->l_zold.p, c->st, c->partials.p, 1, c->stream);
+        launch_fpcg_pupdate(n, c->l_z.p, c->p.p, c->l_zold.p, c->st, c->stream);
+        HXK();
+        for (;;) {
+            mv64(c, c->p.p, c->q.p, c->E.p, o.shift, MV_PCG, HX_MV_AUTO, 1);
+            launch_fpcg_update(n, c->x.p, c->p.p, c->r.p, c->q.p, c->minv.p, c->st, c->partials.p, c->stream);
+            HXK();
+            if ((rc = sync_state(c, 1))) return rc;
+            if (c->st_host->done) break;
+            unet_apply(c->unet, c->r.p, c->E.p, c->minv.p, std::sqrt(c->st_host->rr / (double)n), c->l_z.p,
+                       c->stream);
+            launch_fpcg_beta(n, c->r.p, c->l_z.p, c->l_zold.p, c->st, c->partials.p, 0, c->stream);
+            launch_fpcg_pupdate(n, c->l_z.p, c->p.p, c->l_zold.p, c->st, c->stream);
+        }
+    }
+    const int lit = c->st_host->iter;
+    int status = c->st_host->status;
+    if (status != 0 && status != 10) {  // Jacobi-PCG fallback from the current iterate
+        hx_pcg_opts jo = o;
+        jo.precond = HX_PC_JACOBI;
+        jo.warm_start = 1;
+        jo.max_iter = std::max(1, o.max_iter - lit);
+        rc = pcg_fp64(c, jo, s, SolveVecs{1, c->f.p, c->x.p, c->r.p, c->p.p, c->q.p, nullptr, nullptr, nullptr,
+                                          nullptr});
+        if (rc == HX_ERR_HIP || rc == HX_ERR_OOM) return rc;
+        fallbacks = 1;
+        if (s) {
+            s->iterations += lit;
+            s->fallbacks = fallbacks;
+        }
+        return rc;
+    }
+    // report the true residual
+    std::vector<PcgState> fin(c->st_host, c->st_host + 1);
+    mv64(c, c->x.p, c->q.p, c->E.p, 0.0, MV_PLAIN, HX_MV_AUTO, 1);
+    launch_true_residual(n, c->f.p, c->q.p, c->x.p, c->minv.p, o.shift, nullptr, nullptr, c->st, c->partials.p, 1,
+                         c->stream, 1);
+    HXK();
+    if ((rc = sync_state(c, 1))) return rc;
+    const double rel = c->st_host->rel;
+    c->st_host[0] = fin[0];
+    c->st_host[0].rel = rel;
+    fill_stats(c, 1, 0, s);
+    if (s) s->fallbacks = fallbacks;
+    return worst_code(c, 1);
 }
 
 SolveVecs single_vecs(hx_ctx* c) {
@@ -573,12 +671,14 @@ void hx_ctx_destroy(hx_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     flush_timing(c);
     clear_graphs(c);
+    if (c->unet) unet_destroy(c->unet);
     for (auto e : c->free_events) (void)hipEventDestroy(e);
     DevBuf<double>* d[] = {&c->E, &c->f, &c->x, &c->r, &c->p, &c->q, &c->minv, &c->Kbin_d, &c->partials,
                            &c->h_b, &c->h_x, &c->h_r, &c->h_p, &c->h_q, &c->h_minv, &c->h_in, &c->h_out,
                            &c->e_pr, &c->e_vm, &c->e_sv, &c->e_l1, &c->e_dir, &c->e_ce, &c->e_dc, &c->e_aux,
                            &c->small, &c->t_x, &c->t_xphys, &c->t_xnew, &c->t_dc, &c->t_dcf, &c->t_dv, &c->t_vm,
-                           &c->t_ones, &c->b_f, &c->b_x, &c->b_r, &c->b_p, &c->b_q};
+                           &c->t_ones, &c->b_f, &c->b_x, &c->b_r, &c->b_p, &c->b_q, &c->l_z, &c->l_zold,
+                           &c->l_ustar, &c->l_samples, &c->l_rho};
     for (auto b : d) b->release();
     DevBuf<float>* fl[] = {&c->Kbin32_d, &c->E32, &c->x32, &c->r32, &c->p32, &c->q32, &c->minv32,
                            &c->b_x32, &c->b_r32, &c->b_p32, &c->b_q32};
@@ -726,6 +826,7 @@ int hx_diagonal(hx_ctx* c, const double* Ee, double* diag) {
 }
 
 static int run_pcg(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& v) {
+    if (o.precond == HX_PC_LEARNED && v.nrhs == 1) return pcg_learned(c, o, s);
     return (o.precision == HX_FP32_IR) ? pcg_fp32ir(c, o, s, v) : pcg_fp64(c, o, s, v);
 }
 
@@ -749,8 +850,12 @@ static int check_opts(hx_ctx* c, const hx_pcg_opts* o) {
     if (!o) return fail(c, HX_ERR_BAD_ARG, "null options");
     if (!(o->tol > 0) || o->max_iter < 1 || !(o->shift >= 0)) return fail(c, HX_ERR_BAD_ARG, "invalid PCG options");
     if (o->precision != HX_FP64 && o->precision != HX_FP32_IR) return fail(c, HX_ERR_BAD_ARG, "precision");
-    if (o->precond != HX_PC_JACOBI) return fail(c, HX_ERR_BAD_ARG, "preconditioner not available in this call");
-    if (o->matvec < HX_MV_AUTO || o->matvec > HX_MV_GATHER) return fail(c, HX_ERR_BAD_ARG, "matvec variant");
+    if (o->precond != HX_PC_JACOBI && o->precond != HX_PC_LEARNED) return fail(c, HX_ERR_BAD_ARG, "preconditioner");
+    if (o->precond == HX_PC_LEARNED && !c->unet_trained)
+        return fail(c, HX_ERR_STATE, "learned preconditioner not trained (hx_learned_train)");
+    if (o->matvec < HX_MV_AUTO || o->matvec > HX_MV_MFMA) return fail(c, HX_ERR_BAD_ARG, "matvec variant");
+    if (o->matvec == HX_MV_MFMA && o->precision != HX_FP32_IR)
+        return fail(c, HX_ERR_BAD_ARG, "the MFMA contraction is FP32 (precision form only)");
     return HX_OK;
 }
 
@@ -1033,13 +1138,167 @@ int hx_to_read(hx_ctx* c, int field, double* out) {
 }
 
 int hx_learned_train(hx_ctx* c, const double* Ee, const double* f, hx_learn_stats* s) {
-    (void)Ee; (void)f; (void)s;
-    return fail(c, HX_ERR_STATE, "learned preconditioner not available in this build");
+    if (!c || !s) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    if (s->samples < 1 || s->epochs < 1 || !(s->lr > 0)) return fail(c, HX_ERR_BAD_ARG, "invalid training options");
+    hipEvent_t ev0 = get_event(c), ev1 = get_event(c);
+    (void)hipEventRecord(ev0, c->stream);
+    if (!c->unet) c->unet = unet_create();
+    if (!unet_bind(c->unet, c->g, (unsigned)s->seed, c->stream)) return fail(c, HX_ERR_OOM, "U-Net buffers");
+    const long n = ndof(c), ne = c->g.ne;
+    // residual/error pairs at these Jacobi-PCG iteration counts of each problem
+    const int ks[] = {0, 2, 8, 32, 128, 512};
+    const int nk = 6;
+    const int nprob = std::max(1, (s->samples + nk - 1) / nk);
+    const int nsamp = nprob * nk;
+    const long stride = 3 * n + ne;  // r, e, minv, E
+    HXA(c->l_samples, (size_t)nsamp * stride);
+    HXA(c->l_ustar, n);
+    HXA(c->l_rho, ne);
+    std::mt19937_64 rng((uint64_t)s->seed * 7919u + 17u);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    std::vector<double> hE(ne), hf(n), hrho(ne);
+    const int NX = c->g.NX, NY = c->g.NY, NZ = c->g.NZ;
+    std::vector<uint8_t> hfixed(n, 0);
+    if (c->has_fixed) HXC(hipMemcpy(hfixed.data(), c->fixed.p, n, hipMemcpyDeviceToHost));
+    int rc;
+    hx_pcg_opts solve{1e-10, 200000, 0.0, HX_FP64, HX_PC_JACOBI, 0, HX_MV_AUTO, 0};
+    for (int pi = 0; pi < nprob; ++pi) {
+        // problem: moduli and loads
+        if (pi == 0 && Ee) {
+            std::memcpy(hE.data(), Ee, ne * sizeof(double));
+            if ((rc = upload(c, c->E.p, hE.data(), ne * sizeof(double)))) return rc;
+        } else {
+            for (long e = 0; e < ne; ++e) hrho[e] = U(rng);
+            if ((rc = upload(c, c->l_rho.p, hrho.data(), ne * sizeof(double)))) return rc;
+            hx_filter_opts fo{0, 1e-6, 500, 1};
+            if ((rc = helm_solve(c, 1.0 + 2.0 * U(rng), fo, c->l_rho.p, c->l_rho.p, nullptr))) return rc;
+            HXC(hipMemcpy(hrho.data(), c->l_rho.p, ne * sizeof(double), hipMemcpyDeviceToHost));
+            // stretch the smoothed field to [0.01, 1], SIMP p = 3
+            double lo = 1e300, hi = -1e300;
+            for (long e = 0; e < ne; ++e) lo = std::min(lo, hrho[e]), hi = std::max(hi, hrho[e]);
+            const double thr = 0.2 + 0.5 * U(rng);
+            for (long e = 0; e < ne; ++e) {
+                double t = hi > lo ? (hrho[e] - lo) / (hi - lo) : 0.5;
+                t = t > thr ? 1.0 : std::max(0.01, t / thr);  // solid above the threshold, graded below
+                hE[e] = 1e-9 + t * t * t;
+            }
+            if ((rc = upload(c, c->E.p, hE.data(), ne * sizeof(double)))) return rc;
+        }
+        if (pi == 0 && f) {
+            std::memcpy(hf.data(), f, n * sizeof(double));
+        } else {
+            std::fill(hf.begin(), hf.end(), 0.0);
+            const int nl = 1 + (int)(U(rng) * 4);
+            for (int l = 0; l < nl; ++l) {  // unit loads at random free nodes of the x = max face
+                for (int tries = 0; tries < 100; ++tries) {
+                    const int j = (int)(U(rng) * NY), k = (int)(U(rng) * NZ);
+                    const long nd = (long)(NX - 1) + (long)NX * (j + (long)NY * k);
+                    if (hfixed[3 * nd] && hfixed[3 * nd + 1] && hfixed[3 * nd + 2]) continue;
+                    double d[3] = {U(rng) - 0.5, U(rng) - 0.5, U(rng) - 0.5};
+                    const double nrm = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) + 1e-12;
+                    for (int q = 0; q < 3; ++q) hf[3 * nd + q] += d[q] / nrm;
+                    break;
+                }
+            }
+        }
+        if ((rc = upload(c, c->f.p, hf.data(), n * sizeof(double)))) return rc;
+        hx_pcg_stats ps{};
+        rc = pcg_fp64(c, solve, &ps, single_vecs(c));
+        if (rc == HX_ERR_HIP || rc == HX_ERR_OOM) return rc;
+        HXC(hipMemcpyAsync(c->l_ustar.p, c->x.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        for (int kk = 0; kk < nk; ++kk) {
+            double* base = c->l_samples.p + (long)(pi * nk + kk) * stride;
+            double *sr = base, *se = base + n, *sm = base + 2 * n, *sE = base + 3 * n;
+            if (ks[kk] == 0) {
+                launch_fill(n, 0.0, c->x.p, c->stream);
+            } else {
+                hx_pcg_opts it{1e-30, ks[kk], 0.0, HX_FP64, HX_PC_JACOBI, 0, HX_MV_AUTO, 0};
+                rc = pcg_fp64(c, it, nullptr, single_vecs(c));
+                if (rc == HX_ERR_HIP || rc == HX_ERR_OOM) return rc;
+            }
+            mv64(c, c->x.p, c->q.p, c->E.p, 0.0, MV_PLAIN, HX_MV_AUTO, 1);
+            launch_elastic_minv<double>(c->g, c->kd, c->E.p, c->has_fixed ? c->fixed.p : nullptr, 0.0, sm,
+                                        c->stream);
+            launch_true_residual(n, c->f.p, c->q.p, c->x.p, sm, 0.0, sr, nullptr, c->st, c->partials.p, 1,
+                                 c->stream, 1);
+            launch_masked_diff(n, c->l_ustar.p, c->x.p, sm, se, c->stream);
+            HXC(hipMemcpyAsync(sE, c->E.p, ne * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            HXK();
+        }
+    }
+    HXC(hipStreamSynchronize(c->stream));
+    c->l_nsamples = nsamp;
+    c->l_sample_stride = stride;
+    // Adam over the harvested samples, batch size 1, seeded order
+    std::vector<int> order(nsamp);
+    for (int i = 0; i < nsamp; ++i) order[i] = i;
+    double last = 0.0, first_epoch = 0.0;
+    for (int ep = 0; ep < s->epochs; ++ep) {
+        std::shuffle(order.begin(), order.end(), rng);
+        double acc = 0.0;
+        for (int i : order) {
+            double* base = c->l_samples.p + (long)i * stride;
+            acc += unet_train_step(c->unet, base, base + n, base + 3 * n, base + 2 * n, (float)s->lr, c->stream);
+        }
+        last = acc / nsamp;
+        if (ep == 0) first_epoch = last;
+    }
+    HXK();
+    (void)hipEventRecord(ev1, c->stream);
+    (void)hipEventSynchronize(ev1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev0, ev1);
+    c->free_events.push_back(ev0);
+    c->free_events.push_back(ev1);
+    c->unet_trained = true;
+    s->samples = nsamp;
+    s->final_loss = last;
+    s->seconds = ms * 1e-3;
+    (void)first_epoch;
+    return HX_OK;
 }
 
 int hx_learned_weights(hx_ctx* c, uint16_t* out, int* count) {
-    (void)out; (void)count;
-    return fail(c, HX_ERR_STATE, "learned preconditioner not available in this build");
+    if (!c || !count) return HX_ERR_BAD_ARG;
+    if (!c->unet) return fail(c, HX_ERR_STATE, "no network (hx_learned_train first)");
+    *count = unet_weights_bf16(c->unet, out, c->stream);
+    return HX_OK;
+}
+
+int hx_test_unet_nparam(hx_ctx* c, long* nparam) {
+    if (!c || !nparam) return HX_ERR_BAD_ARG;
+    if (!c->unet) c->unet = unet_create();
+    *nparam = unet_nparam(c->unet);
+    return HX_OK;
+}
+
+int hx_test_unet_init(hx_ctx* c, int seed, float* params) {
+    if (!c) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    if (!c->unet) c->unet = unet_create();
+    if (!unet_bind(c->unet, c->g, (unsigned)seed, c->stream)) return fail(c, HX_ERR_OOM, "U-Net buffers");
+    unet_init(c->unet, (unsigned)seed, c->stream);
+    if (params) unet_params_f32(c->unet, params, c->stream);
+    HXK();
+    return HX_OK;
+}
+
+int hx_test_unet_forward(hx_ctx* c, const uint16_t* x0, float* y) {
+    if (!c || !x0 || !y) return HX_ERR_BAD_ARG;
+    if (!c->unet || !unet_ready(c->unet)) return fail(c, HX_ERR_STATE, "hx_test_unet_init first");
+    if (!unet_bind(c->unet, c->g, 0u, c->stream)) return fail(c, HX_ERR_OOM, "U-Net buffers");
+    unet_forward_packed(c->unet, x0, y, c->stream);
+    HXK();
+    return HX_OK;
+}
+
+int hx_test_unet_backward(hx_ctx* c, const float* gy, float* grad) {
+    if (!c || !gy || !grad) return HX_ERR_BAD_ARG;
+    if (!c->unet || !unet_ready(c->unet)) return fail(c, HX_ERR_STATE, "hx_test_unet_init first");
+    unet_backward_host(c->unet, gy, grad, c->stream);
+    HXK();
+    return HX_OK;
 }
 
 int hx_timing_enable(hx_ctx* c, int on) {

@@ -20,15 +20,22 @@ struct DenseK {
     const T* k;  // 24x24 binary-order, row-major, unit E
 };
 
-template <typename T, int TX, int TY, bool PARITY>
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// Element-operator modes of the z-marching kernel
+enum { EL_PARITY = 0, EL_DENSE = 1, EL_MFMA = 2 };
+
+template <typename T, int TX, int TY, int MODE>
 __global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T> K, DenseK<T> Kd, MvArgs<T> a,
                                                           int kchunk, int ntx, int nty, int ntiles) {
     constexpr int NT = TX * TY;
     constexpr int PX = TX + 1, PY = TY + 1;
     constexpr int PLANE = 3 * PX * PY;
     constexpr int NL = (PLANE + NT - 1) / NT;
+    static_assert(MODE != EL_MFMA || (TX == 16 && TY == 16), "MFMA layout assumes a 16x16 element tile");
     __shared__ T xs[2][3][PY][PX];
     __shared__ T ys[24][TY][TX];
+    __shared__ T es[MODE == EL_MFMA ? TY : 1][MODE == EL_MFMA ? TX : 1];
     __shared__ double red[NT / 64];
 
     // multi-RHS: blockIdx.y selects the column (own x, y, state, partials)
@@ -97,12 +104,48 @@ __global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T>
         T ee = elem_E(k0 - 1);
         T carry[3] = {T(0), T(0), T(0)};
 
+        // MFMA operand B = Ke^T (binary order), two 16-column blocks, 6 k-steps
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int li = lane & 15, lk = lane >> 4;
+        T kb0[MODE == EL_MFMA ? 6 : 1], kb1[MODE == EL_MFMA ? 6 : 1];
+        if constexpr (MODE == EL_MFMA) {
+#pragma unroll
+            for (int s = 0; s < 6; ++s) {
+                kb0[s] = Kd.k[24 * li + 4 * s + lk];
+                kb1[s] = (li < 8) ? Kd.k[24 * (16 + li) + 4 * s + lk] : T(0);
+            }
+        }
+
         for (int ke = k0 - 1; ke < k1; ++ke) {
             plane_store(ke + 1);
+            if constexpr (MODE == EL_MFMA) es[ty][tx] = ee;
             __syncthreads();
             if (ke + 2 <= k1) plane_load(ke + 2);
             const T ee_next = elem_E(ke + 1);
 
+            if constexpr (MODE == EL_MFMA) {
+                // ---- element phase on MFMA: each wave multiplies the gathered
+                // [16 elements x 24 DOFs] rows of 4 tile rows by Ke^T (24x24)
+#pragma unroll
+                for (int gq = 0; gq < 4; ++gq) {
+                    const int ety = 4 * wave + gq;
+                    floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int s = 0; s < 6; ++s) {
+                        const int k = 4 * s + lk, L = k / 3, c = k - 3 * L;
+                        const float av = xs[(ke + (L >> 2)) & 1][c][ety + ((L >> 1) & 1)][li + (L & 1)];
+                        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, kb0[s], acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, kb1[s], acc1, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int etx = 4 * lk + v;
+                        const float e = es[ety][etx];
+                        ys[li][ety][etx] = e * acc0[v];
+                        if (li < 8) ys[16 + li][ety][etx] = e * acc1[v];
+                    }
+                }
+            } else {
             // ---- element phase: one element per thread
             T y[24];
             if (ee != T(0)) {
@@ -114,7 +157,7 @@ __global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T>
 #pragma unroll
                     for (int c = 0; c < 3; ++c) u[3 * L + c] = xs[sl][c][ty + by][tx + bx];
                 }
-                if constexpr (PARITY) {
+                if constexpr (MODE == EL_PARITY) {
                     element_apply_parity(u, y, K, ee);
                 } else {
 #pragma unroll
@@ -131,6 +174,7 @@ __global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T>
             }
 #pragma unroll
             for (int r = 0; r < 24; ++r) ys[r][ty][tx] = y[r];
+            }
             __syncthreads();
 
             // ---- node phase: gather the 4 elements around each owned node
@@ -314,11 +358,11 @@ void launch_matvec_parity(const GridArgs& g, const ParityK<T>& K, const MvArgs<T
     DenseK<T> none{nullptr};
     if (tile == 1) {
         const int nt = zmarch_tiles<32, 8>(g, kchunk, &ntx, &nty);
-        hipLaunchKernelGGL((k_matvec_zmarch<T, 32, 8, true>), dim3(round8(nt), nrhs), dim3(256), 0, s, g, K, none, a,
+        hipLaunchKernelGGL((k_matvec_zmarch<T, 32, 8, EL_PARITY>), dim3(round8(nt), nrhs), dim3(256), 0, s, g, K, none, a,
                            kchunk, ntx, nty, nt);
     } else {
         const int nt = zmarch_tiles<16, 16>(g, kchunk, &ntx, &nty);
-        hipLaunchKernelGGL((k_matvec_zmarch<T, 16, 16, true>), dim3(round8(nt), nrhs), dim3(256), 0, s, g, K, none, a,
+        hipLaunchKernelGGL((k_matvec_zmarch<T, 16, 16, EL_PARITY>), dim3(round8(nt), nrhs), dim3(256), 0, s, g, K, none, a,
                            kchunk, ntx, nty, nt);
     }
 }
@@ -329,8 +373,17 @@ void launch_matvec_dense(const GridArgs& g, const T* Kbin, const MvArgs<T>& a, i
     int ntx, nty;
     ParityK<T> none{};
     const int nt = zmarch_tiles<16, 16>(g, kchunk, &ntx, &nty);
-    hipLaunchKernelGGL((k_matvec_zmarch<T, 16, 16, false>), dim3(round8(nt), nrhs), dim3(256), 0, s, g, none,
+    hipLaunchKernelGGL((k_matvec_zmarch<T, 16, 16, EL_DENSE>), dim3(round8(nt), nrhs), dim3(256), 0, s, g, none,
                        DenseK<T>{Kbin}, a, kchunk, ntx, nty, nt);
+}
+
+void launch_matvec_mfma(const GridArgs& g, const float* Kbin, const MvArgs<float>& a, int kchunk, hipStream_t s,
+                        int nrhs) {
+    int ntx, nty;
+    ParityK<float> none{};
+    const int nt = zmarch_tiles<16, 16>(g, kchunk, &ntx, &nty);
+    hipLaunchKernelGGL((k_matvec_zmarch<float, 16, 16, EL_MFMA>), dim3(round8(nt), nrhs), dim3(256), 0, s, g, none,
+                       DenseK<float>{Kbin}, a, kchunk, ntx, nty, nt);
 }
 
 void launch_matvec_gather(const GridArgs& g, const double* Kbin, const MvArgs<double>& a, hipStream_t s,

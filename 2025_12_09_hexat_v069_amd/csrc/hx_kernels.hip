@@ -613,6 +613,81 @@ __global__ __launch_bounds__(NT) void k_oc_apply(long ne, const double* __restri
     }
 }
 
+// ---- flexible PCG (variable preconditioner z = P(r)) -----------------------
+// x += alpha p ; r = free ? r - alpha q : 0 ; rr -> convergence (no beta)
+__global__ __launch_bounds__(NT) void k_fpcg_update(long n, double* __restrict__ x, const double* __restrict__ p,
+                                                    double* __restrict__ r, const double* __restrict__ q,
+                                                    const double* __restrict__ minv, PcgState* st,
+                                                    double* partials) {
+    __shared__ double red[NT / 64];
+    if (st->done) return;
+    const double alpha = st->alpha;
+    double rr = 0.0;
+    for (long i = gtid(); i < n; i += gstride()) {
+        x[i] += alpha * p[i];
+        const double ri = minv[i] != 0.0 ? r[i] - alpha * q[i] : 0.0;
+        r[i] = ri;
+        rr += ri * ri;
+    }
+    const double v = block_sum<NT>(rr, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = v;
+    if (publish_and_check_last(&st->ticket[1], gridDim.x)) {
+        const double t = reduce_partials<NT>(partials, 1, 0, gridDim.x, red);
+        if (threadIdx.x == 0) {
+            const double rz = st->rz;
+            pcg_after_update(st, rz, t);  // keeps rz, beta := 1 (recomputed below)
+            st->ticket[1] = 0;
+        }
+    }
+}
+
+// rz_new = r.z, rzo = r.z_old -> beta = (rz_new - rzo)/rz (Polak-Ribiere)
+__global__ __launch_bounds__(NT) void k_fpcg_beta(long n, const double* __restrict__ r, const double* __restrict__ z,
+                                                  const double* __restrict__ zold, PcgState* st, double* partials,
+                                                  int first) {
+    __shared__ double red[NT / 64];
+    double a = 0.0, b = 0.0;
+    for (long i = gtid(); i < n; i += gstride()) {
+        a += r[i] * z[i];
+        if (!first) b += r[i] * zold[i];
+    }
+    const double v0 = block_sum<NT>(a, red);
+    const double v1 = block_sum<NT>(b, red);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = v0;
+        partials[2 * blockIdx.x + 1] = v1;
+    }
+    if (publish_and_check_last(&st->ticket[2], gridDim.x)) {
+        const double s0 = reduce_partials<NT>(partials, 2, 0, gridDim.x, red);
+        const double s1 = reduce_partials<NT>(partials, 2, 1, gridDim.x, red);
+        if (threadIdx.x == 0) {
+            st->beta = first ? 0.0 : (s0 - s1) / st->rz;
+            st->rz = s0;
+            if (!(s0 > 0.0) && !first) {  // preconditioner lost positivity: restart with beta = 0
+                st->beta = 0.0;
+            }
+            st->ticket[2] = 0;
+        }
+    }
+}
+
+// p = z + beta p ; zold = z
+__global__ __launch_bounds__(NT) void k_fpcg_pupdate(long n, const double* __restrict__ z, double* __restrict__ p,
+                                                     double* __restrict__ zold, const PcgState* st) {
+    const double beta = st->beta;
+    for (long i = gtid(); i < n; i += gstride()) {
+        const double zi = z[i];
+        p[i] = zi + beta * p[i];
+        zold[i] = zi;
+    }
+}
+
+// e = a - b (masked by minv != 0)
+__global__ __launch_bounds__(NT) void k_masked_diff(long n, const double* __restrict__ a, const double* __restrict__ b,
+                                                    const double* __restrict__ minv, double* __restrict__ e) {
+    for (long i = gtid(); i < n; i += gstride()) e[i] = minv[i] != 0.0 ? a[i] - b[i] : 0.0;
+}
+
 inline unsigned blocks_for(long n) { return (unsigned)((n + NT - 1) / NT); }
 
 }  // namespace
@@ -695,6 +770,20 @@ void launch_max_reduce(long n, const double* partials_in, int nin, double* out, 
 void launch_simp_modulus(long ne, const double* xphys, double penal, double E0, double Emin, double* E,
                          hipStream_t s) {
     hipLaunchKernelGGL(k_simp_modulus, dim3(kVecBlocks), dim3(NT), 0, s, ne, xphys, penal, E0, Emin, E);
+}
+void launch_fpcg_update(long n, double* x, const double* p, double* r, const double* q, const double* minv,
+                        PcgState* st, double* partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_fpcg_update, dim3(kVecBlocks), dim3(NT), 0, s, n, x, p, r, q, minv, st, partials);
+}
+void launch_fpcg_beta(long n, const double* r, const double* z, const double* zold, PcgState* st, double* partials,
+                      int first, hipStream_t s) {
+    hipLaunchKernelGGL(k_fpcg_beta, dim3(kVecBlocks), dim3(NT), 0, s, n, r, z, zold, st, partials, first);
+}
+void launch_fpcg_pupdate(long n, const double* z, double* p, double* zold, const PcgState* st, hipStream_t s) {
+    hipLaunchKernelGGL(k_fpcg_pupdate, dim3(kVecBlocks), dim3(NT), 0, s, n, z, p, zold, st);
+}
+void launch_masked_diff(long n, const double* a, const double* b, const double* minv, double* e, hipStream_t s) {
+    hipLaunchKernelGGL(k_masked_diff, dim3(kVecBlocks), dim3(NT), 0, s, n, a, b, minv, e);
 }
 void launch_oc_init(PcgState* st, hipStream_t s) { hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(64), 0, s, st); }
 void launch_oc_eval(long ne, const double* x, const double* dc, const double* dv, double volfrac, double move,

@@ -29,6 +29,10 @@ void launch_matvec_dense(const GridArgs& g, const T* Kbin, const MvArgs<T>& a, i
                          int nrhs = 1);
 void launch_matvec_gather(const GridArgs& g, const double* Kbin, const MvArgs<double>& a, hipStream_t s,
                           int nrhs = 1);
+// FP32 element contraction on MFMA: dense [16 elem x 24] x Ke^T tiles with
+// v_mfma_f32_16x16x4f32, inside the same z-marching tile
+void launch_matvec_mfma(const GridArgs& g, const float* Kbin, const MvArgs<float>& a, int kchunk, hipStream_t s,
+                        int nrhs = 1);
 int matvec_parity_blocks(const GridArgs& g, int tile, int kchunk);
 int matvec_dense_blocks(const GridArgs& g, int kchunk);
 int matvec_gather_blocks(const GridArgs& g);
@@ -74,6 +78,14 @@ template <typename T>
 void launch_mask(long n, const T* minv, T* x, hipStream_t s, int nrhs = 1);
 // x64 += x32; x32 = 0
 void launch_accumulate(long n, double* x64, float* x32, hipStream_t s, int nrhs = 1);
+
+// ---- flexible PCG pieces (learned preconditioner) ---------------------------
+void launch_fpcg_update(long n, double* x, const double* p, double* r, const double* q, const double* minv,
+                        PcgState* st, double* partials, hipStream_t s);
+void launch_fpcg_beta(long n, const double* r, const double* z, const double* zold, PcgState* st, double* partials,
+                      int first, hipStream_t s);
+void launch_fpcg_pupdate(long n, const double* z, double* p, double* zold, const PcgState* st, hipStream_t s);
+void launch_masked_diff(long n, const double* a, const double* b, const double* minv, double* e, hipStream_t s);
 
 // ---- Helmholtz filter --------------------------------------------------------
 struct HelmArgs {

@@ -1,0 +1,33 @@
+/*
+ * hexat_testing.h -- test hooks of libhexat.so for the learned
+ * preconditioner's network.  They expose one forward and one backward pass
+ * of the U-Net on caller buffers so the tests can compare them with a CPU
+ * reference.  No reference counterpart; not used by the solver path.
+ */
+#ifndef HEXAT_TESTING_H
+#define HEXAT_TESTING_H
+
+#include <stdint.h>
+
+#include "hexat.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* number of network parameters (conv weights then bias, layer by layer) */
+int hx_test_unet_nparam(hx_ctx* ctx, long* nparam);
+/* (re)initialise the network weights with a seed; copy the FP32 master
+ * parameters out (nullable) */
+int hx_test_unet_init(hx_ctx* ctx, int seed, float* params_f32);
+/* forward pass on the current grid: x0 is the packed input, nNodes x 8 BF16
+ * (node-major); y is the fine output, nNodes x 3 FP32 */
+int hx_test_unet_forward(hx_ctx* ctx, const uint16_t* x0, float* y);
+/* backward pass of the last forward for dL/dy = gy (nNodes x 3); returns the
+ * parameter gradient (nparam FP32) */
+int hx_test_unet_backward(hx_ctx* ctx, const float* gy, float* grad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

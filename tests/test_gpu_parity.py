@@ -264,3 +264,22 @@ def test_graph_replay_matches_plain_launches(ox, hx, ctx, monkeypatch):
     u4, s4 = ctx.pcg(Ee, f, tol=1e-10, batch=4, precision=1)
     assert s1.iterations == s2.iterations and np.array_equal(u1, u2)
     assert s3.iterations == s4.iterations and np.array_equal(u3, u4)
+
+
+@pytest.mark.parametrize("variant", [1, 2, 4])
+def test_fp32_contraction_variants_solve_to_fp64_tolerance(ox, hx, ctx, variant):
+    """The precision form with each FP32 element contraction (parity-basis
+    VALU, dense VALU, dense MFMA) reaches the FP64 true residual 1e-8 and the
+    FP64 oracle's compliance."""
+    _say("fp32 variant", variant)
+    g = _grid_setup(ox, ctx, (24, 8, 8))
+    fixed, f = _cantilever(ox, g)
+    ctx.set_fixed(fixed)
+    Ee = 1e-9 + np.random.default_rng(13).uniform(0.05, 1.0, g.ne) ** 3
+    Ke = ox.ke_hex(1, 1, 1, 0.3)
+    u, s = ctx.pcg(Ee, f, tol=1e-8, precision=1, matvec=variant, max_iter=20000)
+    uo, *_ = ox.pcg(g, Ke, Ee, fixed, f, tol=1e-10)
+    _say("  its", s.iterations, "refinements", s.refinements, "rel", s.rel_res)
+    assert s.status == 0
+    assert _true_rel(ox, g, Ke, Ee, fixed, f, u) < 1.05e-8
+    assert abs(f @ u - f @ uo) < 1e-7 * abs(f @ uo)

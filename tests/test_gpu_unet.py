@@ -1,0 +1,178 @@
+"""The learned preconditioner's network and its use in flexible PCG.
+
+The CPU reference is the same 3-level U-Net written with torch.nn.functional
+in float64 on the CPU (with the library's BF16 roundings of weights and
+stored activations), differentiated by torch autograd.  The library runs
+BF16 MFMA convolutions with FP32 accumulation and BF16 activations, so the
+comparison is norm-wise at 3e-2.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+F = torch.nn.functional
+
+# (cin0, cin1, cout, stride, relu, up0) in parameter order
+LAYERS = [(8, 0, 16, 2, 1, 0), (16, 0, 16, 1, 1, 0), (16, 0, 32, 2, 1, 0), (32, 0, 32, 1, 1, 0),
+          (32, 0, 64, 2, 1, 0), (64, 0, 64, 1, 1, 0), (64, 32, 32, 1, 1, 1), (32, 16, 16, 1, 1, 1),
+          (16, 0, 16, 1, 0, 0)]
+
+
+def bf16(a):
+    t = torch.as_tensor(np.asarray(a, dtype=np.float32))
+    return t.to(torch.bfloat16).to(torch.float64)
+
+
+def bf16_bits(a):
+    t = torch.as_tensor(np.asarray(a, dtype=np.float32)).to(torch.bfloat16)
+    return t.view(torch.int16).numpy().view(np.uint16)
+
+
+def split(params):
+    out, o = [], 0
+    for c0, c1, co, st, relu, up in LAYERS:
+        n = co * 27 * (c0 + c1)
+        W = params[o:o + n].reshape(co, 27, c0 + c1)
+        b = params[o + n:o + n + co]
+        out.append((W, b))
+        o += n + co
+    assert o == params.size
+    return out
+
+
+def torch_w(W):
+    co, _, ci = W.shape  # tap t = (dx+1) + 3(dy+1) + 9(dz+1); torch weight [co, ci, dz, dy, dx]
+    return bf16(W).reshape(co, 3, 3, 3, ci).permute(0, 4, 1, 2, 3).contiguous()
+
+
+def up(x, shape):
+    for d in (2, 3, 4):
+        x = x.repeat_interleave(2, dim=d)
+    return x[:, :, :shape[2], :shape[3], :shape[4]]
+
+
+def pmat(nf, nc):
+    P = np.zeros((nf, nc))
+    for i in range(nf):
+        a = i >> 1
+        if i % 2 == 0 or a + 1 >= nc:
+            P[i, a] = 1.0
+        else:
+            P[i, a] = P[i, a + 1] = 0.5
+    return torch.as_tensor(P)
+
+
+def forward_ref(x0, Ws, bs):
+    def conv(l, x):
+        c0, c1, co, st, relu, _ = LAYERS[l]
+        y = F.conv3d(x, Ws[l], bs[l], stride=st, padding=1)
+        y = F.relu(y) if relu else y
+        return y
+
+    def q(x):  # BF16 storage between layers
+        return x.to(torch.bfloat16).to(torch.float64)
+
+    a1 = q(conv(0, x0))
+    s1 = q(conv(1, a1))
+    a2 = q(conv(2, s1))
+    s2 = q(conv(3, a2))
+    a3 = q(conv(4, s2))
+    b3 = q(conv(5, a3))
+    u2 = q(conv(6, torch.cat([up(b3, s2.shape), s2], 1)))
+    u1 = q(conv(7, torch.cat([up(u2, s1.shape), s1], 1)))
+    y1 = conv(8, u1)[:, :3]
+    NZ, NY, NX = x0.shape[2:]
+    Pz, Py, Px = pmat(NZ, y1.shape[2]), pmat(NY, y1.shape[3]), pmat(NX, y1.shape[4])
+    return torch.einsum("zc,yb,xa,nkcba->nkzyx", Pz, Py, Px, y1)
+
+
+@pytest.fixture(scope="module")
+def net(hx):
+    c = hx.Context(0)
+    nx, ny, nz = 11, 8, 6
+    c.set_grid(nx, ny, nz)
+    params = c.unet_init(seed=3)
+    yield c, params, (nx + 1, ny + 1, nz + 1)
+    c.close()
+
+
+def _inputs(shape, seed=0):
+    NX, NY, NZ = shape
+    rng = np.random.default_rng(seed)
+    x = np.zeros((NZ * NY * NX, 8), np.float32)
+    x[:, :3] = rng.standard_normal((x.shape[0], 3))
+    x[:, 3] = rng.uniform(0, 1, x.shape[0])
+    return x
+
+
+def test_unet_forward_matches_reference(net):
+    c, params, (NX, NY, NZ) = net
+    x = _inputs((NX, NY, NZ))
+    y = c.unet_forward(bf16_bits(x))
+    parts = split(params)
+    Ws = [torch_w(W) for W, _ in parts]
+    bs = [torch.as_tensor(b.astype(np.float64)) for _, b in parts]
+    x0 = bf16(x).reshape(NZ, NY, NX, 8).permute(3, 0, 1, 2)[None]
+    ref = forward_ref(x0, Ws, bs)[0].permute(1, 2, 3, 0).reshape(-1, 3).numpy()
+    err = np.linalg.norm(y - ref) / np.linalg.norm(ref)
+    assert err < 3e-2, err
+
+
+def test_unet_gradients_match_autograd(net):
+    c, params, (NX, NY, NZ) = net
+    x = _inputs((NX, NY, NZ), seed=1)
+    c.unet_forward(bf16_bits(x))
+    gy = np.random.default_rng(2).standard_normal((NX * NY * NZ, 3)).astype(np.float32)
+    grad = c.unet_backward(gy)
+    parts = split(params)
+    Ws = [torch_w(W).requires_grad_() for W, _ in parts]
+    bs = [torch.as_tensor(b.astype(np.float64)).requires_grad_() for _, b in parts]
+    x0 = bf16(x).reshape(NZ, NY, NX, 8).permute(3, 0, 1, 2)[None]
+    y = forward_ref(x0, Ws, bs)
+    g = torch.as_tensor(gy.astype(np.float64)).reshape(NZ, NY, NX, 3).permute(3, 0, 1, 2)[None]
+    (y * g).sum().backward()
+    gparts = split(grad)
+    for l, ((gW, gb), Wt, bt) in enumerate(zip(gparts, Ws, bs)):
+        co, _, ci = gW.shape
+        refW = Wt.grad.permute(0, 2, 3, 4, 1).reshape(co, 27, ci).numpy()
+        eW = np.linalg.norm(gW - refW) / max(np.linalg.norm(refW), 1e-30)
+        eb = np.linalg.norm(gb - bt.grad.numpy()) / max(np.linalg.norm(bt.grad.numpy()), 1e-30)
+        assert eW < 3e-2 and eb < 3e-2, (l, eW, eb)
+
+
+def _problem(ox, g, seed):
+    X = g.node_coords()
+    fixed = np.repeat((X[:, 0] == 0.0)[:, None], 3, axis=1).ravel().astype(np.uint8)
+    f = ox.external_forces(X, [{"location": ["100%", 0, ":"], "F": [0, -1, 0]}])
+    rho = np.random.default_rng(seed).uniform(0, 1, g.ne)
+    rho, *_ = ox.helmholtz(g, 1.5, 0, rho, tol=1e-8, max_iter=500, check_every=1)
+    rho = np.clip((rho - rho.min()) / (rho.max() - rho.min() + 1e-12), 0.01, 1.0)
+    return fixed, f, 1e-9 + rho ** 3
+
+
+def test_learned_preconditioner_ablation(ox, hx):
+    """Train on solver-generated data, then solve an unseen problem with
+    flexible PCG + the network and with plain Jacobi-PCG."""
+    g = ox.Grid(32, 16, 16)
+    with hx.Context(0) as c:
+        c.set_grid(32, 16, 16)
+        fixed, f, Ee = _problem(ox, g, seed=5)
+        c.set_fixed(fixed)
+        s1 = c.learned_train(samples=36, epochs=2, lr=2e-3, seed=0)
+        s2 = c.learned_train(samples=36, epochs=10, lr=1e-3, seed=1)
+        print("train loss", s1.final_loss, "->", s2.final_loss, "time", s1.seconds + s2.seconds, flush=True)
+        assert s2.final_loss < s1.final_loss
+        uj, sj = c.pcg(Ee, f, tol=1e-8)
+        ul, sl = c.pcg(Ee, f, tol=1e-8, precond=hx._lib.HX_PC_LEARNED)
+        print("jacobi its", sj.iterations, sj.seconds, "learned its", sl.iterations, sl.seconds, "fallbacks",
+              sl.fallbacks, flush=True)
+        Ke = ox.ke_hex(1, 1, 1, 0.3)
+        r = f - ox.matvec(g, Ke, Ee, ul)
+        r[fixed.astype(bool)] = 0
+        b = np.where(fixed.astype(bool), 0.0, f)
+        assert sl.status == 0 and np.linalg.norm(r) / np.linalg.norm(b) < 1.05e-8
+        assert abs(f @ ul - f @ uj) < 1e-6 * abs(f @ uj)
+        assert sl.fallbacks == 0
+        assert sl.iterations < sj.iterations

@@ -12,9 +12,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def test_header_symbols_exported(hx):
     L = hx._lib.load()
-    hdr = open(os.path.join(ROOT, "include", "hexat.h")).read()
-    declared = set(re.findall(r"^(?:int|void|const char\*)\s+(hx_\w+)\(", hdr, flags=re.M))
-    assert declared == set(hx._lib.EXPORTS)
+    declared = set()
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            hdr = open(os.path.join(ROOT, "include", h)).read()
+            declared |= set(re.findall(r"^(?:int|void|const char\*)\s+(hx_\w+)\(", hdr, flags=re.M))
+    assert declared == set(hx._lib.EXPORTS) | set(hx._lib.TEST_EXPORTS)
     for name in declared:
         assert hasattr(L, name), name
 
