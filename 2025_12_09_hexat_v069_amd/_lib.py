@@ -43,7 +43,7 @@ EXPORTS = [
     "hx_set_fixed", "hx_element_stiffness", "hx_scalar_matrices", "hx_matvec", "hx_diagonal", "hx_pcg_solve",
     "hx_block_pcg_solve", "hx_helmholtz_filter", "hx_element_stress", "hx_compliance_sens", "hx_oc_update",
     "hx_to_init", "hx_to_step", "hx_to_read", "hx_learned_train", "hx_learned_weights", "hx_timing_enable",
-    "hx_timing_read", "hx_device_info",
+    "hx_timing_read", "hx_device_info", "hx_autotune",
 ]
 #: test hooks declared in include/hexat_testing.h
 TEST_EXPORTS = ["hx_test_unet_nparam", "hx_test_unet_init", "hx_test_unet_forward", "hx_test_unet_backward"]
@@ -85,6 +85,12 @@ class TOStats(C.Structure):
     _fields_ = [("iter", C.c_int), ("compliance", C.c_double), ("change", C.c_double), ("volume", C.c_double),
                 ("pcg_iterations", C.c_int), ("pcg_rel_res", C.c_double), ("pcg_status", C.c_int),
                 ("filter_iterations", C.c_int), ("max_vm", C.c_double), ("seconds", C.c_double)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("tile", C.c_int), ("kchunk", C.c_int), ("variant64", C.c_int), ("variant32", C.c_int),
+                ("ms64", C.c_double), ("ms32", C.c_double), ("ms32_parity", C.c_double), ("ms32_dense", C.c_double),
+                ("ms32_mfma", C.c_double)]
 
 
 class LearnStats(C.Structure):
@@ -135,6 +141,7 @@ def load():
         L.hx_timing_enable.argtypes = [vp, i]
         L.hx_timing_read.argtypes = [vp, i, pd, P(C.c_long)]
         L.hx_device_info.argtypes = [vp, P(C.c_int), P(C.c_int), pd]
+        L.hx_autotune.argtypes = [vp, i, P(TuneResult)]
         pf = P(C.c_float)
         L.hx_test_unet_nparam.argtypes = [vp, P(C.c_long)]
         L.hx_test_unet_init.argtypes = [vp, i, pf]
@@ -389,6 +396,11 @@ class Context:
         self.check(self.L.hx_test_unet_backward(self.h, gy.ctypes.data_as(C.POINTER(C.c_float)),
                                                 g.ctypes.data_as(C.POINTER(C.c_float))), "hx_test_unet_backward")
         return g
+
+    def autotune(self, nrhs=1):
+        r = TuneResult()
+        self.check(self.L.hx_autotune(self.h, int(nrhs), C.byref(r)), "hx_autotune")
+        return {k: getattr(r, k) for k, _ in TuneResult._fields_}
 
     # ---- timing -----------------------------------------------------------------------
     def timing(self, on: bool):

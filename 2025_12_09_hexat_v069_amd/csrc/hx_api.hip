@@ -99,14 +99,14 @@ struct hx_ctx {
     int kchunk = 16;
 
     DevBuf<uint8_t> fixed;
-    DevBuf<double> E, f, x, r, p, q, minv, Kbin_d;
-    DevBuf<float> Kbin32_d, E32, x32, r32, p32, q32, minv32;
+    DevBuf<double> E, f, x, r, p, q, minv, Kbin_d, p2;
+    DevBuf<float> Kbin32_d, E32, x32, r32, p32, q32, minv32, p32b;
     DevBuf<double> partials;
     PcgState* st = nullptr;       // kMaxRhs device states
     PcgState* st_host = nullptr;  // pinned mirror
     // block (multi-RHS) buffers, nrhs * ndof each
-    DevBuf<double> b_f, b_x, b_r, b_p, b_q;
-    DevBuf<float> b_x32, b_r32, b_p32, b_q32;
+    DevBuf<double> b_f, b_x, b_r, b_p, b_q, b_p2;
+    DevBuf<float> b_x32, b_r32, b_p32, b_q32, b_p32b;
     DevBuf<double> h_b, h_x, h_r, h_p, h_q, h_minv, h_in, h_out;
     DevBuf<double> e_pr, e_vm, e_sv, e_l1, e_dir, e_ce, e_dc, e_aux, small;
     // SIMP session
@@ -238,10 +238,11 @@ int run_batch(hx_ctx* c, const std::string& key, F&& launch_batch) {
     return HX_OK;
 }
 
-std::string batch_key(const char* kind, int nr, int batch, int variant, double shift, const void* a, const void* b,
-                      const void* d, const void* e, const void* f) {
-    char buf[256];
-    std::snprintf(buf, sizeof buf, "%s|%d|%d|%d|%a|%p|%p|%p|%p|%p", kind, nr, batch, variant, shift, a, b, d, e, f);
+std::string batch_key(hx_ctx* c, const char* kind, int nr, int batch, int variant, double shift, const void* a,
+                      const void* b, const void* d, const void* e, const void* f) {
+    char buf[320];
+    std::snprintf(buf, sizeof buf, "%s|%d|%d|%d|%a|%p|%p|%p|%p|%p|%d|%d", kind, nr, batch, variant, shift, a, b, d, e,
+                  f, c->tile, c->kchunk);
     return buf;
 }
 
@@ -269,9 +270,10 @@ long ndof(const hx_ctx* c) { return 3 * c->g.nn; }
 long ndof(const hx_ctx* c);
 
 int mv64(hx_ctx* c, const double* x, double* y, const double* E, double shift, int mode, int variant,
-         int nrhs = 1) {
-    MvArgs<double> a{x, y, E, shift, mode, c->st, c->partials.p, ndof(c)};
-    const int v = variant == HX_MV_AUTO ? c->mv_variant : variant;
+         int nrhs = 1, const double* fr = nullptr, const double* fminv = nullptr, double* fp_out = nullptr) {
+    MvArgs<double> a{x, y, E, shift, mode, c->st, c->partials.p, ndof(c), fr, fminv, fp_out};
+    int v = variant == HX_MV_AUTO ? c->mv_variant : variant;
+    if (fp_out && v == HX_MV_GATHER) v = HX_MV_PARITY;  // the gather kernel has no fused p-update
     TimeScope ts(c, HX_K_MATVEC);
     if (v == HX_MV_DENSE) launch_matvec_dense<double>(c->g, c->Kbin_d.p, a, c->kchunk, c->stream, nrhs);
     else if (v == HX_MV_GATHER) launch_matvec_gather(c->g, c->Kbin_d.p, a, c->stream, nrhs);
@@ -279,8 +281,9 @@ int mv64(hx_ctx* c, const double* x, double* y, const double* E, double shift, i
     return HX_OK;
 }
 
-int mv32(hx_ctx* c, const float* x, float* y, const float* E, double shift, int mode, int variant, int nrhs = 1) {
-    MvArgs<float> a{x, y, E, (float)shift, mode, c->st, c->partials.p, ndof(c)};
+int mv32(hx_ctx* c, const float* x, float* y, const float* E, double shift, int mode, int variant, int nrhs = 1,
+         const float* fr = nullptr, const float* fminv = nullptr, float* fp_out = nullptr) {
+    MvArgs<float> a{x, y, E, (float)shift, mode, c->st, c->partials.p, ndof(c), fr, fminv, fp_out};
     TimeScope ts(c, HX_K_MATVEC32);
     if (variant == HX_MV_DENSE) launch_matvec_dense<float>(c->g, c->Kbin32_d.p, a, c->kchunk, c->stream, nrhs);
     else if (variant == HX_MV_MFMA) launch_matvec_mfma(c->g, c->Kbin32_d.p, a, c->kchunk, c->stream, nrhs);
@@ -298,6 +301,8 @@ struct SolveVecs {
     int nrhs;
     double *f, *x, *r, *p, *q;
     float *x32, *r32, *p32, *q32;
+    double* p2;   // second buffer of the double-buffered search direction
+    float* p32b;
 };
 
 bool all_done(const hx_ctx* c, int nrhs) {
@@ -331,7 +336,8 @@ int worst_code(const hx_ctx* c, int nrhs) {
 int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& v) {
     const long n = ndof(c);
     const int nr = v.nrhs;
-    const int batch = o.batch > 0 ? o.batch : (n * nr < 3000000 ? 64 : 16);
+    int batch = o.batch > 0 ? o.batch : (n * nr < 3000000 ? 64 : 16);
+    batch = batch < 2 ? 2 : (batch & ~1);  // even: the double-buffered p returns to buffer 0
     launch_elastic_minv<double>(c->g, c->kd, c->E.p, c->has_fixed ? c->fixed.p : nullptr, o.shift, c->minv.p,
                                 c->stream);
     launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream, nr);
@@ -342,20 +348,20 @@ int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& 
     HXK();
     int rc = sync_state(c, nr);
     if (rc) return rc;
-    const std::string key = batch_key("f64", nr, batch, o.matvec == HX_MV_AUTO ? c->mv_variant : o.matvec, o.shift,
-                                      v.x, v.p, v.q, v.r, c->E.p);
+    const std::string key = batch_key(c, "f64", nr, batch, o.matvec == HX_MV_AUTO ? c->mv_variant : o.matvec, o.shift,
+                                      v.x, v.p, v.q, v.r, v.p2);
+    // p is double-buffered (v.p, v.p2): each matvec forms p_new = D^-1 r + beta p_old
+    // from the plane it stages anyway and writes it beside q; batches are even.
     while (!all_done(c, nr)) {
         rc = run_batch(c, key, [&] {
             for (int b = 0; b < batch; ++b) {
-                mv64(c, v.p, v.q, c->E.p, o.shift, MV_PCG, o.matvec, nr);
+                double* pin = (b & 1) ? v.p2 : v.p;
+                double* pout = (b & 1) ? v.p : v.p2;
+                mv64(c, pin, v.q, c->E.p, o.shift, MV_PCG, o.matvec, nr, v.r, c->minv.p, pout);
                 {
                     TimeScope ts(c, HX_K_UPDATE);
-                    launch_pcg_update<double>(n, v.x, v.p, v.r, v.q, c->minv.p, c->st, c->partials.p, c->stream,
+                    launch_pcg_update<double>(n, v.x, pout, v.r, v.q, c->minv.p, c->st, c->partials.p, c->stream,
                                               nr);
-                }
-                {
-                    TimeScope ts(c, HX_K_PUPDATE);
-                    launch_pcg_pupdate<double>(n, v.r, c->minv.p, v.p, c->st, c->stream, nr);
                 }
             }
         });
@@ -387,6 +393,7 @@ int ensure_fp32(hx_ctx* c) {
     HXA(c->p32, n);
     HXA(c->q32, n);
     HXA(c->minv32, n);
+    HXA(c->p32b, n);
     return HX_OK;
 }
 
@@ -400,7 +407,8 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
     const int nr = v.nrhs;
     int rc = ensure_fp32(c);
     if (rc) return rc;
-    const int batch = o.batch > 0 ? o.batch : (n * nr < 3000000 ? 64 : 16);
+    int batch = o.batch > 0 ? o.batch : (n * nr < 3000000 ? 64 : 16);
+    batch = batch < 2 ? 2 : (batch & ~1);
     const int v32 = (o.matvec == HX_MV_DENSE || o.matvec == HX_MV_MFMA) ? o.matvec
                     : (o.matvec == HX_MV_AUTO ? c->mv32_variant : HX_MV_PARITY);
     launch_convert<double, float>(c->g.ne, c->E.p, c->E32.p, c->stream);
@@ -416,24 +424,21 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
     launch_true_residual(n, v.f, v.q, v.x, c->minv.p, o.shift, nullptr, v.r32, c->st, c->partials.p, 3, c->stream,
                          nr);
     launch_ir_resume(c->st, o.tol, kIrDelta, 1, c->stream, nr);
-    launch_pcg_pupdate<float>(n, v.r32, c->minv32.p, v.p32, c->st, c->stream, nr);
     HXK();
     if ((rc = sync_state(c, nr))) return rc;
     int refinements = 0;
-    const std::string key = batch_key("f32", nr, batch, v32, o.shift, v.x32, v.p32, v.q32, v.r32, c->E32.p);
+    const std::string key = batch_key(c, "f32", nr, batch, v32, o.shift, v.x32, v.p32, v.q32, v.r32, v.p32b);
     while (!all_done(c, nr)) {
         while (!all_done(c, nr)) {
             rc = run_batch(c, key, [&] {
                 for (int b = 0; b < batch; ++b) {
-                    mv32(c, v.p32, v.q32, c->E32.p, o.shift, MV_PCG, v32, nr);
+                    float* pin = (b & 1) ? v.p32b : v.p32;
+                    float* pout = (b & 1) ? v.p32 : v.p32b;
+                    mv32(c, pin, v.q32, c->E32.p, o.shift, MV_PCG, v32, nr, v.r32, c->minv32.p, pout);
                     {
                         TimeScope ts(c, HX_K_UPDATE);
-                        launch_pcg_update<float>(n, v.x32, v.p32, v.r32, v.q32, c->minv32.p, c->st, c->partials.p,
+                        launch_pcg_update<float>(n, v.x32, pout, v.r32, v.q32, c->minv32.p, c->st, c->partials.p,
                                                  c->stream, nr);
-                    }
-                    {
-                        TimeScope ts(c, HX_K_PUPDATE);
-                        launch_pcg_pupdate<float>(n, v.r32, c->minv32.p, v.p32, c->st, c->stream, nr);
                     }
                 }
             });
@@ -441,12 +446,12 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
             if ((rc = sync_state(c, nr))) return rc;
         }
         // reliable update of every column: fold the FP32 correction, replace r
+        launch_fix_p<float>(n, v.p32, v.p32b, c->st, c->stream, nr);
         launch_accumulate(n, v.x, v.x32, c->stream, nr);
         mv64(c, v.x, v.q, c->E.p, 0.0, MV_PLAIN, HX_MV_PARITY, nr);
         launch_true_residual(n, v.f, v.q, v.x, c->minv.p, o.shift, nullptr, v.r32, c->st, c->partials.p, 2,
                              c->stream, nr);
         launch_ir_resume(c->st, o.tol, kIrDelta, 0, c->stream, nr);
-        launch_pcg_pupdate<float>(n, v.r32, c->minv32.p, v.p32, c->st, c->stream, nr);
         HXK();
         ++refinements;
         if ((rc = sync_state(c, nr))) return rc;
@@ -501,7 +506,7 @@ int pcg_learned(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
         jo.warm_start = 1;
         jo.max_iter = std::max(1, o.max_iter - lit);
         rc = pcg_fp64(c, jo, s, SolveVecs{1, c->f.p, c->x.p, c->r.p, c->p.p, c->q.p, nullptr, nullptr, nullptr,
-                                          nullptr});
+                                          nullptr, c->p2.p, nullptr});
         if (rc == HX_ERR_HIP || rc == HX_ERR_OOM) return rc;
         fallbacks = 1;
         if (s) {
@@ -526,7 +531,8 @@ int pcg_learned(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
 }
 
 SolveVecs single_vecs(hx_ctx* c) {
-    return SolveVecs{1, c->f.p, c->x.p, c->r.p, c->p.p, c->q.p, c->x32.p, c->r32.p, c->p32.p, c->q32.p};
+    return SolveVecs{1, c->f.p, c->x.p, c->r.p, c->p.p, c->q.p, c->x32.p, c->r32.p, c->p32.p, c->q32.p,
+                     c->p2.p, c->p32b.p};
 }
 
 // ------------------------------------------------------------- Helmholtz
@@ -677,11 +683,11 @@ void hx_ctx_destroy(hx_ctx* c) {
                            &c->h_b, &c->h_x, &c->h_r, &c->h_p, &c->h_q, &c->h_minv, &c->h_in, &c->h_out,
                            &c->e_pr, &c->e_vm, &c->e_sv, &c->e_l1, &c->e_dir, &c->e_ce, &c->e_dc, &c->e_aux,
                            &c->small, &c->t_x, &c->t_xphys, &c->t_xnew, &c->t_dc, &c->t_dcf, &c->t_dv, &c->t_vm,
-                           &c->t_ones, &c->b_f, &c->b_x, &c->b_r, &c->b_p, &c->b_q, &c->l_z, &c->l_zold,
+                           &c->t_ones, &c->b_f, &c->b_x, &c->b_r, &c->b_p, &c->b_q, &c->b_p2, &c->p2, &c->l_z, &c->l_zold,
                            &c->l_ustar, &c->l_samples, &c->l_rho};
     for (auto b : d) b->release();
     DevBuf<float>* fl[] = {&c->Kbin32_d, &c->E32, &c->x32, &c->r32, &c->p32, &c->q32, &c->minv32,
-                           &c->b_x32, &c->b_r32, &c->b_p32, &c->b_q32};
+                           &c->b_x32, &c->b_r32, &c->b_p32, &c->b_q32, &c->p32b, &c->b_p32b};
     for (auto b : fl) b->release();
     c->fixed.release();
     if (c->st) (void)hipFree(c->st);
@@ -764,6 +770,7 @@ int hx_grid_set(hx_ctx* c, int nx, int ny, int nz, double dx, double dy, double 
     HXA(c->p, n);
     HXA(c->q, n);
     HXA(c->minv, n);
+    HXA(c->p2, n);
     HXA(c->Kbin_d, 576);
     HXA(c->Kbin32_d, 576);
     const long np = std::max<long>({3L * kVecBlocks + 8, (long)matvec_parity_blocks(g, 0, 4) + 8,
@@ -885,19 +892,21 @@ int hx_block_pcg_solve(hx_ctx* c, int nrhs, const double* Ee, const double* F, d
     HXA(c->b_r, nt);
     HXA(c->b_p, nt);
     HXA(c->b_q, nt);
+    HXA(c->b_p2, nt);
     if (o->precision == HX_FP32_IR) {
         if (int rc = ensure_fp32(c)) return rc;
         HXA(c->b_x32, nt);
         HXA(c->b_r32, nt);
         HXA(c->b_p32, nt);
         HXA(c->b_q32, nt);
+        HXA(c->b_p32b, nt);
     }
     int rc;
     if ((rc = upload(c, c->E.p, Ee, c->g.ne * sizeof(double))) || (rc = upload(c, c->b_f.p, F, nt * sizeof(double))))
         return rc;
     if (o->warm_start && (rc = upload(c, c->b_x.p, U, nt * sizeof(double)))) return rc;
     SolveVecs v{nrhs, c->b_f.p, c->b_x.p, c->b_r.p, c->b_p.p, c->b_q.p, c->b_x32.p, c->b_r32.p, c->b_p32.p,
-                c->b_q32.p};
+                c->b_q32.p, c->b_p2.p, c->b_p32b.p};
     hipEvent_t a = get_event(c), b = get_event(c);
     (void)hipEventRecord(a, c->stream);
     const int prc = run_pcg(c, *o, s, v);
@@ -1298,6 +1307,88 @@ int hx_test_unet_backward(hx_ctx* c, const float* gy, float* grad) {
     if (!c->unet || !unet_ready(c->unet)) return fail(c, HX_ERR_STATE, "hx_test_unet_init first");
     unet_backward_host(c->unet, gy, grad, c->stream);
     HXK();
+    return HX_OK;
+}
+
+// Time every matvec launch shape on the current grid (random moduli) and keep
+// the fastest per precision: FP64 {parity 16x16 / 32x8 tiles, z-chunks
+// 8/16/32, dense, gather}, FP32 {parity, dense, MFMA}.
+int hx_autotune(hx_ctx* c, int nrhs, hx_tune_result* out) {
+    if (!c || nrhs < 1 || nrhs > kMaxRhs) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    const long n = ndof(c);
+    if (int rc = ensure_fp32(c)) return rc;
+    DevBuf<double> tx, ty;
+    DevBuf<float> fx, fy;
+    HXA(tx, n * nrhs);
+    HXA(ty, n * nrhs);
+    HXA(fx, n * nrhs);
+    HXA(fy, n * nrhs);
+    launch_fill(n * nrhs, 1.0, tx.p, c->stream);
+    launch_fill32(n * nrhs, 1.0f, fx.p, c->stream);
+    launch_fill(c->g.ne, 0.5, c->E.p, c->stream);
+    launch_convert<double, float>(c->g.ne, c->E.p, c->E32.p, c->stream);
+    const bool was_timing = c->timing;
+    c->timing = false;
+    hipEvent_t a = get_event(c), b = get_event(c);
+    auto time_it = [&](auto&& launch) {
+        for (int w = 0; w < 2; ++w) launch();
+        (void)hipEventRecord(a, c->stream);
+        const int reps = 5;
+        for (int r = 0; r < reps; ++r) launch();
+        (void)hipEventRecord(b, c->stream);
+        (void)hipEventSynchronize(b);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        return (double)ms / reps;
+    };
+    const int tiles[] = {0, 1}, chunks[] = {8, 16, 32};
+    double best64 = 1e30, best32 = 1e30;
+    int bt = 0, bk = 16, bv64 = HX_MV_PARITY, bv32 = HX_MV_PARITY;
+    for (int t : tiles)
+        for (int k : chunks) {
+            c->tile = t;
+            c->kchunk = k;
+            const double ms = time_it([&] { mv64(c, tx.p, ty.p, c->E.p, 0.0, MV_PLAIN, HX_MV_PARITY, nrhs); });
+            if (ms < best64) best64 = ms, bt = t, bk = k, bv64 = HX_MV_PARITY;
+        }
+    c->tile = bt;
+    c->kchunk = bk;
+    for (int v : {HX_MV_DENSE, HX_MV_GATHER}) {
+        const double ms = time_it([&] { mv64(c, tx.p, ty.p, c->E.p, 0.0, MV_PLAIN, v, nrhs); });
+        if (ms < best64) best64 = ms, bv64 = v;
+    }
+    double t32[3] = {0, 0, 0};
+    int vv[3] = {HX_MV_PARITY, HX_MV_DENSE, HX_MV_MFMA};
+    for (int i = 0; i < 3; ++i) {
+        if (vv[i] != HX_MV_PARITY && c->tile != 0) {  // dense/MFMA variants use the 16x16 tile
+            const int keep = c->tile;
+            c->tile = 0;
+            t32[i] = time_it([&] { mv32(c, fx.p, fy.p, c->E32.p, 0.0, MV_PLAIN, vv[i], nrhs); });
+            c->tile = keep;
+        } else {
+            t32[i] = time_it([&] { mv32(c, fx.p, fy.p, c->E32.p, 0.0, MV_PLAIN, vv[i], nrhs); });
+        }
+        if (t32[i] < best32) best32 = t32[i], bv32 = vv[i];
+    }
+    c->free_events.push_back(a);
+    c->free_events.push_back(b);
+    c->mv_variant = bv64;
+    c->mv32_variant = (bv32 != HX_MV_PARITY && c->tile != 0) ? HX_MV_PARITY : bv32;
+    c->timing = was_timing;
+    clear_graphs(c);
+    HXK();
+    if (out) {
+        out->tile = c->tile;
+        out->kchunk = c->kchunk;
+        out->variant64 = c->mv_variant;
+        out->variant32 = c->mv32_variant;
+        out->ms64 = best64;
+        out->ms32 = best32;
+        out->ms32_parity = t32[0];
+        out->ms32_dense = t32[1];
+        out->ms32_mfma = t32[2];
+    }
     return HX_OK;
 }
 

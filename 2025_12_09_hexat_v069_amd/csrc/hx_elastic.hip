@@ -25,7 +25,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // Element-operator modes of the z-marching kernel
 enum { EL_PARITY = 0, EL_DENSE = 1, EL_MFMA = 2 };
 
-template <typename T, int TX, int TY, int MODE>
+template <typename T, int TX, int TY, int MODE, bool FUSE>
 __global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T> K, DenseK<T> Kd, MvArgs<T> a,
                                                           int kchunk, int ntx, int nty, int ntiles) {
     constexpr int NT = TX * TY;
@@ -41,11 +41,16 @@ __global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T>
     // multi-RHS: blockIdx.y selects the column (own x, y, state, partials)
     a.x += (long)blockIdx.y * a.cs;
     a.y += (long)blockIdx.y * a.cs;
+    if constexpr (FUSE) {
+        a.r += (long)blockIdx.y * a.cs;
+        a.p_out += (long)blockIdx.y * a.cs;
+    }
     if (a.mode == MV_PCG) {
         a.st += blockIdx.y;
         a.partials += (long)blockIdx.y * gridDim.x;
     }
     if (a.mode == MV_PCG && a.st->done) return;
+    const double beta = FUSE ? a.st->beta : 0.0;
 
     // XCD-aware tile order: the 8 XCDs take blocks round-robin, so give each
     // XCD a contiguous range of tiles (neighbouring tiles share halo planes).
@@ -68,19 +73,32 @@ __global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T>
         const bool nown = nthr && ni < g.NX && nj < g.NY;
         const long plane_stride = (long)g.NX * g.NY;
 
-        T regs[NL];
+        // Plane loads are issued unconditionally from clamped addresses and
+        // masked when parked in LDS: a load under a divergent branch makes the
+        // compiler wait for it inside the branch, which serialises the
+        // prefetch of plane k+2 with the element work of plane k.
+        const int ie_c = min(max(ie, 0), g.nx - 1), je_c = min(max(je, 0), g.ny - 1);
+        T rx[NL], rr[FUSE ? NL : 1], rm[FUSE ? NL : 1];
+        unsigned valid = 0;
         auto plane_load = [&](int k) {
+            const bool kin = k >= 0 && k < g.NZ;
+            const long kc = min(max(k, 0), g.NZ - 1);
+            valid = 0;
 #pragma unroll
             for (int m = 0; m < NL; ++m) {
-                const int idx = threadIdx.x + m * NT;
-                T v = T(0);
-                if (idx < PLANE && k >= 0 && k < g.NZ) {
-                    const int row = idx / (3 * PX), col3 = idx - row * 3 * PX;
-                    const int ii = i0 - 1 + col3 / 3, jj = j0 - 1 + row;
-                    if (ii >= 0 && ii < g.NX && jj >= 0 && jj < g.NY)
-                        v = a.x[3 * ((long)k * plane_stride + (long)jj * g.NX + (i0 - 1)) + col3];
+                const int idx = min(threadIdx.x + m * NT, (unsigned)PLANE - 1);
+                const int row = idx / (3 * PX), col3 = idx - row * 3 * PX, cx3 = col3 / 3;
+                const int ii = i0 - 1 + cx3, jj = j0 - 1 + row;
+                const bool ok = kin && (int)(threadIdx.x + m * NT) < PLANE && ii >= 0 && ii < g.NX && jj >= 0 &&
+                                jj < g.NY;
+                const int iic = min(max(ii, 0), g.NX - 1), jjc = min(max(jj, 0), g.NY - 1);
+                const long gi = 3 * (kc * plane_stride + (long)jjc * g.NX + iic) + (col3 - 3 * cx3);
+                rx[m] = a.x[gi];
+                if constexpr (FUSE) {
+                    rr[m] = a.r[gi];
+                    rm[m] = a.minv[gi];
                 }
-                regs[m] = v;
+                valid |= (unsigned)ok << m;
             }
         };
         auto plane_store = [&](int k) {
@@ -88,20 +106,28 @@ __global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T>
 #pragma unroll
             for (int m = 0; m < NL; ++m) {
                 const int idx = threadIdx.x + m * NT;
+                T v;
+                if constexpr (FUSE) v = T((double)rm[m] * (double)rr[m] + beta * (double)rx[m]);
+                else v = rx[m];
+                v = ((valid >> m) & 1u) ? v : T(0);
                 if (idx < PLANE) {
                     const int row = idx / (3 * PX), col3 = idx - row * 3 * PX;
-                    xs[sl][col3 % 3][row][col3 / 3] = regs[m];
+                    xs[sl][col3 % 3][row][col3 / 3] = v;
                 }
             }
         };
+        // E of element layer ke, loaded unconditionally from a clamped
+        // address and zeroed by a multiplicative mask (no select, so no load
+        // ends up under a branch and the loop header sees no pending load)
         auto elem_E = [&](int ke) -> T {
-            return (ecol && ke >= 0 && ke < g.nz) ? a.E[(long)ke * g.nx * g.ny + (long)je * g.nx + ie] : T(0);
+            return a.E[(long)min(max(ke, 0), g.nz - 1) * g.nx * g.ny + (long)je_c * g.nx + ie_c];
         };
+        auto elem_ok = [&](int ke) -> T { return (ecol && ke >= 0 && ke < g.nz) ? T(1) : T(0); };
 
         plane_load(k0 - 1);
         plane_store(k0 - 1);
         plane_load(k0);
-        T ee = elem_E(k0 - 1);
+        T ee = elem_E(k0 - 1) * elem_ok(k0 - 1);
         T carry[3] = {T(0), T(0), T(0)};
 
         // MFMA operand B = Ke^T (binary order), two 16-column blocks, 6 k-steps
@@ -121,7 +147,7 @@ __global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T>
             if constexpr (MODE == EL_MFMA) es[ty][tx] = ee;
             __syncthreads();
             if (ke + 2 <= k1) plane_load(ke + 2);
-            const T ee_next = elem_E(ke + 1);
+            const T ee_raw = elem_E(ke + 1);
 
             if constexpr (MODE == EL_MFMA) {
                 // ---- element phase on MFMA: each wave multiplies the gathered
@@ -199,33 +225,21 @@ __global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T>
                         const T xv = xs[sl][c][oy + 1][ox + 1];
                         const T yv = carry[c] + bot[c] + a.shift * xv;
                         a.y[base + c] = yv;
+                        if constexpr (FUSE) a.p_out[base + c] = xv;
                         pq += (double)xv * (double)yv;
                     }
                 }
 #pragma unroll
                 for (int c = 0; c < 3; ++c) carry[c] = top[c];
             }
-            ee = ee_next;
+            ee = ee_raw * elem_ok(ke + 1);
             __syncthreads();
         }
     }
 
-    if (a.mode == MV_PCG) {
+    if (a.mode == MV_PCG) {  // per-block p'q; k_fin_alpha reduces them
         const double v = block_sum<NT>(pq, red);
         if (threadIdx.x == 0) a.partials[blockIdx.x] = v;
-        if (publish_and_check_last(&a.st->ticket[0], gridDim.x)) {
-            const double s = reduce_partials<NT>(a.partials, 1, 0, gridDim.x, red);
-            if (threadIdx.x == 0) {
-                a.st->pq = s;
-                if (!(fabs(s) >= 1e-300)) {  // GPUSolver.jl:462-466 (also catches NaN)
-                    a.st->done = 1;
-                    a.st->status = 4;
-                } else {
-                    a.st->alpha = a.st->rz / s;
-                }
-                a.st->ticket[0] = 0;
-            }
-        }
     }
 }
 
@@ -273,19 +287,6 @@ __global__ __launch_bounds__(256) void k_matvec_gather(GridArgs g, const double*
     if (a.mode == MV_PCG) {
         const double v = block_sum<256>(pq, red);
         if (threadIdx.x == 0) a.partials[blockIdx.x] = v;
-        if (publish_and_check_last(&a.st->ticket[0], gridDim.x)) {
-            const double s = reduce_partials<256>(a.partials, 1, 0, gridDim.x, red);
-            if (threadIdx.x == 0) {
-                a.st->pq = s;
-                if (!(fabs(s) >= 1e-300)) {
-                    a.st->done = 1;
-                    a.st->status = 4;
-                } else {
-                    a.st->alpha = a.st->rz / s;
-                }
-                a.st->ticket[0] = 0;
-            }
-        }
     }
 }
 
@@ -356,15 +357,28 @@ void launch_matvec_parity(const GridArgs& g, const ParityK<T>& K, const MvArgs<T
                           hipStream_t s, int nrhs) {
     int ntx, nty;
     DenseK<T> none{nullptr};
+    int nb;
+    const bool fuse = a.p_out != nullptr;
     if (tile == 1) {
         const int nt = zmarch_tiles<32, 8>(g, kchunk, &ntx, &nty);
-        hipLaunchKernelGGL((k_matvec_zmarch<T, 32, 8, EL_PARITY>), dim3(round8(nt), nrhs), dim3(256), 0, s, g, K, none, a,
-                           kchunk, ntx, nty, nt);
+        nb = round8(nt);
+        if (fuse)
+            hipLaunchKernelGGL((k_matvec_zmarch<T, 32, 8, EL_PARITY, true>), dim3(nb, nrhs), dim3(256), 0, s, g, K,
+                               none, a, kchunk, ntx, nty, nt);
+        else
+            hipLaunchKernelGGL((k_matvec_zmarch<T, 32, 8, EL_PARITY, false>), dim3(nb, nrhs), dim3(256), 0, s, g, K,
+                               none, a, kchunk, ntx, nty, nt);
     } else {
         const int nt = zmarch_tiles<16, 16>(g, kchunk, &ntx, &nty);
-        hipLaunchKernelGGL((k_matvec_zmarch<T, 16, 16, EL_PARITY>), dim3(round8(nt), nrhs), dim3(256), 0, s, g, K, none, a,
-                           kchunk, ntx, nty, nt);
+        nb = round8(nt);
+        if (fuse)
+            hipLaunchKernelGGL((k_matvec_zmarch<T, 16, 16, EL_PARITY, true>), dim3(nb, nrhs), dim3(256), 0, s, g, K,
+                               none, a, kchunk, ntx, nty, nt);
+        else
+            hipLaunchKernelGGL((k_matvec_zmarch<T, 16, 16, EL_PARITY, false>), dim3(nb, nrhs), dim3(256), 0, s, g, K,
+                               none, a, kchunk, ntx, nty, nt);
     }
+    if (a.mode == MV_PCG) launch_fin_alpha(a.st, a.partials, nb, nrhs, s);
 }
 
 template <typename T>
@@ -373,8 +387,13 @@ void launch_matvec_dense(const GridArgs& g, const T* Kbin, const MvArgs<T>& a, i
     int ntx, nty;
     ParityK<T> none{};
     const int nt = zmarch_tiles<16, 16>(g, kchunk, &ntx, &nty);
-    hipLaunchKernelGGL((k_matvec_zmarch<T, 16, 16, EL_DENSE>), dim3(round8(nt), nrhs), dim3(256), 0, s, g, none,
-                       DenseK<T>{Kbin}, a, kchunk, ntx, nty, nt);
+    if (a.p_out != nullptr)
+        hipLaunchKernelGGL((k_matvec_zmarch<T, 16, 16, EL_DENSE, true>), dim3(round8(nt), nrhs), dim3(256), 0, s, g,
+                           none, DenseK<T>{Kbin}, a, kchunk, ntx, nty, nt);
+    else
+        hipLaunchKernelGGL((k_matvec_zmarch<T, 16, 16, EL_DENSE, false>), dim3(round8(nt), nrhs), dim3(256), 0, s, g,
+                           none, DenseK<T>{Kbin}, a, kchunk, ntx, nty, nt);
+    if (a.mode == MV_PCG) launch_fin_alpha(a.st, a.partials, round8(nt), nrhs, s);
 }
 
 void launch_matvec_mfma(const GridArgs& g, const float* Kbin, const MvArgs<float>& a, int kchunk, hipStream_t s,
@@ -382,13 +401,19 @@ void launch_matvec_mfma(const GridArgs& g, const float* Kbin, const MvArgs<float
     int ntx, nty;
     ParityK<float> none{};
     const int nt = zmarch_tiles<16, 16>(g, kchunk, &ntx, &nty);
-    hipLaunchKernelGGL((k_matvec_zmarch<float, 16, 16, EL_MFMA>), dim3(round8(nt), nrhs), dim3(256), 0, s, g, none,
-                       DenseK<float>{Kbin}, a, kchunk, ntx, nty, nt);
+    if (a.p_out != nullptr)
+        hipLaunchKernelGGL((k_matvec_zmarch<float, 16, 16, EL_MFMA, true>), dim3(round8(nt), nrhs), dim3(256), 0, s,
+                           g, none, DenseK<float>{Kbin}, a, kchunk, ntx, nty, nt);
+    else
+        hipLaunchKernelGGL((k_matvec_zmarch<float, 16, 16, EL_MFMA, false>), dim3(round8(nt), nrhs), dim3(256), 0,
+                           s, g, none, DenseK<float>{Kbin}, a, kchunk, ntx, nty, nt);
+    if (a.mode == MV_PCG) launch_fin_alpha(a.st, a.partials, round8(nt), nrhs, s);
 }
 
 void launch_matvec_gather(const GridArgs& g, const double* Kbin, const MvArgs<double>& a, hipStream_t s,
                           int nrhs) {
     hipLaunchKernelGGL(k_matvec_gather, dim3(matvec_gather_blocks(g), nrhs), dim3(256), 0, s, g, Kbin, a);
+    if (a.mode == MV_PCG) launch_fin_alpha(a.st, a.partials, matvec_gather_blocks(g), nrhs, s);
 }
 
 template <typename T>

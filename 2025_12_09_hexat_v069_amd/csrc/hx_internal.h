@@ -180,32 +180,23 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
     return r;
 }
 
-__device__ __forceinline__ double load_coherent(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Publish this block's partials and find out whether it is the last block to
-// finish.  The last block then reduces all partials in a fixed order.
-__device__ __forceinline__ bool publish_and_check_last(unsigned* ticket, unsigned nblocks) {
-    __shared__ bool s_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned t = atomicAdd(ticket, 1u);
-        s_last = (t == nblocks - 1);
-    }
-    __syncthreads();
-    if (s_last) __threadfence();
-    return s_last;
-}
-
 // Sum partials[j + nv*b] over b in fixed order; result valid in thread 0.
+// Partials come from an earlier kernel on the same stream, so plain loads see
+// them (the kernel boundary orders them); no fences or tickets are used --
+// an agent-scope fence writes back the XCD's L2 on gfx950 and costs far more
+// than the extra one-block launch.
 template <int NT>
 __device__ __forceinline__ double reduce_partials(const double* partials, int nv, int j, unsigned nblocks,
                                                   double* sh) {
     double s = 0.0;
-    for (unsigned b = threadIdx.x; b < nblocks; b += NT) s += load_coherent(partials + (size_t)nv * b + j);
+    for (unsigned b = threadIdx.x; b < nblocks; b += NT) s += partials[(size_t)nv * b + j];
     return block_sum<NT>(s, sh);
+}
+template <int NT>
+__device__ __forceinline__ double reduce_partials_max(const double* partials, unsigned nblocks, double* sh) {
+    double s = -INFINITY;
+    for (unsigned b = threadIdx.x; b < nblocks; b += NT) s = fmax(s, partials[b]);
+    return block_max<NT>(s, sh);
 }
 
 }  // namespace hx

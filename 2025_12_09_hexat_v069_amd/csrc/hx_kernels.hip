@@ -69,20 +69,23 @@ __global__ __launch_bounds__(NT) void k_pcg_init(long n, const T* __restrict__ b
         partials[3 * blockIdx.x + 1] = v1;
         partials[3 * blockIdx.x + 2] = v2;
     }
-    if (publish_and_check_last(&st->ticket[1], gridDim.x)) {
-        const double s0 = reduce_partials<NT>(partials, 3, 0, gridDim.x, red);
-        const double s1 = reduce_partials<NT>(partials, 3, 1, gridDim.x, red);
-        const double s2 = reduce_partials<NT>(partials, 3, 2, gridDim.x, red);
-        if (threadIdx.x == 0) {
-            st->rz = s0;
-            st->normb = sqrt(s1);
-            st->rr = s2;
-            st->rel = s1 > 0.0 ? sqrt(s2 / s1) : 0.0;
-            if (s1 == 0.0) {  // GPUSolver.jl:373
-                st->done = 1;
-                st->status = 10;
-            }
-            st->ticket[1] = 0;
+}
+
+__global__ __launch_bounds__(NT) void k_fin_init(PcgState* st0, const double* partials0, unsigned nb) {
+    __shared__ double red[NT / 64];
+    PcgState* st = st0 + blockIdx.x;
+    const double* partials = partials0 + (size_t)blockIdx.x * nb * 3;
+    const double s0 = reduce_partials<NT>(partials, 3, 0, nb, red);
+    const double s1 = reduce_partials<NT>(partials, 3, 1, nb, red);
+    const double s2 = reduce_partials<NT>(partials, 3, 2, nb, red);
+    if (threadIdx.x == 0) {
+        st->rz = s0;
+        st->normb = sqrt(s1);
+        st->rr = s2;
+        st->rel = s1 > 0.0 ? sqrt(s2 / s1) : 0.0;
+        if (s1 == 0.0) {  // GPUSolver.jl:373
+            st->done = 1;
+            st->status = 10;
         }
     }
 }
@@ -145,14 +148,16 @@ __global__ __launch_bounds__(NT) void k_pcg_update(long n, T* __restrict__ x, co
         partials[2 * blockIdx.x + 0] = v0;
         partials[2 * blockIdx.x + 1] = v1;
     }
-    if (publish_and_check_last(&st->ticket[1], gridDim.x)) {
-        const double s0 = reduce_partials<NT>(partials, 2, 0, gridDim.x, red);
-        const double s1 = reduce_partials<NT>(partials, 2, 1, gridDim.x, red);
-        if (threadIdx.x == 0) {
-            pcg_after_update(st, s0, s1);
-            st->ticket[1] = 0;
-        }
-    }
+}
+
+__global__ __launch_bounds__(NT) void k_fin_update(PcgState* st0, const double* partials0, unsigned nb) {
+    __shared__ double red[NT / 64];
+    PcgState* st = st0 + blockIdx.x;
+    if (st->done) return;
+    const double* partials = partials0 + (size_t)blockIdx.x * nb * 2;
+    const double s0 = reduce_partials<NT>(partials, 2, 0, nb, red);
+    const double s1 = reduce_partials<NT>(partials, 2, 1, nb, red);
+    if (threadIdx.x == 0) pcg_after_update(st, s0, s1);
 }
 
 // p = minv r + beta p
@@ -171,19 +176,18 @@ __global__ __launch_bounds__(NT) void k_pcg_pupdate(long n, const T* __restrict_
 }
 
 __global__ __launch_bounds__(NT) void k_dot(long n, const double* __restrict__ a, const double* __restrict__ b,
-                                            double* out, unsigned* ticket, double* partials) {
+                                            double* partials) {
     __shared__ double red[NT / 64];
     double s = 0.0;
     for (long i = gtid(); i < n; i += gstride()) s += a[i] * b[i];
     const double v = block_sum<NT>(s, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = v;
-    if (publish_and_check_last(ticket, gridDim.x)) {
-        const double t = reduce_partials<NT>(partials, 1, 0, gridDim.x, red);
-        if (threadIdx.x == 0) {
-            *out = t;
-            *ticket = 0;
-        }
-    }
+}
+
+__global__ __launch_bounds__(NT) void k_fin_dot(double* out, const double* partials, unsigned nb) {
+    __shared__ double red[NT / 64];
+    const double t = reduce_partials<NT>(partials, 1, 0, nb, red);
+    if (threadIdx.x == 0) *out = t;
 }
 
 __global__ __launch_bounds__(NT) void k_fill(long n, double v, double* a) {
@@ -237,18 +241,21 @@ __global__ __launch_bounds__(NT) void k_true_residual(long n, const double* __re
         partials[3 * blockIdx.x + 1] = v1;
         partials[3 * blockIdx.x + 2] = v2;
     }
-    if (publish_and_check_last(&st->ticket[2], gridDim.x)) {
-        const double s0 = reduce_partials<NT>(partials, 3, 0, gridDim.x, red);
-        const double s1 = reduce_partials<NT>(partials, 3, 1, gridDim.x, red);
-        const double s2 = reduce_partials<NT>(partials, 3, 2, gridDim.x, red);
-        if (threadIdx.x == 0) {
-            if (mode & 1) st->normb = sqrt(s2);
-            if (mode & 2) st->scal[5] = s0;
-            else st->rz = s0;
-            st->scal[4] = s1;  // true ||r||^2
-            st->rel = st->normb > 0.0 ? sqrt(s1) / st->normb : 0.0;
-            st->ticket[2] = 0;
-        }
+}
+
+__global__ __launch_bounds__(NT) void k_fin_true(PcgState* st0, const double* partials0, unsigned nb, int mode) {
+    __shared__ double red[NT / 64];
+    PcgState* st = st0 + blockIdx.x;
+    const double* partials = partials0 + (size_t)blockIdx.x * nb * 3;
+    const double s0 = reduce_partials<NT>(partials, 3, 0, nb, red);
+    const double s1 = reduce_partials<NT>(partials, 3, 1, nb, red);
+    const double s2 = reduce_partials<NT>(partials, 3, 2, nb, red);
+    if (threadIdx.x == 0) {
+        if (mode & 1) st->normb = sqrt(s2);
+        if (mode & 2) st->scal[5] = s0;
+        else st->rz = s0;
+        st->scal[4] = s1;  // true ||r||^2
+        st->rel = st->normb > 0.0 ? sqrt(s1) / st->normb : 0.0;
     }
 }
 
@@ -284,6 +291,21 @@ __global__ void k_ir_resume(PcgState* st0, double tol_final, double delta, int f
     st->tol = fmax(tol_final, delta * st->rel);
     st->done = 0;
     st->status = 0;
+    st->scal[3] = (double)st->iter;  // first iteration of the next FP32 phase
+}
+
+// The FP32 recurrences double-buffer p (p, pb): iteration j of a phase reads
+// buffer j%2 and writes the other.  Before a phase resumes, move each
+// column's live direction into p (it sits in pb when the phase ended on an
+// even iteration count j = 0, 2, ...).
+template <typename T>
+__global__ __launch_bounds__(NT) void k_fix_p(long n, T* __restrict__ p, const T* __restrict__ pb,
+                                              const PcgState* st) {
+    st += blockIdx.y;
+    const long off = (long)blockIdx.y * n;
+    const long k = (long)st->iter - (long)st->scal[3];  // active iterations of the phase that just ended
+    if (k <= 0 || ((k - 1) & 1)) return;                // last write went to p (odd j) or none
+    for (long i = gtid(); i < n; i += gstride()) p[off + i] = pb[off + i];
 }
 
 template <typename T>
@@ -378,18 +400,22 @@ __global__ __launch_bounds__(NT) void k_helm_matvec(GridArgs g, HelmArgs A, cons
     if (mode == MV_PCG) {
         const double v = block_sum<NT>(pq, red);
         if (threadIdx.x == 0) partials[blockIdx.x] = v;
-        if (publish_and_check_last(&st->ticket[0], gridDim.x)) {
-            const double s = reduce_partials<NT>(partials, 1, 0, gridDim.x, red);
-            if (threadIdx.x == 0) {
-                st->pq = s;
-                if (!(fabs(s) >= 1e-300)) {
-                    st->done = 1;
-                    st->status = 4;
-                } else {
-                    st->alpha = st->rz / s;
-                }
-                st->ticket[0] = 0;
-            }
+    }
+}
+
+// alpha = rz / p'q with the breakdown test (GPUSolver.jl:462-466)
+__global__ __launch_bounds__(NT) void k_fin_alpha(PcgState* st0, const double* partials0, unsigned nb) {
+    __shared__ double red[NT / 64];
+    PcgState* st = st0 + blockIdx.x;
+    if (st->done) return;
+    const double s = reduce_partials<NT>(partials0 + (size_t)blockIdx.x * nb, 1, 0, nb, red);
+    if (threadIdx.x == 0) {
+        st->pq = s;
+        if (!(fabs(s) >= 1e-300)) {  // also catches NaN
+            st->done = 1;
+            st->status = 4;
+        } else {
+            st->alpha = st->rz / s;
         }
     }
 }
@@ -574,17 +600,21 @@ __global__ __launch_bounds__(NT) void k_oc_eval(long ne, const double* __restric
     for (long e = gtid(); e < ne; e += gstride()) s += oc_candidate(x[e], dc[e], dv[e], lmid, move, xmin);
     const double v = block_sum<NT>(s, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = v;
-    if (publish_and_check_last(&st->ticket[3], gridDim.x)) {
-        const double t = reduce_partials<NT>(partials, 1, 0, gridDim.x, red);
-        if (threadIdx.x == 0) {
-            if (t > volfrac * (double)ne) st->scal[0] = lmid;
-            else st->scal[1] = lmid;
-            st->scal[2] = lmid;
-            st->iter += 1;
-            const double l1 = st->scal[0], l2 = st->scal[1];
-            if ((l2 - l1) / (l1 + l2) <= rel_tol || st->iter >= 200) st->done = 1;
-            st->ticket[3] = 0;
-        }
+}
+
+__global__ __launch_bounds__(NT) void k_fin_oc_eval(PcgState* st, const double* partials, unsigned nb, long ne,
+                                                    double volfrac, double rel_tol) {
+    __shared__ double red[NT / 64];
+    if (st->done) return;
+    const double t = reduce_partials<NT>(partials, 1, 0, nb, red);
+    if (threadIdx.x == 0) {
+        const double lmid = 0.5 * (st->scal[0] + st->scal[1]);
+        if (t > volfrac * (double)ne) st->scal[0] = lmid;
+        else st->scal[1] = lmid;
+        st->scal[2] = lmid;
+        st->iter += 1;
+        const double l1 = st->scal[0], l2 = st->scal[1];
+        if ((l2 - l1) / (l1 + l2) <= rel_tol || st->iter >= 200) st->done = 1;
     }
 }
 
@@ -602,15 +632,12 @@ __global__ __launch_bounds__(NT) void k_oc_apply(long ne, const double* __restri
     }
     const double v = block_max<NT>(ch, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = v;
-    if (publish_and_check_last(&st->ticket[3], gridDim.x)) {
-        double m = 0.0;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += NT) m = fmax(m, load_coherent(partials + b));
-        m = block_max<NT>(m, red);
-        if (threadIdx.x == 0) {
-            st->scal[3] = m;
-            st->ticket[3] = 0;
-        }
-    }
+}
+
+__global__ __launch_bounds__(NT) void k_fin_max(double* out, const double* partials, unsigned nb) {
+    __shared__ double red[NT / 64];
+    const double m = reduce_partials_max<NT>(partials, nb, red);
+    if (threadIdx.x == 0) *out = fmax(m, 0.0);
 }
 
 // ---- flexible PCG (variable preconditioner z = P(r)) -----------------------
@@ -631,14 +658,13 @@ __global__ __launch_bounds__(NT) void k_fpcg_update(long n, double* __restrict__
     }
     const double v = block_sum<NT>(rr, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = v;
-    if (publish_and_check_last(&st->ticket[1], gridDim.x)) {
-        const double t = reduce_partials<NT>(partials, 1, 0, gridDim.x, red);
-        if (threadIdx.x == 0) {
-            const double rz = st->rz;
-            pcg_after_update(st, rz, t);  // keeps rz, beta := 1 (recomputed below)
-            st->ticket[1] = 0;
-        }
-    }
+}
+
+__global__ __launch_bounds__(NT) void k_fin_fpcg_update(PcgState* st, const double* partials, unsigned nb) {
+    __shared__ double red[NT / 64];
+    if (st->done) return;
+    const double t = reduce_partials<NT>(partials, 1, 0, nb, red);
+    if (threadIdx.x == 0) pcg_after_update(st, st->rz, t);  // keeps rz; beta is recomputed by k_fpcg_beta
 }
 
 // rz_new = r.z, rzo = r.z_old -> beta = (rz_new - rzo)/rz (Polak-Ribiere)
@@ -657,17 +683,16 @@ __global__ __launch_bounds__(NT) void k_fpcg_beta(long n, const double* __restri
         partials[2 * blockIdx.x] = v0;
         partials[2 * blockIdx.x + 1] = v1;
     }
-    if (publish_and_check_last(&st->ticket[2], gridDim.x)) {
-        const double s0 = reduce_partials<NT>(partials, 2, 0, gridDim.x, red);
-        const double s1 = reduce_partials<NT>(partials, 2, 1, gridDim.x, red);
-        if (threadIdx.x == 0) {
-            st->beta = first ? 0.0 : (s0 - s1) / st->rz;
-            st->rz = s0;
-            if (!(s0 > 0.0) && !first) {  // preconditioner lost positivity: restart with beta = 0
-                st->beta = 0.0;
-            }
-            st->ticket[2] = 0;
-        }
+}
+
+__global__ __launch_bounds__(NT) void k_fin_fpcg_beta(PcgState* st, const double* partials, unsigned nb, int first) {
+    __shared__ double red[NT / 64];
+    const double s0 = reduce_partials<NT>(partials, 2, 0, nb, red);
+    const double s1 = reduce_partials<NT>(partials, 2, 1, nb, red);
+    if (threadIdx.x == 0) {
+        st->beta = first ? 0.0 : (s0 - s1) / st->rz;
+        st->rz = s0;
+        if (!(s0 > 0.0) && !first) st->beta = 0.0;  // preconditioner lost positivity: restart
     }
 }
 
@@ -702,11 +727,13 @@ void launch_pcg_init(long n, const T* b, T* x, const T* q, const T* minv, T* r, 
                      double* partials, hipStream_t s, int nrhs) {
     hipLaunchKernelGGL((k_pcg_init<T>), dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, b, x, q, minv, r, p, shift, st,
                        partials);
+    hipLaunchKernelGGL(k_fin_init, dim3(nrhs), dim3(NT), 0, s, st, partials, (unsigned)kVecBlocks);
 }
 template <typename T>
 void launch_pcg_update(long n, T* x, const T* p, T* r, const T* q, const T* minv, PcgState* st, double* partials,
                        hipStream_t s, int nrhs) {
     hipLaunchKernelGGL((k_pcg_update<T>), dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, x, p, r, q, minv, st, partials);
+    hipLaunchKernelGGL(k_fin_update, dim3(nrhs), dim3(NT), 0, s, st, partials, (unsigned)kVecBlocks);
 }
 template <typename T>
 void launch_pcg_pupdate(long n, const T* r, const T* minv, T* p, const PcgState* st, hipStream_t s, int nrhs) {
@@ -714,7 +741,9 @@ void launch_pcg_pupdate(long n, const T* r, const T* minv, T* p, const PcgState*
 }
 void launch_dot(long n, const double* a, const double* b, double* out, unsigned* ticket, double* partials,
                 hipStream_t s) {
-    hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(NT), 0, s, n, a, b, out, ticket, partials);
+    (void)ticket;
+    hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(NT), 0, s, n, a, b, partials);
+    hipLaunchKernelGGL(k_fin_dot, dim3(1), dim3(NT), 0, s, out, partials, (unsigned)kVecBlocks);
 }
 void launch_fill(long n, double v, double* a, hipStream_t s) {
     hipLaunchKernelGGL(k_fill, dim3(kVecBlocks), dim3(NT), 0, s, n, v, a);
@@ -731,7 +760,14 @@ void launch_true_residual(long n, const double* b, const double* q, const double
                           hipStream_t s, int nrhs) {
     hipLaunchKernelGGL(k_true_residual, dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, b, q, x, minv64, shift, r64, r32, st,
                        partials, mode);
+    hipLaunchKernelGGL(k_fin_true, dim3(nrhs), dim3(NT), 0, s, st, partials, (unsigned)kVecBlocks, mode);
 }
+template <typename T>
+void launch_fix_p(long n, T* p, const T* pb, const PcgState* st, hipStream_t s, int nrhs) {
+    hipLaunchKernelGGL((k_fix_p<T>), dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, p, pb, st);
+}
+template void launch_fix_p<float>(long, float*, const float*, const PcgState*, hipStream_t, int);
+template void launch_fix_p<double>(long, double*, const double*, const PcgState*, hipStream_t, int);
 void launch_ir_resume(PcgState* st, double tol_final, double delta, int first, hipStream_t s, int nrhs) {
     hipLaunchKernelGGL(k_ir_resume, dim3(1), dim3(64), 0, s, st, tol_final, delta, first, nrhs);
 }
@@ -755,6 +791,10 @@ int helm_matvec_blocks(const GridArgs& g) { return (int)blocks_for(g.nn); }
 void launch_helm_matvec(const GridArgs& g, const HelmArgs& A, const double* x, double* y, int mode, PcgState* st,
                         double* partials, hipStream_t s) {
     hipLaunchKernelGGL(k_helm_matvec, dim3(blocks_for(g.nn)), dim3(NT), 0, s, g, A, x, y, mode, st, partials);
+    if (mode == MV_PCG) launch_fin_alpha(st, partials, (int)blocks_for(g.nn), 1, s);
+}
+void launch_fin_alpha(PcgState* st, const double* partials, int nblocks, int nrhs, hipStream_t s) {
+    hipLaunchKernelGGL(k_fin_alpha, dim3(nrhs), dim3(NT), 0, s, st, partials, (unsigned)nblocks);
 }
 void launch_helm_extract(const GridArgs& g, const double* xn, double* out_e, hipStream_t s) {
     hipLaunchKernelGGL(k_helm_extract, dim3(blocks_for(g.ne)), dim3(NT), 0, s, g, xn, out_e);
@@ -774,10 +814,12 @@ void launch_simp_modulus(long ne, const double* xphys, double penal, double E0, 
 void launch_fpcg_update(long n, double* x, const double* p, double* r, const double* q, const double* minv,
                         PcgState* st, double* partials, hipStream_t s) {
     hipLaunchKernelGGL(k_fpcg_update, dim3(kVecBlocks), dim3(NT), 0, s, n, x, p, r, q, minv, st, partials);
+    hipLaunchKernelGGL(k_fin_fpcg_update, dim3(1), dim3(NT), 0, s, st, partials, (unsigned)kVecBlocks);
 }
 void launch_fpcg_beta(long n, const double* r, const double* z, const double* zold, PcgState* st, double* partials,
                       int first, hipStream_t s) {
     hipLaunchKernelGGL(k_fpcg_beta, dim3(kVecBlocks), dim3(NT), 0, s, n, r, z, zold, st, partials, first);
+    hipLaunchKernelGGL(k_fin_fpcg_beta, dim3(1), dim3(NT), 0, s, st, partials, (unsigned)kVecBlocks, first);
 }
 void launch_fpcg_pupdate(long n, const double* z, double* p, double* zold, const PcgState* st, hipStream_t s) {
     hipLaunchKernelGGL(k_fpcg_pupdate, dim3(kVecBlocks), dim3(NT), 0, s, n, z, p, zold, st);
@@ -790,10 +832,12 @@ void launch_oc_eval(long ne, const double* x, const double* dc, const double* dv
                     double xmin, double rel_tol, PcgState* st, double* partials, hipStream_t s) {
     hipLaunchKernelGGL(k_oc_eval, dim3(kVecBlocks), dim3(NT), 0, s, ne, x, dc, dv, volfrac, move, xmin, rel_tol, st,
                        partials);
+    hipLaunchKernelGGL(k_fin_oc_eval, dim3(1), dim3(NT), 0, s, st, partials, (unsigned)kVecBlocks, ne, volfrac, rel_tol);
 }
 void launch_oc_apply(long ne, const double* x, const double* dc, const double* dv, double move, double xmin,
                      double* xnew, PcgState* st, double* partials, hipStream_t s) {
     hipLaunchKernelGGL(k_oc_apply, dim3(kVecBlocks), dim3(NT), 0, s, ne, x, dc, dv, move, xmin, xnew, st, partials);
+    hipLaunchKernelGGL(k_fin_max, dim3(1), dim3(NT), 0, s, &st->scal[3], partials, (unsigned)kVecBlocks);
 }
 
 template void launch_pcg_init<double>(long, const double*, double*, const double*, const double*, double*, double*,

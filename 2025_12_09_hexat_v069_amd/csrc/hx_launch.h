@@ -18,6 +18,11 @@ struct MvArgs {
     PcgState* st;      // MV_PCG: pq -> alpha, breakdown test (one state per column)
     double* partials;  // MV_PCG: one double per block and column
     long cs;           // column stride of x / y (multi-RHS: blockIdx.y = column)
+    // fused p-update (MV_PCG only): x holds p_old; the kernel uses
+    // p = minv*r + beta*p_old, writes it to p_out and applies K to it
+    const T* r;
+    const T* minv;     // shared by all columns
+    T* p_out;
 };
 
 // tile: 0 -> 16x16 element columns (default), 1 -> 32x8
@@ -43,6 +48,10 @@ template <typename T>
 void launch_elastic_minv(const GridArgs& g, const double kd[3], const T* E, const uint8_t* fixed, double shift,
                          T* minv, hipStream_t s);
 void launch_elastic_diag(const GridArgs& g, const double kd[3], const double* E, double* diag, hipStream_t s);
+
+// one-block finalisers (one block per column): alpha = rz / p'q with the
+// breakdown test, from the per-block p'q partials of a PCG-mode operator
+void launch_fin_alpha(PcgState* st, const double* partials, int nblocks, int nrhs, hipStream_t s);
 
 // ---- PCG vector kernels ----------------------------------------------------
 // All of them take nrhs columns (blockIdx.y) of length n at stride n; the
@@ -74,6 +83,9 @@ void launch_true_residual(long n, const double* b, const double* q, const double
                           hipStream_t s, int nrhs = 1);
 // per column: resume unless the column has converged (rel < tol_final) or failed
 void launch_ir_resume(PcgState* st, double tol_final, double delta, int first, hipStream_t s, int nrhs = 1);
+// move each column's live search direction from pb into p (double-buffered p)
+template <typename T>
+void launch_fix_p(long n, T* p, const T* pb, const PcgState* st, hipStream_t s, int nrhs = 1);
 template <typename T>
 void launch_mask(long n, const T* minv, T* x, hipStream_t s, int nrhs = 1);
 // x64 += x32; x32 = 0

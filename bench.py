@@ -56,6 +56,9 @@ def cantilever(nx, ny, nz):
 def run_form(hx, L, ctx, nx, ny, nz, f, fixed, precision, steps, warmup, args):
     ctx.set_grid(nx, ny, nz, 1.0, 1.0, 1.0, 0.3)
     ctx.set_fixed(fixed)
+    if not args.no_autotune:
+        t = ctx.autotune(1)
+        log(f"  autotune: {t}")
     p = L.TOParams(E0=1.0, Emin=1e-9, nu=0.3, penal=3.0, volfrac=args.volfrac, move=0.2, xmin=0.0,
                    rmin=args.rmin, pcg_tol=args.tol, pcg_max_iter=args.max_iter, precision=precision, precond=0,
                    filter_tol=1e-10, filter_max_iter=2000, oc_rel_tol=1e-9, with_stress=1)
@@ -126,6 +129,7 @@ def main():
     ap.add_argument("--max-iter", type=int, default=100000)
     ap.add_argument("--forms", default="plain,precision")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-pcg-iters", type=int, default=20)
     ap.add_argument("--cpu-filter-iters", type=int, default=20)
     args = ap.parse_args()
@@ -150,12 +154,14 @@ def main():
         res[form] = {"s_per_iter": sps, "recs": recs, "kern": kern}
     main_form = "precision" if "precision" in res else forms[0]
     r = res[main_form]
-    # roofline of the dominant kernel of the main form
-    mv_bytes64 = 16.0 * ndof + 8.0 * ne  # read x, write y (8 B each per DOF), read E (8 B per element)
-    mv_bytes32 = 8.0 * ndof + 4.0 * ne
-    upd_bytes = {"plain": 56.0 * ndof, "precision": 28.0 * ndof}[main_form] if main_form in ("plain", "precision") \
-        else 56.0 * ndof
-    pup_bytes = {"plain": 32.0 * ndof, "precision": 16.0 * ndof}.get(main_form, 32.0 * ndof)
+    # roofline of the dominant kernel of the main form.  Algorithmic bytes per
+    # launch: the PCG matvec reads r, D^-1, p_old (the p-update is fused into
+    # its plane loads) and E, and writes q and p_new; the update reads x, p, r,
+    # q, D^-1 and writes x, r.
+    mv_bytes64 = 40.0 * ndof + 8.0 * ne
+    mv_bytes32 = 20.0 * ndof + 4.0 * ne
+    upd_bytes = 56.0 * ndof if main_form == "plain" else 28.0 * ndof
+    pup_bytes = 32.0 * ndof if main_form == "plain" else 16.0 * ndof
     cand = []
     for nm, by in (("matvec_fp64", mv_bytes64), ("matvec_fp32", mv_bytes32), ("pcg_update", upd_bytes),
                    ("pcg_pupdate", pup_bytes)):

@@ -196,6 +196,20 @@ int hx_learned_train(hx_ctx* ctx, const double* Ee, const double* f, hx_learn_st
 /* copy of the current weights (BF16 bit patterns) and their count */
 int hx_learned_weights(hx_ctx* ctx, uint16_t* out, int* count);
 
+/* ---- autotuner ---------------------------------------------------------------- */
+typedef struct {
+    int tile;         /* 0: 16x16 element tile, 1: 32x8 */
+    int kchunk;       /* z planes per workgroup */
+    int variant64;    /* chosen FP64 contraction (HX_MV_*) */
+    int variant32;    /* chosen FP32 contraction (HX_MV_*) */
+    double ms64, ms32;                       /* best times of one operator application */
+    double ms32_parity, ms32_dense, ms32_mfma; /* FP32 contraction candidates */
+} hx_tune_result;
+/* Time the operator's launch shapes (tile, z-chunk) and element contractions on
+ * the current grid for nrhs columns and keep the fastest (HX_MV_AUTO then uses
+ * them).  No reference counterpart. */
+int hx_autotune(hx_ctx* ctx, int nrhs, hx_tune_result* result);
+
 /* ---- kernel timing ---------------------------------------------------------- */
 enum { HX_K_MATVEC = 0, HX_K_UPDATE = 1, HX_K_PUPDATE = 2, HX_K_HELM = 3, HX_K_ELEM = 4,
        HX_K_MATVEC32 = 5, HX_K_COUNT = 6 };
