@@ -95,8 +95,7 @@ struct hx_ctx {
     bool has_fixed = false;
     int mv_variant = HX_MV_PARITY;
     int mv32_variant = HX_MV_PARITY;
-    int tile = 0;
-    int kchunk = 16;
+    MvShape shape64, shape32;  // launch shapes of the FP64 / FP32 operator
 
     DevBuf<uint8_t> fixed;
     DevBuf<double> E, f, x, r, p, q, minv, Kbin_d, p2;
@@ -241,8 +240,9 @@ int run_batch(hx_ctx* c, const std::string& key, F&& launch_batch) {
 std::string batch_key(hx_ctx* c, const char* kind, int nr, int batch, int variant, double shift, const void* a,
                       const void* b, const void* d, const void* e, const void* f) {
     char buf[320];
-    std::snprintf(buf, sizeof buf, "%s|%d|%d|%d|%a|%p|%p|%p|%p|%p|%d|%d", kind, nr, batch, variant, shift, a, b, d, e,
-                  f, c->tile, c->kchunk);
+    std::snprintf(buf, sizeof buf, "%s|%d|%d|%d|%a|%p|%p|%p|%p|%p|%d|%d|%d|%d|%d|%d", kind, nr, batch, variant,
+                  shift, a, b, d, e, f, c->shape64.tile, c->shape64.kchunk, c->shape64.pdepth, c->shape32.tile,
+                  c->shape32.kchunk, c->shape32.pdepth);
     return buf;
 }
 
@@ -275,9 +275,9 @@ int mv64(hx_ctx* c, const double* x, double* y, const double* E, double shift, i
     int v = variant == HX_MV_AUTO ? c->mv_variant : variant;
     if (fp_out && v == HX_MV_GATHER) v = HX_MV_PARITY;  // the gather kernel has no fused p-update
     TimeScope ts(c, HX_K_MATVEC);
-    if (v == HX_MV_DENSE) launch_matvec_dense<double>(c->g, c->Kbin_d.p, a, c->kchunk, c->stream, nrhs);
+    if (v == HX_MV_DENSE) launch_matvec_dense<double>(c->g, c->Kbin_d.p, a, c->shape64, c->stream, nrhs);
     else if (v == HX_MV_GATHER) launch_matvec_gather(c->g, c->Kbin_d.p, a, c->stream, nrhs);
-    else launch_matvec_parity<double>(c->g, c->kp64, a, c->tile, c->kchunk, c->stream, nrhs);
+    else launch_matvec_parity<double>(c->g, c->kp64, a, c->shape64, c->stream, nrhs);
     return HX_OK;
 }
 
@@ -285,9 +285,14 @@ int mv32(hx_ctx* c, const float* x, float* y, const float* E, double shift, int 
          const float* fr = nullptr, const float* fminv = nullptr, float* fp_out = nullptr) {
     MvArgs<float> a{x, y, E, (float)shift, mode, c->st, c->partials.p, ndof(c), fr, fminv, fp_out};
     TimeScope ts(c, HX_K_MATVEC32);
-    if (variant == HX_MV_DENSE) launch_matvec_dense<float>(c->g, c->Kbin32_d.p, a, c->kchunk, c->stream, nrhs);
-    else if (variant == HX_MV_MFMA) launch_matvec_mfma(c->g, c->Kbin32_d.p, a, c->kchunk, c->stream, nrhs);
-    else launch_matvec_parity<float>(c->g, c->kp32, a, c->tile, c->kchunk, c->stream, nrhs);
+    if (variant == HX_MV_DENSE || variant == HX_MV_MFMA) {
+        MvShape sh = c->shape32;  // the dense and MFMA contractions use the 16x16 tile
+        sh.tile = 0;
+        if (variant == HX_MV_DENSE) launch_matvec_dense<float>(c->g, c->Kbin32_d.p, a, sh, c->stream, nrhs);
+        else launch_matvec_mfma(c->g, c->Kbin32_d.p, a, sh, c->stream, nrhs);
+    } else {
+        launch_matvec_parity<float>(c->g, c->kp32, a, c->shape32, c->stream, nrhs);
+    }
     return HX_OK;
 }
 
@@ -1310,28 +1315,52 @@ int hx_test_unet_backward(hx_ctx* c, const float* gy, float* grad) {
     return HX_OK;
 }
 
-// Time every matvec launch shape on the current grid (random moduli) and keep
-// the fastest per precision: FP64 {parity 16x16 / 32x8 tiles, z-chunks
-// 8/16/32, dense, gather}, FP32 {parity, dense, MFMA}.
+// Time every launch shape of the operator as the PCG runs it (fused
+// p-update, p'q partials, alpha finaliser) on the current grid with uniform
+// moduli and a non-smooth vector, and keep the fastest per precision:
+// FP64 parity {16x16, 32x8 tiles} x {8, 16, 32 z planes} x {1, 2, 3 planes in
+// flight} and dense; FP32 the same parity shapes, dense and MFMA.
 int hx_autotune(hx_ctx* c, int nrhs, hx_tune_result* out) {
     if (!c || nrhs < 1 || nrhs > kMaxRhs) return HX_ERR_BAD_ARG;
     if (int rc = ensure_grid(c)) return rc;
     const long n = ndof(c);
     if (int rc = ensure_fp32(c)) return rc;
-    DevBuf<double> tx, ty;
-    DevBuf<float> fx, fy;
+    DevBuf<double> tx, ty, tr, tm, tp;
+    DevBuf<float> fx, fy, fr, fm, fp;
     HXA(tx, n * nrhs);
     HXA(ty, n * nrhs);
+    HXA(tr, n * nrhs);
+    HXA(tm, n);
+    HXA(tp, n * nrhs);
     HXA(fx, n * nrhs);
     HXA(fy, n * nrhs);
-    launch_fill(n * nrhs, 1.0, tx.p, c->stream);
-    launch_fill32(n * nrhs, 1.0f, fx.p, c->stream);
+    HXA(fr, n * nrhs);
+    HXA(fm, n);
+    HXA(fp, n * nrhs);
+    {
+        std::vector<double> h(n);
+        uint32_t z = 0x9e3779b9u;
+        for (long i = 0; i < n; ++i) {
+            z = z * 1664525u + 1013904223u;
+            h[i] = (double)(z >> 8) * (1.0 / 16777216.0) - 0.5;
+        }
+        for (int j = 0; j < nrhs; ++j) {
+            HXC(hipMemcpyAsync(tx.p + j * n, h.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            HXC(hipMemcpyAsync(tr.p + j * n, h.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        }
+        HXC(hipStreamSynchronize(c->stream));
+    }
+    launch_fill(n, 1.0, tm.p, c->stream);
+    launch_convert<double, float>(n * nrhs, tx.p, fx.p, c->stream);
+    launch_convert<double, float>(n * nrhs, tr.p, fr.p, c->stream);
+    launch_fill32(n, 1.0f, fm.p, c->stream);
     launch_fill(c->g.ne, 0.5, c->E.p, c->stream);
     launch_convert<double, float>(c->g.ne, c->E.p, c->E32.p, c->stream);
     const bool was_timing = c->timing;
     c->timing = false;
     hipEvent_t a = get_event(c), b = get_event(c);
     auto time_it = [&](auto&& launch) {
+        launch_pcg_reset(c->st, 0.0, 0.0, 1 << 30, 1, c->stream, nrhs);
         for (int w = 0; w < 2; ++w) launch();
         (void)hipEventRecord(a, c->stream);
         const int reps = 5;
@@ -1342,45 +1371,50 @@ int hx_autotune(hx_ctx* c, int nrhs, hx_tune_result* out) {
         (void)hipEventElapsedTime(&ms, a, b);
         return (double)ms / reps;
     };
-    const int tiles[] = {0, 1}, chunks[] = {8, 16, 32};
+    auto run64 = [&](int v) {
+        return time_it([&] { mv64(c, tx.p, ty.p, c->E.p, 0.0, MV_PCG, v, nrhs, tr.p, tm.p, tp.p); });
+    };
+    auto run32 = [&](int v) {
+        return time_it([&] { mv32(c, fx.p, fy.p, c->E32.p, 0.0, MV_PCG, v, nrhs, fr.p, fm.p, fp.p); });
+    };
+    const int tiles[] = {0, 1}, chunks[] = {8, 16, 32}, depths[] = {1, 2, 3};
     double best64 = 1e30, best32 = 1e30;
-    int bt = 0, bk = 16, bv64 = HX_MV_PARITY, bv32 = HX_MV_PARITY;
+    MvShape b64 = c->shape64, b32 = c->shape32;
+    int bv64 = HX_MV_PARITY, bv32 = HX_MV_PARITY;
     for (int t : tiles)
-        for (int k : chunks) {
-            c->tile = t;
-            c->kchunk = k;
-            const double ms = time_it([&] { mv64(c, tx.p, ty.p, c->E.p, 0.0, MV_PLAIN, HX_MV_PARITY, nrhs); });
-            if (ms < best64) best64 = ms, bt = t, bk = k, bv64 = HX_MV_PARITY;
-        }
-    c->tile = bt;
-    c->kchunk = bk;
-    for (int v : {HX_MV_DENSE, HX_MV_GATHER}) {
-        const double ms = time_it([&] { mv64(c, tx.p, ty.p, c->E.p, 0.0, MV_PLAIN, v, nrhs); });
-        if (ms < best64) best64 = ms, bv64 = v;
+        for (int k : chunks)
+            for (int d : depths) {
+                c->shape64 = MvShape{t, k, d};
+                c->shape32 = MvShape{t, k, d};
+                const double m64 = run64(HX_MV_PARITY);
+                if (m64 < best64) best64 = m64, b64 = c->shape64;
+                const double m32 = run32(HX_MV_PARITY);
+                if (m32 < best32) best32 = m32, b32 = c->shape32;
+            }
+    c->shape64 = b64;
+    c->shape32 = b32;
+    {  // the gather kernel has no fused p-update, so it is not a PCG candidate
+        const double ms = run64(HX_MV_DENSE);
+        if (ms < best64) best64 = ms, bv64 = HX_MV_DENSE;
     }
-    double t32[3] = {0, 0, 0};
-    int vv[3] = {HX_MV_PARITY, HX_MV_DENSE, HX_MV_MFMA};
-    for (int i = 0; i < 3; ++i) {
-        if (vv[i] != HX_MV_PARITY && c->tile != 0) {  // dense/MFMA variants use the 16x16 tile
-            const int keep = c->tile;
-            c->tile = 0;
-            t32[i] = time_it([&] { mv32(c, fx.p, fy.p, c->E32.p, 0.0, MV_PLAIN, vv[i], nrhs); });
-            c->tile = keep;
-        } else {
-            t32[i] = time_it([&] { mv32(c, fx.p, fy.p, c->E32.p, 0.0, MV_PLAIN, vv[i], nrhs); });
-        }
+    double t32[3] = {best32, 0, 0};
+    const int vv[3] = {HX_MV_PARITY, HX_MV_DENSE, HX_MV_MFMA};
+    for (int i = 1; i < 3; ++i) {
+        t32[i] = run32(vv[i]);
         if (t32[i] < best32) best32 = t32[i], bv32 = vv[i];
     }
     c->free_events.push_back(a);
     c->free_events.push_back(b);
     c->mv_variant = bv64;
-    c->mv32_variant = (bv32 != HX_MV_PARITY && c->tile != 0) ? HX_MV_PARITY : bv32;
+    c->mv32_variant = bv32;
     c->timing = was_timing;
     clear_graphs(c);
+    launch_pcg_reset(c->st, 0.0, 0.0, 0, 1, c->stream, nrhs);
     HXK();
+    HXC(hipStreamSynchronize(c->stream));
     if (out) {
-        out->tile = c->tile;
-        out->kchunk = c->kchunk;
+        out->tile = c->shape64.tile;
+        out->kchunk = c->shape64.kchunk;
         out->variant64 = c->mv_variant;
         out->variant32 = c->mv32_variant;
         out->ms64 = best64;
@@ -1388,6 +1422,10 @@ int hx_autotune(hx_ctx* c, int nrhs, hx_tune_result* out) {
         out->ms32_parity = t32[0];
         out->ms32_dense = t32[1];
         out->ms32_mfma = t32[2];
+        out->pdepth = c->shape64.pdepth;
+        out->tile32 = c->shape32.tile;
+        out->kchunk32 = c->shape32.kchunk;
+        out->pdepth32 = c->shape32.pdepth;
     }
     return HX_OK;
 }

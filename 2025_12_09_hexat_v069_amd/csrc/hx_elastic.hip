@@ -25,7 +25,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // Element-operator modes of the z-marching kernel
 enum { EL_PARITY = 0, EL_DENSE = 1, EL_MFMA = 2 };
 
-template <typename T, int TX, int TY, int MODE, bool FUSE>
+template <typename T, int TX, int TY, int MODE, bool FUSE, int PD>
 __global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T> K, DenseK<T> Kd, MvArgs<T> a,
                                                           int kchunk, int ntx, int nty, int ntiles) {
     constexpr int NT = TX * TY;
@@ -35,7 +35,7 @@ __global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T>
     static_assert(MODE != EL_MFMA || (TX == 16 && TY == 16), "MFMA layout assumes a 16x16 element tile");
     __shared__ T xs[2][3][PY][PX];
     __shared__ T ys[24][TY][TX];
-    __shared__ T es[MODE == EL_MFMA ? TY : 1][MODE == EL_MFMA ? TX : 1];
+    __shared__ T es[TY][TX];  // E of the element layer being processed
     __shared__ double red[NT / 64];
 
     // multi-RHS: blockIdx.y selects the column (own x, y, state, partials)
@@ -77,13 +77,16 @@ __global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T>
         // masked when parked in LDS: a load under a divergent branch makes the
         // compiler wait for it inside the branch, which serialises the
         // prefetch of plane k+2 with the element work of plane k.
+        // Plane k + 1 + PD is requested while element layer k is processed: a
+        // ring of PD register slots keeps PD planes in flight per block (the
+        // slot of a plane is a compile-time index, see the unrolled loop).
         const int ie_c = min(max(ie, 0), g.nx - 1), je_c = min(max(je, 0), g.ny - 1);
-        T rx[NL], rr[FUSE ? NL : 1], rm[FUSE ? NL : 1];
-        unsigned valid = 0;
-        auto plane_load = [&](int k) {
+        T rx[PD][NL], rr[PD][FUSE ? NL : 1], rm[PD][FUSE ? NL : 1], re[PD];
+        unsigned valid[PD];
+        auto plane_load = [&](int k, int q) {
             const bool kin = k >= 0 && k < g.NZ;
             const long kc = min(max(k, 0), g.NZ - 1);
-            valid = 0;
+            valid[q] = 0;
 #pragma unroll
             for (int m = 0; m < NL; ++m) {
                 const int idx = min(threadIdx.x + m * NT, (unsigned)PLANE - 1);
@@ -93,41 +96,40 @@ __global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T>
                                 jj < g.NY;
                 const int iic = min(max(ii, 0), g.NX - 1), jjc = min(max(jj, 0), g.NY - 1);
                 const long gi = 3 * (kc * plane_stride + (long)jjc * g.NX + iic) + (col3 - 3 * cx3);
-                rx[m] = a.x[gi];
+                rx[q][m] = a.x[gi];
                 if constexpr (FUSE) {
-                    rr[m] = a.r[gi];
-                    rm[m] = a.minv[gi];
+                    rr[q][m] = a.r[gi];
+                    rm[q][m] = a.minv[gi];
                 }
-                valid |= (unsigned)ok << m;
+                valid[q] |= (unsigned)ok << m;
             }
+            // E of element layer k-1 travels with plane k, so one wait covers both
+            re[q] = a.E[(long)min(max(k - 1, 0), g.nz - 1) * g.nx * g.ny + (long)je_c * g.nx + ie_c];
         };
-        auto plane_store = [&](int k) {
+        auto plane_store = [&](int k, int q) {
             const int sl = k & 1;
 #pragma unroll
             for (int m = 0; m < NL; ++m) {
                 const int idx = threadIdx.x + m * NT;
                 T v;
-                if constexpr (FUSE) v = T((double)rm[m] * (double)rr[m] + beta * (double)rx[m]);
-                else v = rx[m];
-                v = ((valid >> m) & 1u) ? v : T(0);
+                if constexpr (FUSE) v = T((double)rm[q][m] * (double)rr[q][m] + beta * (double)rx[q][m]);
+                else v = rx[q][m];
+                v = ((valid[q] >> m) & 1u) ? v : T(0);
                 if (idx < PLANE) {
                     const int row = idx / (3 * PX), col3 = idx - row * 3 * PX;
                     xs[sl][col3 % 3][row][col3 / 3] = v;
                 }
             }
+            es[ty][tx] = (ecol && k - 1 >= 0 && k - 1 < g.nz) ? re[q] : T(0);
         };
-        // E of element layer ke, loaded unconditionally from a clamped
-        // address and zeroed by a multiplicative mask (no select, so no load
-        // ends up under a branch and the loop header sees no pending load)
-        auto elem_E = [&](int ke) -> T {
-            return a.E[(long)min(max(ke, 0), g.nz - 1) * g.nx * g.ny + (long)je_c * g.nx + ie_c];
-        };
-        auto elem_ok = [&](int ke) -> T { return (ecol && ke >= 0 && ke < g.nz) ? T(1) : T(0); };
 
-        plane_load(k0 - 1);
-        plane_store(k0 - 1);
-        plane_load(k0);
-        T ee = elem_E(k0 - 1) * elem_ok(k0 - 1);
+        // prologue: plane k0-1 straight to LDS, planes k0 .. k0-1+PD into the
+        // ring (plane k0-1+j lives in slot j % PD)
+        plane_load(k0 - 1, 0);
+        plane_store(k0 - 1, 0);
+#pragma unroll
+        for (int j = 1; j <= PD; ++j)
+            if (k0 - 1 + j <= k1) plane_load(k0 - 1 + j, j % PD);
         T carry[3] = {T(0), T(0), T(0)};
 
         // MFMA operand B = Ke^T (binary order), two 16-column blocks, 6 k-steps
@@ -142,12 +144,15 @@ __global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T>
             }
         }
 
-        for (int ke = k0 - 1; ke < k1; ++ke) {
-            plane_store(ke + 1);
-            if constexpr (MODE == EL_MFMA) es[ty][tx] = ee;
+        for (int kb = k0 - 1; kb < k1; kb += PD) {
+#pragma unroll
+        for (int u = 0; u < PD; ++u) {
+            const int ke = kb + u;  // element layer; plane ke+1 sits in slot (u+1) % PD
+            if (ke >= k1) break;
+            plane_store(ke + 1, (u + 1) % PD);  // also parks E of layer ke in es
             __syncthreads();
-            if (ke + 2 <= k1) plane_load(ke + 2);
-            const T ee_raw = elem_E(ke + 1);
+            if (ke + 1 + PD <= k1) plane_load(ke + 1 + PD, (u + 1) % PD);
+            const T ee = es[ty][tx];
 
             if constexpr (MODE == EL_MFMA) {
                 // ---- element phase on MFMA: each wave multiplies the gathered
@@ -232,8 +237,8 @@ __global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T>
 #pragma unroll
                 for (int c = 0; c < 3; ++c) carry[c] = top[c];
             }
-            ee = ee_raw * elem_ok(ke + 1);
             __syncthreads();
+        }
         }
     }
 
@@ -352,62 +357,58 @@ int matvec_dense_blocks(const GridArgs& g, int kchunk) {
 }
 int matvec_gather_blocks(const GridArgs& g) { return (int)((g.nn + 255) / 256); }
 
-template <typename T>
-void launch_matvec_parity(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, int tile, int kchunk,
-                          hipStream_t s, int nrhs) {
+namespace {
+
+template <typename T, int TX, int TY, int MODE, bool FUSE>
+void launch_zmarch_pd(const GridArgs& g, const ParityK<T>& K, DenseK<T> Kd, const MvArgs<T>& a, int kchunk, int pd,
+                      int nb, int ntx, int nty, int nt, hipStream_t s, int nrhs) {
+    const dim3 grid(nb, nrhs), blk(TX * TY);
+    if (pd <= 1)
+        hipLaunchKernelGGL((k_matvec_zmarch<T, TX, TY, MODE, FUSE, 1>), grid, blk, 0, s, g, K, Kd, a, kchunk, ntx,
+                           nty, nt);
+    else if (pd == 2)
+        hipLaunchKernelGGL((k_matvec_zmarch<T, TX, TY, MODE, FUSE, 2>), grid, blk, 0, s, g, K, Kd, a, kchunk, ntx,
+                           nty, nt);
+    else
+        hipLaunchKernelGGL((k_matvec_zmarch<T, TX, TY, MODE, FUSE, 3>), grid, blk, 0, s, g, K, Kd, a, kchunk, ntx,
+                           nty, nt);
+}
+
+template <typename T, int TX, int TY, int MODE>
+int launch_zmarch(const GridArgs& g, const ParityK<T>& K, DenseK<T> Kd, const MvArgs<T>& a, const MvShape& sh,
+                  hipStream_t s, int nrhs) {
     int ntx, nty;
-    DenseK<T> none{nullptr};
-    int nb;
-    const bool fuse = a.p_out != nullptr;
-    if (tile == 1) {
-        const int nt = zmarch_tiles<32, 8>(g, kchunk, &ntx, &nty);
-        nb = round8(nt);
-        if (fuse)
-            hipLaunchKernelGGL((k_matvec_zmarch<T, 32, 8, EL_PARITY, true>), dim3(nb, nrhs), dim3(256), 0, s, g, K,
-                               none, a, kchunk, ntx, nty, nt);
-        else
-            hipLaunchKernelGGL((k_matvec_zmarch<T, 32, 8, EL_PARITY, false>), dim3(nb, nrhs), dim3(256), 0, s, g, K,
-                               none, a, kchunk, ntx, nty, nt);
-    } else {
-        const int nt = zmarch_tiles<16, 16>(g, kchunk, &ntx, &nty);
-        nb = round8(nt);
-        if (fuse)
-            hipLaunchKernelGGL((k_matvec_zmarch<T, 16, 16, EL_PARITY, true>), dim3(nb, nrhs), dim3(256), 0, s, g, K,
-                               none, a, kchunk, ntx, nty, nt);
-        else
-            hipLaunchKernelGGL((k_matvec_zmarch<T, 16, 16, EL_PARITY, false>), dim3(nb, nrhs), dim3(256), 0, s, g, K,
-                               none, a, kchunk, ntx, nty, nt);
-    }
+    const int nt = zmarch_tiles<TX, TY>(g, sh.kchunk, &ntx, &nty);
+    const int nb = round8(nt);
+    if (a.p_out != nullptr)
+        launch_zmarch_pd<T, TX, TY, MODE, true>(g, K, Kd, a, sh.kchunk, sh.pdepth, nb, ntx, nty, nt, s, nrhs);
+    else
+        launch_zmarch_pd<T, TX, TY, MODE, false>(g, K, Kd, a, sh.kchunk, sh.pdepth, nb, ntx, nty, nt, s, nrhs);
+    return nb;
+}
+
+}  // namespace
+
+template <typename T>
+void launch_matvec_parity(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, const MvShape& sh,
+                          hipStream_t s, int nrhs) {
+    const DenseK<T> none{nullptr};
+    const int nb = sh.tile == 1 ? launch_zmarch<T, 32, 8, EL_PARITY>(g, K, none, a, sh, s, nrhs)
+                                : launch_zmarch<T, 16, 16, EL_PARITY>(g, K, none, a, sh, s, nrhs);
     if (a.mode == MV_PCG) launch_fin_alpha(a.st, a.partials, nb, nrhs, s);
 }
 
 template <typename T>
-void launch_matvec_dense(const GridArgs& g, const T* Kbin, const MvArgs<T>& a, int kchunk, hipStream_t s,
+void launch_matvec_dense(const GridArgs& g, const T* Kbin, const MvArgs<T>& a, const MvShape& sh, hipStream_t s,
                          int nrhs) {
-    int ntx, nty;
-    ParityK<T> none{};
-    const int nt = zmarch_tiles<16, 16>(g, kchunk, &ntx, &nty);
-    if (a.p_out != nullptr)
-        hipLaunchKernelGGL((k_matvec_zmarch<T, 16, 16, EL_DENSE, true>), dim3(round8(nt), nrhs), dim3(256), 0, s, g,
-                           none, DenseK<T>{Kbin}, a, kchunk, ntx, nty, nt);
-    else
-        hipLaunchKernelGGL((k_matvec_zmarch<T, 16, 16, EL_DENSE, false>), dim3(round8(nt), nrhs), dim3(256), 0, s, g,
-                           none, DenseK<T>{Kbin}, a, kchunk, ntx, nty, nt);
-    if (a.mode == MV_PCG) launch_fin_alpha(a.st, a.partials, round8(nt), nrhs, s);
+    const int nb = launch_zmarch<T, 16, 16, EL_DENSE>(g, ParityK<T>{}, DenseK<T>{Kbin}, a, sh, s, nrhs);
+    if (a.mode == MV_PCG) launch_fin_alpha(a.st, a.partials, nb, nrhs, s);
 }
 
-void launch_matvec_mfma(const GridArgs& g, const float* Kbin, const MvArgs<float>& a, int kchunk, hipStream_t s,
-                        int nrhs) {
-    int ntx, nty;
-    ParityK<float> none{};
-    const int nt = zmarch_tiles<16, 16>(g, kchunk, &ntx, &nty);
-    if (a.p_out != nullptr)
-        hipLaunchKernelGGL((k_matvec_zmarch<float, 16, 16, EL_MFMA, true>), dim3(round8(nt), nrhs), dim3(256), 0, s,
-                           g, none, DenseK<float>{Kbin}, a, kchunk, ntx, nty, nt);
-    else
-        hipLaunchKernelGGL((k_matvec_zmarch<float, 16, 16, EL_MFMA, false>), dim3(round8(nt), nrhs), dim3(256), 0,
-                           s, g, none, DenseK<float>{Kbin}, a, kchunk, ntx, nty, nt);
-    if (a.mode == MV_PCG) launch_fin_alpha(a.st, a.partials, round8(nt), nrhs, s);
+void launch_matvec_mfma(const GridArgs& g, const float* Kbin, const MvArgs<float>& a, const MvShape& sh,
+                        hipStream_t s, int nrhs) {
+    const int nb = launch_zmarch<float, 16, 16, EL_MFMA>(g, ParityK<float>{}, DenseK<float>{Kbin}, a, sh, s, nrhs);
+    if (a.mode == MV_PCG) launch_fin_alpha(a.st, a.partials, nb, nrhs, s);
 }
 
 void launch_matvec_gather(const GridArgs& g, const double* Kbin, const MvArgs<double>& a, hipStream_t s,
@@ -428,13 +429,14 @@ void launch_elastic_diag(const GridArgs& g, const double kd[3], const double* E,
                        E, diag);
 }
 
-template void launch_matvec_parity<double>(const GridArgs&, const ParityK<double>&, const MvArgs<double>&, int, int,
+template void launch_matvec_parity<double>(const GridArgs&, const ParityK<double>&, const MvArgs<double>&, const MvShape&,
                                            hipStream_t, int);
-template void launch_matvec_parity<float>(const GridArgs&, const ParityK<float>&, const MvArgs<float>&, int, int,
+template void launch_matvec_parity<float>(const GridArgs&, const ParityK<float>&, const MvArgs<float>&, const MvShape&,
                                           hipStream_t, int);
-template void launch_matvec_dense<double>(const GridArgs&, const double*, const MvArgs<double>&, int, hipStream_t,
+template void launch_matvec_dense<double>(const GridArgs&, const double*, const MvArgs<double>&, const MvShape&, hipStream_t,
                                           int);
-template void launch_matvec_dense<float>(const GridArgs&, const float*, const MvArgs<float>&, int, hipStream_t, int);
+template void launch_matvec_dense<float>(const GridArgs&, const float*, const MvArgs<float>&, const MvShape&, hipStream_t,
+                                         int);
 template void launch_elastic_minv<double>(const GridArgs&, const double*, const double*, const uint8_t*, double,
                                           double*, hipStream_t);
 template void launch_elastic_minv<float>(const GridArgs&, const double*, const float*, const uint8_t*, double,

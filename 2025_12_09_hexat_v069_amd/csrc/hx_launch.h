@@ -8,6 +8,15 @@ namespace hx {
 // ---- elasticity operator -------------------------------------------------
 enum MvMode { MV_PLAIN = 0, MV_PCG = 1 };
 
+// Launch shape of the z-marching operator: tile 0 = 16x16 elements, 1 = 32x8;
+// kchunk z planes per workgroup; pdepth node planes kept in flight per
+// workgroup (register prefetch ring, 1..3).
+struct MvShape {
+    int tile = 0;
+    int kchunk = 16;
+    int pdepth = 2;
+};
+
 template <typename T>
 struct MvArgs {
     const T* x;        // input vector (3*nn)
@@ -27,17 +36,17 @@ struct MvArgs {
 
 // tile: 0 -> 16x16 element columns (default), 1 -> 32x8
 template <typename T>
-void launch_matvec_parity(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, int tile, int kchunk,
+void launch_matvec_parity(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, const MvShape& sh,
                           hipStream_t s, int nrhs = 1);
 template <typename T>
-void launch_matvec_dense(const GridArgs& g, const T* Kbin, const MvArgs<T>& a, int kchunk, hipStream_t s,
+void launch_matvec_dense(const GridArgs& g, const T* Kbin, const MvArgs<T>& a, const MvShape& sh, hipStream_t s,
                          int nrhs = 1);
 void launch_matvec_gather(const GridArgs& g, const double* Kbin, const MvArgs<double>& a, hipStream_t s,
                           int nrhs = 1);
 // FP32 element contraction on MFMA: dense [16 elem x 24] x Ke^T tiles with
 // v_mfma_f32_16x16x4f32, inside the same z-marching tile
-void launch_matvec_mfma(const GridArgs& g, const float* Kbin, const MvArgs<float>& a, int kchunk, hipStream_t s,
-                        int nrhs = 1);
+void launch_matvec_mfma(const GridArgs& g, const float* Kbin, const MvArgs<float>& a, const MvShape& sh,
+                        hipStream_t s, int nrhs = 1);
 int matvec_parity_blocks(const GridArgs& g, int tile, int kchunk);
 int matvec_dense_blocks(const GridArgs& g, int kchunk);
 int matvec_gather_blocks(const GridArgs& g);

@@ -132,7 +132,20 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-pcg-iters", type=int, default=20)
     ap.add_argument("--cpu-filter-iters", type=int, default=20)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (4 launches), the reference kernel for scaling the "
+                         "FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 1 << 30
+        a = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(4):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read_per_launch": n, "copy_bytes_written_per_launch": n, "launches": 4}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus N>1 is not supported (the path is designed "
                                    "to shard by x-slabs with halo exchange, see DESIGN.md)"}))

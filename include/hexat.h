@@ -204,6 +204,8 @@ typedef struct {
     int variant32;    /* chosen FP32 contraction (HX_MV_*) */
     double ms64, ms32;                       /* best times of one operator application */
     double ms32_parity, ms32_dense, ms32_mfma; /* FP32 contraction candidates */
+    int pdepth;       /* node planes in flight per workgroup (FP64 operator) */
+    int tile32, kchunk32, pdepth32; /* launch shape of the FP32 operator */
 } hx_tune_result;
 /* Time the operator's launch shapes (tile, z-chunk) and element contractions on
  * the current grid for nrhs columns and keep the fastest (HX_MV_AUTO then uses
