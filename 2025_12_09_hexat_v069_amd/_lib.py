@@ -90,7 +90,7 @@ class TOStats(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("tile", C.c_int), ("kchunk", C.c_int), ("variant64", C.c_int), ("variant32", C.c_int),
                 ("ms64", C.c_double), ("ms32", C.c_double), ("ms32_parity", C.c_double), ("ms32_dense", C.c_double),
-                ("ms32_mfma", C.c_double), ("pdepth", C.c_int), ("tile32", C.c_int), ("kchunk32", C.c_int),
+                ("ms32_mfma", C.c_double), ("tile32", C.c_int), ("kchunk32", C.c_int), ("pdepth", C.c_int),
                 ("pdepth32", C.c_int)]
 
 

@@ -752,6 +752,8 @@ int hx_grid_set(hx_ctx* c, int nx, int ny, int nz, double dx, double dy, double 
                     Kp[3 * s + ci][3 * t + di] = acc / 64.0;
                     kmax = std::max(kmax, std::fabs(acc / 64.0));
                 }
+    for (int i = 0; i < 24; ++i)  // exact symmetry, so mirrored entries may share one value
+        for (int j = 0; j < i; ++j) Kp[i][j] = Kp[j][i] = 0.5 * (Kp[i][j] + Kp[j][i]);
     bool inpat[24][24] = {};
     for (int t = 0; t < kParityNnz; ++t) {
         inpat[kParityPattern[t].r][kParityPattern[t].c] = true;
@@ -1318,7 +1320,7 @@ int hx_test_unet_backward(hx_ctx* c, const float* gy, float* grad) {
 // Time every launch shape of the operator as the PCG runs it (fused
 // p-update, p'q partials, alpha finaliser) on the current grid with uniform
 // moduli and a non-smooth vector, and keep the fastest per precision:
-// FP64 parity {16x16, 32x8 tiles} x {8, 16, 32 z planes} x {1, 2, 3 planes in
+// FP64 parity {16x16, 16x32 tiles} x {8, 16, 32 z planes} x {1, 2 planes in
 // flight} and dense; FP32 the same parity shapes, dense and MFMA.
 int hx_autotune(hx_ctx* c, int nrhs, hx_tune_result* out) {
     if (!c || nrhs < 1 || nrhs > kMaxRhs) return HX_ERR_BAD_ARG;
@@ -1377,7 +1379,7 @@ int hx_autotune(hx_ctx* c, int nrhs, hx_tune_result* out) {
     auto run32 = [&](int v) {
         return time_it([&] { mv32(c, fx.p, fy.p, c->E32.p, 0.0, MV_PCG, v, nrhs, fr.p, fm.p, fp.p); });
     };
-    const int tiles[] = {0, 1}, chunks[] = {8, 16, 32}, depths[] = {1, 2, 3};
+    const int tiles[] = {0, 1}, chunks[] = {8, 16, 32}, depths[] = {1, 2};
     double best64 = 1e30, best32 = 1e30;
     MvShape b64 = c->shape64, b32 = c->shape32;
     int bv64 = HX_MV_PARITY, bv32 = HX_MV_PARITY;
@@ -1422,9 +1424,9 @@ int hx_autotune(hx_ctx* c, int nrhs, hx_tune_result* out) {
         out->ms32_parity = t32[0];
         out->ms32_dense = t32[1];
         out->ms32_mfma = t32[2];
-        out->pdepth = c->shape64.pdepth;
         out->tile32 = c->shape32.tile;
         out->kchunk32 = c->shape32.kchunk;
+        out->pdepth = c->shape64.pdepth;
         out->pdepth32 = c->shape32.pdepth;
     }
     return HX_OK;

@@ -22,7 +22,9 @@ struct DenseK {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-// Element-operator modes of the z-marching kernel
+// Element-operator modes of the element-local z-marching kernel (the parity
+// mode there is the reference-shaped form kept for comparison; production
+// parity runs in k_matvec_sf below)
 enum { EL_PARITY = 0, EL_DENSE = 1, EL_MFMA = 2 };
 
 template <typename T, int TX, int TY, int MODE, bool FUSE, int PD>
@@ -248,6 +250,260 @@ __global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T>
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Sum-factorised z-marching operator in the parity basis (the production
+// contraction).  The butterfly that takes an element's 8 node values to its
+// parity modes factorises over x, y and z.  Its x and y stages act on one
+// node plane only: for the face (xy square of an element column) in plane k,
+//   F_k[s], s = sx + 2 sy  (4 modes x 3 components),
+// is computed once per plane and shared by the element layers below and
+// above; the element's modes are xh[s] = F_lo[s] + F_hi[s], xh[s+4] =
+// F_hi[s] - F_lo[s].  The inverse runs backwards: z-inverse per element, the
+// two elements sharing a face add their face modes (a register carry along
+// z, the owner-computes sum over z), then the y- and x-inverse move the face
+// totals to the nodes.  A node receives from 4 faces: one LDS exchange with
+// the face below in y and one DPP lane shift (row_shr:1) from the face to
+// the left in x -- the 16-element tile row is one 16-lane DPP row.
+// Per element and layer: ~180 flops, 6 LDS reads, 6+6 LDS exchange words and
+// 2 barriers, against 234 flops, 27 reads, 24+24 exchange words and 3
+// barriers for the element-local form.
+__device__ __forceinline__ float lane_from_left(float v) {  // lane l <- lane l-1 within 16-lane rows
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, true));
+}
+__device__ __forceinline__ double lane_from_left(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x111, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x111, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+
+template <typename T, int TY, bool FUSE, int PD>
+__global__ __launch_bounds__(16 * TY) void k_matvec_sf(GridArgs g, ParityK<T> K, MvArgs<T> a, int kchunk, int ntx,
+                                                       int nty, int ntiles) {
+    constexpr int TX = 16;  // one tile row = one DPP row
+    constexpr int NT = TX * TY;
+    constexpr int PX = TX + 1, PY = TY + 1;
+    constexpr int NEXTRA = 3 * (PY + TX);  // plane values beyond the thread grid: column TX and row TY
+    static_assert(NEXTRA <= NT, "one extra plane value per thread");
+    static_assert(PD == 1 || PD == 2, "planes in flight");
+    constexpr int OROW = 3 * (TX - 1);       // values of one owned node row
+    constexpr int OVALS = (TY - 1) * OROW;   // owned values of one plane
+    __shared__ T xs[3][PY][PX];  // one node plane (x, or the fused p), component-major
+    __shared__ T ex[6][NT];      // y-exchange: row ty+1 half of each face's y-inverse
+    __shared__ T so[FUSE ? 2 : 1][OVALS];  // results of the owned nodes, node-major, for row-contiguous stores
+    __shared__ double red[NT / 64];
+
+    a.x += (long)blockIdx.y * a.cs;
+    a.y += (long)blockIdx.y * a.cs;
+    if constexpr (FUSE) {
+        a.r += (long)blockIdx.y * a.cs;
+        a.p_out += (long)blockIdx.y * a.cs;
+    }
+    if (a.mode == MV_PCG) {
+        a.st += blockIdx.y;
+        a.partials += (long)blockIdx.y * gridDim.x;
+    }
+    if (a.mode == MV_PCG && a.st->done) return;
+    const T beta = FUSE ? T(a.st->beta) : T(0);
+
+    const unsigned nb = gridDim.x;  // multiple of 8: XCD-contiguous tile ranges
+    const unsigned tile = (blockIdx.x & 7u) * (nb >> 3) + (blockIdx.x >> 3);
+    const bool active = tile < (unsigned)ntiles;
+    const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
+    double pq = 0.0;
+
+    if (active) {
+        const int cx = tile % ntx, cy = (tile / ntx) % nty, cz = tile / (ntx * nty);
+        const int i0 = cx * (TX - 1), j0 = cy * (TY - 1);
+        const int k0 = cz * kchunk, k1 = min(k0 + kchunk, g.NZ);
+        // thread (tx,ty): face / element column (gi, gj), node (gi, gj) owned if tx,ty >= 1
+        const int gi = i0 - 1 + tx, gj = j0 - 1 + ty;
+        const bool own = tx >= 1 && ty >= 1 && gi < g.NX && gj < g.NY;
+        const bool ecol = gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny;
+        const long pstride = 3L * g.NX * g.NY;
+        const int off_own = 3 * (min(max(gj, 0), g.NY - 1) * g.NX + min(max(gi, 0), g.NX - 1));
+        const int eoff = min(max(gj, 0), g.ny - 1) * g.nx + min(max(gi, 0), g.nx - 1);
+        // extra value of thread t < NEXTRA: component t % 3 of node t / 3 of
+        // (TX, 0..TY) then (0..TX-1, TY); the other threads load a clamped copy
+        const int xe = min((int)threadIdx.x, NEXTRA - 1), xn = xe / 3, xc = xe - 3 * xn;
+        const int xtx = xn < PY ? TX : xn - PY, xty = xn < PY ? xn : TY;
+        const bool xstore = (int)threadIdx.x < NEXTRA;
+        const int off_x =
+            3 * (min(max(j0 - 1 + xty, 0), g.NY - 1) * g.NX + min(max(i0 - 1 + xtx, 0), g.NX - 1)) + xc;
+
+        // PD register sets of a prefetched plane: x (and r, D^-1 when fused)
+        // at this thread's node and extra value, plus E of element layer
+        // k-1.  Addresses are clamped, so every load is unconditional (a load
+        // under a branch makes the compiler's wait counts conservative);
+        // out-of-range node values only ever meet E = 0.
+        struct Slot {
+            T x[3], r[FUSE ? 3 : 1], m[FUSE ? 3 : 1], qx, qr, qm, e;
+            int k;
+        };
+        Slot sl[PD];
+        auto plane_load = [&](Slot& q, int k) {
+            q.k = k;
+            const long po = (long)min(max(k, 0), g.NZ - 1) * pstride;
+            const T* X = a.x + po;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) q.x[c] = X[off_own + c];
+            q.qx = X[off_x];
+            if constexpr (FUSE) {
+                const T* R = a.r + po;
+                const T* M = a.minv + po;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    q.r[c] = R[off_own + c];
+                    q.m[c] = M[off_own + c];
+                }
+                q.qr = R[off_x];
+                q.qm = M[off_x];
+            }
+            q.e = a.E[(long)min(max(k - 1, 0), g.nz - 1) * g.nx * g.ny + eoff];
+        };
+        // park a prefetched plane in LDS; sets this thread's node value and
+        // the modulus of its element in layer q.k - 1
+        T xnew[3], ee;
+        auto plane_store = [&](const Slot& q) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                T v;
+                if constexpr (FUSE) v = q.m[c] * q.r[c] + beta * q.x[c];  // p = D^-1 r + beta p
+                else v = q.x[c];
+                xs[c][ty][tx] = v;
+                xnew[c] = v;
+            }
+            T v;
+            if constexpr (FUSE) v = q.qm * q.qr + beta * q.qx;
+            else v = q.qx;
+            if (xstore) xs[xc][xty][xtx] = v;
+            ee = (ecol && q.k - 1 >= 0 && q.k - 1 < g.nz) ? q.e : T(0);
+            asm volatile("" : "+v"(ee));  // settle ee here, before the next plane is requested
+        };
+        auto face_modes = [&](T* F) {  // F[3 s + c]
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T x00 = xs[c][ty][tx], x10 = xs[c][ty][tx + 1];
+                const T x01 = xs[c][ty + 1][tx], x11 = xs[c][ty + 1][tx + 1];
+                const T A0 = x00 + x10, B0 = x10 - x00, A1 = x01 + x11, B1 = x11 - x01;
+                F[c] = A0 + A1;
+                F[3 + c] = B0 + B1;
+                F[6 + c] = A1 - A0;
+                F[9 + c] = B1 - B0;
+            }
+        };
+
+        T F0[12], F1[12], carry[12], xo[3];
+        // Node results of plane ko are parked in LDS (so) at the end of a
+        // layer and written out after the next layer's first barrier, ahead of
+        // its prefetch, as row-contiguous runs of 3*(TX-1) values: a lane
+        // writes consecutive words instead of one component of one node.
+        int ko = -1;
+        auto flush = [&]() {
+            if (ko < k0) return;
+            T* Y = a.y + (long)ko * pstride;
+            T* P = FUSE ? a.p_out + (long)ko * pstride : nullptr;
+#pragma unroll
+            for (int m = 0; m < (OVALS + NT - 1) / NT; ++m) {
+                const int idx = threadIdx.x + m * NT;
+                const int row = idx / OROW, col = idx - row * OROW;
+                if (idx < OVALS && i0 + col / 3 < g.NX && j0 + row < g.NY) {
+                    const int o = 3 * ((j0 + row) * g.NX + i0) + col;
+                    Y[o] = so[0][idx];
+                    if constexpr (FUSE) P[o] = so[FUSE ? 1 : 0][idx];
+                }
+            }
+        };
+        // element layer ke: Flo holds the face modes of plane ke, Fhi receives
+        // plane ke+1's, which sits in slot q; plane ke+1+PD is requested into q
+        auto step = [&](int ke, const T* Flo, T* Fhi, Slot& q) {
+            plane_store(q);  // plane ke+1, E of layer ke
+            __syncthreads();
+            flush();
+            asm volatile("" ::: "memory");
+            plane_load(q, min(ke + 1 + PD, k1));  // unconditional (past k1: re-reads plane k1, a cache hit)
+            asm volatile("" ::: "memory");  // keep the prefetch ahead of this layer's arithmetic
+            face_modes(Fhi);
+
+            // element (gi, gj, ke): modes, contraction (E folded into xh), z-inverse
+            T xh[24];
+#pragma unroll
+            for (int i = 0; i < 12; ++i) {
+                xh[i] = ee * (Flo[i] + Fhi[i]);
+                xh[12 + i] = ee * (Fhi[i] - Flo[i]);
+            }
+            T yh[24];
+#pragma unroll
+            for (int i = 0; i < 24; ++i) yh[i] = T(0);
+#pragma unroll
+            for (int n = 0; n < kParityNnz; ++n)
+                yh[kParityPattern[n].r] += K.v[kParitySym.v[n]] * xh[kParityPattern[n].c];
+            // face totals of plane ke: top half of layer ke-1 (carry) + bottom half of layer ke
+            T H[12];
+#pragma unroll
+            for (int i = 0; i < 12; ++i) {
+                H[i] = carry[i] + (yh[i] - yh[12 + i]);
+                carry[i] = yh[i] + yh[12 + i];
+            }
+            // y-inverse: row ty (A0,B0) stays, row ty+1 (A1,B1) goes to the face above in y
+            T eA[3], eB[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                eA[c] = H[c] - H[6 + c];
+                eB[c] = H[3 + c] - H[9 + c];
+                ex[c][threadIdx.x] = H[c] + H[6 + c];
+                ex[3 + c][threadIdx.x] = H[3 + c] + H[9 + c];
+            }
+            __syncthreads();
+            const int src = ty >= 1 ? threadIdx.x - TX : threadIdx.x;
+            T node[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T A = eA[c] + ex[c][src], B = eB[c] + ex[3 + c][src];
+                node[c] = (A - B) + lane_from_left(A + B);  // x-inverse: own half + left edge's half
+            }
+            ko = ke;
+            const int so_i = (ty - 1) * OROW + 3 * (tx - 1);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T yv = node[c] + a.shift * xo[c];
+                if (tx >= 1 && ty >= 1) {
+                    so[0][so_i + c] = yv;
+                    if constexpr (FUSE) so[FUSE ? 1 : 0][so_i + c] = xo[c];
+                }
+                if (ke >= k0 && own) pq += (double)xo[c] * (double)yv;
+                xo[c] = xnew[c];
+            }
+        };
+
+        plane_load(sl[0], k0 - 1);
+        plane_store(sl[0]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xo[c] = xnew[c];
+        __syncthreads();
+        face_modes(F0);
+        // plane k0+j sits in slot j % PD
+#pragma unroll
+        for (int j = 0; j < PD; ++j) plane_load(sl[j], min(k0 + j, k1));
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 12; ++i) carry[i] = T(0);
+        // two layers per trip (face-mode ping-pong); with PD = 2 the slot
+        // index then follows the layer parity as well
+        for (int ke = k0 - 1; ke < k1; ke += 2) {
+            step(ke, F0, F1, sl[0]);
+            if (ke + 1 < k1) step(ke + 1, F1, F0, sl[PD - 1]);
+        }
+        __syncthreads();
+        flush();
+    }
+
+    if (a.mode == MV_PCG) {  // per-block p'q; k_fin_alpha reduces them
+        const double v = block_sum<NT>(pq, red);
+        if (threadIdx.x == 0) a.partials[blockIdx.x] = v;
+    }
+}
+
 // Reference-simple variant: one thread per node gathers its 8 elements with
 // the dense Ke (x from global memory / cache).  Used to cross-check.
 __global__ __launch_bounds__(256) void k_matvec_gather(GridArgs g, const double* __restrict__ Kb,
@@ -349,7 +605,7 @@ int zmarch_tiles(const GridArgs& g, int kchunk, int* ntx, int* nty) {
 
 int matvec_parity_blocks(const GridArgs& g, int tile, int kchunk) {
     int a, b;
-    return round8(tile == 1 ? zmarch_tiles<32, 8>(g, kchunk, &a, &b) : zmarch_tiles<16, 16>(g, kchunk, &a, &b));
+    return round8(tile == 1 ? zmarch_tiles<16, 32>(g, kchunk, &a, &b) : zmarch_tiles<16, 16>(g, kchunk, &a, &b));
 }
 int matvec_dense_blocks(const GridArgs& g, int kchunk) {
     int a, b;
@@ -359,31 +615,33 @@ int matvec_gather_blocks(const GridArgs& g) { return (int)((g.nn + 255) / 256); 
 
 namespace {
 
-template <typename T, int TX, int TY, int MODE, bool FUSE>
-void launch_zmarch_pd(const GridArgs& g, const ParityK<T>& K, DenseK<T> Kd, const MvArgs<T>& a, int kchunk, int pd,
-                      int nb, int ntx, int nty, int nt, hipStream_t s, int nrhs) {
-    const dim3 grid(nb, nrhs), blk(TX * TY);
-    if (pd <= 1)
-        hipLaunchKernelGGL((k_matvec_zmarch<T, TX, TY, MODE, FUSE, 1>), grid, blk, 0, s, g, K, Kd, a, kchunk, ntx,
-                           nty, nt);
-    else if (pd == 2)
-        hipLaunchKernelGGL((k_matvec_zmarch<T, TX, TY, MODE, FUSE, 2>), grid, blk, 0, s, g, K, Kd, a, kchunk, ntx,
-                           nty, nt);
-    else
-        hipLaunchKernelGGL((k_matvec_zmarch<T, TX, TY, MODE, FUSE, 3>), grid, blk, 0, s, g, K, Kd, a, kchunk, ntx,
-                           nty, nt);
-}
-
 template <typename T, int TX, int TY, int MODE>
-int launch_zmarch(const GridArgs& g, const ParityK<T>& K, DenseK<T> Kd, const MvArgs<T>& a, const MvShape& sh,
-                  hipStream_t s, int nrhs) {
+int launch_zmarch(const GridArgs& g, DenseK<T> Kd, const MvArgs<T>& a, const MvShape& sh, hipStream_t s, int nrhs) {
     int ntx, nty;
     const int nt = zmarch_tiles<TX, TY>(g, sh.kchunk, &ntx, &nty);
     const int nb = round8(nt);
+    const ParityK<T> none{};
     if (a.p_out != nullptr)
-        launch_zmarch_pd<T, TX, TY, MODE, true>(g, K, Kd, a, sh.kchunk, sh.pdepth, nb, ntx, nty, nt, s, nrhs);
+        hipLaunchKernelGGL((k_matvec_zmarch<T, TX, TY, MODE, true, 1>), dim3(nb, nrhs), dim3(TX * TY), 0, s, g, none,
+                           Kd, a, sh.kchunk, ntx, nty, nt);
     else
-        launch_zmarch_pd<T, TX, TY, MODE, false>(g, K, Kd, a, sh.kchunk, sh.pdepth, nb, ntx, nty, nt, s, nrhs);
+        hipLaunchKernelGGL((k_matvec_zmarch<T, TX, TY, MODE, false, 1>), dim3(nb, nrhs), dim3(TX * TY), 0, s, g, none,
+                           Kd, a, sh.kchunk, ntx, nty, nt);
+    return nb;
+}
+
+template <typename T, int TY, int PD>
+int launch_sf(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, const MvShape& sh, hipStream_t s,
+              int nrhs) {
+    int ntx, nty;
+    const int nt = zmarch_tiles<16, TY>(g, sh.kchunk, &ntx, &nty);
+    const int nb = round8(nt);
+    if (a.p_out != nullptr)
+        hipLaunchKernelGGL((k_matvec_sf<T, TY, true, PD>), dim3(nb, nrhs), dim3(16 * TY), 0, s, g, K, a, sh.kchunk,
+                           ntx, nty, nt);
+    else
+        hipLaunchKernelGGL((k_matvec_sf<T, TY, false, PD>), dim3(nb, nrhs), dim3(16 * TY), 0, s, g, K, a, sh.kchunk,
+                           ntx, nty, nt);
     return nb;
 }
 
@@ -392,22 +650,24 @@ int launch_zmarch(const GridArgs& g, const ParityK<T>& K, DenseK<T> Kd, const Mv
 template <typename T>
 void launch_matvec_parity(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, const MvShape& sh,
                           hipStream_t s, int nrhs) {
-    const DenseK<T> none{nullptr};
-    const int nb = sh.tile == 1 ? launch_zmarch<T, 32, 8, EL_PARITY>(g, K, none, a, sh, s, nrhs)
-                                : launch_zmarch<T, 16, 16, EL_PARITY>(g, K, none, a, sh, s, nrhs);
+    int nb;
+    if (sh.pdepth == 2)
+        nb = sh.tile == 1 ? launch_sf<T, 32, 2>(g, K, a, sh, s, nrhs) : launch_sf<T, 16, 2>(g, K, a, sh, s, nrhs);
+    else
+        nb = sh.tile == 1 ? launch_sf<T, 32, 1>(g, K, a, sh, s, nrhs) : launch_sf<T, 16, 1>(g, K, a, sh, s, nrhs);
     if (a.mode == MV_PCG) launch_fin_alpha(a.st, a.partials, nb, nrhs, s);
 }
 
 template <typename T>
 void launch_matvec_dense(const GridArgs& g, const T* Kbin, const MvArgs<T>& a, const MvShape& sh, hipStream_t s,
                          int nrhs) {
-    const int nb = launch_zmarch<T, 16, 16, EL_DENSE>(g, ParityK<T>{}, DenseK<T>{Kbin}, a, sh, s, nrhs);
+    const int nb = launch_zmarch<T, 16, 16, EL_DENSE>(g, DenseK<T>{Kbin}, a, sh, s, nrhs);
     if (a.mode == MV_PCG) launch_fin_alpha(a.st, a.partials, nb, nrhs, s);
 }
 
 void launch_matvec_mfma(const GridArgs& g, const float* Kbin, const MvArgs<float>& a, const MvShape& sh,
                         hipStream_t s, int nrhs) {
-    const int nb = launch_zmarch<float, 16, 16, EL_MFMA>(g, ParityK<float>{}, DenseK<float>{Kbin}, a, sh, s, nrhs);
+    const int nb = launch_zmarch<float, 16, 16, EL_MFMA>(g, DenseK<float>{Kbin}, a, sh, s, nrhs);
     if (a.mode == MV_PCG) launch_fin_alpha(a.st, a.partials, nb, nrhs, s);
 }
 

@@ -45,6 +45,24 @@ constexpr RC kParityPattern[kParityNnz] = {
     // trilinear mode s = 7
     {21, 21}, {22, 22}, {23, 23}};
 
+// Kp is symmetric: entry n and the entry at (c, r) hold the same value.  The
+// lower of the two indices, so kernels touch 33 distinct values, not 45.
+constexpr int parity_sym(int n) {
+    for (int m = 0; m < n; ++m)
+        if (kParityPattern[m].r == kParityPattern[n].c && kParityPattern[m].c == kParityPattern[n].r) return m;
+    return n;
+}
+
+struct ParitySymTable {
+    int v[kParityNnz];
+};
+constexpr ParitySymTable make_parity_sym() {
+    ParitySymTable t{};
+    for (int n = 0; n < kParityNnz; ++n) t.v[n] = parity_sym(n);
+    return t;
+}
+constexpr ParitySymTable kParitySym = make_parity_sym();  // compile-time table
+
 template <typename T>
 struct ParityK {
     T v[kParityNnz];

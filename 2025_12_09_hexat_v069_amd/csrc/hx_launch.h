@@ -8,9 +8,10 @@ namespace hx {
 // ---- elasticity operator -------------------------------------------------
 enum MvMode { MV_PLAIN = 0, MV_PCG = 1 };
 
-// Launch shape of the z-marching operator: tile 0 = 16x16 elements, 1 = 32x8;
-// kchunk z planes per workgroup; pdepth node planes kept in flight per
-// workgroup (register prefetch ring, 1..3).
+// Launch shape of the z-marching operator: tile 0 = 16x16 element columns per
+// workgroup (256 threads), 1 = 16x32 (512 threads, parity contraction only);
+// kchunk z planes per workgroup; pdepth node planes in flight per workgroup
+// (parity contraction: 1 or 2).
 struct MvShape {
     int tile = 0;
     int kchunk = 16;

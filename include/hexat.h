@@ -198,14 +198,14 @@ int hx_learned_weights(hx_ctx* ctx, uint16_t* out, int* count);
 
 /* ---- autotuner ---------------------------------------------------------------- */
 typedef struct {
-    int tile;         /* 0: 16x16 element tile, 1: 32x8 */
+    int tile;         /* 0: 16x16 element columns per workgroup, 1: 16x32 */
     int kchunk;       /* z planes per workgroup */
     int variant64;    /* chosen FP64 contraction (HX_MV_*) */
     int variant32;    /* chosen FP32 contraction (HX_MV_*) */
     double ms64, ms32;                       /* best times of one operator application */
     double ms32_parity, ms32_dense, ms32_mfma; /* FP32 contraction candidates */
-    int pdepth;       /* node planes in flight per workgroup (FP64 operator) */
-    int tile32, kchunk32, pdepth32; /* launch shape of the FP32 operator */
+    int tile32, kchunk32; /* launch shape of the FP32 operator (tile, kchunk: FP64) */
+    int pdepth, pdepth32; /* node planes in flight per workgroup, FP64 / FP32 operator */
 } hx_tune_result;
 /* Time the operator's launch shapes (tile, z-chunk) and element contractions on
  * the current grid for nrhs columns and keep the fastest (HX_MV_AUTO then uses
