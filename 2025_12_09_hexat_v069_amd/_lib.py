@@ -43,10 +43,11 @@ EXPORTS = [
     "hx_set_fixed", "hx_element_stiffness", "hx_scalar_matrices", "hx_matvec", "hx_diagonal", "hx_pcg_solve",
     "hx_block_pcg_solve", "hx_helmholtz_filter", "hx_element_stress", "hx_compliance_sens", "hx_oc_update",
     "hx_to_init", "hx_to_step", "hx_to_read", "hx_learned_train", "hx_learned_weights", "hx_timing_enable",
-    "hx_timing_read", "hx_device_info", "hx_autotune",
+    "hx_timing_read", "hx_device_info", "hx_autotune", "hx_tune_set", "hx_time_operator",
 ]
 #: test hooks declared in include/hexat_testing.h
-TEST_EXPORTS = ["hx_test_unet_nparam", "hx_test_unet_init", "hx_test_unet_forward", "hx_test_unet_backward"]
+TEST_EXPORTS = ["hx_test_unet_nparam", "hx_test_unet_init", "hx_test_unet_forward", "hx_test_unet_backward",
+                "hx_test_copy"]
 
 
 class HexatError(RuntimeError):
@@ -143,6 +144,9 @@ def load():
         L.hx_timing_read.argtypes = [vp, i, pd, P(C.c_long)]
         L.hx_device_info.argtypes = [vp, P(C.c_int), P(C.c_int), pd]
         L.hx_autotune.argtypes = [vp, i, P(TuneResult)]
+        L.hx_tune_set.argtypes = [vp, P(TuneResult)]
+        L.hx_time_operator.argtypes = [vp, i, i, i, P(C.c_double)]
+        L.hx_test_copy.argtypes = [vp, C.c_long, i, P(C.c_double)]
         pf = P(C.c_float)
         L.hx_test_unet_nparam.argtypes = [vp, P(C.c_long)]
         L.hx_test_unet_init.argtypes = [vp, i, pf]
@@ -402,6 +406,21 @@ class Context:
         r = TuneResult()
         self.check(self.L.hx_autotune(self.h, int(nrhs), C.byref(r)), "hx_autotune")
         return {k: getattr(r, k) for k, _ in TuneResult._fields_}
+
+    def tune_set(self, tune: dict):
+        r = TuneResult(**{k: tune[k] for k, _ in TuneResult._fields_ if k in tune})
+        self.check(self.L.hx_tune_set(self.h, C.byref(r)), "hx_tune_set")
+
+    def time_operator(self, precision=HX_FP64, nrhs=1, reps=20) -> float:
+        ms = C.c_double(0)
+        self.check(self.L.hx_time_operator(self.h, int(precision), int(nrhs), int(reps), C.byref(ms)),
+                   "hx_time_operator")
+        return ms.value
+
+    def test_copy(self, nbytes: int, reps: int = 4) -> float:
+        ms = C.c_double(0)
+        self.check(self.L.hx_test_copy(self.h, int(nbytes), int(reps), C.byref(ms)), "hx_test_copy")
+        return ms.value
 
     # ---- timing -----------------------------------------------------------------------
     def timing(self, on: bool):

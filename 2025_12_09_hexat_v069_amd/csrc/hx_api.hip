@@ -31,11 +31,14 @@ UNet* unet_create();
 void unet_destroy(UNet* u);
 long unet_nparam(const UNet* u);
 bool unet_ready(const UNet* u);
-bool unet_bind(UNet* u, const GridArgs& g, unsigned seed, hipStream_t s);
+bool unet_bind(UNet* u, const GridArgs& g, unsigned seed, hipStream_t s, bool zero_output = false);
 void unet_apply(UNet* u, const double* r, const double* E, const double* minv, double rms, double* z, hipStream_t s);
-double unet_train_step(UNet* u, const double* r, const double* e, const double* E, const double* minv, float lr,
-                       hipStream_t s);
-double unet_eval(UNet* u, const double* r, const double* e, const double* E, const double* minv, hipStream_t s);
+void unet_forward_sample(UNet* u, const double* r, const double* E, double inv_scale, hipStream_t s);
+void unet_energy_diff(UNet* u, const double* e, const double* r, const double* minv, double inv_scale, double* d,
+                      hipStream_t s);
+void unet_energy_grad(UNet* u, const double* d, const double* Ad, const double* minv, double w, double* loss_acc,
+                      hipStream_t s);
+void unet_backward_step(UNet* u, float lr, hipStream_t s);
 int unet_weights_bf16(UNet* u, uint16_t* out, hipStream_t s);
 void unet_init(UNet* u, unsigned seed, hipStream_t s);
 void unet_params_f32(UNet* u, float* out, hipStream_t s);
@@ -468,12 +471,18 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
 // ------------------------------------------ flexible PCG, learned preconditioner
 // z = D^-1 r + |r|_rms net(r, E); Polak-Ribiere beta.  If it does not reach
 // tol within max_iter (or breaks down), Jacobi-PCG takes over from its iterate.
+// Flexible PCG with the learned preconditioner z = D^-1 r + |r|_rms net(r, E)
+// (GPUSolver.jl's Jacobi-PCG loop with the preconditioner swapped), the
+// Jacobi-PCG fallback from the current iterate when it stagnates or breaks
+// down, and the true residual as the judge of convergence.
+constexpr int kLearnedStall = 40;  // iterations without a 1% drop of the best residual
+
 int pcg_learned(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
     if (!c->unet || !c->unet_trained) return fail(c, HX_ERR_STATE, "learned preconditioner not trained");
     if (!unet_bind(c->unet, c->g, 1234u, c->stream)) return fail(c, HX_ERR_OOM, "U-Net activations");
     const long n = ndof(c);
     HXA(c->l_z, n);
-    HXA(c->l_zold, n);
+    HXA(c->l_zold, n);  // r of the previous iteration
     launch_elastic_minv<double>(c->g, c->kd, c->E.p, c->has_fixed ? c->fixed.p : nullptr, o.shift, c->minv.p,
                                 c->stream);
     launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream, 1);
@@ -485,53 +494,67 @@ int pcg_learned(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
     HXK();
     int rc = sync_state(c, 1);
     if (rc) return rc;
-    int fallbacks = 0;
+    bool stalled = false;
     if (!c->st_host->done) {
         unet_apply(c->unet, c->r.p, c->E.p, c->minv.p, std::sqrt(c->st_host->rr / (double)n), c->l_z.p, c->stream);
         launch_fpcg_beta(n, c->r.p, c->l_z.p, c->l_zold.p, c->st, c->partials.p, 1, c->stream);
-        launch_fpcg_pupdate(n, c->l_z.p, c->p.p, c->l_zold.p, c->st, c->stream);
+        launch_fpcg_pupdate(n, c->l_z.p, c->p.p, c->r.p, c->l_zold.p, c->st, c->stream);
         HXK();
+        double best = c->st_host->rel;
+        int best_it = 0;
         for (;;) {
             mv64(c, c->p.p, c->q.p, c->E.p, o.shift, MV_PCG, HX_MV_AUTO, 1);
             launch_fpcg_update(n, c->x.p, c->p.p, c->r.p, c->q.p, c->minv.p, c->st, c->partials.p, c->stream);
             HXK();
             if ((rc = sync_state(c, 1))) return rc;
             if (c->st_host->done) break;
+            const int it = c->st_host->iter;
+            if (c->st_host->rel < 0.99 * best) best = c->st_host->rel, best_it = it;
+            if (it - best_it >= kLearnedStall) {
+                stalled = true;
+                break;
+            }
             unet_apply(c->unet, c->r.p, c->E.p, c->minv.p, std::sqrt(c->st_host->rr / (double)n), c->l_z.p,
                        c->stream);
             launch_fpcg_beta(n, c->r.p, c->l_z.p, c->l_zold.p, c->st, c->partials.p, 0, c->stream);
-            launch_fpcg_pupdate(n, c->l_z.p, c->p.p, c->l_zold.p, c->st, c->stream);
+            launch_fpcg_pupdate(n, c->l_z.p, c->p.p, c->r.p, c->l_zold.p, c->st, c->stream);
         }
     }
-    const int lit = c->st_host->iter;
-    int status = c->st_host->status;
-    if (status != 0 && status != 10) {  // Jacobi-PCG fallback from the current iterate
+    int lit = c->st_host->iter;
+    const int status = c->st_host->status;
+    if (!stalled && status == 0) {  // the true residual judges convergence
+        std::vector<PcgState> fin(c->st_host, c->st_host + 1);
+        mv64(c, c->x.p, c->q.p, c->E.p, 0.0, MV_PLAIN, HX_MV_AUTO, 1);
+        launch_true_residual(n, c->f.p, c->q.p, c->x.p, c->minv.p, o.shift, nullptr, nullptr, c->st, c->partials.p, 1,
+                             c->stream, 1);
+        HXK();
+        if ((rc = sync_state(c, 1))) return rc;
+        const double rel = c->st_host->rel;
+        c->st_host[0] = fin[0];
+        c->st_host[0].rel = rel;
+        if (rel <= 1.5 * o.tol) {
+            fill_stats(c, 1, 0, s);
+            if (s) s->fallbacks = 0;
+            return worst_code(c, 1);
+        }
+        stalled = true;  // recursive residual drifted from the true one
+    }
+    if (stalled || (status != 0 && status != 10)) {  // Jacobi-PCG from the current iterate
         hx_pcg_opts jo = o;
         jo.precond = HX_PC_JACOBI;
         jo.warm_start = 1;
-        jo.max_iter = std::max(1, o.max_iter - lit);
+        jo.max_iter = std::max(o.max_iter - lit, o.max_iter / 2);
         rc = pcg_fp64(c, jo, s, SolveVecs{1, c->f.p, c->x.p, c->r.p, c->p.p, c->q.p, nullptr, nullptr, nullptr,
                                           nullptr, c->p2.p, nullptr});
         if (rc == HX_ERR_HIP || rc == HX_ERR_OOM) return rc;
-        fallbacks = 1;
         if (s) {
             s->iterations += lit;
-            s->fallbacks = fallbacks;
+            s->fallbacks = 1;
         }
         return rc;
     }
-    // report the true residual
-    std::vector<PcgState> fin(c->st_host, c->st_host + 1);
-    mv64(c, c->x.p, c->q.p, c->E.p, 0.0, MV_PLAIN, HX_MV_AUTO, 1);
-    launch_true_residual(n, c->f.p, c->q.p, c->x.p, c->minv.p, o.shift, nullptr, nullptr, c->st, c->partials.p, 1,
-                         c->stream, 1);
-    HXK();
-    if ((rc = sync_state(c, 1))) return rc;
-    const double rel = c->st_host->rel;
-    c->st_host[0] = fin[0];
-    c->st_host[0].rel = rel;
-    fill_stats(c, 1, 0, s);
-    if (s) s->fallbacks = fallbacks;
+    fill_stats(c, 1, 0, s);  // zero right-hand side
+    if (s) s->fallbacks = 0;
     return worst_code(c, 1);
 }
 
@@ -1160,7 +1183,7 @@ int hx_learned_train(hx_ctx* c, const double* Ee, const double* f, hx_learn_stat
     hipEvent_t ev0 = get_event(c), ev1 = get_event(c);
     (void)hipEventRecord(ev0, c->stream);
     if (!c->unet) c->unet = unet_create();
-    if (!unet_bind(c->unet, c->g, (unsigned)s->seed, c->stream)) return fail(c, HX_ERR_OOM, "U-Net buffers");
+    if (!unet_bind(c->unet, c->g, (unsigned)s->seed, c->stream, true)) return fail(c, HX_ERR_OOM, "U-Net buffers");
     const long n = ndof(c), ne = c->g.ne;
     // residual/error pairs at these Jacobi-PCG iteration counts of each problem
     const int ks[] = {0, 2, 8, 32, 128, 512};
@@ -1246,19 +1269,46 @@ int hx_learned_train(hx_ctx* c, const double* Ee, const double* f, hx_learn_stat
     HXC(hipStreamSynchronize(c->stream));
     c->l_nsamples = nsamp;
     c->l_sample_stride = stride;
-    // Adam over the harvested samples, batch size 1, seeded order
-    std::vector<int> order(nsamp);
-    for (int i = 0; i < nsamp; ++i) order[i] = i;
-    double last = 0.0, first_epoch = 0.0;
+    // per-sample normalisation: |r|^2 and e'r (= |e|_A^2)
+    std::vector<double> rr(nsamp), er(nsamp);
+    DevBuf<double> nrm;
+    HXA(nrm, 4);
+    for (int i = 0; i < nsamp; ++i) {
+        const double* base = c->l_samples.p + (long)i * stride;
+        double h[2];
+        launch_dot(n, base, base, nrm.p, nullptr, c->partials.p, c->stream);
+        launch_dot(n, base + n, base, nrm.p + 1, nullptr, c->partials.p, c->stream);
+        HXC(hipMemcpyAsync(h, nrm.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HXC(hipStreamSynchronize(c->stream));
+        rr[i] = h[0];
+        er[i] = h[1];
+    }
+    // Adam over the harvested samples, batch size 1, seeded order, on the
+    // energy-norm loss |z - e|_A^2 / |e|_A^2 of the preconditioned residual
+    HXA(c->l_z, n);
+    HXA(c->l_zold, n);
+    std::vector<int> order;
+    for (int i = 0; i < nsamp; ++i)
+        if (rr[i] > 0.0 && er[i] > 0.0) order.push_back(i);
+    if (order.empty()) return fail(c, HX_ERR_STATE, "no usable training samples");
+    double last = 0.0;
     for (int ep = 0; ep < s->epochs; ++ep) {
         std::shuffle(order.begin(), order.end(), rng);
-        double acc = 0.0;
+        HXC(hipMemsetAsync(nrm.p + 2, 0, sizeof(double), c->stream));
         for (int i : order) {
             double* base = c->l_samples.p + (long)i * stride;
-            acc += unet_train_step(c->unet, base, base + n, base + 3 * n, base + 2 * n, (float)s->lr, c->stream);
+            double *sr = base, *se = base + n, *sm = base + 2 * n, *sE = base + 3 * n;
+            const double rms = std::sqrt(rr[i] / (double)n), inv = 1.0 / rms;
+            unet_forward_sample(c->unet, sr, sE, inv, c->stream);
+            unet_energy_diff(c->unet, se, sr, sm, inv, c->l_z.p, c->stream);
+            mv64(c, c->l_z.p, c->l_zold.p, sE, 0.0, MV_PLAIN, HX_MV_AUTO, 1);
+            unet_energy_grad(c->unet, c->l_z.p, c->l_zold.p, sm, rms * rms / er[i], nrm.p + 2, c->stream);
+            unet_backward_step(c->unet, (float)s->lr, c->stream);
         }
-        last = acc / nsamp;
-        if (ep == 0) first_epoch = last;
+        double acc = 0.0;
+        HXC(hipMemcpyAsync(&acc, nrm.p + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HXC(hipStreamSynchronize(c->stream));
+        last = acc / (double)order.size();
     }
     HXK();
     (void)hipEventRecord(ev1, c->stream);
@@ -1271,7 +1321,6 @@ int hx_learned_train(hx_ctx* c, const double* Ee, const double* f, hx_learn_stat
     s->samples = nsamp;
     s->final_loss = last;
     s->seconds = ms * 1e-3;
-    (void)first_epoch;
     return HX_OK;
 }
 
@@ -1429,6 +1478,108 @@ int hx_autotune(hx_ctx* c, int nrhs, hx_tune_result* out) {
         out->pdepth = c->shape64.pdepth;
         out->pdepth32 = c->shape32.pdepth;
     }
+    return HX_OK;
+}
+
+// Operator timing: `reps` back-to-back launches between two HIP events, each
+// on its own set of vectors (kSets sets, > 2x the 256 MB last-level cache
+// together at production sizes), so a launch does not find its inputs left
+// in cache by the previous one -- as in the PCG, where the vector update
+// streams everything between two operator applications.
+extern "C++" {
+template <typename T>
+int time_operator_t(hx_ctx* c, int nrhs, int reps, float* ms) {
+    constexpr int kSets = 4;
+    const long n = ndof(c) * nrhs;
+    DevBuf<T> x[kSets], y[kSets], r[kSets], m[kSets], p[kSets];
+    for (int k = 0; k < kSets; ++k) {
+        HXA(x[k], n);
+        HXA(y[k], n);
+        HXA(r[k], n);
+        HXA(m[k], ndof(c));
+        HXA(p[k], n);
+        if constexpr (sizeof(T) == 8) {
+            launch_fill(n, 0.5, x[k].p, c->stream);
+            launch_fill(n, 0.25, r[k].p, c->stream);
+            launch_fill(ndof(c), 1.0, m[k].p, c->stream);
+        } else {
+            launch_fill32(n, 0.5f, x[k].p, c->stream);
+            launch_fill32(n, 0.25f, r[k].p, c->stream);
+            launch_fill32(ndof(c), 1.0f, m[k].p, c->stream);
+        }
+    }
+    // the PCG-mode kernel without its p'q partial write: same loads and
+    // stores per node (p_old, r, D^-1, E in; q, p out), fused p-update on
+    auto go = [&](int k) {
+        if constexpr (sizeof(T) == 8)
+            mv64(c, x[k].p, y[k].p, c->E.p, 0.0, MV_PLAIN, HX_MV_AUTO, nrhs, r[k].p, m[k].p, p[k].p);
+        else
+            mv32(c, x[k].p, y[k].p, c->E32.p, 0.0, MV_PLAIN, c->mv32_variant, nrhs, r[k].p, m[k].p, p[k].p);
+    };
+    hipEvent_t a = get_event(c), b = get_event(c);
+    for (int k = 0; k < kSets; ++k) go(k);
+    (void)hipEventRecord(a, c->stream);
+    for (int i = 0; i < reps; ++i) go(i % kSets);
+    (void)hipEventRecord(b, c->stream);
+    HXC(hipEventSynchronize(b));
+    (void)hipEventElapsedTime(ms, a, b);
+    c->free_events.push_back(a);
+    c->free_events.push_back(b);
+    return HX_OK;
+}
+}  // extern "C++"
+
+int hx_time_operator(hx_ctx* c, int precision, int nrhs, int reps, double* avg_ms) {
+    if (!c || nrhs < 1 || nrhs > kMaxRhs || reps < 1 || !avg_ms) return HX_ERR_BAD_ARG;
+    if (precision != HX_FP64 && precision != HX_FP32_IR) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    if (int rc = ensure_fp32(c)) return rc;
+    const bool was_timing = c->timing;
+    c->timing = false;
+    launch_pcg_reset(c->st, 0.0, 0.0, 1 << 30, 1, c->stream, nrhs);
+    float ms = 0.f;
+    const int rc = precision == HX_FP64 ? time_operator_t<double>(c, nrhs, reps, &ms)
+                                        : time_operator_t<float>(c, nrhs, reps, &ms);
+    c->timing = was_timing;
+    if (rc) return rc;
+    HXK();
+    *avg_ms = (double)ms / reps;
+    return HX_OK;
+}
+
+int hx_tune_set(hx_ctx* c, const hx_tune_result* t) {
+    if (!c || !t) return HX_ERR_BAD_ARG;
+    auto shape_ok = [](int tile, int kchunk, int pd) { return (tile == 0 || tile == 1) && kchunk >= 4 && (pd == 1 || pd == 2); };
+    auto var_ok = [](int v) { return v == HX_MV_PARITY || v == HX_MV_DENSE || v == HX_MV_GATHER || v == HX_MV_MFMA; };
+    if (!shape_ok(t->tile, t->kchunk, t->pdepth) || !shape_ok(t->tile32, t->kchunk32, t->pdepth32) ||
+        !var_ok(t->variant64) || !var_ok(t->variant32) || t->variant32 == HX_MV_GATHER || t->variant64 == HX_MV_MFMA)
+        return fail(c, HX_ERR_BAD_ARG, "invalid tuning result");
+    c->shape64 = MvShape{t->tile, t->kchunk, t->pdepth};
+    c->shape32 = MvShape{t->tile32, t->kchunk32, t->pdepth32};
+    c->mv_variant = t->variant64;
+    c->mv32_variant = t->variant32;
+    clear_graphs(c);
+    return HX_OK;
+}
+
+int hx_test_copy(hx_ctx* c, long bytes, int reps, double* avg_ms) {
+    if (!c || bytes <= 0 || bytes % 16 || reps < 1 || !avg_ms) return HX_ERR_BAD_ARG;
+    DevBuf<uint8_t> a, b;
+    HXA(a, bytes);
+    HXA(b, bytes);
+    HXC(hipMemsetAsync(a.p, 1, bytes, c->stream));
+    hipEvent_t e0 = get_event(c), e1 = get_event(c);
+    launch_copy16(bytes / 16, a.p, b.p, c->stream);  // warm-up
+    (void)hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps; ++r) launch_copy16(bytes / 16, a.p, b.p, c->stream);
+    (void)hipEventRecord(e1, c->stream);
+    HXC(hipEventSynchronize(e1));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    c->free_events.push_back(e0);
+    c->free_events.push_back(e1);
+    HXK();
+    *avg_ms = (double)ms / reps;
     return HX_OK;
 }
 

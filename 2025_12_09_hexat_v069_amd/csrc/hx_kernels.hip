@@ -175,6 +175,10 @@ __global__ __launch_bounds__(NT) void k_pcg_pupdate(long n, const T* __restrict_
         p[i] = T((double)minv[i] * (double)r[i] + beta * (double)p[i]);
 }
 
+__global__ __launch_bounds__(NT) void k_copy16(long n, const uint4* __restrict__ src, uint4* __restrict__ dst) {
+    for (long i = gtid(); i < n; i += gstride()) dst[i] = src[i];
+}
+
 __global__ __launch_bounds__(NT) void k_dot(long n, const double* __restrict__ a, const double* __restrict__ b,
                                             double* partials) {
     __shared__ double red[NT / 64];
@@ -667,15 +671,17 @@ __global__ __launch_bounds__(NT) void k_fin_fpcg_update(PcgState* st, const doub
     if (threadIdx.x == 0) pcg_after_update(st, st->rz, t);  // keeps rz; beta is recomputed by k_fpcg_beta
 }
 
-// rz_new = r.z, rzo = r.z_old -> beta = (rz_new - rzo)/rz (Polak-Ribiere)
+// Flexible (Polak-Ribiere) beta for a preconditioner that need not be linear
+// or symmetric: rz_new = r_k.z_k, zro = z_k.r_{k-1}, beta = (rz_new - zro)/rz
+// (for a symmetric linear M, z_k.r_{k-1} = r_k.z_{k-1} and this is plain PCG)
 __global__ __launch_bounds__(NT) void k_fpcg_beta(long n, const double* __restrict__ r, const double* __restrict__ z,
-                                                  const double* __restrict__ zold, PcgState* st, double* partials,
+                                                  const double* __restrict__ rold, PcgState* st, double* partials,
                                                   int first) {
     __shared__ double red[NT / 64];
     double a = 0.0, b = 0.0;
     for (long i = gtid(); i < n; i += gstride()) {
         a += r[i] * z[i];
-        if (!first) b += r[i] * zold[i];
+        if (!first) b += z[i] * rold[i];
     }
     const double v0 = block_sum<NT>(a, red);
     const double v1 = block_sum<NT>(b, red);
@@ -697,13 +703,14 @@ __global__ __launch_bounds__(NT) void k_fin_fpcg_beta(PcgState* st, const double
 }
 
 // p = z + beta p ; zold = z
+// p = z + beta p ; rold = r (for the next flexible beta)
 __global__ __launch_bounds__(NT) void k_fpcg_pupdate(long n, const double* __restrict__ z, double* __restrict__ p,
-                                                     double* __restrict__ zold, const PcgState* st) {
+                                                     const double* __restrict__ r, double* __restrict__ rold,
+                                                     const PcgState* st) {
     const double beta = st->beta;
     for (long i = gtid(); i < n; i += gstride()) {
-        const double zi = z[i];
-        p[i] = zi + beta * p[i];
-        zold[i] = zi;
+        p[i] = z[i] + beta * p[i];
+        rold[i] = r[i];
     }
 }
 
@@ -738,6 +745,9 @@ void launch_pcg_update(long n, T* x, const T* p, T* r, const T* q, const T* minv
 template <typename T>
 void launch_pcg_pupdate(long n, const T* r, const T* minv, T* p, const PcgState* st, hipStream_t s, int nrhs) {
     hipLaunchKernelGGL((k_pcg_pupdate<T>), dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, r, minv, p, st);
+}
+void launch_copy16(long n, const void* src, void* dst, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy16, dim3(4 * kVecBlocks), dim3(NT), 0, s, n, (const uint4*)src, (uint4*)dst);
 }
 void launch_dot(long n, const double* a, const double* b, double* out, unsigned* ticket, double* partials,
                 hipStream_t s) {
@@ -816,13 +826,14 @@ void launch_fpcg_update(long n, double* x, const double* p, double* r, const dou
     hipLaunchKernelGGL(k_fpcg_update, dim3(kVecBlocks), dim3(NT), 0, s, n, x, p, r, q, minv, st, partials);
     hipLaunchKernelGGL(k_fin_fpcg_update, dim3(1), dim3(NT), 0, s, st, partials, (unsigned)kVecBlocks);
 }
-void launch_fpcg_beta(long n, const double* r, const double* z, const double* zold, PcgState* st, double* partials,
+void launch_fpcg_beta(long n, const double* r, const double* z, const double* rold, PcgState* st, double* partials,
                       int first, hipStream_t s) {
-    hipLaunchKernelGGL(k_fpcg_beta, dim3(kVecBlocks), dim3(NT), 0, s, n, r, z, zold, st, partials, first);
+    hipLaunchKernelGGL(k_fpcg_beta, dim3(kVecBlocks), dim3(NT), 0, s, n, r, z, rold, st, partials, first);
     hipLaunchKernelGGL(k_fin_fpcg_beta, dim3(1), dim3(NT), 0, s, st, partials, (unsigned)kVecBlocks, first);
 }
-void launch_fpcg_pupdate(long n, const double* z, double* p, double* zold, const PcgState* st, hipStream_t s) {
-    hipLaunchKernelGGL(k_fpcg_pupdate, dim3(kVecBlocks), dim3(NT), 0, s, n, z, p, zold, st);
+void launch_fpcg_pupdate(long n, const double* z, double* p, const double* r, double* rold, const PcgState* st,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_fpcg_pupdate, dim3(kVecBlocks), dim3(NT), 0, s, n, z, p, r, rold, st);
 }
 void launch_masked_diff(long n, const double* a, const double* b, const double* minv, double* e, hipStream_t s) {
     hipLaunchKernelGGL(k_masked_diff, dim3(kVecBlocks), dim3(NT), 0, s, n, a, b, minv, e);

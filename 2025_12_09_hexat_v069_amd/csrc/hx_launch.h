@@ -68,6 +68,8 @@ void launch_fin_alpha(PcgState* st, const double* partials, int nblocks, int nrh
 // Jacobi inverse diagonal is shared; each column has its own PcgState and
 // partials block.
 constexpr int kVecBlocks = 1024;  // blocks of the streaming kernels (multiple of 8)
+// dst[i] = src[i] for n 16-byte words (counter calibration)
+void launch_copy16(long n, const void* src, void* dst, hipStream_t s);
 template <typename T>
 void launch_pcg_init(long n, const T* b, T* x, const T* q, const T* minv, T* r, T* p, double shift,
                      PcgState* st, double* partials, hipStream_t s, int nrhs = 1);
@@ -104,9 +106,12 @@ void launch_accumulate(long n, double* x64, float* x32, hipStream_t s, int nrhs 
 // ---- flexible PCG pieces (learned preconditioner) ---------------------------
 void launch_fpcg_update(long n, double* x, const double* p, double* r, const double* q, const double* minv,
                         PcgState* st, double* partials, hipStream_t s);
-void launch_fpcg_beta(long n, const double* r, const double* z, const double* zold, PcgState* st, double* partials,
+// beta = (r_k.z_k - z_k.r_{k-1}) / rz (flexible Polak-Ribiere), rz = r_k.z_k
+void launch_fpcg_beta(long n, const double* r, const double* z, const double* rold, PcgState* st, double* partials,
                       int first, hipStream_t s);
-void launch_fpcg_pupdate(long n, const double* z, double* p, double* zold, const PcgState* st, hipStream_t s);
+// p = z + beta p ; rold = r
+void launch_fpcg_pupdate(long n, const double* z, double* p, const double* r, double* rold, const PcgState* st,
+                         hipStream_t s);
 void launch_masked_diff(long n, const double* a, const double* b, const double* minv, double* e, hipStream_t s);
 
 // ---- Helmholtz filter --------------------------------------------------------

@@ -139,50 +139,29 @@ __global__ void k_precond_combine(long nn, const float* __restrict__ yf, const d
     }
 }
 
-// loss against target t = (e - minv r)/scale on free DOFs: L = |y - t|^2/|t|^2
-__global__ void k_loss(long nn, const float* __restrict__ yf, const double* __restrict__ e,
-                       const double* __restrict__ r, const double* __restrict__ minv, double inv_scale,
-                       double inv_t2, float* __restrict__ gf, double* acc) {
+// d = y - (e - minv r) * inv_scale on free DOFs, 0 elsewhere
+__global__ void k_energy_diff(long nn, const float* __restrict__ yf, const double* __restrict__ e,
+                              const double* __restrict__ r, const double* __restrict__ minv, double inv_scale,
+                              double* __restrict__ d) {
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < 3 * nn; i += (long)gridDim.x * 256) {
+        const double m = minv[i];
+        d[i] = m != 0.0 ? (double)yf[i] - (e[i] - m * r[i]) * inv_scale : 0.0;
+    }
+}
+
+__global__ void k_energy_grad(long nn, const double* __restrict__ d, const double* __restrict__ Ad,
+                              const double* __restrict__ minv, double w, float* __restrict__ gf, double* acc) {
     __shared__ double red[4];
     double s = 0.0;
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < 3 * nn; i += (long)gridDim.x * 256) {
-        const double m = minv[i];
-        double d = 0.0;
-        if (m != 0.0) d = (double)yf[i] - (e[i] - m * r[i]) * inv_scale;
-        gf[i] = (float)(2.0 * d * inv_t2);
-        s += d * d;
+        const bool fr = minv[i] != 0.0;
+        gf[i] = fr ? (float)(2.0 * w * Ad[i]) : 0.f;
+        s += fr ? d[i] * Ad[i] : 0.0;
     }
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(acc, (red[0] + red[1] + red[2] + red[3]) * inv_t2);
-}
-
-// |t|^2 and |r|^2 of a sample (for its normalisation)
-__global__ void k_sample_norms(long n, const double* __restrict__ e, const double* __restrict__ r,
-                               const double* __restrict__ minv, double* acc) {
-    __shared__ double red[2][4];
-    double st = 0.0, sr = 0.0;
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-        const double m = minv[i];
-        if (m == 0.0) continue;
-        const double t = e[i] - m * r[i];
-        st += t * t;
-        sr += r[i] * r[i];
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        st += __shfl_xor(st, o, 64);
-        sr += __shfl_xor(sr, o, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        red[0][threadIdx.x >> 6] = st;
-        red[1][threadIdx.x >> 6] = sr;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        atomicAdd(acc, red[0][0] + red[0][1] + red[0][2] + red[0][3]);
-        atomicAdd(acc + 1, red[1][0] + red[1][1] + red[1][2] + red[1][3]);
-    }
+    if (threadIdx.x == 0) atomicAdd(acc, (red[0] + red[1] + red[2] + red[3]) * w);
 }
 
 unsigned gb(long n) { return (unsigned)std::min<long>((n + 255) / 256, 8192); }
@@ -267,6 +246,14 @@ struct UNet {
         (void)hipStreamSynchronize(s);
         adam_t = 0;
         ready = true;
+    }
+
+    void zero_output_layer(hipStream_t s) {
+        const size_t l = L.size() - 1;
+        const long nw = wsize(l) + L[l].cout;  // weights then bias
+        launch_fillf(nw, 0.f, w32.p + L[l].woff, s);
+        (void)hipMemsetAsync(wbf.p + L[l].woff, 0, nw * sizeof(uint16_t), s);
+        (void)hipStreamSynchronize(s);
     }
 
     ConvArgs conv(int l, const uint16_t* in0, Vol vc0, const uint16_t* in1, Vol vin, Vol vout, uint16_t* out_bf,
@@ -418,10 +405,15 @@ void unet_destroy(UNet* u) { delete u; }
 long unet_nparam(const UNet* u) { return u->nparam; }
 bool unet_ready(const UNet* u) { return u && u->ready; }
 
-bool unet_bind(UNet* u, const GridArgs& g, unsigned seed, hipStream_t s) {
+bool unet_bind(UNet* u, const GridArgs& g, unsigned seed, hipStream_t s, bool zero_output) {
     if (!u->alloc_params()) return false;
     if (!u->bind(g)) return false;
-    if (!u->ready) u->init_weights(seed, s);
+    if (!u->ready) {
+        u->init_weights(seed, s);
+        // a fresh preconditioner network starts as Jacobi: the output layer
+        // is zero, so z = D^-1 r until training moves it
+        if (zero_output) u->zero_output_layer(s);
+    }
     return true;
 }
 
@@ -434,52 +426,29 @@ void unet_apply(UNet* u, const double* r, const double* E, const double* minv, d
     hipLaunchKernelGGL(k_precond_combine, dim3(gb(3 * u->g.nn)), dim3(256), 0, s, u->g.nn, u->yf.p, r, minv, rms, z);
 }
 
-// one training step on a sample (r, e) with its moduli E; returns the loss
-double unet_train_step(UNet* u, const double* r, const double* e, const double* E, const double* minv, float lr,
-                       hipStream_t s) {
-    const long n = 3 * u->g.nn;
-    double h[2] = {0, 0};
-    (void)hipMemsetAsync(u->sc.p, 0, 4 * sizeof(double), s);
-    hipLaunchKernelGGL(k_sample_norms, dim3(gb(n)), dim3(256), 0, s, n, e, r, minv, u->sc.p);
-    (void)hipMemcpyAsync(h, u->sc.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
-    (void)hipStreamSynchronize(s);
-    const double rms = std::sqrt(h[1] / (double)n);
-    if (!(h[0] > 0.0) || !(rms > 0.0)) return 0.0;
-    const double inv_scale = 1.0 / rms;
-    const double inv_t2 = 1.0 / (h[0] * inv_scale * inv_scale);
+// Energy-norm training pieces.  With z = D^-1 r + rms*y and the sample's
+// error e, z - e = rms*d where d = y - (e - D^-1 r)/rms on free DOFs; the
+// loss is |z - e|_A^2 / (e'r) (e'r = |e|_A^2 since A e = r), so the caller
+// applies the operator to d and passes A d back for the gradient.
+void unet_forward_sample(UNet* u, const double* r, const double* E, double inv_scale, hipStream_t s) {
     launch_pack_input(u->g, r, E, inv_scale, u->x0.p, s);
     u->forward(s);
-    hipLaunchKernelGGL(k_loss, dim3(gb(n)), dim3(256), 0, s, u->g.nn, u->yf.p, e, r, minv, inv_scale, inv_t2, u->gyf.p,
-                       u->sc.p + 2);
+}
+void unet_energy_diff(UNet* u, const double* e, const double* r, const double* minv, double inv_scale, double* d,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_energy_diff, dim3(gb(3 * u->g.nn)), dim3(256), 0, s, u->g.nn, u->yf.p, e, r, minv, inv_scale,
+                       d);
+}
+// loss += w d'(A d); dL/dy = 2 w (A d) on free DOFs, w = rms^2 / (e'r)
+void unet_energy_grad(UNet* u, const double* d, const double* Ad, const double* minv, double w, double* loss_acc,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_energy_grad, dim3(gb(3 * u->g.nn)), dim3(256), 0, s, u->g.nn, d, Ad, minv, w, u->gyf.p,
+                       loss_acc);
+}
+void unet_backward_step(UNet* u, float lr, hipStream_t s) {
     u->backward(s);
     u->adam(lr, 1.0f, s);
-    double loss = 0.0;
-    (void)hipMemcpyAsync(&loss, u->sc.p + 2, sizeof(double), hipMemcpyDeviceToHost, s);
-    (void)hipStreamSynchronize(s);
     u->trained = true;
-    return loss;
-}
-
-// loss of the current network on a sample, without training
-double unet_eval(UNet* u, const double* r, const double* e, const double* E, const double* minv, hipStream_t s) {
-    const long n = 3 * u->g.nn;
-    double h[2] = {0, 0};
-    (void)hipMemsetAsync(u->sc.p, 0, 4 * sizeof(double), s);
-    hipLaunchKernelGGL(k_sample_norms, dim3(gb(n)), dim3(256), 0, s, n, e, r, minv, u->sc.p);
-    (void)hipMemcpyAsync(h, u->sc.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
-    (void)hipStreamSynchronize(s);
-    const double rms = std::sqrt(h[1] / (double)n);
-    if (!(h[0] > 0.0) || !(rms > 0.0)) return 0.0;
-    const double inv_scale = 1.0 / rms;
-    const double inv_t2 = 1.0 / (h[0] * inv_scale * inv_scale);
-    launch_pack_input(u->g, r, E, inv_scale, u->x0.p, s);
-    u->forward(s);
-    hipLaunchKernelGGL(k_loss, dim3(gb(n)), dim3(256), 0, s, u->g.nn, u->yf.p, e, r, minv, inv_scale, inv_t2, u->gyf.p,
-                       u->sc.p + 2);
-    double loss = 0.0;
-    (void)hipMemcpyAsync(&loss, u->sc.p + 2, sizeof(double), hipMemcpyDeviceToHost, s);
-    (void)hipStreamSynchronize(s);
-    return loss;
 }
 
 int unet_weights_bf16(UNet* u, uint16_t* out, hipStream_t s) {

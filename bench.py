@@ -19,11 +19,17 @@ reported beside it.  All inputs are resident in HBM during the timed region.
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import importlib
 import json
 import math
 import os
+import re
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -53,10 +59,13 @@ def cantilever(nx, ny, nz):
     return fixed, f
 
 
-def run_form(hx, L, ctx, nx, ny, nz, f, fixed, precision, steps, warmup, args):
+def run_form(hx, L, ctx, nx, ny, nz, f, fixed, precision, steps, warmup, args, tune=None):
     ctx.set_grid(nx, ny, nz, 1.0, 1.0, 1.0, 0.3)
     ctx.set_fixed(fixed)
-    if not args.no_autotune:
+    if tune is not None:  # the shapes the traffic passes measured
+        ctx.tune_set(tune)
+        log(f"  tuning (from the traffic pass): {tune}")
+    elif not args.no_autotune:
         t = ctx.autotune(1)
         log(f"  autotune: {t}")
     p = L.TOParams(E0=1.0, Emin=1e-9, nu=0.3, penal=3.0, volfrac=args.volfrac, move=0.2, xmin=0.0,
@@ -66,8 +75,7 @@ def run_form(hx, L, ctx, nx, ny, nz, f, fixed, precision, steps, warmup, args):
     for w in range(warmup):
         s = ctx.to_step()
         log(f"  warmup {w + 1}: c={s.compliance:.6e} pcg={s.pcg_iterations} t={s.seconds:.3f}s")
-    ctx.timing(True)
-    ctx.L.hx_device_count  # noqa: B018 (library loaded)
+    # timed region: production launches (hipGraph batches, no per-kernel events)
     t0 = time.perf_counter()
     recs = []
     for k in range(steps):
@@ -76,6 +84,12 @@ def run_form(hx, L, ctx, nx, ny, nz, f, fixed, precision, steps, warmup, args):
         log(f"  step {k + 1}: c={s.compliance:.6e} change={s.change:.4f} pcg={s.pcg_iterations} "
             f"rel={s.pcg_rel_res:.2e} filt={s.filter_iterations} vm_max={s.max_vm:.3e} t={s.seconds:.3f}s")
     wall = time.perf_counter() - t0  # to_step synchronises the stream at its end
+    # operator kernel alone: back-to-back launches at the production shape
+    op_ms = ctx.time_operator(precision, 1, 40)
+    # kernel breakdown: one more TO iteration with HIP events around every hot
+    # launch (direct launches, so its wall time is not the reported value)
+    ctx.timing(True)
+    s = ctx.to_step()
     kern = {}
     names = {L.HX_K_MATVEC: "matvec_fp64", L.HX_K_MATVEC32: "matvec_fp32", L.HX_K_UPDATE: "pcg_update",
              L.HX_K_PUPDATE: "pcg_pupdate", L.HX_K_HELM: "helm_matvec", L.HX_K_ELEM: "element_post"}
@@ -83,7 +97,110 @@ def run_form(hx, L, ctx, nx, ny, nz, f, fixed, precision, steps, warmup, args):
         ms, cnt = ctx.timing_read(kid)
         kern[nm] = {"avg_ms": ms, "count": cnt, "total_ms": ms * cnt}
     ctx.timing(False)
+    kern["operator_back_to_back"] = {"avg_ms": op_ms, "count": 40, "total_ms": 40 * op_ms}
+    log(f"  breakdown step: pcg={s.pcg_iterations} t={s.seconds:.3f}s; operator kernel {op_ms:.4f} ms")
     return wall / steps, recs, kern
+
+
+COPY_BYTES = 1 << 30
+SAMPLE_PCG_ITERS = 40
+
+
+def kernel_sample(args):
+    """Child process run under ``rocprofv3 --pmc``: autotune on the workload's
+    grid, a plain 1 GiB device copy (the counter calibration), then a bounded
+    PCG of the given form on the uniform-density SIMP moduli, so the dominant
+    kernel runs SAMPLE_PCG_ITERS times at its production shape.  Prints the
+    tuning result as JSON."""
+    nx, ny, nz = (int(v) for v in args.grid.lower().split("x"))
+    hx = importlib.import_module("2025_12_09_hexat_v069_amd")
+    L = hx._lib
+    ctx = hx.Context(0)
+    fixed, f = cantilever(nx, ny, nz)
+    ctx.set_grid(nx, ny, nz, 1.0, 1.0, 1.0, 0.3)
+    ctx.set_fixed(fixed)
+    if args.kernel_sample == "tune":
+        print(json.dumps({"tune": ctx.autotune(1)}), flush=True)
+        return
+    tune = json.loads(args.tune_json)
+    ctx.tune_set(tune)
+    ctx.test_copy(COPY_BYTES, 4)
+    Ee = np.full(nx * ny * nz, 1e-9 + args.volfrac ** 3)
+    prec = L.HX_FP64 if args.kernel_sample == "plain" else L.HX_FP32_IR
+    ctx.pcg(Ee, f, tol=1e-30, max_iter=SAMPLE_PCG_ITERS, precision=prec)
+    print(json.dumps({"copy_bytes": COPY_BYTES}), flush=True)
+
+
+def _counter_rows(outdir):
+    rows = []
+    for path in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path, newline="") as fh:
+            rows.extend(csv.DictReader(fh))
+    return rows
+
+
+def measure_traffic(args, form, tune=None):
+    """HBM bytes per launch of the dominant kernel from two rocprofv3 --pmc
+    passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) over a child
+    process (kernel_sample), each scaled by what the same counter reads for a
+    plain copy of known size in the same process.  Runs before this process
+    touches the GPU.  Returns (traffic dict, tuning result or None)."""
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return {"error": "rocprofv3 not found"}, None
+    base = [sys.executable, os.path.abspath(__file__), "--grid", args.grid, "--volfrac", str(args.volfrac)]
+    if tune is None:
+        try:  # tune once, unprofiled; the counter passes and this process reuse the result
+            cp = subprocess.run(base + ["--kernel-sample", "tune"], capture_output=True, text=True, timeout=240)
+            tune = json.loads([ln for ln in cp.stdout.splitlines() if ln.startswith("{")][-1])["tune"]
+        except (subprocess.TimeoutExpired, IndexError, ValueError, KeyError):
+            return {"error": "tuning child failed"}, None
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="hx_pmc_")
+        cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--"] + base + [
+            "--kernel-sample", form, "--tune-json", json.dumps(tune)]
+        log(f"traffic: {counter} pass ...")
+        try:
+            cp = subprocess.run(cmd, capture_output=True, text=True, timeout=240)
+        except subprocess.TimeoutExpired:
+            shutil.rmtree(d, ignore_errors=True)
+            return {"error": f"{counter} pass timed out"}, tune
+        lines = [ln for ln in cp.stdout.splitlines() if ln.startswith("{")]
+        if cp.returncode != 0 or not lines:
+            shutil.rmtree(d, ignore_errors=True)
+            return {"error": f"{counter} pass failed (rc {cp.returncode}): {cp.stderr[-300:]}"}, tune
+        info = json.loads(lines[-1])
+        rows = _counter_rows(d)
+        shutil.rmtree(d, ignore_errors=True)
+
+        def kname(r):
+            return r.get("Kernel_Name") or r.get("kernel_name") or ""
+
+        def val(r):
+            return float(r.get("Counter_Value") or r.get("counter_value") or 0.0)
+
+        copy = [val(r) for r in rows if "k_copy16" in kname(r)]
+        variant = tune["variant32"] if form == "precision" else tune["variant64"]
+        tname = "float" if form == "precision" else "double"
+        key = f"k_matvec_sf<{tname}" if variant == 1 else f"k_matvec_zmarch<{tname}"
+        cand = {}
+        for r in rows:
+            if key in kname(r):
+                cand.setdefault((kname(r), r.get("Grid_Size") or r.get("grid_size")), []).append(val(r))
+        if not copy or not cand:
+            return {"error": f"{counter}: kernels not found in the counter output ({len(rows)} rows)"}, tune
+        (name, _), vals = max(cand.items(), key=lambda kv: len(kv[1]))
+        copy_kib = sum(copy[1:]) / max(len(copy) - 1, 1)  # the first launch is the warm-up
+        scale = (info["copy_bytes"] / 1024.0) / copy_kib
+        m = re.search(r"k_\w+<[^()]*>", name)
+        out[counter] = {"kib_per_launch": sum(vals) / len(vals), "launches": len(vals), "copy_kib": copy_kib,
+                        "scale": scale, "kernel": m.group(0) if m else name[:80]}
+    f, w = out["FETCH_SIZE"], out["WRITE_SIZE"]
+    bytes_ = 1024.0 * (f["kib_per_launch"] * f["scale"] + w["kib_per_launch"] * w["scale"])
+    return {"bytes_per_launch": bytes_, "read_bytes": 1024.0 * f["kib_per_launch"] * f["scale"],
+            "write_bytes": 1024.0 * w["kib_per_launch"] * w["scale"], "fetch_scale": round(f["scale"], 4),
+            "write_scale": round(w["scale"], 4), "kernel": f["kernel"], "launches": f["launches"]}, tune
 
 
 def cpu_baseline(nx, ny, nz, f, fixed, pcg_iters_per_step, filt_iters_per_step, args):
@@ -132,25 +249,28 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-pcg-iters", type=int, default=20)
     ap.add_argument("--cpu-filter-iters", type=int, default=20)
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="only run a plain 1 GiB device copy (4 launches), the reference kernel for scaling the "
-                         "FETCH_SIZE / WRITE_SIZE counters to bytes")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes (roofline.traffic = null)")
+    ap.add_argument("--kernel-sample", choices=("tune", "plain", "precision"), default=None,
+                    help=argparse.SUPPRESS)  # child modes of the traffic passes
+    ap.add_argument("--tune-json", default=None,
+                    help="apply this tuning result (the 'tune' object of an earlier run's JSON line) instead of "
+                         "autotuning, e.g. to profile the same launch shapes")
     args = ap.parse_args()
-    if args.copy_calibration:
-        import torch
-        n = 1 << 30
-        a = torch.ones(n, dtype=torch.uint8, device="cuda:0")
-        b = torch.empty_like(a)
-        for _ in range(4):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read_per_launch": n, "copy_bytes_written_per_launch": n, "launches": 4}))
+    if args.kernel_sample:
+        kernel_sample(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus N>1 is not supported (the path is designed "
                                    "to shard by x-slabs with halo exchange, see DESIGN.md)"}))
         sys.exit(2)
     nx, ny, nz = (int(v) for v in args.grid.lower().split("x"))
+    forms = [s.strip() for s in args.forms.split(",") if s.strip()]
+    main_form = "precision" if "precision" in forms else forms[0]
+    traffic, tune = (None, json.loads(args.tune_json) if args.tune_json else None)
+    if not args.no_traffic:  # before this process initialises the GPU
+        traffic, tune = measure_traffic(args, main_form, tune)
+        log(f"traffic: {traffic}")
     hx = importlib.import_module("2025_12_09_hexat_v069_amd")
     L = hx._lib
     ctx = hx.Context(0)
@@ -158,14 +278,12 @@ def main():
     log(f"device: {info}")
     fixed, f = cantilever(nx, ny, nz)
     ne, ndof = nx * ny * nz, 3 * (nx + 1) * (ny + 1) * (nz + 1)
-    forms = [s.strip() for s in args.forms.split(",") if s.strip()]
     res = {}
     for form in forms:
         prec = L.HX_FP64 if form == "plain" else L.HX_FP32_IR
         log(f"form {form}:")
-        sps, recs, kern = run_form(hx, L, ctx, nx, ny, nz, f, fixed, prec, args.steps, args.warmup, args)
+        sps, recs, kern = run_form(hx, L, ctx, nx, ny, nz, f, fixed, prec, args.steps, args.warmup, args, tune)
         res[form] = {"s_per_iter": sps, "recs": recs, "kern": kern}
-    main_form = "precision" if "precision" in res else forms[0]
     r = res[main_form]
     # roofline of the dominant kernel of the main form.  Algorithmic bytes per
     # launch: the PCG matvec reads r, D^-1, p_old (the p-update is fused into
@@ -185,10 +303,16 @@ def main():
     roof = None
     if cand:
         _, nm, by, avg = cand[0]
+        if nm.startswith("matvec"):  # the kernel alone, back to back, at the production shape
+            avg = r["kern"]["operator_back_to_back"]["avg_ms"]
         ach = by / (avg * 1e-3) / 1e9
+        tr = None
+        if traffic and "bytes_per_launch" in traffic and nm == ("matvec_fp32" if main_form == "precision"
+                                                                else "matvec_fp64"):
+            tr = round(traffic["bytes_per_launch"])
         roof = {"bound": "hbm", "kernel": nm, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": by,
-                "avg_launch_ms": avg}
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": tr, "bytes_per_launch": by,
+                "avg_launch_ms": avg, "traffic_detail": traffic}
     pcg_mean = float(np.mean([s.pcg_iterations for s in res.get("plain", r)["recs"]]))
     filt_mean = float(np.mean([s.filter_iterations for s in res.get("plain", r)["recs"]]))
     cpu = None
@@ -204,7 +328,7 @@ def main():
                                f"{args.volfrac}, PDE filter r={args.rmin}, PCG rel. residual {args.tol}, von Mises "
                                f"each iteration", "form": main_form, "preconditioner": "jacobi",
                    "iterations_timed": f"TO iterations {args.warmup + 1}..{args.warmup + args.steps} from uniform "
-                                       f"rho={args.volfrac}"},
+                                       f"rho={args.volfrac}", "tune": tune},
         "roofline": roof,
         "cpu_baseline": cpu,
         "forms": {k: {"s_per_iter": round(v["s_per_iter"], 4),

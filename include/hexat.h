@@ -211,6 +211,17 @@ typedef struct {
  * the current grid for nrhs columns and keep the fastest (HX_MV_AUTO then uses
  * them).  No reference counterpart. */
 int hx_autotune(hx_ctx* ctx, int nrhs, hx_tune_result* result);
+/* Average duration (ms) of one application of the PCG operator kernel
+ * (fused p-update, FP64 or FP32 contraction per `precision`) at the current
+ * launch shape on the current grid: `reps` back-to-back launches for nrhs
+ * columns between two HIP events on the context stream, rotating over vector
+ * sets larger together than the last-level cache.  No reference
+ * counterpart. */
+int hx_time_operator(hx_ctx* ctx, int precision, int nrhs, int reps, double* avg_ms);
+/* Apply a tuning result (e.g. one cached from an earlier hx_autotune) without
+ * timing anything: its launch shapes and contractions.  No reference
+ * counterpart. */
+int hx_tune_set(hx_ctx* ctx, const hx_tune_result* result);
 
 /* ---- kernel timing ---------------------------------------------------------- */
 enum { HX_K_MATVEC = 0, HX_K_UPDATE = 1, HX_K_PUPDATE = 2, HX_K_HELM = 3, HX_K_ELEM = 4,

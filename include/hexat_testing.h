@@ -15,6 +15,11 @@
 extern "C" {
 #endif
 
+/* Plain device copy of `bytes` bytes (a multiple of 16) between two buffers
+ * the call allocates, `reps` launches of one copy kernel; average launch time
+ * in ms.  The reference kernel for scaling hardware byte counters. */
+int hx_test_copy(hx_ctx* ctx, long bytes, int reps, double* avg_ms);
+
 /* number of network parameters (conv weights then bias, layer by layer) */
 int hx_test_unet_nparam(hx_ctx* ctx, long* nparam);
 /* (re)initialise the network weights with a seed; copy the FP32 master
