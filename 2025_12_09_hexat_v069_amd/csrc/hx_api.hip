@@ -31,13 +31,11 @@ UNet* unet_create();
 void unet_destroy(UNet* u);
 long unet_nparam(const UNet* u);
 bool unet_ready(const UNet* u);
-bool unet_bind(UNet* u, const GridArgs& g, unsigned seed, hipStream_t s, bool zero_output = false);
-void unet_apply(UNet* u, const double* r, const double* E, const double* minv, double rms, double* z, hipStream_t s);
+bool unet_bind(UNet* u, const GridArgs& g, unsigned seed, hipStream_t s);
 void unet_forward_sample(UNet* u, const double* r, const double* E, double inv_scale, hipStream_t s);
-void unet_energy_diff(UNet* u, const double* e, const double* r, const double* minv, double inv_scale, double* d,
-                      hipStream_t s);
-void unet_energy_grad(UNet* u, const double* d, const double* Ad, const double* minv, double w, double* loss_acc,
-                      hipStream_t s);
+void unet_direction(UNet* u, const double* minv, double scale, double* y, hipStream_t s);
+void unet_subspace_grad(UNet* u, const double* r, const double* Au, const double* Ay, const double* minv, double a,
+                        double b, double scale, hipStream_t s);
 void unet_backward_step(UNet* u, float lr, hipStream_t s);
 int unet_weights_bf16(UNet* u, uint16_t* out, hipStream_t s);
 void unet_init(UNet* u, unsigned seed, hipStream_t s);
@@ -119,7 +117,7 @@ struct hx_ctx {
     // learned preconditioner
     UNet* unet = nullptr;
     bool unet_trained = false;
-    DevBuf<double> l_z, l_zold, l_ustar, l_samples, l_rho;
+    DevBuf<double> l_z, l_zold, l_P, l_AP, l_dots, l_ustar, l_samples, l_rho;
     int l_nsamples = 0;
     long l_sample_stride = 0;
     // captured iteration batches (hipGraph), keyed by kernel arguments
@@ -471,18 +469,40 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
 // ------------------------------------------ flexible PCG, learned preconditioner
 // z = D^-1 r + |r|_rms net(r, E); Polak-Ribiere beta.  If it does not reach
 // tol within max_iter (or breaks down), Jacobi-PCG takes over from its iterate.
-// Flexible PCG with the learned preconditioner z = D^-1 r + |r|_rms net(r, E)
-// (GPUSolver.jl's Jacobi-PCG loop with the preconditioner swapped), the
-// Jacobi-PCG fallback from the current iterate when it stagnates or breaks
-// down, and the true residual as the judge of convergence.
-constexpr int kLearnedStall = 40;  // iterations without a 1% drop of the best residual
+// Flexible CG with the learned preconditioner (GPUSolver.jl's Jacobi-PCG loop
+// with the preconditioner swapped).  Each iteration forms u = D^-1 r and the
+// network's direction y = net(r/|r|_rms, E), applies the operator to both in
+// one two-column launch and takes z = a u + b y with (a, b) minimising the
+// energy error over span{u, y} -- which needs no knowledge of the error since
+// A e = r:  [u'Au u'Ay; u'Ay y'Ay] (a, b) = (u'r, y'r).  The search direction
+// is z A-orthogonalised against the last kFcgKeep-1 ones (truncated flexible
+// CG), whose images under A are kept, so A p costs no operator application.
+// Jacobi-PCG takes over from the current iterate when the flexible iteration
+// stalls or breaks down, and the true residual judges convergence.
+// stall: no 1% drop of the best residual for max(kLearnedStall, it/2)
+// iterations (CG residuals plateau for ~100 iterations on healthy runs)
+constexpr int kLearnedStall = 200;
+
+extern "C++" {
+// dot products of n-vectors, one sync for the lot
+int dots_host(hx_ctx* c, long n, std::initializer_list<std::pair<const double*, const double*>> pr, double* out) {
+    int k = 0;
+    for (const auto& q : pr) launch_dot(n, q.first, q.second, c->l_dots.p + k++, nullptr, c->partials.p, c->stream);
+    HXC(hipMemcpyAsync(out, c->l_dots.p, k * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HXC(hipStreamSynchronize(c->stream));
+    return HX_OK;
+}
+}  // extern "C++"
 
 int pcg_learned(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
     if (!c->unet || !c->unet_trained) return fail(c, HX_ERR_STATE, "learned preconditioner not trained");
     if (!unet_bind(c->unet, c->g, 1234u, c->stream)) return fail(c, HX_ERR_OOM, "U-Net activations");
     const long n = ndof(c);
-    HXA(c->l_z, n);
-    HXA(c->l_zold, n);  // r of the previous iteration
+    HXA(c->l_z, 2 * n);     // W = [u | y]
+    HXA(c->l_zold, 2 * n);  // A W
+    HXA(c->l_dots, 8 + 2 * kFcgKeep);
+    hipEvent_t ev0 = get_event(c), ev1 = get_event(c);
+    (void)hipEventRecord(ev0, c->stream);
     launch_elastic_minv<double>(c->g, c->kd, c->E.p, c->has_fixed ? c->fixed.p : nullptr, o.shift, c->minv.p,
                                 c->stream);
     launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream, 1);
@@ -494,68 +514,124 @@ int pcg_learned(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
     HXK();
     int rc = sync_state(c, 1);
     if (rc) return rc;
+    const double normb = c->st_host->normb;
+    double rr = c->st_host->rr, rel = c->st_host->rel, best = rel;
+    int it = 0, best_it = 0, status = c->st_host->status == 10 ? 10 : -1;
     bool stalled = false;
-    if (!c->st_host->done) {
-        unet_apply(c->unet, c->r.p, c->E.p, c->minv.p, std::sqrt(c->st_host->rr / (double)n), c->l_z.p, c->stream);
-        launch_fpcg_beta(n, c->r.p, c->l_z.p, c->l_zold.p, c->st, c->partials.p, 1, c->stream);
-        launch_fpcg_pupdate(n, c->l_z.p, c->p.p, c->r.p, c->l_zold.p, c->st, c->stream);
-        HXK();
-        double best = c->st_host->rel;
-        int best_it = 0;
-        for (;;) {
-            mv64(c, c->p.p, c->q.p, c->E.p, o.shift, MV_PCG, HX_MV_AUTO, 1);
-            launch_fpcg_update(n, c->x.p, c->p.p, c->r.p, c->q.p, c->minv.p, c->st, c->partials.p, c->stream);
-            HXK();
-            if ((rc = sync_state(c, 1))) return rc;
-            if (c->st_host->done) break;
-            const int it = c->st_host->iter;
-            if (c->st_host->rel < 0.99 * best) best = c->st_host->rel, best_it = it;
-            if (it - best_it >= kLearnedStall) {
-                stalled = true;
-                break;
-            }
-            unet_apply(c->unet, c->r.p, c->E.p, c->minv.p, std::sqrt(c->st_host->rr / (double)n), c->l_z.p,
-                       c->stream);
-            launch_fpcg_beta(n, c->r.p, c->l_z.p, c->l_zold.p, c->st, c->partials.p, 0, c->stream);
-            launch_fpcg_pupdate(n, c->l_z.p, c->p.p, c->r.p, c->l_zold.p, c->st, c->stream);
+    double* W = c->l_z.p;
+    double* AW = c->l_zold.p;
+    HXA(c->l_P, (long)kFcgKeep * n);   // stored search directions
+    HXA(c->l_AP, (long)kFcgKeep * n);  // and their images under A
+    double pAp_j[kFcgKeep];
+    int nstored = 0, next = 0;  // ring of kFcgKeep slots; the newest kFcgKeep-1 orthogonalise
+    while (status < 0) {
+        if (rel < o.tol) {
+            status = 0;
+            break;
         }
+        if (it >= o.max_iter) {
+            status = 3;
+            break;
+        }
+        if (it - best_it >= std::max(kLearnedStall, it / 2)) {
+            stalled = true;
+            break;
+        }
+        const double rms = std::sqrt(rr / (double)n);
+        launch_mul(n, c->minv.p, c->r.p, W, c->stream);                       // u
+        unet_forward_sample(c->unet, c->r.p, c->E.p, 1.0 / rms, c->stream);
+        unet_direction(c->unet, c->minv.p, rms, W + n, c->stream);            // y
+        mv64(c, W, AW, c->E.p, o.shift, MV_PLAIN, HX_MV_AUTO, 2);             // [Au | Ay]
+        FcgCoef cf{};
+        cf.m = std::min(nstored, kFcgKeep - 1);
+        for (int j = 0; j < cf.m; ++j) cf.slot[j] = (next - 1 - j + 2 * kFcgKeep) % kFcgKeep;
+        double h[5 + 2 * kFcgKeep];
+        {
+            int k = 0;
+            const std::pair<const double*, const double*> base[5] = {
+                {W, AW}, {W, AW + n}, {W + n, AW + n}, {W, c->r.p}, {W + n, c->r.p}};
+            for (const auto& q : base)
+                launch_dot(n, q.first, q.second, c->l_dots.p + k++, nullptr, c->partials.p, c->stream);
+            for (int j = 0; j < cf.m; ++j) {
+                const double* Pj = c->l_P.p + (long)cf.slot[j] * n;
+                launch_dot(n, AW, Pj, c->l_dots.p + k++, nullptr, c->partials.p, c->stream);
+                launch_dot(n, AW + n, Pj, c->l_dots.p + k++, nullptr, c->partials.p, c->stream);
+            }
+            HXC(hipMemcpyAsync(h, c->l_dots.p, k * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HXC(hipStreamSynchronize(c->stream));
+        }
+        const double uAu = h[0], uAy = h[1], yAy = h[2], ur = h[3], yr = h[4];
+        const double det = uAu * yAy - uAy * uAy;
+        double a = ur / uAu, b = 0.0;  // Jacobi alone when y adds nothing usable
+        if (det > 1e-12 * uAu * yAy) a = (ur * yAy - yr * uAy) / det, b = (yr * uAu - ur * uAy) / det;
+        for (int j = 0; j < cf.m; ++j)  // c_j = (A z)' P_j / P_j' A P_j
+            cf.c[j] = (a * h[5 + 2 * j] + b * h[6 + 2 * j]) / pAp_j[cf.slot[j]];
+        double* pn = c->l_P.p + (long)next * n;
+        double* apn = c->l_AP.p + (long)next * n;
+        launch_fcg_direction(n, a, b, W, AW, cf, c->l_P.p, c->l_AP.p, pn, apn, c->stream);
+        double g[2];
+        if ((rc = dots_host(c, n, {{pn, apn}, {pn, c->r.p}}, g))) return rc;
+        if (!(g[0] > 0.0) || !(g[1] > 0.0)) {
+            status = 4;
+            break;
+        }
+        pAp_j[next] = g[0];
+        next = (next + 1) % kFcgKeep;
+        nstored = std::min(nstored + 1, kFcgKeep);
+        launch_fcg_step(n, g[1] / g[0], c->x.p, pn, c->r.p, apn, c->minv.p, c->stream);
+        if ((rc = dots_host(c, n, {{c->r.p, c->r.p}}, &rr))) return rc;
+        rel = std::sqrt(rr) / normb;
+        ++it;
+        if (!(rel == rel) || rel > 1e3) {
+            status = 5;
+            break;
+        }
+        if (rel < 0.99 * best) best = rel, best_it = it;
     }
-    int lit = c->st_host->iter;
-    const int status = c->st_host->status;
-    if (!stalled && status == 0) {  // the true residual judges convergence
-        std::vector<PcgState> fin(c->st_host, c->st_host + 1);
+    (void)hipEventRecord(ev1, c->stream);
+    (void)hipEventSynchronize(ev1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev0, ev1);
+    c->free_events.push_back(ev0);
+    c->free_events.push_back(ev1);
+    if (status == 10) {  // zero right-hand side
+        fill_stats(c, 1, 0, s);
+        if (s) s->fallbacks = 0;
+        return worst_code(c, 1);
+    }
+    if (status == 0) {  // the true residual judges convergence
         mv64(c, c->x.p, c->q.p, c->E.p, 0.0, MV_PLAIN, HX_MV_AUTO, 1);
         launch_true_residual(n, c->f.p, c->q.p, c->x.p, c->minv.p, o.shift, nullptr, nullptr, c->st, c->partials.p, 1,
                              c->stream, 1);
         HXK();
         if ((rc = sync_state(c, 1))) return rc;
-        const double rel = c->st_host->rel;
-        c->st_host[0] = fin[0];
-        c->st_host[0].rel = rel;
-        if (rel <= 1.5 * o.tol) {
-            fill_stats(c, 1, 0, s);
-            if (s) s->fallbacks = 0;
-            return worst_code(c, 1);
+        if (c->st_host->rel <= 1.5 * o.tol) {
+            if (s) {
+                s->iterations = it;
+                s->status = HX_OK;
+                s->rel_res = c->st_host->rel;
+                s->seconds = ms * 1e-3;
+                s->refinements = 0;
+                s->fallbacks = 0;
+            }
+            return HX_OK;
         }
-        stalled = true;  // recursive residual drifted from the true one
     }
-    if (stalled || (status != 0 && status != 10)) {  // Jacobi-PCG from the current iterate
-        hx_pcg_opts jo = o;
-        jo.precond = HX_PC_JACOBI;
-        jo.warm_start = 1;
-        jo.max_iter = std::max(o.max_iter - lit, o.max_iter / 2);
-        rc = pcg_fp64(c, jo, s, SolveVecs{1, c->f.p, c->x.p, c->r.p, c->p.p, c->q.p, nullptr, nullptr, nullptr,
-                                          nullptr, c->p2.p, nullptr});
-        if (rc == HX_ERR_HIP || rc == HX_ERR_OOM) return rc;
-        if (s) {
-            s->iterations += lit;
-            s->fallbacks = 1;
-        }
-        return rc;
+    // Jacobi-PCG from the current iterate
+    (void)stalled;
+    hx_pcg_opts jo = o;
+    jo.precond = HX_PC_JACOBI;
+    jo.warm_start = 1;
+    jo.max_iter = std::max(o.max_iter - it, o.max_iter / 2);
+    rc = pcg_fp64(c, jo, s, SolveVecs{1, c->f.p, c->x.p, c->r.p, c->p.p, c->q.p, nullptr, nullptr, nullptr, nullptr,
+                                      c->p2.p, nullptr});
+    if (rc == HX_ERR_HIP || rc == HX_ERR_OOM) return rc;
+    if (s) {
+        s->iterations += it;
+        s->seconds += ms * 1e-3;
+        s->fallbacks = 1;
     }
-    fill_stats(c, 1, 0, s);  // zero right-hand side
-    if (s) s->fallbacks = 0;
-    return worst_code(c, 1);
+    return rc;
 }
 
 SolveVecs single_vecs(hx_ctx* c) {
@@ -1183,14 +1259,14 @@ int hx_learned_train(hx_ctx* c, const double* Ee, const double* f, hx_learn_stat
     hipEvent_t ev0 = get_event(c), ev1 = get_event(c);
     (void)hipEventRecord(ev0, c->stream);
     if (!c->unet) c->unet = unet_create();
-    if (!unet_bind(c->unet, c->g, (unsigned)s->seed, c->stream, true)) return fail(c, HX_ERR_OOM, "U-Net buffers");
+    if (!unet_bind(c->unet, c->g, (unsigned)s->seed, c->stream)) return fail(c, HX_ERR_OOM, "U-Net buffers");
     const long n = ndof(c), ne = c->g.ne;
     // residual/error pairs at these Jacobi-PCG iteration counts of each problem
     const int ks[] = {0, 2, 8, 32, 128, 512};
     const int nk = 6;
     const int nprob = std::max(1, (s->samples + nk - 1) / nk);
     const int nsamp = nprob * nk;
-    const long stride = 3 * n + ne;  // r, e, minv, E
+    const long stride = 4 * n + ne;  // r, e, minv, E, A D^-1 r
     HXA(c->l_samples, (size_t)nsamp * stride);
     HXA(c->l_ustar, n);
     HXA(c->l_rho, ne);
@@ -1201,7 +1277,7 @@ int hx_learned_train(hx_ctx* c, const double* Ee, const double* f, hx_learn_stat
     std::vector<uint8_t> hfixed(n, 0);
     if (c->has_fixed) HXC(hipMemcpy(hfixed.data(), c->fixed.p, n, hipMemcpyDeviceToHost));
     int rc;
-    hx_pcg_opts solve{1e-10, 200000, 0.0, HX_FP64, HX_PC_JACOBI, 0, HX_MV_AUTO, 0};
+    hx_pcg_opts solve{1e-12, 200000, 0.0, HX_FP64, HX_PC_JACOBI, 0, HX_MV_AUTO, 0};
     for (int pi = 0; pi < nprob; ++pi) {
         // problem: moduli and loads
         if (pi == 0 && Ee) {
@@ -1263,53 +1339,77 @@ int hx_learned_train(hx_ctx* c, const double* Ee, const double* f, hx_learn_stat
                                  c->stream, 1);
             launch_masked_diff(n, c->l_ustar.p, c->x.p, sm, se, c->stream);
             HXC(hipMemcpyAsync(sE, c->E.p, ne * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            launch_mul(n, sm, sr, c->q.p, c->stream);  // u = D^-1 r
+            mv64(c, c->q.p, sE + ne, c->E.p, 0.0, MV_PLAIN, HX_MV_AUTO, 1);  // A u
             HXK();
         }
     }
     HXC(hipStreamSynchronize(c->stream));
     c->l_nsamples = nsamp;
     c->l_sample_stride = stride;
-    // per-sample normalisation: |r|^2 and e'r (= |e|_A^2)
-    std::vector<double> rr(nsamp), er(nsamp);
+    // per-sample scalars: |r|^2, e'r (= |e|_A^2 since A e = r), and for
+    // u = D^-1 r: u'Au, u'r
     DevBuf<double> nrm;
-    HXA(nrm, 4);
-    for (int i = 0; i < nsamp; ++i) {
-        const double* base = c->l_samples.p + (long)i * stride;
-        double h[2];
-        launch_dot(n, base, base, nrm.p, nullptr, c->partials.p, c->stream);
-        launch_dot(n, base + n, base, nrm.p + 1, nullptr, c->partials.p, c->stream);
-        HXC(hipMemcpyAsync(h, nrm.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HXA(nrm, 8);
+    auto dots = [&](std::initializer_list<std::pair<const double*, const double*>> pr, double* out) -> int {
+        int k = 0;
+        for (const auto& q : pr) launch_dot(n, q.first, q.second, nrm.p + k++, nullptr, c->partials.p, c->stream);
+        HXC(hipMemcpyAsync(out, nrm.p, k * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HXC(hipStreamSynchronize(c->stream));
-        rr[i] = h[0];
-        er[i] = h[1];
-    }
-    // Adam over the harvested samples, batch size 1, seeded order, on the
-    // energy-norm loss |z - e|_A^2 / |e|_A^2 of the preconditioned residual
+        return HX_OK;
+    };
     HXA(c->l_z, n);
     HXA(c->l_zold, n);
+    std::vector<double> rr(nsamp), er(nsamp), uAu(nsamp), ur(nsamp);
     std::vector<int> order;
-    for (int i = 0; i < nsamp; ++i)
-        if (rr[i] > 0.0 && er[i] > 0.0) order.push_back(i);
+    double lj = 0.0;
+    for (int i = 0; i < nsamp; ++i) {
+        const double* base = c->l_samples.p + (long)i * stride;
+        const double *sr = base, *se = base + n, *sm = base + 2 * n, *sAu = base + 3 * n + ne;
+        launch_mul(n, sm, sr, c->l_z.p, c->stream);
+        double h[4];
+        if ((rc = dots({{sr, sr}, {se, sr}, {c->l_z.p, sAu}, {c->l_z.p, sr}}, h))) return rc;
+        rr[i] = h[0], er[i] = h[1], uAu[i] = h[2], ur[i] = h[3];
+        // keep samples whose residual is well above the reference solution's
+        // (relative 1e-12): below ~1e-6 of the problem's first residual, e is
+        // dominated by that solution's own error and A e = r no longer holds
+        const double rr0 = rr[i - i % nk];
+        if (rr[i] > 1e-12 * rr0 && er[i] > 0.0 && uAu[i] > 0.0) {
+            order.push_back(i);
+            lj += 1.0 - ur[i] * ur[i] / (uAu[i] * er[i]);
+        }
+    }
     if (order.empty()) return fail(c, HX_ERR_STATE, "no usable training samples");
+    // Adam, batch size 1, seeded order, on the subspace objective
+    //   L = 1 - max_{a,b} [2(a u'r + b y'r) - |a u + b y|_A^2] / e'r
+    // (the energy left outside span{D^-1 r, y}; scale-invariant in y)
     double last = 0.0;
     for (int ep = 0; ep < s->epochs; ++ep) {
         std::shuffle(order.begin(), order.end(), rng);
-        HXC(hipMemsetAsync(nrm.p + 2, 0, sizeof(double), c->stream));
+        double acc = 0.0;
         for (int i : order) {
             double* base = c->l_samples.p + (long)i * stride;
-            double *sr = base, *se = base + n, *sm = base + 2 * n, *sE = base + 3 * n;
-            const double rms = std::sqrt(rr[i] / (double)n), inv = 1.0 / rms;
+            double *sr = base, *sm = base + 2 * n, *sE = base + 3 * n, *sAu = base + 3 * n + ne;
+            const double inv = 1.0 / std::sqrt(rr[i] / (double)n);
             unet_forward_sample(c->unet, sr, sE, inv, c->stream);
-            unet_energy_diff(c->unet, se, sr, sm, inv, c->l_z.p, c->stream);
-            mv64(c, c->l_z.p, c->l_zold.p, sE, 0.0, MV_PLAIN, HX_MV_AUTO, 1);
-            unet_energy_grad(c->unet, c->l_z.p, c->l_zold.p, sm, rms * rms / er[i], nrm.p + 2, c->stream);
+            unet_direction(c->unet, sm, 1.0, c->l_z.p, c->stream);           // y
+            mv64(c, c->l_z.p, c->l_zold.p, sE, 0.0, MV_PLAIN, HX_MV_AUTO, 1);  // A y
+            double h[3];
+            if ((rc = dots({{sAu, c->l_z.p}, {c->l_z.p, c->l_zold.p}, {c->l_z.p, sr}}, h))) return rc;
+            const double uAy = h[0], yAy = h[1], yr = h[2];
+            const double det = uAu[i] * yAy - uAy * uAy;
+            if (!(det > 1e-12 * uAu[i] * yAy)) {  // y (almost) along D^-1 r: no usable gradient
+                acc += 1.0 - ur[i] * ur[i] / (uAu[i] * er[i]);
+                continue;
+            }
+            const double a = (ur[i] * yAy - yr * uAy) / det, b = (yr * uAu[i] - ur[i] * uAy) / det;
+            acc += 1.0 - (a * ur[i] + b * yr) / er[i];
+            unet_subspace_grad(c->unet, sr, sAu, c->l_zold.p, sm, a, b, -2.0 * b / er[i], c->stream);
             unet_backward_step(c->unet, (float)s->lr, c->stream);
         }
-        double acc = 0.0;
-        HXC(hipMemcpyAsync(&acc, nrm.p + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HXC(hipStreamSynchronize(c->stream));
         last = acc / (double)order.size();
     }
+    s->jacobi_loss = lj / (double)order.size();
     HXK();
     (void)hipEventRecord(ev1, c->stream);
     (void)hipEventSynchronize(ev1);

@@ -646,71 +646,33 @@ __global__ __launch_bounds__(NT) void k_fin_max(double* out, const double* parti
 
 // ---- flexible PCG (variable preconditioner z = P(r)) -----------------------
 // x += alpha p ; r = free ? r - alpha q : 0 ; rr -> convergence (no beta)
-__global__ __launch_bounds__(NT) void k_fpcg_update(long n, double* __restrict__ x, const double* __restrict__ p,
-                                                    double* __restrict__ r, const double* __restrict__ q,
-                                                    const double* __restrict__ minv, PcgState* st,
-                                                    double* partials) {
-    __shared__ double red[NT / 64];
-    if (st->done) return;
-    const double alpha = st->alpha;
-    double rr = 0.0;
+// Flexible CG over the two-direction preconditioner of the learned solve
+// (truncated FCG): z = a u + b y, then p = z - sum_j c_j P_j and
+// Ap = Az - sum_j c_j AP_j over the last m stored directions (A p from the
+// stored images, no operator application)
+__global__ __launch_bounds__(NT) void k_fcg_direction(long n, double a, double b, const double* __restrict__ W,
+                                                      const double* __restrict__ AW, FcgCoef cf,
+                                                      const double* __restrict__ P, const double* __restrict__ AP,
+                                                      double* __restrict__ p, double* __restrict__ Ap) {
+    for (long i = gtid(); i < n; i += gstride()) {
+        double pi = a * W[i] + b * W[n + i];
+        double api = a * AW[i] + b * AW[n + i];
+        for (int j = 0; j < cf.m; ++j) {
+            pi -= cf.c[j] * P[(long)cf.slot[j] * n + i];
+            api -= cf.c[j] * AP[(long)cf.slot[j] * n + i];
+        }
+        p[i] = pi;
+        Ap[i] = api;
+    }
+}
+
+// x += alpha p ; r = free ? r - alpha Ap : 0
+__global__ __launch_bounds__(NT) void k_fcg_step(long n, double alpha, double* __restrict__ x,
+                                                 const double* __restrict__ p, double* __restrict__ r,
+                                                 const double* __restrict__ Ap, const double* __restrict__ minv) {
     for (long i = gtid(); i < n; i += gstride()) {
         x[i] += alpha * p[i];
-        const double ri = minv[i] != 0.0 ? r[i] - alpha * q[i] : 0.0;
-        r[i] = ri;
-        rr += ri * ri;
-    }
-    const double v = block_sum<NT>(rr, red);
-    if (threadIdx.x == 0) partials[blockIdx.x] = v;
-}
-
-__global__ __launch_bounds__(NT) void k_fin_fpcg_update(PcgState* st, const double* partials, unsigned nb) {
-    __shared__ double red[NT / 64];
-    if (st->done) return;
-    const double t = reduce_partials<NT>(partials, 1, 0, nb, red);
-    if (threadIdx.x == 0) pcg_after_update(st, st->rz, t);  // keeps rz; beta is recomputed by k_fpcg_beta
-}
-
-// Flexible (Polak-Ribiere) beta for a preconditioner that need not be linear
-// or symmetric: rz_new = r_k.z_k, zro = z_k.r_{k-1}, beta = (rz_new - zro)/rz
-// (for a symmetric linear M, z_k.r_{k-1} = r_k.z_{k-1} and this is plain PCG)
-__global__ __launch_bounds__(NT) void k_fpcg_beta(long n, const double* __restrict__ r, const double* __restrict__ z,
-                                                  const double* __restrict__ rold, PcgState* st, double* partials,
-                                                  int first) {
-    __shared__ double red[NT / 64];
-    double a = 0.0, b = 0.0;
-    for (long i = gtid(); i < n; i += gstride()) {
-        a += r[i] * z[i];
-        if (!first) b += z[i] * rold[i];
-    }
-    const double v0 = block_sum<NT>(a, red);
-    const double v1 = block_sum<NT>(b, red);
-    if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = v0;
-        partials[2 * blockIdx.x + 1] = v1;
-    }
-}
-
-__global__ __launch_bounds__(NT) void k_fin_fpcg_beta(PcgState* st, const double* partials, unsigned nb, int first) {
-    __shared__ double red[NT / 64];
-    const double s0 = reduce_partials<NT>(partials, 2, 0, nb, red);
-    const double s1 = reduce_partials<NT>(partials, 2, 1, nb, red);
-    if (threadIdx.x == 0) {
-        st->beta = first ? 0.0 : (s0 - s1) / st->rz;
-        st->rz = s0;
-        if (!(s0 > 0.0) && !first) st->beta = 0.0;  // preconditioner lost positivity: restart
-    }
-}
-
-// p = z + beta p ; zold = z
-// p = z + beta p ; rold = r (for the next flexible beta)
-__global__ __launch_bounds__(NT) void k_fpcg_pupdate(long n, const double* __restrict__ z, double* __restrict__ p,
-                                                     const double* __restrict__ r, double* __restrict__ rold,
-                                                     const PcgState* st) {
-    const double beta = st->beta;
-    for (long i = gtid(); i < n; i += gstride()) {
-        p[i] = z[i] + beta * p[i];
-        rold[i] = r[i];
+        r[i] = minv[i] != 0.0 ? r[i] - alpha * Ap[i] : 0.0;
     }
 }
 
@@ -718,6 +680,11 @@ __global__ __launch_bounds__(NT) void k_fpcg_pupdate(long n, const double* __res
 __global__ __launch_bounds__(NT) void k_masked_diff(long n, const double* __restrict__ a, const double* __restrict__ b,
                                                     const double* __restrict__ minv, double* __restrict__ e) {
     for (long i = gtid(); i < n; i += gstride()) e[i] = minv[i] != 0.0 ? a[i] - b[i] : 0.0;
+}
+
+__global__ __launch_bounds__(NT) void k_mul(long n, const double* __restrict__ a, const double* __restrict__ b,
+                                            double* __restrict__ out) {
+    for (long i = gtid(); i < n; i += gstride()) out[i] = a[i] * b[i];
 }
 
 inline unsigned blocks_for(long n) { return (unsigned)((n + NT - 1) / NT); }
@@ -821,19 +788,16 @@ void launch_simp_modulus(long ne, const double* xphys, double penal, double E0, 
                          hipStream_t s) {
     hipLaunchKernelGGL(k_simp_modulus, dim3(kVecBlocks), dim3(NT), 0, s, ne, xphys, penal, E0, Emin, E);
 }
-void launch_fpcg_update(long n, double* x, const double* p, double* r, const double* q, const double* minv,
-                        PcgState* st, double* partials, hipStream_t s) {
-    hipLaunchKernelGGL(k_fpcg_update, dim3(kVecBlocks), dim3(NT), 0, s, n, x, p, r, q, minv, st, partials);
-    hipLaunchKernelGGL(k_fin_fpcg_update, dim3(1), dim3(NT), 0, s, st, partials, (unsigned)kVecBlocks);
+void launch_fcg_direction(long n, double a, double b, const double* W, const double* AW, const FcgCoef& cf,
+                          const double* P, const double* AP, double* p, double* Ap, hipStream_t s) {
+    hipLaunchKernelGGL(k_fcg_direction, dim3(kVecBlocks), dim3(NT), 0, s, n, a, b, W, AW, cf, P, AP, p, Ap);
 }
-void launch_fpcg_beta(long n, const double* r, const double* z, const double* rold, PcgState* st, double* partials,
-                      int first, hipStream_t s) {
-    hipLaunchKernelGGL(k_fpcg_beta, dim3(kVecBlocks), dim3(NT), 0, s, n, r, z, rold, st, partials, first);
-    hipLaunchKernelGGL(k_fin_fpcg_beta, dim3(1), dim3(NT), 0, s, st, partials, (unsigned)kVecBlocks, first);
+void launch_fcg_step(long n, double alpha, double* x, const double* p, double* r, const double* Ap, const double* minv,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_fcg_step, dim3(kVecBlocks), dim3(NT), 0, s, n, alpha, x, p, r, Ap, minv);
 }
-void launch_fpcg_pupdate(long n, const double* z, double* p, const double* r, double* rold, const PcgState* st,
-                         hipStream_t s) {
-    hipLaunchKernelGGL(k_fpcg_pupdate, dim3(kVecBlocks), dim3(NT), 0, s, n, z, p, r, rold, st);
+void launch_mul(long n, const double* a, const double* b, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_mul, dim3(kVecBlocks), dim3(NT), 0, s, n, a, b, out);
 }
 void launch_masked_diff(long n, const double* a, const double* b, const double* minv, double* e, hipStream_t s) {
     hipLaunchKernelGGL(k_masked_diff, dim3(kVecBlocks), dim3(NT), 0, s, n, a, b, minv, e);

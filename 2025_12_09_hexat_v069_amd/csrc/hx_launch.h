@@ -103,16 +103,25 @@ void launch_mask(long n, const T* minv, T* x, hipStream_t s, int nrhs = 1);
 // x64 += x32; x32 = 0
 void launch_accumulate(long n, double* x64, float* x32, hipStream_t s, int nrhs = 1);
 
-// ---- flexible PCG pieces (learned preconditioner) ---------------------------
-void launch_fpcg_update(long n, double* x, const double* p, double* r, const double* q, const double* minv,
-                        PcgState* st, double* partials, hipStream_t s);
-// beta = (r_k.z_k - z_k.r_{k-1}) / rz (flexible Polak-Ribiere), rz = r_k.z_k
-void launch_fpcg_beta(long n, const double* r, const double* z, const double* rold, PcgState* st, double* partials,
-                      int first, hipStream_t s);
-// p = z + beta p ; rold = r
-void launch_fpcg_pupdate(long n, const double* z, double* p, const double* r, double* rold, const PcgState* st,
-                         hipStream_t s);
+// ---- flexible CG pieces (learned preconditioner) ----------------------------
+// Truncated flexible CG: the new direction is A-orthogonalised against the
+// last m (<= kFcgKeep) stored ones, P_j = P[slot[j]], AP_j = AP[slot[j]].
+constexpr int kFcgKeep = 8;
+struct FcgCoef {
+    int m;
+    int slot[kFcgKeep];
+    double c[kFcgKeep];
+};
+// W = [u | y], AW = [Au | Ay] (two columns of n): p = a u + b y - sum c_j P_j,
+// Ap = a Au + b Ay - sum c_j AP_j
+void launch_fcg_direction(long n, double a, double b, const double* W, const double* AW, const FcgCoef& cf,
+                          const double* P, const double* AP, double* p, double* Ap, hipStream_t s);
+// x += alpha p ; r = free ? r - alpha Ap : 0
+void launch_fcg_step(long n, double alpha, double* x, const double* p, double* r, const double* Ap, const double* minv,
+                     hipStream_t s);
 void launch_masked_diff(long n, const double* a, const double* b, const double* minv, double* e, hipStream_t s);
+// out = a * b (elementwise)
+void launch_mul(long n, const double* a, const double* b, double* out, hipStream_t s);
 
 // ---- Helmholtz filter --------------------------------------------------------
 struct HelmArgs {

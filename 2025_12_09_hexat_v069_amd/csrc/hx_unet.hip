@@ -130,38 +130,20 @@ __global__ void k_prolong_adjoint(const float* __restrict__ gf, Vol vf, Vol vc, 
     }
 }
 
-// z = minv r + scale * y (fine, 3 ch) on free DOFs
-__global__ void k_precond_combine(long nn, const float* __restrict__ yf, const double* __restrict__ r,
-                                  const double* __restrict__ minv, double scale, double* __restrict__ z) {
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < 3 * nn; i += (long)gridDim.x * 256) {
-        const double m = minv[i];
-        z[i] = m != 0.0 ? m * r[i] + scale * (double)yf[i] : 0.0;
-    }
+// y (fine network output, FP32) -> FP64 direction on free DOFs, times scale
+__global__ void k_direction(long nn, const float* __restrict__ yf, const double* __restrict__ minv, double scale,
+                            double* __restrict__ y) {
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < 3 * nn; i += (long)gridDim.x * 256)
+        y[i] = minv[i] != 0.0 ? scale * (double)yf[i] : 0.0;
 }
 
-// d = y - (e - minv r) * inv_scale on free DOFs, 0 elsewhere
-__global__ void k_energy_diff(long nn, const float* __restrict__ yf, const double* __restrict__ e,
-                              const double* __restrict__ r, const double* __restrict__ minv, double inv_scale,
-                              double* __restrict__ d) {
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < 3 * nn; i += (long)gridDim.x * 256) {
-        const double m = minv[i];
-        d[i] = m != 0.0 ? (double)yf[i] - (e[i] - m * r[i]) * inv_scale : 0.0;
-    }
-}
-
-__global__ void k_energy_grad(long nn, const double* __restrict__ d, const double* __restrict__ Ad,
-                              const double* __restrict__ minv, double w, float* __restrict__ gf, double* acc) {
-    __shared__ double red[4];
-    double s = 0.0;
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < 3 * nn; i += (long)gridDim.x * 256) {
-        const bool fr = minv[i] != 0.0;
-        gf[i] = fr ? (float)(2.0 * w * Ad[i]) : 0.f;
-        s += fr ? d[i] * Ad[i] : 0.0;
-    }
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(acc, (red[0] + red[1] + red[2] + red[3]) * w);
+// dL/dy = -2 b (r - a Au - b Ay) / e'r on free DOFs (envelope of the 2x2
+// energy minimisation over span{D^-1 r, y})
+__global__ void k_subspace_grad(long nn, const double* __restrict__ r, const double* __restrict__ Au,
+                                const double* __restrict__ Ay, const double* __restrict__ minv, double a, double b,
+                                double scale, float* __restrict__ gf) {
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < 3 * nn; i += (long)gridDim.x * 256)
+        gf[i] = minv[i] != 0.0 ? (float)(scale * (r[i] - a * Au[i] - b * Ay[i])) : 0.f;
 }
 
 unsigned gb(long n) { return (unsigned)std::min<long>((n + 255) / 256, 8192); }
@@ -246,14 +228,6 @@ struct UNet {
         (void)hipStreamSynchronize(s);
         adam_t = 0;
         ready = true;
-    }
-
-    void zero_output_layer(hipStream_t s) {
-        const size_t l = L.size() - 1;
-        const long nw = wsize(l) + L[l].cout;  // weights then bias
-        launch_fillf(nw, 0.f, w32.p + L[l].woff, s);
-        (void)hipMemsetAsync(wbf.p + L[l].woff, 0, nw * sizeof(uint16_t), s);
-        (void)hipStreamSynchronize(s);
     }
 
     ConvArgs conv(int l, const uint16_t* in0, Vol vc0, const uint16_t* in1, Vol vin, Vol vout, uint16_t* out_bf,
@@ -405,45 +379,29 @@ void unet_destroy(UNet* u) { delete u; }
 long unet_nparam(const UNet* u) { return u->nparam; }
 bool unet_ready(const UNet* u) { return u && u->ready; }
 
-bool unet_bind(UNet* u, const GridArgs& g, unsigned seed, hipStream_t s, bool zero_output) {
+bool unet_bind(UNet* u, const GridArgs& g, unsigned seed, hipStream_t s) {
     if (!u->alloc_params()) return false;
     if (!u->bind(g)) return false;
-    if (!u->ready) {
-        u->init_weights(seed, s);
-        // a fresh preconditioner network starts as Jacobi: the output layer
-        // is zero, so z = D^-1 r until training moves it
-        if (zero_output) u->zero_output_layer(s);
-    }
+    if (!u->ready) u->init_weights(seed, s);
     return true;
 }
 
-// z = minv r + |r|_rms * net(r, E): the preconditioner application.
-void unet_apply(UNet* u, const double* r, const double* E, const double* minv, double rms, double* z,
-                hipStream_t s) {
-    const double inv = rms > 0.0 ? 1.0 / rms : 0.0;
-    launch_pack_input(u->g, r, E, inv, u->x0.p, s);
-    u->forward(s);
-    hipLaunchKernelGGL(k_precond_combine, dim3(gb(3 * u->g.nn)), dim3(256), 0, s, u->g.nn, u->yf.p, r, minv, rms, z);
-}
-
-// Energy-norm training pieces.  With z = D^-1 r + rms*y and the sample's
-// error e, z - e = rms*d where d = y - (e - D^-1 r)/rms on free DOFs; the
-// loss is |z - e|_A^2 / (e'r) (e'r = |e|_A^2 since A e = r), so the caller
-// applies the operator to d and passes A d back for the gradient.
+// Training pieces for the subspace objective: the caller forms the network's
+// direction y, applies the operator and solves the 2x2 energy problem over
+// span{D^-1 r, y}, then hands the output gradient back.
 void unet_forward_sample(UNet* u, const double* r, const double* E, double inv_scale, hipStream_t s) {
     launch_pack_input(u->g, r, E, inv_scale, u->x0.p, s);
     u->forward(s);
 }
-void unet_energy_diff(UNet* u, const double* e, const double* r, const double* minv, double inv_scale, double* d,
-                      hipStream_t s) {
-    hipLaunchKernelGGL(k_energy_diff, dim3(gb(3 * u->g.nn)), dim3(256), 0, s, u->g.nn, u->yf.p, e, r, minv, inv_scale,
-                       d);
+// y = scale * net output on free DOFs
+void unet_direction(UNet* u, const double* minv, double scale, double* y, hipStream_t s) {
+    hipLaunchKernelGGL(k_direction, dim3(gb(3 * u->g.nn)), dim3(256), 0, s, u->g.nn, u->yf.p, minv, scale, y);
 }
-// loss += w d'(A d); dL/dy = 2 w (A d) on free DOFs, w = rms^2 / (e'r)
-void unet_energy_grad(UNet* u, const double* d, const double* Ad, const double* minv, double w, double* loss_acc,
-                      hipStream_t s) {
-    hipLaunchKernelGGL(k_energy_grad, dim3(gb(3 * u->g.nn)), dim3(256), 0, s, u->g.nn, d, Ad, minv, w, u->gyf.p,
-                       loss_acc);
+// output gradient: scale * (r - a Au - b Ay) on free DOFs
+void unet_subspace_grad(UNet* u, const double* r, const double* Au, const double* Ay, const double* minv, double a,
+                        double b, double scale, hipStream_t s) {
+    hipLaunchKernelGGL(k_subspace_grad, dim3(gb(3 * u->g.nn)), dim3(256), 0, s, u->g.nn, r, Au, Ay, minv, a, b, scale,
+                       u->gyf.p);
 }
 void unet_backward_step(UNet* u, float lr, hipStream_t s) {
     u->backward(s);

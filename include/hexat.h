@@ -183,14 +183,18 @@ int hx_to_read(hx_ctx* ctx, int field, double* out);
 
 /* ---- learned preconditioner (BF16 3-D conv U-Net) --------------------------- */
 /* Train the network on (density, residual, error) triples that the Jacobi-PCG
- * of the current grid generates; loss, epochs and timings in the stats. */
+ * of the current grid generates; loss, epochs and timings in the stats.  The
+ * loss is the fraction of the error's energy |e|_A^2 left outside the best
+ * combination of D^-1 r and the network's direction (0 = exact, 1 = none);
+ * jacobi_loss is the same fraction for D^-1 r alone on the same samples. */
 typedef struct {
     int samples;      /* (rho, r, e) triples to harvest */
     int epochs;
     double lr;
     int seed;
-    double final_loss;
+    double final_loss;   /* mean loss of the last epoch */
     double seconds;
+    double jacobi_loss;  /* mean loss of D^-1 r alone */
 } hx_learn_stats;
 int hx_learned_train(hx_ctx* ctx, const double* Ee, const double* f, hx_learn_stats* stats);
 /* copy of the current weights (BF16 bit patterns) and their count */
