@@ -283,3 +283,56 @@ def test_fp32_contraction_variants_solve_to_fp64_tolerance(ox, hx, ctx, variant)
     assert s.status == 0
     assert _true_rel(ox, g, Ke, Ee, fixed, f, u) < 1.05e-8
     assert abs(f @ u - f @ uo) < 1e-7 * abs(f @ uo)
+
+
+SHAPES = [(0, 8, 1), (0, 16, 2), (0, 32, 1), (1, 8, 1), (1, 16, 2)]
+
+
+def _tune(tile, kchunk, pdepth):
+    return {"tile": tile, "kchunk": kchunk, "pdepth": pdepth, "tile32": tile, "kchunk32": kchunk,
+            "pdepth32": pdepth, "variant64": 1, "variant32": 1}
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_operator_launch_shapes_match_oracle(ox, hx, ctx, shape):
+    """Every launch shape the autotuner can pick (16x16 / 16x32 tiles, z-chunks,
+    one or two prefetched planes) applies the operator to FP64 round-off and
+    drives both PCG forms (fused p-update) to the FP64 tolerance."""
+    _say("operator shape", shape)
+    for dims in ((17, 16, 18), (31, 9, 40)):
+        g = _grid_setup(ox, ctx, dims)
+        ctx.tune_set(_tune(*shape))
+        rng = np.random.default_rng(11)
+        Ee = rng.uniform(0.01, 2.0, g.ne)
+        Ee[rng.random(g.ne) < 0.1] = 0.0
+        x = rng.standard_normal(g.ndof)
+        yo = ox.matvec(g, ox.ke_hex(1, 1, 1, 0.3), Ee, x)
+        assert np.allclose(ctx.matvec(Ee, x, 1), yo, rtol=1e-12, atol=1e-12 * np.abs(yo).max())
+    g = _grid_setup(ox, ctx, (24, 8, 8))
+    ctx.tune_set(_tune(*shape))
+    fixed, f = _cantilever(ox, g)
+    ctx.set_fixed(fixed)
+    Ee = 1e-9 + np.clip(np.random.default_rng(3).uniform(0, 1, g.ne), 0.01, 1.0) ** 3
+    Ke = ox.ke_hex(1, 1, 1, 0.3)
+    for precision in (0, 1):
+        u, st = ctx.pcg(Ee, f, tol=1e-8, precision=precision)
+        assert st.status == 0 and _true_rel(ox, g, Ke, Ee, fixed, f, u) < 1.05e-8, (precision, st.rel_res)
+
+
+def test_tune_set_rejects_invalid_shapes(hx, ctx):
+    with pytest.raises(hx.HexatError):
+        ctx.tune_set(_tune(5, 16, 1))
+    with pytest.raises(hx.HexatError):
+        ctx.tune_set(_tune(0, 16, 3))
+
+
+def test_timing_hooks(ox, hx, ctx):
+    """Operator timing (both precisions, cache-defeating buffer sets) and the
+    copy-kernel calibration run and report positive durations."""
+    _grid_setup(ox, ctx, (32, 16, 16))
+    ctx.tune_set(_tune(0, 8, 1))
+    assert ctx.time_operator(hx._lib.HX_FP64, 1, 4) > 0.0
+    assert ctx.time_operator(hx._lib.HX_FP32_IR, 2, 4) > 0.0
+    assert ctx.test_copy(1 << 20, 2) > 0.0
+    with pytest.raises(hx.HexatError):
+        ctx.test_copy(1000, 1)  # not a multiple of 16 bytes
