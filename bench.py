@@ -59,6 +59,93 @@ def cantilever(nx, ny, nz):
     return fixed, f
 
 
+def mbb_half(nx, ny, nz):
+    """Half MBB beam in the reference's location grammar: symmetry rollers
+    [0, ':', ':'] DoF 1, support line [100%, 0, ':'] DoF 2, z-symmetry
+    [':', ':', 0] DoF 3; unit load [0, 100%, 50%] F (0, -1, 0)."""
+    NX, NY, NZ = nx + 1, ny + 1, nz + 1
+    k, j, i = np.meshgrid(np.arange(NZ), np.arange(NY), np.arange(NX), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    fixed = np.zeros((NX * NY * NZ, 3), dtype=np.uint8)
+    fixed[i == 0, 0] = 1
+    fixed[(i == nx) & (j == 0), 1] = 1
+    fixed[k == 0, 2] = 1
+    f = np.zeros(3 * NX * NY * NZ)
+    top = int(np.nonzero((i == 0) & (j == ny) & (k == nz // 2))[0][0])
+    f[3 * top + 1] = -1.0
+    return fixed.ravel(), f
+
+
+def smoothed_density(ctx, ne, seed, R):
+    """rho ~ U(0.01, 1), one Helmholtz filter pass of radius R (on the device)."""
+    rho = np.random.default_rng(seed).uniform(0.01, 1.0, ne)
+    out, _ = ctx.helmholtz(R / (2 * math.sqrt(3)), rho, quirk=0, tol=1e-8, max_iter=2000, check_every=1)
+    return np.clip(out, 0.01, 1.0)
+
+
+def other_workload(args, hx, L, ctx):
+    """Solver-only measurements of the other configurations (not the headline
+    metric): 'mbb' -- FP64 vs precision-form PCG on the 192x96x96 half MBB
+    beam; 'block8' -- 8 load cases on 128x64x64 as one block solve vs eight
+    single solves (precision form)."""
+    out = {"workload": args.workload}
+    if args.workload == "mbb":
+        nx, ny, nz = 192, 96, 96
+        fixed, f = mbb_half(nx, ny, nz)
+        ctx.set_grid(nx, ny, nz, 1.0, 1.0, 1.0, 0.3)
+        ctx.set_fixed(fixed)
+        ctx.tune_set(ctx.autotune(1))
+        Ee = 1e-9 + smoothed_density(ctx, nx * ny * nz, 1, 2.0) ** 3
+        for name, prec in (("plain", L.HX_FP64), ("precision", L.HX_FP32_IR)):
+            ctx.pcg(Ee, f, tol=1e-8, precision=prec)  # warm-up (graphs)
+            t0 = time.perf_counter()
+            u, st = ctx.pcg(Ee, f, tol=1e-8, precision=prec)
+            out[name] = {"s": time.perf_counter() - t0, "iterations": st.iterations, "rel_res": st.rel_res,
+                         "compliance": float(f @ u), "refinements": st.refinements}
+            log(f"  {name}: {out[name]}")
+        out["speedup"] = out["plain"]["s"] / out["precision"]["s"]
+        out["compliance_rel_diff"] = abs(out["plain"]["compliance"] - out["precision"]["compliance"]) / abs(
+            out["plain"]["compliance"])
+        out["config"] = "half MBB 192x96x96 (5,447,811 DOFs), smoothed random density, SIMP p=3, tol 1e-8"
+    elif args.workload == "block8":
+        nx, ny, nz = 128, 64, 64
+        NX, NY, NZ = nx + 1, ny + 1, nz + 1
+        fixed, _ = cantilever(nx, ny, nz)
+        ctx.set_grid(nx, ny, nz, 1.0, 1.0, 1.0, 0.3)
+        ctx.set_fixed(fixed)
+        rng = np.random.default_rng(8)
+        F = np.zeros((8, 3 * NX * NY * NZ))
+        picks = rng.choice(NY * NZ, size=8, replace=False)  # 8 distinct nodes of the x = nx face
+        for c in range(8):
+            j, k = int(picks[c] % NY), int(picks[c] // NY)
+            d = rng.standard_normal(3)
+            F[c, 3 * (nx + NX * (j + NY * k)):3 * (nx + NX * (j + NY * k)) + 3] = d / np.linalg.norm(d)
+        Ee = 1e-9 + smoothed_density(ctx, nx * ny * nz, 3, 2.0) ** 3
+        ctx.tune_set(ctx.autotune(8))
+        ctx.block_pcg(Ee, F, tol=1e-8, precision=L.HX_FP32_IR)  # warm-up
+        t0 = time.perf_counter()
+        U, S = ctx.block_pcg(Ee, F, tol=1e-8, precision=L.HX_FP32_IR)
+        tb = time.perf_counter() - t0
+        ctx.tune_set(ctx.autotune(1))
+        ctx.pcg(Ee, F[0], tol=1e-8, precision=L.HX_FP32_IR)  # warm-up
+        t0 = time.perf_counter()
+        cs, its = [], []
+        for c in range(8):
+            u, st = ctx.pcg(Ee, F[c], tol=1e-8, precision=L.HX_FP32_IR)
+            cs.append(float(F[c] @ u))
+            its.append(st.iterations)
+        ts = time.perf_counter() - t0
+        wc_block = sum(float(F[c] @ U[c]) for c in range(8)) / 8
+        out.update({"block_s": tb, "single_s": ts, "speedup": ts / tb, "block_iterations": [s.iterations for s in S],
+                    "single_iterations": its, "weighted_compliance": wc_block,
+                    "weighted_compliance_rel_diff": abs(wc_block - sum(cs) / 8) / abs(wc_block),
+                    "max_rel_res": max(s.rel_res for s in S),
+                    "config": "128x64x64 (1,635,075 DOFs), 8 unit loads on the x=128 face, clamped x=0, smoothed "
+                              "random density, precision form, tol 1e-8"})
+        log(f"  block8: {out}")
+    print(json.dumps(out), flush=True)
+
+
 def run_form(hx, L, ctx, nx, ny, nz, f, fixed, precision, steps, warmup, args, tune=None):
     ctx.set_grid(nx, ny, nz, 1.0, 1.0, 1.0, 0.3)
     ctx.set_fixed(fixed)
@@ -249,6 +336,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-pcg-iters", type=int, default=20)
     ap.add_argument("--cpu-filter-iters", type=int, default=20)
+    ap.add_argument("--workload", choices=("cantilever", "mbb", "block8"), default="cantilever",
+                    help="cantilever: the headline TO-iteration bench; mbb / block8: solver-only measurements of "
+                         "the precision and batching configurations (their own JSON line)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes (roofline.traffic = null)")
     ap.add_argument("--kernel-sample", choices=("tune", "plain", "precision"), default=None,
@@ -264,6 +354,11 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus N>1 is not supported (the path is designed "
                                    "to shard by x-slabs with halo exchange, see DESIGN.md)"}))
         sys.exit(2)
+    if args.workload != "cantilever":
+        hx = importlib.import_module("2025_12_09_hexat_v069_amd")
+        ctx = hx.Context(0)
+        other_workload(args, hx, hx._lib, ctx)
+        return
     nx, ny, nz = (int(v) for v in args.grid.lower().split("x"))
     forms = [s.strip() for s in args.forms.split(",") if s.strip()]
     main_form = "precision" if "precision" in forms else forms[0]
