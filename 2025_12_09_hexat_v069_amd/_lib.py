@@ -28,9 +28,9 @@ HX_ERR_OOM = 9
 HX_ZERO_RHS = 10
 
 HX_FP64, HX_FP32_IR = 0, 1
-HX_PC_JACOBI, HX_PC_LEARNED = 0, 1
+HX_PC_JACOBI, HX_PC_LEARNED, HX_PC_MULTILEVEL = 0, 1, 2
 HX_MV_AUTO, HX_MV_PARITY, HX_MV_DENSE, HX_MV_GATHER, HX_MV_MFMA = 0, 1, 2, 3, 4
-HX_K_MATVEC, HX_K_UPDATE, HX_K_PUPDATE, HX_K_HELM, HX_K_ELEM, HX_K_MATVEC32 = range(6)
+HX_K_MATVEC, HX_K_UPDATE, HX_K_PUPDATE, HX_K_HELM, HX_K_ELEM, HX_K_MATVEC32, HX_K_PRECOND = range(7)
 HX_FIELD_X, HX_FIELD_XPHYS, HX_FIELD_U, HX_FIELD_VM, HX_FIELD_DC = range(5)
 
 STATUS_NAMES = {0: "ok", 1: "no device", 2: "bad argument", 3: "not converged", 4: "breakdown",
@@ -47,7 +47,7 @@ EXPORTS = [
 ]
 #: test hooks declared in include/hexat_testing.h
 TEST_EXPORTS = ["hx_test_unet_nparam", "hx_test_unet_init", "hx_test_unet_forward", "hx_test_unet_backward",
-                "hx_test_copy"]
+                "hx_test_copy", "hx_test_ml_apply"]
 
 
 class HexatError(RuntimeError):
@@ -147,6 +147,7 @@ def load():
         L.hx_tune_set.argtypes = [vp, P(TuneResult)]
         L.hx_time_operator.argtypes = [vp, i, i, i, P(C.c_double)]
         L.hx_test_copy.argtypes = [vp, C.c_long, i, P(C.c_double)]
+        L.hx_test_ml_apply.argtypes = [vp, pd, pd, pd, i, P(C.c_int)]
         pf = P(C.c_float)
         L.hx_test_unet_nparam.argtypes = [vp, P(C.c_long)]
         L.hx_test_unet_init.argtypes = [vp, i, pf]
@@ -421,6 +422,15 @@ class Context:
         ms = C.c_double(0)
         self.check(self.L.hx_test_copy(self.h, int(nbytes), int(reps), C.byref(ms)), "hx_test_copy")
         return ms.value
+
+    def test_ml_apply(self, Ee, r, precision=HX_FP64):
+        """z = M^-1 r of the multilevel preconditioner; returns (z, coarse levels)."""
+        Ee, r = f64(Ee, self.ne), f64(r, self.ndof)
+        z = np.empty(self.ndof)
+        lv = C.c_int(0)
+        self.check(self.L.hx_test_ml_apply(self.h, dptr(Ee), dptr(r), dptr(z), int(precision), C.byref(lv)),
+                   "hx_test_ml_apply")
+        return z, lv.value
 
     # ---- timing -----------------------------------------------------------------------
     def timing(self, on: bool):

@@ -120,6 +120,13 @@ struct hx_ctx {
     DevBuf<double> l_z, l_zold, l_P, l_AP, l_dots, l_ustar, l_samples, l_rho;
     int l_nsamples = 0;
     long l_sample_stride = 0;
+    // multilevel preconditioner: level moduli, inverse diagonals, work vectors
+    int ml_max_levels = kMlMaxLevels;
+    DevBuf<double> ml_E[kMlMaxLevels + 1], ml_r[kMlMaxLevels + 1], ml_y[kMlMaxLevels + 1],
+        ml_minv[kMlMaxLevels + 1], ml_z;
+    DevBuf<float> ml_r32[kMlMaxLevels + 1], ml_y32[kMlMaxLevels + 1], ml_minv32[kMlMaxLevels + 1], ml_z32;
+    MlHierarchy<double> ml64;
+    MlHierarchy<float> ml32;
     // captured iteration batches (hipGraph), keyed by kernel arguments
     bool graphs_on = true;
     std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
@@ -338,6 +345,54 @@ int worst_code(const hx_ctx* c, int nrhs) {
     return rc;
 }
 
+// Levels of the multilevel preconditioner for the current moduli: coarse
+// moduli by 2x2x2 averaging, coarse inverse diagonals, work vectors.
+int ml_setup(hx_ctx* c, bool f32) {
+    const int L = ml_levels_for(c->g, std::min(c->ml_max_levels, kMlMaxLevels));
+    const long n = ndof(c);
+    MlHierarchy<double>& h = c->ml64;
+    MlHierarchy<float>& h32 = c->ml32;
+    h.levels = h32.levels = L;
+    h.g[0] = h32.g[0] = c->g;
+    if (f32) HXA(c->ml_z32, n);
+    else HXA(c->ml_z, n);
+    GridArgs gl = c->g;
+    const double* Ef = c->E.p;
+    for (int l = 1; l <= L; ++l) {
+        const GridArgs gc = ml_coarse_grid(gl);
+        const long nc = 3 * gc.nn;
+        HXA(c->ml_E[l], gc.ne);
+        launch_ml_coarsen_E(gl, Ef, gc, c->ml_E[l].p, nullptr, c->stream);
+        const double kdl[3] = {c->kd[0] * (1 << l), c->kd[1] * (1 << l), c->kd[2] * (1 << l)};
+        const uint8_t* fx = c->has_fixed ? c->fixed.p : nullptr;
+        if (f32) {
+            HXA(c->ml_r32[l], nc);
+            HXA(c->ml_y32[l], nc);
+            HXA(c->ml_minv32[l], nc);
+            launch_ml_minv<float>(gc, kdl, c->ml_E[l].p, fx, 1 << l, c->g, c->ml_minv32[l].p, c->stream);
+            h32.g[l] = gc;
+            h32.r[l] = c->ml_r32[l].p;
+            h32.y[l] = c->ml_y32[l].p;
+            h32.minv[l] = c->ml_minv32[l].p;
+            h32.w[l] = 1.0;
+        } else {
+            HXA(c->ml_r[l], nc);
+            HXA(c->ml_y[l], nc);
+            HXA(c->ml_minv[l], nc);
+            launch_ml_minv<double>(gc, kdl, c->ml_E[l].p, fx, 1 << l, c->g, c->ml_minv[l].p, c->stream);
+            h.g[l] = gc;
+            h.r[l] = c->ml_r[l].p;
+            h.y[l] = c->ml_y[l].p;
+            h.minv[l] = c->ml_minv[l].p;
+            h.w[l] = 1.0;
+        }
+        gl = gc;
+        Ef = c->ml_E[l].p;
+    }
+    HXK();
+    return HX_OK;
+}
+
 // ---------------------------------------------------------------- FP64 PCG
 int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& v) {
     const long n = ndof(c);
@@ -346,16 +401,22 @@ int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& 
     batch = batch < 2 ? 2 : (batch & ~1);  // even: the double-buffered p returns to buffer 0
     launch_elastic_minv<double>(c->g, c->kd, c->E.p, c->has_fixed ? c->fixed.p : nullptr, o.shift, c->minv.p,
                                 c->stream);
+    // multilevel: explicit z = M^-1 r after each update; the operator forms
+    // p = z + beta p_old (parity contraction, which has that fused form)
+    const bool ml = o.precond == HX_PC_MULTILEVEL;
+    int rc = ml ? ml_setup(c, false) : HX_OK;
+    if (rc) return rc;
     launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream, nr);
     if (o.warm_start) launch_mask<double>(n, c->minv.p, v.x, c->stream, nr);
     else launch_fill(n * nr, 0.0, v.x, c->stream);
     mv64(c, v.x, v.q, c->E.p, 0.0, MV_PLAIN, o.matvec, nr);
     launch_pcg_init<double>(n, v.f, v.x, v.q, c->minv.p, v.r, v.p, o.shift, c->st, c->partials.p, c->stream, nr);
+    if (ml) launch_ml_apply<double>(c->ml64, v.r, c->minv.p, c->ml_z.p, c->st, c->partials.p, 1, c->stream);
     HXK();
-    int rc = sync_state(c, nr);
-    if (rc) return rc;
-    const std::string key = batch_key(c, "f64", nr, batch, o.matvec == HX_MV_AUTO ? c->mv_variant : o.matvec, o.shift,
-                                      v.x, v.p, v.q, v.r, v.p2);
+    if ((rc = sync_state(c, nr))) return rc;
+    const int mvv = ml ? HX_MV_PARITY : o.matvec;
+    const std::string key = batch_key(c, ml ? "f64ml" : "f64", nr, batch,
+                                      mvv == HX_MV_AUTO ? c->mv_variant : mvv, o.shift, v.x, v.p, v.q, v.r, v.p2);
     // p is double-buffered (v.p, v.p2): each matvec forms p_new = D^-1 r + beta p_old
     // from the plane it stages anyway and writes it beside q; batches are even.
     while (!all_done(c, nr)) {
@@ -363,11 +424,16 @@ int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& 
             for (int b = 0; b < batch; ++b) {
                 double* pin = (b & 1) ? v.p2 : v.p;
                 double* pout = (b & 1) ? v.p : v.p2;
-                mv64(c, pin, v.q, c->E.p, o.shift, MV_PCG, o.matvec, nr, v.r, c->minv.p, pout);
+                if (ml) mv64(c, pin, v.q, c->E.p, o.shift, MV_PCG, HX_MV_PARITY, nr, c->ml_z.p, nullptr, pout);
+                else mv64(c, pin, v.q, c->E.p, o.shift, MV_PCG, o.matvec, nr, v.r, c->minv.p, pout);
                 {
                     TimeScope ts(c, HX_K_UPDATE);
                     launch_pcg_update<double>(n, v.x, pout, v.r, v.q, c->minv.p, c->st, c->partials.p, c->stream,
-                                              nr);
+                                              nr, !ml);
+                }
+                if (ml) {
+                    TimeScope ts(c, HX_K_PRECOND);
+                    launch_ml_apply<double>(c->ml64, v.r, c->minv.p, c->ml_z.p, c->st, c->partials.p, 0, c->stream);
                 }
             }
         });
@@ -421,6 +487,8 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
     launch_elastic_minv<double>(c->g, c->kd, c->E.p, c->has_fixed ? c->fixed.p : nullptr, o.shift, c->minv.p,
                                 c->stream);
     launch_convert<double, float>(n, c->minv.p, c->minv32.p, c->stream);
+    const bool ml = o.precond == HX_PC_MULTILEVEL;  // explicit z = M^-1 r (see pcg_fp64)
+    if (ml && (rc = ml_setup(c, true))) return rc;
     launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream, nr);
     if (o.warm_start) launch_mask<double>(n, c->minv.p, v.x, c->stream, nr);
     else launch_fill(n * nr, 0.0, v.x, c->stream);
@@ -429,22 +497,32 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
     mv64(c, v.x, v.q, c->E.p, 0.0, MV_PLAIN, HX_MV_PARITY, nr);
     launch_true_residual(n, v.f, v.q, v.x, c->minv.p, o.shift, nullptr, v.r32, c->st, c->partials.p, 3, c->stream,
                          nr);
+    if (ml) launch_ml_apply<float>(c->ml32, v.r32, c->minv32.p, c->ml_z32.p, c->st, c->partials.p, 2, c->stream);
     launch_ir_resume(c->st, o.tol, kIrDelta, 1, c->stream, nr);
     HXK();
     if ((rc = sync_state(c, nr))) return rc;
     int refinements = 0;
-    const std::string key = batch_key(c, "f32", nr, batch, v32, o.shift, v.x32, v.p32, v.q32, v.r32, v.p32b);
+    const std::string key = batch_key(c, ml ? "f32ml" : "f32", nr, batch, ml ? HX_MV_PARITY : v32, o.shift, v.x32,
+                                      v.p32, v.q32, v.r32, v.p32b);
     while (!all_done(c, nr)) {
         while (!all_done(c, nr)) {
             rc = run_batch(c, key, [&] {
                 for (int b = 0; b < batch; ++b) {
                     float* pin = (b & 1) ? v.p32b : v.p32;
                     float* pout = (b & 1) ? v.p32 : v.p32b;
-                    mv32(c, pin, v.q32, c->E32.p, o.shift, MV_PCG, v32, nr, v.r32, c->minv32.p, pout);
+                    if (ml)
+                        mv32(c, pin, v.q32, c->E32.p, o.shift, MV_PCG, HX_MV_PARITY, nr, c->ml_z32.p, nullptr, pout);
+                    else
+                        mv32(c, pin, v.q32, c->E32.p, o.shift, MV_PCG, v32, nr, v.r32, c->minv32.p, pout);
                     {
                         TimeScope ts(c, HX_K_UPDATE);
                         launch_pcg_update<float>(n, v.x32, pout, v.r32, v.q32, c->minv32.p, c->st, c->partials.p,
-                                                 c->stream, nr);
+                                                 c->stream, nr, !ml);
+                    }
+                    if (ml) {
+                        TimeScope ts(c, HX_K_PRECOND);
+                        launch_ml_apply<float>(c->ml32, v.r32, c->minv32.p, c->ml_z32.p, c->st, c->partials.p, 0,
+                                               c->stream);
                     }
                 }
             });
@@ -457,6 +535,7 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
         mv64(c, v.x, v.q, c->E.p, 0.0, MV_PLAIN, HX_MV_PARITY, nr);
         launch_true_residual(n, v.f, v.q, v.x, c->minv.p, o.shift, nullptr, v.r32, c->st, c->partials.p, 2,
                              c->stream, nr);
+        if (ml) launch_ml_apply<float>(c->ml32, v.r32, c->minv32.p, c->ml_z32.p, c->st, c->partials.p, 2, c->stream);
         launch_ir_resume(c->st, o.tol, kIrDelta, 0, c->stream, nr);
         HXK();
         ++refinements;
@@ -963,7 +1042,8 @@ static int check_opts(hx_ctx* c, const hx_pcg_opts* o) {
     if (!o) return fail(c, HX_ERR_BAD_ARG, "null options");
     if (!(o->tol > 0) || o->max_iter < 1 || !(o->shift >= 0)) return fail(c, HX_ERR_BAD_ARG, "invalid PCG options");
     if (o->precision != HX_FP64 && o->precision != HX_FP32_IR) return fail(c, HX_ERR_BAD_ARG, "precision");
-    if (o->precond != HX_PC_JACOBI && o->precond != HX_PC_LEARNED) return fail(c, HX_ERR_BAD_ARG, "preconditioner");
+    if (o->precond != HX_PC_JACOBI && o->precond != HX_PC_LEARNED && o->precond != HX_PC_MULTILEVEL)
+        return fail(c, HX_ERR_BAD_ARG, "preconditioner");
     if (o->precond == HX_PC_LEARNED && !c->unet_trained)
         return fail(c, HX_ERR_STATE, "learned preconditioner not trained (hx_learned_train)");
     if (o->matvec < HX_MV_AUTO || o->matvec > HX_MV_MFMA) return fail(c, HX_ERR_BAD_ARG, "matvec variant");
@@ -992,6 +1072,8 @@ int hx_block_pcg_solve(hx_ctx* c, int nrhs, const double* Ee, const double* F, d
     if (int rc = ensure_grid(c)) return rc;
     if (nrhs < 1 || nrhs > kMaxRhs) return fail(c, HX_ERR_BAD_ARG, "nrhs must be 1..8");
     if (int rc = check_opts(c, o)) return rc;
+    if (o->precond == HX_PC_MULTILEVEL && nrhs > 1)
+        return fail(c, HX_ERR_BAD_ARG, "the multilevel preconditioner takes one right-hand side");
     const long n = ndof(c), nt = n * nrhs;
     HXA(c->b_f, nt);
     HXA(c->b_x, nt);
@@ -1140,6 +1222,11 @@ int hx_to_init(hx_ctx* c, const hx_to_params* p, const double* f, const double* 
     if (!(p->penal >= 1) || !(p->volfrac > 0 && p->volfrac < 1) || !(p->rmin >= 0) || !(p->pcg_tol > 0) ||
         p->pcg_max_iter < 1 || !(p->filter_tol > 0) || !(p->oc_rel_tol > 0) || !(p->move > 0))
         return fail(c, HX_ERR_BAD_ARG, "invalid TO parameters");
+    if (p->precision != HX_FP64 && p->precision != HX_FP32_IR) return fail(c, HX_ERR_BAD_ARG, "precision");
+    if (p->precond != HX_PC_JACOBI && p->precond != HX_PC_LEARNED && p->precond != HX_PC_MULTILEVEL)
+        return fail(c, HX_ERR_BAD_ARG, "preconditioner");
+    if (p->precond == HX_PC_LEARNED && !c->unet_trained)
+        return fail(c, HX_ERR_STATE, "learned preconditioner not trained (hx_learned_train)");
     const long ne = c->g.ne, n = ndof(c);
     HXA(c->t_x, ne);
     HXA(c->t_xphys, ne);
@@ -1681,6 +1768,33 @@ int hx_test_copy(hx_ctx* c, long bytes, int reps, double* avg_ms) {
     HXK();
     *avg_ms = (double)ms / reps;
     return HX_OK;
+}
+
+int hx_test_ml_apply(hx_ctx* c, const double* Ee, const double* r, double* z, int precision, int* levels) {
+    if (!c || !Ee || !r || !z) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    if (precision != HX_FP64 && precision != HX_FP32_IR) return fail(c, HX_ERR_BAD_ARG, "precision");
+    const long n = ndof(c);
+    int rc;
+    if ((rc = upload(c, c->E.p, Ee, c->g.ne * sizeof(double))) || (rc = upload(c, c->r.p, r, n * sizeof(double))))
+        return rc;
+    launch_elastic_minv<double>(c->g, c->kd, c->E.p, c->has_fixed ? c->fixed.p : nullptr, 0.0, c->minv.p, c->stream);
+    launch_pcg_reset(c->st, 1e-8, 0.0, 1, 1, c->stream, 1);
+    const bool f32 = precision == HX_FP32_IR;
+    if ((rc = f32 ? ensure_fp32(c) : HX_OK)) return rc;
+    if ((rc = ml_setup(c, f32))) return rc;
+    if (f32) {
+        launch_convert<double, float>(n, c->minv.p, c->minv32.p, c->stream);
+        launch_convert<double, float>(n, c->r.p, c->r32.p, c->stream);
+        launch_ml_apply<float>(c->ml32, c->r32.p, c->minv32.p, c->ml_z32.p, c->st, c->partials.p, 1, c->stream);
+        launch_convert<float, double>(n, c->ml_z32.p, c->q.p, c->stream);
+    } else {
+        launch_ml_apply<double>(c->ml64, c->r.p, c->minv.p, c->ml_z.p, c->st, c->partials.p, 1, c->stream);
+        HXC(hipMemcpyAsync(c->q.p, c->ml_z.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
+    HXK();
+    if (levels) *levels = c->ml64.levels;
+    return download(c, z, c->q.p, n * sizeof(double));
 }
 
 int hx_timing_enable(hx_ctx* c, int on) {

@@ -277,7 +277,9 @@ __device__ __forceinline__ double lane_from_left(double v) {
     return __hiloint2double(hi, lo);
 }
 
-template <typename T, int TY, bool FUSE, int PD>
+// FUSE: 0 plain operator; 1 fused p = D^-1 r + beta p_old; 2 fused p = z + beta p_old
+// with z handed in a.r (an explicit preconditioner's output; a.minv unused)
+template <typename T, int TY, int FUSE, int PD>
 __global__ __launch_bounds__(16 * TY) void k_matvec_sf(GridArgs g, ParityK<T> K, MvArgs<T> a, int kchunk, int ntx,
                                                        int nty, int ntiles) {
     constexpr int TX = 16;  // one tile row = one DPP row
@@ -337,7 +339,7 @@ __global__ __launch_bounds__(16 * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
         // under a branch makes the compiler's wait counts conservative);
         // out-of-range node values only ever meet E = 0.
         struct Slot {
-            T x[3], r[FUSE ? 3 : 1], m[FUSE ? 3 : 1], qx, qr, qm, e;
+            T x[3], r[FUSE ? 3 : 1], m[FUSE == 1 ? 3 : 1], qx, qr, qm, e;
             int k;
         };
         Slot sl[PD];
@@ -348,15 +350,16 @@ __global__ __launch_bounds__(16 * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
 #pragma unroll
             for (int c = 0; c < 3; ++c) q.x[c] = X[off_own + c];
             q.qx = X[off_x];
-            if constexpr (FUSE) {
+            if constexpr (FUSE != 0) {
                 const T* R = a.r + po;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) q.r[c] = R[off_own + c];
+                q.qr = R[off_x];
+            }
+            if constexpr (FUSE == 1) {
                 const T* M = a.minv + po;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    q.r[c] = R[off_own + c];
-                    q.m[c] = M[off_own + c];
-                }
-                q.qr = R[off_x];
+                for (int c = 0; c < 3; ++c) q.m[c] = M[off_own + c];
                 q.qm = M[off_x];
             }
             q.e = a.E[(long)min(max(k - 1, 0), g.nz - 1) * g.nx * g.ny + eoff];
@@ -368,13 +371,15 @@ __global__ __launch_bounds__(16 * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 T v;
-                if constexpr (FUSE) v = q.m[c] * q.r[c] + beta * q.x[c];  // p = D^-1 r + beta p
+                if constexpr (FUSE == 1) v = q.m[c] * q.r[c] + beta * q.x[c];  // p = D^-1 r + beta p
+                else if constexpr (FUSE == 2) v = q.r[c] + beta * q.x[c];        // p = z + beta p
                 else v = q.x[c];
                 xs[c][ty][tx] = v;
                 xnew[c] = v;
             }
             T v;
-            if constexpr (FUSE) v = q.qm * q.qr + beta * q.qx;
+            if constexpr (FUSE == 1) v = q.qm * q.qr + beta * q.qx;
+            else if constexpr (FUSE == 2) v = q.qr + beta * q.qx;
             else v = q.qx;
             if (xstore) xs[xc][xty][xtx] = v;
             ee = (ecol && q.k - 1 >= 0 && q.k - 1 < g.nz) ? q.e : T(0);
@@ -636,11 +641,14 @@ int launch_sf(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, const 
     int ntx, nty;
     const int nt = zmarch_tiles<16, TY>(g, sh.kchunk, &ntx, &nty);
     const int nb = round8(nt);
-    if (a.p_out != nullptr)
-        hipLaunchKernelGGL((k_matvec_sf<T, TY, true, PD>), dim3(nb, nrhs), dim3(16 * TY), 0, s, g, K, a, sh.kchunk,
+    if (a.p_out != nullptr && a.minv == nullptr)
+        hipLaunchKernelGGL((k_matvec_sf<T, TY, 2, PD>), dim3(nb, nrhs), dim3(16 * TY), 0, s, g, K, a, sh.kchunk,
+                           ntx, nty, nt);
+    else if (a.p_out != nullptr)
+        hipLaunchKernelGGL((k_matvec_sf<T, TY, 1, PD>), dim3(nb, nrhs), dim3(16 * TY), 0, s, g, K, a, sh.kchunk,
                            ntx, nty, nt);
     else
-        hipLaunchKernelGGL((k_matvec_sf<T, TY, false, PD>), dim3(nb, nrhs), dim3(16 * TY), 0, s, g, K, a, sh.kchunk,
+        hipLaunchKernelGGL((k_matvec_sf<T, TY, 0, PD>), dim3(nb, nrhs), dim3(16 * TY), 0, s, g, K, a, sh.kchunk,
                            ntx, nty, nt);
     return nb;
 }

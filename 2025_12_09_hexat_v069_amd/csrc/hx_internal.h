@@ -147,6 +147,35 @@ struct PcgState {
     unsigned ticket[4];
 };
 
+// Convergence bookkeeping after an r update (GPUSolver.jl:472-511): record
+// ||r|| and the iteration, then decide: instability (:480-484), explosion
+// (:486-490), convergence at a check iteration, iteration cap.
+__device__ inline void pcg_record(PcgState* st, double rr) {
+    st->iter += 1;
+    st->rr = rr;
+    st->rel = sqrt(rr) / st->normb;
+}
+__device__ inline void pcg_decide(PcgState* st) {
+    const int it = st->iter;
+    const double rel = st->rel;
+    if (it > 50 && rel > st->best_rel * 100.0) {
+        st->done = 1;
+        st->status = 5;
+    } else if (!(rel == rel) || isinf(rel) || rel > 1000.0) {
+        st->done = 1;
+        st->status = 5;
+    } else {
+        if (rel < st->best_rel) st->best_rel = rel;
+        if (rel < st->tol && (it % st->check_every) == 0) {
+            st->done = 1;
+            st->status = 0;
+        } else if (it >= st->max_iter) {
+            st->done = 1;
+            st->status = 3;
+        }
+    }
+}
+
 // Grid description passed by value to every kernel.
 struct GridArgs {
     int nx, ny, nz;
@@ -198,22 +227,57 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
     return r;
 }
 
-// Sum partials[j + nv*b] over b in fixed order; result valid in thread 0.
-// Partials come from an earlier kernel on the same stream, so plain loads see
-// them (the kernel boundary orders them); no fences or tickets are used --
-// an agent-scope fence writes back the XCD's L2 on gfx950 and costs far more
-// than the extra one-block launch.
+// Sum partials[nv*b + j] over b, j = 0..NV-1, in a fixed order; results valid
+// in thread 0.  Partials come from an earlier kernel on the same stream, so
+// plain loads see them (the kernel boundary orders them); no fences or tickets
+// are used -- an agent-scope fence writes back the XCD's L2 on gfx950 and
+// costs far more than the extra one-block launch.  Each thread issues the
+// loads of kUnroll blocks before adding any of them, so the pass costs about
+// one memory latency per kUnroll*NT blocks instead of one per NT; the per-
+// thread order (b = tid, tid+NT, ...) is the plain loop's.
+constexpr int kUnroll = 8;
+template <int NT, int NV>
+__device__ __forceinline__ void reduce_partials_n(const double* partials, int nv, unsigned nblocks, double* sh,
+                                                  double (&out)[NV]) {
+    double s[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) s[j] = 0.0;
+    for (unsigned b0 = 0; b0 < nblocks; b0 += kUnroll * NT) {
+        double t[kUnroll][NV];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const unsigned b = b0 + u * NT + threadIdx.x;
+#pragma unroll
+            for (int j = 0; j < NV; ++j) t[u][j] = b < nblocks ? partials[(size_t)nv * b + j] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u)
+#pragma unroll
+            for (int j = 0; j < NV; ++j) s[j] += t[u][j];
+    }
+#pragma unroll
+    for (int j = 0; j < NV; ++j) out[j] = block_sum<NT>(s[j], sh);
+}
 template <int NT>
 __device__ __forceinline__ double reduce_partials(const double* partials, int nv, int j, unsigned nblocks,
                                                   double* sh) {
-    double s = 0.0;
-    for (unsigned b = threadIdx.x; b < nblocks; b += NT) s += partials[(size_t)nv * b + j];
-    return block_sum<NT>(s, sh);
+    double r[1];
+    reduce_partials_n<NT, 1>(partials + j, nv, nblocks, sh, r);
+    return r[0];
 }
 template <int NT>
 __device__ __forceinline__ double reduce_partials_max(const double* partials, unsigned nblocks, double* sh) {
     double s = -INFINITY;
-    for (unsigned b = threadIdx.x; b < nblocks; b += NT) s = fmax(s, partials[b]);
+    for (unsigned b0 = 0; b0 < nblocks; b0 += kUnroll * NT) {
+        double t[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const unsigned b = b0 + u * NT + threadIdx.x;
+            t[u] = b < nblocks ? partials[b] : -INFINITY;
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) s = fmax(s, t[u]);
+    }
     return block_max<NT>(s, sh);
 }
 

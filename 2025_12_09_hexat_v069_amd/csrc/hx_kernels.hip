@@ -1,7 +1,7 @@
 // hx_kernels.hip -- streaming kernels of the PCG recurrences, the Helmholtz
 // filter, the element post-processing (stress, energies, sensitivities) and
 // the OC update.  All reductions are two-stage and deterministic: per-block
-// partials in a fixed tree, then the last block (ticket counter) sums the
+// partials in a fixed tree, then a one-block finaliser kernel sums the
 // partials in block order and updates the device-side PcgState, so no scalar
 // ever returns to the host inside a solve (the reference pays three
 // blocking CUBLAS dots per iteration, GPUSolver.jl:460,472,522).
@@ -15,6 +15,15 @@ constexpr int NT = kRedThreads;
 
 __device__ __forceinline__ long gtid() { return (long)blockIdx.x * NT + threadIdx.x; }
 __device__ __forceinline__ long gstride() { return (long)gridDim.x * NT; }
+
+// 16-byte vectors of the streaming kernels: 4 floats or 2 doubles per load,
+// so one wave keeps 1 KiB per array in flight instead of 256 B.
+template <typename T>
+struct Vec16 {
+    static constexpr int n = 16 / sizeof(T);
+    typedef T type __attribute__((ext_vector_type(16 / sizeof(T))));
+};
+__device__ __forceinline__ bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
 __global__ void k_pcg_reset(PcgState* st0, double tol, double shift, int max_iter, int check_every, int nrhs) {
     if (blockIdx.x == 0 && (int)threadIdx.x < nrhs) {
@@ -75,9 +84,9 @@ __global__ __launch_bounds__(NT) void k_fin_init(PcgState* st0, const double* pa
     __shared__ double red[NT / 64];
     PcgState* st = st0 + blockIdx.x;
     const double* partials = partials0 + (size_t)blockIdx.x * nb * 3;
-    const double s0 = reduce_partials<NT>(partials, 3, 0, nb, red);
-    const double s1 = reduce_partials<NT>(partials, 3, 1, nb, red);
-    const double s2 = reduce_partials<NT>(partials, 3, 2, nb, red);
+    double sv[3];
+    reduce_partials_n<NT, 3>(partials, 3, nb, red, sv);
+    const double s0 = sv[0], s1 = sv[1], s2 = sv[2];
     if (threadIdx.x == 0) {
         st->rz = s0;
         st->normb = sqrt(s1);
@@ -90,29 +99,14 @@ __global__ __launch_bounds__(NT) void k_fin_init(PcgState* st0, const double* pa
     }
 }
 
-// Convergence bookkeeping after the r update (GPUSolver.jl:472-511).
-__device__ void pcg_after_update(PcgState* st, double rz_new, double rr) {
-    const int it = st->iter + 1;
-    st->iter = it;
-    const double rel = sqrt(rr) / st->normb;
-    st->rr = rr;
-    st->rel = rel;
-    if (it > 50 && rel > st->best_rel * 100.0) {  // instability (:480-484)
-        st->done = 1;
-        st->status = 5;
-    } else if (!(rel == rel) || isinf(rel) || rel > 1000.0) {  // explosion (:486-490)
-        st->done = 1;
-        st->status = 5;
-    } else {
-        if (rel < st->best_rel) st->best_rel = rel;
-        if (rel < st->tol && (it % st->check_every) == 0) {
-            st->done = 1;
-            st->status = 0;
-        } else if (it >= st->max_iter) {
-            st->done = 1;
-            st->status = 3;
-        }
-    }
+// Convergence bookkeeping after the r update (pcg_record / pcg_decide in
+// hx_internal.h); with set_beta the Jacobi r'z also gives beta.  An explicit
+// preconditioner (set_beta = false) records only: its own finaliser sets beta
+// from r'z and then decides, so a phase that stops leaves a consistent beta.
+__device__ void pcg_after_update(PcgState* st, double rz_new, double rr, bool set_beta = true) {
+    pcg_record(st, rr);
+    if (!set_beta) return;
+    pcg_decide(st);
     st->beta = rz_new / st->rz;
     st->rz = rz_new;
 }
@@ -132,15 +126,52 @@ __global__ __launch_bounds__(NT) void k_pcg_update(long n, T* __restrict__ x, co
     if (st->done) return;
     const double alpha = st->alpha;
     double rz = 0.0, rr = 0.0;
-    for (long i = gtid(); i < n; i += gstride()) {
-        const double m = (double)minv[i];
-        const T xt = T((double)x[i] + alpha * (double)p[i]);
-        const T rt = m != 0.0 ? T((double)r[i] - alpha * (double)q[i]) : T(0);
-        x[i] = xt;
-        r[i] = rt;
-        const double rd = (double)rt;
+    auto one = [&](T& xi, T pi, T& ri, T qi, T mi) {
+        const double m = (double)mi;
+        xi = T((double)xi + alpha * (double)pi);
+        ri = m != 0.0 ? T((double)ri - alpha * (double)qi) : T(0);
+        const double rd = (double)ri;
         rz += rd * (m * rd);
         rr += rd * rd;
+    };
+    // Thread t takes the V-element groups t, t + stride, ... then the tail;
+    // minv is shared by all columns, so a column offset can misalign the
+    // others against it: such columns load the same groups element by
+    // element, keeping the summation order (and so the bits) of the
+    // single-column solve.
+    typedef typename Vec16<T>::type VT;
+    constexpr int V = Vec16<T>::n;
+    const long nv = n / V;
+    if (aligned16(x) && aligned16(p) && aligned16(r) && aligned16(q) && aligned16(minv)) {
+        for (long v = gtid(); v < nv; v += gstride()) {
+            VT xv = ((const VT*)x)[v], rv = ((const VT*)r)[v];
+            const VT pv = ((const VT*)p)[v], qv = ((const VT*)q)[v], mv = ((const VT*)minv)[v];
+#pragma unroll
+            for (int l = 0; l < V; ++l) {
+                T xi = xv[l], ri = rv[l];
+                one(xi, pv[l], ri, qv[l], mv[l]);
+                xv[l] = xi;
+                rv[l] = ri;
+            }
+            ((VT*)x)[v] = xv;
+            ((VT*)r)[v] = rv;
+        }
+    } else {
+        for (long v = gtid(); v < nv; v += gstride())
+#pragma unroll
+            for (int l = 0; l < V; ++l) {
+                const long i = v * V + l;
+                T xi = x[i], ri = r[i];
+                one(xi, p[i], ri, q[i], minv[i]);
+                x[i] = xi;
+                r[i] = ri;
+            }
+    }
+    for (long i = nv * V + gtid(); i < n; i += gstride()) {
+        T xi = x[i], ri = r[i];
+        one(xi, p[i], ri, q[i], minv[i]);
+        x[i] = xi;
+        r[i] = ri;
     }
     const double v0 = block_sum<NT>(rz, red);
     const double v1 = block_sum<NT>(rr, red);
@@ -150,14 +181,16 @@ __global__ __launch_bounds__(NT) void k_pcg_update(long n, T* __restrict__ x, co
     }
 }
 
-__global__ __launch_bounds__(NT) void k_fin_update(PcgState* st0, const double* partials0, unsigned nb) {
+__global__ __launch_bounds__(NT) void k_fin_update(PcgState* st0, const double* partials0, unsigned nb,
+                                                   int set_beta) {
     __shared__ double red[NT / 64];
     PcgState* st = st0 + blockIdx.x;
     if (st->done) return;
     const double* partials = partials0 + (size_t)blockIdx.x * nb * 2;
-    const double s0 = reduce_partials<NT>(partials, 2, 0, nb, red);
-    const double s1 = reduce_partials<NT>(partials, 2, 1, nb, red);
-    if (threadIdx.x == 0) pcg_after_update(st, s0, s1);
+    double sv[2];
+    reduce_partials_n<NT, 2>(partials, 2, nb, red, sv);
+    const double s0 = sv[0], s1 = sv[1];
+    if (threadIdx.x == 0) pcg_after_update(st, s0, s1, set_beta != 0);
 }
 
 // p = minv r + beta p
@@ -251,9 +284,9 @@ __global__ __launch_bounds__(NT) void k_fin_true(PcgState* st0, const double* pa
     __shared__ double red[NT / 64];
     PcgState* st = st0 + blockIdx.x;
     const double* partials = partials0 + (size_t)blockIdx.x * nb * 3;
-    const double s0 = reduce_partials<NT>(partials, 3, 0, nb, red);
-    const double s1 = reduce_partials<NT>(partials, 3, 1, nb, red);
-    const double s2 = reduce_partials<NT>(partials, 3, 2, nb, red);
+    double sv[3];
+    reduce_partials_n<NT, 3>(partials, 3, nb, red, sv);
+    const double s0 = sv[0], s1 = sv[1], s2 = sv[2];
     if (threadIdx.x == 0) {
         if (mode & 1) st->normb = sqrt(s2);
         if (mode & 2) st->scal[5] = s0;
@@ -705,9 +738,10 @@ void launch_pcg_init(long n, const T* b, T* x, const T* q, const T* minv, T* r, 
 }
 template <typename T>
 void launch_pcg_update(long n, T* x, const T* p, T* r, const T* q, const T* minv, PcgState* st, double* partials,
-                       hipStream_t s, int nrhs) {
+                       hipStream_t s, int nrhs, bool set_beta) {
     hipLaunchKernelGGL((k_pcg_update<T>), dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, x, p, r, q, minv, st, partials);
-    hipLaunchKernelGGL(k_fin_update, dim3(nrhs), dim3(NT), 0, s, st, partials, (unsigned)kVecBlocks);
+    hipLaunchKernelGGL(k_fin_update, dim3(nrhs), dim3(NT), 0, s, st, partials, (unsigned)kVecBlocks,
+                       set_beta ? 1 : 0);
 }
 template <typename T>
 void launch_pcg_pupdate(long n, const T* r, const T* minv, T* p, const PcgState* st, hipStream_t s, int nrhs) {
@@ -820,9 +854,9 @@ template void launch_pcg_init<double>(long, const double*, double*, const double
 template void launch_pcg_init<float>(long, const float*, float*, const float*, const float*, float*, float*, double,
                                      PcgState*, double*, hipStream_t, int);
 template void launch_pcg_update<double>(long, double*, const double*, double*, const double*, const double*,
-                                        PcgState*, double*, hipStream_t, int);
+                                        PcgState*, double*, hipStream_t, int, bool);
 template void launch_pcg_update<float>(long, float*, const float*, float*, const float*, const float*, PcgState*,
-                                       double*, hipStream_t, int);
+                                       double*, hipStream_t, int, bool);
 template void launch_pcg_pupdate<double>(long, const double*, const double*, double*, const PcgState*, hipStream_t,
                                          int);
 template void launch_pcg_pupdate<float>(long, const float*, const float*, float*, const PcgState*, hipStream_t, int);

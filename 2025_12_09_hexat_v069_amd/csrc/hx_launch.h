@@ -73,9 +73,10 @@ void launch_copy16(long n, const void* src, void* dst, hipStream_t s);
 template <typename T>
 void launch_pcg_init(long n, const T* b, T* x, const T* q, const T* minv, T* r, T* p, double shift,
                      PcgState* st, double* partials, hipStream_t s, int nrhs = 1);
+// set_beta = false leaves beta and rz to an explicit preconditioner's r'z
 template <typename T>
 void launch_pcg_update(long n, T* x, const T* p, T* r, const T* q, const T* minv, PcgState* st,
-                       double* partials, hipStream_t s, int nrhs = 1);
+                       double* partials, hipStream_t s, int nrhs = 1, bool set_beta = true);
 template <typename T>
 void launch_pcg_pupdate(long n, const T* r, const T* minv, T* p, const PcgState* st, hipStream_t s, int nrhs = 1);
 void launch_pcg_reset(PcgState* st, double tol, double shift, int max_iter, int check_every, hipStream_t s,
@@ -122,6 +123,32 @@ void launch_fcg_step(long n, double alpha, double* x, const double* p, double* r
 void launch_masked_diff(long n, const double* a, const double* b, const double* minv, double* e, hipStream_t s);
 // out = a * b (elementwise)
 void launch_mul(long n, const double* a, const double* b, double* out, hipStream_t s);
+
+// ---- multilevel diagonal scaling (explicit preconditioner z = M^-1 r) ---------
+constexpr int kMlMaxLevels = 12;
+template <typename T>
+struct MlHierarchy {
+    int levels = 0;                  // coarse levels L (0: Jacobi only)
+    GridArgs g[kMlMaxLevels + 1];    // g[0] = fine grid, g[l] = level l
+    T* r[kMlMaxLevels + 1] = {};     // restricted residual of level l >= 1
+    T* y[kMlMaxLevels + 1] = {};     // level output D_l^-1 r_l + P y_{l+1}
+    T* minv[kMlMaxLevels + 1] = {};  // level Jacobi inverse diagonal
+    double w[kMlMaxLevels + 1] = {};  // level weights
+};
+// number of 2:1 coarsenings of g (all element counts even), at most max_levels
+int ml_levels_for(const GridArgs& g, int max_levels);
+GridArgs ml_coarse_grid(const GridArgs& g);
+void launch_ml_coarsen_E(const GridArgs& gf, const double* Ef, const GridArgs& gc, double* Ec, float* Ec32,
+                         hipStream_t s);
+template <typename T>
+void launch_ml_minv(const GridArgs& gc, const double kd[3], const double* Ec, const uint8_t* fixed_f, int stride,
+                    const GridArgs& gfine, T* minv, hipStream_t s);
+// z = M^-1 r (masked where minv0 = 0) and r'z into the state: fin_mode 0 = an
+// iteration (beta, rz), 1 = start (rz, beta = 0), 2 = park in scal[5]
+template <typename T>
+void launch_ml_apply(const MlHierarchy<T>& h, const T* r, const T* minv0, T* z, PcgState* st, double* partials,
+                     int fin_mode, hipStream_t s);
+int ml_partial_blocks();
 
 // ---- Helmholtz filter --------------------------------------------------------
 struct HelmArgs {

@@ -146,7 +146,12 @@ def other_workload(args, hx, L, ctx):
     print(json.dumps(out), flush=True)
 
 
-def run_form(hx, L, ctx, nx, ny, nz, f, fixed, precision, steps, warmup, args, tune=None):
+# form -> (precision, preconditioner)
+FORMS = {"plain": ("HX_FP64", "HX_PC_JACOBI"), "precision": ("HX_FP32_IR", "HX_PC_JACOBI"),
+         "plain_ml": ("HX_FP64", "HX_PC_MULTILEVEL"), "precision_ml": ("HX_FP32_IR", "HX_PC_MULTILEVEL")}
+
+
+def run_form(hx, L, ctx, nx, ny, nz, f, fixed, precision, steps, warmup, args, tune=None, precond=0):
     ctx.set_grid(nx, ny, nz, 1.0, 1.0, 1.0, 0.3)
     ctx.set_fixed(fixed)
     if tune is not None:  # the shapes the traffic passes measured
@@ -156,7 +161,7 @@ def run_form(hx, L, ctx, nx, ny, nz, f, fixed, precision, steps, warmup, args, t
         t = ctx.autotune(1)
         log(f"  autotune: {t}")
     p = L.TOParams(E0=1.0, Emin=1e-9, nu=0.3, penal=3.0, volfrac=args.volfrac, move=0.2, xmin=0.0,
-                   rmin=args.rmin, pcg_tol=args.tol, pcg_max_iter=args.max_iter, precision=precision, precond=0,
+                   rmin=args.rmin, pcg_tol=args.tol, pcg_max_iter=args.max_iter, precision=precision, precond=precond,
                    filter_tol=1e-10, filter_max_iter=2000, oc_rel_tol=1e-9, with_stress=1)
     ctx.to_init(p, f, np.full(nx * ny * nz, args.volfrac))
     for w in range(warmup):
@@ -179,7 +184,8 @@ def run_form(hx, L, ctx, nx, ny, nz, f, fixed, precision, steps, warmup, args, t
     s = ctx.to_step()
     kern = {}
     names = {L.HX_K_MATVEC: "matvec_fp64", L.HX_K_MATVEC32: "matvec_fp32", L.HX_K_UPDATE: "pcg_update",
-             L.HX_K_PUPDATE: "pcg_pupdate", L.HX_K_HELM: "helm_matvec", L.HX_K_ELEM: "element_post"}
+             L.HX_K_PUPDATE: "pcg_pupdate", L.HX_K_HELM: "helm_matvec", L.HX_K_ELEM: "element_post",
+             L.HX_K_PRECOND: "ml_precond"}
     for kid, nm in names.items():
         ms, cnt = ctx.timing_read(kid)
         kern[nm] = {"avg_ms": ms, "count": cnt, "total_ms": ms * cnt}
@@ -331,7 +337,10 @@ def main():
     ap.add_argument("--rmin", type=float, default=2.5)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--max-iter", type=int, default=100000)
-    ap.add_argument("--forms", default="plain,precision")
+    ap.add_argument("--forms", default="plain,precision",
+                    help="comma-separated: plain (FP64 Jacobi-PCG), precision (FP32 operator + FP64 refinement), "
+                         "plain_ml / precision_ml (the same with the multilevel preconditioner)")
+    ap.add_argument("--main-form", default=None, help="the form `value` reports (default: precision)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-pcg-iters", type=int, default=20)
@@ -361,10 +370,15 @@ def main():
         return
     nx, ny, nz = (int(v) for v in args.grid.lower().split("x"))
     forms = [s.strip() for s in args.forms.split(",") if s.strip()]
-    main_form = "precision" if "precision" in forms else forms[0]
+    for fm in forms:
+        if fm not in FORMS:
+            sys.exit(f"unknown form {fm!r} (choose from {', '.join(FORMS)})")
+    main_form = args.main_form or ("precision" if "precision" in forms else forms[0])
+    if main_form not in forms:
+        sys.exit(f"--main-form {main_form!r} is not among --forms")
     traffic, tune = (None, json.loads(args.tune_json) if args.tune_json else None)
     if not args.no_traffic:  # before this process initialises the GPU
-        traffic, tune = measure_traffic(args, main_form, tune)
+        traffic, tune = measure_traffic(args, "plain" if FORMS[main_form][0] == "HX_FP64" else "precision", tune)
         log(f"traffic: {traffic}")
     hx = importlib.import_module("2025_12_09_hexat_v069_amd")
     L = hx._lib
@@ -375,9 +389,10 @@ def main():
     ne, ndof = nx * ny * nz, 3 * (nx + 1) * (ny + 1) * (nz + 1)
     res = {}
     for form in forms:
-        prec = L.HX_FP64 if form == "plain" else L.HX_FP32_IR
+        prec, pc = (getattr(L, nm) for nm in FORMS[form])
         log(f"form {form}:")
-        sps, recs, kern = run_form(hx, L, ctx, nx, ny, nz, f, fixed, prec, args.steps, args.warmup, args, tune)
+        sps, recs, kern = run_form(hx, L, ctx, nx, ny, nz, f, fixed, prec, args.steps, args.warmup, args, tune,
+                                   precond=pc)
         res[form] = {"s_per_iter": sps, "recs": recs, "kern": kern}
     r = res[main_form]
     # roofline of the dominant kernel of the main form.  Algorithmic bytes per
@@ -386,8 +401,9 @@ def main():
     # q, D^-1 and writes x, r.
     mv_bytes64 = 40.0 * ndof + 8.0 * ne
     mv_bytes32 = 20.0 * ndof + 4.0 * ne
-    upd_bytes = 56.0 * ndof if main_form == "plain" else 28.0 * ndof
-    pup_bytes = 32.0 * ndof if main_form == "plain" else 16.0 * ndof
+    fp64_main = FORMS[main_form][0] == "HX_FP64"
+    upd_bytes = 56.0 * ndof if fp64_main else 28.0 * ndof
+    pup_bytes = 32.0 * ndof if fp64_main else 16.0 * ndof
     cand = []
     for nm, by in (("matvec_fp64", mv_bytes64), ("matvec_fp32", mv_bytes32), ("pcg_update", upd_bytes),
                    ("pcg_pupdate", pup_bytes)):
@@ -402,8 +418,7 @@ def main():
             avg = r["kern"]["operator_back_to_back"]["avg_ms"]
         ach = by / (avg * 1e-3) / 1e9
         tr = None
-        if traffic and "bytes_per_launch" in traffic and nm == ("matvec_fp32" if main_form == "precision"
-                                                                else "matvec_fp64"):
+        if traffic and "bytes_per_launch" in traffic and nm == ("matvec_fp64" if fp64_main else "matvec_fp32"):
             tr = round(traffic["bytes_per_launch"])
         roof = {"bound": "hbm", "kernel": nm, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": tr, "bytes_per_launch": by,
@@ -418,10 +433,11 @@ def main():
         "metric": METRIC, "value": round(r["s_per_iter"], 4), "unit": "s/TO-iteration", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * r["s_per_iter"], 2),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f64" if main_form == "precision" else "f64", "data": "synthetic",
+        "dtype": "f64" if fp64_main else "f32+f64", "data": "synthetic",
         "config": {"workload": f"cantilever {nx}x{ny}x{nz} ({ne} elements, {ndof} DOFs), SIMP p=3, volfrac "
                                f"{args.volfrac}, PDE filter r={args.rmin}, PCG rel. residual {args.tol}, von Mises "
-                               f"each iteration", "form": main_form, "preconditioner": "jacobi",
+                               f"each iteration", "form": main_form,
+                   "preconditioner": "multilevel" if main_form.endswith("_ml") else "jacobi",
                    "iterations_timed": f"TO iterations {args.warmup + 1}..{args.warmup + args.steps} from uniform "
                                        f"rho={args.volfrac}", "tune": tune},
         "roofline": roof,
@@ -433,9 +449,9 @@ def main():
                       "kernels": {n: {"avg_ms": round(x["avg_ms"], 5), "count": x["count"]}
                                   for n, x in v["kern"].items() if x["count"]}} for k, v in res.items()},
     }
-    if "plain" in res and "precision" in res:
-        out["speedup_vs_plain_fp64"] = round(res["plain"]["s_per_iter"] / res["precision"]["s_per_iter"], 3)
-        cp, cq = res["plain"]["recs"][-1].compliance, res["precision"]["recs"][-1].compliance
+    if "plain" in res and main_form != "plain":
+        out["speedup_vs_plain_fp64"] = round(res["plain"]["s_per_iter"] / res[main_form]["s_per_iter"], 3)
+        cp, cq = res["plain"]["recs"][-1].compliance, res[main_form]["recs"][-1].compliance
         out["compliance_rel_diff_vs_fp64"] = abs(cq - cp) / abs(cp)
     print(json.dumps(out), flush=True)
 

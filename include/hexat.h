@@ -65,7 +65,13 @@ int hx_scalar_matrices(double dx, double dy, double dz, int quirk, double* Ks, d
 
 /* ---- elasticity ------------------------------------------------------------ */
 enum { HX_FP64 = 0, HX_FP32_IR = 1 };           /* precision form */
-enum { HX_PC_JACOBI = 0, HX_PC_LEARNED = 1 };   /* preconditioner */
+/* preconditioner.  HX_PC_MULTILEVEL: additive multilevel diagonal scaling
+ * z = D0^-1 r + sum_l P_l D_l^-1 P_l^T r over every 2:1 coarsening of the
+ * grid (trilinear P_l, coarse moduli averaged over 2x2x2 children), the linear
+ * symmetric form of a U-Net preconditioner; plain PCG, one right-hand side.
+ * No reference counterpart (the reference's strong-preconditioner slot is the
+ * learned one, and its CPU AMG branch, GPUSolver.jl:168-268). */
+enum { HX_PC_JACOBI = 0, HX_PC_LEARNED = 1, HX_PC_MULTILEVEL = 2 };
 enum { HX_MV_AUTO = 0, HX_MV_PARITY = 1, HX_MV_DENSE = 2, HX_MV_GATHER = 3, HX_MV_MFMA = 4 };
 
 typedef struct {
@@ -229,7 +235,7 @@ int hx_tune_set(hx_ctx* ctx, const hx_tune_result* result);
 
 /* ---- kernel timing ---------------------------------------------------------- */
 enum { HX_K_MATVEC = 0, HX_K_UPDATE = 1, HX_K_PUPDATE = 2, HX_K_HELM = 3, HX_K_ELEM = 4,
-       HX_K_MATVEC32 = 5, HX_K_COUNT = 6 };
+       HX_K_MATVEC32 = 5, HX_K_PRECOND = 6, HX_K_COUNT = 7 };
 /* 1: bracket each hot-kernel launch with HIP events on the context stream */
 int hx_timing_enable(hx_ctx* ctx, int on);
 /* average launch duration (ms) and launch count of kernel class `kernel`

@@ -20,6 +20,12 @@ extern "C" {
  * in ms.  The reference kernel for scaling hardware byte counters. */
 int hx_test_copy(hx_ctx* ctx, long bytes, int reps, double* avg_ms);
 
+/* One application of the multilevel preconditioner (HX_PC_MULTILEVEL) for
+ * moduli Ee and the DOF mask of hx_set_fixed: z = M^-1 r, in FP64
+ * (precision HX_FP64) or with FP32 vectors (HX_FP32_IR); *levels receives
+ * the number of coarse levels.  No reference counterpart. */
+int hx_test_ml_apply(hx_ctx* ctx, const double* Ee, const double* r, double* z, int precision, int* levels);
+
 /* number of network parameters (conv weights then bias, layer by layer) */
 int hx_test_unet_nparam(hx_ctx* ctx, long* nparam);
 /* (re)initialise the network weights with a seed; copy the FP32 master

@@ -16,6 +16,9 @@ pieces in numpy:
   grammar on a structured grid.
 * ``simp_*`` -- textbook SIMP/OC (Andreassen et al. 2011 "top88", Liu & Tovar
   2014 "top3d", Lazarov & Sigmund 2011 PDE filter); no reference code exists.
+* ``ml_apply`` / ``ml_pcg`` -- additive multilevel diagonal scaling (the BPX
+  form of Bramble, Pasciak & Xu 1990 with Jacobi level scalings) in separable
+  numpy passes; no reference code exists.
 """
 from __future__ import annotations
 
@@ -392,3 +395,117 @@ def simp_to_loop(g: Grid, f, fixed, n_iter, nu=0.3, E0=1.0, Emin=1e-9, penal=3.0
         hist.append({"iter": it, "compliance": c, "change": change, "pcg_iterations": its, "pcg_rel": rel,
                      "volume": float(xphys.mean()), "lmid": lmid})
     return hist, x, xphys, u
+
+
+# ---------------------------------------------------------------------------------------------
+# multilevel diagonal scaling:  z = D0^-1 r + sum_l P_l D_l^-1 P_l^T r
+# (no reference counterpart; the library's HX_PC_MULTILEVEL)
+
+def ml_levels(g: Grid, max_levels=12):
+    """2:1 coarsenings while every element count is even."""
+    L, n = 0, [g.nx, g.ny, g.nz]
+    while L < max_levels and all(v % 2 == 0 for v in n):
+        n = [v // 2 for v in n]
+        L += 1
+    return L
+
+
+def _restrict_axis(a, axis):
+    """full weighting along one node axis: c[I] = f[2I] + (f[2I-1] + f[2I+1]) / 2"""
+    f = np.moveaxis(a, axis, 0)
+    c = f[0::2].copy()
+    c[:-1] += 0.5 * f[1::2]
+    c[1:] += 0.5 * f[1::2]
+    return np.moveaxis(c, 0, axis)
+
+
+def _prolong_axis(a, axis):
+    """linear interpolation along one node axis: f[2I] = c[I], f[2I+1] = (c[I] + c[I+1]) / 2"""
+    c = np.moveaxis(a, axis, 0)
+    f = np.empty((2 * c.shape[0] - 1,) + c.shape[1:])
+    f[0::2] = c
+    f[1::2] = 0.5 * (c[:-1] + c[1:])
+    return np.moveaxis(f, 0, axis)
+
+
+def _coarsen_E(E, nx, ny, nz):
+    e = E.reshape(nz, ny, nx)
+    return 0.125 * sum(e[c::2, b::2, a::2] for c in (0, 1) for b in (0, 1) for a in (0, 1)).ravel()
+
+
+def _node_sum(E, nx, ny, nz):
+    e = E.reshape(nz, ny, nx)
+    s = np.zeros((nz + 1, ny + 1, nx + 1))
+    for c in (0, 1):
+        for b in (0, 1):
+            for a in (0, 1):
+                s[c:c + nz, b:b + ny, a:a + nx] += e
+    return s
+
+
+def ml_hierarchy(g: Grid, Ke, Ee, fixed, max_levels=12):
+    """Per level (l = 0..L): element counts, inverse diagonal (NZ, NY, NX, 3)."""
+    kd = np.diag(_f64(Ke).reshape(24, 24))[:3]
+    fx = np.asarray(fixed, dtype=bool).reshape(g.nz + 1, g.ny + 1, g.nx + 1, 3)
+    L = ml_levels(g, max_levels)
+    levels = []
+    E, n = _f64(Ee), (g.nx, g.ny, g.nz)
+    for l in range(L + 1):
+        if l > 0:
+            E = _coarsen_E(E, *n)
+            n = tuple(v // 2 for v in n)
+        h = 2 ** l
+        S = _node_sum(E, *n)[..., None]
+        d = h * kd[None, None, None, :] * S
+        free = ~fx[::h, ::h, ::h] & (S > 0)
+        levels.append((n, np.where(free, 1.0 / np.where(free, d, 1.0), 0.0)))
+    return levels
+
+
+def ml_apply(g: Grid, Ke, Ee, fixed, r, max_levels=12, levels=None):
+    """z = M^-1 r of the multilevel preconditioner (masked where D0^-1 = 0)."""
+    levels = levels or ml_hierarchy(g, Ke, Ee, fixed, max_levels)
+    L = len(levels) - 1
+    rs = [_f64(r).reshape(g.nz + 1, g.ny + 1, g.nx + 1, 3)]
+    for l in range(1, L + 1):
+        a = rs[-1]
+        for ax in (0, 1, 2):
+            a = _restrict_axis(a, ax)
+        rs.append(a)
+    y = levels[L][1] * rs[L]
+    for l in range(L - 1, -1, -1):
+        p = y
+        for ax in (0, 1, 2):
+            p = _prolong_axis(p, ax)
+        m = levels[l][1]
+        y = m * rs[l] + (p if l > 0 else np.where(m != 0.0, p, 0.0))
+    if L == 0:
+        y = levels[0][1] * rs[0]
+    return y.ravel()
+
+
+def ml_pcg(g: Grid, Ke, Ee, fixed, b, tol=1e-8, max_iter=100000, max_levels=12):
+    """Plain PCG with the multilevel preconditioner (free DOFs; fixed stay 0)."""
+    Ke, Ee, b = _f64(Ke), _f64(Ee), _f64(b)
+    levels = ml_hierarchy(g, Ke, Ee, fixed, max_levels)
+    mask = levels[0][1].ravel() != 0.0
+    b = np.where(mask, b, 0.0)
+    x = np.zeros(g.ndof)
+    r = b.copy()
+    z = ml_apply(g, Ke, Ee, fixed, r, levels=levels)
+    p = z.copy()
+    rz = r @ z
+    nb = np.linalg.norm(b)
+    for it in range(1, max_iter + 1):
+        q = np.where(mask, matvec(g, Ke, Ee, p), 0.0)
+        a = rz / (p @ q)
+        x += a * p
+        r -= a * q
+        rel = np.linalg.norm(r) / nb
+        if rel < tol:
+            return x, rel, it
+        z = ml_apply(g, Ke, Ee, fixed, r, levels=levels)
+        rzn = r @ z
+        p = z + (rzn / rz) * p
+        rz = rzn
+    return x, rel, max_iter
