@@ -43,7 +43,7 @@ EXPORTS = [
     "hx_set_fixed", "hx_element_stiffness", "hx_scalar_matrices", "hx_matvec", "hx_diagonal", "hx_pcg_solve",
     "hx_block_pcg_solve", "hx_helmholtz_filter", "hx_element_stress", "hx_compliance_sens", "hx_oc_update",
     "hx_to_init", "hx_to_step", "hx_to_read", "hx_learned_train", "hx_learned_weights", "hx_timing_enable",
-    "hx_timing_read", "hx_device_info", "hx_autotune", "hx_tune_set", "hx_time_operator",
+    "hx_timing_read", "hx_device_info", "hx_autotune", "hx_tune_set", "hx_time_operator", "hx_time_operator_pc",
 ]
 #: test hooks declared in include/hexat_testing.h
 TEST_EXPORTS = ["hx_test_unet_nparam", "hx_test_unet_init", "hx_test_unet_forward", "hx_test_unet_backward",
@@ -146,6 +146,7 @@ def load():
         L.hx_autotune.argtypes = [vp, i, P(TuneResult)]
         L.hx_tune_set.argtypes = [vp, P(TuneResult)]
         L.hx_time_operator.argtypes = [vp, i, i, i, P(C.c_double)]
+        L.hx_time_operator_pc.argtypes = [vp, i, i, i, i, P(C.c_double)]
         L.hx_test_copy.argtypes = [vp, C.c_long, i, P(C.c_double)]
         L.hx_test_ml_apply.argtypes = [vp, pd, pd, pd, i, P(C.c_int)]
         pf = P(C.c_float)
@@ -412,10 +413,12 @@ class Context:
         r = TuneResult(**{k: tune[k] for k, _ in TuneResult._fields_ if k in tune})
         self.check(self.L.hx_tune_set(self.h, C.byref(r)), "hx_tune_set")
 
-    def time_operator(self, precision=HX_FP64, nrhs=1, reps=20) -> float:
+    def time_operator(self, precision=HX_FP64, nrhs=1, reps=20, precond=HX_PC_JACOBI) -> float:
+        """Average ms of the PCG operator launch (back to back, rotating vector sets) in the form the
+        preconditioner uses (fused D^-1 r + beta p for Jacobi, z + beta p for the multilevel one)."""
         ms = C.c_double(0)
-        self.check(self.L.hx_time_operator(self.h, int(precision), int(nrhs), int(reps), C.byref(ms)),
-                   "hx_time_operator")
+        self.check(self.L.hx_time_operator_pc(self.h, int(precision), int(precond), int(nrhs), int(reps),
+                                              C.byref(ms)), "hx_time_operator_pc")
         return ms.value
 
     def test_copy(self, nbytes: int, reps: int = 4) -> float:

@@ -348,6 +348,7 @@ int worst_code(const hx_ctx* c, int nrhs) {
 // Levels of the multilevel preconditioner for the current moduli: coarse
 // moduli by 2x2x2 averaging, coarse inverse diagonals, work vectors.
 int ml_setup(hx_ctx* c, bool f32) {
+    ml_prepare_kernels();
     const int L = ml_levels_for(c->g, std::min(c->ml_max_levels, kMlMaxLevels));
     const long n = ndof(c);
     MlHierarchy<double>& h = c->ml64;
@@ -1675,7 +1676,7 @@ int hx_autotune(hx_ctx* c, int nrhs, hx_tune_result* out) {
 // streams everything between two operator applications.
 extern "C++" {
 template <typename T>
-int time_operator_t(hx_ctx* c, int nrhs, int reps, float* ms) {
+int time_operator_t(hx_ctx* c, int nrhs, int reps, float* ms, bool zform) {
     constexpr int kSets = 4;
     const long n = ndof(c) * nrhs;
     DevBuf<T> x[kSets], y[kSets], r[kSets], m[kSets], p[kSets];
@@ -1696,12 +1697,15 @@ int time_operator_t(hx_ctx* c, int nrhs, int reps, float* ms) {
         }
     }
     // the PCG-mode kernel without its p'q partial write: same loads and
-    // stores per node (p_old, r, D^-1, E in; q, p out), fused p-update on
+    // stores per node (p_old, r, D^-1, E in; q, p out), fused p-update on;
+    // zform: the explicit-preconditioner form (p_old, z, E in), parity contraction
     auto go = [&](int k) {
+        const T* mk = zform ? nullptr : m[k].p;
         if constexpr (sizeof(T) == 8)
-            mv64(c, x[k].p, y[k].p, c->E.p, 0.0, MV_PLAIN, HX_MV_AUTO, nrhs, r[k].p, m[k].p, p[k].p);
+            mv64(c, x[k].p, y[k].p, c->E.p, 0.0, MV_PLAIN, zform ? HX_MV_PARITY : HX_MV_AUTO, nrhs, r[k].p, mk, p[k].p);
         else
-            mv32(c, x[k].p, y[k].p, c->E32.p, 0.0, MV_PLAIN, c->mv32_variant, nrhs, r[k].p, m[k].p, p[k].p);
+            mv32(c, x[k].p, y[k].p, c->E32.p, 0.0, MV_PLAIN, zform ? HX_MV_PARITY : c->mv32_variant, nrhs, r[k].p, mk,
+                 p[k].p);
     };
     hipEvent_t a = get_event(c), b = get_event(c);
     for (int k = 0; k < kSets; ++k) go(k);
@@ -1717,16 +1721,22 @@ int time_operator_t(hx_ctx* c, int nrhs, int reps, float* ms) {
 }  // extern "C++"
 
 int hx_time_operator(hx_ctx* c, int precision, int nrhs, int reps, double* avg_ms) {
+    return hx_time_operator_pc(c, precision, HX_PC_JACOBI, nrhs, reps, avg_ms);
+}
+
+int hx_time_operator_pc(hx_ctx* c, int precision, int precond, int nrhs, int reps, double* avg_ms) {
     if (!c || nrhs < 1 || nrhs > kMaxRhs || reps < 1 || !avg_ms) return HX_ERR_BAD_ARG;
     if (precision != HX_FP64 && precision != HX_FP32_IR) return HX_ERR_BAD_ARG;
+    if (precond != HX_PC_JACOBI && precond != HX_PC_MULTILEVEL) return HX_ERR_BAD_ARG;
+    const bool zform = precond == HX_PC_MULTILEVEL;
     if (int rc = ensure_grid(c)) return rc;
     if (int rc = ensure_fp32(c)) return rc;
     const bool was_timing = c->timing;
     c->timing = false;
     launch_pcg_reset(c->st, 0.0, 0.0, 1 << 30, 1, c->stream, nrhs);
     float ms = 0.f;
-    const int rc = precision == HX_FP64 ? time_operator_t<double>(c, nrhs, reps, &ms)
-                                        : time_operator_t<float>(c, nrhs, reps, &ms);
+    const int rc = precision == HX_FP64 ? time_operator_t<double>(c, nrhs, reps, &ms, zform)
+                                        : time_operator_t<float>(c, nrhs, reps, &ms, zform);
     c->timing = was_timing;
     if (rc) return rc;
     HXK();

@@ -149,6 +149,7 @@ template <typename T>
 void launch_ml_apply(const MlHierarchy<T>& h, const T* r, const T* minv0, T* z, PcgState* st, double* partials,
                      int fin_mode, hipStream_t s);
 int ml_partial_blocks();
+void ml_prepare_kernels();  // once per process, outside graph capture
 
 // ---- Helmholtz filter --------------------------------------------------------
 struct HelmArgs {

@@ -20,16 +20,20 @@
 // learned one (north-star path) and its CPU AMG branch (GPUSolver.jl:168-268).
 #include "hx_launch.h"
 
+#include <algorithm>
+
 namespace hx {
 
 namespace {
 
 constexpr int NT = 256;
 
+// node counts stay below 2^31 (3 nn fits an int, checked when the grid is set)
 __device__ __forceinline__ void node_ijk(const GridArgs& g, long n, int& i, int& j, int& k) {
-    i = (int)(n % g.NX);
-    j = (int)((n / g.NX) % g.NY);
-    k = (int)(n / ((long)g.NX * g.NY));
+    const int m = (int)n, q = m / g.NX;
+    i = m - q * g.NX;
+    k = q / g.NY;
+    j = q - k * g.NY;
 }
 
 // Ec[I,J,K] = mean of the 8 children (2I+a, 2J+b, 2K+c) of the finer level
@@ -74,43 +78,73 @@ __global__ __launch_bounds__(NT) void k_ml_minv(GridArgs gc, double kd0, double 
     }
 }
 
-// rc = R rf (full weighting: weight 1/2 per axis offset); on the coarsest
-// level also yc = w Dc^-1 rc
+// rc = R rf (full weighting: weight 1/2 per axis offset; on the coarsest
+// level also yc = w Dc^-1 rc) over a tile of 16x4x4 coarse nodes per
+// workgroup: the 33x9x9 fine nodes under it are staged in LDS with
+// row-contiguous loads (out-of-grid nodes as 0), then each thread sums its 27
+// fine nodes from LDS.  Gathering the 81 values per thread straight from the
+// cache instead leaves the kernel bound by address processing.
+constexpr int kRtX = 16, kRtY = 4, kRtZ = 4;
 template <typename T, bool SCALE>
-__global__ __launch_bounds__(NT) void k_ml_restrict(GridArgs gf, const T* __restrict__ rf, GridArgs gc,
-                                                    T* __restrict__ rc, const T* __restrict__ minvc,
-                                                    T* __restrict__ yc, T w, const PcgState* st) {
+__global__ __launch_bounds__(NT) void k_ml_restrict_tiled(GridArgs gf, const T* __restrict__ rf, GridArgs gc,
+                                                          T* __restrict__ rc, const T* __restrict__ minvc,
+                                                          T* __restrict__ yc, T w, const PcgState* st, int ntx,
+                                                          int nty) {
+    constexpr int FX = 2 * kRtX + 1, FY = 2 * kRtY + 1, FZ = 2 * kRtZ + 1, ROW = 3 * FX;
+    __shared__ T sf[FZ * FY * ROW];
     if (st != nullptr && st->done) return;
-    const long n = (long)blockIdx.x * NT + threadIdx.x;
-    if (n >= gc.nn) return;
-    int I, J, K;
-    node_ijk(gc, n, I, J, K);
+    const int bx = blockIdx.x % ntx, by = (blockIdx.x / ntx) % nty, bz = blockIdx.x / (ntx * nty);
+    const int I0 = bx * kRtX, J0 = by * kRtY, K0 = bz * kRtZ;
+    const int i0 = 2 * I0 - 1, j0 = 2 * J0 - 1, k0 = 2 * K0 - 1;
+    // every load of the tile is issued before any is waited for: clamped
+    // addresses, out-of-grid values replaced by 0 with a select
+    constexpr int TOTAL = FZ * FY * ROW, PER = (TOTAL + NT - 1) / NT;
+    T v[PER];
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+        const int idx = min((int)threadIdx.x + m * NT, TOTAL - 1);
+        const int z = idx / (FY * ROW), rem = idx - z * (FY * ROW), y = rem / ROW, x3 = rem - y * ROW;
+        const int x = x3 / 3, i = i0 + x, j = j0 + y, k = k0 + z;
+        const bool in = i >= 0 && i < gf.NX && j >= 0 && j < gf.NY && k >= 0 && k < gf.NZ;
+        const int ic = min(max(i, 0), gf.NX - 1), jc = min(max(j, 0), gf.NY - 1), kc = min(max(k, 0), gf.NZ - 1);
+        const T val = rf[3 * ((long)kc * gf.NY * gf.NX + (long)jc * gf.NX + ic) + (x3 - 3 * x)];
+        v[m] = in ? val : T(0);
+    }
+#pragma unroll
+    for (int m = 0; m < PER; ++m)
+        if ((int)threadIdx.x + m * NT < TOTAL) sf[threadIdx.x + m * NT] = v[m];
+    __syncthreads();
+    const int tx = threadIdx.x % kRtX, ty = (threadIdx.x / kRtX) % kRtY, tz = threadIdx.x / (kRtX * kRtY);
+    const int I = I0 + tx, J = J0 + ty, K = K0 + tz;
+    if (I >= gc.NX || J >= gc.NY || K >= gc.NZ) return;
     T acc[3] = {T(0), T(0), T(0)};
-    for (int dk = -1; dk <= 1; ++dk) {
-        const int k = 2 * K + dk;
-        if (k < 0 || k >= gf.NZ) continue;
-        const T wk = dk ? T(0.5) : T(1);
-        for (int dj = -1; dj <= 1; ++dj) {
-            const int j = 2 * J + dj;
-            if (j < 0 || j >= gf.NY) continue;
-            const T wj = dj ? wk * T(0.5) : wk;
-            const long row = 3 * ((long)k * gf.NY * gf.NX + (long)j * gf.NX);
-            for (int di = -1; di <= 1; ++di) {
-                const int i = 2 * I + di;
-                if (i < 0 || i >= gf.NX) continue;
-                const T wt = di ? wj * T(0.5) : wj;
-                const T* v = rf + row + 3L * i;
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const T wt = T((dx == 1 ? 1.0 : 0.5) * (dy == 1 ? 1.0 : 0.5) * (dz == 1 ? 1.0 : 0.5));
+                const T* v = &sf[((2 * tz + dz) * FY + (2 * ty + dy)) * ROW + 3 * (2 * tx + dx)];
                 acc[0] += wt * v[0];
                 acc[1] += wt * v[1];
                 acc[2] += wt * v[2];
             }
-        }
-    }
+    const long n = (long)K * gc.NY * gc.NX + (long)J * gc.NX + I;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         rc[3 * n + c] = acc[c];
         if constexpr (SCALE) yc[3 * n + c] = w * minvc[3 * n + c] * acc[c];
     }
+}
+
+// interpolation weights of fine index i from the coarse line: corners I0 and
+// I0+1 (clamped; weight 0 when i is even), so every thread runs 8 corners
+__device__ __forceinline__ void lin_corners(int i, int Nc, int& a0, int& a1, float& w0, float& w1) {
+    a0 = i >> 1;
+    a1 = min(a0 + 1, Nc - 1);
+    w0 = (i & 1) ? 0.5f : 1.0f;
+    w1 = (i & 1) ? 0.5f : 0.0f;
 }
 
 // y_f = w D_f^-1 r_f + P y_c (intermediate level), or on the fine grid
@@ -125,21 +159,26 @@ __global__ __launch_bounds__(NT) void k_ml_prolong(GridArgs gc, const T* __restr
     for (long n = (long)blockIdx.x * NT + threadIdx.x; n < gf.nn; n += (long)gridDim.x * NT) {
         int i, j, k;
         node_ijk(gf, n, i, j, k);
-        // per axis: even index -> one coarse node (weight 1), odd -> two (1/2 each)
-        const int I0 = i >> 1, J0 = j >> 1, K0 = k >> 1;
-        const int nI = (i & 1) ? 2 : 1, nJ = (j & 1) ? 2 : 1, nK = (k & 1) ? 2 : 1;
-        const T wI = (i & 1) ? T(0.5) : T(1), wJ = (j & 1) ? T(0.5) : T(1), wK = (k & 1) ? T(0.5) : T(1);
         T acc[3] = {T(0), T(0), T(0)};
-        if (yc != nullptr)
-        for (int c3 = 0; c3 < nK; ++c3)
-            for (int b = 0; b < nJ; ++b)
-                for (int a = 0; a < nI; ++a) {
-                    const T* v = yc + 3 * ((long)(K0 + c3) * gc.NY * gc.NX + (long)(J0 + b) * gc.NX + (I0 + a));
-                    const T wt = wI * wJ * wK;
-                    acc[0] += wt * v[0];
-                    acc[1] += wt * v[1];
-                    acc[2] += wt * v[2];
-                }
+        if (yc != nullptr) {
+            int xa[2], ya[2], za[2];
+            float xw[2], yw[2], zw[2];
+            lin_corners(i, gc.NX, xa[0], xa[1], xw[0], xw[1]);
+            lin_corners(j, gc.NY, ya[0], ya[1], yw[0], yw[1]);
+            lin_corners(k, gc.NZ, za[0], za[1], zw[0], zw[1]);
+#pragma unroll
+            for (int c3 = 0; c3 < 2; ++c3)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int a = 0; a < 2; ++a) {
+                        const T* v = yc + 3 * ((long)za[c3] * gc.NY * gc.NX + (long)ya[b] * gc.NX + xa[a]);
+                        const T wt = T(xw[a] * yw[b] * zw[c3]);
+                        acc[0] += wt * v[0];
+                        acc[1] += wt * v[1];
+                        acc[2] += wt * v[2];
+                    }
+        }
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const T m = minvf[3 * n + c], r = rf[3 * n + c];
@@ -156,6 +195,119 @@ __global__ __launch_bounds__(NT) void k_ml_prolong(GridArgs gc, const T* __restr
         const double v = block_sum<NT>(rz, red);
         if (threadIdx.x == 0) partials[blockIdx.x] = v;
     }
+}
+
+// Levels s..L in one workgroup: restriction from level s-1 down to L, the
+// coarsest scaling, prolongation back up to y_s.  These grids are small
+// (a few thousand nodes at most), where one launch per level would cost more
+// than the work.  Their residuals and outputs live in LDS (dynamic, laid out
+// level by level: r_s, y_s, r_{s+1}, ...), so only the first restriction
+// reads memory and only y_s is written back; passes are separated by
+// workgroup barriers.
+constexpr int kCoarseThreads = 1024;
+
+// restriction of node n of grid gc from the finer grid gf held at rf: 27
+// taps with clamped addresses and weight 0 outside the grid
+template <typename T>
+__device__ __forceinline__ void restrict_node(const GridArgs& gf, const T* rf, const GridArgs& gc, long n, T* acc) {
+    int I, J, K;
+    node_ijk(gc, n, I, J, K);
+    int ia[3], ja[3], ka[3];
+    float wi[3], wj[3], wk[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int i = 2 * I + d - 1, j = 2 * J + d - 1, k = 2 * K + d - 1;
+        ia[d] = min(max(i, 0), gf.NX - 1);
+        ja[d] = min(max(j, 0), gf.NY - 1);
+        ka[d] = min(max(k, 0), gf.NZ - 1);
+        const float wd = d == 1 ? 1.0f : 0.5f;
+        wi[d] = (i >= 0 && i < gf.NX) ? wd : 0.0f;
+        wj[d] = (j >= 0 && j < gf.NY) ? wd : 0.0f;
+        wk[d] = (k >= 0 && k < gf.NZ) ? wd : 0.0f;
+    }
+    acc[0] = acc[1] = acc[2] = T(0);
+#pragma unroll 1
+    for (int dk = 0; dk < 3; ++dk)  // 27 loads in flight per plane (register budget of 1024 threads)
+#pragma unroll
+        for (int dj = 0; dj < 3; ++dj)
+#pragma unroll
+            for (int di = 0; di < 3; ++di) {
+                const T wt = T(wk[dk] * wj[dj] * wi[di]);
+                const T* v = rf + 3 * ((long)ka[dk] * gf.NY * gf.NX + (long)ja[dj] * gf.NX + ia[di]);
+                acc[0] += wt * v[0];
+                acc[1] += wt * v[1];
+                acc[2] += wt * v[2];
+            }
+}
+
+// trilinear interpolation at node n of grid gf from the coarser grid gc at yc
+template <typename T>
+__device__ __forceinline__ void prolong_node(const GridArgs& gc, const T* yc, const GridArgs& gf, long n, T* acc) {
+    int i, j, k;
+    node_ijk(gf, n, i, j, k);
+    int xa[2], ya[2], za[2];
+    float xw[2], yw[2], zw[2];
+    lin_corners(i, gc.NX, xa[0], xa[1], xw[0], xw[1]);
+    lin_corners(j, gc.NY, ya[0], ya[1], yw[0], yw[1]);
+    lin_corners(k, gc.NZ, za[0], za[1], zw[0], zw[1]);
+    acc[0] = acc[1] = acc[2] = T(0);
+#pragma unroll
+    for (int c3 = 0; c3 < 2; ++c3)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const T* v = yc + 3 * ((long)za[c3] * gc.NY * gc.NX + (long)ya[b] * gc.NX + xa[a]);
+                const T wt = T(xw[a] * yw[b] * zw[c3]);
+                acc[0] += wt * v[0];
+                acc[1] += wt * v[1];
+                acc[2] += wt * v[2];
+            }
+}
+
+// LDS values of levels s..L (r and y of each)
+template <typename T>
+long coarse_lds_values(const MlHierarchy<T>& h, int s0) {
+    long v = 0;
+    for (int l = s0; l <= h.levels; ++l) v += 6 * h.g[l].nn;
+    return v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kCoarseThreads) void k_ml_coarse(MlHierarchy<T> h, int s0, const PcgState* st) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T* lds = reinterpret_cast<T*>(lds_raw);
+    if (st != nullptr && st->done) return;
+    const int L = h.levels;
+    auto R = [&](int l) {  // r_l in LDS; y_l follows it
+        long o = 0;
+        for (int m = s0; m < l; ++m) o += 6 * h.g[m].nn;
+        return lds + o;
+    };
+    auto Y = [&](int l) { return R(l) + 3 * h.g[l].nn; };
+    for (int l = s0; l <= L; ++l) {
+        const GridArgs gf = h.g[l - 1], gc = h.g[l];
+        for (long n = threadIdx.x; n < gc.nn; n += kCoarseThreads) {
+            T acc[3];
+            if (l == s0) restrict_node(gf, (const T*)h.r[l - 1], gc, n, acc);  // from memory
+            else restrict_node(gf, (const T*)R(l - 1), gc, n, acc);           // from LDS
+            for (int c = 0; c < 3; ++c) {
+                R(l)[3 * n + c] = acc[c];
+                if (l == L) Y(l)[3 * n + c] = T(h.w[l]) * h.minv[l][3 * n + c] * acc[c];
+            }
+        }
+        __syncthreads();
+    }
+    for (int l = L - 1; l >= s0; --l) {
+        const GridArgs gf = h.g[l], gc = h.g[l + 1];
+        for (long n = threadIdx.x; n < gf.nn; n += kCoarseThreads) {
+            T acc[3];
+            prolong_node(gc, (const T*)Y(l + 1), gf, n, acc);
+            for (int c = 0; c < 3; ++c) Y(l)[3 * n + c] = T(h.w[l]) * h.minv[l][3 * n + c] * R(l)[3 * n + c] + acc[c];
+        }
+        __syncthreads();
+    }
+    for (long i = threadIdx.x; i < 3 * h.g[s0].nn; i += kCoarseThreads) h.y[s0][i] = Y(s0)[i];
 }
 
 // r'z of the new residual: mode 0 beta = rz/rz_old, rz = rz (an iteration);
@@ -229,6 +381,20 @@ void launch_ml_minv(const GridArgs& gc, const double kd[3], const double* Ec, co
                        stride, gfine.NX, gfine.NY, minv);
 }
 
+// first level handled by the one-workgroup kernel: the first whose grid has
+// at most kCoarseNodes nodes and whose levels fit the LDS budget (levels above
+// it run one multi-block launch each); levels + 1 when none qualifies
+constexpr long kCoarseNodes = 2048;
+constexpr long kCoarseLdsBytes = 128 * 1024;
+template <typename T>
+int ml_coarse_start(const MlHierarchy<T>& h) {
+    int s0 = 1;
+    while (s0 <= h.levels &&
+           (h.g[s0].nn > kCoarseNodes || coarse_lds_values(h, s0) * (long)sizeof(T) > kCoarseLdsBytes))
+        ++s0;
+    return s0;
+}
+
 template <typename T>
 void launch_ml_apply(const MlHierarchy<T>& h, const T* r, const T* minv0, T* z, PcgState* st_, double* partials,
                      int fin_mode, hipStream_t s) {
@@ -240,17 +406,28 @@ void launch_ml_apply(const MlHierarchy<T>& h, const T* r, const T* minv0, T* z, 
         hipLaunchKernelGGL((k_ml_prolong<T, true>), dim3(kProlongBlocks), dim3(NT), 0, s, h.g[0], (const T*)nullptr,
                            h.g[0], r, minv0, T(1), z, st, partials);
     } else {
-        const T* rf = r;
-        for (int l = 1; l <= L; ++l) {
+        const int s0 = ml_coarse_start(h);
+        // levels 1..s0-1 (all of them when no level fits the LDS kernel): tiled restriction
+        const int last_tiled = s0 > L ? L : s0 - 1;
+        for (int l = 1; l <= last_tiled; ++l) {
+            const GridArgs& gc = h.g[l];
+            const int ntx = (gc.NX + kRtX - 1) / kRtX, nty = (gc.NY + kRtY - 1) / kRtY, ntz = (gc.NZ + kRtZ - 1) / kRtZ;
+            const T* rf = l == 1 ? r : h.r[l - 1];
             if (l == L)
-                hipLaunchKernelGGL((k_ml_restrict<T, true>), dim3(blocks_for(h.g[l].nn)), dim3(NT), 0, s, h.g[l - 1],
-                                   rf, h.g[l], h.r[l], h.minv[l], h.y[l], T(h.w[l]), st);
+                hipLaunchKernelGGL((k_ml_restrict_tiled<T, true>), dim3(ntx * nty * ntz), dim3(NT), 0, s, h.g[l - 1],
+                                   rf, gc, h.r[l], h.minv[l], h.y[l], T(h.w[l]), st, ntx, nty);
             else
-                hipLaunchKernelGGL((k_ml_restrict<T, false>), dim3(blocks_for(h.g[l].nn)), dim3(NT), 0, s,
-                                   h.g[l - 1], rf, h.g[l], h.r[l], (const T*)nullptr, (T*)nullptr, T(1), st);
-            rf = h.r[l];
+                hipLaunchKernelGGL((k_ml_restrict_tiled<T, false>), dim3(ntx * nty * ntz), dim3(NT), 0, s,
+                                   h.g[l - 1], rf, gc, h.r[l], (const T*)nullptr, (T*)nullptr, T(1), st, ntx, nty);
         }
-        for (int l = L - 1; l >= 1; --l)
+        if (last_tiled < L) {
+            MlHierarchy<T> hc = h;
+            hc.r[0] = const_cast<T*>(r);  // level 0's residual, when s0 = 1
+            hipLaunchKernelGGL((k_ml_coarse<T>), dim3(1), dim3(kCoarseThreads),
+                               coarse_lds_values(h, s0) * sizeof(T), s, hc, s0, st);
+        }
+        const int top = s0 <= L ? s0 - 1 : L - 1;  // finest level whose y the coarse pass did not produce
+        for (int l = top; l >= 1; --l)
             hipLaunchKernelGGL((k_ml_prolong<T, false>), dim3(blocks_for(h.g[l].nn)), dim3(NT), 0, s, h.g[l + 1],
                                h.y[l + 1], h.g[l], h.r[l], h.minv[l], T(h.w[l]), h.y[l], st, partials);
         hipLaunchKernelGGL((k_ml_prolong<T, true>), dim3(kProlongBlocks), dim3(NT), 0, s, h.g[1], h.y[1], h.g[0], r,
@@ -260,6 +437,17 @@ void launch_ml_apply(const MlHierarchy<T>& h, const T* r, const T* minv0, T* z, 
 }
 
 int ml_partial_blocks() { return kProlongBlocks; }
+
+// allow the coarse kernel's dynamic LDS above the default 64 KiB
+void ml_prepare_kernels() {
+    static bool done = false;
+    if (done) return;
+    (void)hipFuncSetAttribute((const void*)k_ml_coarse<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kCoarseLdsBytes);
+    (void)hipFuncSetAttribute((const void*)k_ml_coarse<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kCoarseLdsBytes);
+    done = true;
+}
 
 template void launch_ml_minv<double>(const GridArgs&, const double*, const double*, const uint8_t*, int,
                                      const GridArgs&, double*, hipStream_t);

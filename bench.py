@@ -146,6 +146,20 @@ def other_workload(args, hx, L, ctx):
     print(json.dumps(out), flush=True)
 
 
+def ml_apply_bytes(nx, ny, nz, word):
+    """Algorithmic HBM bytes of one multilevel preconditioner application:
+    every restriction reads its level and writes the next; every
+    prolongation reads r_l, D_l^-1 and y_{l+1} and writes y_l (z on the fine
+    grid); the coarsest level reads D_L^-1 and writes y_L."""
+    dims, n = [(nx, ny, nz)], []
+    while all(d % 2 == 0 for d in dims[-1]):
+        dims.append(tuple(d // 2 for d in dims[-1]))
+    n = [3 * (a + 1) * (b + 1) * (c + 1) for a, b, c in dims]
+    L = len(n) - 1
+    v = sum(n[l] + n[l + 1] for l in range(L)) + sum(3 * n[l] + n[l + 1] for l in range(L)) + 2 * n[L]
+    return float(v * word)
+
+
 # form -> (precision, preconditioner)
 FORMS = {"plain": ("HX_FP64", "HX_PC_JACOBI"), "precision": ("HX_FP32_IR", "HX_PC_JACOBI"),
          "plain_ml": ("HX_FP64", "HX_PC_MULTILEVEL"), "precision_ml": ("HX_FP32_IR", "HX_PC_MULTILEVEL")}
@@ -177,7 +191,7 @@ def run_form(hx, L, ctx, nx, ny, nz, f, fixed, precision, steps, warmup, args, t
             f"rel={s.pcg_rel_res:.2e} filt={s.filter_iterations} vm_max={s.max_vm:.3e} t={s.seconds:.3f}s")
     wall = time.perf_counter() - t0  # to_step synchronises the stream at its end
     # operator kernel alone: back-to-back launches at the production shape
-    op_ms = ctx.time_operator(precision, 1, 40)
+    op_ms = ctx.time_operator(precision, 1, 40, precond if precond == L.HX_PC_MULTILEVEL else L.HX_PC_JACOBI)
     # kernel breakdown: one more TO iteration with HIP events around every hot
     # launch (direct launches, so its wall time is not the reported value)
     ctx.timing(True)
@@ -219,8 +233,8 @@ def kernel_sample(args):
     ctx.tune_set(tune)
     ctx.test_copy(COPY_BYTES, 4)
     Ee = np.full(nx * ny * nz, 1e-9 + args.volfrac ** 3)
-    prec = L.HX_FP64 if args.kernel_sample == "plain" else L.HX_FP32_IR
-    ctx.pcg(Ee, f, tol=1e-30, max_iter=SAMPLE_PCG_ITERS, precision=prec)
+    prec, pc = (getattr(L, nm) for nm in FORMS[args.kernel_sample])
+    ctx.pcg(Ee, f, tol=1e-30, max_iter=SAMPLE_PCG_ITERS, precision=prec, precond=pc)
     print(json.dumps({"copy_bytes": COPY_BYTES}), flush=True)
 
 
@@ -274,9 +288,11 @@ def measure_traffic(args, form, tune=None):
             return float(r.get("Counter_Value") or r.get("counter_value") or 0.0)
 
         copy = [val(r) for r in rows if "k_copy16" in kname(r)]
-        variant = tune["variant32"] if form == "precision" else tune["variant64"]
-        tname = "float" if form == "precision" else "double"
-        key = f"k_matvec_sf<{tname}" if variant == 1 else f"k_matvec_zmarch<{tname}"
+        fp32 = FORMS[form][0] == "HX_FP32_IR"
+        variant = tune["variant32"] if fp32 else tune["variant64"]
+        tname = "float" if fp32 else "double"
+        # the multilevel forms always run the parity contraction (its z + beta p form)
+        key = f"k_matvec_sf<{tname}" if (variant == 1 or form.endswith("_ml")) else f"k_matvec_zmarch<{tname}"
         cand = {}
         for r in rows:
             if key in kname(r):
@@ -337,10 +353,11 @@ def main():
     ap.add_argument("--rmin", type=float, default=2.5)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--max-iter", type=int, default=100000)
-    ap.add_argument("--forms", default="plain,precision",
+    ap.add_argument("--forms", default="plain,precision,precision_ml",
                     help="comma-separated: plain (FP64 Jacobi-PCG), precision (FP32 operator + FP64 refinement), "
                          "plain_ml / precision_ml (the same with the multilevel preconditioner)")
-    ap.add_argument("--main-form", default=None, help="the form `value` reports (default: precision)")
+    ap.add_argument("--main-form", default=None,
+                    help="the form `value` reports (default: precision_ml when run, else precision)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-pcg-iters", type=int, default=20)
@@ -350,7 +367,7 @@ def main():
                          "the precision and batching configurations (their own JSON line)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes (roofline.traffic = null)")
-    ap.add_argument("--kernel-sample", choices=("tune", "plain", "precision"), default=None,
+    ap.add_argument("--kernel-sample", choices=("tune", "plain", "precision", "plain_ml", "precision_ml"), default=None,
                     help=argparse.SUPPRESS)  # child modes of the traffic passes
     ap.add_argument("--tune-json", default=None,
                     help="apply this tuning result (the 'tune' object of an earlier run's JSON line) instead of "
@@ -373,12 +390,12 @@ def main():
     for fm in forms:
         if fm not in FORMS:
             sys.exit(f"unknown form {fm!r} (choose from {', '.join(FORMS)})")
-    main_form = args.main_form or ("precision" if "precision" in forms else forms[0])
+    main_form = args.main_form or next((fm for fm in ("precision_ml", "precision") if fm in forms), forms[0])
     if main_form not in forms:
         sys.exit(f"--main-form {main_form!r} is not among --forms")
     traffic, tune = (None, json.loads(args.tune_json) if args.tune_json else None)
     if not args.no_traffic:  # before this process initialises the GPU
-        traffic, tune = measure_traffic(args, "plain" if FORMS[main_form][0] == "HX_FP64" else "precision", tune)
+        traffic, tune = measure_traffic(args, main_form, tune)
         log(f"traffic: {traffic}")
     hx = importlib.import_module("2025_12_09_hexat_v069_amd")
     L = hx._lib
@@ -399,8 +416,10 @@ def main():
     # launch: the PCG matvec reads r, D^-1, p_old (the p-update is fused into
     # its plane loads) and E, and writes q and p_new; the update reads x, p, r,
     # q, D^-1 and writes x, r.
-    mv_bytes64 = 40.0 * ndof + 8.0 * ne
-    mv_bytes32 = 20.0 * ndof + 4.0 * ne
+    # (multilevel forms: the operator reads p_old and z = M^-1 r instead of r and D^-1)
+    ml_main = main_form.endswith("_ml")
+    mv_bytes64 = (32.0 if ml_main else 40.0) * ndof + 8.0 * ne
+    mv_bytes32 = (16.0 if ml_main else 20.0) * ndof + 4.0 * ne
     fp64_main = FORMS[main_form][0] == "HX_FP64"
     upd_bytes = 56.0 * ndof if fp64_main else 28.0 * ndof
     pup_bytes = 32.0 * ndof if fp64_main else 16.0 * ndof
@@ -423,6 +442,14 @@ def main():
         roof = {"bound": "hbm", "kernel": nm, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": tr, "bytes_per_launch": by,
                 "avg_launch_ms": avg, "traffic_detail": traffic}
+    if ml_main and r["kern"].get("ml_precond", {}).get("count"):
+        k = r["kern"]["ml_precond"]
+        by = ml_apply_bytes(nx, ny, nz, 8 if fp64_main else 4)
+        roof_ml = {"kernels": "k_ml_restrict_tiled, k_ml_coarse, k_ml_prolong, k_fin_rz (one application)",
+                   "avg_ms": k["avg_ms"], "bytes_per_apply": by, "achieved": round(by / (k["avg_ms"] * 1e-3) / 1e9, 1),
+                   "unit": "GB/s", "frac": round(by / (k["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    else:
+        roof_ml = None
     pcg_mean = float(np.mean([s.pcg_iterations for s in res.get("plain", r)["recs"]]))
     filt_mean = float(np.mean([s.filter_iterations for s in res.get("plain", r)["recs"]]))
     cpu = None
@@ -441,6 +468,7 @@ def main():
                    "iterations_timed": f"TO iterations {args.warmup + 1}..{args.warmup + args.steps} from uniform "
                                        f"rho={args.volfrac}", "tune": tune},
         "roofline": roof,
+        "preconditioner_apply": roof_ml,
         "cpu_baseline": cpu,
         "forms": {k: {"s_per_iter": round(v["s_per_iter"], 4),
                       "pcg_iterations": [s.pcg_iterations for s in v["recs"]],

@@ -228,6 +228,11 @@ int hx_autotune(hx_ctx* ctx, int nrhs, hx_tune_result* result);
  * sets larger together than the last-level cache.  No reference
  * counterpart. */
 int hx_time_operator(hx_ctx* ctx, int precision, int nrhs, int reps, double* avg_ms);
+/* The same for the operator form a preconditioner uses: HX_PC_JACOBI (fused
+ * p = D^-1 r + beta p_old, as hx_time_operator) or HX_PC_MULTILEVEL (fused
+ * p = z + beta p_old from an explicit z, parity contraction).  No reference
+ * counterpart. */
+int hx_time_operator_pc(hx_ctx* ctx, int precision, int precond, int nrhs, int reps, double* avg_ms);
 /* Apply a tuning result (e.g. one cached from an earlier hx_autotune) without
  * timing anything: its launch shapes and contractions.  No reference
  * counterpart. */
