@@ -56,6 +56,15 @@ def _precision_of(config):
     return _lib.HX_FP32_IR if p in ("fp32_mfma_ir", "fp32_ir", "fp32") else _lib.HX_FP64
 
 
+def _precond_of(config):
+    """solver_parameters.preconditioner: "jacobi" (the reference's) or "multilevel" (added here)."""
+    sp = (config or {}).get("solver_parameters", {})
+    p = str(sp.get("preconditioner", "jacobi")).lower()
+    if p not in ("jacobi", "multilevel"):
+        raise ValueError(f"solver_parameters.preconditioner: {p!r} (jacobi | multilevel)")
+    return (_lib.HX_PC_MULTILEVEL, "Multilevel") if p == "multilevel" else (_lib.HX_PC_JACOBI, "Jacobi")
+
+
 def solve_system_gpu(nodes, elements, E, nu, bc_indicator, f, density, *, max_iter=40000, tol=1e-6,
                      method="native", solver="cg", use_precond=True, shift_factor=1.0e-6,
                      min_stiffness_threshold=1.0e-3, u_guess=None, config=None, prune=None):
@@ -80,6 +89,7 @@ def solve_system_gpu(nodes, elements, E, nu, bc_indicator, f, density, *, max_it
     live = free & (diag > 0)
     max_diag = float(np.abs(diag[live]).max()) if live.any() else 0.0
     prec = _precision_of(config)
+    pc, pc_name = _precond_of(config)
     u0 = None
     if u_guess is not None and len(u_guess) == ctx.ndof:
         u0 = np.asarray(u_guess, dtype=np.float64)
@@ -89,19 +99,19 @@ def solve_system_gpu(nodes, elements, E, nu, bc_indicator, f, density, *, max_it
     for attempt in range(1, 4):  # GPUSolver.jl:390-594
         shift = shift_f * max_diag
         u, st = ctx.pcg(Ee, f, u0=u0 if attempt == 1 else None, tol=float(tol), max_iter=int(max_iter),
-                        shift=shift, precision=prec)
+                        shift=shift, precision=prec, precond=pc)
         last_rel = st.rel_res
         if st.status == _lib.HX_OK:
-            return u, st.rel_res, "Jacobi"
+            return u, st.rel_res, pc_name
         if st.rel_res < best_rel:
             best_u, best_rel = u, st.rel_res
         diverged = st.status in (_lib.HX_ERR_DIVERGED, _lib.HX_ERR_BREAKDOWN)
         failed_badly = diverged or best_rel > 0.1
         if not failed_badly and best_rel < 5e-2:  # acceptable stagnation (:549-557)
-            return best_u, best_rel, "Jacobi"
+            return best_u, best_rel, pc_name
         shift_f *= 10.0 if failed_badly else 5.0
     warnings.warn("All solver attempts failed. Returning best stagnated result.")
-    return (best_u if best_u is not None else np.zeros(ctx.ndof)), last_rel, "Jacobi"
+    return (best_u if best_u is not None else np.zeros(ctx.ndof)), last_rel, pc_name
 
 
 def adaptive_tolerance(config, solver_type="gpu"):

@@ -57,3 +57,16 @@ def test_context_without_gpu_fails_loudly(hx):
     with pytest.raises(hx.HexatError) as ei:
         hx.Context(0)
     assert ei.value.code == hx._lib.HX_ERR_NO_DEVICE
+
+
+def test_solver_config_preconditioner_mapping(hx):
+    """solver_parameters.preconditioner selects the multilevel form; anything
+    else but jacobi is rejected before any device work."""
+    import importlib
+    solver = importlib.import_module(hx.__name__ + ".solver")
+    L = hx._lib
+    assert solver._precond_of(None) == (L.HX_PC_JACOBI, "Jacobi")
+    assert solver._precond_of({"solver_parameters": {"preconditioner": "Multilevel"}}) == (L.HX_PC_MULTILEVEL,
+                                                                                           "Multilevel")
+    with pytest.raises(ValueError):
+        solver._precond_of({"solver_parameters": {"preconditioner": "amg"}})
