@@ -960,7 +960,8 @@ int hx_grid_set(hx_ctx* c, int nx, int ny, int nz, double dx, double dy, double 
     HXA(c->Kbin_d, 576);
     HXA(c->Kbin32_d, 576);
     const long np = std::max<long>({3L * kVecBlocks + 8, (long)matvec_parity_blocks(g, 0, 4) + 8,
-                                    (long)matvec_parity_blocks(g, 1, 4) + 8, (long)matvec_gather_blocks(g) + 8,
+                                    (long)matvec_parity_blocks(g, 1, 4) + 8, (long)matvec_parity_blocks(g, 2, 4) + 8,
+                                    (long)matvec_gather_blocks(g) + 8,
                                     (long)helm_matvec_blocks(g) + 8, (long)element_post_blocks(g) + 8});
     HXA(c->partials, np * kMaxRhs);
     HXA(c->small, 64);
@@ -1616,7 +1617,7 @@ int hx_autotune(hx_ctx* c, int nrhs, hx_tune_result* out) {
     auto run32 = [&](int v) {
         return time_it([&] { mv32(c, fx.p, fy.p, c->E32.p, 0.0, MV_PCG, v, nrhs, fr.p, fm.p, fp.p); });
     };
-    const int tiles[] = {0, 1}, chunks[] = {8, 16, 32}, depths[] = {1, 2};
+    const int tiles[] = {0, 1, 2}, chunks[] = {8, 16, 32}, depths[] = {1, 2};
     double best64 = 1e30, best32 = 1e30;
     MvShape b64 = c->shape64, b32 = c->shape32;
     int bv64 = HX_MV_PARITY, bv32 = HX_MV_PARITY;
@@ -1746,7 +1747,9 @@ int hx_time_operator_pc(hx_ctx* c, int precision, int precond, int nrhs, int rep
 
 int hx_tune_set(hx_ctx* c, const hx_tune_result* t) {
     if (!c || !t) return HX_ERR_BAD_ARG;
-    auto shape_ok = [](int tile, int kchunk, int pd) { return (tile == 0 || tile == 1) && kchunk >= 4 && (pd == 1 || pd == 2); };
+    auto shape_ok = [](int tile, int kchunk, int pd) {
+        return tile >= 0 && tile <= 2 && kchunk >= 4 && (pd == 1 || pd == 2);
+    };
     auto var_ok = [](int v) { return v == HX_MV_PARITY || v == HX_MV_DENSE || v == HX_MV_GATHER || v == HX_MV_MFMA; };
     if (!shape_ok(t->tile, t->kchunk, t->pdepth) || !shape_ok(t->tile32, t->kchunk32, t->pdepth32) ||
         !var_ok(t->variant64) || !var_ok(t->variant32) || t->variant32 == HX_MV_GATHER || t->variant64 == HX_MV_MFMA)

@@ -268,21 +268,31 @@ __global__ __launch_bounds__(TX* TY) void k_matvec_zmarch(GridArgs g, ParityK<T>
 // Per element and layer: ~180 flops, 6 LDS reads, 6+6 LDS exchange words and
 // 2 barriers, against 234 flops, 27 reads, 24+24 exchange words and 3
 // barriers for the element-local form.
-__device__ __forceinline__ float lane_from_left(float v) {  // lane l <- lane l-1 within 16-lane rows
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, true));
+// lane l <- lane l-1: within 16-lane DPP rows (row_shr:1) for 16-wide tile
+// rows, across the wave (wave_shr:1) for 32-wide ones; lane 0 of a row / the
+// wave receives 0 (bound_ctrl).  A tile row's first thread is never an owned
+// node, so what crosses a row boundary is discarded.
+template <int TX>
+__device__ __forceinline__ int lane_from_left_i(int v) {
+    static_assert(TX == 16 || TX == 32, "tile rows of 16 or 32 lanes");
+    return __builtin_amdgcn_update_dpp(0, v, TX == 16 ? 0x111 : 0x138, 0xF, 0xF, true);
 }
+template <int TX>
+__device__ __forceinline__ float lane_from_left(float v) {
+    return __int_as_float(lane_from_left_i<TX>(__float_as_int(v)));
+}
+template <int TX>
 __device__ __forceinline__ double lane_from_left(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x111, 0xF, 0xF, true);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x111, 0xF, 0xF, true);
+    const int lo = lane_from_left_i<TX>(__double2loint(v));
+    const int hi = lane_from_left_i<TX>(__double2hiint(v));
     return __hiloint2double(hi, lo);
 }
 
 // FUSE: 0 plain operator; 1 fused p = D^-1 r + beta p_old; 2 fused p = z + beta p_old
 // with z handed in a.r (an explicit preconditioner's output; a.minv unused)
-template <typename T, int TY, int FUSE, int PD>
-__global__ __launch_bounds__(16 * TY) void k_matvec_sf(GridArgs g, ParityK<T> K, MvArgs<T> a, int kchunk, int ntx,
+template <typename T, int TX, int TY, int FUSE, int PD>
+__global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K, MvArgs<T> a, int kchunk, int ntx,
                                                        int nty, int ntiles) {
-    constexpr int TX = 16;  // one tile row = one DPP row
     constexpr int NT = TX * TY;
     constexpr int PX = TX + 1, PY = TY + 1;
     constexpr int NEXTRA = 3 * (PY + TX);  // plane values beyond the thread grid: column TX and row TY
@@ -465,7 +475,7 @@ __global__ __launch_bounds__(16 * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const T A = eA[c] + ex[c][src], B = eB[c] + ex[3 + c][src];
-                node[c] = (A - B) + lane_from_left(A + B);  // x-inverse: own half + left edge's half
+                node[c] = (A - B) + lane_from_left<TX>(A + B);  // x-inverse: own half + left edge's half
             }
             ko = ke;
             const int so_i = (ty - 1) * OROW + 3 * (tx - 1);
@@ -610,7 +620,9 @@ int zmarch_tiles(const GridArgs& g, int kchunk, int* ntx, int* nty) {
 
 int matvec_parity_blocks(const GridArgs& g, int tile, int kchunk) {
     int a, b;
-    return round8(tile == 1 ? zmarch_tiles<16, 32>(g, kchunk, &a, &b) : zmarch_tiles<16, 16>(g, kchunk, &a, &b));
+    return round8(tile == 1   ? zmarch_tiles<16, 32>(g, kchunk, &a, &b)
+                  : tile == 2 ? zmarch_tiles<32, 8>(g, kchunk, &a, &b)
+                              : zmarch_tiles<16, 16>(g, kchunk, &a, &b));
 }
 int matvec_dense_blocks(const GridArgs& g, int kchunk) {
     int a, b;
@@ -635,20 +647,20 @@ int launch_zmarch(const GridArgs& g, DenseK<T> Kd, const MvArgs<T>& a, const MvS
     return nb;
 }
 
-template <typename T, int TY, int PD>
+template <typename T, int TX, int TY, int PD>
 int launch_sf(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, const MvShape& sh, hipStream_t s,
               int nrhs) {
     int ntx, nty;
-    const int nt = zmarch_tiles<16, TY>(g, sh.kchunk, &ntx, &nty);
+    const int nt = zmarch_tiles<TX, TY>(g, sh.kchunk, &ntx, &nty);
     const int nb = round8(nt);
     if (a.p_out != nullptr && a.minv == nullptr)
-        hipLaunchKernelGGL((k_matvec_sf<T, TY, 2, PD>), dim3(nb, nrhs), dim3(16 * TY), 0, s, g, K, a, sh.kchunk,
+        hipLaunchKernelGGL((k_matvec_sf<T, TX, TY, 2, PD>), dim3(nb, nrhs), dim3(TX * TY), 0, s, g, K, a, sh.kchunk,
                            ntx, nty, nt);
     else if (a.p_out != nullptr)
-        hipLaunchKernelGGL((k_matvec_sf<T, TY, 1, PD>), dim3(nb, nrhs), dim3(16 * TY), 0, s, g, K, a, sh.kchunk,
+        hipLaunchKernelGGL((k_matvec_sf<T, TX, TY, 1, PD>), dim3(nb, nrhs), dim3(TX * TY), 0, s, g, K, a, sh.kchunk,
                            ntx, nty, nt);
     else
-        hipLaunchKernelGGL((k_matvec_sf<T, TY, 0, PD>), dim3(nb, nrhs), dim3(16 * TY), 0, s, g, K, a, sh.kchunk,
+        hipLaunchKernelGGL((k_matvec_sf<T, TX, TY, 0, PD>), dim3(nb, nrhs), dim3(TX * TY), 0, s, g, K, a, sh.kchunk,
                            ntx, nty, nt);
     return nb;
 }
@@ -660,9 +672,13 @@ void launch_matvec_parity(const GridArgs& g, const ParityK<T>& K, const MvArgs<T
                           hipStream_t s, int nrhs) {
     int nb;
     if (sh.pdepth == 2)
-        nb = sh.tile == 1 ? launch_sf<T, 32, 2>(g, K, a, sh, s, nrhs) : launch_sf<T, 16, 2>(g, K, a, sh, s, nrhs);
+        nb = sh.tile == 1   ? launch_sf<T, 16, 32, 2>(g, K, a, sh, s, nrhs)
+             : sh.tile == 2 ? launch_sf<T, 32, 8, 2>(g, K, a, sh, s, nrhs)
+                            : launch_sf<T, 16, 16, 2>(g, K, a, sh, s, nrhs);
     else
-        nb = sh.tile == 1 ? launch_sf<T, 32, 1>(g, K, a, sh, s, nrhs) : launch_sf<T, 16, 1>(g, K, a, sh, s, nrhs);
+        nb = sh.tile == 1   ? launch_sf<T, 16, 32, 1>(g, K, a, sh, s, nrhs)
+             : sh.tile == 2 ? launch_sf<T, 32, 8, 1>(g, K, a, sh, s, nrhs)
+                            : launch_sf<T, 16, 16, 1>(g, K, a, sh, s, nrhs);
     if (a.mode == MV_PCG) launch_fin_alpha(a.st, a.partials, nb, nrhs, s);
 }
 

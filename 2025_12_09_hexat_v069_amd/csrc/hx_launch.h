@@ -9,7 +9,8 @@ namespace hx {
 enum MvMode { MV_PLAIN = 0, MV_PCG = 1 };
 
 // Launch shape of the z-marching operator: tile 0 = 16x16 element columns per
-// workgroup (256 threads), 1 = 16x32 (512 threads, parity contraction only);
+// workgroup (256 threads), 1 = 16x32 (512 threads, parity contraction only),
+// 2 = 32x8 (256 threads, parity contraction only: longer row segments);
 // kchunk z planes per workgroup; pdepth node planes in flight per workgroup
 // (parity contraction: 1 or 2).
 struct MvShape {
@@ -35,7 +36,7 @@ struct MvArgs {
     T* p_out;
 };
 
-// tile: 0 -> 16x16 element columns (default), 1 -> 32x8
+// tile: 0 -> 16x16 element columns (default), 1 -> 16x32, 2 -> 32x8
 template <typename T>
 void launch_matvec_parity(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, const MvShape& sh,
                           hipStream_t s, int nrhs = 1);
