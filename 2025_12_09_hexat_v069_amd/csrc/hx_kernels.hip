@@ -790,8 +790,10 @@ template <typename T>
 void launch_pcg_update(long n, T* x, const T* p, T* r, const T* q, const T* minv, PcgState* st, double* partials,
                        hipStream_t s, int nrhs, bool set_beta) {
     hipLaunchKernelGGL((k_pcg_update<T>), dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, x, p, r, q, minv, st, partials);
-    hipLaunchKernelGGL(k_fin_update, dim3(nrhs), dim3(NT), 0, s, st, partials, (unsigned)kVecBlocks,
-                       set_beta ? 1 : 0);
+    // without set_beta the explicit preconditioner's finaliser reads these
+    // partials (r'r at index 2b+1) after its own r'z and finishes the iteration
+    if (set_beta)
+        hipLaunchKernelGGL(k_fin_update, dim3(nrhs), dim3(NT), 0, s, st, partials, (unsigned)kVecBlocks, 1);
 }
 template <typename T>
 void launch_pcg_pupdate(long n, const T* r, const T* minv, T* p, const PcgState* st, hipStream_t s, int nrhs) {

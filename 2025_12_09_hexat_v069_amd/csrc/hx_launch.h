@@ -74,7 +74,9 @@ void launch_copy16(long n, const void* src, void* dst, hipStream_t s);
 template <typename T>
 void launch_pcg_init(long n, const T* b, T* x, const T* q, const T* minv, T* r, T* p, double shift,
                      PcgState* st, double* partials, hipStream_t s, int nrhs = 1);
-// set_beta = false leaves beta and rz to an explicit preconditioner's r'z
+// set_beta = false leaves the iteration's bookkeeping (||r||, beta, the
+// convergence decision) to an explicit preconditioner's finaliser, which reads
+// the update's partials (launch_ml_apply); no finaliser is launched here
 template <typename T>
 void launch_pcg_update(long n, T* x, const T* p, T* r, const T* q, const T* minv, PcgState* st,
                        double* partials, hipStream_t s, int nrhs = 1, bool set_beta = true);
@@ -145,7 +147,10 @@ template <typename T>
 void launch_ml_minv(const GridArgs& gc, const double kd[3], const double* Ec, const uint8_t* fixed_f, int stride,
                     const GridArgs& gfine, T* minv, hipStream_t s);
 // z = M^-1 r (masked where minv0 = 0) and r'z into the state: fin_mode 0 = an
-// iteration (beta, rz), 1 = start (rz, beta = 0), 2 = park in scal[5]
+// iteration (records ||r|| from the update's partials at `partials`, r'r at
+// index 2b+1 of kVecBlocks blocks, then beta and the convergence decision),
+// 1 = start (rz, beta = 0), 2 = park in scal[5].  Its own partials go after
+// the update's.
 template <typename T>
 void launch_ml_apply(const MlHierarchy<T>& h, const T* r, const T* minv0, T* z, PcgState* st, double* partials,
                      int fin_mode, hipStream_t s);

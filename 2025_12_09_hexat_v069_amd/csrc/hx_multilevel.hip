@@ -85,26 +85,42 @@ __global__ __launch_bounds__(NT) void k_ml_minv(GridArgs gc, double kd0, double 
 // fine nodes from LDS.  Gathering the 81 values per thread straight from the
 // cache instead leaves the kernel bound by address processing.
 constexpr int kRtX = 16, kRtY = 4, kRtZ = 4;
-template <typename T, bool SCALE>
+template <typename T, bool SCALE, int RX = kRtX, int RY = kRtY, int RZ = kRtZ>
 __global__ __launch_bounds__(NT) void k_ml_restrict_tiled(GridArgs gf, const T* __restrict__ rf, GridArgs gc,
                                                           T* __restrict__ rc, const T* __restrict__ minvc,
                                                           T* __restrict__ yc, T w, const PcgState* st, int ntx,
                                                           int nty) {
-    constexpr int FX = 2 * kRtX + 1, FY = 2 * kRtY + 1, FZ = 2 * kRtZ + 1, ROW = 3 * FX;
+    static_assert(RX * RY * RZ == NT, "one coarse node per thread");
+    constexpr int FX = 2 * RX + 1, FY = 2 * RY + 1, FZ = 2 * RZ + 1, ROW = 3 * FX;
     __shared__ T sf[FZ * FY * ROW];
     if (st != nullptr && st->done) return;
     const int bx = blockIdx.x % ntx, by = (blockIdx.x / ntx) % nty, bz = blockIdx.x / (ntx * nty);
-    const int I0 = bx * kRtX, J0 = by * kRtY, K0 = bz * kRtZ;
+    const int I0 = bx * RX, J0 = by * RY, K0 = bz * RZ;
     const int i0 = 2 * I0 - 1, j0 = 2 * J0 - 1, k0 = 2 * K0 - 1;
     // every load of the tile is issued before any is waited for: clamped
     // addresses, out-of-grid values replaced by 0 with a select
     constexpr int TOTAL = FZ * FY * ROW, PER = (TOTAL + NT - 1) / NT;
+    // element idx = t + m NT as (z, y, x3): one division for m = 0, then a
+    // step of NT = DY rows + DX3 positions per m with carries
+    constexpr int DY = NT / ROW, DX3 = NT - DY * ROW;
+    int z = (int)threadIdx.x / (FY * ROW), y = ((int)threadIdx.x / ROW) % FY, x3 = (int)threadIdx.x % ROW;
     T v[PER];
 #pragma unroll
     for (int m = 0; m < PER; ++m) {
-        const int idx = min((int)threadIdx.x + m * NT, TOTAL - 1);
-        const int z = idx / (FY * ROW), rem = idx - z * (FY * ROW), y = rem / ROW, x3 = rem - y * ROW;
-        const int x = x3 / 3, i = i0 + x, j = j0 + y, k = k0 + z;
+        if (m > 0) {
+            x3 += DX3;
+            y += DY;
+            if (x3 >= ROW) {
+                x3 -= ROW;
+                ++y;
+            }
+            if (y >= FY) {
+                y -= FY;
+                ++z;
+            }
+        }
+        const int zc = min(z, FZ - 1);  // past the tile's end: clamped, never stored
+        const int x = x3 / 3, i = i0 + x, j = j0 + y, k = k0 + zc;
         const bool in = i >= 0 && i < gf.NX && j >= 0 && j < gf.NY && k >= 0 && k < gf.NZ;
         const int ic = min(max(i, 0), gf.NX - 1), jc = min(max(j, 0), gf.NY - 1), kc = min(max(k, 0), gf.NZ - 1);
         const T val = rf[3 * ((long)kc * gf.NY * gf.NX + (long)jc * gf.NX + ic) + (x3 - 3 * x)];
@@ -114,7 +130,7 @@ __global__ __launch_bounds__(NT) void k_ml_restrict_tiled(GridArgs gf, const T* 
     for (int m = 0; m < PER; ++m)
         if ((int)threadIdx.x + m * NT < TOTAL) sf[threadIdx.x + m * NT] = v[m];
     __syncthreads();
-    const int tx = threadIdx.x % kRtX, ty = (threadIdx.x / kRtX) % kRtY, tz = threadIdx.x / (kRtX * kRtY);
+    const int tx = threadIdx.x % RX, ty = (threadIdx.x / RX) % RY, tz = threadIdx.x / (RX * RY);
     const int I = I0 + tx, J = J0 + ty, K = K0 + tz;
     if (I >= gc.NX || J >= gc.NY || K >= gc.NZ) return;
     T acc[3] = {T(0), T(0), T(0)};
@@ -310,14 +326,20 @@ __global__ __launch_bounds__(kCoarseThreads) void k_ml_coarse(MlHierarchy<T> h, 
     for (long i = threadIdx.x; i < 3 * h.g[s0].nn; i += kCoarseThreads) h.y[s0][i] = Y(s0)[i];
 }
 
-// r'z of the new residual: mode 0 beta = rz/rz_old, rz = rz (an iteration);
-// 1 rz = rz, beta = 0 (start); 2 park rz in scal[5] (reliable update)
-__global__ __launch_bounds__(NT) void k_fin_rz(PcgState* st, const double* partials, unsigned nb, int mode) {
+// r'z of the new residual.  mode 0 (an iteration): record ||r|| from the
+// update's partials (r'r at index 2b+1 of nb_upd blocks), beta = rz/rz_old,
+// rz = rz, then the convergence decision; 1 (start): rz = rz, beta = 0;
+// 2: park rz in scal[5] (reliable update)
+__global__ __launch_bounds__(NT) void k_fin_rz(PcgState* st, const double* partials, unsigned nb,
+                                               const double* upd_partials, unsigned nb_upd, int mode) {
     __shared__ double red[NT / 64];
     if (mode != 2 && st->done) return;  // parking follows a finished phase
     const double s = reduce_partials<NT>(partials, 1, 0, nb, red);
+    double rr = 0.0;
+    if (mode == 0) rr = reduce_partials<NT>(upd_partials, 2, 1, nb_upd, red);
     if (threadIdx.x == 0) {
-        if (mode == 0) {  // the update recorded ||r||; decide here, after beta
+        if (mode == 0) {
+            pcg_record(st, rr);
             if (!(s > 0.0) || !(st->rz > 0.0)) {  // M^-1 is SPD: r'z <= 0 only at r = 0 or in breakdown
                 st->done = 1;
                 st->status = st->rel == 0.0 ? 0 : 4;
@@ -402,6 +424,8 @@ void launch_ml_apply(const MlHierarchy<T>& h, const T* r, const T* minv0, T* z, 
     // an iteration stops once the solve is done; parking (after a finished
     // phase) and the start always run
     const PcgState* st = fin_mode == 0 ? st_ : nullptr;
+    double* const upd_partials = partials;
+    partials += 2 * kVecBlocks;  // after the update's (read by the mode-0 finaliser)
     if (L == 0) {  // no coarse level: plain Jacobi through the same finaliser
         hipLaunchKernelGGL((k_ml_prolong<T, true>), dim3(kProlongBlocks), dim3(NT), 0, s, h.g[0], (const T*)nullptr,
                            h.g[0], r, minv0, T(1), z, st, partials);
@@ -433,7 +457,8 @@ void launch_ml_apply(const MlHierarchy<T>& h, const T* r, const T* minv0, T* z, 
         hipLaunchKernelGGL((k_ml_prolong<T, true>), dim3(kProlongBlocks), dim3(NT), 0, s, h.g[1], h.y[1], h.g[0], r,
                            minv0, T(1), z, st, partials);
     }
-    hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(NT), 0, s, st_, partials, (unsigned)kProlongBlocks, fin_mode);
+    hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(NT), 0, s, st_, partials, (unsigned)kProlongBlocks, upd_partials,
+                       (unsigned)kVecBlocks, fin_mode);
 }
 
 int ml_partial_blocks() { return kProlongBlocks; }
