@@ -208,7 +208,7 @@ int hx_learned_weights(hx_ctx* ctx, uint16_t* out, int* count);
 
 /* ---- autotuner ---------------------------------------------------------------- */
 typedef struct {
-    int tile;         /* 0: 16x16 element columns per workgroup, 1: 16x32 */
+    int tile;         /* 0: 16x16 element columns per workgroup, 1: 16x32, 2: 32x8 */
     int kchunk;       /* z planes per workgroup */
     int variant64;    /* chosen FP64 contraction (HX_MV_*) */
     int variant32;    /* chosen FP32 contraction (HX_MV_*) */
