@@ -122,6 +122,9 @@ struct hx_ctx {
     long l_sample_stride = 0;
     // multilevel preconditioner: level moduli, inverse diagonals, work vectors
     int ml_max_levels = kMlMaxLevels;
+    bool ml_exact = true;  // solve the first level with at most kMlExactMax DOFs exactly (and stop there)
+    DevBuf<double> ml_A, ml_ainv;
+    DevBuf<float> ml_ainv32;
     DevBuf<double> ml_E[kMlMaxLevels + 1], ml_r[kMlMaxLevels + 1], ml_y[kMlMaxLevels + 1],
         ml_minv[kMlMaxLevels + 1], ml_z;
     DevBuf<float> ml_r32[kMlMaxLevels + 1], ml_y32[kMlMaxLevels + 1], ml_minv32[kMlMaxLevels + 1], ml_z32;
@@ -277,30 +280,44 @@ long ndof(const hx_ctx* c) { return 3 * c->g.nn; }
 // y = K(E) x (+ shift x); MV_PCG mode also reduces p'q into the state.
 long ndof(const hx_ctx* c);
 
+// The operator's p'q partials live after the vector kernels' (3 per block of
+// kVecBlocks, kMaxRhs columns), so a PCG update that reduces them itself
+// (defer_alpha) never overwrites them while other blocks still read.
+constexpr long kOpPartials = 3L * kVecBlocks * kMaxRhs;
+
+// returns the operator's workgroups per column (its p'q partials) in *npq
 int mv64(hx_ctx* c, const double* x, double* y, const double* E, double shift, int mode, int variant,
-         int nrhs = 1, const double* fr = nullptr, const double* fminv = nullptr, double* fp_out = nullptr) {
-    MvArgs<double> a{x, y, E, shift, mode, c->st, c->partials.p, ndof(c), fr, fminv, fp_out};
+         int nrhs = 1, const double* fr = nullptr, const double* fminv = nullptr, double* fp_out = nullptr,
+         bool defer_alpha = false, int* npq = nullptr) {
+    MvArgs<double> a{x, y, E, shift, mode, c->st, c->partials.p + kOpPartials, ndof(c), fr, fminv, fp_out};
+    a.defer_alpha = defer_alpha ? 1 : 0;
     int v = variant == HX_MV_AUTO ? c->mv_variant : variant;
     if (fp_out && v == HX_MV_GATHER) v = HX_MV_PARITY;  // the gather kernel has no fused p-update
     TimeScope ts(c, HX_K_MATVEC);
-    if (v == HX_MV_DENSE) launch_matvec_dense<double>(c->g, c->Kbin_d.p, a, c->shape64, c->stream, nrhs);
-    else if (v == HX_MV_GATHER) launch_matvec_gather(c->g, c->Kbin_d.p, a, c->stream, nrhs);
-    else launch_matvec_parity<double>(c->g, c->kp64, a, c->shape64, c->stream, nrhs);
+    int nb;
+    if (v == HX_MV_DENSE) nb = launch_matvec_dense<double>(c->g, c->Kbin_d.p, a, c->shape64, c->stream, nrhs);
+    else if (v == HX_MV_GATHER) nb = launch_matvec_gather(c->g, c->Kbin_d.p, a, c->stream, nrhs);
+    else nb = launch_matvec_parity<double>(c->g, c->kp64, a, c->shape64, c->stream, nrhs);
+    if (npq) *npq = nb;
     return HX_OK;
 }
 
 int mv32(hx_ctx* c, const float* x, float* y, const float* E, double shift, int mode, int variant, int nrhs = 1,
-         const float* fr = nullptr, const float* fminv = nullptr, float* fp_out = nullptr) {
-    MvArgs<float> a{x, y, E, (float)shift, mode, c->st, c->partials.p, ndof(c), fr, fminv, fp_out};
+         const float* fr = nullptr, const float* fminv = nullptr, float* fp_out = nullptr, bool defer_alpha = false,
+         int* npq = nullptr) {
+    MvArgs<float> a{x, y, E, (float)shift, mode, c->st, c->partials.p + kOpPartials, ndof(c), fr, fminv, fp_out};
+    a.defer_alpha = defer_alpha ? 1 : 0;
     TimeScope ts(c, HX_K_MATVEC32);
+    int nb;
     if (variant == HX_MV_DENSE || variant == HX_MV_MFMA) {
         MvShape sh = c->shape32;  // the dense and MFMA contractions use the 16x16 tile
         sh.tile = 0;
-        if (variant == HX_MV_DENSE) launch_matvec_dense<float>(c->g, c->Kbin32_d.p, a, sh, c->stream, nrhs);
-        else launch_matvec_mfma(c->g, c->Kbin32_d.p, a, sh, c->stream, nrhs);
+        if (variant == HX_MV_DENSE) nb = launch_matvec_dense<float>(c->g, c->Kbin32_d.p, a, sh, c->stream, nrhs);
+        else nb = launch_matvec_mfma(c->g, c->Kbin32_d.p, a, sh, c->stream, nrhs);
     } else {
-        launch_matvec_parity<float>(c->g, c->kp32, a, c->shape32, c->stream, nrhs);
+        nb = launch_matvec_parity<float>(c->g, c->kp32, a, c->shape32, c->stream, nrhs);
     }
+    if (npq) *npq = nb;
     return HX_OK;
 }
 
@@ -349,11 +366,25 @@ int worst_code(const hx_ctx* c, int nrhs) {
 // moduli by 2x2x2 averaging, coarse inverse diagonals, work vectors.
 int ml_setup(hx_ctx* c, bool f32) {
     ml_prepare_kernels();
-    const int L = ml_levels_for(c->g, std::min(c->ml_max_levels, kMlMaxLevels));
+    int L = ml_levels_for(c->g, std::min(c->ml_max_levels, kMlMaxLevels));
+    bool exact = false;
+    if (c->ml_exact) {  // the first level small enough to solve exactly ends the hierarchy
+        GridArgs gl = c->g;
+        for (int l = 1; l <= L; ++l) {
+            gl = ml_coarse_grid(gl);
+            if (3 * gl.nn <= kMlExactMax) {
+                L = l;
+                exact = true;
+                break;
+            }
+        }
+    }
     const long n = ndof(c);
     MlHierarchy<double>& h = c->ml64;
     MlHierarchy<float>& h32 = c->ml32;
     h.levels = h32.levels = L;
+    h.ainv = nullptr;
+    h32.ainv = nullptr;
     h.g[0] = h32.g[0] = c->g;
     if (f32) HXA(c->ml_z32, n);
     else HXA(c->ml_z, n);
@@ -390,6 +421,22 @@ int ml_setup(hx_ctx* c, bool f32) {
         gl = gc;
         Ef = c->ml_E[l].p;
     }
+    if (exact) {
+        const long N = 3 * gl.nn;
+        const uint8_t* fx = c->has_fixed ? c->fixed.p : nullptr;
+        HXA(c->ml_A, N * N);
+        if (f32) {
+            HXA(c->ml_ainv32, N * N);
+            launch_ml_exact_setup<float>(gl, Ef, c->Kbin_d.p, (double)(1 << L), fx, 1 << L, c->g, c->ml_A.p,
+                                         c->ml_ainv32.p, c->stream);
+            h32.ainv = c->ml_ainv32.p;
+        } else {
+            HXA(c->ml_ainv, N * N);
+            launch_ml_exact_setup<double>(gl, Ef, c->Kbin_d.p, (double)(1 << L), fx, 1 << L, c->g, c->ml_A.p,
+                                          c->ml_ainv.p, c->stream);
+            h.ainv = c->ml_ainv.p;
+        }
+    }
     HXK();
     return HX_OK;
 }
@@ -425,12 +472,14 @@ int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& 
             for (int b = 0; b < batch; ++b) {
                 double* pin = (b & 1) ? v.p2 : v.p;
                 double* pout = (b & 1) ? v.p : v.p2;
-                if (ml) mv64(c, pin, v.q, c->E.p, o.shift, MV_PCG, HX_MV_PARITY, nr, c->ml_z.p, nullptr, pout);
-                else mv64(c, pin, v.q, c->E.p, o.shift, MV_PCG, o.matvec, nr, v.r, c->minv.p, pout);
+                int npq = 0;  // alpha: reduced by the update from the operator's partials
+                if (ml)
+                    mv64(c, pin, v.q, c->E.p, o.shift, MV_PCG, HX_MV_PARITY, nr, c->ml_z.p, nullptr, pout, true, &npq);
+                else mv64(c, pin, v.q, c->E.p, o.shift, MV_PCG, o.matvec, nr, v.r, c->minv.p, pout, true, &npq);
                 {
                     TimeScope ts(c, HX_K_UPDATE);
                     launch_pcg_update<double>(n, v.x, pout, v.r, v.q, c->minv.p, c->st, c->partials.p, c->stream,
-                                              nr, !ml);
+                                              nr, !ml, c->partials.p + kOpPartials, npq);
                 }
                 if (ml) {
                     TimeScope ts(c, HX_K_PRECOND);
@@ -511,14 +560,16 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
                 for (int b = 0; b < batch; ++b) {
                     float* pin = (b & 1) ? v.p32b : v.p32;
                     float* pout = (b & 1) ? v.p32 : v.p32b;
+                    int npq = 0;  // alpha: reduced by the update from the operator's partials
                     if (ml)
-                        mv32(c, pin, v.q32, c->E32.p, o.shift, MV_PCG, HX_MV_PARITY, nr, c->ml_z32.p, nullptr, pout);
+                        mv32(c, pin, v.q32, c->E32.p, o.shift, MV_PCG, HX_MV_PARITY, nr, c->ml_z32.p, nullptr, pout,
+                             true, &npq);
                     else
-                        mv32(c, pin, v.q32, c->E32.p, o.shift, MV_PCG, v32, nr, v.r32, c->minv32.p, pout);
+                        mv32(c, pin, v.q32, c->E32.p, o.shift, MV_PCG, v32, nr, v.r32, c->minv32.p, pout, true, &npq);
                     {
                         TimeScope ts(c, HX_K_UPDATE);
                         launch_pcg_update<float>(n, v.x32, pout, v.r32, v.q32, c->minv32.p, c->st, c->partials.p,
-                                                 c->stream, nr, !ml);
+                                                 c->stream, nr, !ml, c->partials.p + kOpPartials, npq);
                     }
                     if (ml) {
                         TimeScope ts(c, HX_K_PRECOND);
@@ -844,6 +895,9 @@ int hx_ctx_create(int device, hx_ctx** out) {
     hx_ctx* c = new hx_ctx();
     c->device = device;
     if (const char* ev = std::getenv("HEXAT_NO_GRAPH")) c->graphs_on = ev[0] == '0';
+    // cap on the multilevel preconditioner's coarse levels (default: every 2:1 coarsening)
+    if (const char* ev = std::getenv("HEXAT_ML_LEVELS")) c->ml_max_levels = std::max(0, std::atoi(ev));
+    if (const char* ev = std::getenv("HEXAT_ML_EXACT")) c->ml_exact = ev[0] != '0';
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc((void**)&c->st, kMaxRhs * sizeof(PcgState)) != hipSuccess ||
         hipHostMalloc((void**)&c->st_host, kMaxRhs * sizeof(PcgState), hipHostMallocDefault) != hipSuccess) {
@@ -963,7 +1017,7 @@ int hx_grid_set(hx_ctx* c, int nx, int ny, int nz, double dx, double dy, double 
                                     (long)matvec_parity_blocks(g, 1, 4) + 8, (long)matvec_parity_blocks(g, 2, 4) + 8,
                                     (long)matvec_gather_blocks(g) + 8,
                                     (long)helm_matvec_blocks(g) + 8, (long)element_post_blocks(g) + 8});
-    HXA(c->partials, np * kMaxRhs);
+    HXA(c->partials, kOpPartials + np * kMaxRhs);
     HXA(c->small, 64);
     float kb32[576];
     for (int t = 0; t < 576; ++t) kb32[t] = (float)c->Kbin[t];

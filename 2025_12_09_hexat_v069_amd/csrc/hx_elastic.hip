@@ -668,8 +668,8 @@ int launch_sf(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, const 
 }  // namespace
 
 template <typename T>
-void launch_matvec_parity(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, const MvShape& sh,
-                          hipStream_t s, int nrhs) {
+int launch_matvec_parity(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, const MvShape& sh,
+                         hipStream_t s, int nrhs) {
     int nb;
     if (sh.pdepth == 2)
         nb = sh.tile == 1   ? launch_sf<T, 16, 32, 2>(g, K, a, sh, s, nrhs)
@@ -679,26 +679,31 @@ void launch_matvec_parity(const GridArgs& g, const ParityK<T>& K, const MvArgs<T
         nb = sh.tile == 1   ? launch_sf<T, 16, 32, 1>(g, K, a, sh, s, nrhs)
              : sh.tile == 2 ? launch_sf<T, 32, 8, 1>(g, K, a, sh, s, nrhs)
                             : launch_sf<T, 16, 16, 1>(g, K, a, sh, s, nrhs);
-    if (a.mode == MV_PCG) launch_fin_alpha(a.st, a.partials, nb, nrhs, s);
+    if (a.mode == MV_PCG && !a.defer_alpha) launch_fin_alpha(a.st, a.partials, nb, nrhs, s);
+    return nb;
 }
 
 template <typename T>
-void launch_matvec_dense(const GridArgs& g, const T* Kbin, const MvArgs<T>& a, const MvShape& sh, hipStream_t s,
-                         int nrhs) {
+int launch_matvec_dense(const GridArgs& g, const T* Kbin, const MvArgs<T>& a, const MvShape& sh, hipStream_t s,
+                        int nrhs) {
     const int nb = launch_zmarch<T, 16, 16, EL_DENSE>(g, DenseK<T>{Kbin}, a, sh, s, nrhs);
-    if (a.mode == MV_PCG) launch_fin_alpha(a.st, a.partials, nb, nrhs, s);
+    if (a.mode == MV_PCG && !a.defer_alpha) launch_fin_alpha(a.st, a.partials, nb, nrhs, s);
+    return nb;
 }
 
-void launch_matvec_mfma(const GridArgs& g, const float* Kbin, const MvArgs<float>& a, const MvShape& sh,
-                        hipStream_t s, int nrhs) {
+int launch_matvec_mfma(const GridArgs& g, const float* Kbin, const MvArgs<float>& a, const MvShape& sh,
+                       hipStream_t s, int nrhs) {
     const int nb = launch_zmarch<float, 16, 16, EL_MFMA>(g, DenseK<float>{Kbin}, a, sh, s, nrhs);
-    if (a.mode == MV_PCG) launch_fin_alpha(a.st, a.partials, nb, nrhs, s);
+    if (a.mode == MV_PCG && !a.defer_alpha) launch_fin_alpha(a.st, a.partials, nb, nrhs, s);
+    return nb;
 }
 
-void launch_matvec_gather(const GridArgs& g, const double* Kbin, const MvArgs<double>& a, hipStream_t s,
-                          int nrhs) {
-    hipLaunchKernelGGL(k_matvec_gather, dim3(matvec_gather_blocks(g), nrhs), dim3(256), 0, s, g, Kbin, a);
-    if (a.mode == MV_PCG) launch_fin_alpha(a.st, a.partials, matvec_gather_blocks(g), nrhs, s);
+int launch_matvec_gather(const GridArgs& g, const double* Kbin, const MvArgs<double>& a, hipStream_t s,
+                         int nrhs) {
+    const int nb = matvec_gather_blocks(g);
+    hipLaunchKernelGGL(k_matvec_gather, dim3(nb, nrhs), dim3(256), 0, s, g, Kbin, a);
+    if (a.mode == MV_PCG && !a.defer_alpha) launch_fin_alpha(a.st, a.partials, nb, nrhs, s);
+    return nb;
 }
 
 template <typename T>
@@ -713,13 +718,13 @@ void launch_elastic_diag(const GridArgs& g, const double kd[3], const double* E,
                        E, diag);
 }
 
-template void launch_matvec_parity<double>(const GridArgs&, const ParityK<double>&, const MvArgs<double>&, const MvShape&,
+template int launch_matvec_parity<double>(const GridArgs&, const ParityK<double>&, const MvArgs<double>&, const MvShape&,
                                            hipStream_t, int);
-template void launch_matvec_parity<float>(const GridArgs&, const ParityK<float>&, const MvArgs<float>&, const MvShape&,
+template int launch_matvec_parity<float>(const GridArgs&, const ParityK<float>&, const MvArgs<float>&, const MvShape&,
                                           hipStream_t, int);
-template void launch_matvec_dense<double>(const GridArgs&, const double*, const MvArgs<double>&, const MvShape&, hipStream_t,
+template int launch_matvec_dense<double>(const GridArgs&, const double*, const MvArgs<double>&, const MvShape&, hipStream_t,
                                           int);
-template void launch_matvec_dense<float>(const GridArgs&, const float*, const MvArgs<float>&, const MvShape&, hipStream_t,
+template int launch_matvec_dense<float>(const GridArgs&, const float*, const MvArgs<float>&, const MvShape&, hipStream_t,
                                          int);
 template void launch_elastic_minv<double>(const GridArgs&, const double*, const double*, const uint8_t*, double,
                                           double*, hipStream_t);

@@ -115,8 +115,10 @@ __device__ void pcg_after_update(PcgState* st, double rz_new, double rr, bool se
 template <typename T>
 __global__ __launch_bounds__(NT) void k_pcg_update(long n, T* __restrict__ x, const T* __restrict__ p,
                                                    T* __restrict__ r, const T* __restrict__ q,
-                                                   const T* __restrict__ minv, PcgState* st, double* partials) {
+                                                   const T* __restrict__ minv, PcgState* st, double* partials,
+                                                   const double* __restrict__ pq_partials, unsigned npq) {
     __shared__ double red[NT / 64];
+    __shared__ double s_alpha;
     {
         const long off = (long)blockIdx.y * n;
         x += off; p += off; r += off; q += off;
@@ -124,7 +126,28 @@ __global__ __launch_bounds__(NT) void k_pcg_update(long n, T* __restrict__ x, co
         partials += (long)blockIdx.y * gridDim.x * 2;
     }
     if (st->done) return;
-    const double alpha = st->alpha;
+    double alpha;
+    if (npq > 0) {  // alpha = rz / p'q from the operator's partials, in every block (same order, same value)
+        const double pq = reduce_partials<NT>(pq_partials + (size_t)blockIdx.y * npq, 1, 0, npq, red);
+        if (threadIdx.x == 0) {
+            const bool ok = fabs(pq) >= 1e-300;  // also catches NaN (GPUSolver.jl:462-466)
+            s_alpha = ok ? st->rz / pq : 0.0;
+            if (blockIdx.x == 0) {
+                st->pq = pq;
+                if (ok) st->alpha = s_alpha;
+                else {
+                    st->done = 1;
+                    st->status = 4;
+                }
+            }
+            if (!ok) s_alpha = NAN;
+        }
+        __syncthreads();
+        alpha = s_alpha;
+        if (alpha != alpha) return;  // breakdown: block 0 has recorded it
+    } else {
+        alpha = st->alpha;
+    }
     double rz = 0.0, rr = 0.0;
     auto one = [&](T& xi, T pi, T& ri, T qi, T mi) {
         const double m = (double)mi;
@@ -788,8 +811,9 @@ void launch_pcg_init(long n, const T* b, T* x, const T* q, const T* minv, T* r, 
 }
 template <typename T>
 void launch_pcg_update(long n, T* x, const T* p, T* r, const T* q, const T* minv, PcgState* st, double* partials,
-                       hipStream_t s, int nrhs, bool set_beta) {
-    hipLaunchKernelGGL((k_pcg_update<T>), dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, x, p, r, q, minv, st, partials);
+                       hipStream_t s, int nrhs, bool set_beta, const double* pq_partials, int npq) {
+    hipLaunchKernelGGL((k_pcg_update<T>), dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, x, p, r, q, minv, st, partials,
+                       pq_partials, (unsigned)(pq_partials ? npq : 0));
     // without set_beta the explicit preconditioner's finaliser reads these
     // partials (r'r at index 2b+1) after its own r'z and finishes the iteration
     if (set_beta)
@@ -911,9 +935,9 @@ template void launch_pcg_init<double>(long, const double*, double*, const double
 template void launch_pcg_init<float>(long, const float*, float*, const float*, const float*, float*, float*, double,
                                      PcgState*, double*, hipStream_t, int);
 template void launch_pcg_update<double>(long, double*, const double*, double*, const double*, const double*,
-                                        PcgState*, double*, hipStream_t, int, bool);
+                                        PcgState*, double*, hipStream_t, int, bool, const double*, int);
 template void launch_pcg_update<float>(long, float*, const float*, float*, const float*, const float*, PcgState*,
-                                       double*, hipStream_t, int, bool);
+                                       double*, hipStream_t, int, bool, const double*, int);
 template void launch_pcg_pupdate<double>(long, const double*, const double*, double*, const PcgState*, hipStream_t,
                                          int);
 template void launch_pcg_pupdate<float>(long, const float*, const float*, float*, const PcgState*, hipStream_t, int);

@@ -34,20 +34,24 @@ struct MvArgs {
     const T* r;
     const T* minv;     // shared by all columns
     T* p_out;
+    // host side only: MV_PCG without the alpha finaliser -- the PCG update
+    // then reduces the p'q partials itself (launch_pcg_update's pq_partials)
+    int defer_alpha = 0;
 };
 
-// tile: 0 -> 16x16 element columns (default), 1 -> 16x32, 2 -> 32x8
+// tile: 0 -> 16x16 element columns (default), 1 -> 16x32, 2 -> 32x8.
+// The launchers return the number of workgroups per column (= p'q partials).
 template <typename T>
-void launch_matvec_parity(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, const MvShape& sh,
+int launch_matvec_parity(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, const MvShape& sh,
                           hipStream_t s, int nrhs = 1);
 template <typename T>
-void launch_matvec_dense(const GridArgs& g, const T* Kbin, const MvArgs<T>& a, const MvShape& sh, hipStream_t s,
+int launch_matvec_dense(const GridArgs& g, const T* Kbin, const MvArgs<T>& a, const MvShape& sh, hipStream_t s,
                          int nrhs = 1);
-void launch_matvec_gather(const GridArgs& g, const double* Kbin, const MvArgs<double>& a, hipStream_t s,
+int launch_matvec_gather(const GridArgs& g, const double* Kbin, const MvArgs<double>& a, hipStream_t s,
                           int nrhs = 1);
 // FP32 element contraction on MFMA: dense [16 elem x 24] x Ke^T tiles with
 // v_mfma_f32_16x16x4f32, inside the same z-marching tile
-void launch_matvec_mfma(const GridArgs& g, const float* Kbin, const MvArgs<float>& a, const MvShape& sh,
+int launch_matvec_mfma(const GridArgs& g, const float* Kbin, const MvArgs<float>& a, const MvShape& sh,
                         hipStream_t s, int nrhs = 1);
 int matvec_parity_blocks(const GridArgs& g, int tile, int kchunk);
 int matvec_dense_blocks(const GridArgs& g, int kchunk);
@@ -76,10 +80,14 @@ void launch_pcg_init(long n, const T* b, T* x, const T* q, const T* minv, T* r, 
                      PcgState* st, double* partials, hipStream_t s, int nrhs = 1);
 // set_beta = false leaves the iteration's bookkeeping (||r||, beta, the
 // convergence decision) to an explicit preconditioner's finaliser, which reads
-// the update's partials (launch_ml_apply); no finaliser is launched here
+// the update's partials (launch_ml_apply); no finaliser is launched here.
+// pq_partials (npq per column): the operator's p'q partials of an MV_PCG
+// launch with defer_alpha; every block reduces them to alpha itself (block 0
+// records it and the breakdown), so no alpha finaliser runs in between.
 template <typename T>
 void launch_pcg_update(long n, T* x, const T* p, T* r, const T* q, const T* minv, PcgState* st,
-                       double* partials, hipStream_t s, int nrhs = 1, bool set_beta = true);
+                       double* partials, hipStream_t s, int nrhs = 1, bool set_beta = true,
+                       const double* pq_partials = nullptr, int npq = 0);
 template <typename T>
 void launch_pcg_pupdate(long n, const T* r, const T* minv, T* p, const PcgState* st, hipStream_t s, int nrhs = 1);
 void launch_pcg_reset(PcgState* st, double tol, double shift, int max_iter, int check_every, hipStream_t s,
@@ -129,9 +137,11 @@ void launch_mul(long n, const double* a, const double* b, double* out, hipStream
 
 // ---- multilevel diagonal scaling (explicit preconditioner z = M^-1 r) ---------
 constexpr int kMlMaxLevels = 12;
+constexpr int kMlExactMax = 192;  // DOFs of a coarsest level solved exactly
 template <typename T>
 struct MlHierarchy {
     int levels = 0;                  // coarse levels L (0: Jacobi only)
+    const T* ainv = nullptr;         // non-null: level L is solved exactly, y_L = A_L^-1 r_L (dense, 3 nn_L square)
     GridArgs g[kMlMaxLevels + 1];    // g[0] = fine grid, g[l] = level l
     T* r[kMlMaxLevels + 1] = {};     // restricted residual of level l >= 1
     T* y[kMlMaxLevels + 1] = {};     // level output D_l^-1 r_l + P y_{l+1}
@@ -155,6 +165,13 @@ template <typename T>
 void launch_ml_apply(const MlHierarchy<T>& h, const T* r, const T* minv0, T* z, PcgState* st, double* partials,
                      int fin_mode, hipStream_t s);
 int ml_partial_blocks();
+// A_L^-1 of the coarsest level gc (moduli Ec, element stiffness h Kbin, DOFs
+// fixed where the fine DOF at stride * (I,J,K) is, or without stiffness, give
+// zero rows and columns), built and inverted on the device; A is scratch of
+// (3 nn)^2 doubles
+template <typename T>
+void launch_ml_exact_setup(const GridArgs& gc, const double* Ec, const double* Kbin, double h, const uint8_t* fixed_f,
+                           int stride, const GridArgs& gfine, double* A, T* ainv, hipStream_t s);
 void ml_prepare_kernels();  // once per process, outside graph capture
 
 // ---- Helmholtz filter --------------------------------------------------------

@@ -309,8 +309,18 @@ __global__ __launch_bounds__(kCoarseThreads) void k_ml_coarse(MlHierarchy<T> h, 
             else restrict_node(gf, (const T*)R(l - 1), gc, n, acc);           // from LDS
             for (int c = 0; c < 3; ++c) {
                 R(l)[3 * n + c] = acc[c];
-                if (l == L) Y(l)[3 * n + c] = T(h.w[l]) * h.minv[l][3 * n + c] * acc[c];
+                if (l == L && h.ainv == nullptr) Y(l)[3 * n + c] = T(h.w[l]) * h.minv[l][3 * n + c] * acc[c];
             }
+        }
+        __syncthreads();
+    }
+    if (h.ainv != nullptr) {  // exact coarsest level: y_L = A_L^-1 r_L
+        const int N = 3 * (int)h.g[L].nn;
+        const T* rl = R(L);
+        for (int d = threadIdx.x; d < N; d += kCoarseThreads) {
+            T acc = T(0);
+            for (int j = 0; j < N; ++j) acc += h.ainv[d * N + j] * rl[j];
+            Y(L)[d] = acc;
         }
         __syncthreads();
     }
@@ -354,6 +364,64 @@ __global__ __launch_bounds__(NT) void k_fin_rz(PcgState* st, const double* parti
         } else {
             st->scal[5] = s;
         }
+    }
+}
+
+// The coarsest level solved exactly: A assembled element by element (an
+// element's 24 DOFs are distinct, so the 576 entries of one element go in
+// parallel), boundary DOFs and DOFs without stiffness replaced by identity
+// rows and columns, in-place Gauss-Jordan inversion (A is symmetric positive
+// definite then: no pivoting), those rows and columns of the inverse zeroed.
+// One workgroup, A in global memory; run once per solve.
+constexpr int kExactThreads = 1024;
+template <typename T>
+__global__ __launch_bounds__(kExactThreads) void k_ml_exact_setup(GridArgs gc, const double* __restrict__ Ec,
+                                                                  const double* __restrict__ Kbin, double h,
+                                                                  const uint8_t* __restrict__ fixed_f, int stride,
+                                                                  int NXf, int NYf, double* A, T* __restrict__ ainv) {
+    __shared__ int keep[kMlExactMax];
+    const int N = 3 * (int)gc.nn, t = threadIdx.x;
+    for (int i = t; i < N * N; i += kExactThreads) A[i] = 0.0;
+    __syncthreads();
+    for (int e = 0; e < (int)gc.ne; ++e) {
+        const int I = e % gc.nx, J = (e / gc.nx) % gc.ny, K = e / (gc.nx * gc.ny);
+        const double Ee = h * Ec[e];
+        for (int ab = t; ab < 576; ab += kExactThreads) {
+            const int ra = ab / 24, cb = ab - 24 * ra, La = ra / 3, Lb = cb / 3;
+            const int na = (I + (La & 1)) + gc.NX * ((J + ((La >> 1) & 1)) + gc.NY * (K + (La >> 2)));
+            const int nb = (I + (Lb & 1)) + gc.NX * ((J + ((Lb >> 1) & 1)) + gc.NY * (K + (Lb >> 2)));
+            A[(3 * na + ra - 3 * La) * N + 3 * nb + cb - 3 * Lb] += Ee * Kbin[ab];
+        }
+        __syncthreads();
+    }
+    for (int d = t; d < N; d += kExactThreads) {
+        int i, j, k;
+        node_ijk(gc, d / 3, i, j, k);
+        const long f = (long)stride * i + (long)NXf * ((long)stride * j + (long)NYf * ((long)stride * k));
+        keep[d] = (fixed_f == nullptr || fixed_f[3 * f + d % 3] == 0) && A[(long)d * N + d] > 0.0;
+    }
+    __syncthreads();
+    for (int idx = t; idx < N * N; idx += kExactThreads) {
+        const int r = idx / N, q = idx - r * N;
+        if (!keep[r] || !keep[q]) A[idx] = r == q ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int k = 0; k < N; ++k) {  // sweep k: pivot row, the rest, pivot column
+        const double d = A[k * N + k];
+        for (int j = t; j < N; j += kExactThreads)
+            if (j != k) A[k * N + j] /= d;
+        __syncthreads();
+        for (int idx = t; idx < N * N; idx += kExactThreads) {
+            const int i = idx / N, j = idx - i * N;
+            if (i != k && j != k) A[idx] -= A[i * N + k] * A[k * N + j];
+        }
+        __syncthreads();
+        for (int i = t; i < N; i += kExactThreads) A[i * N + k] = i == k ? 1.0 / d : -A[i * N + k] / d;
+        __syncthreads();
+    }
+    for (int idx = t; idx < N * N; idx += kExactThreads) {
+        const int r = idx / N, q = idx - r * N;
+        ainv[idx] = (keep[r] && keep[q]) ? T(A[idx]) : T(0);
     }
 }
 
@@ -462,6 +530,17 @@ void launch_ml_apply(const MlHierarchy<T>& h, const T* r, const T* minv0, T* z, 
 }
 
 int ml_partial_blocks() { return kProlongBlocks; }
+
+template <typename T>
+void launch_ml_exact_setup(const GridArgs& gc, const double* Ec, const double* Kbin, double h, const uint8_t* fixed_f,
+                           int stride, const GridArgs& gfine, double* A, T* ainv, hipStream_t s) {
+    hipLaunchKernelGGL((k_ml_exact_setup<T>), dim3(1), dim3(kExactThreads), 0, s, gc, Ec, Kbin, h, fixed_f, stride,
+                       gfine.NX, gfine.NY, A, ainv);
+}
+template void launch_ml_exact_setup<double>(const GridArgs&, const double*, const double*, double, const uint8_t*, int,
+                                            const GridArgs&, double*, double*, hipStream_t);
+template void launch_ml_exact_setup<float>(const GridArgs&, const double*, const double*, double, const uint8_t*, int,
+                                           const GridArgs&, double*, float*, hipStream_t);
 
 // allow the coarse kernel's dynamic LDS above the default 64 KiB
 void ml_prepare_kernels() {

@@ -443,11 +443,28 @@ def _node_sum(E, nx, ny, nz):
     return s
 
 
-def ml_hierarchy(g: Grid, Ke, Ee, fixed, max_levels=12):
-    """Per level (l = 0..L): element counts, inverse diagonal (NZ, NY, NX, 3)."""
-    kd = np.diag(_f64(Ke).reshape(24, 24))[:3]
+ML_EXACT_MAX = 192  # DOFs of a coarsest level solved exactly
+
+
+def ml_hierarchy(g: Grid, Ke, Ee, fixed, max_levels=12, exact=True):
+    """Per level (l = 0..L): (element counts, inverse diagonal (NZ, NY, NX, 3), A_L^-1 or None).
+    With `exact`, the first level with at most ML_EXACT_MAX DOFs ends the hierarchy and is solved
+    exactly: its stiffness assembled from the averaged moduli and h Ke, DOFs fixed where the fine DOF
+    at the same place is (or without stiffness) as identity rows and columns, inverted, those rows and
+    columns of the inverse zeroed."""
+    Ke = _f64(Ke).reshape(24, 24)
+    kd = np.diag(Ke)[:3]
     fx = np.asarray(fixed, dtype=bool).reshape(g.nz + 1, g.ny + 1, g.nx + 1, 3)
     L = ml_levels(g, max_levels)
+    if exact:
+        n = (g.nx, g.ny, g.nz)
+        for l in range(1, L + 1):
+            n = tuple(v // 2 for v in n)
+            if 3 * (n[0] + 1) * (n[1] + 1) * (n[2] + 1) <= ML_EXACT_MAX:
+                L = l
+                break
+        else:
+            exact = False
     levels = []
     E, n = _f64(Ee), (g.nx, g.ny, g.nz)
     for l in range(L + 1):
@@ -458,13 +475,28 @@ def ml_hierarchy(g: Grid, Ke, Ee, fixed, max_levels=12):
         S = _node_sum(E, *n)[..., None]
         d = h * kd[None, None, None, :] * S
         free = ~fx[::h, ::h, ::h] & (S > 0)
-        levels.append((n, np.where(free, 1.0 / np.where(free, d, 1.0), 0.0)))
+        ainv = None
+        if exact and l == L and l > 0:
+            gc = Grid(*n)
+            N = gc.ndof
+            A = np.zeros((N, N))
+            dof = (3 * gc.elements()[:, :, None] + np.arange(3)[None, None, :]).reshape(-1, 24)
+            for e in range(gc.ne):
+                A[np.ix_(dof[e], dof[e])] += E[e] * h * Ke
+            keep = (~fx[::h, ::h, ::h]).reshape(-1) & (np.diag(A) > 0)
+            A[~keep, :] = 0.0
+            A[:, ~keep] = 0.0
+            A[~keep, ~keep] = 1.0
+            ainv = np.linalg.inv(A)
+            ainv[~keep, :] = 0.0
+            ainv[:, ~keep] = 0.0
+        levels.append((n, np.where(free, 1.0 / np.where(free, d, 1.0), 0.0), ainv))
     return levels
 
 
-def ml_apply(g: Grid, Ke, Ee, fixed, r, max_levels=12, levels=None):
+def ml_apply(g: Grid, Ke, Ee, fixed, r, max_levels=12, levels=None, exact=True):
     """z = M^-1 r of the multilevel preconditioner (masked where D0^-1 = 0)."""
-    levels = levels or ml_hierarchy(g, Ke, Ee, fixed, max_levels)
+    levels = levels or ml_hierarchy(g, Ke, Ee, fixed, max_levels, exact)
     L = len(levels) - 1
     rs = [_f64(r).reshape(g.nz + 1, g.ny + 1, g.nx + 1, 3)]
     for l in range(1, L + 1):
@@ -472,7 +504,10 @@ def ml_apply(g: Grid, Ke, Ee, fixed, r, max_levels=12, levels=None):
         for ax in (0, 1, 2):
             a = _restrict_axis(a, ax)
         rs.append(a)
-    y = levels[L][1] * rs[L]
+    if levels[L][2] is not None:
+        y = (levels[L][2] @ rs[L].ravel()).reshape(rs[L].shape)
+    else:
+        y = levels[L][1] * rs[L]
     for l in range(L - 1, -1, -1):
         p = y
         for ax in (0, 1, 2):
@@ -484,10 +519,10 @@ def ml_apply(g: Grid, Ke, Ee, fixed, r, max_levels=12, levels=None):
     return y.ravel()
 
 
-def ml_pcg(g: Grid, Ke, Ee, fixed, b, tol=1e-8, max_iter=100000, max_levels=12):
+def ml_pcg(g: Grid, Ke, Ee, fixed, b, tol=1e-8, max_iter=100000, max_levels=12, exact=True):
     """Plain PCG with the multilevel preconditioner (free DOFs; fixed stay 0)."""
     Ke, Ee, b = _f64(Ke), _f64(Ee), _f64(b)
-    levels = ml_hierarchy(g, Ke, Ee, fixed, max_levels)
+    levels = ml_hierarchy(g, Ke, Ee, fixed, max_levels, exact)
     mask = levels[0][1].ravel() != 0.0
     b = np.where(mask, b, 0.0)
     x = np.zeros(g.ndof)

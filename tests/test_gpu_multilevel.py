@@ -50,9 +50,30 @@ def test_ml_apply_matches_oracle(ox, hx, ctx, dims, h, precision):
     r = np.where(m0 != 0, rng.standard_normal(g.ndof), 0.0)
     z, L = ctx.test_ml_apply(Ee, r, precision)
     zo = ox.ml_apply(g, Ke, Ee, fixed, r)
-    assert L == ox.ml_levels(g)
-    tol = 1e-12 if precision == 0 else 2e-6
+    assert L == len(ox.ml_hierarchy(g, Ke, Ee, fixed)) - 1
+    tol = 1e-10 if precision == 0 else 2e-6  # the exact coarsest level: two inversions agree to ~cond * eps
     assert np.allclose(z, zo, rtol=tol, atol=tol * np.abs(zo).max())
+
+
+@pytest.mark.parametrize("dims", [(16, 8, 8), (64, 32, 32)])
+def test_ml_apply_diagonal_coarsest_matches_oracle(ox, hx, monkeypatch, dims):
+    """HEXAT_ML_EXACT=0: every level scaled by its diagonal, down to the last
+    grid that halves."""
+    _say("ml apply, no exact level", dims)
+    monkeypatch.setenv("HEXAT_ML_EXACT", "0")
+    c = hx.Context(0)
+    try:
+        g, fixed, _ = _setup(ox, c, dims)
+        rng = np.random.default_rng(7)
+        Ee = 1e-9 + rng.uniform(0, 1, g.ne) ** 3
+        Ke = ox.ke_hex(1, 1, 1, 0.3)
+        r = np.where(fixed == 0, rng.standard_normal(g.ndof), 0.0)
+        z, L = c.test_ml_apply(Ee, r, 0)
+        zo = ox.ml_apply(g, Ke, Ee, fixed, r, exact=False)
+        assert L == ox.ml_levels(g)
+        assert np.allclose(z, zo, rtol=1e-12, atol=1e-12 * np.abs(zo).max())
+    finally:
+        c.close()
 
 
 @pytest.mark.parametrize("dims", [(16, 8, 8), (32, 16, 16)])
