@@ -28,7 +28,7 @@ HX_ERR_OOM = 9
 HX_ZERO_RHS = 10
 
 HX_FP64, HX_FP32_IR = 0, 1
-HX_PC_JACOBI, HX_PC_LEARNED, HX_PC_MULTILEVEL = 0, 1, 2
+HX_PC_JACOBI, HX_PC_LEARNED, HX_PC_MULTILEVEL, HX_PC_VCYCLE = 0, 1, 2, 3
 HX_MV_AUTO, HX_MV_PARITY, HX_MV_DENSE, HX_MV_GATHER, HX_MV_MFMA = 0, 1, 2, 3, 4
 HX_K_MATVEC, HX_K_UPDATE, HX_K_PUPDATE, HX_K_HELM, HX_K_ELEM, HX_K_MATVEC32, HX_K_PRECOND = range(7)
 HX_FIELD_X, HX_FIELD_XPHYS, HX_FIELD_U, HX_FIELD_VM, HX_FIELD_DC = range(5)
@@ -47,7 +47,7 @@ EXPORTS = [
 ]
 #: test hooks declared in include/hexat_testing.h
 TEST_EXPORTS = ["hx_test_unet_nparam", "hx_test_unet_init", "hx_test_unet_forward", "hx_test_unet_backward",
-                "hx_test_copy", "hx_test_ml_apply"]
+                "hx_test_copy", "hx_test_ml_apply", "hx_test_vc_apply"]
 
 
 class HexatError(RuntimeError):
@@ -149,6 +149,7 @@ def load():
         L.hx_time_operator_pc.argtypes = [vp, i, i, i, i, P(C.c_double)]
         L.hx_test_copy.argtypes = [vp, C.c_long, i, P(C.c_double)]
         L.hx_test_ml_apply.argtypes = [vp, pd, pd, pd, i, P(C.c_int)]
+        L.hx_test_vc_apply.argtypes = [vp, pd, pd, pd, i, P(C.c_int), P(C.c_int)]
         pf = P(C.c_float)
         L.hx_test_unet_nparam.argtypes = [vp, P(C.c_long)]
         L.hx_test_unet_init.argtypes = [vp, i, pf]
@@ -434,6 +435,15 @@ class Context:
         self.check(self.L.hx_test_ml_apply(self.h, dptr(Ee), dptr(r), dptr(z), int(precision), C.byref(lv)),
                    "hx_test_ml_apply")
         return z, lv.value
+
+    def test_vc_apply(self, Ee, r, precision=HX_FP64):
+        """z = M^-1 r of the V-cycle form; returns (z, coarse levels, smoothed levels)."""
+        Ee, r = f64(Ee, self.ne), f64(r, self.ndof)
+        z = np.empty(self.ndof)
+        lv, vl = C.c_int(0), C.c_int(0)
+        self.check(self.L.hx_test_vc_apply(self.h, dptr(Ee), dptr(r), dptr(z), int(precision), C.byref(lv),
+                                           C.byref(vl)), "hx_test_vc_apply")
+        return z, lv.value, vl.value
 
     # ---- timing -----------------------------------------------------------------------
     def timing(self, on: bool):

@@ -123,13 +123,18 @@ struct hx_ctx {
     // multilevel preconditioner: level moduli, inverse diagonals, work vectors
     int ml_max_levels = kMlMaxLevels;
     bool ml_exact = true;  // solve the first level with at most kMlExactMax DOFs exactly (and stop there)
-    DevBuf<double> ml_A, ml_ainv;
+    DevBuf<double> ml_ainv;
     DevBuf<float> ml_ainv32;
     DevBuf<double> ml_E[kMlMaxLevels + 1], ml_r[kMlMaxLevels + 1], ml_y[kMlMaxLevels + 1],
         ml_minv[kMlMaxLevels + 1], ml_z;
     DevBuf<float> ml_r32[kMlMaxLevels + 1], ml_y32[kMlMaxLevels + 1], ml_minv32[kMlMaxLevels + 1], ml_z32;
     MlHierarchy<double> ml64;
     MlHierarchy<float> ml32;
+    // V-cycle form: smoother weight, per smoothed level the smoother diagonal,
+    // residual and operator moduli
+    double vc_omega = 1.95;
+    DevBuf<double> vc_s[kMlMaxLevels + 1], vc_res[kMlMaxLevels + 1], vc_E[kMlMaxLevels + 1];
+    DevBuf<float> vc_s32[kMlMaxLevels + 1], vc_res32[kMlMaxLevels + 1], vc_E32[kMlMaxLevels + 1];
     // captured iteration batches (hipGraph), keyed by kernel arguments
     bool graphs_on = true;
     std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
@@ -363,8 +368,11 @@ int worst_code(const hx_ctx* c, int nrhs) {
 }
 
 // Levels of the multilevel preconditioner for the current moduli: coarse
-// moduli by 2x2x2 averaging, coarse inverse diagonals, work vectors.
-int ml_setup(hx_ctx* c, bool f32) {
+// moduli by 2x2x2 averaging, coarse inverse diagonals, work vectors; for the
+// V-cycle form also the smoothed levels' smoother diagonals, operator moduli
+// and vectors.  The fine level's moduli (E, E32) and inverse diagonal (minv,
+// minv32) are the solve's own.
+int ml_setup(hx_ctx* c, bool f32, bool vcycle = false) {
     ml_prepare_kernels();
     int L = ml_levels_for(c->g, std::min(c->ml_max_levels, kMlMaxLevels));
     bool exact = false;
@@ -386,6 +394,8 @@ int ml_setup(hx_ctx* c, bool f32) {
     h.ainv = nullptr;
     h32.ainv = nullptr;
     h.g[0] = h32.g[0] = c->g;
+    h.minv[0] = c->minv.p;
+    h32.minv[0] = c->minv32.p;
     if (f32) HXA(c->ml_z32, n);
     else HXA(c->ml_z, n);
     GridArgs gl = c->g;
@@ -393,6 +403,7 @@ int ml_setup(hx_ctx* c, bool f32) {
     for (int l = 1; l <= L; ++l) {
         const GridArgs gc = ml_coarse_grid(gl);
         const long nc = 3 * gc.nn;
+        h.g[l] = h32.g[l] = gc;  // both: the grids do not depend on the precision
         HXA(c->ml_E[l], gc.ne);
         launch_ml_coarsen_E(gl, Ef, gc, c->ml_E[l].p, nullptr, c->stream);
         const double kdl[3] = {c->kd[0] * (1 << l), c->kd[1] * (1 << l), c->kd[2] * (1 << l)};
@@ -402,7 +413,6 @@ int ml_setup(hx_ctx* c, bool f32) {
             HXA(c->ml_y32[l], nc);
             HXA(c->ml_minv32[l], nc);
             launch_ml_minv<float>(gc, kdl, c->ml_E[l].p, fx, 1 << l, c->g, c->ml_minv32[l].p, c->stream);
-            h32.g[l] = gc;
             h32.r[l] = c->ml_r32[l].p;
             h32.y[l] = c->ml_y32[l].p;
             h32.minv[l] = c->ml_minv32[l].p;
@@ -412,7 +422,6 @@ int ml_setup(hx_ctx* c, bool f32) {
             HXA(c->ml_y[l], nc);
             HXA(c->ml_minv[l], nc);
             launch_ml_minv<double>(gc, kdl, c->ml_E[l].p, fx, 1 << l, c->g, c->ml_minv[l].p, c->stream);
-            h.g[l] = gc;
             h.r[l] = c->ml_r[l].p;
             h.y[l] = c->ml_y[l].p;
             h.minv[l] = c->ml_minv[l].p;
@@ -424,20 +433,77 @@ int ml_setup(hx_ctx* c, bool f32) {
     if (exact) {
         const long N = 3 * gl.nn;
         const uint8_t* fx = c->has_fixed ? c->fixed.p : nullptr;
-        HXA(c->ml_A, N * N);
         if (f32) {
             HXA(c->ml_ainv32, N * N);
-            launch_ml_exact_setup<float>(gl, Ef, c->Kbin_d.p, (double)(1 << L), fx, 1 << L, c->g, c->ml_A.p,
-                                         c->ml_ainv32.p, c->stream);
+            launch_ml_exact_setup<float>(gl, Ef, c->Kbin_d.p, (double)(1 << L), fx, 1 << L, c->g, c->ml_ainv32.p,
+                                         c->stream);
             h32.ainv = c->ml_ainv32.p;
         } else {
             HXA(c->ml_ainv, N * N);
-            launch_ml_exact_setup<double>(gl, Ef, c->Kbin_d.p, (double)(1 << L), fx, 1 << L, c->g, c->ml_A.p,
-                                          c->ml_ainv.p, c->stream);
+            launch_ml_exact_setup<double>(gl, Ef, c->Kbin_d.p, (double)(1 << L), fx, 1 << L, c->g, c->ml_ainv.p,
+                                          c->stream);
             h.ainv = c->ml_ainv.p;
         }
     }
+    if (vcycle) {
+        const int nv = f32 ? vc_levels(h32) : vc_levels(h);
+        const uint8_t* fx = c->has_fixed ? c->fixed.p : nullptr;
+        for (int l = 0; l < nv; ++l) {
+            const GridArgs& gl = f32 ? h32.g[l] : h.g[l];
+            const long nl = 3 * gl.nn;
+            const double* El = l == 0 ? c->E.p : c->ml_E[l].p;
+            const double hl = (double)(1 << l);
+            if (f32) {
+                HXA(c->vc_s32[l], nl);
+                HXA(c->vc_res32[l], nl);
+                launch_vc_smoother<float>(gl, El, hl, c->Kbin_d.p, fx, 1 << l, c->g, c->vc_omega, c->vc_s32[l].p,
+                                          c->stream);
+                if (l > 0) {
+                    HXA(c->vc_E32[l], gl.ne);
+                    launch_vc_scale_E<float>(gl.ne, El, hl, c->vc_E32[l].p, c->stream);
+                }
+                h32.sinv[l] = c->vc_s32[l].p;
+                h32.vres[l] = c->vc_res32[l].p;
+                h32.Eop[l] = l == 0 ? c->E32.p : c->vc_E32[l].p;
+            } else {
+                HXA(c->vc_s[l], nl);
+                HXA(c->vc_res[l], nl);
+                launch_vc_smoother<double>(gl, El, hl, c->Kbin_d.p, fx, 1 << l, c->g, c->vc_omega, c->vc_s[l].p,
+                                           c->stream);
+                if (l > 0) {
+                    HXA(c->vc_E[l], gl.ne);
+                    launch_vc_scale_E<double>(gl.ne, El, hl, c->vc_E[l].p, c->stream);
+                }
+                h.sinv[l] = c->vc_s[l].p;
+                h.vres[l] = c->vc_res[l].p;
+                h.Eop[l] = l == 0 ? c->E.p : c->vc_E[l].p;
+            }
+        }
+    }
     HXK();
+    return HX_OK;
+}
+
+// z = M^-1 r of the explicit preconditioner into ml_z / ml_z32 (fin_mode as
+// launch_ml_apply's)
+int pc_apply64(hx_ctx* c, bool vcycle, const double* r, int fin_mode) {
+    if (vcycle) {
+        if (launch_vc_apply<double>(c->ml64, c->kp64, c->shape64, r, c->ml_z.p, c->st, c->partials.p, kOpPartials,
+                                    fin_mode, c->stream) != HX_OK)
+            return fail(c, HX_ERR_BAD_ARG, "grid too large for the V-cycle's partial sums");
+    } else {
+        launch_ml_apply<double>(c->ml64, r, c->minv.p, c->ml_z.p, c->st, c->partials.p, fin_mode, c->stream);
+    }
+    return HX_OK;
+}
+int pc_apply32(hx_ctx* c, bool vcycle, const float* r, int fin_mode) {
+    if (vcycle) {
+        if (launch_vc_apply<float>(c->ml32, c->kp32, c->shape32, r, c->ml_z32.p, c->st, c->partials.p, kOpPartials,
+                                   fin_mode, c->stream) != HX_OK)
+            return fail(c, HX_ERR_BAD_ARG, "grid too large for the V-cycle's partial sums");
+    } else {
+        launch_ml_apply<float>(c->ml32, r, c->minv32.p, c->ml_z32.p, c->st, c->partials.p, fin_mode, c->stream);
+    }
     return HX_OK;
 }
 
@@ -451,19 +517,20 @@ int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& 
                                 c->stream);
     // multilevel: explicit z = M^-1 r after each update; the operator forms
     // p = z + beta p_old (parity contraction, which has that fused form)
-    const bool ml = o.precond == HX_PC_MULTILEVEL;
-    int rc = ml ? ml_setup(c, false) : HX_OK;
+    const bool vc = o.precond == HX_PC_VCYCLE;
+    const bool ml = o.precond == HX_PC_MULTILEVEL || vc;
+    int rc = ml ? ml_setup(c, false, vc) : HX_OK;
     if (rc) return rc;
     launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream, nr);
     if (o.warm_start) launch_mask<double>(n, c->minv.p, v.x, c->stream, nr);
     else launch_fill(n * nr, 0.0, v.x, c->stream);
     mv64(c, v.x, v.q, c->E.p, 0.0, MV_PLAIN, o.matvec, nr);
     launch_pcg_init<double>(n, v.f, v.x, v.q, c->minv.p, v.r, v.p, o.shift, c->st, c->partials.p, c->stream, nr);
-    if (ml) launch_ml_apply<double>(c->ml64, v.r, c->minv.p, c->ml_z.p, c->st, c->partials.p, 1, c->stream);
+    if (ml && (rc = pc_apply64(c, vc, v.r, 1))) return rc;
     HXK();
     if ((rc = sync_state(c, nr))) return rc;
     const int mvv = ml ? HX_MV_PARITY : o.matvec;
-    const std::string key = batch_key(c, ml ? "f64ml" : "f64", nr, batch,
+    const std::string key = batch_key(c, vc ? "f64vc" : ml ? "f64ml" : "f64", nr, batch,
                                       mvv == HX_MV_AUTO ? c->mv_variant : mvv, o.shift, v.x, v.p, v.q, v.r, v.p2);
     // p is double-buffered (v.p, v.p2): each matvec forms p_new = D^-1 r + beta p_old
     // from the plane it stages anyway and writes it beside q; batches are even.
@@ -483,7 +550,7 @@ int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& 
                 }
                 if (ml) {
                     TimeScope ts(c, HX_K_PRECOND);
-                    launch_ml_apply<double>(c->ml64, v.r, c->minv.p, c->ml_z.p, c->st, c->partials.p, 0, c->stream);
+                    (void)pc_apply64(c, vc, v.r, 0);  // its one failure is caught at the start call
                 }
             }
         });
@@ -537,8 +604,9 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
     launch_elastic_minv<double>(c->g, c->kd, c->E.p, c->has_fixed ? c->fixed.p : nullptr, o.shift, c->minv.p,
                                 c->stream);
     launch_convert<double, float>(n, c->minv.p, c->minv32.p, c->stream);
-    const bool ml = o.precond == HX_PC_MULTILEVEL;  // explicit z = M^-1 r (see pcg_fp64)
-    if (ml && (rc = ml_setup(c, true))) return rc;
+    const bool vc = o.precond == HX_PC_VCYCLE;
+    const bool ml = o.precond == HX_PC_MULTILEVEL || vc;  // explicit z = M^-1 r (see pcg_fp64)
+    if (ml && (rc = ml_setup(c, true, vc))) return rc;
     launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream, nr);
     if (o.warm_start) launch_mask<double>(n, c->minv.p, v.x, c->stream, nr);
     else launch_fill(n * nr, 0.0, v.x, c->stream);
@@ -547,12 +615,12 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
     mv64(c, v.x, v.q, c->E.p, 0.0, MV_PLAIN, HX_MV_PARITY, nr);
     launch_true_residual(n, v.f, v.q, v.x, c->minv.p, o.shift, nullptr, v.r32, c->st, c->partials.p, 3, c->stream,
                          nr);
-    if (ml) launch_ml_apply<float>(c->ml32, v.r32, c->minv32.p, c->ml_z32.p, c->st, c->partials.p, 2, c->stream);
+    if (ml && (rc = pc_apply32(c, vc, v.r32, 2))) return rc;
     launch_ir_resume(c->st, o.tol, kIrDelta, 1, c->stream, nr);
     HXK();
     if ((rc = sync_state(c, nr))) return rc;
     int refinements = 0;
-    const std::string key = batch_key(c, ml ? "f32ml" : "f32", nr, batch, ml ? HX_MV_PARITY : v32, o.shift, v.x32,
+    const std::string key = batch_key(c, vc ? "f32vc" : ml ? "f32ml" : "f32", nr, batch, ml ? HX_MV_PARITY : v32, o.shift, v.x32,
                                       v.p32, v.q32, v.r32, v.p32b);
     while (!all_done(c, nr)) {
         while (!all_done(c, nr)) {
@@ -573,8 +641,7 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
                     }
                     if (ml) {
                         TimeScope ts(c, HX_K_PRECOND);
-                        launch_ml_apply<float>(c->ml32, v.r32, c->minv32.p, c->ml_z32.p, c->st, c->partials.p, 0,
-                                               c->stream);
+                        (void)pc_apply32(c, vc, v.r32, 0);  // its one failure is caught at the start call
                     }
                 }
             });
@@ -587,7 +654,7 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
         mv64(c, v.x, v.q, c->E.p, 0.0, MV_PLAIN, HX_MV_PARITY, nr);
         launch_true_residual(n, v.f, v.q, v.x, c->minv.p, o.shift, nullptr, v.r32, c->st, c->partials.p, 2,
                              c->stream, nr);
-        if (ml) launch_ml_apply<float>(c->ml32, v.r32, c->minv32.p, c->ml_z32.p, c->st, c->partials.p, 2, c->stream);
+        if (ml && (rc = pc_apply32(c, vc, v.r32, 2))) return rc;
         launch_ir_resume(c->st, o.tol, kIrDelta, 0, c->stream, nr);
         HXK();
         ++refinements;
@@ -898,6 +965,10 @@ int hx_ctx_create(int device, hx_ctx** out) {
     // cap on the multilevel preconditioner's coarse levels (default: every 2:1 coarsening)
     if (const char* ev = std::getenv("HEXAT_ML_LEVELS")) c->ml_max_levels = std::max(0, std::atoi(ev));
     if (const char* ev = std::getenv("HEXAT_ML_EXACT")) c->ml_exact = ev[0] != '0';
+    if (const char* ev = std::getenv("HEXAT_VC_OMEGA")) {  // V-cycle smoother weight, 0 < omega < 2
+        const double w = std::atof(ev);
+        if (w > 0.0 && w < 2.0) c->vc_omega = w;
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc((void**)&c->st, kMaxRhs * sizeof(PcgState)) != hipSuccess ||
         hipHostMalloc((void**)&c->st_host, kMaxRhs * sizeof(PcgState), hipHostMallocDefault) != hipSuccess) {
@@ -1098,7 +1169,8 @@ static int check_opts(hx_ctx* c, const hx_pcg_opts* o) {
     if (!o) return fail(c, HX_ERR_BAD_ARG, "null options");
     if (!(o->tol > 0) || o->max_iter < 1 || !(o->shift >= 0)) return fail(c, HX_ERR_BAD_ARG, "invalid PCG options");
     if (o->precision != HX_FP64 && o->precision != HX_FP32_IR) return fail(c, HX_ERR_BAD_ARG, "precision");
-    if (o->precond != HX_PC_JACOBI && o->precond != HX_PC_LEARNED && o->precond != HX_PC_MULTILEVEL)
+    if (o->precond != HX_PC_JACOBI && o->precond != HX_PC_LEARNED && o->precond != HX_PC_MULTILEVEL &&
+        o->precond != HX_PC_VCYCLE)
         return fail(c, HX_ERR_BAD_ARG, "preconditioner");
     if (o->precond == HX_PC_LEARNED && !c->unet_trained)
         return fail(c, HX_ERR_STATE, "learned preconditioner not trained (hx_learned_train)");
@@ -1128,7 +1200,7 @@ int hx_block_pcg_solve(hx_ctx* c, int nrhs, const double* Ee, const double* F, d
     if (int rc = ensure_grid(c)) return rc;
     if (nrhs < 1 || nrhs > kMaxRhs) return fail(c, HX_ERR_BAD_ARG, "nrhs must be 1..8");
     if (int rc = check_opts(c, o)) return rc;
-    if (o->precond == HX_PC_MULTILEVEL && nrhs > 1)
+    if ((o->precond == HX_PC_MULTILEVEL || o->precond == HX_PC_VCYCLE) && nrhs > 1)
         return fail(c, HX_ERR_BAD_ARG, "the multilevel preconditioner takes one right-hand side");
     const long n = ndof(c), nt = n * nrhs;
     HXA(c->b_f, nt);
@@ -1279,7 +1351,8 @@ int hx_to_init(hx_ctx* c, const hx_to_params* p, const double* f, const double* 
         p->pcg_max_iter < 1 || !(p->filter_tol > 0) || !(p->oc_rel_tol > 0) || !(p->move > 0))
         return fail(c, HX_ERR_BAD_ARG, "invalid TO parameters");
     if (p->precision != HX_FP64 && p->precision != HX_FP32_IR) return fail(c, HX_ERR_BAD_ARG, "precision");
-    if (p->precond != HX_PC_JACOBI && p->precond != HX_PC_LEARNED && p->precond != HX_PC_MULTILEVEL)
+    if (p->precond != HX_PC_JACOBI && p->precond != HX_PC_LEARNED && p->precond != HX_PC_MULTILEVEL &&
+        p->precond != HX_PC_VCYCLE)
         return fail(c, HX_ERR_BAD_ARG, "preconditioner");
     if (p->precond == HX_PC_LEARNED && !c->unet_trained)
         return fail(c, HX_ERR_STATE, "learned preconditioner not trained (hx_learned_train)");
@@ -1782,8 +1855,8 @@ int hx_time_operator(hx_ctx* c, int precision, int nrhs, int reps, double* avg_m
 int hx_time_operator_pc(hx_ctx* c, int precision, int precond, int nrhs, int reps, double* avg_ms) {
     if (!c || nrhs < 1 || nrhs > kMaxRhs || reps < 1 || !avg_ms) return HX_ERR_BAD_ARG;
     if (precision != HX_FP64 && precision != HX_FP32_IR) return HX_ERR_BAD_ARG;
-    if (precond != HX_PC_JACOBI && precond != HX_PC_MULTILEVEL) return HX_ERR_BAD_ARG;
-    const bool zform = precond == HX_PC_MULTILEVEL;
+    if (precond != HX_PC_JACOBI && precond != HX_PC_MULTILEVEL && precond != HX_PC_VCYCLE) return HX_ERR_BAD_ARG;
+    const bool zform = precond != HX_PC_JACOBI;
     if (int rc = ensure_grid(c)) return rc;
     if (int rc = ensure_fp32(c)) return rc;
     const bool was_timing = c->timing;
@@ -1802,7 +1875,7 @@ int hx_time_operator_pc(hx_ctx* c, int precision, int precond, int nrhs, int rep
 int hx_tune_set(hx_ctx* c, const hx_tune_result* t) {
     if (!c || !t) return HX_ERR_BAD_ARG;
     auto shape_ok = [](int tile, int kchunk, int pd) {
-        return tile >= 0 && tile <= 2 && kchunk >= 4 && (pd == 1 || pd == 2);
+        return tile >= 0 && tile <= 2 && kchunk >= 1 && (pd == 1 || pd == 2);
     };
     auto var_ok = [](int v) { return v == HX_MV_PARITY || v == HX_MV_DENSE || v == HX_MV_GATHER || v == HX_MV_MFMA; };
     if (!shape_ok(t->tile, t->kchunk, t->pdepth) || !shape_ok(t->tile32, t->kchunk32, t->pdepth32) ||
@@ -1837,7 +1910,9 @@ int hx_test_copy(hx_ctx* c, long bytes, int reps, double* avg_ms) {
     return HX_OK;
 }
 
-int hx_test_ml_apply(hx_ctx* c, const double* Ee, const double* r, double* z, int precision, int* levels) {
+namespace {
+int test_pc_apply(hx_ctx* c, const double* Ee, const double* r, double* z, int precision, bool vcycle, int* levels,
+                  int* vlevels) {
     if (!c || !Ee || !r || !z) return HX_ERR_BAD_ARG;
     if (int rc = ensure_grid(c)) return rc;
     if (precision != HX_FP64 && precision != HX_FP32_IR) return fail(c, HX_ERR_BAD_ARG, "precision");
@@ -1849,19 +1924,33 @@ int hx_test_ml_apply(hx_ctx* c, const double* Ee, const double* r, double* z, in
     launch_pcg_reset(c->st, 1e-8, 0.0, 1, 1, c->stream, 1);
     const bool f32 = precision == HX_FP32_IR;
     if ((rc = f32 ? ensure_fp32(c) : HX_OK)) return rc;
-    if ((rc = ml_setup(c, f32))) return rc;
     if (f32) {
+        launch_convert<double, float>(c->g.ne, c->E.p, c->E32.p, c->stream);
         launch_convert<double, float>(n, c->minv.p, c->minv32.p, c->stream);
         launch_convert<double, float>(n, c->r.p, c->r32.p, c->stream);
-        launch_ml_apply<float>(c->ml32, c->r32.p, c->minv32.p, c->ml_z32.p, c->st, c->partials.p, 1, c->stream);
+    }
+    if ((rc = ml_setup(c, f32, vcycle))) return rc;
+    if (f32) {
+        if ((rc = pc_apply32(c, vcycle, c->r32.p, 1))) return rc;
         launch_convert<float, double>(n, c->ml_z32.p, c->q.p, c->stream);
     } else {
-        launch_ml_apply<double>(c->ml64, c->r.p, c->minv.p, c->ml_z.p, c->st, c->partials.p, 1, c->stream);
+        if ((rc = pc_apply64(c, vcycle, c->r.p, 1))) return rc;
         HXC(hipMemcpyAsync(c->q.p, c->ml_z.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     }
     HXK();
     if (levels) *levels = c->ml64.levels;
+    if (vlevels) *vlevels = f32 ? vc_levels(c->ml32) : vc_levels(c->ml64);
     return download(c, z, c->q.p, n * sizeof(double));
+}
+}  // namespace
+
+int hx_test_ml_apply(hx_ctx* c, const double* Ee, const double* r, double* z, int precision, int* levels) {
+    return test_pc_apply(c, Ee, r, z, precision, false, levels, nullptr);
+}
+
+int hx_test_vc_apply(hx_ctx* c, const double* Ee, const double* r, double* z, int precision, int* levels,
+                     int* vlevels) {
+    return test_pc_apply(c, Ee, r, z, precision, true, levels, vlevels);
 }
 
 int hx_timing_enable(hx_ctx* c, int on) {

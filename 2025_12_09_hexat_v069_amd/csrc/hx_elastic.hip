@@ -289,7 +289,17 @@ __device__ __forceinline__ double lane_from_left(double v) {
 }
 
 // FUSE: 0 plain operator; 1 fused p = D^-1 r + beta p_old; 2 fused p = z + beta p_old
-// with z handed in a.r (an explicit preconditioner's output; a.minv unused)
+// with z handed in a.r (an explicit preconditioner's output; a.minv unused).
+// The two smoothing forms of a V-cycle level, S = a.minv the smoother's
+// diagonal (0 where the DOF is not solved for):
+// 3 (pre-smoothing from zero) x = S r and the residual y = r - K x where
+//   S != 0 (0 elsewhere); a.x unused;
+// 4 (post-smoothing) x = S r + mask(P e), the coarse correction e (a.ec, node
+//   grid cNX x cNY x cNZ) interpolated trilinearly where S != 0, then
+//   y = x + S (r - K x), with the per-block partial sums of r'y over the owned
+//   nodes when a.partials is set.  The tile's footprint on two coarse planes
+//   sits in LDS; a new coarse plane is loaded once per two fine planes.
+// Forms 3 and 4 return at once when a.st is set and its solve is done.
 template <typename T, int TX, int TY, int FUSE, int PD>
 __global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K, MvArgs<T> a, int kchunk, int ntx,
                                                        int nty, int ntiles) {
@@ -302,21 +312,28 @@ __global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
     constexpr int OVALS = (TY - 1) * OROW;   // owned values of one plane
     __shared__ T xs[3][PY][PX];  // one node plane (x, or the fused p), component-major
     __shared__ T ex[6][NT];      // y-exchange: row ty+1 half of each face's y-inverse
-    __shared__ T so[FUSE ? 2 : 1][OVALS];  // results of the owned nodes, node-major, for row-contiguous stores
+    constexpr bool POUT = FUSE == 1 || FUSE == 2;  // forms that write the staged vector to p_out
+    constexpr bool SMOOTH = FUSE >= 3;
+    constexpr bool PRO = FUSE == 4;  // fused prolongation of a coarse correction
+    // coarse footprint of the tile's node columns (i0-1 .. i0-1+TX, j0-1 .. j0-1+TY)
+    constexpr int CPX = TX / 2 + 2, CPY = TY / 2 + 2, CVALS = 3 * CPX * CPY, CPER = (CVALS + NT - 1) / NT;
+    __shared__ T cpl[PRO ? 2 : 1][PRO ? CVALS : 1];  // slot (coarse plane & 1): [(c CPY + ly) CPX + lx]
+    __shared__ T so[POUT ? 2 : 1][OVALS];  // results of the owned nodes, node-major, for row-contiguous stores
     __shared__ double red[NT / 64];
 
     a.x += (long)blockIdx.y * a.cs;
     a.y += (long)blockIdx.y * a.cs;
-    if constexpr (FUSE) {
-        a.r += (long)blockIdx.y * a.cs;
-        a.p_out += (long)blockIdx.y * a.cs;
-    }
+    if constexpr (FUSE) a.r += (long)blockIdx.y * a.cs;
+    if constexpr (POUT) a.p_out += (long)blockIdx.y * a.cs;
     if (a.mode == MV_PCG) {
         a.st += blockIdx.y;
         a.partials += (long)blockIdx.y * gridDim.x;
     }
     if (a.mode == MV_PCG && a.st->done) return;
-    const T beta = FUSE ? T(a.st->beta) : T(0);
+    if constexpr (SMOOTH) {
+        if (a.st != nullptr && a.st->done) return;
+    }
+    const T beta = (FUSE == 1 || FUSE == 2) ? T(a.st->beta) : T(0);
 
     const unsigned nb = gridDim.x;  // multiple of 8: XCD-contiguous tile ranges
     const unsigned tile = (blockIdx.x & 7u) * (nb >> 3) + (blockIdx.x >> 3);
@@ -343,30 +360,108 @@ __global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
         const int off_x =
             3 * (min(max(j0 - 1 + xty, 0), g.NY - 1) * g.NX + min(max(i0 - 1 + xtx, 0), g.NX - 1)) + xc;
 
+        // form 4: coarse planes of the correction.  held[s] is the coarse
+        // plane in slot s (uniform); loads are issued after a layer's first
+        // barrier and parked in LDS before its second.
+        [[maybe_unused]] const int cbx = (i0 - 1) < 0 ? -1 : (i0 - 1) >> 1;
+        [[maybe_unused]] const int cby = (j0 - 1) < 0 ? -1 : (j0 - 1) >> 1;
+        [[maybe_unused]] int held0 = -1, held1 = -1;      // coarse plane in slot 0 / 1
+        [[maybe_unused]] bool pend0 = false, pend1 = false;  // slot to park before the next barrier
+        [[maybe_unused]] T cv00 = T(0), cv01 = T(0), cv10 = T(0), cv11 = T(0);  // its two values per thread
+        static_assert(!PRO || CPER == 2, "two coarse values per thread and plane");
+        // value m of this thread's share of coarse plane K (clamped addresses)
+        auto coarse_load = [&](int K, int m) __attribute__((always_inline)) -> T {
+            const int idx = min((int)threadIdx.x + m * NT, CVALS - 1);
+            const int ly = idx / (3 * CPX), rem = idx - ly * 3 * CPX, lx = rem / 3, c = rem - 3 * lx;
+            const int I = min(max(cbx + lx, 0), a.cNX - 1), J = min(max(cby + ly, 0), a.cNY - 1);
+            return a.ec[3 * ((long)K * a.cNY * a.cNX + (long)J * a.cNX + I) + c];
+        };
+        auto coarse_store = [&](int slot, int m, T v) __attribute__((always_inline)) {
+            const int idx = (int)threadIdx.x + m * NT;
+            if (idx < CVALS) {
+                const int ly = idx / (3 * CPX), rem = idx - ly * 3 * CPX, lx = rem / 3, c = rem - 3 * lx;
+                cpl[PRO ? slot : 0][(c * CPY + ly) * CPX + lx] = v;
+            }
+        };
+        // request coarse plane K into slot K & 1 unless it is there (value
+        // selects, not a branch per slot: the state stays in registers)
+        auto coarse_want = [&](int K) __attribute__((always_inline)) {
+            const bool w0 = (K & 1) == 0 && held0 != K, w1 = (K & 1) != 0 && held1 != K;
+            T v0 = T(0), v1 = T(0);
+            if (w0 || w1) {  // uniform
+                v0 = coarse_load(K, 0);
+                v1 = coarse_load(K, 1);
+            }
+            cv00 = w0 ? v0 : cv00;
+            cv01 = w0 ? v1 : cv01;
+            cv10 = w1 ? v0 : cv10;
+            cv11 = w1 ? v1 : cv11;
+            held0 = w0 ? K : held0;
+            held1 = w1 ? K : held1;
+            pend0 = pend0 || w0;
+            pend1 = pend1 || w1;
+        };
+        // the coarse planes fine plane mf needs: A = mf >> 1, and A + 1 for odd mf
+        auto coarse_need = [&](int mf) __attribute__((always_inline)) {
+            if constexpr (PRO) {
+                const int mm = min(max(mf, 0), g.NZ - 1), A = mm >> 1;
+                coarse_want(A);
+                if (mm & 1) coarse_want(A + 1);
+            }
+        };
+        auto coarse_flush = [&]() __attribute__((always_inline)) {
+            if constexpr (PRO) {
+                if (pend0) {
+                    coarse_store(0, 0, cv00);
+                    coarse_store(0, 1, cv01);
+                }
+                if (pend1) {
+                    coarse_store(1, 0, cv10);
+                    coarse_store(1, 1, cv11);
+                }
+                pend0 = pend1 = false;
+            }
+        };
+        // trilinear value of component c of the correction at fine node (fi, fj) of fine plane mf
+        [[maybe_unused]] auto coarse_at = [&](int fi, int fj, int c, int mf) __attribute__((always_inline)) -> T {
+            const int mm = min(max(mf, 0), g.NZ - 1), A = mm >> 1;
+            const int lx = (fi < 0 ? -1 : fi >> 1) - cbx, ly = (fj < 0 ? -1 : fj >> 1) - cby;
+            const T wx1 = (fi & 1) ? T(0.5) : T(0), wx0 = T(1) - wx1;
+            const T wy1 = (fj & 1) ? T(0.5) : T(0), wy0 = T(1) - wy1;
+            auto plane = [&](int K) {
+                const T* P = &cpl[PRO ? (K & 1) : 0][(c * CPY + ly) * CPX + lx];
+                return wy0 * (wx0 * P[0] + wx1 * P[1]) + wy1 * (wx0 * P[CPX] + wx1 * P[CPX + 1]);
+            };
+            if (mm & 1) return T(0.5) * (plane(A) + plane(A + 1));
+            return plane(A);
+        };
+
         // PD register sets of a prefetched plane: x (and r, D^-1 when fused)
         // at this thread's node and extra value, plus E of element layer
         // k-1.  Addresses are clamped, so every load is unconditional (a load
         // under a branch makes the compiler's wait counts conservative);
         // out-of-range node values only ever meet E = 0.
         struct Slot {
-            T x[3], r[FUSE ? 3 : 1], m[FUSE == 1 ? 3 : 1], qx, qr, qm, e;
+            T x[3], r[FUSE ? 3 : 1], m[(FUSE == 1 || SMOOTH) ? 3 : 1], qx, qr, qm, e;
             int k;
         };
         Slot sl[PD];
-        auto plane_load = [&](Slot& q, int k) {
+        auto plane_load = [&](Slot& q, int k) __attribute__((always_inline)) {
             q.k = k;
             const long po = (long)min(max(k, 0), g.NZ - 1) * pstride;
-            const T* X = a.x + po;
+            if constexpr (FUSE != 3 && FUSE != 4) {
+                const T* X = a.x + po;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) q.x[c] = X[off_own + c];
-            q.qx = X[off_x];
+                for (int c = 0; c < 3; ++c) q.x[c] = X[off_own + c];
+                q.qx = X[off_x];
+            }
             if constexpr (FUSE != 0) {
                 const T* R = a.r + po;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) q.r[c] = R[off_own + c];
                 q.qr = R[off_x];
             }
-            if constexpr (FUSE == 1) {
+            if constexpr (FUSE == 1 || SMOOTH) {
                 const T* M = a.minv + po;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) q.m[c] = M[off_own + c];
@@ -377,25 +472,38 @@ __global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
         // park a prefetched plane in LDS; sets this thread's node value and
         // the modulus of its element in layer q.k - 1
         T xnew[3], ee;
-        auto plane_store = [&](const Slot& q) {
+        T rnew[SMOOTH ? 3 : 1], mnew[SMOOTH ? 3 : 1];  // r and S of this thread's node (smoothing forms)
+        auto plane_store = [&](const Slot& q) __attribute__((always_inline)) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 T v;
                 if constexpr (FUSE == 1) v = q.m[c] * q.r[c] + beta * q.x[c];  // p = D^-1 r + beta p
                 else if constexpr (FUSE == 2) v = q.r[c] + beta * q.x[c];        // p = z + beta p
+                else if constexpr (FUSE == 3) v = q.m[c] * q.r[c];              // x = S r
+                else if constexpr (FUSE == 4)                                     // x = S r + mask(P e)
+                    v = q.m[c] * q.r[c] + (q.m[c] != T(0) ? coarse_at(gi, gj, c, q.k) : T(0));
                 else v = q.x[c];
                 xs[c][ty][tx] = v;
                 xnew[c] = v;
+                if constexpr (SMOOTH) {
+                    rnew[c] = q.r[c];
+                    mnew[c] = q.m[c];
+                }
             }
             T v;
             if constexpr (FUSE == 1) v = q.qm * q.qr + beta * q.qx;
             else if constexpr (FUSE == 2) v = q.qr + beta * q.qx;
+            else if constexpr (FUSE == 3) v = q.qm * q.qr;
+            else if constexpr (FUSE == 4) {
+                v = q.qm * q.qr;
+                if (xstore && q.qm != T(0)) v += coarse_at(i0 - 1 + xtx, j0 - 1 + xty, xc, q.k);
+            }
             else v = q.qx;
             if (xstore) xs[xc][xty][xtx] = v;
             ee = (ecol && q.k - 1 >= 0 && q.k - 1 < g.nz) ? q.e : T(0);
             asm volatile("" : "+v"(ee));  // settle ee here, before the next plane is requested
         };
-        auto face_modes = [&](T* F) {  // F[3 s + c]
+        auto face_modes = [&](T* F) __attribute__((always_inline)) {  // F[3 s + c]
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const T x00 = xs[c][ty][tx], x10 = xs[c][ty][tx + 1];
@@ -409,15 +517,16 @@ __global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
         };
 
         T F0[12], F1[12], carry[12], xo[3];
+        T ro[SMOOTH ? 3 : 1], mo[SMOOTH ? 3 : 1];  // r and S of the plane being finished
         // Node results of plane ko are parked in LDS (so) at the end of a
         // layer and written out after the next layer's first barrier, ahead of
         // its prefetch, as row-contiguous runs of 3*(TX-1) values: a lane
         // writes consecutive words instead of one component of one node.
         int ko = -1;
-        auto flush = [&]() {
+        auto flush = [&]() __attribute__((always_inline)) {
             if (ko < k0) return;
             T* Y = a.y + (long)ko * pstride;
-            T* P = FUSE ? a.p_out + (long)ko * pstride : nullptr;
+            T* P = POUT ? a.p_out + (long)ko * pstride : nullptr;
 #pragma unroll
             for (int m = 0; m < (OVALS + NT - 1) / NT; ++m) {
                 const int idx = threadIdx.x + m * NT;
@@ -425,19 +534,20 @@ __global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
                 if (idx < OVALS && i0 + col / 3 < g.NX && j0 + row < g.NY) {
                     const int o = 3 * ((j0 + row) * g.NX + i0) + col;
                     Y[o] = so[0][idx];
-                    if constexpr (FUSE) P[o] = so[FUSE ? 1 : 0][idx];
+                    if constexpr (POUT) P[o] = so[POUT ? 1 : 0][idx];
                 }
             }
         };
         // element layer ke: Flo holds the face modes of plane ke, Fhi receives
         // plane ke+1's, which sits in slot q; plane ke+1+PD is requested into q
-        auto step = [&](int ke, const T* Flo, T* Fhi, Slot& q) {
+        auto step = [&](int ke, const T* Flo, T* Fhi, Slot& q) __attribute__((always_inline)) {
             plane_store(q);  // plane ke+1, E of layer ke
             __syncthreads();
             flush();
             asm volatile("" ::: "memory");
             plane_load(q, min(ke + 1 + PD, k1));  // unconditional (past k1: re-reads plane k1, a cache hit)
             asm volatile("" ::: "memory");  // keep the prefetch ahead of this layer's arithmetic
+            coarse_need(ke + 2);  // form 4: the coarse planes of the plane the next layer stages
             face_modes(Fhi);
 
             // element (gi, gj, ke): modes, contraction (E folded into xh), z-inverse
@@ -469,6 +579,7 @@ __global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
                 ex[c][threadIdx.x] = H[c] + H[6 + c];
                 ex[3 + c][threadIdx.x] = H[3 + c] + H[9 + c];
             }
+            coarse_flush();
             __syncthreads();
             const int src = ty >= 1 ? threadIdx.x - TX : threadIdx.x;
             T node[3];
@@ -482,19 +593,43 @@ __global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const T yv = node[c] + a.shift * xo[c];
+                T out = yv;
+                if constexpr (FUSE == 3) out = mo[c] != T(0) ? ro[c] - yv : T(0);  // residual after x = S r
+                else if constexpr (FUSE == 4) out = xo[c] + mo[c] * (ro[c] - yv);  // x + S (r - K x)
                 if (tx >= 1 && ty >= 1) {
-                    so[0][so_i + c] = yv;
-                    if constexpr (FUSE) so[FUSE ? 1 : 0][so_i + c] = xo[c];
+                    so[0][so_i + c] = out;
+                    if constexpr (POUT) so[POUT ? 1 : 0][so_i + c] = xo[c];
                 }
-                if (ke >= k0 && own) pq += (double)xo[c] * (double)yv;
+                if constexpr (FUSE == 4) {
+                    if (ke >= k0 && own) pq += (double)ro[c] * (double)out;
+                } else {
+                    if (ke >= k0 && own) pq += (double)xo[c] * (double)yv;
+                }
                 xo[c] = xnew[c];
+                if constexpr (SMOOTH) {
+                    ro[c] = rnew[c];
+                    mo[c] = mnew[c];
+                }
             }
         };
 
         plane_load(sl[0], k0 - 1);
+        if constexpr (PRO) {  // coarse planes of fine planes k0-1 and k0
+            const int A = min(max(k0 - 1, 0), g.NZ - 1) >> 1;
+            coarse_want(A);
+            if (A + 1 < a.cNZ) coarse_want(A + 1);
+            coarse_flush();
+            __syncthreads();
+        }
         plane_store(sl[0]);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) xo[c] = xnew[c];
+        for (int c = 0; c < 3; ++c) {
+            xo[c] = xnew[c];
+            if constexpr (SMOOTH) {
+                ro[c] = rnew[c];
+                mo[c] = mnew[c];
+            }
+        }
         __syncthreads();
         face_modes(F0);
         // plane k0+j sits in slot j % PD
@@ -513,7 +648,7 @@ __global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
         flush();
     }
 
-    if (a.mode == MV_PCG) {  // per-block p'q; k_fin_alpha reduces them
+    if (a.mode == MV_PCG || (FUSE == 4 && a.partials != nullptr)) {  // per-block p'q (k_fin_alpha), or r'y
         const double v = block_sum<NT>(pq, red);
         if (threadIdx.x == 0) a.partials[blockIdx.x] = v;
     }
@@ -653,7 +788,13 @@ int launch_sf(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, const 
     int ntx, nty;
     const int nt = zmarch_tiles<TX, TY>(g, sh.kchunk, &ntx, &nty);
     const int nb = round8(nt);
-    if (a.p_out != nullptr && a.minv == nullptr)
+    if (a.form == 3)
+        hipLaunchKernelGGL((k_matvec_sf<T, TX, TY, 3, PD>), dim3(nb, nrhs), dim3(TX * TY), 0, s, g, K, a, sh.kchunk,
+                           ntx, nty, nt);
+    else if (a.form == 4)
+        hipLaunchKernelGGL((k_matvec_sf<T, TX, TY, 4, PD>), dim3(nb, nrhs), dim3(TX * TY), 0, s, g, K, a, sh.kchunk,
+                           ntx, nty, nt);
+    else if (a.p_out != nullptr && a.minv == nullptr)
         hipLaunchKernelGGL((k_matvec_sf<T, TX, TY, 2, PD>), dim3(nb, nrhs), dim3(TX * TY), 0, s, g, K, a, sh.kchunk,
                            ntx, nty, nt);
     else if (a.p_out != nullptr)

@@ -37,6 +37,13 @@ struct MvArgs {
     // host side only: MV_PCG without the alpha finaliser -- the PCG update
     // then reduces the p'q partials itself (launch_pcg_update's pq_partials)
     int defer_alpha = 0;
+    // parity contraction, MV_PLAIN: 3 = pre-smoothing (x = S r, y = masked
+    // r - K x), 4 = post-smoothing (x = S r + mask(P ec), y = x + S (r - K x),
+    // r'y partials when partials is set); S in minv; st (if set) skips a
+    // finished solve
+    int form = 0;
+    const T* ec = nullptr;         // form 4: coarse correction on the next level's node grid
+    int cNX = 0, cNY = 0, cNZ = 0;  // its node counts
 };
 
 // tile: 0 -> 16x16 element columns (default), 1 -> 16x32, 2 -> 32x8.
@@ -147,6 +154,11 @@ struct MlHierarchy {
     T* y[kMlMaxLevels + 1] = {};     // level output D_l^-1 r_l + P y_{l+1}
     T* minv[kMlMaxLevels + 1] = {};  // level Jacobi inverse diagonal
     double w[kMlMaxLevels + 1] = {};  // level weights
+    // V-cycle levels l < vc_levels(): smoother diagonal, residual after
+    // pre-smoothing, operator moduli (level 0: the fine moduli)
+    T* sinv[kMlMaxLevels + 1] = {};
+    T* vres[kMlMaxLevels + 1] = {};
+    const T* Eop[kMlMaxLevels + 1] = {};
 };
 // number of 2:1 coarsenings of g (all element counts even), at most max_levels
 int ml_levels_for(const GridArgs& g, int max_levels);
@@ -167,12 +179,30 @@ void launch_ml_apply(const MlHierarchy<T>& h, const T* r, const T* minv0, T* z, 
 int ml_partial_blocks();
 // A_L^-1 of the coarsest level gc (moduli Ec, element stiffness h Kbin, DOFs
 // fixed where the fine DOF at stride * (I,J,K) is, or without stiffness, give
-// zero rows and columns), built and inverted on the device; A is scratch of
-// (3 nn)^2 doubles
+// zero rows and columns), built and inverted on the device in one workgroup
+// (3 nn <= kMlExactMax)
 template <typename T>
 void launch_ml_exact_setup(const GridArgs& gc, const double* Ec, const double* Kbin, double h, const uint8_t* fixed_f,
-                           int stride, const GridArgs& gfine, double* A, T* ainv, hipStream_t s);
+                           int stride, const GridArgs& gfine, T* ainv, hipStream_t s);
 void ml_prepare_kernels();  // once per process, outside graph capture
+// V-cycle form: levels 0..c-1 smoothed (c = vc_levels), the additive form from
+// level c down as the coarse correction
+template <typename T>
+int vc_levels(const MlHierarchy<T>& h);
+// smoother diagonal of a level: omega / absolute row sums (0 where the DOF is
+// not solved for); arguments as launch_ml_exact_setup's
+template <typename T>
+void launch_vc_smoother(const GridArgs& gc, const double* Ec, double h, const double* Kbin, const uint8_t* fixed_f,
+                        int stride, const GridArgs& gfine, double omega, T* sinv, hipStream_t s);
+template <typename T>
+void launch_vc_scale_E(long ne, const double* Ec, double h, T* Eop, hipStream_t s);
+// z = V(r) with the parity operator K at shape sh on every smoothed level; the
+// r'z partials (one per level-0 operator block) go after the update's and the
+// finaliser runs as in launch_ml_apply.  HX_ERR_BAD_ARG when those partials
+// would pass max_partials.
+template <typename T>
+int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const MvShape& sh, const T* r, T* z,
+                    PcgState* st, double* partials, long max_partials, int fin_mode, hipStream_t s);
 
 // ---- Helmholtz filter --------------------------------------------------------
 struct HelmArgs {

@@ -16,11 +16,24 @@
 // z = D0^-1 r + mask(P y_1) on the fine grid.  All gathers; no atomics, so the
 // result is deterministic.
 //
+// The V-cycle form (launch_vc_apply) smooths on the multi-block levels
+// 0..c-1 and takes the additive form above from level c down as its coarse
+// correction: per level l < c, with S_l = omega / (absolute row sums of the
+// level's stiffness), rediscretised (averaged moduli, element stiffness scaled
+// by the element size) and restricted to the DOFs solved for,
+//   x = S r;  res = mask(r - A x);  e = V_{l+1}(R res);
+//   x += mask(P e);  z = x + S (r - A x)
+// The row-sum diagonal bounds A from above (x'Ax <= x'D x for the absolute
+// row sums D), so every omega < 2 keeps the smoothing convergent and the
+// V-cycle symmetric positive definite whatever the moduli contrast.
+//
 // The reference has no counterpart; its strong-preconditioner slot is the
 // learned one (north-star path) and its CPU AMG branch (GPUSolver.jl:168-268).
 #include "hx_launch.h"
 
 #include <algorithm>
+
+#include "../../include/hexat.h"
 
 namespace hx {
 
@@ -368,20 +381,25 @@ __global__ __launch_bounds__(NT) void k_fin_rz(PcgState* st, const double* parti
 }
 
 // The coarsest level solved exactly: A assembled element by element (an
-// element's 24 DOFs are distinct, so the 576 entries of one element go in
-// parallel), boundary DOFs and DOFs without stiffness replaced by identity
-// rows and columns, in-place Gauss-Jordan inversion (A is symmetric positive
-// definite then: no pivoting), those rows and columns of the inverse zeroed.
-// One workgroup, A in global memory; run once per solve.
+// element's 24 DOFs are distinct, so the entries of one element go in
+// parallel) into the lower triangle of a symmetric matrix held in LDS,
+// boundary DOFs and DOFs without stiffness replaced by identity rows and
+// columns, then swept on every pivot with the symmetric sweep operator
+//   a_ij -= a_ik a_jk / a_kk (i, j != k);  a_ik /= a_kk;  a_kk = -1 / a_kk
+// which leaves -A^-1 (A is symmetric positive definite: no pivoting); those
+// rows and columns of the inverse zeroed.  One workgroup; run once per solve.
 constexpr int kExactThreads = 1024;
+__device__ __forceinline__ int tri(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
 template <typename T>
 __global__ __launch_bounds__(kExactThreads) void k_ml_exact_setup(GridArgs gc, const double* __restrict__ Ec,
                                                                   const double* __restrict__ Kbin, double h,
                                                                   const uint8_t* __restrict__ fixed_f, int stride,
-                                                                  int NXf, int NYf, double* A, T* __restrict__ ainv) {
+                                                                  int NXf, int NYf, T* __restrict__ ainv) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    double* P = reinterpret_cast<double*>(lds_raw);  // (i, j <= i) at i (i + 1) / 2 + j
     __shared__ int keep[kMlExactMax];
-    const int N = 3 * (int)gc.nn, t = threadIdx.x;
-    for (int i = t; i < N * N; i += kExactThreads) A[i] = 0.0;
+    const int N = 3 * (int)gc.nn, t = threadIdx.x, NP = N * (N + 1) / 2;
+    for (int q = t; q < NP; q += kExactThreads) P[q] = 0.0;
     __syncthreads();
     for (int e = 0; e < (int)gc.ne; ++e) {
         const int I = e % gc.nx, J = (e / gc.nx) % gc.ny, K = e / (gc.nx * gc.ny);
@@ -390,7 +408,8 @@ __global__ __launch_bounds__(kExactThreads) void k_ml_exact_setup(GridArgs gc, c
             const int ra = ab / 24, cb = ab - 24 * ra, La = ra / 3, Lb = cb / 3;
             const int na = (I + (La & 1)) + gc.NX * ((J + ((La >> 1) & 1)) + gc.NY * (K + (La >> 2)));
             const int nb = (I + (Lb & 1)) + gc.NX * ((J + ((Lb >> 1) & 1)) + gc.NY * (K + (Lb >> 2)));
-            A[(3 * na + ra - 3 * La) * N + 3 * nb + cb - 3 * Lb] += Ee * Kbin[ab];
+            const int row = 3 * na + ra - 3 * La, col = 3 * nb + cb - 3 * Lb;
+            if (row >= col) P[row * (row + 1) / 2 + col] += Ee * Kbin[ab];
         }
         __syncthreads();
     }
@@ -398,34 +417,122 @@ __global__ __launch_bounds__(kExactThreads) void k_ml_exact_setup(GridArgs gc, c
         int i, j, k;
         node_ijk(gc, d / 3, i, j, k);
         const long f = (long)stride * i + (long)NXf * ((long)stride * j + (long)NYf * ((long)stride * k));
-        keep[d] = (fixed_f == nullptr || fixed_f[3 * f + d % 3] == 0) && A[(long)d * N + d] > 0.0;
+        keep[d] = (fixed_f == nullptr || fixed_f[3 * f + d % 3] == 0) && P[tri(d, d)] > 0.0;
     }
     __syncthreads();
     for (int idx = t; idx < N * N; idx += kExactThreads) {
-        const int r = idx / N, q = idx - r * N;
-        if (!keep[r] || !keep[q]) A[idx] = r == q ? 1.0 : 0.0;
+        const int i = idx / N, j = idx - i * N;
+        if (j <= i && (!keep[i] || !keep[j])) P[tri(i, j)] = i == j ? 1.0 : 0.0;
     }
     __syncthreads();
-    for (int k = 0; k < N; ++k) {  // sweep k: pivot row, the rest, pivot column
-        const double d = A[k * N + k];
-        for (int j = t; j < N; j += kExactThreads)
-            if (j != k) A[k * N + j] /= d;
-        __syncthreads();
-        for (int idx = t; idx < N * N; idx += kExactThreads) {
-            const int i = idx / N, j = idx - i * N;
-            if (i != k && j != k) A[idx] -= A[i * N + k] * A[k * N + j];
+    // sweeps: thread (ti, tj) of a 32 x 32 grid updates the entries (ti + 32 a, tj + 32 b) of the lower
+    // triangle, with its pivot-column values read once per pivot
+    constexpr int G = 32, MB = (kMlExactMax + G - 1) / G;
+    const int ti = t / G, tj = t % G;
+    for (int k = 0; k < N; ++k) {
+        const double rd = 1.0 / P[tri(k, k)];
+        double ci[MB], cj[MB];
+#pragma unroll
+        for (int a = 0; a < MB; ++a) {
+            const int i = ti + G * a, j = tj + G * a;
+            ci[a] = i < N ? P[tri(i, k)] * rd : 0.0;
+            cj[a] = j < N ? P[tri(j, k)] : 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < MB; ++a) {
+            const int i = ti + G * a;
+            if (i >= N || i == k) continue;
+            const int row = i * (i + 1) / 2;
+#pragma unroll
+            for (int b = 0; b <= a; ++b) {
+                const int j = tj + G * b;
+                if (j > i || j == k) continue;
+                P[row + j] -= ci[a] * cj[b];
+            }
         }
         __syncthreads();
-        for (int i = t; i < N; i += kExactThreads) A[i * N + k] = i == k ? 1.0 / d : -A[i * N + k] / d;
+        for (int i = t; i < N; i += kExactThreads) P[tri(i, k)] = i == k ? -rd : P[tri(i, k)] * rd;
         __syncthreads();
     }
     for (int idx = t; idx < N * N; idx += kExactThreads) {
-        const int r = idx / N, q = idx - r * N;
-        ainv[idx] = (keep[r] && keep[q]) ? T(A[idx]) : T(0);
+        const int i = idx / N, j = idx - i * N;
+        ainv[idx] = (keep[i] && keep[j]) ? T(-P[tri(i, j)]) : T(0);
     }
 }
 
+// V-cycle smoother diagonal of level grid gc (moduli Ec, element stiffness h
+// Kbin in binary node order): S = omega / sum_j |A_ij| over the DOFs j solved
+// for (not fixed, nonzero diagonal), 0 where DOF i is not solved for.  The
+// coupling block of node n with neighbour n + (di,dj,dk) sums, over the
+// elements around n that hold both, E_e Kbin[La][Lb]; the absolute value is
+// taken of the sum.  Every loop is unrolled, so the Kbin indices are
+// constants (uniform loads).  The fine node of coarse node (I,J,K) is
+// stride * (I,J,K) on the fine grid (NXf x NYf nodes).
+template <typename T>
+__global__ __launch_bounds__(NT) void k_vc_l1(GridArgs gc, const double* __restrict__ Ec, double h,
+                                              const double* __restrict__ Kbin, const uint8_t* __restrict__ fixed_f,
+                                              int stride, int NXf, int NYf, double omega, T* __restrict__ sinv) {
+    const long n = (long)blockIdx.x * NT + threadIdx.x;
+    if (n >= gc.nn) return;
+    int i, j, k;
+    node_ijk(gc, n, i, j, k);
+    auto fixed_at = [&](int a, int b, int cz, int comp) {
+        if (fixed_f == nullptr) return false;
+        const long f = (long)stride * a + (long)NXf * ((long)stride * b + (long)NYf * ((long)stride * cz));
+        return fixed_f[3 * f + comp] != 0;
+    };
+    double Ee[8];  // element (i-1+ex, j-1+ey, k-1+ez) at ex + 2 ey + 4 ez, 0 outside the grid
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int ie = i - 1 + (e & 1), je = j - 1 + ((e >> 1) & 1), ke = k - 1 + (e >> 2);
+        const bool in = ie >= 0 && je >= 0 && ke >= 0 && ie < gc.nx && je < gc.ny && ke < gc.nz;
+        Ee[e] = in ? h * Ec[(long)ke * gc.nx * gc.ny + (long)je * gc.nx + ie] : 0.0;
+    }
+    double l1[3] = {0.0, 0.0, 0.0}, diag[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
+        const int di = d % 3 - 1, dj = (d / 3) % 3 - 1, dk = d / 9 - 1;
+        double B[3][3] = {};
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
+            const int bx = di + 1 - ex, by = dj + 1 - ey, bz = dk + 1 - ez;  // neighbour's place in the element
+            if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+            const int La = (1 - ex) + 2 * (1 - ey) + 4 * (1 - ez), Lb = bx + 2 * by + 4 * bz;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) B[a][b] += Ee[e] * Kbin[(3 * La + a) * 24 + 3 * Lb + b];
+        }
+        const int mi = i + di, mj = j + dj, mk = k + dk;
+        const bool in = mi >= 0 && mj >= 0 && mk >= 0 && mi < gc.NX && mj < gc.NY && mk < gc.NZ;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            if (!in || fixed_at(mi, mj, mk, b)) continue;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) l1[a] += fabs(B[a][b]);
+        }
+        if (d == 13)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) diag[a] = B[a][a];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const bool solved = !fixed_at(i, j, k, a) && diag[a] > 0.0;
+        sinv[3 * n + a] = solved ? T(omega / l1[a]) : T(0);
+    }
+}
+
+// operator moduli of a level: h * Ec in the operator's precision
+template <typename T>
+__global__ __launch_bounds__(NT) void k_vc_scale_E(long ne, const double* __restrict__ Ec, double h,
+                                                   T* __restrict__ Eop) {
+    const long e = (long)blockIdx.x * NT + threadIdx.x;
+    if (e < ne) Eop[e] = T(h * Ec[e]);
+}
+
 constexpr int kProlongBlocks = 1024;  // fine-level blocks (multiple of 8)
+constexpr int kVcMinBlocks = 512;     // operator workgroups a coarse V-cycle level aims for
 
 inline unsigned blocks_for(long n) { return (unsigned)((n + NT - 1) / NT); }
 
@@ -532,15 +639,101 @@ void launch_ml_apply(const MlHierarchy<T>& h, const T* r, const T* minv0, T* z, 
 int ml_partial_blocks() { return kProlongBlocks; }
 
 template <typename T>
+int vc_levels(const MlHierarchy<T>& h) {
+    const int L = h.levels;
+    if (L == 0) return 0;
+    return std::min(ml_coarse_start(h), L);
+}
+
+template <typename T>
+void launch_vc_smoother(const GridArgs& gc, const double* Ec, double h, const double* Kbin, const uint8_t* fixed_f,
+                        int stride, const GridArgs& gfine, double omega, T* sinv, hipStream_t s) {
+    hipLaunchKernelGGL((k_vc_l1<T>), dim3(blocks_for(gc.nn)), dim3(NT), 0, s, gc, Ec, h, Kbin, fixed_f, stride,
+                       gfine.NX, gfine.NY, omega, sinv);
+}
+
+template <typename T>
+void launch_vc_scale_E(long ne, const double* Ec, double h, T* Eop, hipStream_t s) {
+    hipLaunchKernelGGL((k_vc_scale_E<T>), dim3(blocks_for(ne)), dim3(NT), 0, s, ne, Ec, h, Eop);
+}
+
+template <typename T>
+int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const MvShape& sh, const T* r, T* z,
+                    PcgState* st_, double* partials, long max_partials, int fin_mode, hipStream_t s) {
+    const int c = vc_levels(h);
+    if (c == 0) {  // no coarse level: the additive form is plain Jacobi
+        launch_ml_apply<T>(h, r, h.minv[0], z, st_, partials, fin_mode, s);
+        return HX_OK;
+    }
+    const int L = h.levels;
+    PcgState* const st = fin_mode == 0 ? st_ : nullptr;
+    double* const upd_partials = partials;
+    double* const rz_partials = partials + 2 * kVecBlocks;
+    if (2L * kVecBlocks + matvec_parity_blocks(h.g[0], sh.tile, sh.kchunk) > max_partials) return HX_ERR_BAD_ARG;
+    auto level_r = [&](int l) { return l == 0 ? r : (const T*)h.r[l]; };
+    // coarse levels: shallower z chunks until the launch has kVcMinBlocks
+    // workgroups (or one plane each), so the small grids are not latency-bound
+    auto level_shape = [&](int l) {
+        MvShape s2 = sh;
+        if (l > 0)
+            while (s2.kchunk > 1 && matvec_parity_blocks(h.g[l], s2.tile, s2.kchunk) < kVcMinBlocks) s2.kchunk /= 2;
+        return s2;
+    };
+    auto restrict_to = [&](int l, bool scale) {  // R vres[l-1] -> r[l] (and y[l] = D_l^-1 r[l])
+        const GridArgs& gc = h.g[l];
+        const int ntx = (gc.NX + kRtX - 1) / kRtX, nty = (gc.NY + kRtY - 1) / kRtY, ntz = (gc.NZ + kRtZ - 1) / kRtZ;
+        if (scale)
+            hipLaunchKernelGGL((k_ml_restrict_tiled<T, true>), dim3(ntx * nty * ntz), dim3(NT), 0, s, h.g[l - 1],
+                               (const T*)h.vres[l - 1], gc, h.r[l], h.minv[l], h.y[l], T(h.w[l]), st, ntx, nty);
+        else
+            hipLaunchKernelGGL((k_ml_restrict_tiled<T, false>), dim3(ntx * nty * ntz), dim3(NT), 0, s, h.g[l - 1],
+                               (const T*)h.vres[l - 1], gc, h.r[l], (const T*)nullptr, (T*)nullptr, T(1), st, ntx,
+                               nty);
+    };
+    for (int l = 0; l < c; ++l) {  // pre-smoothing x = S r, residual, restriction
+        MvArgs<T> a{nullptr, h.vres[l], h.Eop[l], T(0), MV_PLAIN, st, nullptr, 3 * h.g[l].nn, level_r(l), h.sinv[l],
+                    nullptr};
+        a.form = 3;
+        launch_matvec_parity<T>(h.g[l], K, a, level_shape(l), s, 1);
+        if (l + 1 < c) restrict_to(l + 1, false);
+    }
+    const int s0 = ml_coarse_start(h);
+    if (c == s0) {  // levels c..L additively in one workgroup, from vres[c-1]
+        MlHierarchy<T> hc = h;
+        hc.r[c - 1] = h.vres[c - 1];
+        hipLaunchKernelGGL((k_ml_coarse<T>), dim3(1), dim3(kCoarseThreads), coarse_lds_values(h, s0) * sizeof(T), s,
+                           hc, s0, st);
+    } else {  // c = L: the coarsest level by its diagonal
+        restrict_to(L, true);
+    }
+    int nb0 = 0;
+    for (int l = c - 1; l >= 0; --l) {  // post-smoothing with the prolongation of the correction fused
+        MvArgs<T> a{nullptr, l == 0 ? z : h.y[l], h.Eop[l], T(0), MV_PLAIN, st, l == 0 ? rz_partials : nullptr,
+                    3 * h.g[l].nn, level_r(l), h.sinv[l], nullptr};
+        a.form = 4;
+        a.ec = h.y[l + 1];
+        a.cNX = h.g[l + 1].NX;
+        a.cNY = h.g[l + 1].NY;
+        a.cNZ = h.g[l + 1].NZ;
+        const int nb = launch_matvec_parity<T>(h.g[l], K, a, level_shape(l), s, 1);
+        if (l == 0) nb0 = nb;
+    }
+    hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(NT), 0, s, st_, rz_partials, (unsigned)nb0, upd_partials,
+                       (unsigned)kVecBlocks, fin_mode);
+    return HX_OK;
+}
+
+template <typename T>
 void launch_ml_exact_setup(const GridArgs& gc, const double* Ec, const double* Kbin, double h, const uint8_t* fixed_f,
-                           int stride, const GridArgs& gfine, double* A, T* ainv, hipStream_t s) {
-    hipLaunchKernelGGL((k_ml_exact_setup<T>), dim3(1), dim3(kExactThreads), 0, s, gc, Ec, Kbin, h, fixed_f, stride,
-                       gfine.NX, gfine.NY, A, ainv);
+                           int stride, const GridArgs& gfine, T* ainv, hipStream_t s) {
+    const long N = 3 * gc.nn;
+    hipLaunchKernelGGL((k_ml_exact_setup<T>), dim3(1), dim3(kExactThreads), N * (N + 1) / 2 * sizeof(double), s, gc,
+                       Ec, Kbin, h, fixed_f, stride, gfine.NX, gfine.NY, ainv);
 }
 template void launch_ml_exact_setup<double>(const GridArgs&, const double*, const double*, double, const uint8_t*, int,
-                                            const GridArgs&, double*, double*, hipStream_t);
+                                            const GridArgs&, double*, hipStream_t);
 template void launch_ml_exact_setup<float>(const GridArgs&, const double*, const double*, double, const uint8_t*, int,
-                                           const GridArgs&, double*, float*, hipStream_t);
+                                           const GridArgs&, float*, hipStream_t);
 
 // allow the coarse kernel's dynamic LDS above the default 64 KiB
 void ml_prepare_kernels() {
@@ -550,6 +743,11 @@ void ml_prepare_kernels() {
                               (int)kCoarseLdsBytes);
     (void)hipFuncSetAttribute((const void*)k_ml_coarse<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)kCoarseLdsBytes);
+    constexpr int kExactLds = kMlExactMax * (kMlExactMax + 1) / 2 * (int)sizeof(double);
+    (void)hipFuncSetAttribute((const void*)k_ml_exact_setup<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              kExactLds);
+    (void)hipFuncSetAttribute((const void*)k_ml_exact_setup<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              kExactLds);
     done = true;
 }
 
@@ -561,5 +759,17 @@ template void launch_ml_apply<double>(const MlHierarchy<double>&, const double*,
                                       double*, int, hipStream_t);
 template void launch_ml_apply<float>(const MlHierarchy<float>&, const float*, const float*, float*, PcgState*,
                                      double*, int, hipStream_t);
+template int vc_levels<double>(const MlHierarchy<double>&);
+template int vc_levels<float>(const MlHierarchy<float>&);
+template void launch_vc_smoother<double>(const GridArgs&, const double*, double, const double*, const uint8_t*, int,
+                                         const GridArgs&, double, double*, hipStream_t);
+template void launch_vc_smoother<float>(const GridArgs&, const double*, double, const double*, const uint8_t*, int,
+                                        const GridArgs&, double, float*, hipStream_t);
+template void launch_vc_scale_E<double>(long, const double*, double, double*, hipStream_t);
+template void launch_vc_scale_E<float>(long, const double*, double, float*, hipStream_t);
+template int launch_vc_apply<double>(const MlHierarchy<double>&, const ParityK<double>&, const MvShape&, const double*,
+                                     double*, PcgState*, double*, long, int, hipStream_t);
+template int launch_vc_apply<float>(const MlHierarchy<float>&, const ParityK<float>&, const MvShape&, const float*,
+                                    float*, PcgState*, double*, long, int, hipStream_t);
 
 }  // namespace hx

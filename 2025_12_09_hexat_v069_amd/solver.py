@@ -57,12 +57,14 @@ def _precision_of(config):
 
 
 def _precond_of(config):
-    """solver_parameters.preconditioner: "jacobi" (the reference's) or "multilevel" (added here)."""
+    """solver_parameters.preconditioner: "jacobi" (the reference's), "multilevel" or "vcycle" (added here)."""
     sp = (config or {}).get("solver_parameters", {})
     p = str(sp.get("preconditioner", "jacobi")).lower()
-    if p not in ("jacobi", "multilevel"):
-        raise ValueError(f"solver_parameters.preconditioner: {p!r} (jacobi | multilevel)")
-    return (_lib.HX_PC_MULTILEVEL, "Multilevel") if p == "multilevel" else (_lib.HX_PC_JACOBI, "Jacobi")
+    kinds = {"jacobi": (_lib.HX_PC_JACOBI, "Jacobi"), "multilevel": (_lib.HX_PC_MULTILEVEL, "Multilevel"),
+             "vcycle": (_lib.HX_PC_VCYCLE, "V-cycle")}
+    if p not in kinds:
+        raise ValueError(f"solver_parameters.preconditioner: {p!r} (jacobi | multilevel | vcycle)")
+    return kinds[p]
 
 
 def solve_system_gpu(nodes, elements, E, nu, bc_indicator, f, density, *, max_iter=40000, tol=1e-6,

@@ -146,23 +146,61 @@ def other_workload(args, hx, L, ctx):
     print(json.dumps(out), flush=True)
 
 
+def level_dims(nx, ny, nz):
+    """Element counts of the preconditioner's levels: 2:1 coarsenings while every count is even,
+    ending at the first level with at most 192 DOFs (solved exactly)."""
+    dims = [(nx, ny, nz)]
+    while all(d % 2 == 0 for d in dims[-1]):
+        dims.append(tuple(d // 2 for d in dims[-1]))
+        a, b, c = dims[-1]
+        if 3 * (a + 1) * (b + 1) * (c + 1) <= 192:
+            break
+    return dims
+
+
+def smoothed_levels(nx, ny, nz, word):
+    """Levels the V-cycle smooths: those above the first level (>= 1) with at most 2048 nodes whose
+    levels' r and y (6 values per node) fit 128 KiB, i.e. above the one-workgroup coarse pass."""
+    nn = [(a + 1) * (b + 1) * (c + 1) for a, b, c in level_dims(nx, ny, nz)]
+    L = len(nn) - 1
+    s0 = 1
+    while s0 <= L and (nn[s0] > 2048 or 6 * sum(nn[s0:]) * word > 128 * 1024):
+        s0 += 1
+    return 0 if L == 0 else min(s0, L)
+
+
 def ml_apply_bytes(nx, ny, nz, word):
     """Algorithmic HBM bytes of one multilevel preconditioner application:
     every restriction reads its level and writes the next; every
     prolongation reads r_l, D_l^-1 and y_{l+1} and writes y_l (z on the fine
     grid); the coarsest level reads D_L^-1 and writes y_L."""
-    dims, n = [(nx, ny, nz)], []
-    while all(d % 2 == 0 for d in dims[-1]):
-        dims.append(tuple(d // 2 for d in dims[-1]))
+    dims = level_dims(nx, ny, nz)
     n = [3 * (a + 1) * (b + 1) * (c + 1) for a, b, c in dims]
     L = len(n) - 1
     v = sum(n[l] + n[l + 1] for l in range(L)) + sum(3 * n[l] + n[l + 1] for l in range(L)) + 2 * n[L]
     return float(v * word)
 
 
+def vc_apply_bytes(nx, ny, nz, word, smoothed):
+    """Algorithmic HBM bytes of one V-cycle application: on each smoothed level
+    the pre-smoothing operator reads r, S and the moduli and writes x and the
+    residual, the restriction reads the residual and writes the next level's
+    r, the prolongation reads x, S and the coarse correction and writes x, the
+    post-smoothing operator reads x, r, S and the moduli and writes its output;
+    the additive levels below read D_l^-1 and write y_l."""
+    dims = level_dims(nx, ny, nz)
+    n = [3 * (a + 1) * (b + 1) * (c + 1) for a, b, c in dims]
+    ne = [a * b * c for a, b, c in dims]
+    v = sum(12 * n[l] + 2 * ne[l] + 2 * n[l + 1] for l in range(smoothed))
+    v += sum(2 * n[l] for l in range(smoothed, len(n)))
+    return float(v * word)
+
+
 # form -> (precision, preconditioner)
 FORMS = {"plain": ("HX_FP64", "HX_PC_JACOBI"), "precision": ("HX_FP32_IR", "HX_PC_JACOBI"),
-         "plain_ml": ("HX_FP64", "HX_PC_MULTILEVEL"), "precision_ml": ("HX_FP32_IR", "HX_PC_MULTILEVEL")}
+         "plain_ml": ("HX_FP64", "HX_PC_MULTILEVEL"), "precision_ml": ("HX_FP32_IR", "HX_PC_MULTILEVEL"),
+         "plain_vc": ("HX_FP64", "HX_PC_VCYCLE"), "precision_vc": ("HX_FP32_IR", "HX_PC_VCYCLE")}
+EXPLICIT_PC = ("_ml", "_vc")  # forms whose operator is the z-form (p = z + beta p_old)
 
 
 def run_form(hx, L, ctx, nx, ny, nz, f, fixed, precision, steps, warmup, args, tune=None, precond=0):
@@ -191,7 +229,8 @@ def run_form(hx, L, ctx, nx, ny, nz, f, fixed, precision, steps, warmup, args, t
             f"rel={s.pcg_rel_res:.2e} filt={s.filter_iterations} vm_max={s.max_vm:.3e} t={s.seconds:.3f}s")
     wall = time.perf_counter() - t0  # to_step synchronises the stream at its end
     # operator kernel alone: back-to-back launches at the production shape
-    op_ms = ctx.time_operator(precision, 1, 40, precond if precond == L.HX_PC_MULTILEVEL else L.HX_PC_JACOBI)
+    op_ms = ctx.time_operator(precision, 1, 40, precond if precond in (L.HX_PC_MULTILEVEL, L.HX_PC_VCYCLE)
+                              else L.HX_PC_JACOBI)
     # kernel breakdown: one more TO iteration with HIP events around every hot
     # launch (direct launches, so its wall time is not the reported value)
     ctx.timing(True)
@@ -291,11 +330,16 @@ def measure_traffic(args, form, tune=None):
         fp32 = FORMS[form][0] == "HX_FP32_IR"
         variant = tune["variant32"] if fp32 else tune["variant64"]
         tname = "float" if fp32 else "double"
-        # the multilevel forms always run the parity contraction (its z + beta p form)
-        key = f"k_matvec_sf<{tname}" if (variant == 1 or form.endswith("_ml")) else f"k_matvec_zmarch<{tname}"
+        # the explicit-preconditioner forms always run the parity contraction (its z + beta p form,
+        # template argument FUSE = 2; the V-cycle's smoothing forms 3 and 4 run as often)
+        explicit = form.endswith(EXPLICIT_PC)
+        key = f"k_matvec_sf<{tname}" if (variant == 1 or explicit) else f"k_matvec_zmarch<{tname}"
         cand = {}
         for r in rows:
             if key in kname(r):
+                m = re.search(r"k_matvec_sf<\w+, \d+, \d+, (\d), \d>", kname(r))
+                if explicit and (m is None or m.group(1) != "2"):
+                    continue
                 cand.setdefault((kname(r), r.get("Grid_Size") or r.get("grid_size")), []).append(val(r))
         if not copy or not cand:
             return {"error": f"{counter}: kernels not found in the counter output ({len(rows)} rows)"}, tune
@@ -353,9 +397,10 @@ def main():
     ap.add_argument("--rmin", type=float, default=2.5)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--max-iter", type=int, default=100000)
-    ap.add_argument("--forms", default="plain,precision,precision_ml",
+    ap.add_argument("--forms", default="plain,precision,precision_ml,precision_vc",
                     help="comma-separated: plain (FP64 Jacobi-PCG), precision (FP32 operator + FP64 refinement), "
-                         "plain_ml / precision_ml (the same with the multilevel preconditioner)")
+                         "plain_ml / precision_ml (the same with the additive multilevel preconditioner), "
+                         "plain_vc / precision_vc (with the V-cycle)")
     ap.add_argument("--main-form", default=None,
                     help="the form `value` reports (default: precision_ml when run, else precision)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -367,7 +412,7 @@ def main():
                          "the precision and batching configurations (their own JSON line)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes (roofline.traffic = null)")
-    ap.add_argument("--kernel-sample", choices=("tune", "plain", "precision", "plain_ml", "precision_ml"), default=None,
+    ap.add_argument("--kernel-sample", choices=("tune",) + tuple(FORMS), default=None,
                     help=argparse.SUPPRESS)  # child modes of the traffic passes
     ap.add_argument("--tune-json", default=None,
                     help="apply this tuning result (the 'tune' object of an earlier run's JSON line) instead of "
@@ -390,7 +435,8 @@ def main():
     for fm in forms:
         if fm not in FORMS:
             sys.exit(f"unknown form {fm!r} (choose from {', '.join(FORMS)})")
-    main_form = args.main_form or next((fm for fm in ("precision_ml", "precision") if fm in forms), forms[0])
+    main_form = args.main_form or next((fm for fm in ("precision_vc", "precision_ml", "precision") if fm in forms),
+                                       forms[0])
     if main_form not in forms:
         sys.exit(f"--main-form {main_form!r} is not among --forms")
     traffic, tune = (None, json.loads(args.tune_json) if args.tune_json else None)
@@ -416,8 +462,8 @@ def main():
     # launch: the PCG matvec reads r, D^-1, p_old (the p-update is fused into
     # its plane loads) and E, and writes q and p_new; the update reads x, p, r,
     # q, D^-1 and writes x, r.
-    # (multilevel forms: the operator reads p_old and z = M^-1 r instead of r and D^-1)
-    ml_main = main_form.endswith("_ml")
+    # (explicit-preconditioner forms: the operator reads p_old and z = M^-1 r instead of r and D^-1)
+    ml_main = main_form.endswith(EXPLICIT_PC)
     mv_bytes64 = (32.0 if ml_main else 40.0) * ndof + 8.0 * ne
     mv_bytes32 = (16.0 if ml_main else 20.0) * ndof + 4.0 * ne
     fp64_main = FORMS[main_form][0] == "HX_FP64"
@@ -444,8 +490,15 @@ def main():
                 "avg_launch_ms": avg, "traffic_detail": traffic}
     if ml_main and r["kern"].get("ml_precond", {}).get("count"):
         k = r["kern"]["ml_precond"]
-        by = ml_apply_bytes(nx, ny, nz, 8 if fp64_main else 4)
-        roof_ml = {"kernels": "k_ml_restrict_tiled, k_ml_coarse, k_ml_prolong, k_fin_rz (one application)",
+        if main_form.endswith("_vc"):
+            sl = smoothed_levels(nx, ny, nz, 8 if fp64_main else 4)
+            by = vc_apply_bytes(nx, ny, nz, 8 if fp64_main else 4, sl)
+            kn = ("k_matvec_sf forms 3 and 4, k_ml_restrict_tiled, k_ml_coarse, k_vc_prolong, k_fin_rz "
+                  f"({sl} smoothed levels, one application)")
+        else:
+            by = ml_apply_bytes(nx, ny, nz, 8 if fp64_main else 4)
+            kn = "k_ml_restrict_tiled, k_ml_coarse, k_ml_prolong, k_fin_rz (one application)"
+        roof_ml = {"kernels": kn,
                    "avg_ms": k["avg_ms"], "bytes_per_apply": by, "achieved": round(by / (k["avg_ms"] * 1e-3) / 1e9, 1),
                    "unit": "GB/s", "frac": round(by / (k["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     else:
@@ -464,7 +517,7 @@ def main():
         "config": {"workload": f"cantilever {nx}x{ny}x{nz} ({ne} elements, {ndof} DOFs), SIMP p=3, volfrac "
                                f"{args.volfrac}, PDE filter r={args.rmin}, PCG rel. residual {args.tol}, von Mises "
                                f"each iteration", "form": main_form,
-                   "preconditioner": "multilevel" if main_form.endswith("_ml") else "jacobi",
+                   "preconditioner": {"_ml": "multilevel", "_vc": "vcycle"}.get(main_form[-3:], "jacobi"),
                    "iterations_timed": f"TO iterations {args.warmup + 1}..{args.warmup + args.steps} from uniform "
                                        f"rho={args.volfrac}", "tune": tune},
         "roofline": roof,

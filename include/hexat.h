@@ -71,7 +71,12 @@ enum { HX_FP64 = 0, HX_FP32_IR = 1 };           /* precision form */
  * symmetric form of a U-Net preconditioner; plain PCG, one right-hand side.
  * No reference counterpart (the reference's strong-preconditioner slot is the
  * learned one, and its CPU AMG branch, GPUSolver.jl:168-268). */
-enum { HX_PC_JACOBI = 0, HX_PC_LEARNED = 1, HX_PC_MULTILEVEL = 2 };
+/* HX_PC_VCYCLE: the same hierarchy as a symmetric V-cycle on the levels too
+ * large for one workgroup (l1-Jacobi smoothing, omega / absolute row sums of
+ * the rediscretised level stiffness, one step before and one after the
+ * coarse correction), the additive form above below them; SPD for every
+ * moduli field; one right-hand side.  No reference counterpart. */
+enum { HX_PC_JACOBI = 0, HX_PC_LEARNED = 1, HX_PC_MULTILEVEL = 2, HX_PC_VCYCLE = 3 };
 enum { HX_MV_AUTO = 0, HX_MV_PARITY = 1, HX_MV_DENSE = 2, HX_MV_GATHER = 3, HX_MV_MFMA = 4 };
 
 typedef struct {
@@ -79,7 +84,7 @@ typedef struct {
     int max_iter;     /* PCG iteration cap */
     double shift;     /* absolute diagonal shift s (K_ff + s I), GPUSolver.jl:391 */
     int precision;    /* HX_FP64 | HX_FP32_IR */
-    int precond;      /* HX_PC_JACOBI | HX_PC_LEARNED */
+    int precond;      /* HX_PC_JACOBI | HX_PC_LEARNED | HX_PC_MULTILEVEL | HX_PC_VCYCLE */
     int warm_start;   /* 1: u_inout holds the start vector (GPUSolver.jl:407-412) */
     int matvec;       /* HX_MV_* kernel variant (HX_MV_AUTO = tuned choice) */
     int batch;        /* iterations per host check (0 = default) */
@@ -157,7 +162,7 @@ typedef struct {
     double pcg_tol;       /* elasticity relative residual target */
     int pcg_max_iter;
     int precision;        /* HX_FP64 | HX_FP32_IR */
-    int precond;          /* HX_PC_JACOBI | HX_PC_LEARNED */
+    int precond;          /* HX_PC_JACOBI | HX_PC_LEARNED | HX_PC_MULTILEVEL | HX_PC_VCYCLE */
     double filter_tol;    /* Helmholtz filter relative residual target */
     int filter_max_iter;
     double oc_rel_tol;    /* OC bisection stop */
@@ -229,9 +234,9 @@ int hx_autotune(hx_ctx* ctx, int nrhs, hx_tune_result* result);
  * counterpart. */
 int hx_time_operator(hx_ctx* ctx, int precision, int nrhs, int reps, double* avg_ms);
 /* The same for the operator form a preconditioner uses: HX_PC_JACOBI (fused
- * p = D^-1 r + beta p_old, as hx_time_operator) or HX_PC_MULTILEVEL (fused
- * p = z + beta p_old from an explicit z, parity contraction).  No reference
- * counterpart. */
+ * p = D^-1 r + beta p_old, as hx_time_operator) or HX_PC_MULTILEVEL /
+ * HX_PC_VCYCLE (fused p = z + beta p_old from an explicit z, parity
+ * contraction).  No reference counterpart. */
 int hx_time_operator_pc(hx_ctx* ctx, int precision, int precond, int nrhs, int reps, double* avg_ms);
 /* Apply a tuning result (e.g. one cached from an earlier hx_autotune) without
  * timing anything: its launch shapes and contractions.  No reference

@@ -25,6 +25,11 @@ int hx_test_copy(hx_ctx* ctx, long bytes, int reps, double* avg_ms);
  * (precision HX_FP64) or with FP32 vectors (HX_FP32_IR); *levels receives
  * the number of coarse levels.  No reference counterpart. */
 int hx_test_ml_apply(hx_ctx* ctx, const double* Ee, const double* r, double* z, int precision, int* levels);
+/* The same for the V-cycle form (HX_PC_VCYCLE); *vlevels receives the number
+ * of smoothed levels (the additive form takes over below them).  No
+ * reference counterpart. */
+int hx_test_vc_apply(hx_ctx* ctx, const double* Ee, const double* r, double* z, int precision, int* levels,
+                     int* vlevels);
 
 /* number of network parameters (conv weights then bias, layer by layer) */
 int hx_test_unet_nparam(hx_ctx* ctx, long* nparam);

@@ -19,6 +19,10 @@ pieces in numpy:
 * ``ml_apply`` / ``ml_pcg`` -- additive multilevel diagonal scaling (the BPX
   form of Bramble, Pasciak & Xu 1990 with Jacobi level scalings) in separable
   numpy passes; no reference code exists.
+* ``vc_apply`` / ``vc_pcg`` -- the same hierarchy as a symmetric V-cycle with
+  l1-Jacobi smoothing (absolute row sums, the smoother of Baker, Falgout,
+  Kolev & Yang 2011) on the levels too large for one workgroup; no reference
+  code exists.
 """
 from __future__ import annotations
 
@@ -540,6 +544,175 @@ def ml_pcg(g: Grid, Ke, Ee, fixed, b, tol=1e-8, max_iter=100000, max_levels=12, 
         if rel < tol:
             return x, rel, it
         z = ml_apply(g, Ke, Ee, fixed, r, levels=levels)
+        rzn = r @ z
+        p = z + (rzn / rz) * p
+        rz = rzn
+    return x, rel, max_iter
+
+
+# ---------------------------------------------------------------------------------------------
+# V-cycle form (the library's HX_PC_VCYCLE): V-cycle on levels 0..c-1, the additive form from c
+
+ML_COARSE_NODES = 2048          # a level the one-workgroup coarse pass may take
+ML_COARSE_LDS = 128 * 1024      # its budget: r and y of every level it holds
+
+
+def ml_coarse_start(levels, elem_bytes):
+    """First level (>= 1) of the one-workgroup coarse pass: at most ML_COARSE_NODES nodes and its
+    levels' r, y (6 values per node) within ML_COARSE_LDS bytes; L + 1 when none qualifies."""
+    L = len(levels) - 1
+    nn = [(n[0] + 1) * (n[1] + 1) * (n[2] + 1) for n, _, _ in levels]
+    s0 = 1
+    while s0 <= L and (nn[s0] > ML_COARSE_NODES or 6 * sum(nn[s0:]) * elem_bytes > ML_COARSE_LDS):
+        s0 += 1
+    return s0
+
+
+def vc_levels(levels, elem_bytes=8):
+    """Smoothed levels c: 0 without coarse levels, else min(s0, L)."""
+    L = len(levels) - 1
+    return 0 if L == 0 else min(ml_coarse_start(levels, elem_bytes), L)
+
+
+def _ke_binary(Ke):
+    """24x24 Ke with the element's nodes in binary order L = bx + 2 by + 4 bz."""
+    e = Grid(1, 1, 1).elements()[0]
+    ref = [int(np.flatnonzero(e == (L & 1) + 2 * ((L >> 1) & 1) + 4 * (L >> 2))[0]) for L in range(8)]
+    idx = np.array([3 * ref[L] + c for L in range(8) for c in range(3)])
+    return _f64(Ke).reshape(24, 24)[np.ix_(idx, idx)]
+
+
+def vc_smoother(n, E, h, Ke, fixed_l, omega):
+    """omega / sum_j |A_ij| over the solved DOFs j of the level stiffness (moduli E, element
+    stiffness h Ke), 0 where DOF i is not solved for; as (NZ, NY, NX, 3).  The 3x3 coupling of
+    node (i,j,k) with each of its 27 neighbours is summed over their shared elements first."""
+    nx, ny, nz = n
+    Kb = _ke_binary(Ke)
+    Ep = np.zeros((nz + 2, ny + 2, nx + 2))
+    Ep[1:-1, 1:-1, 1:-1] = h * _f64(E).reshape(nz, ny, nx)
+    NZ, NY, NX = nz + 1, ny + 1, nx + 1
+    B = np.zeros((3, 3, 3, NZ, NY, NX, 3, 3))  # [dk+1, dj+1, di+1, node, a, b]
+    for ez in (0, 1):
+        for ey in (0, 1):
+            for ex in (0, 1):
+                # element (i-1+ex, j-1+ey, k-1+ez) of node (i,j,k): padded index (k+ez, j+ey, i+ex)
+                Ee = Ep[ez:ez + NZ, ey:ey + NY, ex:ex + NX]
+                La = (1 - ex) + 2 * (1 - ey) + 4 * (1 - ez)
+                for bz in (0, 1):
+                    for by in (0, 1):
+                        for bx in (0, 1):
+                            Lb = bx + 2 * by + 4 * bz
+                            di, dj, dk = bx - (1 - ex), by - (1 - ey), bz - (1 - ez)
+                            blk = Kb[3 * La:3 * La + 3, 3 * Lb:3 * Lb + 3]
+                            B[dk + 1, dj + 1, di + 1] += Ee[..., None, None] * blk[None, None, None]
+    fx = np.asarray(fixed_l, dtype=bool)
+    fxp = np.ones((NZ + 2, NY + 2, NX + 2, 3), dtype=bool)
+    fxp[1:-1, 1:-1, 1:-1] = fx
+    l1 = np.zeros((NZ, NY, NX, 3))
+    for dk in (-1, 0, 1):
+        for dj in (-1, 0, 1):
+            for di in (-1, 0, 1):
+                solved = ~fxp[1 + dk:1 + dk + NZ, 1 + dj:1 + dj + NY, 1 + di:1 + di + NX]  # (..., b)
+                l1 += (np.abs(B[dk + 1, dj + 1, di + 1]) * solved[..., None, :]).sum(axis=-1)
+    diag = np.stack([B[1, 1, 1][..., a, a] for a in range(3)], axis=-1)
+    keep = ~fx & (diag > 0)
+    return np.where(keep, omega / np.where(keep, l1, 1.0), 0.0)
+
+
+def _restrict3(a):
+    for ax in (0, 1, 2):
+        a = _restrict_axis(a, ax)
+    return a
+
+
+def _prolong3(a):
+    for ax in (0, 1, 2):
+        a = _prolong_axis(a, ax)
+    return a
+
+
+def _additive_from(levels, c, rc):
+    """y_c of the additive form started at level c: D_l^-1 r_l + P y_{l+1} (unmasked P), the
+    coarsest level by its inverse (exact) or its diagonal."""
+    L = len(levels) - 1
+    rs = [rc]
+    for l in range(c + 1, L + 1):
+        rs.append(_restrict3(rs[-1]))
+    ainv = levels[L][2]
+    y = (ainv @ rs[-1].ravel()).reshape(rs[-1].shape) if ainv is not None else levels[L][1] * rs[-1]
+    for l in range(L - 1, c - 1, -1):
+        y = levels[l][1] * rs[l - c] + _prolong3(y)
+    return y
+
+
+def vc_apply(g: Grid, Ke, Ee, fixed, r, omega=1.95, max_levels=12, exact=True, elem_bytes=8, state=None):
+    """z = M^-1 r of the V-cycle form: on levels l < c (vc_levels), with S_l = vc_smoother and A_l
+    the level operator (moduli 2^l E_l averaged over children, the fine Ke),
+      x = S r;  res = mask(r - A x);  e = V_{l+1}(R res);  x += mask(P e);  z = x + S (r - A x)
+    and level c by the additive form (_additive_from).  `state` caches the levels between calls."""
+    Ke = _f64(Ke)
+    if state is None or "levels" not in state:
+        levels = ml_hierarchy(g, Ke, Ee, fixed, max_levels, exact)
+        c = vc_levels(levels, elem_bytes)
+        fx = np.asarray(fixed, dtype=bool).reshape(g.nz + 1, g.ny + 1, g.nx + 1, 3)
+        E, n, Es, Ss = _f64(Ee), (g.nx, g.ny, g.nz), [], []
+        for l in range(c):
+            if l > 0:
+                E = _coarsen_E(E, *n)
+                n = tuple(v // 2 for v in n)
+            h = 2 ** l
+            Es.append(h * E)
+            Ss.append(vc_smoother(n, E, h, Ke, fx[::h, ::h, ::h], omega))
+        st = {"levels": levels, "c": c, "E": Es, "S": Ss}
+        if state is not None:
+            state.update(st)
+    else:
+        st = state
+    levels, c = st["levels"], st["c"]
+    r0 = _f64(r).reshape(g.nz + 1, g.ny + 1, g.nx + 1, 3)
+    if c == 0:
+        return (levels[0][1] * r0).ravel()
+
+    def A(l, x):
+        n = levels[l][0]
+        return matvec(Grid(*n), Ke, st["E"][l], x.ravel()).reshape(x.shape)
+
+    def V(l, rl):
+        if l == c:
+            return _additive_from(levels, c, rl)
+        S = st["S"][l]
+        x = S * rl
+        res = np.where(S != 0.0, rl - A(l, x), 0.0)
+        e = V(l + 1, _restrict3(res))
+        x = x + np.where(S != 0.0, _prolong3(e), 0.0)
+        return x + S * (rl - A(l, x))
+
+    return V(0, r0).ravel()
+
+
+def vc_pcg(g: Grid, Ke, Ee, fixed, b, tol=1e-8, max_iter=100000, omega=1.95, max_levels=12, exact=True):
+    """Plain PCG with the V-cycle preconditioner (free DOFs; fixed stay 0)."""
+    Ke, Ee, b = _f64(Ke), _f64(Ee), _f64(b)
+    state = {}
+    z = vc_apply(g, Ke, Ee, fixed, b, omega, max_levels, exact, state=state)
+    mask = state["levels"][0][1].ravel() != 0.0
+    b = np.where(mask, b, 0.0)
+    x = np.zeros(g.ndof)
+    r = b.copy()
+    z = vc_apply(g, Ke, Ee, fixed, r, state=state)
+    p = z.copy()
+    rz = r @ z
+    nb = np.linalg.norm(b)
+    rel = 1.0
+    for it in range(1, max_iter + 1):
+        q = np.where(mask, matvec(g, Ke, Ee, p), 0.0)
+        a = rz / (p @ q)
+        x += a * p
+        r -= a * q
+        rel = np.linalg.norm(r) / nb
+        if rel < tol:
+            return x, rel, it
+        z = vc_apply(g, Ke, Ee, fixed, r, state=state)
         rzn = r @ z
         p = z + (rzn / rz) * p
         rz = rzn

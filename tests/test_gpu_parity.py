@@ -285,7 +285,8 @@ def test_fp32_contraction_variants_solve_to_fp64_tolerance(ox, hx, ctx, variant)
     assert abs(f @ u - f @ uo) < 1e-7 * abs(f @ uo)
 
 
-SHAPES = [(0, 8, 1), (0, 16, 2), (0, 32, 1), (1, 8, 1), (1, 16, 2), (2, 8, 1), (2, 16, 2), (2, 32, 1)]
+SHAPES = [(0, 8, 1), (0, 16, 2), (0, 32, 1), (1, 8, 1), (1, 16, 2), (2, 8, 1), (2, 16, 2), (2, 32, 1), (2, 2, 2),
+          (2, 1, 1), (0, 1, 2)]
 
 
 def _tune(tile, kchunk, pdepth):
@@ -297,8 +298,9 @@ def _tune(tile, kchunk, pdepth):
 def test_operator_launch_shapes_match_oracle(ox, hx, ctx, shape):
     """Every launch shape the autotuner can pick (16x16 / 16x32 / 32x8 tiles,
     z-chunks, one or two prefetched planes) applies the operator to FP64
-    round-off and drives both PCG forms (fused p-update) and the multilevel
-    form (fused z + beta p) to the FP64 tolerance."""
+    round-off and drives both PCG forms (fused p-update), the multilevel
+    form (fused z + beta p) and the V-cycle (its smoothing forms) to the FP64
+    tolerance."""
     _say("operator shape", shape)
     for dims in ((17, 16, 18), (31, 9, 40)):
         g = _grid_setup(ox, ctx, dims)
@@ -316,7 +318,7 @@ def test_operator_launch_shapes_match_oracle(ox, hx, ctx, shape):
     Ee = 1e-9 + np.clip(np.random.default_rng(3).uniform(0, 1, g.ne), 0.01, 1.0) ** 3
     Ke = ox.ke_hex(1, 1, 1, 0.3)
     for precision in (0, 1):
-        for pc in (hx._lib.HX_PC_JACOBI, hx._lib.HX_PC_MULTILEVEL):
+        for pc in (hx._lib.HX_PC_JACOBI, hx._lib.HX_PC_MULTILEVEL, hx._lib.HX_PC_VCYCLE):
             u, st = ctx.pcg(Ee, f, tol=1e-8, precision=precision, precond=pc)
             assert st.status == 0 and _true_rel(ox, g, Ke, Ee, fixed, f, u) < 1.05e-8, (precision, pc, st.rel_res)
 

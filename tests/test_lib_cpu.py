@@ -60,13 +60,14 @@ def test_context_without_gpu_fails_loudly(hx):
 
 
 def test_solver_config_preconditioner_mapping(hx):
-    """solver_parameters.preconditioner selects the multilevel form; anything
-    else but jacobi is rejected before any device work."""
+    """solver_parameters.preconditioner selects the multilevel or V-cycle
+    form; anything else but jacobi is rejected before any device work."""
     import importlib
     solver = importlib.import_module(hx.__name__ + ".solver")
     L = hx._lib
     assert solver._precond_of(None) == (L.HX_PC_JACOBI, "Jacobi")
     assert solver._precond_of({"solver_parameters": {"preconditioner": "Multilevel"}}) == (L.HX_PC_MULTILEVEL,
                                                                                            "Multilevel")
+    assert solver._precond_of({"solver_parameters": {"preconditioner": "vcycle"}}) == (L.HX_PC_VCYCLE, "V-cycle")
     with pytest.raises(ValueError):
         solver._precond_of({"solver_parameters": {"preconditioner": "amg"}})
