@@ -44,6 +44,7 @@ EXPORTS = [
     "hx_block_pcg_solve", "hx_helmholtz_filter", "hx_element_stress", "hx_compliance_sens", "hx_oc_update",
     "hx_to_init", "hx_to_step", "hx_to_read", "hx_learned_train", "hx_learned_weights", "hx_timing_enable",
     "hx_timing_read", "hx_device_info", "hx_autotune", "hx_tune_set", "hx_time_operator", "hx_time_operator_pc",
+    "hx_time_smoothing",
 ]
 #: test hooks declared in include/hexat_testing.h
 TEST_EXPORTS = ["hx_test_unet_nparam", "hx_test_unet_init", "hx_test_unet_forward", "hx_test_unet_backward",
@@ -147,6 +148,7 @@ def load():
         L.hx_tune_set.argtypes = [vp, P(TuneResult)]
         L.hx_time_operator.argtypes = [vp, i, i, i, P(C.c_double)]
         L.hx_time_operator_pc.argtypes = [vp, i, i, i, i, P(C.c_double)]
+        L.hx_time_smoothing.argtypes = [vp, i, i, i, P(C.c_double)]
         L.hx_test_copy.argtypes = [vp, C.c_long, i, P(C.c_double)]
         L.hx_test_ml_apply.argtypes = [vp, pd, pd, pd, i, P(C.c_int)]
         L.hx_test_vc_apply.argtypes = [vp, pd, pd, pd, i, P(C.c_int), P(C.c_int)]
@@ -420,6 +422,14 @@ class Context:
         ms = C.c_double(0)
         self.check(self.L.hx_time_operator_pc(self.h, int(precision), int(precond), int(nrhs), int(reps),
                                               C.byref(ms)), "hx_time_operator_pc")
+        return ms.value
+
+    def time_smoothing(self, precision=HX_FP32_IR, form=4, reps=20) -> float:
+        """Average ms of the fine level's V-cycle smoothing operator (form 3 pre-, 4 post-smoothing with
+        the fused prolongation), back to back on rotating vector sets."""
+        ms = C.c_double(0)
+        self.check(self.L.hx_time_smoothing(self.h, int(precision), int(form), int(reps), C.byref(ms)),
+                   "hx_time_smoothing")
         return ms.value
 
     def test_copy(self, nbytes: int, reps: int = 4) -> float:

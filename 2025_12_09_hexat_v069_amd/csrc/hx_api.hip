@@ -1846,7 +1846,82 @@ int time_operator_t(hx_ctx* c, int nrhs, int reps, float* ms, bool zform) {
     c->free_events.push_back(b);
     return HX_OK;
 }
+
+// back-to-back launches of the fine level's V-cycle smoothing operator (form
+// 3 or 4) on kSets vector sets at the current launch shape; form 4 reads a
+// coarse correction on the first coarsening's node grid and writes its r'z
+// partials, as in the solve
+template <typename T>
+int time_smoothing_t(hx_ctx* c, int form, int reps, float* ms) {
+    constexpr int kSets = 4;
+    const long n = ndof(c);
+    const GridArgs gc = ml_coarse_grid(c->g);
+    DevBuf<T> r[kSets], s[kSets], y[kSets], e[kSets];
+    for (int k = 0; k < kSets; ++k) {
+        HXA(r[k], n);
+        HXA(s[k], n);
+        HXA(y[k], n);
+        HXA(e[k], 3 * gc.nn);
+        if constexpr (sizeof(T) == 8) {
+            launch_fill(n, 0.25, r[k].p, c->stream);
+            launch_fill(n, 0.5, s[k].p, c->stream);
+            launch_fill(3 * gc.nn, 0.125, e[k].p, c->stream);
+        } else {
+            launch_fill32(n, 0.25f, r[k].p, c->stream);
+            launch_fill32(n, 0.5f, s[k].p, c->stream);
+            launch_fill32(3 * gc.nn, 0.125f, e[k].p, c->stream);
+        }
+    }
+    auto go = [&](int k) {
+        MvArgs<T> a{nullptr, y[k].p, nullptr, T(0), MV_PLAIN, nullptr, form == 4 ? c->partials.p + 2 * kVecBlocks
+                                                                                 : nullptr,
+                    n, r[k].p, s[k].p, nullptr};
+        a.form = form;
+        a.ec = e[k].p;
+        a.cNX = gc.NX;
+        a.cNY = gc.NY;
+        a.cNZ = gc.NZ;
+        if constexpr (sizeof(T) == 8) {
+            a.E = c->E.p;
+            launch_matvec_parity<double>(c->g, c->kp64, a, c->shape64, c->stream, 1);
+        } else {
+            a.E = c->E32.p;
+            launch_matvec_parity<float>(c->g, c->kp32, a, c->shape32, c->stream, 1);
+        }
+    };
+    hipEvent_t a = get_event(c), b = get_event(c);
+    for (int k = 0; k < kSets; ++k) go(k);
+    (void)hipEventRecord(a, c->stream);
+    for (int i = 0; i < reps; ++i) go(i % kSets);
+    (void)hipEventRecord(b, c->stream);
+    HXC(hipEventSynchronize(b));
+    (void)hipEventElapsedTime(ms, a, b);
+    c->free_events.push_back(a);
+    c->free_events.push_back(b);
+    return HX_OK;
+}
 }  // extern "C++"
+
+int hx_time_smoothing(hx_ctx* c, int precision, int form, int reps, double* avg_ms) {
+    if (!c || reps < 1 || !avg_ms || (form != 3 && form != 4)) return HX_ERR_BAD_ARG;
+    if (precision != HX_FP64 && precision != HX_FP32_IR) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    if (ml_levels_for(c->g, 1) < 1) return fail(c, HX_ERR_BAD_ARG, "the grid has no 2:1 coarsening");
+    if (2L * kVecBlocks + matvec_parity_blocks(c->g, c->shape32.tile, c->shape32.kchunk) > kOpPartials ||
+        2L * kVecBlocks + matvec_parity_blocks(c->g, c->shape64.tile, c->shape64.kchunk) > kOpPartials)
+        return fail(c, HX_ERR_BAD_ARG, "grid too large for the V-cycle's partial sums");
+    if (int rc = ensure_fp32(c)) return rc;
+    const bool was_timing = c->timing;
+    c->timing = false;
+    float ms = 0.f;
+    const int rc = precision == HX_FP64 ? time_smoothing_t<double>(c, form, reps, &ms)
+                                        : time_smoothing_t<float>(c, form, reps, &ms);
+    c->timing = was_timing;
+    if (rc) return rc;
+    HXK();
+    *avg_ms = (double)ms / reps;
+    return HX_OK;
+}
 
 int hx_time_operator(hx_ctx* c, int precision, int nrhs, int reps, double* avg_ms) {
     return hx_time_operator_pc(c, precision, HX_PC_JACOBI, nrhs, reps, avg_ms);

@@ -317,7 +317,9 @@ __global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
     constexpr bool PRO = FUSE == 4;  // fused prolongation of a coarse correction
     // coarse footprint of the tile's node columns (i0-1 .. i0-1+TX, j0-1 .. j0-1+TY)
     constexpr int CPX = TX / 2 + 2, CPY = TY / 2 + 2, CVALS = 3 * CPX * CPY, CPER = (CVALS + NT - 1) / NT;
-    __shared__ T cpl[PRO ? 2 : 1][PRO ? CVALS : 1];  // slot (coarse plane & 1): [(c CPY + ly) CPX + lx]
+    // slot (coarse plane & 1), and slot 2 = the average of the two held planes
+    // (what an odd fine plane needs): [(c CPY + ly) CPX + lx]
+    __shared__ T cpl[PRO ? 3 : 1][PRO ? CVALS : 1];
     __shared__ T so[POUT ? 2 : 1][OVALS];  // results of the owned nodes, node-major, for row-contiguous stores
     __shared__ double red[NT / 64];
 
@@ -419,23 +421,48 @@ __global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
                     coarse_store(1, 0, cv10);
                     coarse_store(1, 1, cv11);
                 }
+                // adjacent planes held: their average, from the entries this thread parked itself
+                if ((pend0 || pend1) && (held0 - held1 == 1 || held1 - held0 == 1)) {
+#pragma unroll
+                    for (int m = 0; m < CPER; ++m) {
+                        const int idx = (int)threadIdx.x + m * NT;
+                        if (idx < CVALS) {
+                            const int ly = idx / (3 * CPX), rem = idx - ly * 3 * CPX, lx = rem / 3, c = rem - 3 * lx;
+                            const int o = (c * CPY + ly) * CPX + lx;
+                            cpl[PRO ? 2 : 0][o] = T(0.5) * (cpl[0][o] + cpl[PRO ? 1 : 0][o]);
+                        }
+                    }
+                }
                 pend0 = pend1 = false;
             }
         };
-        // trilinear value of component c of the correction at fine node (fi, fj) of fine plane mf
-        [[maybe_unused]] auto coarse_at = [&](int fi, int fj, int c, int mf) __attribute__((always_inline)) -> T {
-            const int mm = min(max(mf, 0), g.NZ - 1), A = mm >> 1;
+        // trilinear interpolation of the correction: per thread, the LDS offset of
+        // the lower xy corner and the four xy weights of its node and of its extra
+        // value's node (fixed over the march); per fine plane, slot (plane >> 1) & 1
+        // for an even plane, the averaged slot 2 for an odd one
+        struct Corners {
+            int o;
+            T w00, w01, w10, w11;
+        };
+        auto corners = [&](int fi, int fj) __attribute__((always_inline)) {
+            Corners k;
             const int lx = (fi < 0 ? -1 : fi >> 1) - cbx, ly = (fj < 0 ? -1 : fj >> 1) - cby;
             const T wx1 = (fi & 1) ? T(0.5) : T(0), wx0 = T(1) - wx1;
             const T wy1 = (fj & 1) ? T(0.5) : T(0), wy0 = T(1) - wy1;
-            auto plane = [&](int K) {
-                const T* P = &cpl[PRO ? (K & 1) : 0][(c * CPY + ly) * CPX + lx];
-                return wy0 * (wx0 * P[0] + wx1 * P[1]) + wy1 * (wx0 * P[CPX] + wx1 * P[CPX + 1]);
-            };
-            if (mm & 1) return T(0.5) * (plane(A) + plane(A + 1));
-            return plane(A);
+            k.o = ly * CPX + lx;
+            k.w00 = wy0 * wx0;
+            k.w01 = wy0 * wx1;
+            k.w10 = wy1 * wx0;
+            k.w11 = wy1 * wx1;
+            return k;
         };
-
+        [[maybe_unused]] const Corners cn_own = corners(gi, gj), cn_x = corners(i0 - 1 + xtx, j0 - 1 + xty);
+        [[maybe_unused]] auto coarse_at = [&](const Corners& k, int c, int mf) __attribute__((always_inline)) -> T {
+            const int mm = min(max(mf, 0), g.NZ - 1);
+            const int slot = (mm & 1) ? 2 : ((mm >> 1) & 1);
+            const T* P = &cpl[PRO ? slot : 0][c * CPY * CPX + k.o];
+            return k.w00 * P[0] + k.w01 * P[1] + k.w10 * P[CPX] + k.w11 * P[CPX + 1];
+        };
         // PD register sets of a prefetched plane: x (and r, D^-1 when fused)
         // at this thread's node and extra value, plus E of element layer
         // k-1.  Addresses are clamped, so every load is unconditional (a load
@@ -481,7 +508,7 @@ __global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
                 else if constexpr (FUSE == 2) v = q.r[c] + beta * q.x[c];        // p = z + beta p
                 else if constexpr (FUSE == 3) v = q.m[c] * q.r[c];              // x = S r
                 else if constexpr (FUSE == 4)                                     // x = S r + mask(P e)
-                    v = q.m[c] * q.r[c] + (q.m[c] != T(0) ? coarse_at(gi, gj, c, q.k) : T(0));
+                    v = q.m[c] * q.r[c] + (q.m[c] != T(0) ? coarse_at(cn_own, c, q.k) : T(0));
                 else v = q.x[c];
                 xs[c][ty][tx] = v;
                 xnew[c] = v;
@@ -496,7 +523,7 @@ __global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
             else if constexpr (FUSE == 3) v = q.qm * q.qr;
             else if constexpr (FUSE == 4) {
                 v = q.qm * q.qr;
-                if (xstore && q.qm != T(0)) v += coarse_at(i0 - 1 + xtx, j0 - 1 + xty, xc, q.k);
+                if (xstore && q.qm != T(0)) v += coarse_at(cn_x, xc, q.k);
             }
             else v = q.qx;
             if (xstore) xs[xc][xty][xtx] = v;
@@ -543,11 +570,15 @@ __global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
         auto step = [&](int ke, const T* Flo, T* Fhi, Slot& q) __attribute__((always_inline)) {
             plane_store(q);  // plane ke+1, E of layer ke
             __syncthreads();
+            // form 4: the coarse planes of the plane the next layer stages, requested
+            // ahead of this layer's stores and prefetch: the memory counter is in
+            // order, so waiting for them at the layer's end leaves those in flight
+            coarse_need(ke + 2);
+            asm volatile("" ::: "memory");
             flush();
             asm volatile("" ::: "memory");
             plane_load(q, min(ke + 1 + PD, k1));  // unconditional (past k1: re-reads plane k1, a cache hit)
             asm volatile("" ::: "memory");  // keep the prefetch ahead of this layer's arithmetic
-            coarse_need(ke + 2);  // form 4: the coarse planes of the plane the next layer stages
             face_modes(Fhi);
 
             // element (gi, gj, ke): modes, contraction (E folded into xh), z-inverse

@@ -244,6 +244,11 @@ def run_form(hx, L, ctx, nx, ny, nz, f, fixed, precision, steps, warmup, args, t
         kern[nm] = {"avg_ms": ms, "count": cnt, "total_ms": ms * cnt}
     ctx.timing(False)
     kern["operator_back_to_back"] = {"avg_ms": op_ms, "count": 40, "total_ms": 40 * op_ms}
+    if precond == L.HX_PC_VCYCLE:  # the fine level's smoothing operators, back to back, once per application
+        napp = kern["ml_precond"]["count"]
+        for form, nm in ((3, "vc_pre_smoothing"), (4, "vc_post_smoothing")):
+            ms = ctx.time_smoothing(precision, form, 40)
+            kern[nm] = {"avg_ms": ms, "count": napp, "total_ms": napp * ms}
     log(f"  breakdown step: pcg={s.pcg_iterations} t={s.seconds:.3f}s; operator kernel {op_ms:.4f} ms")
     return wall / steps, recs, kern
 
@@ -330,17 +335,22 @@ def measure_traffic(args, form, tune=None):
         fp32 = FORMS[form][0] == "HX_FP32_IR"
         variant = tune["variant32"] if fp32 else tune["variant64"]
         tname = "float" if fp32 else "double"
-        # the explicit-preconditioner forms always run the parity contraction (its z + beta p form,
-        # template argument FUSE = 2; the V-cycle's smoothing forms 3 and 4 run as often)
+        # the explicit-preconditioner forms always run the parity contraction: the multilevel form's
+        # dominant kernel is its z + beta p operator (template argument FUSE = 2), the V-cycle's the
+        # fine level's post-smoothing operator (FUSE = 4, the largest grid)
         explicit = form.endswith(EXPLICIT_PC)
+        fuse = "4" if form.endswith("_vc") else "2"
         key = f"k_matvec_sf<{tname}" if (variant == 1 or explicit) else f"k_matvec_zmarch<{tname}"
         cand = {}
         for r in rows:
             if key in kname(r):
                 m = re.search(r"k_matvec_sf<\w+, \d+, \d+, (\d), \d>", kname(r))
-                if explicit and (m is None or m.group(1) != "2"):
+                if explicit and (m is None or m.group(1) != fuse):
                     continue
                 cand.setdefault((kname(r), r.get("Grid_Size") or r.get("grid_size")), []).append(val(r))
+        if explicit and cand:  # the fine level: the largest grid among the kept launches
+            top = max(int(float(g or 0)) for _, g in cand)
+            cand = {k: v for k, v in cand.items() if int(float(k[1] or 0)) == top}
         if not copy or not cand:
             return {"error": f"{counter}: kernels not found in the counter output ({len(rows)} rows)"}, tune
         (name, _), vals = max(cand.items(), key=lambda kv: len(kv[1]))
@@ -469,11 +479,17 @@ def main():
     fp64_main = FORMS[main_form][0] == "HX_FP64"
     upd_bytes = 56.0 * ndof if fp64_main else 28.0 * ndof
     pup_bytes = 32.0 * ndof if fp64_main else 16.0 * ndof
+    # V-cycle: the fine level's smoothing operators (r, S and the moduli in; the post-smoothing form
+    # also reads the coarse correction on the first coarsening's nodes; one output vector)
+    word = 8.0 if fp64_main else 4.0
+    ndof1 = 3 * (nx // 2 + 1) * (ny // 2 + 1) * (nz // 2 + 1)
+    pre_bytes = 3 * word * ndof + word * ne
+    post_bytes = 3 * word * ndof + word * ne + word * ndof1
     cand = []
     for nm, by in (("matvec_fp64", mv_bytes64), ("matvec_fp32", mv_bytes32), ("pcg_update", upd_bytes),
-                   ("pcg_pupdate", pup_bytes)):
-        k = r["kern"][nm]
-        if k["count"]:
+                   ("pcg_pupdate", pup_bytes), ("vc_pre_smoothing", pre_bytes), ("vc_post_smoothing", post_bytes)):
+        k = r["kern"].get(nm)
+        if k and k["count"]:
             cand.append((k["total_ms"], nm, by, k["avg_ms"]))
     cand.sort(reverse=True)
     roof = None
@@ -483,7 +499,8 @@ def main():
             avg = r["kern"]["operator_back_to_back"]["avg_ms"]
         ach = by / (avg * 1e-3) / 1e9
         tr = None
-        if traffic and "bytes_per_launch" in traffic and nm == ("matvec_fp64" if fp64_main else "matvec_fp32"):
+        measured = "vc_post_smoothing" if main_form.endswith("_vc") else ("matvec_fp64" if fp64_main else "matvec_fp32")
+        if traffic and "bytes_per_launch" in traffic and nm == measured:
             tr = round(traffic["bytes_per_launch"])
         roof = {"bound": "hbm", "kernel": nm, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": tr, "bytes_per_launch": by,

@@ -238,6 +238,13 @@ int hx_time_operator(hx_ctx* ctx, int precision, int nrhs, int reps, double* avg
  * HX_PC_VCYCLE (fused p = z + beta p_old from an explicit z, parity
  * contraction).  No reference counterpart. */
 int hx_time_operator_pc(hx_ctx* ctx, int precision, int precond, int nrhs, int reps, double* avg_ms);
+/* The fine level's V-cycle smoothing operator (HX_PC_VCYCLE): form 3 =
+ * pre-smoothing (x = S r, residual r - K x), 4 = post-smoothing with the
+ * coarse correction's prolongation fused; `reps` back-to-back launches at the
+ * current launch shape and moduli, rotating over vector sets larger together
+ * than the last-level cache.  The grid needs one 2:1 coarsening.  No reference
+ * counterpart. */
+int hx_time_smoothing(hx_ctx* ctx, int precision, int form, int reps, double* avg_ms);
 /* Apply a tuning result (e.g. one cached from an earlier hx_autotune) without
  * timing anything: its launch shapes and contractions.  No reference
  * counterpart. */

@@ -137,3 +137,17 @@ def test_vc_rejects_block_solves(ox, hx, ctx):
     rc = ctx.L.hx_block_pcg_solve(ctx.h, 2, hx._lib.dptr(Ee), hx._lib.dptr(F), hx._lib.dptr(U),
                                   hx._lib.C.byref(o), S)
     assert rc == hx._lib.HX_ERR_BAD_ARG
+
+
+def test_vc_smoothing_timing_hook(ox, hx, ctx):
+    """Back-to-back timing of the fine level's smoothing operators (both forms, both precisions);
+    a form other than 3 or 4 and a grid without a coarsening are rejected."""
+    _setup(ox, ctx, (32, 16, 16))
+    for precision in (0, 1):
+        for form in (3, 4):
+            assert ctx.time_smoothing(precision, form, 4) > 0.0
+    with pytest.raises(hx.HexatError):
+        ctx.time_smoothing(1, 2, 4)
+    _setup(ox, ctx, (7, 4, 4))
+    with pytest.raises(hx.HexatError):
+        ctx.time_smoothing(1, 4, 4)
