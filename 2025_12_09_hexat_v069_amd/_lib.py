@@ -93,7 +93,9 @@ class TuneResult(C.Structure):
     _fields_ = [("tile", C.c_int), ("kchunk", C.c_int), ("variant64", C.c_int), ("variant32", C.c_int),
                 ("ms64", C.c_double), ("ms32", C.c_double), ("ms32_parity", C.c_double), ("ms32_dense", C.c_double),
                 ("ms32_mfma", C.c_double), ("tile32", C.c_int), ("kchunk32", C.c_int), ("pdepth", C.c_int),
-                ("pdepth32", C.c_int)]
+                ("pdepth32", C.c_int), ("tile_pre", C.c_int), ("kchunk_pre", C.c_int), ("pdepth_pre", C.c_int),
+                ("tile_post", C.c_int), ("kchunk_post", C.c_int), ("pdepth_post", C.c_int), ("ms_pre", C.c_double),
+                ("ms_post", C.c_double)]
 
 
 class LearnStats(C.Structure):

@@ -97,6 +97,9 @@ struct hx_ctx {
     int mv_variant = HX_MV_PARITY;
     int mv32_variant = HX_MV_PARITY;
     MvShape shape64, shape32;  // launch shapes of the FP64 / FP32 operator
+    // launch shapes of the V-cycle's pre- / post-smoothing operators (kchunk 0:
+    // unset, the operator's shape of the solve's precision)
+    MvShape shape_pre{0, 0, 1}, shape_post{0, 0, 1};
 
     DevBuf<uint8_t> fixed;
     DevBuf<double> E, f, x, r, p, q, minv, Kbin_d, p2;
@@ -256,9 +259,10 @@ int run_batch(hx_ctx* c, const std::string& key, F&& launch_batch) {
 std::string batch_key(hx_ctx* c, const char* kind, int nr, int batch, int variant, double shift, const void* a,
                       const void* b, const void* d, const void* e, const void* f) {
     char buf[320];
-    std::snprintf(buf, sizeof buf, "%s|%d|%d|%d|%a|%p|%p|%p|%p|%p|%d|%d|%d|%d|%d|%d", kind, nr, batch, variant,
-                  shift, a, b, d, e, f, c->shape64.tile, c->shape64.kchunk, c->shape64.pdepth, c->shape32.tile,
-                  c->shape32.kchunk, c->shape32.pdepth);
+    std::snprintf(buf, sizeof buf, "%s|%d|%d|%d|%a|%p|%p|%p|%p|%p|%d|%d|%d|%d|%d|%d|%d|%d|%d|%d|%d|%d", kind, nr,
+                  batch, variant, shift, a, b, d, e, f, c->shape64.tile, c->shape64.kchunk, c->shape64.pdepth,
+                  c->shape32.tile, c->shape32.kchunk, c->shape32.pdepth, c->shape_pre.tile, c->shape_pre.kchunk,
+                  c->shape_pre.pdepth, c->shape_post.tile, c->shape_post.kchunk, c->shape_post.pdepth);
     return buf;
 }
 
@@ -484,12 +488,18 @@ int ml_setup(hx_ctx* c, bool f32, bool vcycle = false) {
     return HX_OK;
 }
 
+// launch shape of a V-cycle smoothing form (post: form 4, else form 3)
+MvShape vc_shape(const hx_ctx* c, bool f32, bool post) {
+    const MvShape& s = post ? c->shape_post : c->shape_pre;
+    return s.kchunk > 0 ? s : (f32 ? c->shape32 : c->shape64);
+}
+
 // z = M^-1 r of the explicit preconditioner into ml_z / ml_z32 (fin_mode as
 // launch_ml_apply's)
 int pc_apply64(hx_ctx* c, bool vcycle, const double* r, int fin_mode) {
     if (vcycle) {
-        if (launch_vc_apply<double>(c->ml64, c->kp64, c->shape64, r, c->ml_z.p, c->st, c->partials.p, kOpPartials,
-                                    fin_mode, c->stream) != HX_OK)
+        if (launch_vc_apply<double>(c->ml64, c->kp64, vc_shape(c, false, false), vc_shape(c, false, true), r,
+                                    c->ml_z.p, c->st, c->partials.p, kOpPartials, fin_mode, c->stream) != HX_OK)
             return fail(c, HX_ERR_BAD_ARG, "grid too large for the V-cycle's partial sums");
     } else {
         launch_ml_apply<double>(c->ml64, r, c->minv.p, c->ml_z.p, c->st, c->partials.p, fin_mode, c->stream);
@@ -498,8 +508,8 @@ int pc_apply64(hx_ctx* c, bool vcycle, const double* r, int fin_mode) {
 }
 int pc_apply32(hx_ctx* c, bool vcycle, const float* r, int fin_mode) {
     if (vcycle) {
-        if (launch_vc_apply<float>(c->ml32, c->kp32, c->shape32, r, c->ml_z32.p, c->st, c->partials.p, kOpPartials,
-                                   fin_mode, c->stream) != HX_OK)
+        if (launch_vc_apply<float>(c->ml32, c->kp32, vc_shape(c, true, false), vc_shape(c, true, true), r,
+                                   c->ml_z32.p, c->st, c->partials.p, kOpPartials, fin_mode, c->stream) != HX_OK)
             return fail(c, HX_ERR_BAD_ARG, "grid too large for the V-cycle's partial sums");
     } else {
         launch_ml_apply<float>(c->ml32, r, c->minv32.p, c->ml_z32.p, c->st, c->partials.p, fin_mode, c->stream);
@@ -1687,6 +1697,11 @@ int hx_test_unet_backward(hx_ctx* c, const float* gy, float* grad) {
 // moduli and a non-smooth vector, and keep the fastest per precision:
 // FP64 parity {16x16, 16x32 tiles} x {8, 16, 32 z planes} x {1, 2 planes in
 // flight} and dense; FP32 the same parity shapes, dense and MFMA.
+extern "C++" {
+template <typename T>
+int time_smoothing_t(hx_ctx* c, int form, int reps, const MvShape* shapes, int nshapes, float* ms);
+}
+
 int hx_autotune(hx_ctx* c, int nrhs, hx_tune_result* out) {
     if (!c || nrhs < 1 || nrhs > kMaxRhs) return HX_ERR_BAD_ARG;
     if (int rc = ensure_grid(c)) return rc;
@@ -1774,6 +1789,24 @@ int hx_autotune(hx_ctx* c, int nrhs, hx_tune_result* out) {
     c->free_events.push_back(b);
     c->mv_variant = bv64;
     c->mv32_variant = bv32;
+    // the V-cycle's fine-level smoothing operators (FP32, the production form)
+    double best_pre = 0.0, best_post = 0.0;
+    c->shape_pre = c->shape_post = MvShape{0, 0, 1};
+    if (ml_levels_for(c->g, 1) >= 1) {
+        std::vector<MvShape> cands;
+        for (int t : tiles)
+            for (int k : {8, 16})
+                for (int d : depths) cands.push_back(MvShape{t, k, d});
+        std::vector<float> tp(cands.size()), tq(cands.size());
+        const int reps = 5;
+        int rs = time_smoothing_t<float>(c, 3, reps, cands.data(), (int)cands.size(), tp.data());
+        if (rs == HX_OK) rs = time_smoothing_t<float>(c, 4, reps, cands.data(), (int)cands.size(), tq.data());
+        for (size_t i = 0; rs == HX_OK && i < cands.size(); ++i) {
+            const double mp = tp[i] / reps, mq = tq[i] / reps;
+            if (mp > 0.0 && (best_pre == 0.0 || mp < best_pre)) best_pre = mp, c->shape_pre = cands[i];
+            if (mq > 0.0 && (best_post == 0.0 || mq < best_post)) best_post = mq, c->shape_post = cands[i];
+        }
+    }
     c->timing = was_timing;
     clear_graphs(c);
     launch_pcg_reset(c->st, 0.0, 0.0, 0, 1, c->stream, nrhs);
@@ -1793,6 +1826,14 @@ int hx_autotune(hx_ctx* c, int nrhs, hx_tune_result* out) {
         out->kchunk32 = c->shape32.kchunk;
         out->pdepth = c->shape64.pdepth;
         out->pdepth32 = c->shape32.pdepth;
+        out->tile_pre = c->shape_pre.tile;
+        out->kchunk_pre = c->shape_pre.kchunk;
+        out->pdepth_pre = c->shape_pre.pdepth;
+        out->tile_post = c->shape_post.tile;
+        out->kchunk_post = c->shape_post.kchunk;
+        out->pdepth_post = c->shape_post.pdepth;
+        out->ms_pre = best_pre;
+        out->ms_post = best_post;
     }
     return HX_OK;
 }
@@ -1848,11 +1889,12 @@ int time_operator_t(hx_ctx* c, int nrhs, int reps, float* ms, bool zform) {
 }
 
 // back-to-back launches of the fine level's V-cycle smoothing operator (form
-// 3 or 4) on kSets vector sets at the current launch shape; form 4 reads a
-// coarse correction on the first coarsening's node grid and writes its r'z
-// partials, as in the solve
+// 3 or 4) on kSets vector sets, at each of nshapes launch shapes (ms[i]: the
+// total of `reps` launches at shapes[i], 0 for a shape whose partial sums
+// would not fit); form 4 reads a coarse correction on the first coarsening's
+// node grid and writes its r'z partials, as in the solve
 template <typename T>
-int time_smoothing_t(hx_ctx* c, int form, int reps, float* ms) {
+int time_smoothing_t(hx_ctx* c, int form, int reps, const MvShape* shapes, int nshapes, float* ms) {
     constexpr int kSets = 4;
     const long n = ndof(c);
     const GridArgs gc = ml_coarse_grid(c->g);
@@ -1872,30 +1914,34 @@ int time_smoothing_t(hx_ctx* c, int form, int reps, float* ms) {
             launch_fill32(3 * gc.nn, 0.125f, e[k].p, c->stream);
         }
     }
-    auto go = [&](int k) {
-        MvArgs<T> a{nullptr, y[k].p, nullptr, T(0), MV_PLAIN, nullptr, form == 4 ? c->partials.p + 2 * kVecBlocks
-                                                                                 : nullptr,
-                    n, r[k].p, s[k].p, nullptr};
-        a.form = form;
-        a.ec = e[k].p;
-        a.cNX = gc.NX;
-        a.cNY = gc.NY;
-        a.cNZ = gc.NZ;
-        if constexpr (sizeof(T) == 8) {
-            a.E = c->E.p;
-            launch_matvec_parity<double>(c->g, c->kp64, a, c->shape64, c->stream, 1);
-        } else {
-            a.E = c->E32.p;
-            launch_matvec_parity<float>(c->g, c->kp32, a, c->shape32, c->stream, 1);
-        }
-    };
     hipEvent_t a = get_event(c), b = get_event(c);
-    for (int k = 0; k < kSets; ++k) go(k);
-    (void)hipEventRecord(a, c->stream);
-    for (int i = 0; i < reps; ++i) go(i % kSets);
-    (void)hipEventRecord(b, c->stream);
-    HXC(hipEventSynchronize(b));
-    (void)hipEventElapsedTime(ms, a, b);
+    for (int i = 0; i < nshapes; ++i) {
+        const MvShape sh = shapes[i];
+        ms[i] = 0.f;
+        if (2L * kVecBlocks + matvec_parity_blocks(c->g, sh.tile, sh.kchunk) > kOpPartials) continue;
+        auto go = [&](int k) {
+            MvArgs<T> a{nullptr, y[k].p, nullptr, T(0), MV_PLAIN, nullptr,
+                        form == 4 ? c->partials.p + 2 * kVecBlocks : nullptr, n, r[k].p, s[k].p, nullptr};
+            a.form = form;
+            a.ec = e[k].p;
+            a.cNX = gc.NX;
+            a.cNY = gc.NY;
+            a.cNZ = gc.NZ;
+            if constexpr (sizeof(T) == 8) {
+                a.E = c->E.p;
+                launch_matvec_parity<double>(c->g, c->kp64, a, sh, c->stream, 1);
+            } else {
+                a.E = c->E32.p;
+                launch_matvec_parity<float>(c->g, c->kp32, a, sh, c->stream, 1);
+            }
+        };
+        for (int k = 0; k < kSets; ++k) go(k);
+        (void)hipEventRecord(a, c->stream);
+        for (int it = 0; it < reps; ++it) go(it % kSets);
+        (void)hipEventRecord(b, c->stream);
+        HXC(hipEventSynchronize(b));
+        (void)hipEventElapsedTime(&ms[i], a, b);
+    }
     c->free_events.push_back(a);
     c->free_events.push_back(b);
     return HX_OK;
@@ -1907,15 +1953,15 @@ int hx_time_smoothing(hx_ctx* c, int precision, int form, int reps, double* avg_
     if (precision != HX_FP64 && precision != HX_FP32_IR) return HX_ERR_BAD_ARG;
     if (int rc = ensure_grid(c)) return rc;
     if (ml_levels_for(c->g, 1) < 1) return fail(c, HX_ERR_BAD_ARG, "the grid has no 2:1 coarsening");
-    if (2L * kVecBlocks + matvec_parity_blocks(c->g, c->shape32.tile, c->shape32.kchunk) > kOpPartials ||
-        2L * kVecBlocks + matvec_parity_blocks(c->g, c->shape64.tile, c->shape64.kchunk) > kOpPartials)
+    const MvShape sh = vc_shape(c, precision == HX_FP32_IR, form == 4);
+    if (2L * kVecBlocks + matvec_parity_blocks(c->g, sh.tile, sh.kchunk) > kOpPartials)
         return fail(c, HX_ERR_BAD_ARG, "grid too large for the V-cycle's partial sums");
     if (int rc = ensure_fp32(c)) return rc;
     const bool was_timing = c->timing;
     c->timing = false;
     float ms = 0.f;
-    const int rc = precision == HX_FP64 ? time_smoothing_t<double>(c, form, reps, &ms)
-                                        : time_smoothing_t<float>(c, form, reps, &ms);
+    const int rc = precision == HX_FP64 ? time_smoothing_t<double>(c, form, reps, &sh, 1, &ms)
+                                        : time_smoothing_t<float>(c, form, reps, &sh, 1, &ms);
     c->timing = was_timing;
     if (rc) return rc;
     HXK();
@@ -1956,8 +2002,14 @@ int hx_tune_set(hx_ctx* c, const hx_tune_result* t) {
     if (!shape_ok(t->tile, t->kchunk, t->pdepth) || !shape_ok(t->tile32, t->kchunk32, t->pdepth32) ||
         !var_ok(t->variant64) || !var_ok(t->variant32) || t->variant32 == HX_MV_GATHER || t->variant64 == HX_MV_MFMA)
         return fail(c, HX_ERR_BAD_ARG, "invalid tuning result");
+    // smoothing shapes: kchunk 0 leaves them unset (the operator's shape)
+    if ((t->kchunk_pre != 0 && !shape_ok(t->tile_pre, t->kchunk_pre, t->pdepth_pre)) ||
+        (t->kchunk_post != 0 && !shape_ok(t->tile_post, t->kchunk_post, t->pdepth_post)))
+        return fail(c, HX_ERR_BAD_ARG, "invalid tuning result");
     c->shape64 = MvShape{t->tile, t->kchunk, t->pdepth};
     c->shape32 = MvShape{t->tile32, t->kchunk32, t->pdepth32};
+    c->shape_pre = t->kchunk_pre != 0 ? MvShape{t->tile_pre, t->kchunk_pre, t->pdepth_pre} : MvShape{0, 0, 1};
+    c->shape_post = t->kchunk_post != 0 ? MvShape{t->tile_post, t->kchunk_post, t->pdepth_post} : MvShape{0, 0, 1};
     c->mv_variant = t->variant64;
     c->mv32_variant = t->variant32;
     clear_graphs(c);

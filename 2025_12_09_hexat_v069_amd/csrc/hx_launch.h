@@ -196,13 +196,14 @@ void launch_vc_smoother(const GridArgs& gc, const double* Ec, double h, const do
                         int stride, const GridArgs& gfine, double omega, T* sinv, hipStream_t s);
 template <typename T>
 void launch_vc_scale_E(long ne, const double* Ec, double h, T* Eop, hipStream_t s);
-// z = V(r) with the parity operator K at shape sh on every smoothed level; the
-// r'z partials (one per level-0 operator block) go after the update's and the
-// finaliser runs as in launch_ml_apply.  HX_ERR_BAD_ARG when those partials
-// would pass max_partials.
+// z = V(r) with the parity operator K, its pre- and post-smoothing forms at
+// shapes sh_pre / sh_post (coarse levels: the same with shallower z chunks);
+// the r'z partials (one per level-0 post-smoothing block) go after the
+// update's and the finaliser runs as in launch_ml_apply.  HX_ERR_BAD_ARG when
+// those partials would pass max_partials.
 template <typename T>
-int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const MvShape& sh, const T* r, T* z,
-                    PcgState* st, double* partials, long max_partials, int fin_mode, hipStream_t s);
+int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const MvShape& sh_pre, const MvShape& sh_post,
+                    const T* r, T* z, PcgState* st, double* partials, long max_partials, int fin_mode, hipStream_t s);
 
 // ---- Helmholtz filter --------------------------------------------------------
 struct HelmArgs {

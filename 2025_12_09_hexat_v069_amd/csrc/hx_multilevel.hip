@@ -658,8 +658,9 @@ void launch_vc_scale_E(long ne, const double* Ec, double h, T* Eop, hipStream_t 
 }
 
 template <typename T>
-int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const MvShape& sh, const T* r, T* z,
-                    PcgState* st_, double* partials, long max_partials, int fin_mode, hipStream_t s) {
+int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const MvShape& sh_pre, const MvShape& sh_post,
+                    const T* r, T* z, PcgState* st_, double* partials, long max_partials, int fin_mode,
+                    hipStream_t s) {
     const int c = vc_levels(h);
     if (c == 0) {  // no coarse level: the additive form is plain Jacobi
         launch_ml_apply<T>(h, r, h.minv[0], z, st_, partials, fin_mode, s);
@@ -669,11 +670,12 @@ int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const MvShape&
     PcgState* const st = fin_mode == 0 ? st_ : nullptr;
     double* const upd_partials = partials;
     double* const rz_partials = partials + 2 * kVecBlocks;
-    if (2L * kVecBlocks + matvec_parity_blocks(h.g[0], sh.tile, sh.kchunk) > max_partials) return HX_ERR_BAD_ARG;
+    if (2L * kVecBlocks + matvec_parity_blocks(h.g[0], sh_post.tile, sh_post.kchunk) > max_partials)
+        return HX_ERR_BAD_ARG;
     auto level_r = [&](int l) { return l == 0 ? r : (const T*)h.r[l]; };
     // coarse levels: shallower z chunks until the launch has kVcMinBlocks
     // workgroups (or one plane each), so the small grids are not latency-bound
-    auto level_shape = [&](int l) {
+    auto level_shape = [&](int l, const MvShape& sh) {
         MvShape s2 = sh;
         if (l > 0)
             while (s2.kchunk > 1 && matvec_parity_blocks(h.g[l], s2.tile, s2.kchunk) < kVcMinBlocks) s2.kchunk /= 2;
@@ -694,7 +696,7 @@ int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const MvShape&
         MvArgs<T> a{nullptr, h.vres[l], h.Eop[l], T(0), MV_PLAIN, st, nullptr, 3 * h.g[l].nn, level_r(l), h.sinv[l],
                     nullptr};
         a.form = 3;
-        launch_matvec_parity<T>(h.g[l], K, a, level_shape(l), s, 1);
+        launch_matvec_parity<T>(h.g[l], K, a, level_shape(l, sh_pre), s, 1);
         if (l + 1 < c) restrict_to(l + 1, false);
     }
     const int s0 = ml_coarse_start(h);
@@ -715,7 +717,7 @@ int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const MvShape&
         a.cNX = h.g[l + 1].NX;
         a.cNY = h.g[l + 1].NY;
         a.cNZ = h.g[l + 1].NZ;
-        const int nb = launch_matvec_parity<T>(h.g[l], K, a, level_shape(l), s, 1);
+        const int nb = launch_matvec_parity<T>(h.g[l], K, a, level_shape(l, sh_post), s, 1);
         if (l == 0) nb0 = nb;
     }
     hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(NT), 0, s, st_, rz_partials, (unsigned)nb0, upd_partials,
@@ -767,9 +769,10 @@ template void launch_vc_smoother<float>(const GridArgs&, const double*, double, 
                                         const GridArgs&, double, float*, hipStream_t);
 template void launch_vc_scale_E<double>(long, const double*, double, double*, hipStream_t);
 template void launch_vc_scale_E<float>(long, const double*, double, float*, hipStream_t);
-template int launch_vc_apply<double>(const MlHierarchy<double>&, const ParityK<double>&, const MvShape&, const double*,
-                                     double*, PcgState*, double*, long, int, hipStream_t);
-template int launch_vc_apply<float>(const MlHierarchy<float>&, const ParityK<float>&, const MvShape&, const float*,
-                                    float*, PcgState*, double*, long, int, hipStream_t);
+template int launch_vc_apply<double>(const MlHierarchy<double>&, const ParityK<double>&, const MvShape&,
+                                     const MvShape&, const double*, double*, PcgState*, double*, long, int,
+                                     hipStream_t);
+template int launch_vc_apply<float>(const MlHierarchy<float>&, const ParityK<float>&, const MvShape&, const MvShape&,
+                                    const float*, float*, PcgState*, double*, long, int, hipStream_t);
 
 }  // namespace hx

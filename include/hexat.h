@@ -221,10 +221,16 @@ typedef struct {
     double ms32_parity, ms32_dense, ms32_mfma; /* FP32 contraction candidates */
     int tile32, kchunk32; /* launch shape of the FP32 operator (tile, kchunk: FP64) */
     int pdepth, pdepth32; /* node planes in flight per workgroup, FP64 / FP32 operator */
+    /* launch shapes of the V-cycle's fine-level pre- and post-smoothing
+     * operators (HX_PC_VCYCLE; kchunk 0 = unset: the operator's shape) and
+     * their best FP32 times; 0 when the grid has no 2:1 coarsening */
+    int tile_pre, kchunk_pre, pdepth_pre, tile_post, kchunk_post, pdepth_post;
+    double ms_pre, ms_post;
 } hx_tune_result;
 /* Time the operator's launch shapes (tile, z-chunk) and element contractions on
  * the current grid for nrhs columns and keep the fastest (HX_MV_AUTO then uses
- * them).  No reference counterpart. */
+ * them); likewise the V-cycle's smoothing operators' shapes.  No reference
+ * counterpart. */
 int hx_autotune(hx_ctx* ctx, int nrhs, hx_tune_result* result);
 /* Average duration (ms) of one application of the PCG operator kernel
  * (fused p-update, FP64 or FP32 contraction per `precision`) at the current

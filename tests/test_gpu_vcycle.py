@@ -151,3 +151,26 @@ def test_vc_smoothing_timing_hook(ox, hx, ctx):
     _setup(ox, ctx, (7, 4, 4))
     with pytest.raises(hx.HexatError):
         ctx.time_smoothing(1, 4, 4)
+
+
+@pytest.mark.parametrize("pre,post", [((2, 16, 1), (0, 16, 2)), ((0, 8, 2), (1, 8, 1)), ((1, 16, 2), (2, 4, 1))])
+def test_vc_smoothing_shapes_match_oracle(ox, hx, ctx, pre, post):
+    """Tuned launch shapes of the smoothing forms (restored by hx_tune_set) leave the application
+    at the oracle's value; the autotuner reports shapes for both forms."""
+    g, fixed, _ = _setup(ox, ctx, (64, 32, 32))
+    t = {"tile": 2, "kchunk": 8, "pdepth": 1, "tile32": 2, "kchunk32": 8, "pdepth32": 1, "variant64": 1,
+         "variant32": 1, "tile_pre": pre[0], "kchunk_pre": pre[1], "pdepth_pre": pre[2], "tile_post": post[0],
+         "kchunk_post": post[1], "pdepth_post": post[2]}
+    ctx.tune_set(t)
+    rng = np.random.default_rng(5)
+    Ee = 1e-9 + rng.uniform(0, 1, g.ne) ** 3
+    Ke = ox.ke_hex(1, 1, 1, 0.3)
+    r = np.where(fixed == 0, rng.standard_normal(g.ndof), 0.0)
+    for precision, tol in ((0, 1e-10), (1, 1e-5)):
+        z, _, _ = ctx.test_vc_apply(Ee, r, precision)
+        zo = ox.vc_apply(g, Ke, Ee, fixed, r, elem_bytes=8 if precision == 0 else 4)
+        assert np.allclose(z, zo, rtol=tol, atol=tol * np.abs(zo).max())
+    tuned = ctx.autotune(1)
+    assert tuned["kchunk_pre"] > 0 and tuned["kchunk_post"] > 0 and tuned["ms_post"] > 0.0
+    with pytest.raises(hx.HexatError):
+        ctx.tune_set(dict(t, kchunk_post=4, pdepth_post=3))
