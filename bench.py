@@ -510,7 +510,7 @@ def main():
         if main_form.endswith("_vc"):
             sl = smoothed_levels(nx, ny, nz, 8 if fp64_main else 4)
             by = vc_apply_bytes(nx, ny, nz, 8 if fp64_main else 4, sl)
-            kn = ("k_matvec_sf forms 3 and 4, k_ml_restrict_tiled, k_ml_coarse, k_vc_prolong, k_fin_rz "
+            kn = ("k_matvec_sf forms 3 and 4, k_ml_restrict_tiled, k_ml_coarse, k_fin_rz "
                   f"({sl} smoothed levels, one application)")
         else:
             by = ml_apply_bytes(nx, ny, nz, 8 if fp64_main else 4)
