@@ -1918,7 +1918,8 @@ int time_smoothing_t(hx_ctx* c, int form, int reps, const MvShape* shapes, int n
     for (int i = 0; i < nshapes; ++i) {
         const MvShape sh = shapes[i];
         ms[i] = 0.f;
-        if (2L * kVecBlocks + matvec_parity_blocks(c->g, sh.tile, sh.kchunk) > kOpPartials) continue;
+        const int kc = form == 4 ? smoothing_post_kchunk(sh.tile, sh.kchunk, (int)sizeof(T)) : sh.kchunk;
+        if (2L * kVecBlocks + matvec_parity_blocks(c->g, sh.tile, kc) > kOpPartials) continue;
         auto go = [&](int k) {
             MvArgs<T> a{nullptr, y[k].p, nullptr, T(0), MV_PLAIN, nullptr,
                         form == 4 ? c->partials.p + 2 * kVecBlocks : nullptr, n, r[k].p, s[k].p, nullptr};
@@ -1954,7 +1955,8 @@ int hx_time_smoothing(hx_ctx* c, int precision, int form, int reps, double* avg_
     if (int rc = ensure_grid(c)) return rc;
     if (ml_levels_for(c->g, 1) < 1) return fail(c, HX_ERR_BAD_ARG, "the grid has no 2:1 coarsening");
     const MvShape sh = vc_shape(c, precision == HX_FP32_IR, form == 4);
-    if (2L * kVecBlocks + matvec_parity_blocks(c->g, sh.tile, sh.kchunk) > kOpPartials)
+    const int kc = form == 4 ? smoothing_post_kchunk(sh.tile, sh.kchunk, precision == HX_FP64 ? 8 : 4) : sh.kchunk;
+    if (2L * kVecBlocks + matvec_parity_blocks(c->g, sh.tile, kc) > kOpPartials)
         return fail(c, HX_ERR_BAD_ARG, "grid too large for the V-cycle's partial sums");
     if (int rc = ensure_fp32(c)) return rc;
     const bool was_timing = c->timing;

@@ -297,12 +297,16 @@ __device__ __forceinline__ double lane_from_left(double v) {
 // 4 (post-smoothing) x = S r + mask(P e), the coarse correction e (a.ec, node
 //   grid cNX x cNY x cNZ) interpolated trilinearly where S != 0, then
 //   y = x + S (r - K x), with the per-block partial sums of r'y over the owned
-//   nodes when a.partials is set.  The tile's footprint on two coarse planes
-//   sits in LDS; a new coarse plane is loaded once per two fine planes.
+//   nodes when a.partials is set.  At its start a workgroup interpolates e in
+//   z onto every fine plane of its chunk over the tile's coarse footprint, in
+//   dynamic LDS ((kchunk + 2) planes, smoothing_post_kchunk); the march then
+//   reads the four xy corners of each staged node.
 // Forms 3 and 4 return at once when a.st is set and its solve is done.
+// (the FP32 post-smoothing form with one plane in flight is held to 128 VGPRs:
+// four waves per SIMD instead of three, without spills)
 template <typename T, int TX, int TY, int FUSE, int PD>
-__global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K, MvArgs<T> a, int kchunk, int ntx,
-                                                       int nty, int ntiles) {
+__global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ? 4 : 1) void k_matvec_sf(
+    GridArgs g, ParityK<T> K, MvArgs<T> a, int kchunk, int ntx, int nty, int ntiles) {
     constexpr int NT = TX * TY;
     constexpr int PX = TX + 1, PY = TY + 1;
     constexpr int NEXTRA = 3 * (PY + TX);  // plane values beyond the thread grid: column TX and row TY
@@ -316,10 +320,10 @@ __global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
     constexpr bool SMOOTH = FUSE >= 3;
     constexpr bool PRO = FUSE == 4;  // fused prolongation of a coarse correction
     // coarse footprint of the tile's node columns (i0-1 .. i0-1+TX, j0-1 .. j0-1+TY)
-    constexpr int CPX = TX / 2 + 2, CPY = TY / 2 + 2, CVALS = 3 * CPX * CPY, CPER = (CVALS + NT - 1) / NT;
-    // slot (coarse plane & 1), and slot 2 = the average of the two held planes
-    // (what an odd fine plane needs): [(c CPY + ly) CPX + lx]
-    __shared__ T cpl[PRO ? 3 : 1][PRO ? CVALS : 1];
+    constexpr int CPX = TX / 2 + 2, CPY = TY / 2 + 2, CVALS = 3 * CPX * CPY;
+    // form 4: the z-interpolated coarse correction, fine plane f of the chunk at [f CVALS + (c CPY + ly) CPX + lx]
+    extern __shared__ __align__(16) unsigned char sf_dyn[];
+    [[maybe_unused]] T* const czl = reinterpret_cast<T*>(sf_dyn);
     __shared__ T so[POUT ? 2 : 1][OVALS];  // results of the owned nodes, node-major, for row-contiguous stores
     __shared__ double red[NT / 64];
 
@@ -362,106 +366,58 @@ __global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
         const int off_x =
             3 * (min(max(j0 - 1 + xty, 0), g.NY - 1) * g.NX + min(max(i0 - 1 + xtx, 0), g.NX - 1)) + xc;
 
-        // form 4: coarse planes of the correction.  held[s] is the coarse
-        // plane in slot s (uniform); loads are issued after a layer's first
-        // barrier and parked in LDS before its second.
+        // form 4: the tile's coarse footprint starts at coarse node (cbx, cby);
+        // the chunk stages fine planes fbase .. fbase + nf - 1 (clamped to the grid)
         [[maybe_unused]] const int cbx = (i0 - 1) < 0 ? -1 : (i0 - 1) >> 1;
         [[maybe_unused]] const int cby = (j0 - 1) < 0 ? -1 : (j0 - 1) >> 1;
-        [[maybe_unused]] int held0 = -1, held1 = -1;      // coarse plane in slot 0 / 1
-        [[maybe_unused]] bool pend0 = false, pend1 = false;  // slot to park before the next barrier
-        [[maybe_unused]] T cv00 = T(0), cv01 = T(0), cv10 = T(0), cv11 = T(0);  // its two values per thread
-        static_assert(!PRO || CPER == 2, "two coarse values per thread and plane");
-        // value m of this thread's share of coarse plane K (clamped addresses)
-        auto coarse_load = [&](int K, int m) __attribute__((always_inline)) -> T {
-            const int idx = min((int)threadIdx.x + m * NT, CVALS - 1);
-            const int ly = idx / (3 * CPX), rem = idx - ly * 3 * CPX, lx = rem / 3, c = rem - 3 * lx;
-            const int I = min(max(cbx + lx, 0), a.cNX - 1), J = min(max(cby + ly, 0), a.cNY - 1);
-            return a.ec[3 * ((long)K * a.cNY * a.cNX + (long)J * a.cNX + I) + c];
-        };
-        auto coarse_store = [&](int slot, int m, T v) __attribute__((always_inline)) {
-            const int idx = (int)threadIdx.x + m * NT;
-            if (idx < CVALS) {
-                const int ly = idx / (3 * CPX), rem = idx - ly * 3 * CPX, lx = rem / 3, c = rem - 3 * lx;
-                cpl[PRO ? slot : 0][(c * CPY + ly) * CPX + lx] = v;
-            }
-        };
-        // request coarse plane K into slot K & 1 unless it is there (value
-        // selects, not a branch per slot: the state stays in registers)
-        auto coarse_want = [&](int K) __attribute__((always_inline)) {
-            const bool w0 = (K & 1) == 0 && held0 != K, w1 = (K & 1) != 0 && held1 != K;
-            T v0 = T(0), v1 = T(0);
-            if (w0 || w1) {  // uniform
-                v0 = coarse_load(K, 0);
-                v1 = coarse_load(K, 1);
-            }
-            cv00 = w0 ? v0 : cv00;
-            cv01 = w0 ? v1 : cv01;
-            cv10 = w1 ? v0 : cv10;
-            cv11 = w1 ? v1 : cv11;
-            held0 = w0 ? K : held0;
-            held1 = w1 ? K : held1;
-            pend0 = pend0 || w0;
-            pend1 = pend1 || w1;
-        };
-        // the coarse planes fine plane mf needs: A = mf >> 1, and A + 1 for odd mf
-        auto coarse_need = [&](int mf) __attribute__((always_inline)) {
-            if constexpr (PRO) {
-                const int mm = min(max(mf, 0), g.NZ - 1), A = mm >> 1;
-                coarse_want(A);
-                if (mm & 1) coarse_want(A + 1);
-            }
-        };
-        auto coarse_flush = [&]() __attribute__((always_inline)) {
-            if constexpr (PRO) {
-                if (pend0) {
-                    coarse_store(0, 0, cv00);
-                    coarse_store(0, 1, cv01);
-                }
-                if (pend1) {
-                    coarse_store(1, 0, cv10);
-                    coarse_store(1, 1, cv11);
-                }
-                // adjacent planes held: their average, from the entries this thread parked itself
-                if ((pend0 || pend1) && (held0 - held1 == 1 || held1 - held0 == 1)) {
+        [[maybe_unused]] const int fbase = max(k0 - 1, 0), nf = min(k1, g.NZ - 1) - fbase + 1;
+        // fill: fine plane 2K holds coarse plane K, 2K+1 the mean of K and K+1;
+        // batches of kFillBatch values per thread, every load of a batch issued
+        // before any is waited for (clamped addresses, no stores out of range)
+        auto coarse_fill = [&]() __attribute__((always_inline)) {
+            constexpr int kFillBatch = (18 * CVALS + NT - 1) / NT;  // a 16-plane chunk in one batch
+            const int total = nf * CVALS;
+            for (int base = 0; base < total; base += kFillBatch * NT) {
+                T v0[kFillBatch], v1[kFillBatch];
 #pragma unroll
-                    for (int m = 0; m < CPER; ++m) {
-                        const int idx = (int)threadIdx.x + m * NT;
-                        if (idx < CVALS) {
-                            const int ly = idx / (3 * CPX), rem = idx - ly * 3 * CPX, lx = rem / 3, c = rem - 3 * lx;
-                            const int o = (c * CPY + ly) * CPX + lx;
-                            cpl[PRO ? 2 : 0][o] = T(0.5) * (cpl[0][o] + cpl[PRO ? 1 : 0][o]);
-                        }
+                for (int b = 0; b < kFillBatch; ++b) {
+                    const int idx = min(base + b * NT + (int)threadIdx.x, total - 1);
+                    const int f = idx / CVALS, ii = idx - f * CVALS;
+                    const int ly = ii / (3 * CPX), rem = ii - ly * 3 * CPX, lx = rem / 3, c = rem - 3 * lx;
+                    const int I = min(max(cbx + lx, 0), a.cNX - 1), J = min(max(cby + ly, 0), a.cNY - 1);
+                    const int K = (fbase + f) >> 1, K1 = min(K + 1, a.cNZ - 1);
+                    const long col = 3 * ((long)J * a.cNX + I) + c, pl = 3L * a.cNY * a.cNX;
+                    v0[b] = a.ec[K * pl + col];
+                    v1[b] = a.ec[K1 * pl + col];
+                }
+#pragma unroll
+                for (int b = 0; b < kFillBatch; ++b) {
+                    const int idx = base + b * NT + (int)threadIdx.x;
+                    if (idx < total) {
+                        const int f = idx / CVALS, ii = idx - f * CVALS;
+                        const int ly = ii / (3 * CPX), rem = ii - ly * 3 * CPX, lx = rem / 3, c = rem - 3 * lx;
+                        czl[f * CVALS + (c * CPY + ly) * CPX + lx] =
+                            ((fbase + f) & 1) ? T(0.5) * (v0[b] + v1[b]) : v0[b];
                     }
                 }
-                pend0 = pend1 = false;
             }
         };
         // trilinear interpolation of the correction: per thread, the LDS offset of
         // the lower xy corner and the four xy weights of its node and of its extra
-        // value's node (fixed over the march); per fine plane, slot (plane >> 1) & 1
-        // for an even plane, the averaged slot 2 for an odd one
-        struct Corners {
-            int o;
-            T w00, w01, w10, w11;
-        };
+        // value's node (fixed over the march), applied to the staged plane's
+        // z-interpolated footprint
+        // corner offset and the node's x / y parities packed in one int: o << 2 | py << 1 | px
         auto corners = [&](int fi, int fj) __attribute__((always_inline)) {
-            Corners k;
             const int lx = (fi < 0 ? -1 : fi >> 1) - cbx, ly = (fj < 0 ? -1 : fj >> 1) - cby;
-            const T wx1 = (fi & 1) ? T(0.5) : T(0), wx0 = T(1) - wx1;
-            const T wy1 = (fj & 1) ? T(0.5) : T(0), wy0 = T(1) - wy1;
-            k.o = ly * CPX + lx;
-            k.w00 = wy0 * wx0;
-            k.w01 = wy0 * wx1;
-            k.w10 = wy1 * wx0;
-            k.w11 = wy1 * wx1;
-            return k;
+            return ((ly * CPX + lx) << 2) | ((fj & 1) << 1) | (fi & 1);
         };
-        [[maybe_unused]] const Corners cn_own = corners(gi, gj), cn_x = corners(i0 - 1 + xtx, j0 - 1 + xty);
-        [[maybe_unused]] auto coarse_at = [&](const Corners& k, int c, int mf) __attribute__((always_inline)) -> T {
-            const int mm = min(max(mf, 0), g.NZ - 1);
-            const int slot = (mm & 1) ? 2 : ((mm >> 1) & 1);
-            const T* P = &cpl[PRO ? slot : 0][c * CPY * CPX + k.o];
-            return k.w00 * P[0] + k.w01 * P[1] + k.w10 * P[CPX] + k.w11 * P[CPX + 1];
+        [[maybe_unused]] const int cn_own = corners(gi, gj), cn_x = corners(i0 - 1 + xtx, j0 - 1 + xty);
+        [[maybe_unused]] auto coarse_at = [&](int k, int c, int mf) __attribute__((always_inline)) -> T {
+            const int f = min(max(mf, 0), g.NZ - 1) - fbase;
+            const T* P = &czl[f * CVALS + c * CPY * CPX + (k >> 2)];
+            const T wx = (k & 1) ? T(0.5) : T(0), wy = (k & 2) ? T(0.5) : T(0);
+            const T lo = P[0] + wx * (P[1] - P[0]), hi = P[CPX] + wx * (P[CPX + 1] - P[CPX]);
+            return lo + wy * (hi - lo);
         };
         // PD register sets of a prefetched plane: x (and r, D^-1 when fused)
         // at this thread's node and extra value, plus E of element layer
@@ -570,11 +526,6 @@ __global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
         auto step = [&](int ke, const T* Flo, T* Fhi, Slot& q) __attribute__((always_inline)) {
             plane_store(q);  // plane ke+1, E of layer ke
             __syncthreads();
-            // form 4: the coarse planes of the plane the next layer stages, requested
-            // ahead of this layer's stores and prefetch: the memory counter is in
-            // order, so waiting for them at the layer's end leaves those in flight
-            coarse_need(ke + 2);
-            asm volatile("" ::: "memory");
             flush();
             asm volatile("" ::: "memory");
             plane_load(q, min(ke + 1 + PD, k1));  // unconditional (past k1: re-reads plane k1, a cache hit)
@@ -610,7 +561,6 @@ __global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
                 ex[c][threadIdx.x] = H[c] + H[6 + c];
                 ex[3 + c][threadIdx.x] = H[3 + c] + H[9 + c];
             }
-            coarse_flush();
             __syncthreads();
             const int src = ty >= 1 ? threadIdx.x - TX : threadIdx.x;
             T node[3];
@@ -645,11 +595,8 @@ __global__ __launch_bounds__(TX * TY) void k_matvec_sf(GridArgs g, ParityK<T> K,
         };
 
         plane_load(sl[0], k0 - 1);
-        if constexpr (PRO) {  // coarse planes of fine planes k0-1 and k0
-            const int A = min(max(k0 - 1, 0), g.NZ - 1) >> 1;
-            coarse_want(A);
-            if (A + 1 < a.cNZ) coarse_want(A + 1);
-            coarse_flush();
+        if constexpr (PRO) {  // the chunk's z-interpolated correction, before the first staging
+            coarse_fill();
             __syncthreads();
         }
         plane_store(sl[0]);
@@ -774,6 +721,13 @@ __global__ __launch_bounds__(256) void k_elastic_diag(GridArgs g, double kd0, do
 
 inline int round8(int n) { return (n + 7) & ~7; }
 
+// values of a TX x TY tile's footprint on one coarse plane (3 components)
+template <int TX, int TY>
+constexpr int post_footprint() {
+    return 3 * (TX / 2 + 2) * (TY / 2 + 2);
+}
+constexpr long kPostLdsBytes = 64 * 1024;  // dynamic LDS of the post-smoothing form
+
 template <int TX, int TY>
 int zmarch_tiles(const GridArgs& g, int kchunk, int* ntx, int* nty) {
     *ntx = (g.NX + TX - 2) / (TX - 1);
@@ -783,6 +737,12 @@ int zmarch_tiles(const GridArgs& g, int kchunk, int* ntx, int* nty) {
 }
 
 }  // namespace
+
+int smoothing_post_kchunk(int tile, int kchunk, int elem_bytes) {
+    const int cv = tile == 1 ? post_footprint<16, 32>() : tile == 2 ? post_footprint<32, 8>() : post_footprint<16, 16>();
+    while (kchunk > 1 && (long)(kchunk + 2) * cv * elem_bytes > kPostLdsBytes) kchunk /= 2;
+    return kchunk;
+}
 
 int matvec_parity_blocks(const GridArgs& g, int tile, int kchunk) {
     int a, b;
@@ -814,8 +774,20 @@ int launch_zmarch(const GridArgs& g, DenseK<T> Kd, const MvArgs<T>& a, const MvS
 }
 
 template <typename T, int TX, int TY, int PD>
-int launch_sf(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, const MvShape& sh, hipStream_t s,
+int launch_sf(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, const MvShape& sh_in, hipStream_t s,
               int nrhs) {
+    MvShape sh = sh_in;
+    size_t dyn = 0;
+    if (a.form == 4) {  // the z-interpolated coarse correction of the chunk, (kchunk + 2) footprints
+        sh.kchunk = smoothing_post_kchunk(sh.tile, sh.kchunk, (int)sizeof(T));
+        dyn = (size_t)(sh.kchunk + 2) * post_footprint<TX, TY>() * sizeof(T);
+        static bool lds_set = false;  // first use: the start of a solve, outside graph capture
+        if (!lds_set) {
+            (void)hipFuncSetAttribute((const void*)k_matvec_sf<T, TX, TY, 4, PD>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPostLdsBytes);
+            lds_set = true;
+        }
+    }
     int ntx, nty;
     const int nt = zmarch_tiles<TX, TY>(g, sh.kchunk, &ntx, &nty);
     const int nb = round8(nt);
@@ -823,7 +795,7 @@ int launch_sf(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, const 
         hipLaunchKernelGGL((k_matvec_sf<T, TX, TY, 3, PD>), dim3(nb, nrhs), dim3(TX * TY), 0, s, g, K, a, sh.kchunk,
                            ntx, nty, nt);
     else if (a.form == 4)
-        hipLaunchKernelGGL((k_matvec_sf<T, TX, TY, 4, PD>), dim3(nb, nrhs), dim3(TX * TY), 0, s, g, K, a, sh.kchunk,
+        hipLaunchKernelGGL((k_matvec_sf<T, TX, TY, 4, PD>), dim3(nb, nrhs), dim3(TX * TY), dyn, s, g, K, a, sh.kchunk,
                            ntx, nty, nt);
     else if (a.p_out != nullptr && a.minv == nullptr)
         hipLaunchKernelGGL((k_matvec_sf<T, TX, TY, 2, PD>), dim3(nb, nrhs), dim3(TX * TY), 0, s, g, K, a, sh.kchunk,

@@ -61,6 +61,9 @@ int launch_matvec_gather(const GridArgs& g, const double* Kbin, const MvArgs<dou
 int launch_matvec_mfma(const GridArgs& g, const float* Kbin, const MvArgs<float>& a, const MvShape& sh,
                         hipStream_t s, int nrhs = 1);
 int matvec_parity_blocks(const GridArgs& g, int tile, int kchunk);
+// z chunk the post-smoothing form (MvArgs::form 4) runs at for a requested
+// one: its dynamic LDS, (kchunk + 2) coarse footprints, stays within 64 KiB
+int smoothing_post_kchunk(int tile, int kchunk, int elem_bytes);
 int matvec_dense_blocks(const GridArgs& g, int kchunk);
 int matvec_gather_blocks(const GridArgs& g);
 

@@ -670,8 +670,8 @@ int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const MvShape&
     PcgState* const st = fin_mode == 0 ? st_ : nullptr;
     double* const upd_partials = partials;
     double* const rz_partials = partials + 2 * kVecBlocks;
-    if (2L * kVecBlocks + matvec_parity_blocks(h.g[0], sh_post.tile, sh_post.kchunk) > max_partials)
-        return HX_ERR_BAD_ARG;
+    const int kc0 = smoothing_post_kchunk(sh_post.tile, sh_post.kchunk, (int)sizeof(T));  // as the launch runs it
+    if (2L * kVecBlocks + matvec_parity_blocks(h.g[0], sh_post.tile, kc0) > max_partials) return HX_ERR_BAD_ARG;
     auto level_r = [&](int l) { return l == 0 ? r : (const T*)h.r[l]; };
     // coarse levels: shallower z chunks until the launch has kVcMinBlocks
     // workgroups (or one plane each), so the small grids are not latency-bound
