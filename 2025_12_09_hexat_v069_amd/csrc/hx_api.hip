@@ -76,7 +76,7 @@ struct TimerPair {
     int cls;
 };
 
-constexpr double kIrDelta = 0.05;  // reliable update when ||r|| drops by this factor
+constexpr double kIrDelta = 0.01;  // reliable update when ||r|| drops by this factor (HEXAT_IR_DELTA)
 constexpr int kMaxRhs = 8;         // columns of a block solve
 
 }  // namespace
@@ -136,6 +136,8 @@ struct hx_ctx {
     // V-cycle form: smoother weight, per smoothed level the smoother diagonal,
     // residual and operator moduli
     double vc_omega = 1.95;
+    // FP32 form: reliable update each time a column's recursive residual falls by this factor
+    double ir_delta = kIrDelta;
     DevBuf<double> vc_s[kMlMaxLevels + 1], vc_res[kMlMaxLevels + 1], vc_E[kMlMaxLevels + 1];
     DevBuf<float> vc_s32[kMlMaxLevels + 1], vc_res32[kMlMaxLevels + 1], vc_E32[kMlMaxLevels + 1];
     // captured iteration batches (hipGraph), keyed by kernel arguments
@@ -626,7 +628,7 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
     launch_true_residual(n, v.f, v.q, v.x, c->minv.p, o.shift, nullptr, v.r32, c->st, c->partials.p, 3, c->stream,
                          nr);
     if (ml && (rc = pc_apply32(c, vc, v.r32, 2))) return rc;
-    launch_ir_resume(c->st, o.tol, kIrDelta, 1, c->stream, nr);
+    launch_ir_resume(c->st, o.tol, c->ir_delta, 1, c->stream, nr);
     HXK();
     if ((rc = sync_state(c, nr))) return rc;
     int refinements = 0;
@@ -665,7 +667,7 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
         launch_true_residual(n, v.f, v.q, v.x, c->minv.p, o.shift, nullptr, v.r32, c->st, c->partials.p, 2,
                              c->stream, nr);
         if (ml && (rc = pc_apply32(c, vc, v.r32, 2))) return rc;
-        launch_ir_resume(c->st, o.tol, kIrDelta, 0, c->stream, nr);
+        launch_ir_resume(c->st, o.tol, c->ir_delta, 0, c->stream, nr);
         HXK();
         ++refinements;
         if ((rc = sync_state(c, nr))) return rc;
@@ -975,6 +977,10 @@ int hx_ctx_create(int device, hx_ctx** out) {
     // cap on the multilevel preconditioner's coarse levels (default: every 2:1 coarsening)
     if (const char* ev = std::getenv("HEXAT_ML_LEVELS")) c->ml_max_levels = std::max(0, std::atoi(ev));
     if (const char* ev = std::getenv("HEXAT_ML_EXACT")) c->ml_exact = ev[0] != '0';
+    if (const char* ev = std::getenv("HEXAT_IR_DELTA")) {  // reliable-update factor of the FP32 form, 0 < d < 1
+        const double d = std::atof(ev);
+        if (d > 0.0 && d < 1.0) c->ir_delta = d;
+    }
     if (const char* ev = std::getenv("HEXAT_VC_OMEGA")) {  // V-cycle smoother weight, 0 < omega < 2
         const double w = std::atof(ev);
         if (w > 0.0 && w < 2.0) c->vc_omega = w;
