@@ -923,13 +923,17 @@ void material_consts(double nu, double* l1, double* l2, double* mu) {
     *mu = 0.5 * (1.0 - 2.0 * nu) * f;
 }
 
+// OC update with the multiplier found by bisection (top88, hx_oc_update) or,
+// with kary, by the 8-point search (9x bracket reduction per pass, the same
+// stopping rule; the device-resident loop)
 int oc_run(hx_ctx* c, long ne, const double* x, const double* dc, const double* dv, double volfrac, double move,
-           double xmin, double rel_tol, double* xnew) {
+           double xmin, double rel_tol, double* xnew, bool kary = false) {
     launch_oc_init(c->st, c->stream);
     int rc;
     for (int chunk = 0; chunk < 13; ++chunk) {
-        for (int k = 0; k < 16; ++k)
-            launch_oc_eval(ne, x, dc, dv, volfrac, move, xmin, rel_tol, c->st, c->partials.p, c->stream);
+        for (int k = 0; k < (kary ? 8 : 16); ++k)
+            if (kary) launch_oc_eval_k(ne, x, dc, dv, volfrac, move, xmin, rel_tol, c->st, c->partials.p, c->stream);
+            else launch_oc_eval(ne, x, dc, dv, volfrac, move, xmin, rel_tol, c->st, c->partials.p, c->stream);
         HXK();
         if ((rc = sync_state(c))) return rc;
         if (c->st_host->done) break;
@@ -1439,7 +1443,8 @@ int hx_to_step(hx_ctx* c, hx_to_stats* s) {
     hx_filter_stats f1{}, f2{};
     if ((rc = helm_solve(c, R, fo, c->t_dc.p, c->t_dcf.p, &f1))) return rc;
     // 5. OC (volume of the mass-preserving filter equals that of x_new)
-    if ((rc = oc_run(c, ne, c->t_x.p, c->t_dcf.p, c->t_dv.p, p.volfrac, p.move, p.xmin, p.oc_rel_tol, c->t_xnew.p)))
+    if ((rc = oc_run(c, ne, c->t_x.p, c->t_dcf.p, c->t_dv.p, p.volfrac, p.move, p.xmin, p.oc_rel_tol, c->t_xnew.p,
+                     true)))
         return rc;
     const double change = c->st_host->scal[3];
     // 6. density filter

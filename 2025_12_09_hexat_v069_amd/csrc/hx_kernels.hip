@@ -728,6 +728,62 @@ __global__ __launch_bounds__(NT) void k_fin_oc_eval(PcgState* st, const double* 
     }
 }
 
+// The device-resident loop's search for the same multiplier: per pass the
+// volume at kOcK equally spaced interior points of [l1, l2] (one read of x,
+// dc, dv; the candidate is oc_candidate's, evaluated as x sqrt(-dc/dv) times
+// 1/sqrt(lambda): one square root per element, not per point), and the
+// bracket shrinks to the interval where the volume crosses the target,
+// (kOcK + 1)x per pass against 2x per bisection step; same stopping rule.
+constexpr int kOcK = 8;
+__device__ __forceinline__ double oc_point(double l1, double l2, int j) {
+    return l1 + (l2 - l1) * (double)j / (double)(kOcK + 1);
+}
+__global__ __launch_bounds__(NT) void k_oc_eval_k(long ne, const double* __restrict__ x,
+                                                  const double* __restrict__ dc, const double* __restrict__ dv,
+                                                  double move, double xmin, const PcgState* st, double* partials) {
+    __shared__ double red[NT / 64];
+    if (st->done) return;
+    const double l1 = st->scal[0], l2 = st->scal[1];
+    double isl[kOcK], sum[kOcK];  // 1/sqrt(lambda_j) (lambda_j > 0: l2 > l1 >= 0)
+#pragma unroll
+    for (int j = 0; j < kOcK; ++j) {
+        isl[j] = 1.0 / sqrt(oc_point(l1, l2, j + 1));
+        sum[j] = 0.0;
+    }
+    for (long e = gtid(); e < ne; e += gstride()) {
+        const double xe = x[e], be = -dc[e] / dv[e];
+        const double b = xe * sqrt(be > 0.0 ? be : 0.0);
+        const double lo = fmax(xmin, xe - move), hi = fmin(1.0, xe + move);
+#pragma unroll
+        for (int j = 0; j < kOcK; ++j) sum[j] += fmin(fmax(b * isl[j], lo), hi);
+    }
+#pragma unroll
+    for (int j = 0; j < kOcK; ++j) {  // partials of candidate j at [j nb + block]
+        const double v = block_sum<NT>(sum[j], red);
+        if (threadIdx.x == 0) partials[(long)j * gridDim.x + blockIdx.x] = v;
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_fin_oc_eval_k(PcgState* st, const double* partials, unsigned nb, long ne,
+                                                      double volfrac, double rel_tol) {
+    __shared__ double red[NT / 64];
+    if (st->done) return;
+    double t[kOcK];
+#pragma unroll
+    for (int j = 0; j < kOcK; ++j) t[j] = reduce_partials<NT>(partials + (long)j * nb, 1, 0, nb, red);
+    if (threadIdx.x == 0) {
+        const double l1 = st->scal[0], l2 = st->scal[1], target = volfrac * (double)ne;
+        int j = 0;  // the first candidate whose volume is within the target
+        while (j < kOcK && t[j] > target) ++j;
+        const double lo = oc_point(l1, l2, j), hi = j == kOcK ? l2 : oc_point(l1, l2, j + 1);
+        st->scal[0] = lo;
+        st->scal[1] = hi;
+        st->scal[2] = 0.5 * (lo + hi);
+        st->iter += 1;
+        if ((hi - lo) / (hi + lo) <= rel_tol || st->iter >= 200) st->done = 1;
+    }
+}
+
 __global__ __launch_bounds__(NT) void k_oc_apply(long ne, const double* __restrict__ x,
                                                  const double* __restrict__ dc, const double* __restrict__ dv,
                                                  double move, double xmin, double* __restrict__ xnew, PcgState* st,
@@ -918,6 +974,12 @@ void launch_masked_diff(long n, const double* a, const double* b, const double* 
     hipLaunchKernelGGL(k_masked_diff, dim3(kVecBlocks), dim3(NT), 0, s, n, a, b, minv, e);
 }
 void launch_oc_init(PcgState* st, hipStream_t s) { hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(64), 0, s, st); }
+void launch_oc_eval_k(long ne, const double* x, const double* dc, const double* dv, double volfrac, double move,
+                      double xmin, double rel_tol, PcgState* st, double* partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_oc_eval_k, dim3(kVecBlocks), dim3(NT), 0, s, ne, x, dc, dv, move, xmin, st, partials);
+    hipLaunchKernelGGL(k_fin_oc_eval_k, dim3(1), dim3(NT), 0, s, st, partials, (unsigned)kVecBlocks, ne, volfrac,
+                       rel_tol);
+}
 void launch_oc_eval(long ne, const double* x, const double* dc, const double* dv, double volfrac, double move,
                     double xmin, double rel_tol, PcgState* st, double* partials, hipStream_t s) {
     hipLaunchKernelGGL(k_oc_eval, dim3(kVecBlocks), dim3(NT), 0, s, ne, x, dc, dv, volfrac, move, xmin, rel_tol, st,

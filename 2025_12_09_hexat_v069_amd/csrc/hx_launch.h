@@ -239,6 +239,10 @@ void launch_element_post(const GridArgs& g, const ParityK<double>& Kunit, const 
 int element_post_blocks(const GridArgs& g);
 void launch_simp_modulus(long ne, const double* xphys, double penal, double E0, double Emin, double* E, hipStream_t s);
 void launch_oc_init(PcgState* st, hipStream_t s);
+// one pass of the 8-point search for the same multiplier (the device-resident
+// loop's OC): bracket shrunk 9x per pass; partials: 8 per block
+void launch_oc_eval_k(long ne, const double* x, const double* dc, const double* dv, double volfrac, double move,
+                      double xmin, double rel_tol, PcgState* st, double* partials, hipStream_t s);
 void launch_oc_eval(long ne, const double* x, const double* dc, const double* dv, double volfrac, double move,
                     double xmin, double rel_tol, PcgState* st, double* partials, hipStream_t s);
 void launch_oc_apply(long ne, const double* x, const double* dc, const double* dv, double move, double xmin,
