@@ -500,8 +500,9 @@ MvShape vc_shape(const hx_ctx* c, bool f32, bool post) {
 // launch_ml_apply's)
 int pc_apply64(hx_ctx* c, bool vcycle, const double* r, int fin_mode) {
     if (vcycle) {
-        if (launch_vc_apply<double>(c->ml64, c->kp64, vc_shape(c, false, false), vc_shape(c, false, true), r,
-                                    c->ml_z.p, c->st, c->partials.p, kOpPartials, fin_mode, c->stream) != HX_OK)
+        if (launch_vc_apply<double>(c->ml64, c->kp64, c->Kbin_d.p, vc_shape(c, false, false),
+                                    vc_shape(c, false, true), r, c->ml_z.p, c->st, c->partials.p, kOpPartials,
+                                    fin_mode, c->stream) != HX_OK)
             return fail(c, HX_ERR_BAD_ARG, "grid too large for the V-cycle's partial sums");
     } else {
         launch_ml_apply<double>(c->ml64, r, c->minv.p, c->ml_z.p, c->st, c->partials.p, fin_mode, c->stream);
@@ -510,8 +511,9 @@ int pc_apply64(hx_ctx* c, bool vcycle, const double* r, int fin_mode) {
 }
 int pc_apply32(hx_ctx* c, bool vcycle, const float* r, int fin_mode) {
     if (vcycle) {
-        if (launch_vc_apply<float>(c->ml32, c->kp32, vc_shape(c, true, false), vc_shape(c, true, true), r,
-                                   c->ml_z32.p, c->st, c->partials.p, kOpPartials, fin_mode, c->stream) != HX_OK)
+        if (launch_vc_apply<float>(c->ml32, c->kp32, c->Kbin32_d.p, vc_shape(c, true, false),
+                                   vc_shape(c, true, true), r, c->ml_z32.p, c->st, c->partials.p, kOpPartials,
+                                   fin_mode, c->stream) != HX_OK)
             return fail(c, HX_ERR_BAD_ARG, "grid too large for the V-cycle's partial sums");
     } else {
         launch_ml_apply<float>(c->ml32, r, c->minv32.p, c->ml_z32.p, c->st, c->partials.p, fin_mode, c->stream);

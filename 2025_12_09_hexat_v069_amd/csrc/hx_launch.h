@@ -188,8 +188,8 @@ template <typename T>
 void launch_ml_exact_setup(const GridArgs& gc, const double* Ec, const double* Kbin, double h, const uint8_t* fixed_f,
                            int stride, const GridArgs& gfine, T* ainv, hipStream_t s);
 void ml_prepare_kernels();  // once per process, outside graph capture
-// V-cycle form: levels 0..c-1 smoothed (c = vc_levels), the additive form from
-// level c down as the coarse correction
+// V-cycle form: every level above the coarsest smoothed (vc_levels = levels),
+// the coarsest solved exactly or by its diagonal
 template <typename T>
 int vc_levels(const MlHierarchy<T>& h);
 // smoother diagonal of a level: omega / absolute row sums (0 where the DOF is
@@ -200,13 +200,15 @@ void launch_vc_smoother(const GridArgs& gc, const double* Ec, double h, const do
 template <typename T>
 void launch_vc_scale_E(long ne, const double* Ec, double h, T* Eop, hipStream_t s);
 // z = V(r) with the parity operator K, its pre- and post-smoothing forms at
-// shapes sh_pre / sh_post (coarse levels: the same with shallower z chunks);
-// the r'z partials (one per level-0 post-smoothing block) go after the
-// update's and the finaliser runs as in launch_ml_apply.  HX_ERR_BAD_ARG when
-// those partials would pass max_partials.
+// shapes sh_pre / sh_post on the multi-block levels (coarse ones with
+// shallower z chunks), the small levels in one workgroup with the element
+// stiffness Kb (24x24, binary node order); the r'z partials (one per level-0
+// post-smoothing block) go after the update's and the finaliser runs as in
+// launch_ml_apply.  HX_ERR_BAD_ARG when those partials would pass max_partials.
 template <typename T>
-int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const MvShape& sh_pre, const MvShape& sh_post,
-                    const T* r, T* z, PcgState* st, double* partials, long max_partials, int fin_mode, hipStream_t s);
+int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const T* Kb, const MvShape& sh_pre,
+                    const MvShape& sh_post, const T* r, T* z, PcgState* st, double* partials, long max_partials,
+                    int fin_mode, hipStream_t s);
 
 // ---- Helmholtz filter --------------------------------------------------------
 struct HelmArgs {

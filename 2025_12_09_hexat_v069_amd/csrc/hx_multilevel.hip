@@ -16,11 +16,11 @@
 // z = D0^-1 r + mask(P y_1) on the fine grid.  All gathers; no atomics, so the
 // result is deterministic.
 //
-// The V-cycle form (launch_vc_apply) smooths on the multi-block levels
-// 0..c-1 and takes the additive form above from level c down as its coarse
-// correction: per level l < c, with S_l = omega / (absolute row sums of the
-// level's stiffness), rediscretised (averaged moduli, element stiffness scaled
-// by the element size) and restricted to the DOFs solved for,
+// The V-cycle form (launch_vc_apply) smooths every level above the coarsest,
+// which it solves exactly (or by its diagonal): per level l < L, with S_l =
+// omega / (absolute row sums of the level's stiffness), rediscretised
+// (averaged moduli, element stiffness scaled by the element size) and
+// restricted to the DOFs solved for,
 //   x = S r;  res = mask(r - A x);  e = V_{l+1}(R res);
 //   x += mask(P e);  z = x + S (r - A x)
 // The row-sum diagonal bounds A from above (x'Ax <= x'D x for the absolute
@@ -349,6 +349,133 @@ __global__ __launch_bounds__(kCoarseThreads) void k_ml_coarse(MlHierarchy<T> h, 
     for (long i = threadIdx.x; i < 3 * h.g[s0].nn; i += kCoarseThreads) h.y[s0][i] = Y(s0)[i];
 }
 
+// V-cycle levels s..L in one workgroup (the grids too small for a launch per
+// level): from the residual of level s-1 (memory) down to the coarsest level
+// and back, every level's r, x and a work vector in LDS (dynamic, after the
+// element stiffness: level by level r_l, x_l, t_l), passes separated by
+// barriers; only e_s = x_s is written back.  Level l < L: x = S r;
+// t = mask(r - A x); r_{l+1} = R t; ...; x += mask(P x_{l+1}); t = A x;
+// x += S (r - t).  Level L: x = A_L^-1 r (exact) or D_L^-1 r.  The level
+// operator gathers, per node, the 8 elements around it (moduli h.Eop[l], the
+// fine element stiffness Kb in binary node order, in LDS).
+template <typename T>
+long vc_coarse_lds_values(const MlHierarchy<T>& h, int s0) {
+    long v = 576;
+    for (int l = s0; l <= h.levels; ++l) v += 9 * h.g[l].nn;
+    return v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kCoarseThreads) void k_vc_coarse(MlHierarchy<T> h, int s0, const T* __restrict__ Kb,
+                                                              const PcgState* st) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T* lds = reinterpret_cast<T*>(lds_raw);
+    if (st != nullptr && st->done) return;
+    const int L = h.levels, t = threadIdx.x;
+    T* const kb = lds;
+    auto Rv = [&](int l) {
+        long o = 576;
+        for (int m = s0; m < l; ++m) o += 9 * h.g[m].nn;
+        return lds + o;
+    };
+    auto Xv = [&](int l) { return Rv(l) + 3 * h.g[l].nn; };
+    auto Tv = [&](int l) { return Rv(l) + 6 * h.g[l].nn; };
+    for (int i = t; i < 576; i += kCoarseThreads) kb[i] = Kb[i];
+    {  // restriction of level s0-1's residual (memory) onto level s0
+        const GridArgs gf = h.g[s0 - 1], gc = h.g[s0];
+        for (long n = t; n < gc.nn; n += kCoarseThreads) {
+            T acc[3];
+            restrict_node(gf, (const T*)h.r[s0 - 1], gc, n, acc);
+            for (int c = 0; c < 3; ++c) Rv(s0)[3 * n + c] = acc[c];
+        }
+    }
+    __syncthreads();
+    // y = A_l x over the level's nodes; with res, y = mask(r - A_l x)
+    auto apply = [&](int l, const T* x, T* y, bool res) {
+        const GridArgs g = h.g[l];
+        const T* E = h.Eop[l];
+        const T* S = h.sinv[l];
+        const T* r = Rv(l);
+        for (long n = t; n < g.nn; n += kCoarseThreads) {
+            int i, j, k;
+            node_ijk(g, n, i, j, k);
+            T acc[3] = {T(0), T(0), T(0)};
+#pragma unroll 1
+            for (int e = 0; e < 8; ++e) {  // rolled: the element's 24x24 block is read from LDS per element
+                const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
+                const int ie = i - 1 + ex, je = j - 1 + ey, ke = k - 1 + ez;
+                const bool in = ie >= 0 && je >= 0 && ke >= 0 && ie < g.nx && je < g.ny && ke < g.nz;
+                const int iec = min(max(ie, 0), g.nx - 1), jec = min(max(je, 0), g.ny - 1),
+                          kec = min(max(ke, 0), g.nz - 1);
+                const T Ee = in ? E[(long)kec * g.nx * g.ny + (long)jec * g.nx + iec] : T(0);
+                const int La = (1 - ex) + 2 * (1 - ey) + 4 * (1 - ez);
+#pragma unroll
+                for (int Lb = 0; Lb < 8; ++Lb) {
+                    const long m = (long)(kec + (Lb >> 2)) * g.NY * g.NX + (long)(jec + ((Lb >> 1) & 1)) * g.NX +
+                                   (iec + (Lb & 1));
+                    const T xm0 = x[3 * m], xm1 = x[3 * m + 1], xm2 = x[3 * m + 2];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        const T* kr = kb + (3 * La + a) * 24 + 3 * Lb;
+                        acc[a] += Ee * (kr[0] * xm0 + kr[1] * xm1 + kr[2] * xm2);
+                    }
+                }
+            }
+            for (int c = 0; c < 3; ++c) {
+                const long d = 3 * n + c;
+                y[d] = res ? (S[d] != T(0) ? r[d] - acc[c] : T(0)) : acc[c];
+            }
+        }
+    };
+    for (int l = s0; l < L; ++l) {  // down: pre-smoothing, residual, restriction
+        const long nl = 3 * h.g[l].nn;
+        for (long d = t; d < nl; d += kCoarseThreads) Xv(l)[d] = h.sinv[l][d] * Rv(l)[d];
+        __syncthreads();
+        apply(l, Xv(l), Tv(l), true);
+        __syncthreads();
+        const GridArgs gf = h.g[l], gc = h.g[l + 1];
+        for (long n = t; n < gc.nn; n += kCoarseThreads) {
+            T acc[3];
+            restrict_node(gf, (const T*)Tv(l), gc, n, acc);
+            for (int c = 0; c < 3; ++c) Rv(l + 1)[3 * n + c] = acc[c];
+        }
+        __syncthreads();
+    }
+    {  // the coarsest level
+        const long nL = 3 * h.g[L].nn;
+        const T* rl = Rv(L);
+        if (h.ainv != nullptr) {
+            const int N = (int)nL;
+            for (int d = t; d < N; d += kCoarseThreads) {
+                T acc = T(0);
+                for (int j = 0; j < N; ++j) acc += h.ainv[d * N + j] * rl[j];
+                Xv(L)[d] = acc;
+            }
+        } else {
+            for (long d = t; d < nL; d += kCoarseThreads) Xv(L)[d] = h.minv[L][d] * rl[d];
+        }
+        __syncthreads();
+    }
+    for (int l = L - 1; l >= s0; --l) {  // up: correction, post-smoothing
+        const GridArgs gf = h.g[l], gc = h.g[l + 1];
+        for (long n = t; n < gf.nn; n += kCoarseThreads) {
+            T acc[3];
+            prolong_node(gc, (const T*)Xv(l + 1), gf, n, acc);
+            for (int c = 0; c < 3; ++c) {
+                const long d = 3 * n + c;
+                if (h.sinv[l][d] != T(0)) Xv(l)[d] += acc[c];
+            }
+        }
+        __syncthreads();
+        apply(l, Xv(l), Tv(l), false);
+        __syncthreads();
+        const long nl = 3 * h.g[l].nn;
+        for (long d = t; d < nl; d += kCoarseThreads) Xv(l)[d] += h.sinv[l][d] * (Rv(l)[d] - Tv(l)[d]);
+        __syncthreads();
+    }
+    for (long i = t; i < 3 * h.g[s0].nn; i += kCoarseThreads) h.y[s0][i] = Xv(s0)[i];
+}
+
 // r'z of the new residual.  mode 0 (an iteration): record ||r|| from the
 // update's partials (r'r at index 2b+1 of nb_upd blocks), beta = rz/rz_old,
 // rz = rz, then the convergence decision; 1 (start): rz = rz, beta = 0;
@@ -638,11 +765,21 @@ void launch_ml_apply(const MlHierarchy<T>& h, const T* r, const T* minv0, T* z, 
 
 int ml_partial_blocks() { return kProlongBlocks; }
 
+// every level above the coarsest is smoothed
 template <typename T>
 int vc_levels(const MlHierarchy<T>& h) {
-    const int L = h.levels;
-    if (L == 0) return 0;
-    return std::min(ml_coarse_start(h), L);
+    return h.levels;
+}
+
+// first level of the one-workgroup V-cycle pass: at most kCoarseNodes nodes
+// and its levels' vectors within the LDS budget; levels + 1 when none fits
+template <typename T>
+int vc_coarse_start(const MlHierarchy<T>& h) {
+    int s0 = 1;
+    while (s0 <= h.levels &&
+           (h.g[s0].nn > kCoarseNodes || vc_coarse_lds_values(h, s0) * (long)sizeof(T) > kCoarseLdsBytes))
+        ++s0;
+    return s0;
 }
 
 template <typename T>
@@ -658,15 +795,16 @@ void launch_vc_scale_E(long ne, const double* Ec, double h, T* Eop, hipStream_t 
 }
 
 template <typename T>
-int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const MvShape& sh_pre, const MvShape& sh_post,
-                    const T* r, T* z, PcgState* st_, double* partials, long max_partials, int fin_mode,
-                    hipStream_t s) {
-    const int c = vc_levels(h);
-    if (c == 0) {  // no coarse level: the additive form is plain Jacobi
+int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const T* Kb, const MvShape& sh_pre,
+                    const MvShape& sh_post, const T* r, T* z, PcgState* st_, double* partials, long max_partials,
+                    int fin_mode, hipStream_t s) {
+    const int L = h.levels;
+    if (L == 0) {  // no coarse level: plain Jacobi (the additive form's path)
         launch_ml_apply<T>(h, r, h.minv[0], z, st_, partials, fin_mode, s);
         return HX_OK;
     }
-    const int L = h.levels;
+    const int s0 = vc_coarse_start(h);
+    const int c = std::min(s0, L);  // levels smoothed by multi-block launches
     PcgState* const st = fin_mode == 0 ? st_ : nullptr;
     double* const upd_partials = partials;
     double* const rz_partials = partials + 2 * kVecBlocks;
@@ -699,13 +837,12 @@ int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const MvShape&
         launch_matvec_parity<T>(h.g[l], K, a, level_shape(l, sh_pre), s, 1);
         if (l + 1 < c) restrict_to(l + 1, false);
     }
-    const int s0 = ml_coarse_start(h);
-    if (c == s0) {  // levels c..L additively in one workgroup, from vres[c-1]
+    if (s0 <= L) {  // levels s0..L in one workgroup, from vres[s0-1]
         MlHierarchy<T> hc = h;
-        hc.r[c - 1] = h.vres[c - 1];
-        hipLaunchKernelGGL((k_ml_coarse<T>), dim3(1), dim3(kCoarseThreads), coarse_lds_values(h, s0) * sizeof(T), s,
-                           hc, s0, st);
-    } else {  // c = L: the coarsest level by its diagonal
+        hc.r[s0 - 1] = h.vres[s0 - 1];
+        hipLaunchKernelGGL((k_vc_coarse<T>), dim3(1), dim3(kCoarseThreads),
+                           vc_coarse_lds_values(h, s0) * sizeof(T), s, hc, s0, Kb, st);
+    } else {  // no level fits it: the coarsest level by its diagonal
         restrict_to(L, true);
     }
     int nb0 = 0;
@@ -745,6 +882,10 @@ void ml_prepare_kernels() {
                               (int)kCoarseLdsBytes);
     (void)hipFuncSetAttribute((const void*)k_ml_coarse<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)kCoarseLdsBytes);
+    (void)hipFuncSetAttribute((const void*)k_vc_coarse<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kCoarseLdsBytes);
+    (void)hipFuncSetAttribute((const void*)k_vc_coarse<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kCoarseLdsBytes);
     constexpr int kExactLds = kMlExactMax * (kMlExactMax + 1) / 2 * (int)sizeof(double);
     (void)hipFuncSetAttribute((const void*)k_ml_exact_setup<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               kExactLds);
@@ -769,10 +910,11 @@ template void launch_vc_smoother<float>(const GridArgs&, const double*, double, 
                                         const GridArgs&, double, float*, hipStream_t);
 template void launch_vc_scale_E<double>(long, const double*, double, double*, hipStream_t);
 template void launch_vc_scale_E<float>(long, const double*, double, float*, hipStream_t);
-template int launch_vc_apply<double>(const MlHierarchy<double>&, const ParityK<double>&, const MvShape&,
-                                     const MvShape&, const double*, double*, PcgState*, double*, long, int,
-                                     hipStream_t);
-template int launch_vc_apply<float>(const MlHierarchy<float>&, const ParityK<float>&, const MvShape&, const MvShape&,
-                                    const float*, float*, PcgState*, double*, long, int, hipStream_t);
+template int launch_vc_apply<double>(const MlHierarchy<double>&, const ParityK<double>&, const double*,
+                                     const MvShape&, const MvShape&, const double*, double*, PcgState*, double*,
+                                     long, int, hipStream_t);
+template int launch_vc_apply<float>(const MlHierarchy<float>&, const ParityK<float>&, const float*, const MvShape&,
+                                    const MvShape&, const float*, float*, PcgState*, double*, long, int,
+                                    hipStream_t);
 
 }  // namespace hx

@@ -551,7 +551,8 @@ def ml_pcg(g: Grid, Ke, Ee, fixed, b, tol=1e-8, max_iter=100000, max_levels=12, 
 
 
 # ---------------------------------------------------------------------------------------------
-# V-cycle form (the library's HX_PC_VCYCLE): V-cycle on levels 0..c-1, the additive form from c
+# V-cycle form (the library's HX_PC_VCYCLE): V-cycle on levels 0..L-1, the coarsest level L exactly
+# (or by its diagonal when the hierarchy has no exact level)
 
 ML_COARSE_NODES = 2048          # a level the one-workgroup coarse pass may take
 ML_COARSE_LDS = 128 * 1024      # its budget: r and y of every level it holds
@@ -569,9 +570,9 @@ def ml_coarse_start(levels, elem_bytes):
 
 
 def vc_levels(levels, elem_bytes=8):
-    """Smoothed levels c: 0 without coarse levels, else min(s0, L)."""
-    L = len(levels) - 1
-    return 0 if L == 0 else min(ml_coarse_start(levels, elem_bytes), L)
+    """Smoothed levels: every level above the coarsest, L (0 without coarse levels).  elem_bytes is
+    kept for callers; the cycle does not depend on how the library splits it into kernels."""
+    return len(levels) - 1
 
 
 def _ke_binary(Ke):
@@ -646,10 +647,11 @@ def _additive_from(levels, c, rc):
 
 
 def vc_apply(g: Grid, Ke, Ee, fixed, r, omega=1.95, max_levels=12, exact=True, elem_bytes=8, state=None):
-    """z = M^-1 r of the V-cycle form: on levels l < c (vc_levels), with S_l = vc_smoother and A_l
+    """z = M^-1 r of the V-cycle form: on levels l < L (vc_levels), with S_l = vc_smoother and A_l
     the level operator (moduli 2^l E_l averaged over children, the fine Ke),
       x = S r;  res = mask(r - A x);  e = V_{l+1}(R res);  x += mask(P e);  z = x + S (r - A x)
-    and level c by the additive form (_additive_from).  `state` caches the levels between calls."""
+    and the coarsest level L by its inverse (exact) or its diagonal (_additive_from at L).  `state`
+    caches the levels between calls."""
     Ke = _f64(Ke)
     if state is None or "levels" not in state:
         levels = ml_hierarchy(g, Ke, Ee, fixed, max_levels, exact)

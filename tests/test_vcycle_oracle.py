@@ -48,17 +48,9 @@ def _levels(nx, ny, nz, dx, dy, dz, E, fixed, omega):
     return out, hier
 
 
-def _coarse_start(lv):
-    nn = [(v["n"][0] + 1) * (v["n"][1] + 1) * (v["n"][2] + 1) for v in lv]
-    s0 = 1
-    while s0 < len(lv) and (nn[s0] > 2048 or 6 * sum(nn[s0:]) * 8 > 128 * 1024):
-        s0 += 1
-    return s0
-
-
 def _vcycle_sparse(lv, hier, r):
     L = len(lv) - 1
-    c = 0 if L == 0 else min(_coarse_start(lv), L)
+    c = L  # every level above the coarsest is smoothed
 
     def additive(l, rl):
         if l == L:
@@ -118,7 +110,7 @@ def test_vc_apply_matches_sparse_cycle(shape):
         assert c == state["c"]
         assert np.allclose(z, zr, rtol=1e-10, atol=1e-10 * np.abs(zr).max())
     if shape[0] == 48:
-        assert state["c"] == 2  # two smoothed levels: level 1 has more nodes than one workgroup takes
+        assert state["c"] == 3  # three smoothed levels above a coarsest level solved by its diagonal
 
 
 def test_vc_dense_form_is_spd():
@@ -147,7 +139,7 @@ def test_vc_symmetric_with_two_smoothed_levels():
     mask = ox.ml_hierarchy(g, Ke, E, fixed, 0)[0][1].ravel() != 0
     u, v = (np.where(mask, rng.standard_normal(g.ndof), 0.0) for _ in range(2))
     Mu, Mv = ox.vc_apply(g, Ke, E, fixed, u, state=state), ox.vc_apply(g, Ke, E, fixed, v, state=state)
-    assert state["c"] == 2
+    assert state["c"] == 3
     assert abs(v @ Mu - u @ Mv) < 1e-10 * abs(v @ Mu)
     assert u @ Mu > 0 and v @ Mv > 0
 
