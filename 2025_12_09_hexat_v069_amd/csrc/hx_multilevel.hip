@@ -355,12 +355,14 @@ __global__ __launch_bounds__(kCoarseThreads) void k_ml_coarse(MlHierarchy<T> h, 
 // element stiffness: level by level r_l, x_l, t_l), passes separated by
 // barriers; only e_s = x_s is written back.  Level l < L: x = S r;
 // t = mask(r - A x); r_{l+1} = R t; ...; x += mask(P x_{l+1}); t = A x;
-// x += S (r - t).  Level L: x = A_L^-1 r (exact) or D_L^-1 r.  The level
-// operator gathers, per node, the 8 elements around it (moduli h.Eop[l], the
-// fine element stiffness Kb in binary node order, in LDS).
+// x += S (r - t).  Level L: x = A_L^-1 r (exact: one wave per row, lanes
+// over the columns) or D_L^-1 r.  The level operator gathers, per node, the 8
+// elements around it; the fine element stiffness Kb (binary node order) and
+// the smoothed levels' moduli h.Eop[l] are staged in LDS first.
 template <typename T>
 long vc_coarse_lds_values(const MlHierarchy<T>& h, int s0) {
     long v = 576;
+    for (int l = s0; l < h.levels; ++l) v += h.g[l].ne;
     for (int l = s0; l <= h.levels; ++l) v += 9 * h.g[l].nn;
     return v;
 }
@@ -373,14 +375,22 @@ __global__ __launch_bounds__(kCoarseThreads) void k_vc_coarse(MlHierarchy<T> h, 
     if (st != nullptr && st->done) return;
     const int L = h.levels, t = threadIdx.x;
     T* const kb = lds;
+    auto Ev = [&](int l) {  // moduli of smoothed level l
+        long o = 576;
+        for (int m = s0; m < l; ++m) o += h.g[m].ne;
+        return lds + o;
+    };
     auto Rv = [&](int l) {
         long o = 576;
+        for (int m = s0; m < L; ++m) o += h.g[m].ne;
         for (int m = s0; m < l; ++m) o += 9 * h.g[m].nn;
         return lds + o;
     };
     auto Xv = [&](int l) { return Rv(l) + 3 * h.g[l].nn; };
     auto Tv = [&](int l) { return Rv(l) + 6 * h.g[l].nn; };
     for (int i = t; i < 576; i += kCoarseThreads) kb[i] = Kb[i];
+    for (int l = s0; l < L; ++l)
+        for (long e = t; e < h.g[l].ne; e += kCoarseThreads) Ev(l)[e] = h.Eop[l][e];
     {  // restriction of level s0-1's residual (memory) onto level s0
         const GridArgs gf = h.g[s0 - 1], gc = h.g[s0];
         for (long n = t; n < gc.nn; n += kCoarseThreads) {
@@ -393,7 +403,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_vc_coarse(MlHierarchy<T> h, 
     // y = A_l x over the level's nodes; with res, y = mask(r - A_l x)
     auto apply = [&](int l, const T* x, T* y, bool res) {
         const GridArgs g = h.g[l];
-        const T* E = h.Eop[l];
+        const T* E = Ev(l);
         const T* S = h.sinv[l];
         const T* r = Rv(l);
         for (long n = t; n < g.nn; n += kCoarseThreads) {
@@ -444,12 +454,14 @@ __global__ __launch_bounds__(kCoarseThreads) void k_vc_coarse(MlHierarchy<T> h, 
     {  // the coarsest level
         const long nL = 3 * h.g[L].nn;
         const T* rl = Rv(L);
-        if (h.ainv != nullptr) {
-            const int N = (int)nL;
-            for (int d = t; d < N; d += kCoarseThreads) {
+        if (h.ainv != nullptr) {  // one wave per row: coalesced columns, then a wave sum
+            const int N = (int)nL, lane = t & 63;
+            for (int d = t >> 6; d < N; d += kCoarseThreads / 64) {
                 T acc = T(0);
-                for (int j = 0; j < N; ++j) acc += h.ainv[d * N + j] * rl[j];
-                Xv(L)[d] = acc;
+                for (int j = lane; j < N; j += 64) acc += h.ainv[(long)d * N + j] * rl[j];
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+                if (lane == 0) Xv(L)[d] = acc;
             }
         } else {
             for (long d = t; d < nL; d += kCoarseThreads) Xv(L)[d] = h.minv[L][d] * rl[d];
@@ -771,13 +783,16 @@ int vc_levels(const MlHierarchy<T>& h) {
     return h.levels;
 }
 
-// first level of the one-workgroup V-cycle pass: at most kCoarseNodes nodes
-// and its levels' vectors within the LDS budget; levels + 1 when none fits
+// first level of the one-workgroup V-cycle pass: at most kVcCoarseNodes nodes
+// (its level operator runs on one compute unit, so larger levels are faster
+// as multi-block launches) and its levels' data within the LDS budget;
+// levels + 1 when none fits
+constexpr long kVcCoarseNodes = 512;
 template <typename T>
 int vc_coarse_start(const MlHierarchy<T>& h) {
     int s0 = 1;
     while (s0 <= h.levels &&
-           (h.g[s0].nn > kCoarseNodes || vc_coarse_lds_values(h, s0) * (long)sizeof(T) > kCoarseLdsBytes))
+           (h.g[s0].nn > kVcCoarseNodes || vc_coarse_lds_values(h, s0) * (long)sizeof(T) > kCoarseLdsBytes))
         ++s0;
     return s0;
 }
