@@ -71,10 +71,10 @@ enum { HX_FP64 = 0, HX_FP32_IR = 1 };           /* precision form */
  * symmetric form of a U-Net preconditioner; plain PCG, one right-hand side.
  * No reference counterpart (the reference's strong-preconditioner slot is the
  * learned one, and its CPU AMG branch, GPUSolver.jl:168-268). */
-/* HX_PC_VCYCLE: the same hierarchy as a symmetric V-cycle on the levels too
- * large for one workgroup (l1-Jacobi smoothing, omega / absolute row sums of
- * the rediscretised level stiffness, one step before and one after the
- * coarse correction), the additive form above below them; SPD for every
+/* HX_PC_VCYCLE: the same hierarchy as a symmetric V-cycle: l1-Jacobi
+ * smoothing (omega / absolute row sums of the rediscretised level stiffness,
+ * one step before and one after the coarse correction) on every level above
+ * the coarsest, which is solved exactly (or by its diagonal); SPD for every
  * moduli field; one right-hand side.  No reference counterpart. */
 enum { HX_PC_JACOBI = 0, HX_PC_LEARNED = 1, HX_PC_MULTILEVEL = 2, HX_PC_VCYCLE = 3 };
 enum { HX_MV_AUTO = 0, HX_MV_PARITY = 1, HX_MV_DENSE = 2, HX_MV_GATHER = 3, HX_MV_MFMA = 4 };

@@ -26,8 +26,8 @@ int hx_test_copy(hx_ctx* ctx, long bytes, int reps, double* avg_ms);
  * the number of coarse levels.  No reference counterpart. */
 int hx_test_ml_apply(hx_ctx* ctx, const double* Ee, const double* r, double* z, int precision, int* levels);
 /* The same for the V-cycle form (HX_PC_VCYCLE); *vlevels receives the number
- * of smoothed levels (the additive form takes over below them).  No
- * reference counterpart. */
+ * of smoothed levels (every level above the coarsest).  No reference
+ * counterpart. */
 int hx_test_vc_apply(hx_ctx* ctx, const double* Ee, const double* r, double* z, int precision, int* levels,
                      int* vlevels);
 

@@ -21,8 +21,8 @@ pieces in numpy:
   numpy passes; no reference code exists.
 * ``vc_apply`` / ``vc_pcg`` -- the same hierarchy as a symmetric V-cycle with
   l1-Jacobi smoothing (absolute row sums, the smoother of Baker, Falgout,
-  Kolev & Yang 2011) on the levels too large for one workgroup; no reference
-  code exists.
+  Kolev & Yang 2011) on every level above the coarsest; no reference code
+  exists.
 """
 from __future__ import annotations
 
@@ -554,24 +554,8 @@ def ml_pcg(g: Grid, Ke, Ee, fixed, b, tol=1e-8, max_iter=100000, max_levels=12, 
 # V-cycle form (the library's HX_PC_VCYCLE): V-cycle on levels 0..L-1, the coarsest level L exactly
 # (or by its diagonal when the hierarchy has no exact level)
 
-ML_COARSE_NODES = 2048          # a level the one-workgroup coarse pass may take
-ML_COARSE_LDS = 128 * 1024      # its budget: r and y of every level it holds
-
-
-def ml_coarse_start(levels, elem_bytes):
-    """First level (>= 1) of the one-workgroup coarse pass: at most ML_COARSE_NODES nodes and its
-    levels' r, y (6 values per node) within ML_COARSE_LDS bytes; L + 1 when none qualifies."""
-    L = len(levels) - 1
-    nn = [(n[0] + 1) * (n[1] + 1) * (n[2] + 1) for n, _, _ in levels]
-    s0 = 1
-    while s0 <= L and (nn[s0] > ML_COARSE_NODES or 6 * sum(nn[s0:]) * elem_bytes > ML_COARSE_LDS):
-        s0 += 1
-    return s0
-
-
-def vc_levels(levels, elem_bytes=8):
-    """Smoothed levels: every level above the coarsest, L (0 without coarse levels).  elem_bytes is
-    kept for callers; the cycle does not depend on how the library splits it into kernels."""
+def vc_levels(levels):
+    """Smoothed levels: every level above the coarsest, L (0 without coarse levels)."""
     return len(levels) - 1
 
 
@@ -646,7 +630,7 @@ def _additive_from(levels, c, rc):
     return y
 
 
-def vc_apply(g: Grid, Ke, Ee, fixed, r, omega=1.95, max_levels=12, exact=True, elem_bytes=8, state=None):
+def vc_apply(g: Grid, Ke, Ee, fixed, r, omega=1.95, max_levels=12, exact=True, state=None):
     """z = M^-1 r of the V-cycle form: on levels l < L (vc_levels), with S_l = vc_smoother and A_l
     the level operator (moduli 2^l E_l averaged over children, the fine Ke),
       x = S r;  res = mask(r - A x);  e = V_{l+1}(R res);  x += mask(P e);  z = x + S (r - A x)
@@ -655,7 +639,7 @@ def vc_apply(g: Grid, Ke, Ee, fixed, r, omega=1.95, max_levels=12, exact=True, e
     Ke = _f64(Ke)
     if state is None or "levels" not in state:
         levels = ml_hierarchy(g, Ke, Ee, fixed, max_levels, exact)
-        c = vc_levels(levels, elem_bytes)
+        c = vc_levels(levels)
         fx = np.asarray(fixed, dtype=bool).reshape(g.nz + 1, g.ny + 1, g.nx + 1, 3)
         E, n, Es, Ss = _f64(Ee), (g.nx, g.ny, g.nz), [], []
         for l in range(c):

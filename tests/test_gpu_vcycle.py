@@ -54,7 +54,7 @@ def test_vc_apply_matches_oracle(ox, hx, ctx, dims, h, precision):
     r = np.where(m0 != 0, rng.standard_normal(g.ndof), 0.0)
     z, L, c = ctx.test_vc_apply(Ee, r, precision)
     state = {}
-    zo = ox.vc_apply(g, Ke, Ee, fixed, r, elem_bytes=8 if precision == 0 else 4, state=state)
+    zo = ox.vc_apply(g, Ke, Ee, fixed, r, state=state)
     assert L == len(state["levels"]) - 1 and c == state["c"]
     tol = 1e-10 if precision == 0 else 1e-5
     assert np.allclose(z, zo, rtol=tol, atol=tol * np.abs(zo).max())
@@ -168,7 +168,7 @@ def test_vc_smoothing_shapes_match_oracle(ox, hx, ctx, pre, post):
     r = np.where(fixed == 0, rng.standard_normal(g.ndof), 0.0)
     for precision, tol in ((0, 1e-10), (1, 1e-5)):
         z, _, _ = ctx.test_vc_apply(Ee, r, precision)
-        zo = ox.vc_apply(g, Ke, Ee, fixed, r, elem_bytes=8 if precision == 0 else 4)
+        zo = ox.vc_apply(g, Ke, Ee, fixed, r)
         assert np.allclose(z, zo, rtol=tol, atol=tol * np.abs(zo).max())
     tuned = ctx.autotune(1)
     assert tuned["kchunk_pre"] > 0 and tuned["kchunk_post"] > 0 and tuned["ms_post"] > 0.0
