@@ -92,79 +92,131 @@ __global__ __launch_bounds__(NT) void k_ml_minv(GridArgs gc, double kd0, double 
 }
 
 // rc = R rf (full weighting: weight 1/2 per axis offset; on the coarsest
-// level also yc = w Dc^-1 rc) over a tile of 16x4x4 coarse nodes per
-// workgroup: the 33x9x9 fine nodes under it are staged in LDS with
-// row-contiguous loads (out-of-grid nodes as 0), then each thread sums its 27
-// fine nodes from LDS.  Gathering the 81 values per thread straight from the
-// cache instead leaves the kernel bound by address processing.
-constexpr int kRtX = 16, kRtY = 4, kRtZ = 4;
-template <typename T, bool SCALE, int RX = kRtX, int RY = kRtY, int RZ = kRtZ>
-__global__ __launch_bounds__(NT) void k_ml_restrict_tiled(GridArgs gf, const T* __restrict__ rf, GridArgs gc,
-                                                          T* __restrict__ rc, const T* __restrict__ minvc,
-                                                          T* __restrict__ yc, T w, const PcgState* st, int ntx,
-                                                          int nty) {
-    static_assert(RX * RY * RZ == NT, "one coarse node per thread");
-    constexpr int FX = 2 * RX + 1, FY = 2 * RY + 1, FZ = 2 * RZ + 1, ROW = 3 * FX;
-    __shared__ T sf[FZ * FY * ROW];
+// level also yc = w Dc^-1 rc), marching in z: a workgroup owns a CX x CY
+// column of coarse nodes over kc coarse planes.  Coarse plane K sums fine planes 2K-1,
+// 2K, 2K+1 (weights 1/2, 1, 1/2), each first restricted in xy from a
+// (2CX+1) x (2CY+1) fine tile staged in LDS; plane 2K+1's xy sum is carried
+// to K+1.  The next pair of fine planes is loaded into registers while the
+// current pair is summed, so one launch streams the fine level once (the
+// chunk's first plane and the xy halo aside) with every workgroup resident.
+constexpr int kRzX = 32, kRzY = 8;
+// read in place of out-of-grid fine nodes: an unconditional load from a
+// selected address keeps every load of a plane in flight (a select on the
+// loaded value is compiled into a branch and a wait per load); never
+// written, and not const, so that it shares the global address space with
+// the fine level and the loads stay global loads
+__device__ double kRzZero[1] = {0.0};
+template <typename T, bool SCALE, int CX = kRzX, int CY = kRzY>
+__global__ __launch_bounds__(CX * CY) void k_ml_restrict_zm(GridArgs gf, const T* __restrict__ rf, GridArgs gc,
+                                                            T* __restrict__ rc, const T* __restrict__ minvc,
+                                                            T* __restrict__ yc, T w, const PcgState* st, int ntx,
+                                                            int nty, int kc) {
+    constexpr int NTH = CX * CY, ROW = 3 * (2 * CX + 1), PLANE = (2 * CY + 1) * ROW, PER = (PLANE + NTH - 1) / NTH;
+    __shared__ T sf[2][PLANE];
     if (st != nullptr && st->done) return;
+    const int t = threadIdx.x;
     const int bx = blockIdx.x % ntx, by = (blockIdx.x / ntx) % nty, bz = blockIdx.x / (ntx * nty);
-    const int I0 = bx * RX, J0 = by * RY, K0 = bz * RZ;
-    const int i0 = 2 * I0 - 1, j0 = 2 * J0 - 1, k0 = 2 * K0 - 1;
-    // every load of the tile is issued before any is waited for: clamped
-    // addresses, out-of-grid values replaced by 0 with a select
-    constexpr int TOTAL = FZ * FY * ROW, PER = (TOTAL + NT - 1) / NT;
-    // element idx = t + m NT as (z, y, x3): one division for m = 0, then a
-    // step of NT = DY rows + DX3 positions per m with carries
-    constexpr int DY = NT / ROW, DX3 = NT - DY * ROW;
-    int z = (int)threadIdx.x / (FY * ROW), y = ((int)threadIdx.x / ROW) % FY, x3 = (int)threadIdx.x % ROW;
-    T v[PER];
+    const int I0 = bx * CX, J0 = by * CY, K0 = bz * kc, K1 = min(K0 + kc, gc.NZ);
+    const int i0 = 2 * I0 - 1, j0 = 2 * J0 - 1;
+    const long fplane = 3L * gf.NX * gf.NY;
+    // this thread's slots of a fine plane tile: (y, x, component) with
+    // clamped in-plane offsets; out-of-grid slots read as 0 through a select
+    int off[PER];
+    unsigned okm = 0;
 #pragma unroll
     for (int m = 0; m < PER; ++m) {
-        if (m > 0) {
-            x3 += DX3;
-            y += DY;
-            if (x3 >= ROW) {
-                x3 -= ROW;
-                ++y;
-            }
-            if (y >= FY) {
-                y -= FY;
-                ++z;
-            }
-        }
-        const int zc = min(z, FZ - 1);  // past the tile's end: clamped, never stored
-        const int x = x3 / 3, i = i0 + x, j = j0 + y, k = k0 + zc;
-        const bool in = i >= 0 && i < gf.NX && j >= 0 && j < gf.NY && k >= 0 && k < gf.NZ;
-        const int ic = min(max(i, 0), gf.NX - 1), jc = min(max(j, 0), gf.NY - 1), kc = min(max(k, 0), gf.NZ - 1);
-        const T val = rf[3 * ((long)kc * gf.NY * gf.NX + (long)jc * gf.NX + ic) + (x3 - 3 * x)];
-        v[m] = in ? val : T(0);
+        const int idx = t + m * NTH, y = idx / ROW, x3 = idx - y * ROW, x = x3 / 3, i = i0 + x, j = j0 + y;
+        if (idx < PLANE && i >= 0 && i < gf.NX && j >= 0 && j < gf.NY) okm |= 1u << m;
+        off[m] = 3 * (min(max(j, 0), gf.NY - 1) * gf.NX + min(max(i, 0), gf.NX - 1)) + (x3 - 3 * x);
     }
+    const T* const zero = reinterpret_cast<const T*>(kRzZero);
+    auto load = [&](int k, T (&v)[PER]) __attribute__((always_inline)) {
+        const unsigned ok = (k >= 0 && k < gf.NZ) ? okm : 0u;
+        const T* p = rf + (long)min(max(k, 0), gf.NZ - 1) * fplane;
 #pragma unroll
-    for (int m = 0; m < PER; ++m)
-        if ((int)threadIdx.x + m * NT < TOTAL) sf[threadIdx.x + m * NT] = v[m];
-    __syncthreads();
-    const int tx = threadIdx.x % RX, ty = (threadIdx.x / RX) % RY, tz = threadIdx.x / (RX * RY);
-    const int I = I0 + tx, J = J0 + ty, K = K0 + tz;
-    if (I >= gc.NX || J >= gc.NY || K >= gc.NZ) return;
-    T acc[3] = {T(0), T(0), T(0)};
+        for (int m = 0; m < PER; ++m) v[m] = *(((ok >> m) & 1u) ? p + off[m] : zero);
+    };
+    auto put = [&](int b, const T (&v)[PER]) __attribute__((always_inline)) {
 #pragma unroll
-    for (int dz = 0; dz < 3; ++dz)
+        for (int m = 0; m < PER; ++m)
+            if (m < PER - 1 || t + m * NTH < PLANE) sf[b][t + m * NTH] = v[m];
+    };
+    const int tx = t % CX, ty = t / CX;
+    auto xyr = [&](int b, T (&u)[3]) __attribute__((always_inline)) {
+        u[0] = u[1] = u[2] = T(0);
 #pragma unroll
         for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
             for (int dx = 0; dx < 3; ++dx) {
-                const T wt = T((dx == 1 ? 1.0 : 0.5) * (dy == 1 ? 1.0 : 0.5) * (dz == 1 ? 1.0 : 0.5));
-                const T* v = &sf[((2 * tz + dz) * FY + (2 * ty + dy)) * ROW + 3 * (2 * tx + dx)];
-                acc[0] += wt * v[0];
-                acc[1] += wt * v[1];
-                acc[2] += wt * v[2];
+                const T wt = T((dx == 1 ? 1.0 : 0.5) * (dy == 1 ? 1.0 : 0.5));
+                const T* v = &sf[b][(2 * ty + dy) * ROW + 3 * (2 * tx + dx)];
+                u[0] += wt * v[0];
+                u[1] += wt * v[1];
+                u[2] += wt * v[2];
             }
-    const long n = (long)K * gc.NY * gc.NX + (long)J * gc.NX + I;
+    };
+    const int I = I0 + tx, J = J0 + ty;
+    const bool own = I < gc.NX && J < gc.NY;
+    T va[PER], vb[PER], ulo[3];
+    load(2 * K0 - 1, va);
+    put(0, va);
+    load(2 * K0, va);
+    load(2 * K0 + 1, vb);
+    __syncthreads();
+    xyr(0, ulo);
+    __syncthreads();
+    for (int K = K0; K < K1; ++K) {
+        put(0, va);
+        put(1, vb);
+        if (K + 1 < K1) {
+            load(2 * K + 2, va);
+            load(2 * K + 3, vb);
+        }
+        __syncthreads();
+        T um[3], uh[3];
+        xyr(0, um);
+        xyr(1, uh);
+        if (own) {
+            const long n = (long)K * gc.NY * gc.NX + (long)J * gc.NX + I;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        rc[3 * n + c] = acc[c];
-        if constexpr (SCALE) yc[3 * n + c] = w * minvc[3 * n + c] * acc[c];
+            for (int c = 0; c < 3; ++c) {
+                const T acc = T(0.5) * ulo[c] + um[c] + T(0.5) * uh[c];
+                rc[3 * n + c] = acc;
+                if constexpr (SCALE) yc[3 * n + c] = w * minvc[3 * n + c] * acc;
+            }
+        }
+        ulo[0] = uh[0];
+        ulo[1] = uh[1];
+        ulo[2] = uh[2];
+        __syncthreads();
     }
+}
+
+// coarse planes per workgroup of the restriction onto gc: the deepest chunk
+// (at most kRzMaxKc) that still gives kRzMinBlocks workgroups, two per
+// compute unit.  On the 256x128x128 hierarchy's transitions (FP32) this is
+// 5 planes for the first (17-18 us against 23 us for one plane) and one
+// plane for the smaller ones (4.8-6.6 us; each further plane costs a
+// memory round trip there).
+constexpr int kRzMaxKc = 8, kRzMinBlocks = 512;
+inline int restrict_zm_kc(const GridArgs& gc, int ntx, int nty) {
+    int kc = kRzMaxKc;
+    while (kc > 1 && (long)ntx * nty * ((gc.NZ + kc - 1) / kc) < kRzMinBlocks) --kc;
+    return kc;
+}
+
+// rc = R rf onto gc (and, with minvc, yc = w Dc^-1 rc)
+template <typename T>
+void launch_restrict(const GridArgs& gf, const T* rf, const GridArgs& gc, T* rc, const T* minvc, T* yc, T w,
+                     const PcgState* st, hipStream_t s) {
+    const int ntx = (gc.NX + kRzX - 1) / kRzX, nty = (gc.NY + kRzY - 1) / kRzY, kc = restrict_zm_kc(gc, ntx, nty);
+    const dim3 grid(ntx * nty * ((gc.NZ + kc - 1) / kc));
+    if (minvc != nullptr)
+        hipLaunchKernelGGL((k_ml_restrict_zm<T, true>), grid, dim3(kRzX * kRzY), 0, s, gf, rf, gc, rc, minvc, yc, w,
+                           st, ntx, nty, kc);
+    else
+        hipLaunchKernelGGL((k_ml_restrict_zm<T, false>), grid, dim3(kRzX * kRzY), 0, s, gf, rf, gc, rc, minvc, yc, w,
+                           st, ntx, nty, kc);
 }
 
 // interpolation weights of fine index i from the coarse line: corners I0 and
@@ -357,40 +409,51 @@ __global__ __launch_bounds__(kCoarseThreads) void k_ml_coarse(MlHierarchy<T> h, 
 // t = mask(r - A x); r_{l+1} = R t; ...; x += mask(P x_{l+1}); t = A x;
 // x += S (r - t).  Level L: x = A_L^-1 r (exact: one wave per row, lanes
 // over the columns) or D_L^-1 r.  The level operator gathers, per node, the 8
-// elements around it; the fine element stiffness Kb (binary node order) and
-// the smoothed levels' moduli h.Eop[l] are staged in LDS first.
+// elements around it.  Staged in LDS first, while the first restriction
+// reads memory: the fine element stiffness Kb (binary node order), the
+// smoothed levels' moduli h.Eop[l] and smoother diagonals h.sinv[l], and,
+// when it fits the budget (stage_ainv), the exact inverse.
 template <typename T>
-long vc_coarse_lds_values(const MlHierarchy<T>& h, int s0) {
+long vc_coarse_lds_values(const MlHierarchy<T>& h, int s0, bool stage_ainv = false) {
     long v = 576;
-    for (int l = s0; l < h.levels; ++l) v += h.g[l].ne;
+    for (int l = s0; l < h.levels; ++l) v += h.g[l].ne + 3 * h.g[l].nn;
     for (int l = s0; l <= h.levels; ++l) v += 9 * h.g[l].nn;
+    if (stage_ainv) v += 9 * h.g[h.levels].nn * h.g[h.levels].nn;
     return v;
 }
 
 template <typename T>
 __global__ __launch_bounds__(kCoarseThreads) void k_vc_coarse(MlHierarchy<T> h, int s0, const T* __restrict__ Kb,
-                                                              const PcgState* st) {
+                                                              int stage_ainv, const PcgState* st) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     T* lds = reinterpret_cast<T*>(lds_raw);
     if (st != nullptr && st->done) return;
     const int L = h.levels, t = threadIdx.x;
     T* const kb = lds;
-    auto Ev = [&](int l) {  // moduli of smoothed level l
+    auto Ev = [&](int l) {  // moduli and smoother diagonal of smoothed level l
         long o = 576;
-        for (int m = s0; m < l; ++m) o += h.g[m].ne;
+        for (int m = s0; m < l; ++m) o += h.g[m].ne + 3 * h.g[m].nn;
         return lds + o;
     };
+    auto Sv = [&](int l) { return Ev(l) + h.g[l].ne; };
     auto Rv = [&](int l) {
         long o = 576;
-        for (int m = s0; m < L; ++m) o += h.g[m].ne;
+        for (int m = s0; m < L; ++m) o += h.g[m].ne + 3 * h.g[m].nn;
         for (int m = s0; m < l; ++m) o += 9 * h.g[m].nn;
         return lds + o;
     };
+    T* const Av = Rv(L) + 9 * h.g[L].nn;  // the exact inverse, when staged
     auto Xv = [&](int l) { return Rv(l) + 3 * h.g[l].nn; };
     auto Tv = [&](int l) { return Rv(l) + 6 * h.g[l].nn; };
     for (int i = t; i < 576; i += kCoarseThreads) kb[i] = Kb[i];
-    for (int l = s0; l < L; ++l)
+    for (int l = s0; l < L; ++l) {
         for (long e = t; e < h.g[l].ne; e += kCoarseThreads) Ev(l)[e] = h.Eop[l][e];
+        for (long d = t; d < 3 * h.g[l].nn; d += kCoarseThreads) Sv(l)[d] = h.sinv[l][d];
+    }
+    if (stage_ainv) {
+        const long NN = 9 * h.g[L].nn * h.g[L].nn;
+        for (long i = t; i < NN; i += kCoarseThreads) Av[i] = h.ainv[i];
+    }
     {  // restriction of level s0-1's residual (memory) onto level s0
         const GridArgs gf = h.g[s0 - 1], gc = h.g[s0];
         for (long n = t; n < gc.nn; n += kCoarseThreads) {
@@ -404,7 +467,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_vc_coarse(MlHierarchy<T> h, 
     auto apply = [&](int l, const T* x, T* y, bool res) {
         const GridArgs g = h.g[l];
         const T* E = Ev(l);
-        const T* S = h.sinv[l];
+        const T* S = Sv(l);
         const T* r = Rv(l);
         for (long n = t; n < g.nn; n += kCoarseThreads) {
             int i, j, k;
@@ -439,7 +502,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_vc_coarse(MlHierarchy<T> h, 
     };
     for (int l = s0; l < L; ++l) {  // down: pre-smoothing, residual, restriction
         const long nl = 3 * h.g[l].nn;
-        for (long d = t; d < nl; d += kCoarseThreads) Xv(l)[d] = h.sinv[l][d] * Rv(l)[d];
+        for (long d = t; d < nl; d += kCoarseThreads) Xv(l)[d] = Sv(l)[d] * Rv(l)[d];
         __syncthreads();
         apply(l, Xv(l), Tv(l), true);
         __syncthreads();
@@ -456,9 +519,10 @@ __global__ __launch_bounds__(kCoarseThreads) void k_vc_coarse(MlHierarchy<T> h, 
         const T* rl = Rv(L);
         if (h.ainv != nullptr) {  // one wave per row: coalesced columns, then a wave sum
             const int N = (int)nL, lane = t & 63;
+            const T* A = stage_ainv ? Av : h.ainv;
             for (int d = t >> 6; d < N; d += kCoarseThreads / 64) {
                 T acc = T(0);
-                for (int j = lane; j < N; j += 64) acc += h.ainv[(long)d * N + j] * rl[j];
+                for (int j = lane; j < N; j += 64) acc += A[(long)d * N + j] * rl[j];
 #pragma unroll
                 for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
                 if (lane == 0) Xv(L)[d] = acc;
@@ -475,14 +539,14 @@ __global__ __launch_bounds__(kCoarseThreads) void k_vc_coarse(MlHierarchy<T> h, 
             prolong_node(gc, (const T*)Xv(l + 1), gf, n, acc);
             for (int c = 0; c < 3; ++c) {
                 const long d = 3 * n + c;
-                if (h.sinv[l][d] != T(0)) Xv(l)[d] += acc[c];
+                if (Sv(l)[d] != T(0)) Xv(l)[d] += acc[c];
             }
         }
         __syncthreads();
         apply(l, Xv(l), Tv(l), false);
         __syncthreads();
         const long nl = 3 * h.g[l].nn;
-        for (long d = t; d < nl; d += kCoarseThreads) Xv(l)[d] += h.sinv[l][d] * (Rv(l)[d] - Tv(l)[d]);
+        for (long d = t; d < nl; d += kCoarseThreads) Xv(l)[d] += Sv(l)[d] * (Rv(l)[d] - Tv(l)[d]);
         __syncthreads();
     }
     for (long i = t; i < 3 * h.g[s0].nn; i += kCoarseThreads) h.y[s0][i] = Xv(s0)[i];
@@ -748,15 +812,11 @@ void launch_ml_apply(const MlHierarchy<T>& h, const T* r, const T* minv0, T* z, 
         // levels 1..s0-1 (all of them when no level fits the LDS kernel): tiled restriction
         const int last_tiled = s0 > L ? L : s0 - 1;
         for (int l = 1; l <= last_tiled; ++l) {
-            const GridArgs& gc = h.g[l];
-            const int ntx = (gc.NX + kRtX - 1) / kRtX, nty = (gc.NY + kRtY - 1) / kRtY, ntz = (gc.NZ + kRtZ - 1) / kRtZ;
             const T* rf = l == 1 ? r : h.r[l - 1];
             if (l == L)
-                hipLaunchKernelGGL((k_ml_restrict_tiled<T, true>), dim3(ntx * nty * ntz), dim3(NT), 0, s, h.g[l - 1],
-                                   rf, gc, h.r[l], h.minv[l], h.y[l], T(h.w[l]), st, ntx, nty);
+                launch_restrict<T>(h.g[l - 1], rf, h.g[l], h.r[l], h.minv[l], h.y[l], T(h.w[l]), st, s);
             else
-                hipLaunchKernelGGL((k_ml_restrict_tiled<T, false>), dim3(ntx * nty * ntz), dim3(NT), 0, s,
-                                   h.g[l - 1], rf, gc, h.r[l], (const T*)nullptr, (T*)nullptr, T(1), st, ntx, nty);
+                launch_restrict<T>(h.g[l - 1], rf, h.g[l], h.r[l], nullptr, nullptr, T(1), st, s);
         }
         if (last_tiled < L) {
             MlHierarchy<T> hc = h;
@@ -835,15 +895,8 @@ int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const T* Kb, c
         return s2;
     };
     auto restrict_to = [&](int l, bool scale) {  // R vres[l-1] -> r[l] (and y[l] = D_l^-1 r[l])
-        const GridArgs& gc = h.g[l];
-        const int ntx = (gc.NX + kRtX - 1) / kRtX, nty = (gc.NY + kRtY - 1) / kRtY, ntz = (gc.NZ + kRtZ - 1) / kRtZ;
-        if (scale)
-            hipLaunchKernelGGL((k_ml_restrict_tiled<T, true>), dim3(ntx * nty * ntz), dim3(NT), 0, s, h.g[l - 1],
-                               (const T*)h.vres[l - 1], gc, h.r[l], h.minv[l], h.y[l], T(h.w[l]), st, ntx, nty);
-        else
-            hipLaunchKernelGGL((k_ml_restrict_tiled<T, false>), dim3(ntx * nty * ntz), dim3(NT), 0, s, h.g[l - 1],
-                               (const T*)h.vres[l - 1], gc, h.r[l], (const T*)nullptr, (T*)nullptr, T(1), st, ntx,
-                               nty);
+        launch_restrict<T>(h.g[l - 1], (const T*)h.vres[l - 1], h.g[l], h.r[l], scale ? (const T*)h.minv[l] : nullptr,
+                           scale ? h.y[l] : nullptr, T(scale ? h.w[l] : 1), st, s);
     };
     for (int l = 0; l < c; ++l) {  // pre-smoothing x = S r, residual, restriction
         MvArgs<T> a{nullptr, h.vres[l], h.Eop[l], T(0), MV_PLAIN, st, nullptr, 3 * h.g[l].nn, level_r(l), h.sinv[l],
@@ -855,8 +908,10 @@ int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const T* Kb, c
     if (s0 <= L) {  // levels s0..L in one workgroup, from vres[s0-1]
         MlHierarchy<T> hc = h;
         hc.r[s0 - 1] = h.vres[s0 - 1];
+        const bool stage = h.ainv != nullptr &&
+                           vc_coarse_lds_values(h, s0, true) * (long)sizeof(T) <= kCoarseLdsBytes;
         hipLaunchKernelGGL((k_vc_coarse<T>), dim3(1), dim3(kCoarseThreads),
-                           vc_coarse_lds_values(h, s0) * sizeof(T), s, hc, s0, Kb, st);
+                           vc_coarse_lds_values(h, s0, stage) * sizeof(T), s, hc, s0, Kb, stage ? 1 : 0, st);
     } else {  // no level fits it: the coarsest level by its diagonal
         restrict_to(L, true);
     }

@@ -510,11 +510,11 @@ def main():
         if main_form.endswith("_vc"):
             sl = smoothed_levels(nx, ny, nz, 8 if fp64_main else 4)
             by = vc_apply_bytes(nx, ny, nz, 8 if fp64_main else 4, sl)
-            kn = ("k_matvec_sf forms 3 and 4, k_ml_restrict_tiled, k_ml_coarse, k_fin_rz "
+            kn = ("k_matvec_sf forms 3 and 4, k_ml_restrict_zm, k_ml_coarse, k_fin_rz "
                   f"({sl} smoothed levels, one application)")
         else:
             by = ml_apply_bytes(nx, ny, nz, 8 if fp64_main else 4)
-            kn = "k_ml_restrict_tiled, k_ml_coarse, k_ml_prolong, k_fin_rz (one application)"
+            kn = "k_ml_restrict_zm, k_ml_coarse, k_ml_prolong, k_fin_rz (one application)"
         roof_ml = {"kernels": kn,
                    "avg_ms": k["avg_ms"], "bytes_per_apply": by, "achieved": round(by / (k["avg_ms"] * 1e-3) / 1e9, 1),
                    "unit": "GB/s", "frac": round(by / (k["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
