@@ -11,6 +11,8 @@
 // node value of y is written exactly once; x and E are read ~1.2x (halo).
 #include "hx_launch.h"
 
+#include <type_traits>
+
 namespace hx {
 
 namespace {
@@ -371,34 +373,36 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
         [[maybe_unused]] const int cbx = (i0 - 1) < 0 ? -1 : (i0 - 1) >> 1;
         [[maybe_unused]] const int cby = (j0 - 1) < 0 ? -1 : (j0 - 1) >> 1;
         [[maybe_unused]] const int fbase = max(k0 - 1, 0), nf = min(k1, g.NZ - 1) - fbase + 1;
-        // fill: fine plane 2K holds coarse plane K, 2K+1 the mean of K and K+1;
-        // batches of kFillBatch values per thread, every load of a batch issued
-        // before any is waited for (clamped addresses, no stores out of range)
+        // fill: fine plane 2K holds coarse plane K, 2K+1 the mean of K and K+1.
+        // A thread walks coarse columns (one value of the xy footprint) and
+        // loads each coarse plane of the column once, kFillGroup fine planes
+        // at a time (all loads of a group issued before any is waited for,
+        // clamped addresses); the fine planes' parity against the coarse ones
+        // is fixed per chunk, so the register index of every tap is static.
         auto coarse_fill = [&]() __attribute__((always_inline)) {
-            constexpr int kFillBatch = (18 * CVALS + NT - 1) / NT;  // a 16-plane chunk in one batch
-            const int total = nf * CVALS;
-            for (int base = 0; base < total; base += kFillBatch * NT) {
-                T v0[kFillBatch], v1[kFillBatch];
+            constexpr int kFillGroup = 16, kFillK = kFillGroup / 2 + 1;
+            const long pl = 3L * a.cNY * a.cNX;
+            for (int ii = threadIdx.x; ii < CVALS; ii += NT) {
+                const int ly = ii / (3 * CPX), rem = ii - ly * 3 * CPX, lx = rem / 3, c = rem - 3 * lx;
+                const int I = min(max(cbx + lx, 0), a.cNX - 1), J = min(max(cby + ly, 0), a.cNY - 1);
+                const T* col = a.ec + 3 * ((long)J * a.cNX + I) + c;
+                T* dst = czl + (c * CPY + ly) * CPX + lx;
+                for (int fs = 0; fs < nf; fs += kFillGroup) {
+                    const int Kc = (fbase + fs) >> 1;
+                    T v[kFillK];
 #pragma unroll
-                for (int b = 0; b < kFillBatch; ++b) {
-                    const int idx = min(base + b * NT + (int)threadIdx.x, total - 1);
-                    const int f = idx / CVALS, ii = idx - f * CVALS;
-                    const int ly = ii / (3 * CPX), rem = ii - ly * 3 * CPX, lx = rem / 3, c = rem - 3 * lx;
-                    const int I = min(max(cbx + lx, 0), a.cNX - 1), J = min(max(cby + ly, 0), a.cNY - 1);
-                    const int K = (fbase + f) >> 1, K1 = min(K + 1, a.cNZ - 1);
-                    const long col = 3 * ((long)J * a.cNX + I) + c, pl = 3L * a.cNY * a.cNX;
-                    v0[b] = a.ec[K * pl + col];
-                    v1[b] = a.ec[K1 * pl + col];
-                }
+                    for (int q = 0; q < kFillK; ++q) v[q] = col[min(Kc + q, a.cNZ - 1) * pl];
+                    auto emit = [&](auto parity) __attribute__((always_inline)) {
+                        constexpr int pp = decltype(parity)::value;
 #pragma unroll
-                for (int b = 0; b < kFillBatch; ++b) {
-                    const int idx = base + b * NT + (int)threadIdx.x;
-                    if (idx < total) {
-                        const int f = idx / CVALS, ii = idx - f * CVALS;
-                        const int ly = ii / (3 * CPX), rem = ii - ly * 3 * CPX, lx = rem / 3, c = rem - 3 * lx;
-                        czl[f * CVALS + (c * CPY + ly) * CPX + lx] =
-                            ((fbase + f) & 1) ? T(0.5) * (v0[b] + v1[b]) : v0[b];
-                    }
+                        for (int f = 0; f < kFillGroup; ++f) {
+                            const int q = (f + pp) >> 1;
+                            if (fs + f < nf)
+                                dst[(fs + f) * CVALS] = ((f + pp) & 1) ? T(0.5) * (v[q] + v[q + 1]) : v[q];
+                        }
+                    };
+                    if (fbase & 1) emit(std::integral_constant<int, 1>{});
+                    else emit(std::integral_constant<int, 0>{});
                 }
             }
         };
