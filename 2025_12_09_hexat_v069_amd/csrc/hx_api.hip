@@ -885,18 +885,34 @@ int helm_solve(hx_ctx* c, double R, const hx_filter_opts& o, const double* in_de
         }
         return HX_OK;
     }
+    // batches of 16 iterations, replayed as one graph per operator and
+    // buffer set: the operator leaves its p'q partials apart from the
+    // update's, which forms alpha from them (no finaliser between the two)
     const int batch = 16;
+    unsigned long long ah = 1469598103934665603ull;  // FNV-1a of the element matrix
+    for (int t = 0; t < 64; ++t) {
+        unsigned long long bits;
+        std::memcpy(&bits, &A.Ae[t], sizeof bits);
+        for (int q = 0; q < 8; ++q) ah = (ah ^ ((bits >> (8 * q)) & 0xff)) * 1099511628211ull;
+    }
+    char key[192];
+    std::snprintf(key, sizeof key, "helm %d %d %d %llx %p %p %p %p %p", c->g.nx, c->g.ny, c->g.nz, ah,
+                  (void*)c->h_x.p, (void*)c->h_r.p, (void*)c->h_p.p, (void*)c->h_q.p, (void*)c->h_minv.p);
+    double* const op_partials = c->partials.p + kOpPartials;
     while (!c->st_host->done) {
-        for (int b = 0; b < batch; ++b) {
-            {
-                TimeScope ts(c, HX_K_HELM);
-                launch_helm_matvec(c->g, A, c->h_p.p, c->h_q.p, MV_PCG, c->st, c->partials.p, c->stream);
+        rc = run_batch(c, key, [&] {
+            for (int b = 0; b < batch; ++b) {
+                int npq;
+                {
+                    TimeScope ts(c, HX_K_HELM);
+                    npq = launch_helm_matvec(c->g, A, c->h_p.p, c->h_q.p, MV_PCG, c->st, op_partials, c->stream);
+                }
+                launch_pcg_update<double>(nn, c->h_x.p, c->h_p.p, c->h_r.p, c->h_q.p, c->h_minv.p, c->st,
+                                          c->partials.p, c->stream, 1, true, op_partials, npq);
+                launch_pcg_pupdate<double>(nn, c->h_r.p, c->h_minv.p, c->h_p.p, c->st, c->stream);
             }
-            launch_pcg_update<double>(nn, c->h_x.p, c->h_p.p, c->h_r.p, c->h_q.p, c->h_minv.p, c->st, c->partials.p,
-                                      c->stream);
-            launch_pcg_pupdate<double>(nn, c->h_r.p, c->h_minv.p, c->h_p.p, c->st, c->stream);
-        }
-        HXK();
+        });
+        if (rc) return rc;
         if ((rc = sync_state(c))) return rc;
     }
     launch_helm_extract(c->g, c->h_x.p, out_dev, c->stream);

@@ -934,12 +934,12 @@ int helm_matvec_blocks(const GridArgs& g) {
     int a, b;
     return helm_tiles(g, &a, &b);
 }
-void launch_helm_matvec(const GridArgs& g, const HelmArgs& A, const double* x, double* y, int mode, PcgState* st,
-                        double* partials, hipStream_t s) {
+int launch_helm_matvec(const GridArgs& g, const HelmArgs& A, const double* x, double* y, int mode, PcgState* st,
+                       double* partials, hipStream_t s) {
     int ntx, nty;
     const int nb = helm_tiles(g, &ntx, &nty);
     hipLaunchKernelGGL(k_helm_matvec, dim3(nb), dim3(NT), 0, s, g, A, x, y, mode, st, partials, ntx, nty);
-    if (mode == MV_PCG) launch_fin_alpha(st, partials, nb, 1, s);
+    return nb;
 }
 void launch_fin_alpha(PcgState* st, const double* partials, int nblocks, int nrhs, hipStream_t s) {
     hipLaunchKernelGGL(k_fin_alpha, dim3(nrhs), dim3(NT), 0, s, st, partials, (unsigned)nblocks);

@@ -216,8 +216,10 @@ struct HelmArgs {
 };
 void launch_helm_rhs(const GridArgs& g, const double* in_e, double vol8, double* b, hipStream_t s);
 void launch_helm_minv(const GridArgs& g, const HelmArgs& A, double* minv, hipStream_t s);
-void launch_helm_matvec(const GridArgs& g, const HelmArgs& A, const double* x, double* y, int mode, PcgState* st,
-                        double* partials, hipStream_t s);
+// y = A x; in MV_PCG mode also the per-block p'q partials (one per block,
+// the returned count), from which the update forms alpha
+int launch_helm_matvec(const GridArgs& g, const HelmArgs& A, const double* x, double* y, int mode, PcgState* st,
+                       double* partials, hipStream_t s);
 int helm_matvec_blocks(const GridArgs& g);
 void launch_helm_extract(const GridArgs& g, const double* xn, double* out_e, hipStream_t s);
 
