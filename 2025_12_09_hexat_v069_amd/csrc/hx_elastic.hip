@@ -587,7 +587,7 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
                 }
                 if constexpr (FUSE == 4) {
                     if (ke >= k0 && own) pq += (double)ro[c] * (double)out;
-                } else {
+                } else if constexpr (FUSE != 3) {  // pre-smoothing reduces nothing
                     if (ke >= k0 && own) pq += (double)xo[c] * (double)yv;
                 }
                 xo[c] = xnew[c];
@@ -630,7 +630,7 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
         flush();
     }
 
-    if (a.mode == MV_PCG || (FUSE == 4 && a.partials != nullptr)) {  // per-block p'q (k_fin_alpha), or r'y
+    if (FUSE != 3 && (a.mode == MV_PCG || (FUSE == 4 && a.partials != nullptr))) {  // per-block p'q, or r'y
         const double v = block_sum<NT>(pq, red);
         if (threadIdx.x == 0) a.partials[blockIdx.x] = v;
     }
