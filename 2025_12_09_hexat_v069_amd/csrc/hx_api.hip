@@ -496,13 +496,19 @@ MvShape vc_shape(const hx_ctx* c, bool f32, bool post) {
     return s.kchunk > 0 ? s : (f32 ? c->shape32 : c->shape64);
 }
 
+// The V-cycle PCG updates x and r without the mask (launch_pcg_update_free)
+// and takes r'r from the fine post-smoothing operator, given a coarse level
+bool vc_free_update(const hx_ctx* c, bool vcycle, bool f32) {
+    return vcycle && (f32 ? c->ml32.levels : c->ml64.levels) >= 1;
+}
+
 // z = M^-1 r of the explicit preconditioner into ml_z / ml_z32 (fin_mode as
 // launch_ml_apply's)
 int pc_apply64(hx_ctx* c, bool vcycle, const double* r, int fin_mode) {
     if (vcycle) {
         if (launch_vc_apply<double>(c->ml64, c->kp64, c->Kbin_d.p, vc_shape(c, false, false),
                                     vc_shape(c, false, true), r, c->ml_z.p, c->st, c->partials.p, kOpPartials,
-                                    fin_mode, c->stream) != HX_OK)
+                                    fin_mode, c->stream, vc_free_update(c, true, false)) != HX_OK)
             return fail(c, HX_ERR_BAD_ARG, "grid too large for the V-cycle's partial sums");
     } else {
         launch_ml_apply<double>(c->ml64, r, c->minv.p, c->ml_z.p, c->st, c->partials.p, fin_mode, c->stream);
@@ -513,7 +519,7 @@ int pc_apply32(hx_ctx* c, bool vcycle, const float* r, int fin_mode) {
     if (vcycle) {
         if (launch_vc_apply<float>(c->ml32, c->kp32, c->Kbin32_d.p, vc_shape(c, true, false),
                                    vc_shape(c, true, true), r, c->ml_z32.p, c->st, c->partials.p, kOpPartials,
-                                   fin_mode, c->stream) != HX_OK)
+                                   fin_mode, c->stream, vc_free_update(c, true, true)) != HX_OK)
             return fail(c, HX_ERR_BAD_ARG, "grid too large for the V-cycle's partial sums");
     } else {
         launch_ml_apply<float>(c->ml32, r, c->minv32.p, c->ml_z32.p, c->st, c->partials.p, fin_mode, c->stream);
@@ -535,6 +541,7 @@ int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& 
     const bool ml = o.precond == HX_PC_MULTILEVEL || vc;
     int rc = ml ? ml_setup(c, false, vc) : HX_OK;
     if (rc) return rc;
+    const bool free_upd = vc_free_update(c, vc, false);
     launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream, nr);
     if (o.warm_start) launch_mask<double>(n, c->minv.p, v.x, c->stream, nr);
     else launch_fill(n * nr, 0.0, v.x, c->stream);
@@ -544,7 +551,7 @@ int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& 
     HXK();
     if ((rc = sync_state(c, nr))) return rc;
     const int mvv = ml ? HX_MV_PARITY : o.matvec;
-    const std::string key = batch_key(c, vc ? "f64vc" : ml ? "f64ml" : "f64", nr, batch,
+    const std::string key = batch_key(c, vc ? (free_upd ? "f64vcf" : "f64vc") : ml ? "f64ml" : "f64", nr, batch,
                                       mvv == HX_MV_AUTO ? c->mv_variant : mvv, o.shift, v.x, v.p, v.q, v.r, v.p2);
     // p is double-buffered (v.p, v.p2): each matvec forms p_new = D^-1 r + beta p_old
     // from the plane it stages anyway and writes it beside q; batches are even.
@@ -559,8 +566,12 @@ int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& 
                 else mv64(c, pin, v.q, c->E.p, o.shift, MV_PCG, o.matvec, nr, v.r, c->minv.p, pout, true, &npq);
                 {
                     TimeScope ts(c, HX_K_UPDATE);
-                    launch_pcg_update<double>(n, v.x, pout, v.r, v.q, c->minv.p, c->st, c->partials.p, c->stream,
-                                              nr, !ml, c->partials.p + kOpPartials, npq);
+                    if (free_upd)
+                        launch_pcg_update_free<double>(n, v.x, pout, v.r, v.q, c->st, c->partials.p + kOpPartials,
+                                                       npq, c->stream);
+                    else
+                        launch_pcg_update<double>(n, v.x, pout, v.r, v.q, c->minv.p, c->st, c->partials.p,
+                                                  c->stream, nr, !ml, c->partials.p + kOpPartials, npq);
                 }
                 if (ml) {
                     TimeScope ts(c, HX_K_PRECOND);
@@ -621,6 +632,7 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
     const bool vc = o.precond == HX_PC_VCYCLE;
     const bool ml = o.precond == HX_PC_MULTILEVEL || vc;  // explicit z = M^-1 r (see pcg_fp64)
     if (ml && (rc = ml_setup(c, true, vc))) return rc;
+    const bool free_upd = vc_free_update(c, vc, true);
     launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream, nr);
     if (o.warm_start) launch_mask<double>(n, c->minv.p, v.x, c->stream, nr);
     else launch_fill(n * nr, 0.0, v.x, c->stream);
@@ -634,8 +646,8 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
     HXK();
     if ((rc = sync_state(c, nr))) return rc;
     int refinements = 0;
-    const std::string key = batch_key(c, vc ? "f32vc" : ml ? "f32ml" : "f32", nr, batch, ml ? HX_MV_PARITY : v32, o.shift, v.x32,
-                                      v.p32, v.q32, v.r32, v.p32b);
+    const std::string key = batch_key(c, vc ? (free_upd ? "f32vcf" : "f32vc") : ml ? "f32ml" : "f32", nr, batch,
+                                      ml ? HX_MV_PARITY : v32, o.shift, v.x32, v.p32, v.q32, v.r32, v.p32b);
     while (!all_done(c, nr)) {
         while (!all_done(c, nr)) {
             rc = run_batch(c, key, [&] {
@@ -650,8 +662,13 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
                         mv32(c, pin, v.q32, c->E32.p, o.shift, MV_PCG, v32, nr, v.r32, c->minv32.p, pout, true, &npq);
                     {
                         TimeScope ts(c, HX_K_UPDATE);
-                        launch_pcg_update<float>(n, v.x32, pout, v.r32, v.q32, c->minv32.p, c->st, c->partials.p,
-                                                 c->stream, nr, !ml, c->partials.p + kOpPartials, npq);
+                        if (free_upd)
+                            launch_pcg_update_free<float>(n, v.x32, pout, v.r32, v.q32, c->st,
+                                                          c->partials.p + kOpPartials, npq, c->stream);
+                        else
+                            launch_pcg_update<float>(n, v.x32, pout, v.r32, v.q32, c->minv32.p, c->st,
+                                                     c->partials.p, c->stream, nr, !ml, c->partials.p + kOpPartials,
+                                                     npq);
                     }
                     if (ml) {
                         TimeScope ts(c, HX_K_PRECOND);

@@ -299,7 +299,8 @@ __device__ __forceinline__ double lane_from_left(double v) {
 // 4 (post-smoothing) x = S r + mask(P e), the coarse correction e (a.ec, node
 //   grid cNX x cNY x cNZ) interpolated trilinearly where S != 0, then
 //   y = x + S (r - K x), with the per-block partial sums of r'y over the owned
-//   nodes when a.partials is set.  At its start a workgroup interpolates e in
+//   nodes when a.partials is set (a.partials[b]), and of r'r over the owned
+//   DOFs with S != 0 (a.partials[gridDim.x + b]).  At its start a workgroup interpolates e in
 //   z onto every fine plane of its chunk over the tile's coarse footprint, in
 //   dynamic LDS ((kchunk + 2) planes, smoothing_post_kchunk); the march then
 //   reads the four xy corners of each staged node.
@@ -348,6 +349,7 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
     const bool active = tile < (unsigned)ntiles;
     const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
     double pq = 0.0;
+    [[maybe_unused]] double rr = 0.0;  // form 4: r'r over the solved owned DOFs
 
     if (active) {
         const int cx = tile % ntx, cy = (tile / ntx) % nty, cz = tile / (ntx * nty);
@@ -586,7 +588,10 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
                     if constexpr (POUT) so[POUT ? 1 : 0][so_i + c] = xo[c];
                 }
                 if constexpr (FUSE == 4) {
-                    if (ke >= k0 && own) pq += (double)ro[c] * (double)out;
+                    if (ke >= k0 && own) {
+                        pq += (double)ro[c] * (double)out;
+                        if (mo[c] != T(0)) rr += (double)ro[c] * (double)ro[c];
+                    }
                 } else if constexpr (FUSE != 3) {  // pre-smoothing reduces nothing
                     if (ke >= k0 && own) pq += (double)xo[c] * (double)yv;
                 }
@@ -633,6 +638,10 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
     if (FUSE != 3 && (a.mode == MV_PCG || (FUSE == 4 && a.partials != nullptr))) {  // per-block p'q, or r'y
         const double v = block_sum<NT>(pq, red);
         if (threadIdx.x == 0) a.partials[blockIdx.x] = v;
+        if constexpr (FUSE == 4) {  // and r'r after all blocks' r'y
+            const double w = block_sum<NT>(rr, red);
+            if (threadIdx.x == 0) a.partials[gridDim.x + blockIdx.x] = w;
+        }
     }
 }
 

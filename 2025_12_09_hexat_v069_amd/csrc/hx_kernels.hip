@@ -111,6 +111,29 @@ __device__ void pcg_after_update(PcgState* st, double rz_new, double rr, bool se
     st->rz = rz_new;
 }
 
+// alpha of an update: with npq > 0, rz / p'q from the operator's partials,
+// reduced in every block (same order, same value), block 0 recording p'q and
+// alpha or the breakdown (NaN returned); else the alpha already in the state
+__device__ double update_alpha(PcgState* st, const double* pq_partials, unsigned npq, double* red, double& s_alpha) {
+    if (npq == 0) return st->alpha;
+    const double pq = reduce_partials<NT>(pq_partials, 1, 0, npq, red);
+    if (threadIdx.x == 0) {
+        const bool ok = fabs(pq) >= 1e-300;  // also catches NaN (GPUSolver.jl:462-466)
+        s_alpha = ok ? st->rz / pq : 0.0;
+        if (blockIdx.x == 0) {
+            st->pq = pq;
+            if (ok) st->alpha = s_alpha;
+            else {
+                st->done = 1;
+                st->status = 4;
+            }
+        }
+        if (!ok) s_alpha = NAN;
+    }
+    __syncthreads();
+    return s_alpha;
+}
+
 // x += alpha p ; r = free ? r - alpha q : 0 ; rz = r'(minv r), rr = r'r
 template <typename T>
 __global__ __launch_bounds__(NT) void k_pcg_update(long n, T* __restrict__ x, const T* __restrict__ p,
@@ -126,28 +149,8 @@ __global__ __launch_bounds__(NT) void k_pcg_update(long n, T* __restrict__ x, co
         partials += (long)blockIdx.y * gridDim.x * 2;
     }
     if (st->done) return;
-    double alpha;
-    if (npq > 0) {  // alpha = rz / p'q from the operator's partials, in every block (same order, same value)
-        const double pq = reduce_partials<NT>(pq_partials + (size_t)blockIdx.y * npq, 1, 0, npq, red);
-        if (threadIdx.x == 0) {
-            const bool ok = fabs(pq) >= 1e-300;  // also catches NaN (GPUSolver.jl:462-466)
-            s_alpha = ok ? st->rz / pq : 0.0;
-            if (blockIdx.x == 0) {
-                st->pq = pq;
-                if (ok) st->alpha = s_alpha;
-                else {
-                    st->done = 1;
-                    st->status = 4;
-                }
-            }
-            if (!ok) s_alpha = NAN;
-        }
-        __syncthreads();
-        alpha = s_alpha;
-        if (alpha != alpha) return;  // breakdown: block 0 has recorded it
-    } else {
-        alpha = st->alpha;
-    }
+    const double alpha = update_alpha(st, pq_partials + (size_t)blockIdx.y * npq, npq, red, s_alpha);
+    if (alpha != alpha) return;  // breakdown: block 0 has recorded it
     double rz = 0.0, rr = 0.0;
     auto one = [&](T& xi, T pi, T& ri, T qi, T mi) {
         const double m = (double)mi;
@@ -214,6 +217,48 @@ __global__ __launch_bounds__(NT) void k_fin_update(PcgState* st0, const double* 
     reduce_partials_n<NT, 2>(partials, 2, nb, red, sv);
     const double s0 = sv[0], s1 = sv[1];
     if (threadIdx.x == 0) pcg_after_update(st, s0, s1, set_beta != 0);
+}
+
+// x += alpha p ; r -= alpha q, unmasked and with no reductions, for the
+// V-cycle preconditioner: its smoother S is zero exactly where minv is
+// (fixed DOFs, DOFs without stiffness), it never reads r there, and its
+// post-smoothing operator forms r'r over the other DOFs.  r at those DOFs
+// then holds the change of the reaction, K (x - x0), bounded by the loads.
+// Free DOFs get the bits of k_pcg_update.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_pcg_update_free(long n, T* __restrict__ x, const T* __restrict__ p,
+                                                        T* __restrict__ r, const T* __restrict__ q, PcgState* st,
+                                                        const double* __restrict__ pq_partials, unsigned npq) {
+    __shared__ double red[NT / 64];
+    __shared__ double s_alpha;
+    if (st->done) return;
+    const double alpha = update_alpha(st, pq_partials, npq, red, s_alpha);
+    if (alpha != alpha) return;
+    auto one = [&](T& xi, T pi, T& ri, T qi) {
+        xi = T((double)xi + alpha * (double)pi);
+        ri = T((double)ri - alpha * (double)qi);
+    };
+    typedef typename Vec16<T>::type VT;
+    constexpr int V = Vec16<T>::n;
+    const long nv = n / V;
+    if (aligned16(x) && aligned16(p) && aligned16(r) && aligned16(q)) {
+        for (long v = gtid(); v < nv; v += gstride()) {
+            VT xv = ((const VT*)x)[v], rv = ((const VT*)r)[v];
+            const VT pv = ((const VT*)p)[v], qv = ((const VT*)q)[v];
+#pragma unroll
+            for (int l = 0; l < V; ++l) {
+                T xi = xv[l], ri = rv[l];
+                one(xi, pv[l], ri, qv[l]);
+                xv[l] = xi;
+                rv[l] = ri;
+            }
+            ((VT*)x)[v] = xv;
+            ((VT*)r)[v] = rv;
+        }
+    } else {
+        for (long i = gtid(); i < nv * V; i += gstride()) one(x[i], p[i], r[i], q[i]);
+    }
+    for (long i = nv * V + gtid(); i < n; i += gstride()) one(x[i], p[i], r[i], q[i]);
 }
 
 // p = minv r + beta p
@@ -876,6 +921,12 @@ void launch_pcg_update(long n, T* x, const T* p, T* r, const T* q, const T* minv
         hipLaunchKernelGGL(k_fin_update, dim3(nrhs), dim3(NT), 0, s, st, partials, (unsigned)kVecBlocks, 1);
 }
 template <typename T>
+void launch_pcg_update_free(long n, T* x, const T* p, T* r, const T* q, PcgState* st, const double* pq_partials,
+                            int npq, hipStream_t s) {
+    hipLaunchKernelGGL((k_pcg_update_free<T>), dim3(kVecBlocks), dim3(NT), 0, s, n, x, p, r, q, st, pq_partials,
+                       (unsigned)(pq_partials ? npq : 0));
+}
+template <typename T>
 void launch_pcg_pupdate(long n, const T* r, const T* minv, T* p, const PcgState* st, hipStream_t s, int nrhs) {
     hipLaunchKernelGGL((k_pcg_pupdate<T>), dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, r, minv, p, st);
 }
@@ -1000,6 +1051,10 @@ template void launch_pcg_update<double>(long, double*, const double*, double*, c
                                         PcgState*, double*, hipStream_t, int, bool, const double*, int);
 template void launch_pcg_update<float>(long, float*, const float*, float*, const float*, const float*, PcgState*,
                                        double*, hipStream_t, int, bool, const double*, int);
+template void launch_pcg_update_free<double>(long, double*, const double*, double*, const double*, PcgState*,
+                                             const double*, int, hipStream_t);
+template void launch_pcg_update_free<float>(long, float*, const float*, float*, const float*, PcgState*,
+                                            const double*, int, hipStream_t);
 template void launch_pcg_pupdate<double>(long, const double*, const double*, double*, const PcgState*, hipStream_t,
                                          int);
 template void launch_pcg_pupdate<float>(long, const float*, const float*, float*, const PcgState*, hipStream_t, int);

@@ -98,6 +98,11 @@ template <typename T>
 void launch_pcg_update(long n, T* x, const T* p, T* r, const T* q, const T* minv, PcgState* st,
                        double* partials, hipStream_t s, int nrhs = 1, bool set_beta = true,
                        const double* pq_partials = nullptr, int npq = 0);
+// x += alpha p ; r -= alpha q without the mask or the r'z, r'r partials (the
+// V-cycle's post-smoothing operator forms r'r); one right-hand side
+template <typename T>
+void launch_pcg_update_free(long n, T* x, const T* p, T* r, const T* q, PcgState* st, const double* pq_partials,
+                            int npq, hipStream_t s);
 template <typename T>
 void launch_pcg_pupdate(long n, const T* r, const T* minv, T* p, const PcgState* st, hipStream_t s, int nrhs = 1);
 void launch_pcg_reset(PcgState* st, double tol, double shift, int max_iter, int check_every, hipStream_t s,
@@ -206,9 +211,12 @@ void launch_vc_scale_E(long ne, const double* Ec, double h, T* Eop, hipStream_t 
 // post-smoothing block) go after the update's and the finaliser runs as in
 // launch_ml_apply.  HX_ERR_BAD_ARG when those partials would pass max_partials.
 template <typename T>
+// rr_post: the finaliser takes r'r from the fine post-smoothing operator
+// (after a launch_pcg_update_free) instead of the update's partials; needs a
+// coarse level
 int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const T* Kb, const MvShape& sh_pre,
                     const MvShape& sh_post, const T* r, T* z, PcgState* st, double* partials, long max_partials,
-                    int fin_mode, hipStream_t s);
+                    int fin_mode, hipStream_t s, bool rr_post = false);
 
 // ---- Helmholtz filter --------------------------------------------------------
 struct HelmArgs {

@@ -556,13 +556,14 @@ __global__ __launch_bounds__(kCoarseThreads) void k_vc_coarse(MlHierarchy<T> h, 
 // update's partials (r'r at index 2b+1 of nb_upd blocks), beta = rz/rz_old,
 // rz = rz, then the convergence decision; 1 (start): rz = rz, beta = 0;
 // 2: park rz in scal[5] (reliable update)
+// r'r (mode 0): rr_partials[j * rr_stride], j < nb_rr
 __global__ __launch_bounds__(NT) void k_fin_rz(PcgState* st, const double* partials, unsigned nb,
-                                               const double* upd_partials, unsigned nb_upd, int mode) {
+                                               const double* rr_partials, int rr_stride, unsigned nb_rr, int mode) {
     __shared__ double red[NT / 64];
     if (mode != 2 && st->done) return;  // parking follows a finished phase
     const double s = reduce_partials<NT>(partials, 1, 0, nb, red);
     double rr = 0.0;
-    if (mode == 0) rr = reduce_partials<NT>(upd_partials, 2, 1, nb_upd, red);
+    if (mode == 0) rr = reduce_partials<NT>(rr_partials, rr_stride, 0, nb_rr, red);
     if (threadIdx.x == 0) {
         if (mode == 0) {
             pcg_record(st, rr);
@@ -831,8 +832,8 @@ void launch_ml_apply(const MlHierarchy<T>& h, const T* r, const T* minv0, T* z, 
         hipLaunchKernelGGL((k_ml_prolong<T, true>), dim3(kProlongBlocks), dim3(NT), 0, s, h.g[1], h.y[1], h.g[0], r,
                            minv0, T(1), z, st, partials);
     }
-    hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(NT), 0, s, st_, partials, (unsigned)kProlongBlocks, upd_partials,
-                       (unsigned)kVecBlocks, fin_mode);
+    hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(NT), 0, s, st_, partials, (unsigned)kProlongBlocks,
+                       (const double*)upd_partials + 1, 2, (unsigned)kVecBlocks, fin_mode);
 }
 
 int ml_partial_blocks() { return kProlongBlocks; }
@@ -872,8 +873,9 @@ void launch_vc_scale_E(long ne, const double* Ec, double h, T* Eop, hipStream_t 
 template <typename T>
 int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const T* Kb, const MvShape& sh_pre,
                     const MvShape& sh_post, const T* r, T* z, PcgState* st_, double* partials, long max_partials,
-                    int fin_mode, hipStream_t s) {
+                    int fin_mode, hipStream_t s, bool rr_post) {
     const int L = h.levels;
+    if (rr_post && L == 0) return HX_ERR_BAD_ARG;  // no post-smoothing operator to form r'r
     if (L == 0) {  // no coarse level: plain Jacobi (the additive form's path)
         launch_ml_apply<T>(h, r, h.minv[0], z, st_, partials, fin_mode, s);
         return HX_OK;
@@ -884,7 +886,7 @@ int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const T* Kb, c
     double* const upd_partials = partials;
     double* const rz_partials = partials + 2 * kVecBlocks;
     const int kc0 = smoothing_post_kchunk(sh_post.tile, sh_post.kchunk, (int)sizeof(T));  // as the launch runs it
-    if (2L * kVecBlocks + matvec_parity_blocks(h.g[0], sh_post.tile, kc0) > max_partials) return HX_ERR_BAD_ARG;
+    if (2L * kVecBlocks + 2L * matvec_parity_blocks(h.g[0], sh_post.tile, kc0) > max_partials) return HX_ERR_BAD_ARG;
     auto level_r = [&](int l) { return l == 0 ? r : (const T*)h.r[l]; };
     // coarse levels: shallower z chunks until the launch has kVcMinBlocks
     // workgroups (or one plane each), so the small grids are not latency-bound
@@ -927,8 +929,14 @@ int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const T* Kb, c
         const int nb = launch_matvec_parity<T>(h.g[l], K, a, level_shape(l, sh_post), s, 1);
         if (l == 0) nb0 = nb;
     }
-    hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(NT), 0, s, st_, rz_partials, (unsigned)nb0, upd_partials,
-                       (unsigned)kVecBlocks, fin_mode);
+    // r'r from the update's partials (unmasked update) or from the fine
+    // post-smoothing operator's, after its r'z (free update)
+    if (rr_post)
+        hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(NT), 0, s, st_, rz_partials, (unsigned)nb0,
+                           (const double*)rz_partials + nb0, 1, (unsigned)nb0, fin_mode);
+    else
+        hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(NT), 0, s, st_, rz_partials, (unsigned)nb0,
+                           (const double*)upd_partials + 1, 2, (unsigned)kVecBlocks, fin_mode);
     return HX_OK;
 }
 
@@ -982,9 +990,9 @@ template void launch_vc_scale_E<double>(long, const double*, double, double*, hi
 template void launch_vc_scale_E<float>(long, const double*, double, float*, hipStream_t);
 template int launch_vc_apply<double>(const MlHierarchy<double>&, const ParityK<double>&, const double*,
                                      const MvShape&, const MvShape&, const double*, double*, PcgState*, double*,
-                                     long, int, hipStream_t);
+                                     long, int, hipStream_t, bool);
 template int launch_vc_apply<float>(const MlHierarchy<float>&, const ParityK<float>&, const float*, const MvShape&,
                                     const MvShape&, const float*, float*, PcgState*, double*, long, int,
-                                    hipStream_t);
+                                    hipStream_t, bool);
 
 }  // namespace hx
