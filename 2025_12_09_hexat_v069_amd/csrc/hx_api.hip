@@ -136,6 +136,7 @@ struct hx_ctx {
     // V-cycle form: smoother weight, per smoothed level the smoother diagonal,
     // residual and operator moduli
     double vc_omega = 1.95;
+    int vc_min_blocks = 512;  // coarse V-cycle levels' operator workgroups (two per compute unit)
     // FP32 form: reliable update each time a column's recursive residual falls by this factor
     double ir_delta = kIrDelta;
     DevBuf<double> vc_s[kMlMaxLevels + 1], vc_res[kMlMaxLevels + 1], vc_E[kMlMaxLevels + 1];
@@ -468,6 +469,7 @@ int ml_setup(hx_ctx* c, bool f32, bool vcycle = false) {
                     HXA(c->vc_E32[l], gl.ne);
                     launch_vc_scale_E<float>(gl.ne, El, hl, c->vc_E32[l].p, c->stream);
                 }
+                h32.min_blocks = c->vc_min_blocks;
                 h32.sinv[l] = c->vc_s32[l].p;
                 h32.vres[l] = c->vc_res32[l].p;
                 h32.Eop[l] = l == 0 ? c->E32.p : c->vc_E32[l].p;
@@ -480,6 +482,7 @@ int ml_setup(hx_ctx* c, bool f32, bool vcycle = false) {
                     HXA(c->vc_E[l], gl.ne);
                     launch_vc_scale_E<double>(gl.ne, El, hl, c->vc_E[l].p, c->stream);
                 }
+                h.min_blocks = c->vc_min_blocks;
                 h.sinv[l] = c->vc_s[l].p;
                 h.vres[l] = c->vc_res[l].p;
                 h.Eop[l] = l == 0 ? c->E.p : c->vc_E[l].p;
@@ -1023,6 +1026,10 @@ int hx_ctx_create(int device, hx_ctx** out) {
     if (const char* ev = std::getenv("HEXAT_VC_OMEGA")) {  // V-cycle smoother weight, 0 < omega < 2
         const double w = std::atof(ev);
         if (w > 0.0 && w < 2.0) c->vc_omega = w;
+    }
+    if (const char* ev = std::getenv("HEXAT_VC_MINBLOCKS")) {  // coarse V-cycle levels' launch width
+        const int b = std::atoi(ev);
+        if (b >= 1) c->vc_min_blocks = b;
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc((void**)&c->st, kMaxRhs * sizeof(PcgState)) != hipSuccess ||

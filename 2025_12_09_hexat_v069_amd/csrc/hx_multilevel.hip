@@ -736,7 +736,6 @@ __global__ __launch_bounds__(NT) void k_vc_scale_E(long ne, const double* __rest
 }
 
 constexpr int kProlongBlocks = 1024;  // fine-level blocks (multiple of 8)
-constexpr int kVcMinBlocks = 512;     // operator workgroups a coarse V-cycle level aims for
 
 inline unsigned blocks_for(long n) { return (unsigned)((n + NT - 1) / NT); }
 
@@ -888,12 +887,12 @@ int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const T* Kb, c
     const int kc0 = smoothing_post_kchunk(sh_post.tile, sh_post.kchunk, (int)sizeof(T));  // as the launch runs it
     if (2L * kVecBlocks + 2L * matvec_parity_blocks(h.g[0], sh_post.tile, kc0) > max_partials) return HX_ERR_BAD_ARG;
     auto level_r = [&](int l) { return l == 0 ? r : (const T*)h.r[l]; };
-    // coarse levels: shallower z chunks until the launch has kVcMinBlocks
+    // coarse levels: shallower z chunks until the launch has h.min_blocks
     // workgroups (or one plane each), so the small grids are not latency-bound
     auto level_shape = [&](int l, const MvShape& sh) {
         MvShape s2 = sh;
         if (l > 0)
-            while (s2.kchunk > 1 && matvec_parity_blocks(h.g[l], s2.tile, s2.kchunk) < kVcMinBlocks) s2.kchunk /= 2;
+            while (s2.kchunk > 1 && matvec_parity_blocks(h.g[l], s2.tile, s2.kchunk) < h.min_blocks) s2.kchunk /= 2;
         return s2;
     };
     auto restrict_to = [&](int l, bool scale) {  // R vres[l-1] -> r[l] (and y[l] = D_l^-1 r[l])
