@@ -445,15 +445,26 @@ __global__ __launch_bounds__(kCoarseThreads) void k_vc_coarse(MlHierarchy<T> h, 
     T* const Av = Rv(L) + 9 * h.g[L].nn;  // the exact inverse, when staged
     auto Xv = [&](int l) { return Rv(l) + 3 * h.g[l].nn; };
     auto Tv = [&](int l) { return Rv(l) + 6 * h.g[l].nn; };
-    for (int i = t; i < 576; i += kCoarseThreads) kb[i] = Kb[i];
+    // copies into LDS, kStageBatch loads per thread in flight before any
+    // store; indices clamped on both sides (surplus lanes rewrite the last
+    // value with itself), so no load sits under a branch of its own -- a
+    // plain copy loop waits for every load before the next
+    constexpr int kStageBatch = 8;
+    auto stage = [&](T* dst, const T* src, long cnt) __attribute__((always_inline)) {
+        for (long base = 0; base < cnt; base += (long)kStageBatch * kCoarseThreads) {
+            T v[kStageBatch];
+#pragma unroll
+            for (int b = 0; b < kStageBatch; ++b) v[b] = src[min(base + b * kCoarseThreads + t, cnt - 1)];
+#pragma unroll
+            for (int b = 0; b < kStageBatch; ++b) dst[min(base + b * kCoarseThreads + t, cnt - 1)] = v[b];
+        }
+    };
+    stage(kb, Kb, 576);
     for (int l = s0; l < L; ++l) {
-        for (long e = t; e < h.g[l].ne; e += kCoarseThreads) Ev(l)[e] = h.Eop[l][e];
-        for (long d = t; d < 3 * h.g[l].nn; d += kCoarseThreads) Sv(l)[d] = h.sinv[l][d];
+        stage(Ev(l), h.Eop[l], h.g[l].ne);
+        stage(Sv(l), h.sinv[l], 3 * h.g[l].nn);
     }
-    if (stage_ainv) {
-        const long NN = 9 * h.g[L].nn * h.g[L].nn;
-        for (long i = t; i < NN; i += kCoarseThreads) Av[i] = h.ainv[i];
-    }
+    if (stage_ainv) stage(Av, h.ainv, 9 * h.g[L].nn * h.g[L].nn);
     {  // restriction of level s0-1's residual (memory) onto level s0
         const GridArgs gf = h.g[s0 - 1], gc = h.g[s0];
         for (long n = t; n < gc.nn; n += kCoarseThreads) {
