@@ -699,17 +699,26 @@ __global__ __launch_bounds__(256) void k_elastic_minv(GridArgs g, double kd0, do
     const long n = (long)blockIdx.x * 256 + threadIdx.x;
     if (n >= g.nn) return;
     const int i = (int)(n % g.NX), j = (int)((n / g.NX) % g.NY), k = (int)(n / ((long)g.NX * g.NY));
-    double S = 0.0;
+    double S = 0.0;  // out-of-grid elements add the zero word (same sum, every load in flight)
+#pragma unroll
     for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
         for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
             for (int dx = 0; dx < 2; ++dx) {
                 const int ie = i - 1 + dx, je = j - 1 + dy, ke = k - 1 + dz;
-                if (ie < 0 || je < 0 || ke < 0 || ie >= g.nx || je >= g.ny || ke >= g.nz) continue;
-                S += (double)E[(long)ke * g.nx * g.ny + (long)je * g.nx + ie];
+                const bool in = ie >= 0 && je >= 0 && ke >= 0 && ie < g.nx && je < g.ny && ke < g.nz;
+                S += (double)*(in ? E + ((long)ke * g.nx * g.ny + (long)je * g.nx + ie)
+                                  : reinterpret_cast<const T*>(kZeroWord));
             }
     const double kd[3] = {kd0, kd1, kd2};
+    uint8_t fx[3] = {0, 0, 0};
+    if (fixed != nullptr)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) fx[c] = fixed[3 * n + c];
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const bool fr = (fixed == nullptr || fixed[3 * n + c] == 0) && S > 0.0;
+        const bool fr = fx[c] == 0 && S > 0.0;
         minv[3 * n + c] = fr ? T(1.0 / (kd[c] * S + shift)) : T(0);
     }
 }

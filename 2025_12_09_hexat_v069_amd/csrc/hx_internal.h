@@ -177,6 +177,13 @@ __device__ inline void pcg_decide(PcgState* st) {
 }
 
 // Grid description passed by value to every kernel.
+// A zero word in global memory: a gather reads it in place of an
+// out-of-grid value through a selected address, so the load is unconditional
+// (a select on the loaded value, or a load under a branch of its own, is
+// compiled into a wait per load).  Never written; not const, so it shares
+// the global address space with the arrays (the loads stay global loads).
+static __device__ double kZeroWord[2] = {0.0, 0.0};
+
 struct GridArgs {
     int nx, ny, nz;
     int NX, NY, NZ;

@@ -445,13 +445,16 @@ __global__ __launch_bounds__(NT) void k_helm_rhs(GridArgs g, const double* __res
     if (n >= g.nn) return;
     int i, j, k;
     node_ijk(g, n, i, j, k);
-    double s = 0.0;
+    double s = 0.0;  // out-of-grid elements add the zero word (same sum, every load in flight)
+#pragma unroll
     for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
         for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
             for (int dx = 0; dx < 2; ++dx) {
                 const int ie = i - 1 + dx, je = j - 1 + dy, ke = k - 1 + dz;
-                if (ie < 0 || je < 0 || ke < 0 || ie >= g.nx || je >= g.ny || ke >= g.nz) continue;
-                s += in_e[(long)ke * g.nx * g.ny + (long)je * g.nx + ie];
+                const bool in = ie >= 0 && je >= 0 && ke >= 0 && ie < g.nx && je < g.ny && ke < g.nz;
+                s += *(in ? in_e + ((long)ke * g.nx * g.ny + (long)je * g.nx + ie) : kZeroWord);
             }
     b[n] = s * vol8;
 }
