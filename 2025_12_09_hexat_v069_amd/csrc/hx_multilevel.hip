@@ -685,32 +685,17 @@ __global__ __launch_bounds__(NT) void k_vc_l1(GridArgs gc, const double* __restr
     if (n >= gc.nn) return;
     int i, j, k;
     node_ijk(gc, n, i, j, k);
-    // operands first, every load unconditional (out-of-grid elements and
-    // neighbours read kZeroWord): a load under a branch of its own is waited
-    // for before the next one is issued
-    const double* const zero = kZeroWord;
+    auto fixed_at = [&](int a, int b, int cz, int comp) {
+        if (fixed_f == nullptr) return false;
+        const long f = (long)stride * a + (long)NXf * ((long)stride * b + (long)NYf * ((long)stride * cz));
+        return fixed_f[3 * f + comp] != 0;
+    };
     double Ee[8];  // element (i-1+ex, j-1+ey, k-1+ez) at ex + 2 ey + 4 ez, 0 outside the grid
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         const int ie = i - 1 + (e & 1), je = j - 1 + ((e >> 1) & 1), ke = k - 1 + (e >> 2);
         const bool in = ie >= 0 && je >= 0 && ke >= 0 && ie < gc.nx && je < gc.ny && ke < gc.nz;
-        Ee[e] = h * *(in ? Ec + ((long)ke * gc.nx * gc.ny + (long)je * gc.nx + ie) : zero);
-    }
-    // skip[b] bit d: component b of neighbour d (d = (di+1) + 3 (dj+1) + 9 (dk+1))
-    // is outside the grid or fixed, so its coupling stays out of the row sum
-    uint32_t skip[3] = {0u, 0u, 0u};
-#pragma unroll
-    for (int d = 0; d < 27; ++d) {
-        const int mi = i + d % 3 - 1, mj = j + (d / 3) % 3 - 1, mk = k + d / 9 - 1;
-        const bool in = mi >= 0 && mj >= 0 && mk >= 0 && mi < gc.NX && mj < gc.NY && mk < gc.NZ;
-#pragma unroll
-        for (int b = 0; b < 3; ++b) skip[b] |= (uint32_t)(!in) << d;
-        if (fixed_f != nullptr) {  // every flag load taken (a zero byte outside the grid)
-            const long f = (long)stride * mi + (long)NXf * ((long)stride * mj + (long)NYf * ((long)stride * mk));
-            const uint8_t* fp = in ? fixed_f + 3 * f : reinterpret_cast<const uint8_t*>(zero);
-#pragma unroll
-            for (int b = 0; b < 3; ++b) skip[b] |= (uint32_t)(fp[b] != 0) << d;
-        }
+        Ee[e] = in ? h * Ec[(long)ke * gc.nx * gc.ny + (long)je * gc.nx + ie] : 0.0;
     }
     double l1[3] = {0.0, 0.0, 0.0}, diag[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -728,9 +713,11 @@ __global__ __launch_bounds__(NT) void k_vc_l1(GridArgs gc, const double* __restr
 #pragma unroll
                 for (int b = 0; b < 3; ++b) B[a][b] += Ee[e] * Kbin[(3 * La + a) * 24 + 3 * Lb + b];
         }
+        const int mi = i + di, mj = j + dj, mk = k + dk;
+        const bool in = mi >= 0 && mj >= 0 && mk >= 0 && mi < gc.NX && mj < gc.NY && mk < gc.NZ;
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-            if ((skip[b] >> d) & 1u) continue;
+            if (!in || fixed_at(mi, mj, mk, b)) continue;
 #pragma unroll
             for (int a = 0; a < 3; ++a) l1[a] += fabs(B[a][b]);
         }
@@ -740,7 +727,7 @@ __global__ __launch_bounds__(NT) void k_vc_l1(GridArgs gc, const double* __restr
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const bool solved = !((skip[a] >> 13) & 1u) && diag[a] > 0.0;
+        const bool solved = !fixed_at(i, j, k, a) && diag[a] > 0.0;
         sinv[3 * n + a] = solved ? T(omega / l1[a]) : T(0);
     }
 }
