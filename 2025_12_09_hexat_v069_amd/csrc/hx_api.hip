@@ -968,11 +968,19 @@ int oc_run(hx_ctx* c, long ne, const double* x, const double* dc, const double* 
            double xmin, double rel_tol, double* xnew, bool kary = false) {
     launch_oc_init(c->st, c->stream);
     int rc;
+    // chunks of passes replayed as one graph per field set and parameters
+    // (a finished search returns at once from the remaining passes)
+    char key[256];
+    std::snprintf(key, sizeof key, "oc %d %ld %p %p %p %p %.17g %.17g %.17g %.17g", kary ? 1 : 0, ne, (const void*)x,
+                  (const void*)dc, (const void*)dv, (void*)c->partials.p, volfrac, move, xmin, rel_tol);
     for (int chunk = 0; chunk < 13; ++chunk) {
-        for (int k = 0; k < (kary ? 8 : 16); ++k)
-            if (kary) launch_oc_eval_k(ne, x, dc, dv, volfrac, move, xmin, rel_tol, c->st, c->partials.p, c->stream);
-            else launch_oc_eval(ne, x, dc, dv, volfrac, move, xmin, rel_tol, c->st, c->partials.p, c->stream);
-        HXK();
+        rc = run_batch(c, key, [&] {
+            for (int k = 0; k < (kary ? 8 : 16); ++k)
+                if (kary)
+                    launch_oc_eval_k(ne, x, dc, dv, volfrac, move, xmin, rel_tol, c->st, c->partials.p, c->stream);
+                else launch_oc_eval(ne, x, dc, dv, volfrac, move, xmin, rel_tol, c->st, c->partials.p, c->stream);
+        });
+        if (rc) return rc;
         if ((rc = sync_state(c))) return rc;
         if (c->st_host->done) break;
     }
