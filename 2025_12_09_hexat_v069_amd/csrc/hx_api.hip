@@ -136,7 +136,7 @@ struct hx_ctx {
     // V-cycle form: smoother weight, per smoothed level the smoother diagonal,
     // residual and operator moduli
     double vc_omega = 1.95;
-    int vc_min_blocks = 512;  // coarse V-cycle levels' operator workgroups (two per compute unit)
+    int vc_min_blocks = 512;  // coarse levels' launch width in workgroups (two per compute unit)
     // FP32 form: reliable update each time a column's recursive residual falls by this factor
     double ir_delta = kIrDelta;
     DevBuf<double> vc_s[kMlMaxLevels + 1], vc_res[kMlMaxLevels + 1], vc_E[kMlMaxLevels + 1];
@@ -398,6 +398,7 @@ int ml_setup(hx_ctx* c, bool f32, bool vcycle = false) {
     MlHierarchy<double>& h = c->ml64;
     MlHierarchy<float>& h32 = c->ml32;
     h.levels = h32.levels = L;
+    h.min_blocks = h32.min_blocks = c->vc_min_blocks;
     h.ainv = nullptr;
     h32.ainv = nullptr;
     h.g[0] = h32.g[0] = c->g;
@@ -469,7 +470,6 @@ int ml_setup(hx_ctx* c, bool f32, bool vcycle = false) {
                     HXA(c->vc_E32[l], gl.ne);
                     launch_vc_scale_E<float>(gl.ne, El, hl, c->vc_E32[l].p, c->stream);
                 }
-                h32.min_blocks = c->vc_min_blocks;
                 h32.sinv[l] = c->vc_s32[l].p;
                 h32.vres[l] = c->vc_res32[l].p;
                 h32.Eop[l] = l == 0 ? c->E32.p : c->vc_E32[l].p;
@@ -482,7 +482,6 @@ int ml_setup(hx_ctx* c, bool f32, bool vcycle = false) {
                     HXA(c->vc_E[l], gl.ne);
                     launch_vc_scale_E<double>(gl.ne, El, hl, c->vc_E[l].p, c->stream);
                 }
-                h.min_blocks = c->vc_min_blocks;
                 h.sinv[l] = c->vc_s[l].p;
                 h.vres[l] = c->vc_res[l].p;
                 h.Eop[l] = l == 0 ? c->E.p : c->vc_E[l].p;

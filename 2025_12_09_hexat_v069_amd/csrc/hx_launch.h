@@ -167,8 +167,8 @@ struct MlHierarchy {
     T* sinv[kMlMaxLevels + 1] = {};
     T* vres[kMlMaxLevels + 1] = {};
     const T* Eop[kMlMaxLevels + 1] = {};
-    // coarse V-cycle levels: operator workgroups a launch aims for (shallower
-    // z chunks until reached)
+    // workgroups a coarse level's launch aims for: the V-cycle's level
+    // operators and every restriction take shallower z chunks until reached
     int min_blocks = 512;
 };
 // number of 2:1 coarsenings of g (all element counts even), at most max_levels

@@ -187,23 +187,24 @@ __global__ __launch_bounds__(CX * CY) void k_ml_restrict_zm(GridArgs gf, const T
 }
 
 // coarse planes per workgroup of the restriction onto gc: the deepest chunk
-// (at most kRzMaxKc) that still gives kRzMinBlocks workgroups, two per
-// compute unit.  On the 256x128x128 hierarchy's transitions (FP32) this is
+// (at most kRzMaxKc) that still gives min_blocks workgroups (the hierarchy's
+// launch width, 512 by default: two per compute unit).  On the 256x128x128 hierarchy's transitions (FP32) this is
 // 5 planes for the first (17-18 us against 23 us for one plane) and one
 // plane for the smaller ones (4.8-6.6 us; each further plane costs a
 // memory round trip there).
-constexpr int kRzMaxKc = 8, kRzMinBlocks = 512;
-inline int restrict_zm_kc(const GridArgs& gc, int ntx, int nty) {
+constexpr int kRzMaxKc = 8;
+inline int restrict_zm_kc(const GridArgs& gc, int ntx, int nty, int min_blocks) {
     int kc = kRzMaxKc;
-    while (kc > 1 && (long)ntx * nty * ((gc.NZ + kc - 1) / kc) < kRzMinBlocks) --kc;
+    while (kc > 1 && (long)ntx * nty * ((gc.NZ + kc - 1) / kc) < min_blocks) --kc;
     return kc;
 }
 
 // rc = R rf onto gc (and, with minvc, yc = w Dc^-1 rc)
 template <typename T>
 void launch_restrict(const GridArgs& gf, const T* rf, const GridArgs& gc, T* rc, const T* minvc, T* yc, T w,
-                     const PcgState* st, hipStream_t s) {
-    const int ntx = (gc.NX + kRzX - 1) / kRzX, nty = (gc.NY + kRzY - 1) / kRzY, kc = restrict_zm_kc(gc, ntx, nty);
+                     const PcgState* st, int min_blocks, hipStream_t s) {
+    const int ntx = (gc.NX + kRzX - 1) / kRzX, nty = (gc.NY + kRzY - 1) / kRzY;
+    const int kc = restrict_zm_kc(gc, ntx, nty, min_blocks);
     const dim3 grid(ntx * nty * ((gc.NZ + kc - 1) / kc));
     if (minvc != nullptr)
         hipLaunchKernelGGL((k_ml_restrict_zm<T, true>), grid, dim3(kRzX * kRzY), 0, s, gf, rf, gc, rc, minvc, yc, w,
@@ -819,9 +820,9 @@ void launch_ml_apply(const MlHierarchy<T>& h, const T* r, const T* minv0, T* z, 
         for (int l = 1; l <= last_tiled; ++l) {
             const T* rf = l == 1 ? r : h.r[l - 1];
             if (l == L)
-                launch_restrict<T>(h.g[l - 1], rf, h.g[l], h.r[l], h.minv[l], h.y[l], T(h.w[l]), st, s);
+                launch_restrict<T>(h.g[l - 1], rf, h.g[l], h.r[l], h.minv[l], h.y[l], T(h.w[l]), st, h.min_blocks, s);
             else
-                launch_restrict<T>(h.g[l - 1], rf, h.g[l], h.r[l], nullptr, nullptr, T(1), st, s);
+                launch_restrict<T>(h.g[l - 1], rf, h.g[l], h.r[l], nullptr, nullptr, T(1), st, h.min_blocks, s);
         }
         if (last_tiled < L) {
             MlHierarchy<T> hc = h;
@@ -902,7 +903,7 @@ int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const T* Kb, c
     };
     auto restrict_to = [&](int l, bool scale) {  // R vres[l-1] -> r[l] (and y[l] = D_l^-1 r[l])
         launch_restrict<T>(h.g[l - 1], (const T*)h.vres[l - 1], h.g[l], h.r[l], scale ? (const T*)h.minv[l] : nullptr,
-                           scale ? h.y[l] : nullptr, T(scale ? h.w[l] : 1), st, s);
+                           scale ? h.y[l] : nullptr, T(scale ? h.w[l] : 1), st, h.min_blocks, s);
     };
     for (int l = 0; l < c; ++l) {  // pre-smoothing x = S r, residual, restriction
         MvArgs<T> a{nullptr, h.vres[l], h.Eop[l], T(0), MV_PLAIN, st, nullptr, 3 * h.g[l].nn, level_r(l), h.sinv[l],

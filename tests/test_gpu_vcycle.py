@@ -41,6 +41,34 @@ APPLY_CASES = [((8, 4, 4), (1.0, 1.0, 1.0)), ((16, 8, 8), (1.0, 0.5, 2.0)), ((64
                ((50, 26, 26), (1.0, 1.0, 1.0)), ((7, 4, 4), (1.0, 1.0, 1.0))]
 
 
+# Coarse levels' launch width 1 (HEXAT_VC_MINBLOCKS, read when a context is
+# created): every restriction then marches 8 coarse planes per workgroup,
+# carrying plane 2K+1's xy sum to K+1 (at the default width only production
+# grids do), and the level operators keep their full z chunks.
+@pytest.mark.parametrize("dims", [(64, 32, 32), (50, 26, 26), (16, 8, 8)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_deep_restriction_chunks_match_oracle(ox, hx, monkeypatch, dims, precision):
+    _say("deep restriction chunks", dims, precision)
+    monkeypatch.setenv("HEXAT_VC_MINBLOCKS", "1")
+    c = hx.Context(0)
+    try:
+        g, fixed, _ = _setup(ox, c, dims)
+        rng = np.random.default_rng(7 * sum(dims) + precision)
+        Ee = 1e-9 + rng.uniform(0, 1, g.ne) ** 3
+        Ke = ox.ke_hex(1.0, 1.0, 1.0, 0.3)
+        m0 = ox.ml_hierarchy(g, Ke, Ee, fixed, 0)[0][1].ravel()
+        r = np.where(m0 != 0, rng.standard_normal(g.ndof), 0.0)
+        tol = 1e-10 if precision == 0 else 1e-5
+        z, L, _ = c.test_vc_apply(Ee, r, precision)
+        zo = ox.vc_apply(g, Ke, Ee, fixed, r)
+        assert L >= 1 and np.allclose(z, zo, rtol=tol, atol=tol * np.abs(zo).max())
+        z, L = c.test_ml_apply(Ee, r, precision)
+        zo = ox.ml_apply(g, Ke, Ee, fixed, r)
+        assert L >= 1 and np.allclose(z, zo, rtol=tol, atol=tol * np.abs(zo).max())
+    finally:
+        c.close()
+
+
 @pytest.mark.parametrize("dims,h", APPLY_CASES)
 @pytest.mark.parametrize("precision", [0, 1])
 def test_vc_apply_matches_oracle(ox, hx, ctx, dims, h, precision):
