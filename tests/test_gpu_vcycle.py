@@ -181,7 +181,8 @@ def test_vc_smoothing_timing_hook(ox, hx, ctx):
         ctx.time_smoothing(1, 4, 4)
 
 
-@pytest.mark.parametrize("pre,post", [((2, 16, 1), (0, 16, 2)), ((0, 8, 2), (1, 8, 1)), ((1, 16, 2), (2, 4, 1))])
+@pytest.mark.parametrize("pre,post", [((2, 16, 1), (0, 16, 2)), ((0, 8, 2), (1, 8, 1)), ((1, 16, 2), (2, 4, 1)),
+                                      ((2, 12, 1), (2, 12, 1))])
 def test_vc_smoothing_shapes_match_oracle(ox, hx, ctx, pre, post):
     """Tuned launch shapes of the smoothing forms (restored by hx_tune_set) leave the application
     at the oracle's value; the autotuner reports shapes for both forms."""
