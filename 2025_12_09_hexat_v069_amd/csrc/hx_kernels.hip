@@ -485,8 +485,16 @@ __global__ __launch_bounds__(NT) void k_helm_minv(GridArgs g, HelmArgs A, double
 // registers: one new 3x3 plane (9 loads from clamped addresses, coalesced
 // along i) per node instead of 64 gathers.  Elements outside the grid get
 // weight 0.  In the PCG mode it also writes the per-block x'y partials.
+// With BOX the element is a box (both filter forms are: the reference's is
+// 1.5 dz tall), so an entry of Ae depends only on which coordinates its two
+// nodes share: eight values (16 SGPRs) instead of 64, which the compiler
+// otherwise spills to vector lanes and reads back (72 v_readlane per plane).
 constexpr int kHx = 64, kHy = 4, kHkc = 16;
-__global__ __launch_bounds__(NT) void k_helm_matvec(GridArgs g, HelmArgs A, const double* __restrict__ x,
+struct HelmBox {
+    double c[8];  // at sx + 2 sy + 4 sz: s = 1 where the two nodes share that coordinate
+};
+template <bool BOX>
+__global__ __launch_bounds__(NT) void k_helm_matvec(GridArgs g, HelmArgs A, HelmBox hb, const double* __restrict__ x,
                                                     double* __restrict__ y, int mode, PcgState* st,
                                                     double* partials, int ntx, int nty) {
     __shared__ double red[NT / 64];
@@ -532,8 +540,15 @@ __global__ __launch_bounds__(NT) void k_helm_matvec(GridArgs g, HelmArgs A, cons
                     const int a = kBinToRef[(1 - dx) + 2 * (1 - dy) + 4 * (1 - dz)];
                     double e = 0.0;
 #pragma unroll
-                    for (int L = 0; L < 8; ++L)
-                        e += A.Ae[8 * a + kBinToRef[L]] * V[dz + (L >> 2)][dy + ((L >> 1) & 1)][dx + (L & 1)];
+                    for (int L = 0; L < 8; ++L) {
+                        double coef;
+                        if constexpr (BOX)
+                            coef = hb.c[((1 - dx) == (L & 1)) + 2 * ((1 - dy) == ((L >> 1) & 1)) +
+                                        4 * ((1 - dz) == (L >> 2))];
+                        else
+                            coef = A.Ae[8 * a + kBinToRef[L]];
+                        e += coef * V[dz + (L >> 2)][dy + ((L >> 1) & 1)][dx + (L & 1)];
+                    }
                     acc += (wx[dx] * wy[dy] * wz) * e;
                 }
         }
@@ -992,7 +1007,26 @@ int launch_helm_matvec(const GridArgs& g, const HelmArgs& A, const double* x, do
                        double* partials, hipStream_t s) {
     int ntx, nty;
     const int nb = helm_tiles(g, &ntx, &nty);
-    hipLaunchKernelGGL(k_helm_matvec, dim3(nb), dim3(NT), 0, s, g, A, x, y, mode, st, partials, ntx, nty);
+    // the eight box values, taken from node 0's row and checked against every entry
+    HelmBox hb{};
+    for (int sc = 0; sc < 8; ++sc) {
+        const int b = ((sc & 1) ? 0 : 1) + 2 * ((sc & 2) ? 0 : 1) + 4 * ((sc & 4) ? 0 : 1);
+        hb.c[sc] = A.Ae[8 * kBinToRef[0] + kBinToRef[b]];
+    }
+    double amax = 0.0;
+    for (int t = 0; t < 64; ++t) amax = std::max(amax, std::fabs(A.Ae[t]));
+    bool box = true;
+    for (int p = 0; p < 8 && box; ++p)
+        for (int q = 0; q < 8 && box; ++q) {
+            const int sc = ((p & 1) == (q & 1)) + 2 * (((p >> 1) & 1) == ((q >> 1) & 1)) + 4 * ((p >> 2) == (q >> 2));
+            box = std::fabs(A.Ae[8 * kBinToRef[p] + kBinToRef[q]] - hb.c[sc]) <= 1e-14 * amax;
+        }
+    if (box)
+        hipLaunchKernelGGL((k_helm_matvec<true>), dim3(nb), dim3(NT), 0, s, g, A, hb, x, y, mode, st, partials, ntx,
+                           nty);
+    else
+        hipLaunchKernelGGL((k_helm_matvec<false>), dim3(nb), dim3(NT), 0, s, g, A, hb, x, y, mode, st, partials,
+                           ntx, nty);
     return nb;
 }
 void launch_fin_alpha(PcgState* st, const double* partials, int nblocks, int nrhs, hipStream_t s) {
