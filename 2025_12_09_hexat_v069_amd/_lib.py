@@ -41,7 +41,8 @@ STATUS_NAMES = {0: "ok", 1: "no device", 2: "bad argument", 3: "not converged", 
 EXPORTS = [
     "hx_version", "hx_device_count", "hx_ctx_create", "hx_ctx_destroy", "hx_last_error", "hx_grid_set",
     "hx_set_fixed", "hx_element_stiffness", "hx_scalar_matrices", "hx_matvec", "hx_diagonal", "hx_pcg_solve",
-    "hx_block_pcg_solve", "hx_helmholtz_filter", "hx_element_stress", "hx_compliance_sens", "hx_oc_update",
+    "hx_block_pcg_solve", "hx_helmholtz_filter", "hx_element_stress", "hx_web_pack", "hx_compliance_sens",
+    "hx_oc_update",
     "hx_to_init", "hx_to_step", "hx_to_read", "hx_learned_train", "hx_learned_weights", "hx_timing_enable",
     "hx_timing_read", "hx_device_info", "hx_autotune", "hx_tune_set", "hx_time_operator", "hx_time_operator_pc",
     "hx_time_smoothing",
@@ -136,6 +137,8 @@ def load():
         L.hx_block_pcg_solve.argtypes = [vp, i, pd, pd, pd, P(PcgOpts), P(PcgStats)]
         L.hx_helmholtz_filter.argtypes = [vp, d, P(FilterOpts), pd, pd, P(FilterStats)]
         L.hx_element_stress.argtypes = [vp, pd, pd, pd, pd, pd, pd, pd]
+        L.hx_web_pack.argtypes = [vp, P(C.c_float), P(C.c_float), P(C.c_float), C.c_float, P(C.c_float),
+                                  P(C.c_float), P(C.c_float), P(C.c_uint32)]
         L.hx_compliance_sens.argtypes = [vp, pd, d, d, d, pd, pd, pd, pd, pd]
         L.hx_oc_update.argtypes = [vp, pd, pd, pd, d, d, d, d, pd, pd, P(C.c_int), pd]
         L.hx_to_init.argtypes = [vp, P(TOParams), pd, pd]
@@ -327,6 +330,23 @@ class Context:
                                             dptr(out["voigt"]), dptr(out["l1"]), dptr(out["dir"])),
                    "hx_element_stress")
         return out
+
+    def web_pack(self, density, l1, principal, threshold):
+        """Kept elements of the viewer/checkpoint file (density >= threshold,
+        element order): (centroids count x 3, densities, signed L1), FP32."""
+        ne = self.ne
+        rho = np.ascontiguousarray(density, dtype=np.float32).reshape(ne)
+        l1v = np.ascontiguousarray(l1, dtype=np.float32).reshape(ne)
+        pr = np.ascontiguousarray(principal, dtype=np.float32).reshape(ne, 3)
+        cent = np.empty((ne, 3), np.float32)
+        dout = np.empty(ne, np.float32)
+        sl1 = np.empty(ne, np.float32)
+        n = C.c_uint32(0)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        self.check(self.L.hx_web_pack(self.h, fp(rho), fp(l1v), fp(pr), C.c_float(np.float32(threshold)),
+                                      fp(cent), fp(dout), fp(sl1), C.byref(n)), "hx_web_pack")
+        k = int(n.value)
+        return cent[:k].copy(), dout[:k].copy(), sl1[:k].copy()
 
     def compliance_sens(self, u, xphys=None, penal=3.0, E0=1.0, Emin=1e-9, f=None):
         u = f64(u, self.ndof)

@@ -112,6 +112,8 @@ struct hx_ctx {
     DevBuf<float> b_x32, b_r32, b_p32, b_q32, b_p32b;
     DevBuf<double> h_b, h_x, h_r, h_p, h_q, h_minv, h_in, h_out;
     DevBuf<double> e_pr, e_vm, e_sv, e_l1, e_dir, e_ce, e_dc, e_aux, small;
+    DevBuf<float> w_rho, w_l1, w_pr, w_cent, w_rho_out, w_sl1;  // hx_web_pack
+    DevBuf<unsigned> w_cnt, w_offs;
     // SIMP session
     bool to_ready = false;
     hx_to_params tp{};
@@ -1065,8 +1067,11 @@ void hx_ctx_destroy(hx_ctx* c) {
                            &c->l_ustar, &c->l_samples, &c->l_rho};
     for (auto b : d) b->release();
     DevBuf<float>* fl[] = {&c->Kbin32_d, &c->E32, &c->x32, &c->r32, &c->p32, &c->q32, &c->minv32,
-                           &c->b_x32, &c->b_r32, &c->b_p32, &c->b_q32, &c->p32b, &c->b_p32b};
+                           &c->b_x32, &c->b_r32, &c->b_p32, &c->b_q32, &c->p32b, &c->b_p32b,
+                           &c->w_rho, &c->w_l1, &c->w_pr, &c->w_cent, &c->w_rho_out, &c->w_sl1};
     for (auto b : fl) b->release();
+    c->w_cnt.release();
+    c->w_offs.release();
     c->fixed.release();
     if (c->st) (void)hipFree(c->st);
     if (c->st_host) (void)hipHostFree(c->st_host);
@@ -1360,6 +1365,40 @@ int hx_element_stress(hx_ctx* c, const double* Ee, const double* u, double* prin
     if (l1 && (rc = download(c, l1, c->e_l1.p, ne * sizeof(double)))) return rc;
     if (dir && (rc = download(c, dir, c->e_dir.p, 3 * ne * sizeof(double)))) return rc;
     HXC(hipStreamSynchronize(c->stream));
+    return HX_OK;
+}
+
+int hx_web_pack(hx_ctx* c, const float* density, const float* l1, const float* principal, float threshold,
+                float* centroids, float* density_out, float* signed_l1, uint32_t* count) {
+    if (!c || !density || !l1 || !principal || !centroids || !density_out || !signed_l1 || !count)
+        return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    const long ne = c->g.ne;
+    const int nt = web_tiles(ne);
+    HXA(c->w_rho, ne);
+    HXA(c->w_l1, ne);
+    HXA(c->w_pr, 3 * ne);
+    HXA(c->w_cent, 3 * ne);
+    HXA(c->w_rho_out, ne);
+    HXA(c->w_sl1, ne);
+    HXA(c->w_cnt, nt);
+    HXA(c->w_offs, nt + 1);
+    int rc;
+    if ((rc = upload(c, c->w_rho.p, density, ne * sizeof(float))) || (rc = upload(c, c->w_l1.p, l1, ne * sizeof(float))) ||
+        (rc = upload(c, c->w_pr.p, principal, 3 * ne * sizeof(float))))
+        return rc;
+    launch_web_pack(c->g, c->w_rho.p, c->w_l1.p, c->w_pr.p, threshold, c->w_cnt.p, c->w_offs.p, c->w_cent.p,
+                    c->w_rho_out.p, c->w_sl1.p, c->stream);
+    HXK();
+    unsigned n = 0;
+    if ((rc = download(c, &n, c->w_offs.p + nt, sizeof(unsigned)))) return rc;
+    if ((long)n > ne) return fail(c, HX_ERR_HIP, "hx_web_pack: kept count exceeds the element count");
+    *count = n;
+    if (n == 0) return HX_OK;
+    if ((rc = download(c, centroids, c->w_cent.p, 3L * n * sizeof(float))) ||
+        (rc = download(c, density_out, c->w_rho_out.p, (long)n * sizeof(float))) ||
+        (rc = download(c, signed_l1, c->w_sl1.p, (long)n * sizeof(float))))
+        return rc;
     return HX_OK;
 }
 

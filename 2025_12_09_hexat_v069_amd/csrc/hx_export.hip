@@ -1,0 +1,147 @@
+// hx_export.hip -- payload of the viewer/checkpoint file: the elements whose
+// density reaches a threshold, kept in element order, with their centroids,
+// densities and signed L1 stress (Postprocessing.export_binary_for_web,
+// Postprocessing.jl:159-195).
+//
+// An ordered stream compaction in three launches over tiles of 2048 elements:
+// count the kept elements of each tile, scan the tile counts in one
+// workgroup, then write each kept element at its tile's offset plus its rank
+// inside the tile (wave ballots, popcounts and a 4-entry LDS prefix per
+// 256-element step).  The centroid is computed from (i,j,k), not read: the
+// structured grid has no node array on the device.  Bytes per element: 4 for
+// the density test (read twice, the second time mostly from L2) plus 16 read
+// (l1, 3 principal values) and 20 written for each kept element.
+#include "hx_launch.h"
+
+namespace hx {
+
+namespace {
+
+constexpr int WT = 256;             // threads per workgroup
+constexpr int WSTEPS = 8;           // 256-element steps per tile
+constexpr int WTILE = WT * WSTEPS;  // elements per tile
+
+__device__ __forceinline__ bool kept(const float* density, long e, long ne, float thr) {
+    return e < ne && density[e] >= thr;  // NaN is never kept, as findall(x -> x >= t)
+}
+
+__global__ void __launch_bounds__(WT) k_web_count(const float* __restrict__ density, long ne, float thr,
+                                                  unsigned* __restrict__ tile_count) {
+    __shared__ unsigned wsum[WT / 64];
+    const long base = (long)blockIdx.x * WTILE;
+    unsigned n = 0;
+#pragma unroll
+    for (int s = 0; s < WSTEPS; ++s) {
+        const long e = base + s * WT + threadIdx.x;
+        n += __popcll(__ballot(kept(density, e, ne, thr)));
+    }
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) tile_count[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// Exclusive scan of ntiles counts in one workgroup of 1024 threads;
+// offs[ntiles] receives the total.
+__global__ void __launch_bounds__(1024) k_web_scan(const unsigned* __restrict__ tile_count, int ntiles,
+                                                   unsigned* __restrict__ offs) {
+    __shared__ unsigned part[1024];
+    unsigned carry = 0;
+    for (int c0 = 0; c0 < ntiles; c0 += 1024) {
+        const int t = c0 + (int)threadIdx.x;
+        const unsigned v = t < ntiles ? tile_count[t] : 0u;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        // Hillis-Steele inclusive scan over the chunk
+        for (int o = 1; o < 1024; o <<= 1) {
+            const unsigned a = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0u;
+            __syncthreads();
+            part[threadIdx.x] += a;
+            __syncthreads();
+        }
+        if (t < ntiles) offs[t] = carry + part[threadIdx.x] - v;
+        carry += part[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) offs[ntiles] = carry;
+}
+
+// Centroid of element (i,j,k) as MeshUtilities.element_centroid
+// (MeshUtilities.jl:18-32) computes it from Mesh.generate_mesh's node
+// coordinates (Mesh.jl:36-38, Float32((i-1))*dx): the eight corner values
+// summed in FP32 in node order n1..n8, then scaled by 0.125f.  Contraction
+// into FMAs is off so that every sum rounds as the reference's does.
+__device__ __forceinline__ float corner_sum(float a, float b, const int* pat) {
+#pragma clang fp contract(off)
+    float s = 0.0f;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) s = s + (pat[n] ? b : a);
+    return s * 0.125f;
+}
+
+__device__ __forceinline__ float fsign(float v) { return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : v); }
+
+__global__ void __launch_bounds__(WT) k_web_write(const float* __restrict__ density, const float* __restrict__ l1,
+                                                  const float* __restrict__ principal, long ne, float thr, int nx,
+                                                  int ny, float dx, float dy, float dz,
+                                                  const unsigned* __restrict__ offs, float* __restrict__ cent,
+                                                  float* __restrict__ dens_out, float* __restrict__ sl1_out) {
+#pragma clang fp contract(off)
+    __shared__ unsigned wcnt[WT / 64];
+    // corner patterns of the node order (-,-,-),(+,-,-),(+,+,-),(-,+,-),(-,-,+),(+,-,+),(+,+,+),(-,+,+)
+    const int px[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+    const int py[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+    const int pz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    const long base = (long)blockIdx.x * WTILE;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    unsigned run = offs[blockIdx.x];
+    for (int s = 0; s < WSTEPS; ++s) {
+        const long e = base + s * WT + threadIdx.x;
+        const bool k = kept(density, e, ne, thr);
+        const unsigned long long m = __ballot(k);
+        if (lane == 0) wcnt[wave] = (unsigned)__popcll(m);
+        __syncthreads();
+        unsigned before = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < WT / 64; ++w) {
+            before += w < wave ? wcnt[w] : 0u;
+            total += wcnt[w];
+        }
+        if (k) {
+            const unsigned pos = run + before + (unsigned)__popcll(m & below);
+            const int i = (int)(e % nx);
+            const int j = (int)((e / nx) % ny);
+            const int kk = (int)(e / ((long)nx * ny));
+            cent[3L * pos + 0] = corner_sum((float)i * dx, (float)(i + 1) * dx, px);
+            cent[3L * pos + 1] = corner_sum((float)j * dy, (float)(j + 1) * dy, py);
+            cent[3L * pos + 2] = corner_sum((float)kk * dz, (float)(kk + 1) * dz, pz);
+            dens_out[pos] = density[e];
+            // sign of the principal stress of largest magnitude, first one on ties
+            // (Postprocessing.jl:187-195)
+            const float s1 = principal[3 * e], s2 = principal[3 * e + 1], s3 = principal[3 * e + 2];
+            float amax = fabsf(s1), sg = fsign(s1);
+            if (fabsf(s2) > amax) { amax = fabsf(s2); sg = fsign(s2); }
+            if (fabsf(s3) > amax) { sg = fsign(s3); }
+            sl1_out[pos] = l1[e] * sg;
+        }
+        run += total;
+        __syncthreads();  // wcnt is rewritten by the next step
+    }
+}
+
+}  // namespace
+
+int web_tiles(long ne) { return (int)((ne + WTILE - 1) / WTILE); }
+
+void launch_web_pack(const GridArgs& g, const float* density, const float* l1, const float* principal, float thr,
+                     unsigned* tile_count, unsigned* offs, float* cent, float* dens_out, float* sl1_out,
+                     hipStream_t s) {
+    const int nt = web_tiles(g.ne);
+    if (nt == 0) return;
+    hipLaunchKernelGGL(k_web_count, dim3(nt), dim3(WT), 0, s, density, g.ne, thr, tile_count);
+    hipLaunchKernelGGL(k_web_scan, dim3(1), dim3(1024), 0, s, tile_count, nt, offs);
+    hipLaunchKernelGGL(k_web_write, dim3(nt), dim3(WT), 0, s, density, l1, principal, g.ne, thr, g.nx, g.ny,
+                       (float)g.dx, (float)g.dy, (float)g.dz, offs, cent, dens_out, sl1_out);
+}
+
+}  // namespace hx

@@ -85,6 +85,14 @@ void launch_fin_alpha(PcgState* st, const double* partials, int nblocks, int nrh
 constexpr int kVecBlocks = 1024;  // blocks of the streaming kernels (multiple of 8)
 // dst[i] = src[i] for n 16-byte words (counter calibration)
 void launch_copy16(long n, const void* src, void* dst, hipStream_t s);
+
+// ---- viewer/checkpoint payload (hx_export.hip) ----------------------------
+// tiles of the ordered compaction (tile_count needs web_tiles(ne) entries,
+// offs web_tiles(ne) + 1; offs[web_tiles(ne)] receives the kept count)
+int web_tiles(long ne);
+void launch_web_pack(const GridArgs& g, const float* density, const float* l1, const float* principal, float thr,
+                     unsigned* tile_count, unsigned* offs, float* cent, float* dens_out, float* sl1_out,
+                     hipStream_t s);
 template <typename T>
 void launch_pcg_init(long n, const T* b, T* x, const T* q, const T* minv, T* r, T* p, double shift,
                      PcgState* st, double* partials, hipStream_t s, int nrhs = 1);

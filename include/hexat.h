@@ -143,6 +143,17 @@ int hx_helmholtz_filter(hx_ctx* ctx, double R, const hx_filter_opts* opts, const
  * stress (sign arbitrary).  Replaces Stress.compute_stress_field (Stress.jl:57-100). */
 int hx_element_stress(hx_ctx* ctx, const double* Ee, const double* u, double* principal,
                       double* vm, double* voigt, double* l1, double* dir);
+/* Payload of the viewer/checkpoint file: the elements with
+ * density[e] >= threshold, in element order; for each, its centroid (FP32 mean
+ * of the 8 node coordinates, MeshUtilities.jl:18-32), its density and its L1
+ * stress signed by the principal stress of largest magnitude (first on ties).
+ * principal: Ne x 3 as hx_element_stress.  centroids (3 Ne), density_out and
+ * signed_l1 (Ne each) are sized for the worst case; *count receives the number
+ * kept (0: nothing is written, as the reference writes no file).  Replaces
+ * the packing loop of Postprocessing.export_binary_for_web
+ * (Postprocessing.jl:173-195); the file itself is written by the host. */
+int hx_web_pack(hx_ctx* ctx, const float* density, const float* l1, const float* principal, float threshold,
+                float* centroids, float* density_out, float* signed_l1, uint32_t* count);
 /* ce_e = u_e^T Ke(E=1) u_e and SIMP sensitivity
  * dc_e = -penal xphys^(penal-1) (E0-Emin) ce_e; compliance = f^T u.
  * No reference counterpart (the reference has no sensitivities). */

@@ -23,6 +23,8 @@ pieces in numpy:
   l1-Jacobi smoothing (absolute row sums, the smoother of Baker, Falgout,
   Kolev & Yang 2011) on every level above the coarsest; no reference code
   exists.
+* ``web_pack`` -- Postprocessing.jl:173-195 (kept elements, centroids, signed
+  L1 of the viewer/checkpoint file) in FP32 numpy.
 """
 from __future__ import annotations
 
@@ -703,3 +705,51 @@ def vc_pcg(g: Grid, Ke, Ee, fixed, b, tol=1e-8, max_iter=100000, omega=1.95, max
         p = z + (rzn / rz) * p
         rz = rzn
     return x, rel, max_iter
+
+
+# ----------------------------------------------------------------------------
+# viewer/checkpoint payload
+# ----------------------------------------------------------------------------
+def web_pack(nx, ny, nz, dx, dy, dz, density, l1, principal, threshold):
+    """Packing loop of Postprocessing.export_binary_for_web
+    (Postprocessing.jl:173-195) on the mesh of Mesh.generate_mesh (Mesh.jl:23-60),
+    in FP32 as the reference computes it:
+
+    * kept = findall(rho >= threshold), in element order (:173);
+    * centroid = MeshUtilities.element_centroid (MeshUtilities.jl:18-32): the
+      FP32 node coordinates Float32(i-1)*dx of the 8 nodes summed in node order
+      from 0f0, times 0.125f0;
+    * signed L1 = l1 * sign of the principal stress of largest magnitude, the
+      first one on ties (strict >, :187-195); sign(0) = 0.
+
+    principal is (Ne, 3).  Returns (centroids (count, 3), density, signed_l1)
+    as float32 arrays."""
+    f32 = np.float32
+    NX, NY = nx + 1, ny + 1
+    rho = np.asarray(density, dtype=f32).ravel()
+    keep = np.nonzero(rho >= f32(threshold))[0]
+    # node coordinates as generate_mesh stores them
+    xs = np.arange(nx + 1, dtype=f32) * f32(dx)
+    ys = np.arange(ny + 1, dtype=f32) * f32(dy)
+    zs = np.arange(nz + 1, dtype=f32) * f32(dz)
+    i, j, k = keep % nx, (keep // nx) % ny, keep // (nx * ny)
+    order = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    cx = np.zeros(keep.size, f32)
+    cy = np.zeros(keep.size, f32)
+    cz = np.zeros(keep.size, f32)
+    for a, b, c in order:  # element row n1..n8 of Mesh.jl:45-53
+        node = (i + a) + NX * ((j + b) + NY * (k + c))
+        cx = cx + xs[node % NX]
+        cy = cy + ys[(node // NX) % NY]
+        cz = cz + zs[node // (NX * NY)]
+    inv8 = f32(0.125)
+    cent = np.stack([cx * inv8, cy * inv8, cz * inv8], axis=1).astype(f32)
+    pr = np.asarray(principal, dtype=f32).reshape(-1, 3)[keep]
+    s1, s2, s3 = pr[:, 0], pr[:, 1], pr[:, 2]
+    amax, sg = np.abs(s1), np.sign(s1)
+    t = np.abs(s2) > amax
+    amax, sg = np.where(t, np.abs(s2), amax), np.where(t, np.sign(s2), sg)
+    t = np.abs(s3) > amax
+    sg = np.where(t, np.sign(s3), sg).astype(f32)
+    sl1 = (np.asarray(l1, dtype=f32).ravel()[keep] * sg).astype(f32)
+    return cent, rho[keep].copy(), sl1
