@@ -41,11 +41,12 @@ STATUS_NAMES = {0: "ok", 1: "no device", 2: "bad argument", 3: "not converged", 
 EXPORTS = [
     "hx_version", "hx_device_count", "hx_ctx_create", "hx_ctx_destroy", "hx_last_error", "hx_grid_set",
     "hx_set_fixed", "hx_element_stiffness", "hx_scalar_matrices", "hx_matvec", "hx_diagonal", "hx_pcg_solve",
+    "hx_solve_system",
     "hx_block_pcg_solve", "hx_helmholtz_filter", "hx_element_stress", "hx_web_pack", "hx_compliance_sens",
     "hx_oc_update",
     "hx_to_init", "hx_to_step", "hx_to_read", "hx_learned_train", "hx_learned_weights", "hx_timing_enable",
     "hx_timing_read", "hx_device_info", "hx_autotune", "hx_tune_set", "hx_time_operator", "hx_time_operator_pc",
-    "hx_time_smoothing",
+    "hx_time_smoothing", "hx_device_memory",
 ]
 #: test hooks declared in include/hexat_testing.h
 TEST_EXPORTS = ["hx_test_unet_nparam", "hx_test_unet_init", "hx_test_unet_forward", "hx_test_unet_backward",
@@ -66,6 +67,21 @@ class PcgOpts(C.Structure):
 class PcgStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("status", C.c_int), ("rel_res", C.c_double), ("seconds", C.c_double),
                 ("refinements", C.c_int), ("fallbacks", C.c_int)]
+
+
+HX_SOLVE_CONVERGED, HX_SOLVE_ACCEPTED, HX_SOLVE_FAILED, HX_SOLVE_ZERO_RHS = range(4)
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("E", C.c_double), ("min_stiffness_threshold", C.c_double), ("prune_voids", C.c_int),
+                ("tol", C.c_double), ("max_iter", C.c_int), ("shift_factor", C.c_double), ("precision", C.c_int),
+                ("precond", C.c_int), ("max_attempts", C.c_int), ("warm_start", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("status", C.c_int), ("outcome", C.c_int), ("attempts", C.c_int), ("iterations", C.c_int),
+                ("rel_res", C.c_double), ("true_rel", C.c_double), ("best_rel", C.c_double), ("shift", C.c_double),
+                ("n_active", C.c_long), ("n_kept", C.c_long), ("pruned", C.c_int), ("seconds", C.c_double)]
 
 
 class FilterOpts(C.Structure):
@@ -135,6 +151,7 @@ def load():
         L.hx_diagonal.argtypes = [vp, pd, pd]
         L.hx_pcg_solve.argtypes = [vp, pd, pd, pd, P(PcgOpts), P(PcgStats)]
         L.hx_block_pcg_solve.argtypes = [vp, i, pd, pd, pd, P(PcgOpts), P(PcgStats)]
+        L.hx_solve_system.argtypes = [vp, pd, pd, pd, P(SolveOpts), P(SolveStats)]
         L.hx_helmholtz_filter.argtypes = [vp, d, P(FilterOpts), pd, pd, P(FilterStats)]
         L.hx_element_stress.argtypes = [vp, pd, pd, pd, pd, pd, pd, pd]
         L.hx_web_pack.argtypes = [vp, P(C.c_float), P(C.c_float), P(C.c_float), C.c_float, P(C.c_float),
@@ -149,6 +166,7 @@ def load():
         L.hx_timing_enable.argtypes = [vp, i]
         L.hx_timing_read.argtypes = [vp, i, pd, P(C.c_long)]
         L.hx_device_info.argtypes = [vp, P(C.c_int), P(C.c_int), pd]
+        L.hx_device_memory.argtypes = [vp, pd, pd]
         L.hx_autotune.argtypes = [vp, i, P(TuneResult)]
         L.hx_tune_set.argtypes = [vp, P(TuneResult)]
         L.hx_time_operator.argtypes = [vp, i, i, i, P(C.c_double)]
@@ -295,6 +313,22 @@ class Context:
         ok = (HX_OK,) if raise_on_fail else (HX_OK, HX_ERR_NOT_CONVERGED, HX_ERR_BREAKDOWN, HX_ERR_DIVERGED)
         self.check(rc, "hx_pcg_solve", ok)
         return u, s
+
+    def solve_system(self, density, f, u_prev=None, *, E=1.0, min_stiffness_threshold=1e-3, prune_voids=True,
+                     tol=1e-6, max_iter=40000, shift_factor=1e-6, precision=HX_FP64, precond=HX_PC_JACOBI,
+                     max_attempts=3):
+        """hx_solve_system: the reference's solve policy (pruning, shift, retries,
+        best iterate) on the device.  Returns (u, SolveStats); raises only on
+        errors other than "all attempts failed" (stats.outcome tells)."""
+        rho, f = f64(density, self.ne), f64(f, self.ndof)
+        u = np.zeros(self.ndof) if u_prev is None else f64(u_prev, self.ndof).copy()
+        o = SolveOpts(float(E), float(min_stiffness_threshold), 1 if prune_voids else 0, float(tol), int(max_iter),
+                      float(shift_factor), int(precision), int(precond), int(max_attempts),
+                      0 if u_prev is None else 1)
+        st = SolveStats()
+        rc = self.L.hx_solve_system(self.h, dptr(rho), dptr(f), dptr(u), C.byref(o), C.byref(st))
+        self.check(rc, "hx_solve_system", (HX_OK, HX_ERR_NOT_CONVERGED))
+        return u, st
 
     def block_pcg(self, Ee, F, tol=1e-8, max_iter=100000, shift=0.0, precision=HX_FP64, matvec=HX_MV_AUTO):
         Ee = f64(Ee, self.ne)
@@ -485,6 +519,11 @@ class Context:
         ms, cnt = C.c_double(0), C.c_long(0)
         self.check(self.L.hx_timing_read(self.h, int(kernel), C.byref(ms), C.byref(cnt)), "hx_timing_read")
         return ms.value, cnt.value
+
+    def memory_used_gb(self):
+        used, tot = C.c_double(0), C.c_double(0)
+        self.check(self.L.hx_device_memory(self.h, C.byref(used), C.byref(tot)), "hx_device_memory")
+        return used.value
 
     def device_info(self):
         cu, clk = C.c_int(0), C.c_int(0)

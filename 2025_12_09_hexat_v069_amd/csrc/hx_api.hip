@@ -48,6 +48,11 @@ using namespace hx;
 
 namespace {
 
+// Generation of the device buffers: advanced by every (re)allocation and
+// release, so a captured graph whose kernel arguments may name a freed buffer
+// is never replayed (run_batch drops the cache when it has moved).
+unsigned long g_buf_gen = 0;
+
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
@@ -55,6 +60,7 @@ struct DevBuf {
     hipError_t ensure(size_t m) {
         if (m <= n && p) return hipSuccess;
         release();
+        ++g_buf_gen;
         hipError_t e = hipMalloc((void**)&p, std::max<size_t>(m, 1) * sizeof(T));
         if (e != hipSuccess) {
             p = nullptr;
@@ -65,7 +71,10 @@ struct DevBuf {
         return hipSuccess;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            ++g_buf_gen;
+        }
         p = nullptr;
         n = 0;
     }
@@ -146,6 +155,14 @@ struct hx_ctx {
     // captured iteration batches (hipGraph), keyed by kernel arguments
     bool graphs_on = true;
     std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
+    unsigned long graphs_gen = 0;  // g_buf_gen when the cached graphs were captured
+    // best-iterate snapshot of the single-column solves, the best accepted
+    // iterate of hx_solve_system's attempts, its densities
+    DevBuf<double> x_best, x_glob, s_rho;
+    // what the last single-column solve saw: best recursive residual, the
+    // recursive residual it stopped at, and whether the snapshot was returned
+    double last_best_rel = 0.0, last_rec_rel = 0.0;
+    bool last_restored = false;
     // timing
     bool timing = false;
     std::vector<TimerPair> pending;
@@ -223,9 +240,13 @@ void flush_timing(hx_ctx* c) {
     c->pending.clear();
 }
 
+// Destroys the cached graphs once the stream has drained, so no exec is
+// destroyed while a launch of it may still be running.
 void clear_graphs(hx_ctx* c) {
+    if (!c->graphs.empty() && c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
     c->graphs.clear();
+    c->graphs_gen = g_buf_gen;
 }
 
 // Launch one batch of iterations.  Without timing, the batch is captured
@@ -238,6 +259,7 @@ int run_batch(hx_ctx* c, const std::string& key, F&& launch_batch) {
         HXK();
         return HX_OK;
     }
+    if (c->graphs_gen != g_buf_gen) clear_graphs(c);  // a buffer moved since the capture
     hipGraphExec_t exec = nullptr;
     for (auto& g : c->graphs)
         if (g.first == key) exec = g.second;
@@ -347,7 +369,20 @@ struct SolveVecs {
     float *x32, *r32, *p32, *q32;
     double* p2;   // second buffer of the double-buffered search direction
     float* p32b;
+    // single column: best-iterate snapshot (GPUSolver.jl:492-499), returned
+    // in x when the solve fails; nullptr: none (block solves)
+    double* xbest = nullptr;
 };
+
+// A solve that did not converge returns its best iterate (GPUSolver.jl:603-606):
+// the snapshot, which starts as the initial iterate (:441) and is replaced
+// at check points by any better iterate with a residual below 1e-2.
+bool restore_best(hx_ctx* c, const SolveVecs& v, long n) {
+    const int st = c->st_host[0].status;
+    if (!v.xbest || v.nrhs != 1 || st == 0 || st == 10) return false;
+    (void)hipMemcpyAsync(v.x, v.xbest, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
+    return true;
+}
 
 bool all_done(const hx_ctx* c, int nrhs) {
     for (int k = 0; k < nrhs; ++k)
@@ -549,6 +584,7 @@ int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& 
     launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream, nr);
     if (o.warm_start) launch_mask<double>(n, c->minv.p, v.x, c->stream, nr);
     else launch_fill(n * nr, 0.0, v.x, c->stream);
+    if (v.xbest) HXC(hipMemcpyAsync(v.xbest, v.x, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     mv64(c, v.x, v.q, c->E.p, 0.0, MV_PLAIN, o.matvec, nr);
     launch_pcg_init<double>(n, v.f, v.x, v.q, c->minv.p, v.r, v.p, o.shift, c->st, c->partials.p, c->stream, nr);
     if (ml && (rc = pc_apply64(c, vc, v.r, 1))) return rc;
@@ -584,9 +620,13 @@ int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& 
             }
         });
         if (rc) return rc;
+        if (v.xbest) launch_snapshot(n, v.x, v.xbest, c->st, c->stream);  // a check point per batch
         if ((rc = sync_state(c, nr))) return rc;
     }
     std::vector<PcgState> fin(c->st_host, c->st_host + nr);
+    c->last_best_rel = fin[0].best_rel;
+    c->last_rec_rel = fin[0].rel;
+    c->last_restored = restore_best(c, v, n);
     // true residual of the returned iterate (FP64)
     mv64(c, v.x, v.q, c->E.p, 0.0, MV_PLAIN, o.matvec, nr);
     launch_true_residual(n, v.f, v.q, v.x, c->minv.p, o.shift, nullptr, nullptr, c->st, c->partials.p, 1, c->stream,
@@ -640,6 +680,7 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
     launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream, nr);
     if (o.warm_start) launch_mask<double>(n, c->minv.p, v.x, c->stream, nr);
     else launch_fill(n * nr, 0.0, v.x, c->stream);
+    if (v.xbest) HXC(hipMemcpyAsync(v.xbest, v.x, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     launch_fill32(n * nr, 0.0f, v.x32, c->stream);
     launch_fill32(n * nr, 0.0f, v.p32, c->stream);
     mv64(c, v.x, v.q, c->E.p, 0.0, MV_PLAIN, HX_MV_PARITY, nr);
@@ -689,11 +730,29 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
         mv64(c, v.x, v.q, c->E.p, 0.0, MV_PLAIN, HX_MV_PARITY, nr);
         launch_true_residual(n, v.f, v.q, v.x, c->minv.p, o.shift, nullptr, v.r32, c->st, c->partials.p, 2,
                              c->stream, nr);
+        // check point: x is the FP64 iterate and rel its true residual, unless
+        // the phase failed (its folded correction is then not an iterate)
+        if (v.xbest && c->st_host[0].status != 4 && c->st_host[0].status != 5)
+            launch_snapshot(n, v.x, v.xbest, c->st, c->stream);
         if (ml && (rc = pc_apply32(c, vc, v.r32, 2))) return rc;
         launch_ir_resume(c->st, o.tol, c->ir_delta, 0, c->stream, nr);
         HXK();
         ++refinements;
         if ((rc = sync_state(c, nr))) return rc;
+    }
+    c->last_best_rel = c->st_host[0].best_rel;
+    c->last_rec_rel = c->st_host[0].rel;
+    c->last_restored = false;
+    if (restore_best(c, v, n)) {  // report the true residual of the snapshot
+        c->last_restored = true;
+        const int status = c->st_host[0].status, iters = c->st_host[0].iter;
+        mv64(c, v.x, v.q, c->E.p, 0.0, MV_PLAIN, HX_MV_PARITY, 1);
+        launch_true_residual(n, v.f, v.q, v.x, c->minv.p, o.shift, nullptr, nullptr, c->st, c->partials.p, 0,
+                             c->stream, 1);
+        HXK();
+        if ((rc = sync_state(c, 1))) return rc;
+        c->st_host[0].status = status;
+        c->st_host[0].iter = iters;
     }
     fill_stats(c, nr, refinements, s);
     return worst_code(c, nr);
@@ -869,7 +928,7 @@ int pcg_learned(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
 
 SolveVecs single_vecs(hx_ctx* c) {
     return SolveVecs{1, c->f.p, c->x.p, c->r.p, c->p.p, c->q.p, c->x32.p, c->r32.p, c->p32.p, c->q32.p,
-                     c->p2.p, c->p32b.p};
+                     c->p2.p, c->p32b.p, c->x_best.p};
 }
 
 // ------------------------------------------------------------- Helmholtz
@@ -1005,7 +1064,7 @@ int download(hx_ctx* c, void* dst, const void* src, size_t bytes) {
 // =================================================================== C ABI
 extern "C" {
 
-int hx_version(void) { return 10; }
+int hx_version(void) { return 11; }
 
 int hx_device_count(int* count) {
     if (!count) return HX_ERR_BAD_ARG;
@@ -1064,7 +1123,7 @@ void hx_ctx_destroy(hx_ctx* c) {
                            &c->e_pr, &c->e_vm, &c->e_sv, &c->e_l1, &c->e_dir, &c->e_ce, &c->e_dc, &c->e_aux,
                            &c->small, &c->t_x, &c->t_xphys, &c->t_xnew, &c->t_dc, &c->t_dcf, &c->t_dv, &c->t_vm,
                            &c->t_ones, &c->b_f, &c->b_x, &c->b_r, &c->b_p, &c->b_q, &c->b_p2, &c->p2, &c->l_z, &c->l_zold,
-                           &c->l_ustar, &c->l_samples, &c->l_rho};
+                           &c->l_ustar, &c->l_samples, &c->l_rho, &c->x_best, &c->x_glob, &c->s_rho};
     for (auto b : d) b->release();
     DevBuf<float>* fl[] = {&c->Kbin32_d, &c->E32, &c->x32, &c->r32, &c->p32, &c->q32, &c->minv32,
                            &c->b_x32, &c->b_r32, &c->b_p32, &c->b_q32, &c->p32b, &c->b_p32b,
@@ -1156,6 +1215,7 @@ int hx_grid_set(hx_ctx* c, int nx, int ny, int nz, double dx, double dy, double 
     HXA(c->q, n);
     HXA(c->minv, n);
     HXA(c->p2, n);
+    HXA(c->x_best, n);
     HXA(c->Kbin_d, 576);
     HXA(c->Kbin32_d, 576);
     const long np = std::max<long>({3L * kVecBlocks + 8, (long)matvec_parity_blocks(g, 0, 4) + 8,
@@ -1254,10 +1314,18 @@ static int check_opts(hx_ctx* c, const hx_pcg_opts* o) {
     return HX_OK;
 }
 
+// GPUSolver.jl:132-134: a solve needs at least one element with stiffness
+bool any_active(const double* Ee, long ne) {
+    for (long e = 0; e < ne; ++e)
+        if (Ee[e] != 0.0) return true;
+    return false;
+}
+
 int hx_pcg_solve(hx_ctx* c, const double* Ee, const double* f, double* u, const hx_pcg_opts* o, hx_pcg_stats* s) {
     if (!c || !Ee || !f || !u) return HX_ERR_BAD_ARG;
     if (int rc = ensure_grid(c)) return rc;
     if (int rc = check_opts(c, o)) return rc;
+    if (!any_active(Ee, c->g.ne)) return fail(c, HX_ERR_NO_ACTIVE, "no active elements (every modulus is 0)");
     const long n = ndof(c);
     int rc;
     if ((rc = upload(c, c->E.p, Ee, c->g.ne * sizeof(double))) || (rc = upload(c, c->f.p, f, n * sizeof(double))))
@@ -1266,6 +1334,131 @@ int hx_pcg_solve(hx_ctx* c, const double* Ee, const double* f, double* u, const 
     const int prc = pcg_dispatch(c, *o, s);
     if ((rc = download(c, u, c->x.p, n * sizeof(double)))) return rc;
     return prc;
+}
+
+int hx_solve_system(hx_ctx* c, const double* density, const double* f, double* u, const hx_solve_opts* o,
+                    hx_solve_stats* s) {
+    if (!c || !density || !f || !u || !o) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    if (!(o->tol > 0) || o->max_iter < 1 || !(o->shift_factor >= 0) || o->max_attempts < 1 ||
+        !(o->min_stiffness_threshold >= 0))
+        return fail(c, HX_ERR_BAD_ARG, "invalid solve options");
+    if (o->precision != HX_FP64 && o->precision != HX_FP32_IR) return fail(c, HX_ERR_BAD_ARG, "precision");
+    if (o->precond != HX_PC_JACOBI && o->precond != HX_PC_MULTILEVEL && o->precond != HX_PC_VCYCLE)
+        return fail(c, HX_ERR_BAD_ARG, "preconditioner (jacobi, multilevel or vcycle)");
+    const long ne = c->g.ne, n = ndof(c);
+    hx_solve_stats st{};
+    HXA(c->s_rho, ne);
+    HXA(c->x_glob, n);
+    int rc;
+    if ((rc = upload(c, c->s_rho.p, density, ne * sizeof(double))) || (rc = upload(c, c->f.p, f, n * sizeof(double))))
+        return rc;
+    if (o->warm_start && (rc = upload(c, c->x.p, u, n * sizeof(double)))) return rc;
+    if (o->precision == HX_FP32_IR && (rc = ensure_fp32(c))) return rc;
+    hipEvent_t ev0 = get_event(c), ev1 = get_event(c);
+    struct Back {  // events back to the pool on every return
+        hx_ctx* c;
+        hipEvent_t a, b;
+        ~Back() {
+            c->free_events.push_back(a);
+            c->free_events.push_back(b);
+        }
+    } back{c, ev0, ev1};
+    (void)hipEventRecord(ev0, c->stream);
+    // element selection (Solver.jl:61-65: the pruning threshold is an FP32 product)
+    const double thr = o->min_stiffness_threshold;
+    const double thr_keep = (double)((float)thr * 1.01f);
+    launch_count_active(ne, c->s_rho.p, thr, thr_keep, c->partials.p, c->small.p, c->stream);
+    HXK();
+    double counts[2];
+    if ((rc = download(c, counts, c->small.p, 2 * sizeof(double)))) return rc;
+    st.n_active = (long)counts[0];
+    const long n_keep = (long)counts[1];
+    st.pruned = (o->prune_voids && (double)n_keep < 0.99 * (double)ne) ? 1 : 0;
+    st.n_kept = st.pruned ? n_keep : st.n_active;
+    if (st.n_kept == 0) {
+        if (s) *s = st;
+        return fail(c, HX_ERR_NO_ACTIVE, "no active elements found (threshold " + std::to_string(thr) + ")");
+    }
+    launch_solve_moduli(ne, c->s_rho.p, o->E, thr, thr_keep, st.pruned, c->E.p, c->stream);
+    // shift = factor * the largest free diagonal entry (GPUSolver.jl:315-316, 391)
+    launch_elastic_diag(c->g, c->kd, c->E.p, c->q.p, c->stream);
+    launch_free_absmax(n, c->q.p, c->has_fixed ? c->fixed.p : nullptr, c->partials.p, c->small.p + 2, c->stream);
+    HXK();
+    double max_diag = 0.0;
+    if ((rc = download(c, &max_diag, c->small.p + 2, sizeof(double)))) return rc;
+    double factor = o->shift_factor;
+    bool success = false, have_glob = false;
+    double glob_rel = INFINITY;
+    int last_code = HX_OK;
+    for (int attempt = 1; attempt <= o->max_attempts; ++attempt) {
+        const double shift = factor * max_diag;
+        hx_pcg_opts po{o->tol, o->max_iter, shift, o->precision, o->precond, (o->warm_start && attempt == 1) ? 1 : 0,
+                       HX_MV_AUTO, 0};
+        hx_pcg_stats ps{};
+        const int prc = run_pcg(c, po, &ps, single_vecs(c));
+        if (prc == HX_ERR_HIP || prc == HX_ERR_OOM || prc == HX_ERR_STATE || prc == HX_ERR_BAD_ARG) return prc;
+        st.attempts = attempt;
+        st.iterations += ps.iterations;
+        st.shift = shift;
+        st.rel_res = c->last_rec_rel;
+        st.best_rel = c->last_best_rel;
+        st.true_rel = ps.rel_res;
+        if (ps.status == HX_ZERO_RHS) {  // zeros, 0.0, "None"
+            launch_fill(n, 0.0, c->x.p, c->stream);
+            st.outcome = HX_SOLVE_ZERO_RHS;
+            st.rel_res = st.true_rel = st.best_rel = 0.0;
+            success = true;
+            break;
+        }
+        if (prc == HX_OK) {
+            st.outcome = HX_SOLVE_CONVERGED;
+            success = true;
+            break;
+        }
+        last_code = prc;
+        const double best = c->last_best_rel;
+        const bool diverged = prc == HX_ERR_BREAKDOWN || prc == HX_ERR_DIVERGED;
+        const bool failed_badly = diverged || !(best <= 0.1);
+        if (!failed_badly && best < 5e-2) {  // acceptable stagnation (GPUSolver.jl:549-557): x is the best iterate
+            st.outcome = HX_SOLVE_ACCEPTED;
+            success = true;
+            break;
+        }
+        if (failed_badly) {
+            factor *= 10.0;
+        } else {
+            if (best < glob_rel) {
+                glob_rel = best;
+                HXC(hipMemcpyAsync(c->x_glob.p, c->x.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+                have_glob = true;
+            }
+            factor *= 5.0;
+        }
+    }
+    int code = HX_OK;
+    if (!success) {  // "All solver attempts failed. Returning best stagnated result." (GPUSolver.jl:603-606)
+        st.outcome = HX_SOLVE_FAILED;
+        code = HX_ERR_NOT_CONVERGED;
+        if (have_glob) HXC(hipMemcpyAsync(c->x.p, c->x_glob.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        else launch_fill(n, 0.0, c->x.p, c->stream);
+        mv64(c, c->x.p, c->q.p, c->E.p, 0.0, MV_PLAIN, HX_MV_AUTO);
+        launch_true_residual(n, c->f.p, c->q.p, c->x.p, c->minv.p, st.shift, nullptr, nullptr, c->st, c->partials.p, 0,
+                             c->stream);
+        HXK();
+        if ((rc = sync_state(c))) return rc;
+        st.true_rel = c->st_host[0].rel;
+        (void)last_code;
+    }
+    (void)hipEventRecord(ev1, c->stream);
+    if ((rc = download(c, u, c->x.p, n * sizeof(double)))) return rc;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev0, ev1);
+    st.seconds = ms * 1e-3;
+    st.status = code;
+    if (s) *s = st;
+    if (code != HX_OK) return fail(c, code, "all solver attempts failed; best stagnated result returned");
+    return HX_OK;
 }
 
 int hx_block_pcg_solve(hx_ctx* c, int nrhs, const double* Ee, const double* F, double* U, const hx_pcg_opts* o,
@@ -1493,7 +1686,15 @@ int hx_to_step(hx_ctx* c, hx_to_stats* s) {
     if (!c->to_ready) return fail(c, HX_ERR_STATE, "hx_to_init first");
     const hx_to_params& p = c->tp;
     const long ne = c->g.ne, n = ndof(c);
-    hipEvent_t ev0 = get_event(c), ev1 = get_event(c);
+    struct Events {  // back to the pool on every return
+        hx_ctx* c;
+        hipEvent_t a, b;
+        ~Events() {
+            c->free_events.push_back(a);
+            c->free_events.push_back(b);
+        }
+    } evs{c, get_event(c), get_event(c)};
+    hipEvent_t ev0 = evs.a, ev1 = evs.b;
     (void)hipEventRecord(ev0, c->stream);
     int rc;
     // 1. SIMP moduli of the physical densities
@@ -1504,7 +1705,21 @@ int hx_to_step(hx_ctx* c, hx_to_stats* s) {
     if (p.precision == HX_FP32_IR)
         if ((rc = ensure_fp32(c))) return rc;
     const int prc = run_pcg(c, po, &ps, single_vecs(c));
-    if (prc == HX_ERR_HIP || prc == HX_ERR_OOM) return prc;
+    if (prc != HX_OK) {
+        // the densities stay as they are: a failed solve (its best iterate is
+        // left in the displacement) gives no sensitivities to update them with
+        (void)hipStreamSynchronize(c->stream);
+        if (s) {
+            *s = hx_to_stats{};
+            s->iter = c->to_iter;
+            s->pcg_iterations = ps.iterations;
+            s->pcg_rel_res = ps.rel_res;
+            s->pcg_status = ps.status;
+        }
+        if (prc == HX_ERR_HIP || prc == HX_ERR_OOM) return prc;
+        return fail(c, prc, "TO step: the elasticity solve failed (" + std::to_string(prc) +
+                                "); densities not updated");
+    }
     // 3. compliance, sensitivities, von Mises
     launch_dot(n, c->f.p, c->x.p, c->small.p, &c->st->ticket[0], c->partials.p, c->stream);
     ElemArgs a{};
@@ -1545,8 +1760,6 @@ int hx_to_step(hx_ctx* c, hx_to_stats* s) {
     if ((rc = download(c, sm, c->small.p, 3 * sizeof(double)))) return rc;
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ev0, ev1);
-    c->free_events.push_back(ev0);
-    c->free_events.push_back(ev1);
     c->to_iter += 1;
     if (s) {
         s->iter = c->to_iter;
@@ -2211,6 +2424,16 @@ int hx_device_info(hx_ctx* c, int* cu_count, int* clock_khz, double* mem_gb) {
     if (cu_count) *cu_count = pr.multiProcessorCount;
     if (clock_khz) *clock_khz = pr.clockRate;
     if (mem_gb) *mem_gb = pr.totalGlobalMem / 1073741824.0;
+    return HX_OK;
+}
+
+int hx_device_memory(hx_ctx* c, double* used_gb, double* total_gb) {
+    if (!c) return HX_ERR_BAD_ARG;
+    (void)hipSetDevice(c->device);
+    size_t fr = 0, tot = 0;
+    HXC(hipMemGetInfo(&fr, &tot));
+    if (used_gb) *used_gb = (double)(tot - fr) / 1073741824.0;
+    if (total_gb) *total_gb = (double)tot / 1073741824.0;
     return HX_OK;
 }
 

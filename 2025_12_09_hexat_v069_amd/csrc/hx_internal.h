@@ -138,6 +138,7 @@ struct PcgState {
     double tol;
     double shift;
     double scal[6];  // spare scalar slots (compliance, OC bracket, change ...)
+    double snap_rel; // residual of the last best-iterate snapshot (infinite: none)
     int iter;
     int max_iter;
     int done;        // 0 running, 1 finished

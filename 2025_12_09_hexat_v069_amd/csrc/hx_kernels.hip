@@ -30,6 +30,7 @@ __global__ void k_pcg_reset(PcgState* st0, double tol, double shift, int max_ite
         PcgState* st = st0 + threadIdx.x;
         st->rz = st->pq = st->alpha = st->beta = st->rr = st->normb = st->rel = 0.0;
         st->best_rel = INFINITY;
+        st->snap_rel = INFINITY;
         st->tol = tol;
         st->shift = shift;
         for (int i = 0; i < 6; ++i) st->scal[i] = 0.0;
@@ -912,6 +913,75 @@ __global__ __launch_bounds__(NT) void k_mul(long n, const double* __restrict__ a
     for (long i = gtid(); i < n; i += gstride()) out[i] = a[i] * b[i];
 }
 
+// Best-iterate snapshot (GPUSolver.jl:492-499): xbest = x when the state's
+// residual is below 1e-2 and below the last snapshot's.  The state is only
+// read here, so every block takes the same decision; k_snapshot_fin records
+// the snapshot's residual afterwards.
+__global__ __launch_bounds__(NT) void k_snapshot(long n, const double* __restrict__ x, double* __restrict__ xbest,
+                                                 const PcgState* st) {
+    const double rel = st->rel;
+    if (!(rel < 1e-2 && rel < st->snap_rel)) return;
+    typedef Vec16<double>::type VT;
+    const long nv = n / 2;
+    for (long v = gtid(); v < nv; v += gstride()) ((VT*)xbest)[v] = ((const VT*)x)[v];
+    for (long i = 2 * nv + gtid(); i < n; i += gstride()) xbest[i] = x[i];
+}
+__global__ void k_snapshot_fin(PcgState* st) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const double rel = st->rel;
+    if (rel < 1e-2 && rel < st->snap_rel) st->snap_rel = rel;
+}
+
+// Element selection of a solve (GPUSolver.jl:128-129, Solver.jl:60-67):
+// per block, the count of rho >= thr (active) and of rho > thr_keep (kept
+// by the void pruning)
+__global__ __launch_bounds__(NT) void k_count_active(long ne, const double* __restrict__ rho, double thr,
+                                                     double thr_keep, double* partials) {
+    __shared__ double red[NT / 64];
+    double na = 0.0, nk = 0.0;
+    for (long e = gtid(); e < ne; e += gstride()) {
+        const double r = rho[e];
+        na += r >= thr ? 1.0 : 0.0;
+        nk += r > thr_keep ? 1.0 : 0.0;
+    }
+    const double v0 = block_sum<NT>(na, red);
+    const double v1 = block_sum<NT>(nk, red);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x + 0] = v0;
+        partials[2 * blockIdx.x + 1] = v1;
+    }
+}
+__global__ __launch_bounds__(NT) void k_fin_sum2(const double* partials, unsigned nb, double* out) {
+    __shared__ double red[NT / 64];
+    double sv[2];
+    reduce_partials_n<NT, 2>(partials, 2, nb, red, sv);
+    if (threadIdx.x == 0) {
+        out[0] = sv[0];
+        out[1] = sv[1];
+    }
+}
+// moduli E rho of the elements in the operator: rho >= thr and, with the
+// pruning, rho > thr_keep (the linear factor of GPUSolver.jl:147); 0 removes
+// the element, and the DOFs it alone carried leave the system (zero diagonal)
+__global__ __launch_bounds__(NT) void k_solve_moduli(long ne, const double* __restrict__ rho, double E, double thr,
+                                                     double thr_keep, int prune, double* __restrict__ Eout) {
+    for (long e = gtid(); e < ne; e += gstride()) {
+        const double r = rho[e];
+        const bool on = r >= thr && (!prune || r > thr_keep);
+        Eout[e] = on ? E * r : 0.0;
+    }
+}
+// per block, the largest |diag| over free DOFs (GPUSolver.jl:315-316)
+__global__ __launch_bounds__(NT) void k_free_absmax(long n, const double* __restrict__ diag,
+                                                    const uint8_t* __restrict__ fixed, double* partials) {
+    __shared__ double red[NT / 64];
+    double m = 0.0;
+    for (long i = gtid(); i < n; i += gstride())
+        if (fixed == nullptr || fixed[i] == 0) m = fmax(m, fabs(diag[i]));
+    const double v = block_max<NT>(m, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = v;
+}
+
 inline unsigned blocks_for(long n) { return (unsigned)((n + NT - 1) / NT); }
 
 }  // namespace
@@ -991,6 +1061,25 @@ template void launch_mask<double>(long, const double*, double*, hipStream_t, int
 template void launch_mask<float>(long, const float*, float*, hipStream_t, int);
 void launch_accumulate(long n, double* x64, float* x32, hipStream_t s, int nrhs) {
     hipLaunchKernelGGL(k_accumulate, dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, x64, x32);
+}
+
+void launch_snapshot(long n, const double* x, double* xbest, PcgState* st, hipStream_t s) {
+    hipLaunchKernelGGL(k_snapshot, dim3(kVecBlocks), dim3(NT), 0, s, n, x, xbest, (const PcgState*)st);
+    hipLaunchKernelGGL(k_snapshot_fin, dim3(1), dim3(64), 0, s, st);
+}
+void launch_count_active(long ne, const double* rho, double thr, double thr_keep, double* partials, double* out,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_count_active, dim3(kVecBlocks), dim3(NT), 0, s, ne, rho, thr, thr_keep, partials);
+    hipLaunchKernelGGL(k_fin_sum2, dim3(1), dim3(NT), 0, s, (const double*)partials, (unsigned)kVecBlocks, out);
+}
+void launch_solve_moduli(long ne, const double* rho, double E, double thr, double thr_keep, int prune, double* Eout,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_solve_moduli, dim3(kVecBlocks), dim3(NT), 0, s, ne, rho, E, thr, thr_keep, prune, Eout);
+}
+void launch_free_absmax(long n, const double* diag, const uint8_t* fixed, double* partials, double* out,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_free_absmax, dim3(kVecBlocks), dim3(NT), 0, s, n, diag, fixed, partials);
+    launch_max_reduce(kVecBlocks, partials, kVecBlocks, out, s);
 }
 
 void launch_helm_rhs(const GridArgs& g, const double* in_e, double vol8, double* b, hipStream_t s) {

@@ -137,6 +137,18 @@ template <typename T>
 void launch_mask(long n, const T* minv, T* x, hipStream_t s, int nrhs = 1);
 // x64 += x32; x32 = 0
 void launch_accumulate(long n, double* x64, float* x32, hipStream_t s, int nrhs = 1);
+// xbest = x when st->rel < min(1e-2, st->snap_rel); then snap_rel = rel
+// (best-iterate snapshot of one column)
+void launch_snapshot(long n, const double* x, double* xbest, PcgState* st, hipStream_t s);
+// out[0] = #(rho >= thr), out[1] = #(rho > thr_keep) (kVecBlocks x 2 partials)
+void launch_count_active(long ne, const double* rho, double thr, double thr_keep, double* partials, double* out,
+                         hipStream_t s);
+// Eout = E rho where rho >= thr (and rho > thr_keep with prune), else 0
+void launch_solve_moduli(long ne, const double* rho, double E, double thr, double thr_keep, int prune, double* Eout,
+                         hipStream_t s);
+// out[0] = max |diag| over the DOFs not fixed (kVecBlocks partials)
+void launch_free_absmax(long n, const double* diag, const uint8_t* fixed, double* partials, double* out,
+                        hipStream_t s);
 
 // ---- flexible CG pieces (learned preconditioner) ----------------------------
 // Truncated flexible CG: the new direction is A-orthogonalised against the

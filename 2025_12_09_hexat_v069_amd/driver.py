@@ -27,12 +27,35 @@ LOG_HEADER = ("| Iter | Mesh Size | Total El | Active El | Radius | Cutoff | Com
               "---------------|----------|-----------|---------|----------|--------|----------|-----------|------|")
 
 
-def format_log_line(it, dims, n_total, n_active, R, thr, comp, se, avg_l1, vf, drho, status, t, res, prec):
-    """Diagnostics.jl:52-83 line format."""
+def format_log_line(it, dims, n_total, n_active, R, thr, comp, se, avg_l1, vf, drho, status, t, res, prec,
+                    wall=None, vram=None):
+    """Diagnostics.jl:52-83 line format (wall: HH:MM:SS, now by default; vram:
+    device memory in use, "%.1fG")."""
+    wall = wall if wall is not None else datetime.datetime.now().strftime("%H:%M:%S")
+    vram = vram if vram is not None else _vram_str()
     return ("| %4d | %9s | %8d | %9d | %6.3f | %6.3f | %10.3e | %13.3e | %13.3e | %8.4f | %8.2f%% | %7s | %8.1e "
             "| %-6s | %8.2f | %9s | %4s |" % (it, dims, n_total, n_active, R, thr, comp, se, avg_l1, vf, drho * 100,
-                                              status, res, prec, t, datetime.datetime.now().strftime("%H:%M:%S"),
-                                              "HBM"))
+                                              status, res, prec, t, wall, vram))
+
+
+def _vram_str():
+    """Diagnostics.jl:32-39: device memory in use."""
+    try:
+        from .solver import get_context
+        used = get_context().memory_used_gb()
+        return "%.1fG" % used
+    except Exception:
+        return "CPU"
+
+
+def init_log_file(path, config):
+    """Diagnostics.jl:41-49"""
+    with open(path, "w") as fh:
+        fh.write("HEXA FEM TOPOLOGY OPTIMIZATION LOG\n")
+        fh.write("Start Date: %s\n" % datetime.datetime.now().isoformat(timespec="milliseconds"))
+        fh.write("Config Geometry: %s\n" % (config.get("geometry"),))
+        fh.write("=" * 200 + "\n")
+        fh.write(LOG_HEADER + "\n")
 
 
 def build_problem(config):
@@ -54,8 +77,9 @@ def build_problem(config):
     return geo, nodes, elements, bc, F
 
 
-def run_stress_ratio(config, n_iter=None, log_path=None, verbose=True):
-    """Main.jl:331-474 without refinement/export."""
+def run_stress_ratio(config, n_iter=None, log_path=None, verbose=True, keep_density=False):
+    """Main.jl:331-474 without refinement/export.  keep_density: each
+    iteration's record also holds a copy of the density after its update."""
     geo, nodes, elements, bc, F = build_problem(config)
     E = np.float32(config["material"]["E"])
     nu = np.float32(config["material"]["nu"])
@@ -70,9 +94,7 @@ def run_stress_ratio(config, n_iter=None, log_path=None, verbose=True):
     dims = f"{geo['nElem_x']}x{geo['nElem_y']}x{geo['nElem_z']}"
     hist = []
     if log_path:
-        with open(log_path, "w") as fh:
-            fh.write("HEXA FEM TOPOLOGY OPTIMIZATION LOG\nStart Date: %s\n%s\n%s\n" % (
-                datetime.datetime.now().isoformat(), "=" * 200, LOG_HEADER))
+        init_log_file(log_path, config)
     for it in range(1, n_iter + 1):
         t0 = time.perf_counter()
         config["current_outer_iter"] = it
@@ -90,10 +112,14 @@ def run_stress_ratio(config, n_iter=None, log_path=None, verbose=True):
         dt = time.perf_counter() - t0
         rec = dict(iter=it, compliance=comp, strain_energy=0.5 * comp, avg_l1=avg_l1, vol_frac=vf, delta_rho=mc,
                    radius=R, cutoff=thr, lin_res=res, prec=prec, time=dt, active=n_active)
+        if keep_density:
+            rec["density"] = density.copy()
         hist.append(rec)
         line = format_log_line(it, dims, density.size, n_active, R, thr, comp, 0.5 * comp, avg_l1, vf, mc,
                                "Nominal", dt, res, prec)
         if verbose:
+            if it == 1 or it % 10 == 0:  # Diagnostics.jl:77-79
+                print(LOG_HEADER)
             print(line, flush=True)
         if log_path:
             with open(log_path, "a") as fh:

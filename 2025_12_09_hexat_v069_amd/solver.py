@@ -1,20 +1,25 @@
 """Elasticity solve entry points (mirror of Solver.jl:38-164,
 IterativeSolver.jl:20-43 and GPUSolver.jl:274-627) on libhexat's HIP PCG.
 
-Semantics kept from the reference:
+The whole solve policy runs in the library (``hx_solve_system``), on device
+data:
 * active elements: rho >= min_stiffness_threshold, modulus E*rho (linear,
-  GPUSolver.jl:128-147); with ``prune_voids`` and fewer than 99 % active
-  elements (rho > 1.01*threshold), voids and the nodes they orphan leave the
-  system (Solver.jl:60-91): here their moduli are 0 and orphan DOFs are masked,
-  which is the same reduced system without re-indexing;
+  GPUSolver.jl:128-147); with ``prune_voids`` and fewer than 99 % of the
+  elements above 1.01*threshold, voids and the nodes they orphan leave the
+  system (Solver.jl:60-91, MeshPruner.jl:18-138): their moduli are 0 and the
+  orphan DOFs carry no stiffness, so they are not solved for and return 0; a
+  warm start keeps u_prev on the kept nodes (Solver.jl:71-83);
 * shift s = shift_factor * max(diag over free DOFs) (GPUSolver.jl:391);
-* adaptive tolerance 10^((1-t)(-5) + t log10(tol)) with the GPU floor 5e-7
-  (Solver.jl:99-120) in ``parity`` mode; ``northstar`` mode uses the
-  configured tolerance as is;
-* retries with the shift multiplied by 10 (hard failure) or 5 (stagnation),
-  acceptance of a stagnated result below 5e-2 (GPUSolver.jl:531-593); no AMG
-  (the CPU AMG branch is out of scope; "jacobi_amg" runs as Jacobi);
+* three attempts: acceptance of a stagnated attempt below 5e-2, otherwise the
+  shift factor x10 (hard failure) or x5 (stagnation, its best iterate kept),
+  and the best kept iterate (or zeros) when all fail (GPUSolver.jl:531-606);
+  no AMG (the CPU AMG branch is out of scope; "amg" and "jacobi_amg" run as
+  Jacobi);
 * zero RHS returns zeros, 0.0, "None" (GPUSolver.jl:373).
+
+``solve_system`` adds the adaptive tolerance 10^((1-t)(-5) + t log10(tol))
+with the GPU floor 5e-7 (Solver.jl:99-120) in ``parity`` mode; ``northstar``
+mode uses the configured tolerance as is.
 
 Results are float64 (the reference returns Float32).
 """
@@ -57,63 +62,49 @@ def _precision_of(config):
 
 
 def _precond_of(config):
-    """solver_parameters.preconditioner: "jacobi" (the reference's), "multilevel" or "vcycle" (added here)."""
+    """solver_parameters.preconditioner: "jacobi" (the reference's; "amg" and
+    "jacobi_amg" also run Jacobi, the CPU AMG branch being out of scope),
+    "multilevel" or "vcycle" (added here)."""
     sp = (config or {}).get("solver_parameters", {})
     p = str(sp.get("preconditioner", "jacobi")).lower()
-    kinds = {"jacobi": (_lib.HX_PC_JACOBI, "Jacobi"), "multilevel": (_lib.HX_PC_MULTILEVEL, "Multilevel"),
+    kinds = {"jacobi": (_lib.HX_PC_JACOBI, "Jacobi"), "jacobi_amg": (_lib.HX_PC_JACOBI, "Jacobi"),
+             "amg": (_lib.HX_PC_JACOBI, "Jacobi"), "multilevel": (_lib.HX_PC_MULTILEVEL, "Multilevel"),
              "vcycle": (_lib.HX_PC_VCYCLE, "V-cycle")}
     if p not in kinds:
         raise ValueError(f"solver_parameters.preconditioner: {p!r} (jacobi | multilevel | vcycle)")
     return kinds[p]
 
 
-def solve_system_gpu(nodes, elements, E, nu, bc_indicator, f, density, *, max_iter=40000, tol=1e-6,
-                     method="native", solver="cg", use_precond=True, shift_factor=1.0e-6,
-                     min_stiffness_threshold=1.0e-3, u_guess=None, config=None, prune=None):
-    """GPUSolver.jl:611-627 -> (x_full, final_rel_res, prec_name)."""
+def _solve(nodes, elements, E, nu, bc_indicator, f, density, *, max_iter, tol, shift_factor, min_stiffness_threshold,
+           u_guess, config, prune_voids):
     ctx = get_context()
     _bind_grid(ctx, nodes, elements, nu)
-    density = np.asarray(density, dtype=np.float64)
-    active = density >= min_stiffness_threshold
-    if prune is not None:
-        active &= prune
-    if not active.any():
-        raise RuntimeError(f"No active elements found (Threshold: {min_stiffness_threshold}).")
-    Ee = np.where(active, float(E) * density, 0.0)
-    fixed = _bnd.fixed_dof_mask(bc_indicator)
-    ctx.set_fixed(fixed)
-    f = np.asarray(f, dtype=np.float64)
-    free = (fixed == 0)
-    b = np.where(free, f, 0.0)
-    if not np.any(b):  # GPUSolver.jl:372-373
-        return np.zeros(ctx.ndof), 0.0, "None"
-    diag = ctx.diagonal(Ee)
-    live = free & (diag > 0)
-    max_diag = float(np.abs(diag[live]).max()) if live.any() else 0.0
-    prec = _precision_of(config)
+    ctx.set_fixed(_bnd.fixed_dof_mask(bc_indicator))
     pc, pc_name = _precond_of(config)
     u0 = None
     if u_guess is not None and len(u_guess) == ctx.ndof:
         u0 = np.asarray(u_guess, dtype=np.float64)
-    shift_f = float(shift_factor)
-    best_u, best_rel = None, math.inf
-    last_rel = 0.0
-    for attempt in range(1, 4):  # GPUSolver.jl:390-594
-        shift = shift_f * max_diag
-        u, st = ctx.pcg(Ee, f, u0=u0 if attempt == 1 else None, tol=float(tol), max_iter=int(max_iter),
-                        shift=shift, precision=prec, precond=pc)
-        last_rel = st.rel_res
-        if st.status == _lib.HX_OK:
-            return u, st.rel_res, pc_name
-        if st.rel_res < best_rel:
-            best_u, best_rel = u, st.rel_res
-        diverged = st.status in (_lib.HX_ERR_DIVERGED, _lib.HX_ERR_BREAKDOWN)
-        failed_badly = diverged or best_rel > 0.1
-        if not failed_badly and best_rel < 5e-2:  # acceptable stagnation (:549-557)
-            return best_u, best_rel, pc_name
-        shift_f *= 10.0 if failed_badly else 5.0
-    warnings.warn("All solver attempts failed. Returning best stagnated result.")
-    return (best_u if best_u is not None else np.zeros(ctx.ndof)), last_rel, pc_name
+    u, st = ctx.solve_system(np.asarray(density, dtype=np.float64), np.asarray(f, dtype=np.float64), u0,
+                             E=float(E), min_stiffness_threshold=float(min_stiffness_threshold),
+                             prune_voids=bool(prune_voids), tol=float(tol), max_iter=int(max_iter),
+                             shift_factor=float(shift_factor), precision=_precision_of(config), precond=pc)
+    if st.outcome == _lib.HX_SOLVE_ZERO_RHS:  # GPUSolver.jl:372-373
+        return np.zeros(ctx.ndof), 0.0, "None"
+    if st.outcome == _lib.HX_SOLVE_FAILED:
+        warnings.warn("All solver attempts failed. Returning best stagnated result.")
+    return u, st.rel_res, pc_name
+
+
+def solve_system_gpu(nodes, elements, E, nu, bc_indicator, f, density, *, max_iter=40000, tol=1e-6,
+                     method="native", solver="cg", use_precond=True, shift_factor=1.0e-6,
+                     min_stiffness_threshold=1.0e-3, u_guess=None, config=None):
+    """GPUSolver.jl:611-627 -> (x_full, final_rel_res, prec_name).  Raises
+    RuntimeError when no element is active (GPUSolver.jl:132-134)."""
+    if _lib.device_count() == 0:
+        raise RuntimeError("CUDA not found.")  # GPUSolver.jl:620 -- no CPU fallback here
+    return _solve(nodes, elements, E, nu, bc_indicator, f, density, max_iter=max_iter, tol=tol,
+                  shift_factor=shift_factor, min_stiffness_threshold=min_stiffness_threshold, u_guess=u_guess,
+                  config=config, prune_voids=False)
 
 
 def adaptive_tolerance(config, solver_type="gpu"):
@@ -147,19 +138,12 @@ def solve_system(nodes, elements, E, nu, bc_indicator, F, *, density=None, confi
     ne = elements.shape[0]
     if density is None:
         density = np.ones(ne, dtype=np.float32)
-    density = np.asarray(density)
-    prune = None
-    if prune_voids:
-        thr = np.float32(min_stiffness_threshold) * np.float32(1.01)
-        keep = density > thr
-        if keep.sum() < ne * 0.99:  # Solver.jl:65
-            prune = keep
     sp = config["solver_parameters"]
     stype = choose_solver(nodes.shape[0], ne, config)
     mode = str(sp.get("mode", "parity")).lower()
     tol = adaptive_tolerance(config, stype) if mode == "parity" else float(sp.get("tolerance", 1e-8))
-    return solve_system_gpu(nodes, elements, np.float32(E), np.float32(nu), bc_indicator, F, density,
-                            max_iter=int(sp.get("max_iterations", 1000)), tol=tol,
-                            shift_factor=float(sp.get("diagonal_shift_factor", 1.0e-6)),
-                            min_stiffness_threshold=float(min_stiffness_threshold),
-                            u_guess=u_prev, config=config, prune=prune)
+    return _solve(nodes, elements, np.float32(E), np.float32(nu), bc_indicator, F, density,
+                  max_iter=int(sp.get("max_iterations", 1000)), tol=tol,
+                  shift_factor=float(np.float32(sp.get("diagonal_shift_factor", 1.0e-6))),
+                  min_stiffness_threshold=float(np.float32(min_stiffness_threshold)), u_guess=u_prev, config=config,
+                  prune_voids=prune_voids)

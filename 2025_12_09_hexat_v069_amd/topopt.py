@@ -48,6 +48,24 @@ def radius_schedule(opt, iter_, n_iter, max_domain_dim, avg_h):
     return float(R), float(R / np.float32(2.5))
 
 
+def _filter_or_exact(field, nx, ny, nz, dx, dy, dz, R, elements, ftol, what):
+    """The GPU filter; when it signals failure (the empty vector), the
+    reference falls back to a direct CHOLMOD solve of the same system
+    (TopOpt.jl:195-205, 302-321).  Here the fallback solves that system again
+    on the device to the exact answer's precision (relative residual 1e-12,
+    residual checked every iteration, no iteration cap to speak of); if that
+    fails too, the update stops with an error."""
+    out = apply_gpu_filter(field, nx, ny, nz, dx, dy, dz, R, elements, ftol)
+    if out[0].size:
+        return out
+    t0 = time.perf_counter()
+    ex, _, it, res = apply_gpu_filter(field, nx, ny, nz, dx, dy, dz, R, elements, 1e-12, max_iter=100000,
+                                      check_every=1)
+    if ex.size == 0:
+        raise RuntimeError(f"{what} filter failed on the device (direct-solve fallback failed too)")
+    return ex, time.perf_counter() - t0, 1, 0.0
+
+
 def update_density(density, l1_stress_norm_field, protected_elements_mask, E, l1_stress_allowable, iter_,
                    number_of_iterations, original_density, min_density, max_density, config, elements=None,
                    is_annealing=False):
@@ -64,10 +82,8 @@ def update_density(density, l1_stress_norm_field, protected_elements_mask, E, l1
 
     ratio = float(opt.get("stress_regularization_ratio", 0.0))
     if ratio > 1e-6:  # :176-211
-        fs, _, _, _ = apply_gpu_filter(l1_stress_norm_field, nx, ny, nz, dx, dy, dz, ratio * float(avg_h),
-                                       elements, ftol)
-        if fs.size == 0:
-            raise RuntimeError("GPU stress filter failed (the CPU CHOLMOD fallback is out of scope)")
+        fs, _, _, _ = _filter_or_exact(l1_stress_norm_field, nx, ny, nz, dx, dy, dz, ratio * float(avg_h),
+                                       elements, ftol, "stress")
         l1_stress_norm_field[:] = fs
 
     R_len, R_eff = radius_schedule(opt, iter_, number_of_iterations, max_dim, avg_h)
@@ -76,10 +92,8 @@ def update_density(density, l1_stress_norm_field, protected_elements_mask, E, l1
                             min_density, max_density)).astype(density.dtype)
     filt, f_time, f_it, f_res = prop, 0.0, 0, 0.0
     if R_eff > 1e-4:  # :281-322
-        out, f_time, f_it, f_res = apply_gpu_filter(prop, nx, ny, nz, dx, dy, dz, R_eff, elements, ftol)
-        if out.size == 0:
-            raise RuntimeError("GPU density filter failed (the CPU CHOLMOD fallback is out of scope)")
-        filt = out
+        filt, f_time, f_it, f_res = _filter_or_exact(prop, nx, ny, nz, dx, dy, dz, R_eff, elements, ftol,
+                                                     "density")
     new = np.clip(filt, min_density, max_density).astype(density.dtype)
     diff = np.where(prot, 0.0, np.abs(new.astype(np.float64) - density))
     max_change = float(diff.max()) if diff.size else 0.0

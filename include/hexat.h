@@ -108,6 +108,61 @@ int hx_diagonal(hx_ctx* ctx, const double* Ee, double* diag);
  * Replaces GPUSolver.gpu_matrix_free_cg_solve (GPUSolver.jl:274-609). */
 int hx_pcg_solve(hx_ctx* ctx, const double* Ee, const double* f, double* u_inout,
                  const hx_pcg_opts* opts, hx_pcg_stats* stats);
+/* The solve policy of Solver.solve_system (Solver.jl:38-164) and
+ * GPUSolver.gpu_matrix_free_cg_solve (GPUSolver.jl:274-609) on the device.
+ * Elements with density >= min_stiffness_threshold enter the operator with
+ * modulus E*density (GPUSolver.jl:128-147).  With prune_voids, when fewer than
+ * 99 % of the elements have density > 1.01*threshold (the product taken in
+ * FP32, Solver.jl:61-65), only those stay and the nodes no kept element
+ * touches leave the system (MeshPruner.prune_system, MeshPruner.jl:18-138):
+ * here their DOFs get a zero diagonal, are not solved for and return 0, and a
+ * warm start keeps u_prev on the kept nodes (Solver.jl:71-83).
+ * shift = shift_factor * max |diag| over free DOFs (GPUSolver.jl:315-316, 391).
+ * Up to max_attempts PCG attempts (GPUSolver.jl:390-594): converged -> done;
+ * not diverged with best residual < 5e-2 -> accepted (the attempt's best
+ * iterate); diverged / broken down / best > 0.1 -> shift factor x10; else the
+ * attempt's best iterate is kept if it is the best so far and the factor x5.
+ * The reference's switch to CPU AMG ("jacobi_amg") is not built.  After all
+ * attempts fail: the best kept iterate (zeros if none) and
+ * HX_ERR_NOT_CONVERGED.  Zero right-hand side on the free DOFs: zeros,
+ * stats->outcome HX_SOLVE_ZERO_RHS, return HX_OK (GPUSolver.jl:372-373).
+ * No element in the operator: HX_ERR_NO_ACTIVE (GPUSolver.jl:132-134,
+ * MeshPruner.jl:35-37).  density: Ne, f and u_inout: ndof (u_inout holds
+ * u_prev when warm_start). */
+typedef struct {
+    double E;                        /* Young's modulus */
+    double min_stiffness_threshold;  /* density >= this: in the operator */
+    int prune_voids;                 /* Solver.jl:60-91 */
+    double tol;                      /* relative residual target (after the caller's schedule) */
+    int max_iter;
+    double shift_factor;             /* diagonal_shift_factor */
+    int precision;                   /* HX_FP64 | HX_FP32_IR */
+    int precond;                     /* HX_PC_JACOBI | HX_PC_MULTILEVEL | HX_PC_VCYCLE */
+    int max_attempts;                /* 3 in the reference (GPUSolver.jl:368) */
+    int warm_start;                  /* first attempt starts from u_inout (GPUSolver.jl:407-412) */
+} hx_solve_opts;
+
+enum { HX_SOLVE_CONVERGED = 0, HX_SOLVE_ACCEPTED = 1, HX_SOLVE_FAILED = 2, HX_SOLVE_ZERO_RHS = 3 };
+
+typedef struct {
+    int status;         /* hx_status returned */
+    int outcome;        /* HX_SOLVE_* */
+    int attempts;       /* PCG attempts made */
+    int iterations;     /* PCG iterations, all attempts */
+    double rel_res;     /* recursive relative residual where the last attempt stopped
+                           (the reference's final_rel_res_val) */
+    double true_rel;    /* FP64 relative residual of the returned vector, this attempt's shift */
+    double best_rel;    /* best recursive residual of the last attempt */
+    double shift;       /* absolute shift of the last attempt */
+    long n_active;      /* elements with density >= threshold */
+    long n_kept;        /* elements in the operator */
+    int pruned;         /* 1: void pruning applied */
+    double seconds;     /* device time */
+} hx_solve_stats;
+
+int hx_solve_system(hx_ctx* ctx, const double* density, const double* f, double* u_inout,
+                    const hx_solve_opts* opts, hx_solve_stats* stats);
+
 /* nrhs independent systems with one shared operator, solved together
  * (fused multi-RHS kernels).  F, U are nrhs x ndof row-major (RHS-major).
  * stats: nrhs entries.  No reference counterpart (the reference solves one
@@ -277,6 +332,9 @@ int hx_timing_enable(hx_ctx* ctx, int on);
 int hx_timing_read(hx_ctx* ctx, int kernel, double* avg_ms, long* count);
 /* hardware facts used for roofline reporting */
 int hx_device_info(hx_ctx* ctx, int* cu_count, int* clock_khz, double* mem_gb);
+/* device memory in use and in total (GiB): the VRAM column of the iteration
+ * log (Diagnostics.jl:32-39) */
+int hx_device_memory(hx_ctx* ctx, double* used_gb, double* total_gb);
 
 #ifdef __cplusplus
 }

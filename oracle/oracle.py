@@ -362,6 +362,94 @@ def parity_update_density(g: Grid, density, l1, protected, E, l1_allowable, iter
     return max_change, R_len, thr
 
 
+def used_dofs(g: Grid, Ee):
+    """DOF mask of the nodes touched by an element with non-zero modulus
+    (MeshPruner.jl:39-48: the nodes an active element uses)."""
+    on = (np.asarray(Ee) != 0).reshape(g.nz, g.ny, g.nx)
+    used = np.zeros((g.nz + 1, g.ny + 1, g.nx + 1), bool)
+    for dk in (0, 1):
+        for dj in (0, 1):
+            for di in (0, 1):
+                used[dk:dk + g.nz, dj:dj + g.ny, di:di + g.nx] |= on
+    return np.repeat(used.ravel(), 3)
+
+
+def parity_tolerance(base_tol, iter_, n_iter, gpu=True):
+    """Solver.jl:99-120: 10^((1-t)(-5) + t log10(tol)) in Float32, GPU floor 5e-7."""
+    t = min(max(np.float32(iter_) / np.float32(n_iter), 0.0), 1.0)
+    tol = float(np.float32(10.0 ** ((1.0 - t) * -5.0 + t * math.log10(float(np.float32(base_tol))))))
+    return max(tol, float(np.float32(5.0e-7))) if gpu else tol
+
+
+def parity_solve(g: Grid, nu, E, density, fixed, F, threshold, prune_voids=True, tol=1e-6, max_iter=40000,
+                 shift_factor=1e-6, u_prev=None, fp32=False):
+    """One attempt of Solver.solve_system (Solver.jl:38-164) on the structured
+    grid: active elements rho >= threshold with modulus E*rho
+    (GPUSolver.jl:128-147); with prune_voids and fewer than 99 % of the elements
+    above the FP32 product 1.01*threshold, only those stay and the nodes they
+    do not touch leave the system (MeshPruner.jl:18-138; here: those DOFs are
+    held at 0); shift = shift_factor * max free diagonal (GPUSolver.jl:315-316,
+    391); warm start from u_prev (Solver.jl:71-83).  Returns
+    (u, status, rel, iterations, info)."""
+    density = np.asarray(density, dtype=np.float64)
+    thr_keep = float(np.float32(threshold) * np.float32(1.01))
+    active = density >= threshold
+    keep = density > thr_keep
+    pruned = bool(prune_voids and keep.sum() < 0.99 * density.size)
+    sel = keep if pruned else active
+    if not sel.any():
+        raise RuntimeError("No active elements found")
+    Ee = np.where(sel, E * density, 0.0)
+    Ke = ke_hex(g.dx, g.dy, g.dz, nu)
+    fixed = np.asarray(fixed, dtype=np.uint8)
+    d = diag(g, Ke, Ee)
+    free = fixed == 0
+    shift = shift_factor * float(np.abs(d[free]).max()) if free.any() else 0.0
+    fixed_eff = (fixed.astype(bool) | ~used_dofs(g, Ee)).astype(np.uint8)
+    x0 = None if u_prev is None else np.where(fixed_eff == 0, u_prev, 0.0)
+    if fp32:  # the reference's own arithmetic (FP32 throughout, recursive residual)
+        u, st, rel, it = pcg_f32(g, Ke, Ee, fixed_eff, F, x0=x0, shift=shift, tol=tol, max_iter=max_iter)
+        u = u.astype(np.float64)
+    else:
+        u, st, rel, it = pcg(g, Ke, Ee, fixed_eff, F, x0=x0, shift=shift, tol=tol, max_iter=max_iter)
+    return u, st, rel, it, {"pruned": pruned, "n_active": int(active.sum()), "n_kept": int(sel.sum()),
+                            "shift": shift, "Ee": Ee}
+
+
+def parity_loop(g: Grid, nu, E, fixed, F, n_iter, opt, n_nominal, max_dim, l1_allowable=1.0, base_tol=1e-6,
+                max_iter=40000, shift_factor=1e-6, filter_tol=1e-5, fixed_tol=None, fp32=False):
+    """The stress-ratio TO loop of Main.jl:331-474 without refinement, export,
+    gravity or thermal loads, from rho = 1 (no protected shapes): solve
+    (parity_solve), compliance F'u, centroid stress (Stress.jl:57-100), update
+    (parity_update_density with n_nominal + 10, Main.jl:446).  fixed_tol
+    replaces the adaptive schedule.  Returns per-iteration records and the
+    density after each update."""
+    min_density = float(opt.get("min_density", 1e-3))
+    max_density = float(opt.get("density_clamp_max", 1.0))
+    density = np.ones(g.ne)
+    original = np.ones(g.ne)
+    protected = np.zeros(g.ne, bool)
+    u = np.zeros(g.ndof)
+    hist, dens = [], []
+    for it in range(1, n_iter + 1):
+        tol = fixed_tol if fixed_tol is not None else parity_tolerance(base_tol, it, n_nominal)
+        u, st, rel, its, info = parity_solve(g, nu, E, density, fixed, F, min_density, True, tol, max_iter,
+                                             shift_factor, u_prev=u, fp32=fp32)
+        comp = float(F @ u)
+        _, _, _, l1, _ = stress(g, nu, E, density, u)
+        act = density > 0.1
+        avg_l1 = float(l1[act].mean()) if act.any() else 0.0
+        vf = float(density.sum() / density.size)
+        n_active = int((density > min_density).sum())
+        mc, R, thr = parity_update_density(g, density, l1, protected, E, l1_allowable, it, n_nominal + 10, original,
+                                           min_density, max_density, opt, max_dim, filter_tol)
+        hist.append({"iter": it, "compliance": comp, "avg_l1": avg_l1, "vol_frac": vf, "active": n_active,
+                     "delta_rho": mc, "radius": R, "cutoff": thr, "pcg_status": st, "pcg_rel": rel,
+                     "pcg_iterations": its, "tol": tol, "pruned": info["pruned"]})
+        dens.append(density.copy())
+    return hist, dens
+
+
 # ---------------------------------------------------------------------------
 # Textbook SIMP pieces
 # ---------------------------------------------------------------------------

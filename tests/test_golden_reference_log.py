@@ -95,3 +95,18 @@ def test_iteration1_compliance_and_stress_vs_reference_log(hx, ox):
     true_rel = np.linalg.norm(r) / np.linalg.norm(F)
     assert true_rel > 100 * tol
     assert abs(F @ u32 - c) < 0.01 * c
+
+
+def test_log_line_format_reproduces_the_reference_lines(hx):
+    """Diagnostics.jl:52-83: the reference's own numbers through format_log_line
+    give its log lines character for character."""
+    from importlib import import_module
+    drv = import_module("2025_12_09_hexat_v069_amd.driver")
+    g1, g2 = GOLD["iteration1"], GOLD["iteration2"]
+    l1 = drv.format_log_line(1, "97x32x32", 99328, 99328, 1.2, 0.0245, 1.521e4, 7.604e3, 0.5314, 1.0, 0.998,
+                             "Nominal", 8.94, 8.9e-6, "Jacobi", wall="23:30:06", vram="1.2G")
+    l2 = drv.format_log_line(2, "97x32x32", 99328, 98568, 1.168, 0.049, 3.322e4, 1.661e4, 0.5558, 0.2282, 0.2666,
+                             "Nominal", 1.35, 8.2e-6, "Jacobi", wall="23:30:08", vram="1.2G")
+    assert l1 == GOLD["line1"] and l2 == GOLD["line2"]
+    assert drv.LOG_HEADER.splitlines() == GOLD["header"]
+    assert (g1["cutoff"], g2["cutoff"]) == (0.025, 0.049)

@@ -1,0 +1,76 @@
+"""The reference's own TO loop (Main.jl:331-474, stress-ratio update) run end
+to end through the Python mirrors of its entry points -- driver.run_stress_ratio
+-> solver.solve_system (hx_solve_system) -> stress.compute_stress_field ->
+topopt.update_density (two GPU filter solves) -- on the cantilever of the
+reference's published run (tests/golden/cantilever_4loads.yaml, 97x32x32),
+against the oracle's restatement of the same loop (oracle.parity_loop, FP64)
+and against the reference's log (R/simulation_log.txt lines 7-8).
+"""
+import json
+import os
+import re
+import sys
+
+import numpy as np
+import pytest
+import yaml
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "reference_log.json")))
+CFG = os.path.join(HERE, "golden", "cantilever_4loads.yaml")
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_stress_ratio_loop_matches_oracle_and_log(ox, hx, tmp_path):
+    from importlib import import_module
+    drv = import_module("2025_12_09_hexat_v069_amd.driver")
+    conf = import_module("2025_12_09_hexat_v069_amd.configuration")
+    bnd = import_module("2025_12_09_hexat_v069_amd.boundary")
+    cfg = conf.apply_hardware_profile(yaml.safe_load(open(CFG)))
+    log = tmp_path / "log.txt"
+    _say("run_stress_ratio 97x32x32, 2 iterations")
+    hist, density, U = drv.run_stress_ratio(cfg, n_iter=2, log_path=str(log), verbose=False, keep_density=True)
+    geo, nodes, elements, bc, F = drv.build_problem(yaml.safe_load(open(CFG)))
+    g = ox.Grid(geo["nElem_x"], geo["nElem_y"], geo["nElem_z"], float(geo["dx"]), float(geo["dy"]), float(geo["dz"]))
+    fixed = bnd.fixed_dof_mask(bc)
+    sp = cfg["solver_parameters"]
+    _say("oracle parity loop")
+    ohist, odens = ox.parity_loop(g, 0.3, 1.0, fixed, F, 2, cfg["optimization_parameters"], 30,
+                                  float(geo["max_domain_dim"]), base_tol=sp["tolerance"],
+                                  shift_factor=sp["diagonal_shift_factor"], filter_tol=sp["filter_tolerance"])
+    for h, o, od in zip(hist, ohist, odens):
+        _say("it", h["iter"], "c", h["compliance"], o["compliance"], "l1", h["avg_l1"], o["avg_l1"], "vf",
+             h["vol_frac"], o["vol_frac"], "active", h["active"], o["active"], "drho", h["delta_rho"], o["delta_rho"],
+             "res", h["lin_res"], o["pcg_rel"], "max|drho|", np.abs(h["density"] - od).max())
+        assert abs(h["compliance"] / o["compliance"] - 1) < 1e-6
+        assert abs(h["avg_l1"] / o["avg_l1"] - 1) < 1e-6
+        assert abs(h["vol_frac"] - o["vol_frac"]) < 1e-6
+        assert h["active"] == o["active"]
+        assert h["radius"] == pytest.approx(o["radius"], rel=1e-6) and h["cutoff"] == pytest.approx(o["cutoff"])
+        assert abs(h["delta_rho"] - o["delta_rho"]) < 1e-6
+        assert h["lin_res"] < o["tol"] and h["prec"] == "Jacobi"
+        assert np.abs(h["density"] - od).max() < 1e-6
+    # the reference's log, to the spread its FP32 solver leaves (0.2-1 %)
+    g1, g2 = GOLD["iteration1"], GOLD["iteration2"]
+    assert abs(hist[0]["compliance"] / g1["compliance"] - 1) < 1e-2
+    assert abs(hist[0]["avg_l1"] / g1["avg_l1_stress"] - 1) < 2e-2
+    assert abs(hist[1]["compliance"] / g2["compliance"] - 1) < 1e-2
+    assert abs(hist[1]["vol_frac"] / g2["vol_frac"] - 1) < 2e-2
+    assert abs(hist[1]["active"] - g2["active_el"]) <= 10
+    assert round(hist[0]["delta_rho"] * 100, 2) == round(g1["delta_rho"] * 100, 2)
+    # the log file: the reference's header and one line per iteration in its format
+    text = log.read_text().splitlines()
+    assert text[0] == "HEXA FEM TOPOLOGY OPTIMIZATION LOG" and text[3] == "=" * 200
+    assert text[4:6] == GOLD["header"]
+    pat = re.compile(r"^\| +(\d+) \|  97x32x32 \|    99328 \| +(\d+) \| +([0-9.]+) \| +([0-9.]+) \| .* \| Nominal \| "
+                     r".* \| Jacobi \| .* \|$")
+    for line, h in zip(text[6:8], hist):
+        m = pat.match(line)
+        assert m and int(m.group(1)) == h["iter"] and int(m.group(2)) == h["active"]
+        assert [len(c) for c in line.split("|")[:-2]] == [len(c) for c in GOLD["line1"].split("|")[:-2]]

@@ -1,0 +1,222 @@
+"""The solve policy on the device (hx_solve_system) and its Python mirrors
+(solver.solve_system, solver.solve_system_gpu, gpu_helmholtz.apply_gpu_filter,
+topopt.update_density), against the oracle and against the rules of
+Solver.jl:38-164, MeshPruner.jl:18-138 and GPUSolver.jl:274-606.
+
+Failures are forced from the host (an iteration cap of a few iterations, a
+NaN in the load, every density below the threshold): every kernel still
+exits normally.
+"""
+import sys
+import warnings
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def mods(hx):
+    from importlib import import_module
+    pkg = "2025_12_09_hexat_v069_amd"
+    return {k: import_module(f"{pkg}.{k}") for k in ("solver", "mesh", "boundary", "gpu_helmholtz", "topopt", "_lib")}
+
+
+def _problem(ox, mods, dims=(24, 8, 8), seed=0, void_frac=0.0, void=1e-4):
+    nx, ny, nz = dims
+    g = ox.Grid(nx, ny, nz)
+    nodes, elements, _ = mods["mesh"].generate_mesh(nx, ny, nz, np.float32(1), np.float32(1), np.float32(1))
+    bc = mods["boundary"].get_bc_indicator(nodes.shape[0], nodes, [{"location": [0, ":", ":"], "DoFs": [1, 2, 3]}])
+    F = np.zeros(3 * nodes.shape[0])
+    mods["boundary"].apply_external_forces(F, [{"location": ["100%", 0, ":"], "F": [0, -1, 0]}], nodes, elements)
+    rng = np.random.default_rng(seed)
+    rho = rng.uniform(0.05, 1.0, g.ne).astype(np.float32)
+    if void_frac:
+        # a void block inside the beam (its interior nodes are orphaned) and
+        # scattered voids away from the clamp and the load
+        e = np.arange(g.ne)
+        i, j, k = e % nx, (e // nx) % ny, e // (nx * ny)
+        block = (i >= 8) & (i < 13) & (j >= 2) & (j < 6) & (k >= 2) & (k < 6)
+        rho[block] = np.float32(void)
+        cand = np.nonzero((i > 2) & (i < nx - 3) & ~block)[0]
+        rho[rng.choice(cand, int(void_frac * g.ne), replace=False)] = np.float32(void)
+    fixed = mods["boundary"].fixed_dof_mask(bc)
+    return g, nodes, elements, bc, F, rho.astype(np.float64), fixed
+
+
+def test_solve_system_gpu_matches_oracle(ox, mods):
+    g, nodes, elements, bc, F, rho, fixed = _problem(ox, mods, seed=1)
+    rho[::17] = 5e-4  # below the threshold: out of the operator (GPUSolver.jl:128-129)
+    u, res, prec = mods["solver"].solve_system_gpu(nodes, elements, np.float32(1), np.float32(0.3), bc, F, rho,
+                                                   tol=1e-10, shift_factor=1e-6, min_stiffness_threshold=1e-3)
+    uo, st, rel, it, info = ox.parity_solve(g, 0.3, 1.0, rho, fixed, F, 1e-3, prune_voids=False, tol=1e-10,
+                                            shift_factor=1e-6)
+    _say("solve_system_gpu", res, prec, "oracle", rel, it)
+    assert st == 0 and prec == "Jacobi" and res < 1e-10
+    assert np.abs(u - uo).max() <= 1e-7 * np.abs(uo).max()
+    assert abs(F @ u - F @ uo) <= 1e-9 * abs(F @ uo)
+
+
+def test_solve_system_prunes_voids_and_remaps_warm_start(ox, mods, hx):
+    g, nodes, elements, bc, F, rho, fixed = _problem(ox, mods, seed=2, void_frac=0.3)
+    ctx = mods["solver"].get_context()
+    mods["solver"]._bind_grid(ctx, nodes, elements, 0.3)
+    ctx.set_fixed(fixed)
+    thr = float(np.float32(1e-4))  # the reference's Float32 threshold against Float32 densities
+    u0, st0 = ctx.solve_system(rho, F, None, E=1.0, min_stiffness_threshold=thr, prune_voids=True, tol=1e-10,
+                               shift_factor=1e-9)
+    uo, sto, rel, it, info = ox.parity_solve(g, 0.3, 1.0, rho, fixed, F, thr, prune_voids=True, tol=1e-10,
+                                             shift_factor=1e-9)
+    _say("pruned", st0.pruned, st0.n_kept, info["n_kept"], st0.iterations, it)
+    assert st0.pruned == 1 and info["pruned"] and st0.n_kept == info["n_kept"] == int((rho > 2 * thr).sum())
+    assert st0.n_active == g.ne  # the voids sit exactly at the threshold: active, but pruned
+    assert st0.outcome == hx._lib.HX_SOLVE_CONVERGED and st0.attempts == 1
+    assert abs(st0.shift - info["shift"]) <= 1e-12 * info["shift"]
+    orphan = ~ox.used_dofs(g, info["Ee"])
+    assert orphan.any() and np.all(u0[orphan] == 0.0)
+    assert np.abs(u0 - uo).max() <= 1e-7 * np.abs(uo).max()
+    # warm start from a perturbed previous solution, with non-zero values on the orphan nodes
+    rng = np.random.default_rng(0)
+    prev = uo * (1 + 1e-3 * rng.standard_normal(uo.size)) + np.where(orphan, 1.0, 0.0)
+    u1, st1 = ctx.solve_system(rho, F, prev, E=1.0, min_stiffness_threshold=thr, prune_voids=True, tol=1e-10,
+                               shift_factor=1e-9)
+    uw, _, _, itw, _ = ox.parity_solve(g, 0.3, 1.0, rho, fixed, F, thr, prune_voids=True, tol=1e-10,
+                                       shift_factor=1e-9, u_prev=prev)
+    _say("warm", st1.iterations, itw)
+    assert st1.iterations < st0.iterations and abs(st1.iterations - itw) <= max(2, itw // 50)
+    assert np.all(u1[orphan] == 0.0) and np.abs(u1 - uw).max() <= 1e-7 * np.abs(uw).max()
+    # the mirror: same answer through solve_system (pruning on, northstar tolerance)
+    cfg = {"solver_parameters": {"solver_type": "gpu", "mode": "northstar", "tolerance": 1e-10,
+                                 "diagonal_shift_factor": 1e-9, "max_iterations": 40000}}
+    U, res, prec = mods["solver"].solve_system(nodes, elements, 1.0, 0.3, bc, F, density=rho.astype(np.float32),
+                                               config=cfg, min_stiffness_threshold=np.float32(1e-4))
+    assert prec == "Jacobi" and res < 1e-10 and np.abs(U - uo).max() <= 1e-7 * np.abs(uo).max()
+
+
+def test_failed_attempts_escalate_shift_and_return_zeros(ox, mods, hx):
+    """An iteration cap of 3: every attempt ends with its best residual above
+    0.1 (failed badly), the shift factor grows x10 per attempt, and with no
+    stagnated attempt to keep the result is zeros (GPUSolver.jl:583-606)."""
+    g, nodes, elements, bc, F, rho, fixed = _problem(ox, mods, seed=3)
+    ctx = mods["solver"].get_context()
+    mods["solver"]._bind_grid(ctx, nodes, elements, 0.3)
+    ctx.set_fixed(fixed)
+    u, st = ctx.solve_system(rho, F, None, E=1.0, min_stiffness_threshold=1e-3, prune_voids=False, tol=1e-10,
+                             max_iter=3, shift_factor=1e-6)
+    d = ox.diag(g, ox.ke_hex(1, 1, 1, 0.3), rho)
+    maxd = np.abs(d[fixed == 0]).max()
+    _say("failed", st.outcome, st.attempts, st.best_rel, st.shift)
+    assert st.status == hx._lib.HX_ERR_NOT_CONVERGED and st.outcome == hx._lib.HX_SOLVE_FAILED
+    assert st.attempts == 3 and st.iterations == 9 and st.best_rel > 0.1
+    assert abs(st.shift - 1e-6 * 100 * maxd) <= 1e-12 * maxd
+    assert np.all(u == 0.0)
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        U, res, prec = mods["solver"].solve_system_gpu(nodes, elements, 1.0, 0.3, bc, F, rho, tol=1e-10,
+                                                       max_iter=3, shift_factor=1e-6)
+    assert any("All solver attempts failed" in str(x.message) for x in w)
+    assert np.all(U == 0.0) and res == pytest.approx(st.rel_res)
+
+
+def test_stagnation_below_5e_2_is_accepted_with_the_best_iterate(ox, mods, hx):
+    g, nodes, elements, bc, F, rho, fixed = _problem(ox, mods, seed=4)
+    ctx = mods["solver"].get_context()
+    mods["solver"]._bind_grid(ctx, nodes, elements, 0.3)
+    ctx.set_fixed(fixed)
+    u, st = ctx.solve_system(rho, F, None, E=1.0, min_stiffness_threshold=1e-3, prune_voids=False, tol=1e-15,
+                             max_iter=250, shift_factor=1e-6)
+    Ke = ox.ke_hex(1, 1, 1, 0.3)
+    r = np.where(fixed == 0, F - ox.matvec(g, Ke, rho, u) - st.shift * u, 0.0)
+    true_rel = np.linalg.norm(r) / np.linalg.norm(F)
+    _say("accepted", st.outcome, st.best_rel, st.true_rel, true_rel)
+    assert st.status == hx._lib.HX_OK and st.outcome == hx._lib.HX_SOLVE_ACCEPTED and st.attempts == 1
+    assert st.best_rel < 5e-2 and abs(true_rel - st.true_rel) <= 1e-6 * true_rel
+    assert true_rel < 1e-2 and np.any(u != 0)  # a snapshot (residual < 1e-2), not the zero start
+
+
+def test_nan_load_breaks_down_every_attempt(ox, mods, hx):
+    g, nodes, elements, bc, F, rho, fixed = _problem(ox, mods, seed=5)
+    F = F.copy()
+    F[np.nonzero(fixed == 0)[0][7]] = np.nan
+    ctx = mods["solver"].get_context()
+    mods["solver"]._bind_grid(ctx, nodes, elements, 0.3)
+    ctx.set_fixed(fixed)
+    u, st = ctx.solve_system(rho, F, None, E=1.0, min_stiffness_threshold=1e-3, prune_voids=False, tol=1e-8)
+    _say("nan", st.outcome, st.attempts, st.iterations)
+    assert st.outcome == hx._lib.HX_SOLVE_FAILED and st.attempts == 3 and np.all(u == 0.0)
+    # the plain entry point reports the breakdown and returns its start vector
+    Ee = rho.copy()
+    u2, s2 = ctx.pcg(Ee, F, tol=1e-8)
+    assert s2.status in (hx._lib.HX_ERR_BREAKDOWN, hx._lib.HX_ERR_DIVERGED) and np.all(u2 == 0.0)
+
+
+def test_no_active_element_and_zero_load(ox, mods, hx):
+    g, nodes, elements, bc, F, rho, fixed = _problem(ox, mods, seed=6)
+    low = np.full(g.ne, 1e-5)
+    with pytest.raises(RuntimeError) as ei:
+        mods["solver"].solve_system_gpu(nodes, elements, 1.0, 0.3, bc, F, low, min_stiffness_threshold=1e-3)
+    assert getattr(ei.value, "code", None) == hx._lib.HX_ERR_NO_ACTIVE
+    ctx = mods["solver"].get_context()
+    with pytest.raises(hx._lib.HexatError) as e2:
+        ctx.pcg(np.zeros(g.ne), F)
+    assert e2.value.code == hx._lib.HX_ERR_NO_ACTIVE
+    U, res, prec = mods["solver"].solve_system_gpu(nodes, elements, 1.0, 0.3, bc, np.zeros_like(F), rho)
+    assert np.all(U == 0.0) and res == 0.0 and prec == "None"
+
+
+def test_pcg_not_converged_returns_best_iterate(ox, mods, hx):
+    g, nodes, elements, bc, F, rho, fixed = _problem(ox, mods, seed=7)
+    ctx = mods["solver"].get_context()
+    mods["solver"]._bind_grid(ctx, nodes, elements, 0.3)
+    ctx.set_fixed(fixed)
+    u, s = ctx.pcg(rho, F, tol=1e-15, max_iter=300, batch=16)
+    Ke = ox.ke_hex(1, 1, 1, 0.3)
+    r = np.where(fixed == 0, F - ox.matvec(g, Ke, rho, u), 0.0)
+    true_rel = np.linalg.norm(r) / np.linalg.norm(F)
+    assert s.status == hx._lib.HX_ERR_NOT_CONVERGED and s.iterations == 300
+    assert abs(true_rel - s.rel_res) <= 1e-6 * true_rel and true_rel < 1e-2
+
+
+def test_to_step_keeps_densities_when_the_solve_fails(ox, mods, hx):
+    L = hx._lib
+    ctx = hx.Context(0)
+    try:
+        g = ox.Grid(24, 8, 8)
+        ctx.set_grid(24, 8, 8, 1, 1, 1, 0.3)
+        X = g.node_coords()
+        fixed = np.repeat((X[:, 0] == 0.0)[:, None], 3, axis=1).ravel().astype(np.uint8)
+        ctx.set_fixed(fixed)
+        f = ox.external_forces(X, [{"location": ["100%", 0, ":"], "F": [0, -1, 0]}])
+        f[np.nonzero(fixed == 0)[0][3]] = np.nan
+        p = L.TOParams(E0=1.0, Emin=1e-9, nu=0.3, penal=3.0, volfrac=0.3, move=0.2, xmin=0.0, rmin=1.5,
+                       pcg_tol=1e-8, pcg_max_iter=1000, precision=0, precond=0, filter_tol=1e-10,
+                       filter_max_iter=1000, oc_rel_tol=1e-9, with_stress=1)
+        ctx.to_init(p, f, np.full(g.ne, 0.3))
+        x0, xp0 = ctx.to_read(L.HX_FIELD_X), ctx.to_read(L.HX_FIELD_XPHYS)
+        with pytest.raises(L.HexatError) as ei:
+            ctx.to_step()
+        assert ei.value.code in (L.HX_ERR_BREAKDOWN, L.HX_ERR_DIVERGED)
+        assert np.array_equal(ctx.to_read(L.HX_FIELD_X), x0)
+        assert np.array_equal(ctx.to_read(L.HX_FIELD_XPHYS), xp0)
+    finally:
+        ctx.close()
+
+
+def test_filter_failure_gives_the_empty_sentinel(ox, mods):
+    """GPUHelmholtz.jl:365-375: failure is an empty vector, never an exception."""
+    gh = mods["gpu_helmholtz"]
+    rho = np.full(24 * 8 * 8, 0.5, np.float32)
+    rho[5] = np.nan
+    out, t, it, res = gh.apply_gpu_filter(rho, 24, 8, 8, 1.0, 1.0, 1.0, 0.8)
+    assert out.size == 0
+    out2, *_ = gh.apply_gpu_filter(np.full(24 * 8 * 8, 0.5, np.float32), 24, 8, 8, 1.0, 1.0, 1.0, -1.0)
+    assert out2.size == 0
+    good, t, it, res = gh.apply_gpu_filter(np.full(24 * 8 * 8, 0.5, np.float32), 24, 8, 8, 1.0, 1.0, 1.0, 0.8,
+                                           tol=1e-12, max_iter=2000, check_every=1)
+    assert good.size == 24 * 8 * 8 and np.allclose(good, 1.0 / 3.0, atol=1e-6)  # constant -> x2/3

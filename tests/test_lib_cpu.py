@@ -61,7 +61,8 @@ def test_context_without_gpu_fails_loudly(hx):
 
 def test_solver_config_preconditioner_mapping(hx):
     """solver_parameters.preconditioner selects the multilevel or V-cycle
-    form; anything else but jacobi is rejected before any device work."""
+    form; "amg" and "jacobi_amg" (the reference's CPU AMG, not built) run
+    Jacobi; anything else is rejected before any device work."""
     import importlib
     solver = importlib.import_module(hx.__name__ + ".solver")
     L = hx._lib
@@ -69,5 +70,7 @@ def test_solver_config_preconditioner_mapping(hx):
     assert solver._precond_of({"solver_parameters": {"preconditioner": "Multilevel"}}) == (L.HX_PC_MULTILEVEL,
                                                                                            "Multilevel")
     assert solver._precond_of({"solver_parameters": {"preconditioner": "vcycle"}}) == (L.HX_PC_VCYCLE, "V-cycle")
+    assert solver._precond_of({"solver_parameters": {"preconditioner": "jacobi_amg"}}) == (L.HX_PC_JACOBI, "Jacobi")
+    assert solver._precond_of({"solver_parameters": {"preconditioner": "amg"}}) == (L.HX_PC_JACOBI, "Jacobi")
     with pytest.raises(ValueError):
-        solver._precond_of({"solver_parameters": {"preconditioner": "amg"}})
+        solver._precond_of({"solver_parameters": {"preconditioner": "ilu"}})
