@@ -581,7 +581,7 @@ int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& 
     int rc = ml ? ml_setup(c, false, vc) : HX_OK;
     if (rc) return rc;
     const bool free_upd = vc_free_update(c, vc, false);
-    launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream, nr);
+    launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream, nr, v.xbest ? 1 : 0);
     if (o.warm_start) launch_mask<double>(n, c->minv.p, v.x, c->stream, nr);
     else launch_fill(n * nr, 0.0, v.x, c->stream);
     if (v.xbest) HXC(hipMemcpyAsync(v.xbest, v.x, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
@@ -608,10 +608,10 @@ int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& 
                     TimeScope ts(c, HX_K_UPDATE);
                     if (free_upd)
                         launch_pcg_update_free<double>(n, v.x, pout, v.r, v.q, c->st, c->partials.p + kOpPartials,
-                                                       npq, c->stream);
+                                                       npq, c->stream, v.xbest);
                     else
                         launch_pcg_update<double>(n, v.x, pout, v.r, v.q, c->minv.p, c->st, c->partials.p,
-                                                  c->stream, nr, !ml, c->partials.p + kOpPartials, npq);
+                                                  c->stream, nr, !ml, c->partials.p + kOpPartials, npq, v.xbest);
                 }
                 if (ml) {
                     TimeScope ts(c, HX_K_PRECOND);
@@ -620,8 +620,11 @@ int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& 
             }
         });
         if (rc) return rc;
-        if (v.xbest) launch_snapshot(n, v.x, v.xbest, c->st, c->stream);  // a check point per batch
         if ((rc = sync_state(c, nr))) return rc;
+    }
+    if (v.xbest) {  // a best iterate the last finaliser flagged
+        launch_snapshot_pending(n, v.x, v.xbest, c->st, c->stream);
+        HXK();
     }
     std::vector<PcgState> fin(c->st_host, c->st_host + nr);
     c->last_best_rel = fin[0].best_rel;
@@ -677,7 +680,7 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
     const bool ml = o.precond == HX_PC_MULTILEVEL || vc;  // explicit z = M^-1 r (see pcg_fp64)
     if (ml && (rc = ml_setup(c, true, vc))) return rc;
     const bool free_upd = vc_free_update(c, vc, true);
-    launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream, nr);
+    launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream, nr, v.xbest ? 2 : 0);
     if (o.warm_start) launch_mask<double>(n, c->minv.p, v.x, c->stream, nr);
     else launch_fill(n * nr, 0.0, v.x, c->stream);
     if (v.xbest) HXC(hipMemcpyAsync(v.xbest, v.x, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
@@ -1704,9 +1707,11 @@ int hx_to_step(hx_ctx* c, hx_to_stats* s) {
     hx_pcg_stats ps{};
     if (p.precision == HX_FP32_IR)
         if ((rc = ensure_fp32(c))) return rc;
-    const int prc = run_pcg(c, po, &ps, single_vecs(c));
+    SolveVecs tv = single_vecs(c);
+    tv.xbest = nullptr;  // no best-iterate copies: a failed solve ends the step anyway
+    const int prc = run_pcg(c, po, &ps, tv);
     if (prc != HX_OK) {
-        // the densities stay as they are: a failed solve (its best iterate is
+        // the densities stay as they are: a failed solve (its last iterate is
         // left in the displacement) gives no sensitivities to update them with
         (void)hipStreamSynchronize(c->stream);
         if (s) {

@@ -144,6 +144,12 @@ struct PcgState {
     int done;        // 0 running, 1 finished
     int status;      // hx_status
     int check_every;
+    // best-iterate snapshot (GPUSolver.jl:492-499): mode 1 = the finaliser of
+    // each iteration flags a new best iterate with a residual below 1e-2
+    // (snap_pending), which the next update kernel copies as it reads x
+    // anyway; mode 2 = copies at the FP32 form's check points only; 0 = none
+    int snap_pending;
+    int snap_mode;
     int pad;
     unsigned ticket[4];
 };
@@ -159,6 +165,7 @@ __device__ inline void pcg_record(PcgState* st, double rr) {
 __device__ inline void pcg_decide(PcgState* st) {
     const int it = st->iter;
     const double rel = st->rel;
+    st->snap_pending = 0;
     if (it > 50 && rel > st->best_rel * 100.0) {
         st->done = 1;
         st->status = 5;
@@ -167,6 +174,10 @@ __device__ inline void pcg_decide(PcgState* st) {
         st->status = 5;
     } else {
         if (rel < st->best_rel) st->best_rel = rel;
+        if (st->snap_mode == 1 && rel < 1e-2 && rel < st->snap_rel) {
+            st->snap_rel = rel;
+            st->snap_pending = 1;
+        }
         if (rel < st->tol && (it % st->check_every) == 0) {
             st->done = 1;
             st->status = 0;

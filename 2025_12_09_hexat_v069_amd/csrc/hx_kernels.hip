@@ -25,7 +25,8 @@ struct Vec16 {
 };
 __device__ __forceinline__ bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
-__global__ void k_pcg_reset(PcgState* st0, double tol, double shift, int max_iter, int check_every, int nrhs) {
+__global__ void k_pcg_reset(PcgState* st0, double tol, double shift, int max_iter, int check_every, int nrhs,
+                            int snap_mode) {
     if (blockIdx.x == 0 && (int)threadIdx.x < nrhs) {
         PcgState* st = st0 + threadIdx.x;
         st->rz = st->pq = st->alpha = st->beta = st->rr = st->normb = st->rel = 0.0;
@@ -39,6 +40,8 @@ __global__ void k_pcg_reset(PcgState* st0, double tol, double shift, int max_ite
         st->done = 0;
         st->status = 0;
         st->check_every = check_every > 0 ? check_every : 1;
+        st->snap_pending = 0;
+        st->snap_mode = snap_mode;
         for (int i = 0; i < 4; ++i) st->ticket[i] = 0;
     }
 }
@@ -140,16 +143,20 @@ template <typename T>
 __global__ __launch_bounds__(NT) void k_pcg_update(long n, T* __restrict__ x, const T* __restrict__ p,
                                                    T* __restrict__ r, const T* __restrict__ q,
                                                    const T* __restrict__ minv, PcgState* st, double* partials,
-                                                   const double* __restrict__ pq_partials, unsigned npq) {
+                                                   const double* __restrict__ pq_partials, unsigned npq,
+                                                   T* __restrict__ xbest) {
     __shared__ double red[NT / 64];
     __shared__ double s_alpha;
     {
         const long off = (long)blockIdx.y * n;
         x += off; p += off; r += off; q += off;
+        if (xbest) xbest += off;
         st += blockIdx.y;
         partials += (long)blockIdx.y * gridDim.x * 2;
     }
     if (st->done) return;
+    // the last finaliser flagged x as the best iterate so far: keep it as it is read
+    const bool snap = xbest != nullptr && st->snap_pending;
     const double alpha = update_alpha(st, pq_partials + (size_t)blockIdx.y * npq, npq, red, s_alpha);
     if (alpha != alpha) return;  // breakdown: block 0 has recorded it
     double rz = 0.0, rr = 0.0;
@@ -169,10 +176,12 @@ __global__ __launch_bounds__(NT) void k_pcg_update(long n, T* __restrict__ x, co
     typedef typename Vec16<T>::type VT;
     constexpr int V = Vec16<T>::n;
     const long nv = n / V;
-    if (aligned16(x) && aligned16(p) && aligned16(r) && aligned16(q) && aligned16(minv)) {
+    if (aligned16(x) && aligned16(p) && aligned16(r) && aligned16(q) && aligned16(minv) &&
+        (!snap || aligned16(xbest))) {
         for (long v = gtid(); v < nv; v += gstride()) {
             VT xv = ((const VT*)x)[v], rv = ((const VT*)r)[v];
             const VT pv = ((const VT*)p)[v], qv = ((const VT*)q)[v], mv = ((const VT*)minv)[v];
+            if (snap) ((VT*)xbest)[v] = xv;
 #pragma unroll
             for (int l = 0; l < V; ++l) {
                 T xi = xv[l], ri = rv[l];
@@ -189,6 +198,7 @@ __global__ __launch_bounds__(NT) void k_pcg_update(long n, T* __restrict__ x, co
             for (int l = 0; l < V; ++l) {
                 const long i = v * V + l;
                 T xi = x[i], ri = r[i];
+                if (snap) xbest[i] = xi;
                 one(xi, p[i], ri, q[i], minv[i]);
                 x[i] = xi;
                 r[i] = ri;
@@ -196,6 +206,7 @@ __global__ __launch_bounds__(NT) void k_pcg_update(long n, T* __restrict__ x, co
     }
     for (long i = nv * V + gtid(); i < n; i += gstride()) {
         T xi = x[i], ri = r[i];
+        if (snap) xbest[i] = xi;
         one(xi, p[i], ri, q[i], minv[i]);
         x[i] = xi;
         r[i] = ri;
@@ -229,10 +240,12 @@ __global__ __launch_bounds__(NT) void k_fin_update(PcgState* st0, const double* 
 template <typename T>
 __global__ __launch_bounds__(NT) void k_pcg_update_free(long n, T* __restrict__ x, const T* __restrict__ p,
                                                         T* __restrict__ r, const T* __restrict__ q, PcgState* st,
-                                                        const double* __restrict__ pq_partials, unsigned npq) {
+                                                        const double* __restrict__ pq_partials, unsigned npq,
+                                                        T* __restrict__ xbest) {
     __shared__ double red[NT / 64];
     __shared__ double s_alpha;
     if (st->done) return;
+    const bool snap = xbest != nullptr && st->snap_pending;  // as in k_pcg_update
     const double alpha = update_alpha(st, pq_partials, npq, red, s_alpha);
     if (alpha != alpha) return;
     auto one = [&](T& xi, T pi, T& ri, T qi) {
@@ -242,10 +255,11 @@ __global__ __launch_bounds__(NT) void k_pcg_update_free(long n, T* __restrict__ 
     typedef typename Vec16<T>::type VT;
     constexpr int V = Vec16<T>::n;
     const long nv = n / V;
-    if (aligned16(x) && aligned16(p) && aligned16(r) && aligned16(q)) {
+    if (aligned16(x) && aligned16(p) && aligned16(r) && aligned16(q) && (!snap || aligned16(xbest))) {
         for (long v = gtid(); v < nv; v += gstride()) {
             VT xv = ((const VT*)x)[v], rv = ((const VT*)r)[v];
             const VT pv = ((const VT*)p)[v], qv = ((const VT*)q)[v];
+            if (snap) ((VT*)xbest)[v] = xv;
 #pragma unroll
             for (int l = 0; l < V; ++l) {
                 T xi = xv[l], ri = rv[l];
@@ -257,9 +271,15 @@ __global__ __launch_bounds__(NT) void k_pcg_update_free(long n, T* __restrict__ 
             ((VT*)r)[v] = rv;
         }
     } else {
-        for (long i = gtid(); i < nv * V; i += gstride()) one(x[i], p[i], r[i], q[i]);
+        for (long i = gtid(); i < nv * V; i += gstride()) {
+            if (snap) xbest[i] = x[i];
+            one(x[i], p[i], r[i], q[i]);
+        }
     }
-    for (long i = nv * V + gtid(); i < n; i += gstride()) one(x[i], p[i], r[i], q[i]);
+    for (long i = nv * V + gtid(); i < n; i += gstride()) {
+        if (snap) xbest[i] = x[i];
+        one(x[i], p[i], r[i], q[i]);
+    }
 }
 
 // p = minv r + beta p
@@ -931,6 +951,16 @@ __global__ void k_snapshot_fin(PcgState* st) {
     const double rel = st->rel;
     if (rel < 1e-2 && rel < st->snap_rel) st->snap_rel = rel;
 }
+// end of a per-iteration-snapshot solve: a best iterate flagged by the last
+// finaliser had no further update kernel to copy it
+__global__ __launch_bounds__(NT) void k_snapshot_pending(long n, const double* __restrict__ x,
+                                                         double* __restrict__ xbest, const PcgState* st) {
+    if (!st->snap_pending) return;
+    for (long i = gtid(); i < n; i += gstride()) xbest[i] = x[i];
+}
+__global__ void k_snapshot_clear(PcgState* st) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->snap_pending = 0;
+}
 
 // Element selection of a solve (GPUSolver.jl:128-129, Solver.jl:60-67):
 // per block, the count of rho >= thr (active) and of rho > thr_keep (kept
@@ -988,8 +1018,8 @@ inline unsigned blocks_for(long n) { return (unsigned)((n + NT - 1) / NT); }
 
 // ------------------------------------------------------------- launchers
 void launch_pcg_reset(PcgState* st, double tol, double shift, int max_iter, int check_every, hipStream_t s,
-                      int nrhs) {
-    hipLaunchKernelGGL(k_pcg_reset, dim3(1), dim3(64), 0, s, st, tol, shift, max_iter, check_every, nrhs);
+                      int nrhs, int snap_mode) {
+    hipLaunchKernelGGL(k_pcg_reset, dim3(1), dim3(64), 0, s, st, tol, shift, max_iter, check_every, nrhs, snap_mode);
 }
 template <typename T>
 void launch_pcg_init(long n, const T* b, T* x, const T* q, const T* minv, T* r, T* p, double shift, PcgState* st,
@@ -1000,9 +1030,9 @@ void launch_pcg_init(long n, const T* b, T* x, const T* q, const T* minv, T* r, 
 }
 template <typename T>
 void launch_pcg_update(long n, T* x, const T* p, T* r, const T* q, const T* minv, PcgState* st, double* partials,
-                       hipStream_t s, int nrhs, bool set_beta, const double* pq_partials, int npq) {
+                       hipStream_t s, int nrhs, bool set_beta, const double* pq_partials, int npq, T* xbest) {
     hipLaunchKernelGGL((k_pcg_update<T>), dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, x, p, r, q, minv, st, partials,
-                       pq_partials, (unsigned)(pq_partials ? npq : 0));
+                       pq_partials, (unsigned)(pq_partials ? npq : 0), xbest);
     // without set_beta the explicit preconditioner's finaliser reads these
     // partials (r'r at index 2b+1) after its own r'z and finishes the iteration
     if (set_beta)
@@ -1010,9 +1040,9 @@ void launch_pcg_update(long n, T* x, const T* p, T* r, const T* q, const T* minv
 }
 template <typename T>
 void launch_pcg_update_free(long n, T* x, const T* p, T* r, const T* q, PcgState* st, const double* pq_partials,
-                            int npq, hipStream_t s) {
+                            int npq, hipStream_t s, T* xbest) {
     hipLaunchKernelGGL((k_pcg_update_free<T>), dim3(kVecBlocks), dim3(NT), 0, s, n, x, p, r, q, st, pq_partials,
-                       (unsigned)(pq_partials ? npq : 0));
+                       (unsigned)(pq_partials ? npq : 0), xbest);
 }
 template <typename T>
 void launch_pcg_pupdate(long n, const T* r, const T* minv, T* p, const PcgState* st, hipStream_t s, int nrhs) {
@@ -1063,6 +1093,10 @@ void launch_accumulate(long n, double* x64, float* x32, hipStream_t s, int nrhs)
     hipLaunchKernelGGL(k_accumulate, dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, x64, x32);
 }
 
+void launch_snapshot_pending(long n, const double* x, double* xbest, PcgState* st, hipStream_t s) {
+    hipLaunchKernelGGL(k_snapshot_pending, dim3(kVecBlocks), dim3(NT), 0, s, n, x, xbest, (const PcgState*)st);
+    hipLaunchKernelGGL(k_snapshot_clear, dim3(1), dim3(64), 0, s, st);
+}
 void launch_snapshot(long n, const double* x, double* xbest, PcgState* st, hipStream_t s) {
     hipLaunchKernelGGL(k_snapshot, dim3(kVecBlocks), dim3(NT), 0, s, n, x, xbest, (const PcgState*)st);
     hipLaunchKernelGGL(k_snapshot_fin, dim3(1), dim3(64), 0, s, st);
@@ -1174,13 +1208,13 @@ template void launch_pcg_init<double>(long, const double*, double*, const double
 template void launch_pcg_init<float>(long, const float*, float*, const float*, const float*, float*, float*, double,
                                      PcgState*, double*, hipStream_t, int);
 template void launch_pcg_update<double>(long, double*, const double*, double*, const double*, const double*,
-                                        PcgState*, double*, hipStream_t, int, bool, const double*, int);
+                                        PcgState*, double*, hipStream_t, int, bool, const double*, int, double*);
 template void launch_pcg_update<float>(long, float*, const float*, float*, const float*, const float*, PcgState*,
-                                       double*, hipStream_t, int, bool, const double*, int);
+                                       double*, hipStream_t, int, bool, const double*, int, float*);
 template void launch_pcg_update_free<double>(long, double*, const double*, double*, const double*, PcgState*,
-                                             const double*, int, hipStream_t);
+                                             const double*, int, hipStream_t, double*);
 template void launch_pcg_update_free<float>(long, float*, const float*, float*, const float*, PcgState*,
-                                            const double*, int, hipStream_t);
+                                            const double*, int, hipStream_t, float*);
 template void launch_pcg_pupdate<double>(long, const double*, const double*, double*, const PcgState*, hipStream_t,
                                          int);
 template void launch_pcg_pupdate<float>(long, const float*, const float*, float*, const PcgState*, hipStream_t, int);

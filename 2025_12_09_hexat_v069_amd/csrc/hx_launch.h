@@ -102,19 +102,22 @@ void launch_pcg_init(long n, const T* b, T* x, const T* q, const T* minv, T* r, 
 // pq_partials (npq per column): the operator's p'q partials of an MV_PCG
 // launch with defer_alpha; every block reduces them to alpha itself (block 0
 // records it and the breakdown), so no alpha finaliser runs in between.
+// xbest (non-null, one column): the state's per-iteration snapshot (mode 1),
+// x copied there before the update when the last finaliser flagged it
 template <typename T>
 void launch_pcg_update(long n, T* x, const T* p, T* r, const T* q, const T* minv, PcgState* st,
                        double* partials, hipStream_t s, int nrhs = 1, bool set_beta = true,
-                       const double* pq_partials = nullptr, int npq = 0);
+                       const double* pq_partials = nullptr, int npq = 0, T* xbest = nullptr);
 // x += alpha p ; r -= alpha q without the mask or the r'z, r'r partials (the
 // V-cycle's post-smoothing operator forms r'r); one right-hand side
 template <typename T>
 void launch_pcg_update_free(long n, T* x, const T* p, T* r, const T* q, PcgState* st, const double* pq_partials,
-                            int npq, hipStream_t s);
+                            int npq, hipStream_t s, T* xbest = nullptr);
 template <typename T>
 void launch_pcg_pupdate(long n, const T* r, const T* minv, T* p, const PcgState* st, hipStream_t s, int nrhs = 1);
+// snap_mode: PcgState::snap_mode of every column
 void launch_pcg_reset(PcgState* st, double tol, double shift, int max_iter, int check_every, hipStream_t s,
-                      int nrhs = 1);
+                      int nrhs = 1, int snap_mode = 0);
 
 // generic reductions / helpers
 void launch_dot(long n, const double* a, const double* b, double* out, unsigned* ticket, double* partials,
@@ -138,8 +141,10 @@ void launch_mask(long n, const T* minv, T* x, hipStream_t s, int nrhs = 1);
 // x64 += x32; x32 = 0
 void launch_accumulate(long n, double* x64, float* x32, hipStream_t s, int nrhs = 1);
 // xbest = x when st->rel < min(1e-2, st->snap_rel); then snap_rel = rel
-// (best-iterate snapshot of one column)
+// (best-iterate snapshot of one column at a check point, mode 2)
 void launch_snapshot(long n, const double* x, double* xbest, PcgState* st, hipStream_t s);
+// xbest = x if the last finaliser flagged x (mode 1, end of the solve)
+void launch_snapshot_pending(long n, const double* x, double* xbest, PcgState* st, hipStream_t s);
 // out[0] = #(rho >= thr), out[1] = #(rho > thr_keep) (kVecBlocks x 2 partials)
 void launch_count_active(long ne, const double* rho, double thr, double thr_keep, double* partials, double* out,
                          hipStream_t s);

