@@ -37,50 +37,16 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+_wl = importlib.import_module("2025_12_09_hexat_v069_amd.workloads")
+cantilever, mbb_half, block_loads, smoothed_density = (_wl.cantilever, _wl.mbb_half, _wl.block_loads,
+                                                       _wl.smoothed_density)
+
 METRIC = "s/TO-iteration at 12.6M DOFs: plain FP64 PCG vs FP32-MFMA+learned-PC"
 HBM_PEAK_GBS = 8000.0
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def cantilever(nx, ny, nz):
-    """Clamp x=0 (DoFs 1-3); total load (0,-1,0) split over the nodes of the
-    edge x=nx, y=0 (location [100%, 0, ':'] in the reference grammar)."""
-    NX, NY, NZ = nx + 1, ny + 1, nz + 1
-    nn = NX * NY * NZ
-    k, j, i = np.meshgrid(np.arange(NZ), np.arange(NY), np.arange(NX), indexing="ij")
-    i, j, k = i.ravel(), j.ravel(), k.ravel()
-    fixed = np.repeat((i == 0)[:, None], 3, axis=1).ravel().astype(np.uint8)
-    f = np.zeros(3 * nn)
-    edge = np.nonzero((i == nx) & (j == 0))[0]
-    f[3 * edge + 1] = -1.0 / edge.size
-    return fixed, f
-
-
-def mbb_half(nx, ny, nz):
-    """Half MBB beam in the reference's location grammar: symmetry rollers
-    [0, ':', ':'] DoF 1, support line [100%, 0, ':'] DoF 2, z-symmetry
-    [':', ':', 0] DoF 3; unit load [0, 100%, 50%] F (0, -1, 0)."""
-    NX, NY, NZ = nx + 1, ny + 1, nz + 1
-    k, j, i = np.meshgrid(np.arange(NZ), np.arange(NY), np.arange(NX), indexing="ij")
-    i, j, k = i.ravel(), j.ravel(), k.ravel()
-    fixed = np.zeros((NX * NY * NZ, 3), dtype=np.uint8)
-    fixed[i == 0, 0] = 1
-    fixed[(i == nx) & (j == 0), 1] = 1
-    fixed[k == 0, 2] = 1
-    f = np.zeros(3 * NX * NY * NZ)
-    top = int(np.nonzero((i == 0) & (j == ny) & (k == nz // 2))[0][0])
-    f[3 * top + 1] = -1.0
-    return fixed.ravel(), f
-
-
-def smoothed_density(ctx, ne, seed, R):
-    """rho ~ U(0.01, 1), one Helmholtz filter pass of radius R (on the device)."""
-    rho = np.random.default_rng(seed).uniform(0.01, 1.0, ne)
-    out, _ = ctx.helmholtz(R / (2 * math.sqrt(3)), rho, quirk=0, tol=1e-8, max_iter=2000, check_every=1)
-    return np.clip(out, 0.01, 1.0)
 
 
 def other_workload(args, hx, L, ctx):
@@ -113,13 +79,7 @@ def other_workload(args, hx, L, ctx):
         fixed, _ = cantilever(nx, ny, nz)
         ctx.set_grid(nx, ny, nz, 1.0, 1.0, 1.0, 0.3)
         ctx.set_fixed(fixed)
-        rng = np.random.default_rng(8)
-        F = np.zeros((8, 3 * NX * NY * NZ))
-        picks = rng.choice(NY * NZ, size=8, replace=False)  # 8 distinct nodes of the x = nx face
-        for c in range(8):
-            j, k = int(picks[c] % NY), int(picks[c] // NY)
-            d = rng.standard_normal(3)
-            F[c, 3 * (nx + NX * (j + NY * k)):3 * (nx + NX * (j + NY * k)) + 3] = d / np.linalg.norm(d)
+        F = block_loads(nx, ny, nz, 8, seed=8)
         Ee = 1e-9 + smoothed_density(ctx, nx * ny * nz, 3, 2.0) ** 3
         ctx.tune_set(ctx.autotune(8))
         ctx.block_pcg(Ee, F, tol=1e-8, precision=L.HX_FP32_IR)  # warm-up

@@ -1,0 +1,150 @@
+"""The measured configurations, each on its real workload:
+
+* 60x20x20 cantilever, 20 SIMP/OC iterations, in the plain FP64 Jacobi form
+  and in the bench's main form (FP32 operator and vectors with FP64 reliable
+  updates, V-cycle preconditioner), against the oracle's loop: compliance
+  and physical densities of every iteration to 1e-6;
+* 192x96x96 half MBB beam, solver only, on a seeded smoothed density: every
+  form's FP64 true residual (recomputed by the oracle) below the tolerance and
+  its compliance within 1e-6 of the FP64 Jacobi form's;
+* 256x128x128 cantilever (12.8M DOFs), two TO iterations of the main form
+  against the plain FP64 form on the device: compliance, physical densities
+  and von Mises stress to 1e-6, the volume constraint held;
+* 128x64x64, eight load cases as one block solve: every column equal to its
+  single solve, the 1/8-weighted compliance, every column's true residual.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def wl(hx):
+    from importlib import import_module
+    return import_module("2025_12_09_hexat_v069_amd.workloads")
+
+
+@pytest.fixture(scope="module")
+def ctx(hx):
+    c = hx.Context(0)
+    yield c
+    c.close()
+
+
+def _to_params(L, precision, precond, tol, volfrac, rmin):
+    return L.TOParams(E0=1.0, Emin=1e-9, nu=0.3, penal=3.0, volfrac=volfrac, move=0.2, xmin=0.0, rmin=rmin,
+                      pcg_tol=tol, pcg_max_iter=200000, precision=precision, precond=precond, filter_tol=1e-12,
+                      filter_max_iter=5000, oc_rel_tol=1e-10, with_stress=1)
+
+
+@pytest.fixture(scope="module")
+def cfg0_oracle(ox, wl):
+    g = ox.Grid(60, 20, 20)
+    fixed, f = wl.cantilever(60, 20, 20)
+    _say("oracle: 20 SIMP iterations at 60x20x20")
+    hist, x, xphys, u = ox.simp_to_loop(g, f, fixed, 20, pcg_tol=1e-10, filter_tol=1e-12, oc_rel_tol=1e-10)
+    return g, fixed, f, hist, xphys
+
+
+@pytest.mark.parametrize("form", ["plain", "precision_vc"])
+def test_cfg0_cantilever_20_iterations(hx, ctx, cfg0_oracle, form):
+    L = hx._lib
+    g, fixed, f, hist, xphys_o = cfg0_oracle
+    prec, pc = (L.HX_FP64, L.HX_PC_JACOBI) if form == "plain" else (L.HX_FP32_IR, L.HX_PC_VCYCLE)
+    ctx.set_grid(60, 20, 20, 1.0, 1.0, 1.0, 0.3)
+    ctx.set_fixed(fixed)
+    ctx.to_init(_to_params(L, prec, pc, 1e-10, 0.3, 1.5), f, np.full(g.ne, 0.3))
+    for h in hist:
+        s = ctx.to_step()
+        _say(form, "it", s.iter, "c", s.compliance, h["compliance"], "pcg", s.pcg_iterations, h["pcg_iterations"])
+        assert s.pcg_status == L.HX_OK and s.pcg_rel_res < 1e-10
+        assert abs(s.compliance - h["compliance"]) <= 1e-6 * abs(h["compliance"])
+        assert abs(s.change - h["change"]) <= 1e-6
+    xp = ctx.to_read(L.HX_FIELD_XPHYS)
+    assert np.abs(xp - xphys_o).max() <= 1e-6
+    assert abs(xp.mean() - 0.3) <= 1e-8
+
+
+def test_cfg1_mbb_solver_forms(ox, hx, ctx, wl):
+    L = hx._lib
+    nx, ny, nz = 192, 96, 96
+    fixed, f = wl.mbb_half(nx, ny, nz)
+    ctx.set_grid(nx, ny, nz, 1.0, 1.0, 1.0, 0.3)
+    ctx.set_fixed(fixed)
+    Ee = 1e-9 + wl.smoothed_density(ctx, nx * ny * nz, 1, 2.0) ** 3
+    g = ox.Grid(nx, ny, nz)
+    Ke = ox.ke_hex(1, 1, 1, 0.3)
+    comp = {}
+    for name, prec, pc in (("plain", L.HX_FP64, L.HX_PC_JACOBI), ("precision", L.HX_FP32_IR, L.HX_PC_JACOBI),
+                           ("plain_vc", L.HX_FP64, L.HX_PC_VCYCLE), ("precision_vc", L.HX_FP32_IR, L.HX_PC_VCYCLE)):
+        _say("cfg1", name)
+        u, st = ctx.pcg(Ee, f, tol=1e-8, precision=prec, precond=pc, raise_on_fail=True)
+        r = np.where(fixed == 0, f - ox.matvec(g, Ke, Ee, u), 0.0)
+        true_rel = np.linalg.norm(r) / np.linalg.norm(f)
+        comp[name] = float(f @ u)
+        _say("  its", st.iterations, "rel", st.rel_res, "oracle true rel", true_rel, "c", comp[name])
+        assert true_rel < 1.05e-8 and abs(true_rel - st.rel_res) <= 1e-3 * true_rel
+    for name, c in comp.items():
+        assert abs(c - comp["plain"]) <= 1e-6 * abs(comp["plain"])
+
+
+def test_cfg2_main_form_against_plain_fp64(hx, ctx, wl):
+    """Two TO iterations at 12.8M DOFs: the main form (FP32 contraction,
+    V-cycle) against the plain FP64 Jacobi form, both to 1e-8."""
+    L = hx._lib
+    nx, ny, nz = 256, 128, 128
+    fixed, f = wl.cantilever(nx, ny, nz)
+    out = {}
+    for form, prec, pc in (("plain", L.HX_FP64, L.HX_PC_JACOBI), ("precision_vc", L.HX_FP32_IR, L.HX_PC_VCYCLE)):
+        ctx.set_grid(nx, ny, nz, 1.0, 1.0, 1.0, 0.3)
+        ctx.set_fixed(fixed)
+        p = _to_params(L, prec, pc, 1e-8, 0.15, 2.5)
+        p.filter_tol = 1e-10
+        ctx.to_init(p, f, np.full(nx * ny * nz, 0.15))
+        rec = []
+        for _ in range(2):
+            s = ctx.to_step()
+            _say("cfg2", form, "it", s.iter, "c", s.compliance, "pcg", s.pcg_iterations, "rel", s.pcg_rel_res,
+                 "t", s.seconds)
+            assert s.pcg_status == L.HX_OK and s.pcg_rel_res < 1e-8
+            rec.append((s.compliance, ctx.to_read(L.HX_FIELD_XPHYS), ctx.to_read(L.HX_FIELD_VM)))
+        out[form] = rec
+    for (c0, x0, v0), (c1, x1, v1) in zip(out["plain"], out["precision_vc"]):
+        _say("  dc", abs(c1 - c0) / c0, "dx", np.abs(x1 - x0).max(), "dvm", np.abs(v1 - v0).max() / v0.max())
+        assert abs(c1 - c0) <= 1e-6 * c0
+        assert np.abs(x1 - x0).max() <= 1e-6
+        assert np.abs(v1 - v0).max() <= 1e-6 * v0.max()
+        assert abs(x1.mean() - 0.15) <= 1e-8
+
+
+def test_cfg3_block8(ox, hx, ctx, wl):
+    L = hx._lib
+    nx, ny, nz = 128, 64, 64
+    fixed, _ = wl.cantilever(nx, ny, nz)
+    ctx.set_grid(nx, ny, nz, 1.0, 1.0, 1.0, 0.3)
+    ctx.set_fixed(fixed)
+    F = wl.block_loads(nx, ny, nz, 8, seed=8)
+    Ee = 1e-9 + wl.smoothed_density(ctx, nx * ny * nz, 3, 2.0) ** 3
+    U, S = ctx.block_pcg(Ee, F, tol=1e-8, precision=L.HX_FP32_IR)
+    g = ox.Grid(nx, ny, nz)
+    Ke = ox.ke_hex(1, 1, 1, 0.3)
+    singles = []
+    for c in range(8):
+        u, st = ctx.pcg(Ee, F[c], tol=1e-8, precision=L.HX_FP32_IR, raise_on_fail=True)
+        singles.append(float(F[c] @ u))
+        r = np.where(fixed == 0, F[c] - ox.matvec(g, Ke, Ee, U[c]), 0.0)
+        true_rel = np.linalg.norm(r) / np.linalg.norm(F[c])
+        _say("cfg3 column", c, "its", S[c].iterations, st.iterations, "true rel", true_rel)
+        assert S[c].status == L.HX_OK and true_rel < 1.05e-8
+        assert np.array_equal(U[c], u)
+    wc = sum(float(F[c] @ U[c]) for c in range(8)) / 8
+    assert abs(wc - sum(singles) / 8) <= 1e-12 * abs(wc)
