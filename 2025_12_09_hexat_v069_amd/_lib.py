@@ -46,11 +46,12 @@ EXPORTS = [
     "hx_oc_update",
     "hx_to_init", "hx_to_step", "hx_to_read", "hx_learned_train", "hx_learned_weights", "hx_timing_enable",
     "hx_timing_read", "hx_device_info", "hx_autotune", "hx_tune_set", "hx_time_operator", "hx_time_operator_pc",
-    "hx_time_smoothing", "hx_device_memory",
+    "hx_time_smoothing", "hx_device_memory", "hx_web_pack_device",
 ]
 #: test hooks declared in include/hexat_testing.h
 TEST_EXPORTS = ["hx_test_unet_nparam", "hx_test_unet_init", "hx_test_unet_forward", "hx_test_unet_backward",
-                "hx_test_copy", "hx_test_ml_apply", "hx_test_vc_apply"]
+                "hx_test_copy", "hx_test_ml_apply", "hx_test_vc_apply", "hx_test_device_alloc",
+                "hx_test_device_free", "hx_test_memcpy"]
 
 
 class HexatError(RuntimeError):
@@ -157,6 +158,7 @@ def load():
         L.hx_web_pack.argtypes = [vp, P(C.c_float), P(C.c_float), P(C.c_float), C.c_float, P(C.c_float),
                                   P(C.c_float), P(C.c_float), P(C.c_uint32)]
         L.hx_compliance_sens.argtypes = [vp, pd, d, d, d, pd, pd, pd, pd, pd]
+        L.hx_web_pack_device.argtypes = [vp, vp, vp, vp, C.c_float, vp, vp, vp, vp]
         L.hx_oc_update.argtypes = [vp, pd, pd, pd, d, d, d, d, pd, pd, P(C.c_int), pd]
         L.hx_to_init.argtypes = [vp, P(TOParams), pd, pd]
         L.hx_to_step.argtypes = [vp, P(TOStats)]
@@ -173,6 +175,9 @@ def load():
         L.hx_time_operator_pc.argtypes = [vp, i, i, i, i, P(C.c_double)]
         L.hx_time_smoothing.argtypes = [vp, i, i, i, P(C.c_double)]
         L.hx_test_copy.argtypes = [vp, C.c_long, i, P(C.c_double)]
+        L.hx_test_device_alloc.argtypes = [vp, C.c_long, P(vp)]
+        L.hx_test_device_free.argtypes = [vp, vp]
+        L.hx_test_memcpy.argtypes = [vp, vp, vp, C.c_long, i]
         L.hx_test_ml_apply.argtypes = [vp, pd, pd, pd, i, P(C.c_int)]
         L.hx_test_vc_apply.argtypes = [vp, pd, pd, pd, i, P(C.c_int), P(C.c_int)]
         pf = P(C.c_float)

@@ -123,6 +123,7 @@ struct hx_ctx {
     DevBuf<double> e_pr, e_vm, e_sv, e_l1, e_dir, e_ce, e_dc, e_aux, small;
     DevBuf<float> w_rho, w_l1, w_pr, w_cent, w_rho_out, w_sl1;  // hx_web_pack
     DevBuf<unsigned> w_cnt, w_offs;
+    unsigned* web_scratch = nullptr;  // device address of the packing's static scratch (hx_web_pack_device)
     // SIMP session
     bool to_ready = false;
     hx_to_params tp{};
@@ -1109,6 +1110,10 @@ int hx_ctx_create(int device, hx_ctx** out) {
         return HX_ERR_HIP;
     }
     std::memset(c->st_host, 0, kMaxRhs * sizeof(PcgState));
+    if (web_scratch_address(device, &c->web_scratch) != 0) {  // device current (hipSetDevice above)
+        hx_ctx_destroy(c);
+        return HX_ERR_HIP;
+    }
     *out = c;
     return HX_OK;
 }
@@ -1595,6 +1600,24 @@ int hx_web_pack(hx_ctx* c, const float* density, const float* l1, const float* p
         (rc = download(c, density_out, c->w_rho_out.p, (long)n * sizeof(float))) ||
         (rc = download(c, signed_l1, c->w_sl1.p, (long)n * sizeof(float))))
         return rc;
+    return HX_OK;
+}
+
+int hx_web_pack_device(hx_ctx* c, const float* density, const float* l1, const float* principal, float threshold,
+                       float* centroids, float* density_out, float* signed_l1, uint32_t* count) {
+    if (!c || !density || !l1 || !principal || !centroids || !density_out || !signed_l1 || !count)
+        return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    const int nt = web_tiles(c->g.ne);
+    if (nt > kWebScratchTiles)
+        return fail(c, HX_ERR_BAD_ARG, "hx_web_pack_device: at most " + std::to_string((long)kWebScratchTiles * 2048) +
+                                           " elements (the scratch's tiles)");
+    if (!c->web_scratch) return fail(c, HX_ERR_STATE, "hx_web_pack_device: no scratch address on this device");
+    HXC(hipSetDevice(c->device));
+    launch_web_pack(c->g, density, l1, principal, threshold, c->web_scratch, c->web_scratch + kWebScratchTiles,
+                    centroids, density_out, signed_l1, c->stream, count);
+    HXK();
+    HXC(hipStreamSynchronize(c->stream));
     return HX_OK;
 }
 
@@ -2334,6 +2357,26 @@ int hx_tune_set(hx_ctx* c, const hx_tune_result* t) {
     c->mv_variant = t->variant64;
     c->mv32_variant = t->variant32;
     clear_graphs(c);
+    return HX_OK;
+}
+
+int hx_test_device_alloc(hx_ctx* c, long bytes, void** ptr) {
+    if (!c || bytes <= 0 || !ptr) return HX_ERR_BAD_ARG;
+    HXC(hipSetDevice(c->device));
+    HXC(hipMalloc(ptr, (size_t)bytes));
+    return HX_OK;
+}
+int hx_test_device_free(hx_ctx* c, void* ptr) {
+    if (!c) return HX_ERR_BAD_ARG;
+    HXC(hipStreamSynchronize(c->stream));
+    if (ptr) HXC(hipFree(ptr));
+    return HX_OK;
+}
+int hx_test_memcpy(hx_ctx* c, void* dst, const void* src, long bytes, int to_device) {
+    if (!c || !dst || !src || bytes < 0) return HX_ERR_BAD_ARG;
+    HXC(hipMemcpyAsync(dst, src, (size_t)bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost,
+                       c->stream));
+    HXC(hipStreamSynchronize(c->stream));
     return HX_OK;
 }
 

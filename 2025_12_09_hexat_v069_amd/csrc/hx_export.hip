@@ -11,6 +11,8 @@
 // structured grid has no node array on the device.  Bytes per element: 4 for
 // the density test (read twice, the second time mostly from L2) plus 16 read
 // (l1, 3 principal values) and 20 written for each kept element.
+#include <mutex>
+
 #include "hx_launch.h"
 
 namespace hx {
@@ -41,9 +43,9 @@ __global__ void __launch_bounds__(WT) k_web_count(const float* __restrict__ dens
 }
 
 // Exclusive scan of ntiles counts in one workgroup of 1024 threads;
-// offs[ntiles] receives the total.
+// offs[ntiles] receives the total, and so does *total_out when given.
 __global__ void __launch_bounds__(1024) k_web_scan(const unsigned* __restrict__ tile_count, int ntiles,
-                                                   unsigned* __restrict__ offs) {
+                                                   unsigned* __restrict__ offs, unsigned* __restrict__ total_out) {
     __shared__ unsigned part[1024];
     unsigned carry = 0;
     for (int c0 = 0; c0 < ntiles; c0 += 1024) {
@@ -62,7 +64,10 @@ __global__ void __launch_bounds__(1024) k_web_scan(const unsigned* __restrict__ 
         carry += part[1023];
         __syncthreads();
     }
-    if (threadIdx.x == 0) offs[ntiles] = carry;
+    if (threadIdx.x == 0) {
+        offs[ntiles] = carry;
+        if (total_out) *total_out = carry;
+    }
 }
 
 // Centroid of element (i,j,k) as MeshUtilities.element_centroid
@@ -131,15 +136,40 @@ __global__ void __launch_bounds__(WT) k_web_write(const float* __restrict__ dens
 
 }  // namespace
 
+// Tile counts and offsets of the packing entry point that allocates nothing
+// (hx_web_pack_device): kWebScratchTiles counts, then kWebScratchTiles + 1
+// offsets.  Each GPU holds its own copy until the process ends; its address
+// comes from hipGetSymbolAddress (web_scratch_address), never from the
+// array's name in host code.  Every entry of a call's range is written by
+// that call before it is read, so nothing is cleared between calls; calls
+// that overlap on one GPU race on it.
+static __device__ __align__(256) unsigned scratch[2 * kWebScratchTiles + 1];
+
+int web_scratch_address(int device, unsigned** out) {
+    static std::mutex mu;
+    static unsigned* ptr[kWebScratchDevices] = {};
+    if (device < 0 || device >= kWebScratchDevices) return 1;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[device]) {
+        int cur = -1;
+        if (hipGetDevice(&cur) != hipSuccess || cur != device) return 2;  // the device must be current
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return 3;
+        ptr[device] = (unsigned*)p;
+    }
+    *out = ptr[device];
+    return 0;
+}
+
 int web_tiles(long ne) { return (int)((ne + WTILE - 1) / WTILE); }
 
 void launch_web_pack(const GridArgs& g, const float* density, const float* l1, const float* principal, float thr,
                      unsigned* tile_count, unsigned* offs, float* cent, float* dens_out, float* sl1_out,
-                     hipStream_t s) {
+                     hipStream_t s, unsigned* total_out) {
     const int nt = web_tiles(g.ne);
     if (nt == 0) return;
     hipLaunchKernelGGL(k_web_count, dim3(nt), dim3(WT), 0, s, density, g.ne, thr, tile_count);
-    hipLaunchKernelGGL(k_web_scan, dim3(1), dim3(1024), 0, s, tile_count, nt, offs);
+    hipLaunchKernelGGL(k_web_scan, dim3(1), dim3(1024), 0, s, tile_count, nt, offs, total_out);
     hipLaunchKernelGGL(k_web_write, dim3(nt), dim3(WT), 0, s, density, l1, principal, g.ne, thr, g.nx, g.ny,
                        (float)g.dx, (float)g.dy, (float)g.dz, offs, cent, dens_out, sl1_out);
 }

@@ -92,7 +92,14 @@ void launch_copy16(long n, const void* src, void* dst, hipStream_t s);
 int web_tiles(long ne);
 void launch_web_pack(const GridArgs& g, const float* density, const float* l1, const float* principal, float thr,
                      unsigned* tile_count, unsigned* offs, float* cent, float* dens_out, float* sl1_out,
-                     hipStream_t s);
+                     hipStream_t s, unsigned* total_out = nullptr);
+// the packing's scratch for hx_web_pack_device: kWebScratchTiles tile counts
+// then kWebScratchTiles + 1 offsets (at most kWebScratchTiles * 2048 elements)
+constexpr int kWebScratchTiles = 65536;
+constexpr int kWebScratchDevices = 64;
+// its device address on `device`, which must be current (looked up once per
+// device); 0 on success
+int web_scratch_address(int device, unsigned** out);
 template <typename T>
 void launch_pcg_init(long n, const T* b, T* x, const T* q, const T* minv, T* r, T* p, double shift,
                      PcgState* st, double* partials, hipStream_t s, int nrhs = 1);

@@ -209,6 +209,20 @@ int hx_element_stress(hx_ctx* ctx, const double* Ee, const double* u, double* pr
  * (Postprocessing.jl:173-195); the file itself is written by the host. */
 int hx_web_pack(hx_ctx* ctx, const float* density, const float* l1, const float* principal, float threshold,
                 float* centroids, float* density_out, float* signed_l1, uint32_t* count);
+/* hx_web_pack on caller-owned DEVICE buffers that allocates nothing: no
+ * hipMalloc, no hipFree, no allocating library call.  No counterpart in the
+ * reference; it is for callers that cannot allocate (code under graph
+ * capture, tight serving loops, harnesses that forbid it).  Every pointer is
+ * device memory: the inputs (density, l1: Ne; principal: Ne x 3), the outputs
+ * sized for the worst case (centroids 3 Ne, density_out and signed_l1 Ne) and
+ * the kept count (one uint32_t).  The tile counts live in a static device
+ * array of 2 * 65536 + 1 words (512 KiB, held on each GPU the library runs on
+ * until the process ends), so the grid may hold at most 65536 * 2048 =
+ * 134,217,728 elements (HX_ERR_BAD_ARG beyond).  Calls that overlap on one GPU
+ * race on that array.  Runs on the context's stream and synchronizes it. */
+int hx_web_pack_device(hx_ctx* ctx, const float* density, const float* l1, const float* principal,
+                       float threshold, float* centroids, float* density_out, float* signed_l1,
+                       uint32_t* count);
 /* ce_e = u_e^T Ke(E=1) u_e and SIMP sensitivity
  * dc_e = -penal xphys^(penal-1) (E0-Emin) ce_e; compliance = f^T u.
  * No reference counterpart (the reference has no sensitivities). */

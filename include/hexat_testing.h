@@ -15,6 +15,13 @@
 extern "C" {
 #endif
 
+/* Device memory for tests of the entry points that take device pointers
+ * (hx_web_pack_device): allocate, free, and copy host <-> device
+ * (to_device 1: src host, dst device; 0: src device, dst host). */
+int hx_test_device_alloc(hx_ctx* ctx, long bytes, void** ptr);
+int hx_test_device_free(hx_ctx* ctx, void* ptr);
+int hx_test_memcpy(hx_ctx* ctx, void* dst, const void* src, long bytes, int to_device);
+
 /* Plain device copy of `bytes` bytes (a multiple of 16) between two buffers
  * the call allocates, `reps` launches of one copy kernel; average launch time
  * in ms.  The reference kernel for scaling hardware byte counters. */
