@@ -53,10 +53,31 @@ namespace {
 // is never replayed (run_batch drops the cache when it has moved).
 unsigned long g_buf_gen = 0;
 
+// Device buffer that frees itself: owned by one object at a time (moves,
+// no copies), so the timing and tuning helpers' local buffers and the
+// context's members are all released.
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
     hipError_t ensure(size_t m) {
         if (m <= n && p) return hipSuccess;
         release();

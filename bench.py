@@ -172,10 +172,7 @@ def run_form(hx, L, ctx, nx, ny, nz, f, fixed, precision, steps, warmup, args, t
     elif not args.no_autotune:
         t = ctx.autotune(1)
         log(f"  autotune: {t}")
-    p = L.TOParams(E0=1.0, Emin=1e-9, nu=0.3, penal=3.0, volfrac=args.volfrac, move=0.2, xmin=0.0,
-                   rmin=args.rmin, pcg_tol=args.tol, pcg_max_iter=args.max_iter, precision=precision, precond=precond,
-                   filter_tol=1e-10, filter_max_iter=2000, oc_rel_tol=1e-9, with_stress=1)
-    ctx.to_init(p, f, np.full(nx * ny * nz, args.volfrac))
+    ctx.to_init(to_params(L, args, precision, precond), f, np.full(nx * ny * nz, args.volfrac))
     for w in range(warmup):
         s = ctx.to_step()
         log(f"  warmup {w + 1}: c={s.compliance:.6e} pcg={s.pcg_iterations} t={s.seconds:.5f}s")
@@ -210,18 +207,77 @@ def run_form(hx, L, ctx, nx, ny, nz, f, fixed, precision, steps, warmup, args, t
             ms = ctx.time_smoothing(precision, form, 40)
             kern[nm] = {"avg_ms": ms, "count": napp, "total_ms": napp * ms}
     log(f"  breakdown step: pcg={s.pcg_iterations} t={s.seconds:.3f}s; operator kernel {op_ms:.4f} ms")
-    return wall / steps, recs, kern
+    return wall / steps, recs, kern, ctx.to_read(L.HX_FIELD_X)
+
+
+def to_params(L, args, precision, precond):
+    return L.TOParams(E0=1.0, Emin=1e-9, nu=0.3, penal=3.0, volfrac=args.volfrac, move=0.2, xmin=0.0,
+                      rmin=args.rmin, pcg_tol=args.tol, pcg_max_iter=args.max_iter, precision=precision,
+                      precond=precond, filter_tol=1e-10, filter_max_iter=2000, oc_rel_tol=1e-9, with_stress=1)
+
+
+def per_result_error(L, ctx, nx, ny, nz, f, fixed, x, main, tune, args):
+    """One TO iteration from the same design x in the main form and in the
+    plain FP64 form (fresh sessions, the main form's launch shapes): the
+    difference of each result -- compliance, physical densities after the
+    update, von Mises stress -- is the main form's error against FP64."""
+    out = {}
+    for form in ("plain", main):
+        prec, pc = (getattr(L, nm) for nm in FORMS[form])
+        ctx.set_grid(nx, ny, nz, 1.0, 1.0, 1.0, 0.3)
+        ctx.set_fixed(fixed)
+        if tune is not None:
+            ctx.tune_set(tune)
+        ctx.to_init(to_params(L, args, prec, pc), f, x)
+        s = ctx.to_step()
+        out[form] = (s, ctx.to_read(L.HX_FIELD_XPHYS), ctx.to_read(L.HX_FIELD_VM), ctx.to_read(L.HX_FIELD_U))
+        log(f"  per-result check, {form}: c={s.compliance:.9e} pcg={s.pcg_iterations} rel={s.pcg_rel_res:.2e}")
+    (s0, x0, v0, u0), (s1, x1, v1, u1) = out["plain"], out[main]
+    return {"compliance_rel": abs(s1.compliance - s0.compliance) / abs(s0.compliance),
+            "xphys_max_abs": float(np.abs(x1 - x0).max()),
+            "von_mises_max_rel": float(np.abs(v1 - v0).max() / np.abs(v0).max()),
+            "displacement_max_rel": float(np.abs(u1 - u0).max() / np.abs(u0).max()),
+            "volume_main": float(x1.mean()), "pcg_rel_res_main": s1.pcg_rel_res, "pcg_rel_res_plain": s0.pcg_rel_res,
+            "from": "one TO iteration of each form from the main form's design after its timed steps"}
+
+
+def full_loop(L, ctx, nx, ny, nz, f, fixed, form, n_iter, tune, args):
+    """The whole TO loop of the configuration (n_iter iterations from uniform
+    rho) in one form: wall time and solver record of every iteration."""
+    prec, pc = (getattr(L, nm) for nm in FORMS[form])
+    ctx.set_grid(nx, ny, nz, 1.0, 1.0, 1.0, 0.3)
+    ctx.set_fixed(fixed)
+    if tune is not None:
+        ctx.tune_set(tune)
+    ctx.to_init(to_params(L, args, prec, pc), f, np.full(nx * ny * nz, args.volfrac))
+    t, its, ch, c = [], [], [], []
+    for k in range(n_iter):
+        t0 = time.perf_counter()
+        s = ctx.to_step()
+        t.append(time.perf_counter() - t0)
+        its.append(s.pcg_iterations)
+        ch.append(round(s.change, 5))
+        c.append(s.compliance)
+        if k % 10 == 9:
+            log(f"  full loop {form}: iteration {k + 1}: c={s.compliance:.6e} change={s.change:.4f} "
+                f"pcg={s.pcg_iterations} t={t[-1]:.4f}s")
+    return {"form": form, "iterations": n_iter, "s_per_iter_mean": round(float(np.mean(t)), 6),
+            "s_per_iter_first10": round(float(np.mean(t[:10])), 6), "s_per_iter_last10": round(float(np.mean(t[-10:])), 6),
+            "s_per_iteration": [round(v, 5) for v in t], "pcg_iterations": its, "change": ch,
+            "compliance_final": c[-1]}
 
 
 COPY_BYTES = 1 << 30
-SAMPLE_PCG_ITERS = 40
+SAMPLE_LAUNCHES = 40
 
 
 def kernel_sample(args):
     """Child process run under ``rocprofv3 --pmc``: autotune on the workload's
-    grid, a plain 1 GiB device copy (the counter calibration), then a bounded
-    PCG of the given form on the uniform-density SIMP moduli, so the dominant
-    kernel runs SAMPLE_PCG_ITERS times at its production shape.  Prints the
+    grid, a plain 1 GiB device copy (the counter calibration), then the
+    dominant kernel of the given form SAMPLE_LAUNCHES times back to back at its
+    production shape, on vector sets larger together than the last-level
+    cache -- the launches whose average duration is the bench's
+    avg_launch_ms, so counters and time come from one regime.  Prints the
     tuning result as JSON."""
     nx, ny, nz = (int(v) for v in args.grid.lower().split("x"))
     hx = importlib.import_module("2025_12_09_hexat_v069_amd")
@@ -238,7 +294,12 @@ def kernel_sample(args):
     ctx.test_copy(COPY_BYTES, 4)
     Ee = np.full(nx * ny * nz, 1e-9 + args.volfrac ** 3)
     prec, pc = (getattr(L, nm) for nm in FORMS[args.kernel_sample])
-    ctx.pcg(Ee, f, tol=1e-30, max_iter=SAMPLE_PCG_ITERS, precision=prec, precond=pc)
+    ctx.pcg(Ee, f, tol=1e-30, max_iter=1, precision=prec)  # moduli on the device (Jacobi: no smoothing launches)
+    # the dominant kernel through the same back-to-back hook that gives the bench its launch time
+    if args.kernel_sample.endswith("_vc"):
+        ctx.time_smoothing(prec, 4, SAMPLE_LAUNCHES)
+    else:
+        ctx.time_operator(prec, 1, SAMPLE_LAUNCHES, pc)
     print(json.dumps({"copy_bytes": COPY_BYTES}), flush=True)
 
 
@@ -349,7 +410,14 @@ def cpu_baseline(nx, ny, nz, f, fixed, pcg_iters_per_step, filt_iters_per_step, 
     t_elem = time.perf_counter() - t0
     per_step = t_pcg * pcg_iters_per_step + t_filt * filt_iters_per_step + t_elem
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": per_step, "unit": "s/TO-iteration", "cores": cores, "kind": "port",
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), None)
+    except OSError:
+        pass
+    return {"value": per_step, "unit": "s/TO-iteration", "cores": cores, "kind": "port", "cpu_model": model,
+            "nproc": os.cpu_count(), "extrapolated": True,
             "sample": (f"{n_pcg} FP64 Jacobi-PCG iterations ({t_pcg:.3f} s each), {n_filt} Helmholtz PCG iterations "
                        f"({t_filt:.4f} s each) and one sensitivity+stress pass ({t_elem:.3f} s) on the "
                        f"{nx}x{ny}x{nz} grid with the oracle, extrapolated with the GPU run's mean "
@@ -367,12 +435,18 @@ def main():
     ap.add_argument("--rmin", type=float, default=2.5)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--max-iter", type=int, default=100000)
-    ap.add_argument("--forms", default="plain,precision,precision_ml,precision_vc",
+    ap.add_argument("--forms", default="plain,precision_vc",
                     help="comma-separated: plain (FP64 Jacobi-PCG), precision (FP32 operator + FP64 refinement), "
                          "plain_ml / precision_ml (the same with the additive multilevel preconditioner), "
                          "plain_vc / precision_vc (with the V-cycle)")
     ap.add_argument("--main-form", default=None,
-                    help="the form `value` reports (default: precision_ml when run, else precision)")
+                    help="the form `value` reports (default: precision_vc when run, else precision_ml, else "
+                         "precision, else the first form)")
+    ap.add_argument("--full-loop", type=int, default=50,
+                    help="after the timed steps, run the main form's whole loop of this many TO iterations from "
+                         "the start and report every iteration's time (0: skip)")
+    ap.add_argument("--no-error-check", action="store_true",
+                    help="skip the per-result comparison of the main form against plain FP64")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-pcg-iters", type=int, default=20)
@@ -424,10 +498,19 @@ def main():
     for form in forms:
         prec, pc = (getattr(L, nm) for nm in FORMS[form])
         log(f"form {form}:")
-        sps, recs, kern = run_form(hx, L, ctx, nx, ny, nz, f, fixed, prec, args.steps, args.warmup, args, tune,
-                                   precond=pc)
-        res[form] = {"s_per_iter": sps, "recs": recs, "kern": kern}
+        sps, recs, kern, xfinal = run_form(hx, L, ctx, nx, ny, nz, f, fixed, prec, args.steps, args.warmup, args,
+                                           tune, precond=pc)
+        res[form] = {"s_per_iter": sps, "recs": recs, "kern": kern, "x": xfinal}
     r = res[main_form]
+    err = None
+    if not args.no_error_check and main_form != "plain":
+        log("per-result error of the main form against plain FP64:")
+        err = per_result_error(L, ctx, nx, ny, nz, f, fixed, r["x"], main_form, tune, args)
+        log(f"  {err}")
+    loop = None
+    if args.full_loop > 0:
+        log(f"full loop of {args.full_loop} TO iterations, {main_form}:")
+        loop = full_loop(L, ctx, nx, ny, nz, f, fixed, main_form, args.full_loop, tune, args)
     # roofline of the dominant kernel of the main form.  Algorithmic bytes per
     # launch: the PCG matvec reads r, D^-1, p_old (the p-update is fused into
     # its plane loads) and E, and writes q and p_new; the update reads x, p, r,
@@ -496,7 +579,13 @@ def main():
                                f"each iteration", "form": main_form,
                    "preconditioner": {"_ml": "multilevel", "_vc": "vcycle"}.get(main_form[-3:], "jacobi"),
                    "iterations_timed": f"TO iterations {args.warmup + 1}..{args.warmup + args.steps} from uniform "
-                                       f"rho={args.volfrac}", "tune": tune},
+                                       f"rho={args.volfrac}",
+                   "contraction": "FP32 parity-basis sum factorisation on VALU (the FP32 MFMA contraction is built "
+                                  "and tested, and slower)" if FORMS[main_form][0] == "HX_FP32_IR"
+                                  else "FP64 parity-basis sum factorisation on VALU",
+                   "tune": tune},
+        "per_result_error_vs_fp64": err,
+        "full_loop": loop,
         "roofline": roof,
         "preconditioner_apply": roof_ml,
         "cpu_baseline": cpu,
