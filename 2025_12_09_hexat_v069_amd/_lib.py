@@ -46,7 +46,7 @@ EXPORTS = [
     "hx_oc_update",
     "hx_to_init", "hx_to_step", "hx_to_read", "hx_learned_train", "hx_learned_weights", "hx_timing_enable",
     "hx_timing_read", "hx_device_info", "hx_autotune", "hx_tune_set", "hx_time_operator", "hx_time_operator_pc",
-    "hx_time_smoothing", "hx_device_memory", "hx_web_pack_device",
+    "hx_time_smoothing", "hx_device_memory", "hx_web_pack_device", "hx_vtk_pack",
 ]
 #: test hooks declared in include/hexat_testing.h
 TEST_EXPORTS = ["hx_test_unet_nparam", "hx_test_unet_init", "hx_test_unet_forward", "hx_test_unet_backward",
@@ -159,6 +159,9 @@ def load():
                                   P(C.c_float), P(C.c_float), P(C.c_uint32)]
         L.hx_compliance_sens.argtypes = [vp, pd, d, d, d, pd, pd, pd, pd, pd]
         L.hx_web_pack_device.argtypes = [vp, vp, vp, vp, C.c_float, vp, vp, vp, vp]
+        pu32 = P(C.c_uint32)
+        L.hx_vtk_pack.argtypes = [vp, pd, pd, P(C.c_uint8), P(C.c_float), P(C.c_float), P(C.c_float), C.c_float, pu32,
+                                  pu32, pu32, pu32]
         L.hx_oc_update.argtypes = [vp, pd, pd, pd, d, d, d, d, pd, pd, P(C.c_int), pd]
         L.hx_to_init.argtypes = [vp, P(TOParams), pd, pd]
         L.hx_to_step.argtypes = [vp, P(TOStats)]
@@ -386,6 +389,29 @@ class Context:
                                       fp(cent), fp(dout), fp(sl1), C.byref(n)), "hx_web_pack")
         k = int(n.value)
         return cent[:k].copy(), dout[:k].copy(), sl1[:k].copy()
+
+    def vtk_pack(self, u, f, fixed, density, l1, vm, threshold):
+        """Big-endian data blocks of the legacy VTK file (hx_vtk_pack):
+        dict of uint32 arrays points (13 nn), cells (9 n), l1, vm, density (n)."""
+        nn, ne = self.nn, self.ne
+        fp = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float32).reshape(ne)
+        rho, l1v, vmv = fp(density), fp(l1), fp(vm)
+        uu = None if u is None else f64(u, self.ndof)
+        ff = None if f is None else f64(f, self.ndof)
+        fx = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8).reshape(self.ndof)
+        pts = np.empty(13 * nn, np.uint32)
+        cells = np.empty(9 * ne, np.uint32)
+        cs = np.empty(3 * ne, np.uint32)
+        n = C.c_uint32(0)
+        pf = lambda a: None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+        pu = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))
+        self.check(self.L.hx_vtk_pack(self.h, dptr(uu), dptr(ff),
+                                      None if fx is None else fx.ctypes.data_as(C.POINTER(C.c_uint8)), pf(rho),
+                                      pf(l1v), pf(vmv), C.c_float(np.float32(threshold)), pu(pts), pu(cells), pu(cs),
+                                      C.byref(n)), "hx_vtk_pack")
+        k = int(n.value)
+        return {"points": pts, "cells": cells[:9 * k].copy(), "l1": cs[:k].copy(), "vm": cs[k:2 * k].copy(),
+                "density": cs[2 * k:3 * k].copy(), "count": k}
 
     def compliance_sens(self, u, xphys=None, penal=3.0, E0=1.0, Emin=1e-9, f=None):
         u = f64(u, self.ndof)

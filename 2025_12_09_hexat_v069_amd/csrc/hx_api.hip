@@ -145,6 +145,10 @@ struct hx_ctx {
     DevBuf<float> w_rho, w_l1, w_pr, w_cent, w_rho_out, w_sl1;  // hx_web_pack
     DevBuf<unsigned> w_cnt, w_offs;
     unsigned* web_scratch = nullptr;  // device address of the packing's static scratch (hx_web_pack_device)
+    DevBuf<unsigned> v_pts, v_cells;  // hx_vtk_pack words
+    DevBuf<float> v_l1, v_vm;
+    DevBuf<double> v_u, v_f;
+    DevBuf<uint8_t> v_fixed;
     // SIMP session
     bool to_ready = false;
     hx_to_params tp{};
@@ -1639,6 +1643,58 @@ int hx_web_pack_device(hx_ctx* c, const float* density, const float* l1, const f
                     centroids, density_out, signed_l1, c->stream, count);
     HXK();
     HXC(hipStreamSynchronize(c->stream));
+    return HX_OK;
+}
+
+int hx_vtk_pack(hx_ctx* c, const double* u, const double* f, const uint8_t* fixed, const float* density,
+                const float* l1, const float* vm, float threshold, uint32_t* points, uint32_t* cells,
+                uint32_t* cell_scalars, uint32_t* count) {
+    if (!c || !density || !points || !cells || !cell_scalars || !count) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    const long ne = c->g.ne, nn = c->g.nn, n = ndof(c);
+    const int nt = web_tiles(ne);
+    HXA(c->v_pts, 13 * nn);
+    HXA(c->v_cells, 12 * ne);
+    HXA(c->w_rho, ne);
+    HXA(c->v_l1, ne);
+    HXA(c->v_vm, ne);
+    HXA(c->w_cnt, nt);
+    HXA(c->w_offs, nt + 1);
+    int rc;
+    if ((rc = upload(c, c->w_rho.p, density, ne * sizeof(float)))) return rc;
+    if (l1) { if ((rc = upload(c, c->v_l1.p, l1, ne * sizeof(float)))) return rc; }
+    else launch_fill32(ne, 0.0f, c->v_l1.p, c->stream);
+    if (vm) { if ((rc = upload(c, c->v_vm.p, vm, ne * sizeof(float)))) return rc; }
+    else launch_fill32(ne, 0.0f, c->v_vm.p, c->stream);
+    if (u) {
+        HXA(c->v_u, n);
+        if ((rc = upload(c, c->v_u.p, u, n * sizeof(double)))) return rc;
+    }
+    if (f) {
+        HXA(c->v_f, n);
+        if ((rc = upload(c, c->v_f.p, f, n * sizeof(double)))) return rc;
+    }
+    if (fixed) {
+        HXA(c->v_fixed, n);
+        if ((rc = upload(c, c->v_fixed.p, fixed, n))) return rc;
+    }
+    unsigned* P = c->v_pts.p;
+    unsigned* Cw = c->v_cells.p;
+    launch_vtk_pack(c->g, u ? c->v_u.p : nullptr, f ? c->v_f.p : nullptr, fixed ? c->v_fixed.p : nullptr, c->w_rho.p,
+                    c->v_l1.p, c->v_vm.p, threshold, c->w_cnt.p, c->w_offs.p, c->w_offs.p + nt, P, P + 3 * nn,
+                    P + 6 * nn, P + 9 * nn, P + 12 * nn, Cw, Cw + 9 * ne, Cw + 10 * ne, Cw + 11 * ne, c->stream);
+    HXK();
+    unsigned kept = 0;
+    if ((rc = download(c, &kept, c->w_offs.p + nt, sizeof(unsigned)))) return rc;
+    if ((long)kept > ne) return fail(c, HX_ERR_HIP, "hx_vtk_pack: kept count exceeds the element count");
+    *count = kept;
+    if ((rc = download(c, points, P, 13 * nn * sizeof(unsigned)))) return rc;
+    if (kept) {
+        if ((rc = download(c, cells, Cw, 9L * kept * sizeof(unsigned)))) return rc;
+        for (int q = 0; q < 3; ++q)
+            if ((rc = download(c, cell_scalars + (long)q * kept, Cw + (9 + q) * ne, (long)kept * sizeof(unsigned))))
+                return rc;
+    }
     return HX_OK;
 }
 

@@ -134,7 +134,109 @@ __global__ void __launch_bounds__(WT) k_web_write(const float* __restrict__ dens
     }
 }
 
+// ---- legacy VTK (ExportVTK.export_solution, ExportVTK.jl:20-172) ------------
+// Every value the file holds is an FP32 (or Int32) word in big-endian byte
+// order (`hton`); the kernels write those words, the host writes the ASCII
+// section headers around them.
+
+__device__ __forceinline__ unsigned be32(float v) { return __builtin_bswap32(__float_as_uint(v)); }
+__device__ __forceinline__ unsigned be32i(int v) { return __builtin_bswap32((unsigned)v); }
+
+// sanitize_data (ExportVTK.jl:35-42): NaN and +-Inf -> 0; then, if the largest
+// magnitude exceeds 1e10, a clamp to +-1e10 -- which changes nothing when it
+// does not, so the clamp is applied to every value
+__device__ __forceinline__ float sanitize(float v) {
+    if (!(v == v) || isinf(v)) return 0.0f;
+    return fminf(fmaxf(v, -1.0e10f), 1.0e10f);
+}
+
+// Point data, one thread per node: coordinates Float32(i)*dx (Mesh.jl:36-38),
+// displacement and external force as FP32, the boundary-condition indicator
+// (1 where fixed), and the displacement magnitude sqrt((d1^2 + d2^2) + d3^2)
+// in FP32 (ExportVTK.jl:58-85).  u, f, fixed: 3 nn (nullable: zeros).
+__global__ void __launch_bounds__(WT) k_vtk_points(GridArgs g, float dx, float dy, float dz,
+                                                   const double* __restrict__ u, const double* __restrict__ f,
+                                                   const uint8_t* __restrict__ fixed, unsigned* __restrict__ pts,
+                                                   unsigned* __restrict__ disp, unsigned* __restrict__ frc,
+                                                   unsigned* __restrict__ bcs, unsigned* __restrict__ dmag) {
+#pragma clang fp contract(off)
+    const long n = (long)blockIdx.x * WT + threadIdx.x;
+    if (n >= g.nn) return;
+    const int i = (int)(n % g.NX), j = (int)((n / g.NX) % g.NY), k = (int)(n / ((long)g.NX * g.NY));
+    pts[3 * n + 0] = be32((float)i * dx);
+    pts[3 * n + 1] = be32((float)j * dy);
+    pts[3 * n + 2] = be32((float)k * dz);
+    float d[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        d[c] = u ? sanitize((float)u[3 * n + c]) : 0.0f;
+        disp[3 * n + c] = be32(d[c]);
+        frc[3 * n + c] = be32(f ? sanitize((float)f[3 * n + c]) : 0.0f);
+        bcs[3 * n + c] = be32((fixed && fixed[3 * n + c]) ? 1.0f : 0.0f);
+    }
+    const float s2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+    dmag[n] = be32(sqrtf(s2));
+}
+
+// Cells of the kept elements (density >= threshold, element order; the
+// ranks as k_web_write's): 8 and the 0-based node ids in the reference's node
+// order, and the cell scalars L1 (sanitized), von Mises and density.
+__global__ void __launch_bounds__(WT) k_vtk_cells(const float* __restrict__ density, const float* __restrict__ l1,
+                                                  const float* __restrict__ vm, long ne, float thr, int nx, int ny,
+                                                  const unsigned* __restrict__ offs, unsigned* __restrict__ cells,
+                                                  unsigned* __restrict__ l1_out, unsigned* __restrict__ vm_out,
+                                                  unsigned* __restrict__ dens_out) {
+    __shared__ unsigned wcnt[WT / 64];
+    const long base = (long)blockIdx.x * WTILE;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const long NX = nx + 1, NY = ny + 1;
+    unsigned run = offs[blockIdx.x];
+    for (int s = 0; s < WSTEPS; ++s) {
+        const long e = base + s * WT + threadIdx.x;
+        const bool k = kept(density, e, ne, thr);
+        const unsigned long long m = __ballot(k);
+        if (lane == 0) wcnt[wave] = (unsigned)__popcll(m);
+        __syncthreads();
+        unsigned before = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < WT / 64; ++w) {
+            before += w < wave ? wcnt[w] : 0u;
+            total += wcnt[w];
+        }
+        if (k) {
+            const unsigned pos = run + before + (unsigned)__popcll(m & below);
+            const long i = e % nx, j = (e / nx) % ny, kk = e / ((long)nx * ny);
+            const long n0 = i + NX * (j + NY * kk);
+            const long nodes[8] = {n0, n0 + 1, n0 + 1 + NX, n0 + NX,
+                                   n0 + NX * NY, n0 + 1 + NX * NY, n0 + 1 + NX + NX * NY, n0 + NX + NX * NY};
+            unsigned* c = cells + 9L * pos;
+            c[0] = be32i(8);
+#pragma unroll
+            for (int a = 0; a < 8; ++a) c[1 + a] = be32i((int)nodes[a]);
+            l1_out[pos] = be32(sanitize(l1[e]));
+            vm_out[pos] = be32(vm[e]);
+            dens_out[pos] = be32(density[e]);
+        }
+        run += total;
+        __syncthreads();
+    }
+}
+
 }  // namespace
+
+void launch_vtk_pack(const GridArgs& g, const double* u, const double* f, const uint8_t* fixed, const float* density,
+                     const float* l1, const float* vm, float thr, unsigned* tile_count, unsigned* offs,
+                     unsigned* total, unsigned* pts, unsigned* disp, unsigned* frc, unsigned* bcs, unsigned* dmag,
+                     unsigned* cells, unsigned* l1_out, unsigned* vm_out, unsigned* dens_out, hipStream_t s) {
+    hipLaunchKernelGGL(k_vtk_points, dim3((unsigned)((g.nn + WT - 1) / WT)), dim3(WT), 0, s, g, (float)g.dx,
+                       (float)g.dy, (float)g.dz, u, f, fixed, pts, disp, frc, bcs, dmag);
+    const int nt = web_tiles(g.ne);
+    hipLaunchKernelGGL(k_web_count, dim3(nt), dim3(WT), 0, s, density, g.ne, thr, tile_count);
+    hipLaunchKernelGGL(k_web_scan, dim3(1), dim3(1024), 0, s, tile_count, nt, offs, total);
+    hipLaunchKernelGGL(k_vtk_cells, dim3(nt), dim3(WT), 0, s, density, l1, vm, g.ne, thr, g.nx, g.ny, offs, cells,
+                       l1_out, vm_out, dens_out);
+}
 
 // Tile counts and offsets of the packing entry point that allocates nothing
 // (hx_web_pack_device): kWebScratchTiles counts, then kWebScratchTiles + 1

@@ -93,6 +93,13 @@ int web_tiles(long ne);
 void launch_web_pack(const GridArgs& g, const float* density, const float* l1, const float* principal, float thr,
                      unsigned* tile_count, unsigned* offs, float* cent, float* dens_out, float* sl1_out,
                      hipStream_t s, unsigned* total_out = nullptr);
+// legacy VTK words (big-endian FP32 / Int32) of ExportVTK.export_solution:
+// point data of every node, cells and cell scalars of the elements with
+// density >= thr in element order (*total receives their count)
+void launch_vtk_pack(const GridArgs& g, const double* u, const double* f, const uint8_t* fixed, const float* density,
+                     const float* l1, const float* vm, float thr, unsigned* tile_count, unsigned* offs,
+                     unsigned* total, unsigned* pts, unsigned* disp, unsigned* frc, unsigned* bcs, unsigned* dmag,
+                     unsigned* cells, unsigned* l1_out, unsigned* vm_out, unsigned* dens_out, hipStream_t s);
 // the packing's scratch for hx_web_pack_device: kWebScratchTiles tile counts
 // then kWebScratchTiles + 1 offsets (at most kWebScratchTiles * 2048 elements)
 constexpr int kWebScratchTiles = 65536;

@@ -14,11 +14,12 @@ from __future__ import annotations
 
 import argparse
 import datetime
+import os
 import time
 
 import numpy as np
 
-from . import boundary, configuration, mesh, stress, topopt
+from . import boundary, configuration, mesh, postprocessing, stress, topopt
 from .solver import solve_system
 
 LOG_HEADER = ("| Iter | Mesh Size | Total El | Active El | Radius | Cutoff | Compliance | Strain Energy | "
@@ -77,32 +78,76 @@ def build_problem(config):
     return geo, nodes, elements, bc, F
 
 
-def run_stress_ratio(config, n_iter=None, log_path=None, verbose=True, keep_density=False):
-    """Main.jl:331-474 without refinement/export.  keep_density: each
-    iteration's record also holds a copy of the density after its update."""
+def _export(results_dir, base, it, nodes, elements, geo, density, l1, principal, R, config, iso):
+    """Main.jl:433-466, Postprocessing.export_iteration_results (:264-312):
+    the viewer/checkpoint file of this iteration."""
+    os.makedirs(results_dir, exist_ok=True)
+    fn = os.path.join(results_dir, f"iter_{it}_{base}_webdata.bin")
+    postprocessing.export_binary_for_web(fn, nodes, elements, density, l1, principal, geo, np.float32(iso), it,
+                                         np.float32(R), config)
+    return fn
+
+
+def _restart(config, n_elements):
+    """Main.jl:198-227, 259-261: (density, start_iter, radius, threshold) of
+    an enabled restart whose file exists, else None; the configuration stored
+    in the file is merged under the current one."""
+    rc = config.get("restart_configuration", {}) or {}
+    path = rc.get("file_path", "")
+    if not rc.get("enable_restart", False):
+        return None
+    if not os.path.isfile(path):
+        print(f"!!! WARNING: Restart requested but file not found: '{path}'\n!!! Falling back to FRESH START.")
+        return None
+    saved, rho, it, R, thr = postprocessing.load_checkpoint(path)
+    merged = dict(saved)
+    merged.update(config)
+    config.clear()
+    config.update(merged)
+    return postprocessing.restore_density(rho, n_elements), it + 1, R, thr
+
+
+def run_stress_ratio(config, n_iter=None, log_path=None, verbose=True, keep_density=False, results_dir=None,
+                     base_name="run"):
+    """Main.jl:331-474 without refinement.  keep_density: each iteration's
+    record also holds a copy of the density after its update.  results_dir:
+    write the viewer/checkpoint files at output_settings.save_bin_frequency
+    (export_frequency, default 5; iteration 0 at the start).  A restart
+    (restart_configuration.enable_restart / file_path) continues from the
+    file's iteration + 1 with its density, radius and threshold."""
     geo, nodes, elements, bc, F = build_problem(config)
+    restart = _restart(config, elements.shape[0])
     E = np.float32(config["material"]["E"])
     nu = np.float32(config["material"]["nu"])
     opt = config["optimization_parameters"]
     min_density = np.float32(opt.get("min_density", 1e-3))
     max_density = np.float32(opt.get("density_clamp_max", 1.0))
     density, original, protected, _ = configuration.initialize_density_field(nodes, elements, geo["shapes"], config)
+    start, R, thr = 1, 0.0, 0.0
+    if restart is not None:
+        density, start, R, thr = restart
+        density = density.copy()
     allow = np.float32(config.get("l1_stress_allowable", 1.0)) or np.float32(1.0)
     nominal = int(config.get("number_of_iterations", 30))
     n_iter = n_iter or nominal
+    out = config.get("output_settings", {}) or {}
+    bin_freq = int(out.get("save_bin_frequency", out.get("export_frequency", 5)))
+    iso = float(out.get("iso_surface_threshold", 0.8))
     U = np.zeros(3 * nodes.shape[0])
     dims = f"{geo['nElem_x']}x{geo['nElem_y']}x{geo['nElem_z']}"
     hist = []
-    if log_path:
+    if log_path and restart is None:
         init_log_file(log_path, config)
-    for it in range(1, n_iter + 1):
+    for it in range(start, n_iter + 1):
         t0 = time.perf_counter()
         config["current_outer_iter"] = it
         U, res, prec = solve_system(nodes, elements, E, nu, bc, F, density=density, config=config,
                                     min_stiffness_threshold=min_density, prune_voids=True, u_prev=U)
         comp = float(F @ U)
-        _, _, _, l1, _ = stress.compute_stress_field(nodes, elements, U, E, nu, density)
+        principal, _, _, l1, _ = stress.compute_stress_field(nodes, elements, U, E, nu, density)
         l1 = l1.astype(np.float32)
+        if results_dir and it == 1 and bin_freq > 0:  # the initial state as iteration 0 (Main.jl:433-439)
+            _export(results_dir, base_name, 0, nodes, elements, geo, density, l1, principal, R, config, iso)
         act = density > np.float32(0.1)
         avg_l1 = float(l1[act].mean()) if act.any() else 0.0
         vf = float(density.astype(np.float64).sum() / density.size)
@@ -124,6 +169,9 @@ def run_stress_ratio(config, n_iter=None, log_path=None, verbose=True, keep_dens
         if log_path:
             with open(log_path, "a") as fh:
                 fh.write(line + "\n")
+        if results_dir and bin_freq > 0 and it % bin_freq == 0:
+            rec["bin"] = _export(results_dir, base_name, it, nodes, elements, geo, density, l1, principal, R, config,
+                                 iso)
     return hist, density, U
 
 

@@ -18,10 +18,21 @@ The JSON block holds the same keys as the reference's, but not the same
 bytes: Julia's Dict iteration order is hash order, so no byte-identical key
 order can be asked of it.
 
-Restart defect kept visible: the file holds only the ``count`` elements at or
-above the threshold, and ``load_checkpoint`` returns that many densities
-(Configuration.jl:55-57); a restart on the full mesh is refused when the two
-differ (Main.jl:259-261), which ``restore_density`` reproduces.
+Restart: the reference's file holds only the ``count`` elements at or above
+the threshold and its reader returns that many densities
+(Configuration.jl:55-57), so a restart on the full mesh is refused unless
+every element was kept (Main.jl:259-261).  Files written here carry, after
+the JSON block, a restart section the viewer and the reference's reader never
+reach (both stop at the JSON block):
+
+    u32 magic 0x53525848 ("HXRS")   u32 version 1   u32 nx   u32 ny   u32 nz
+    f32 density[nx*ny*nz]           (the whole field, element order)
+
+``load_checkpoint`` returns that whole field when the section is there, and
+the ``count`` densities of a file without it (``restore_density`` then keeps
+the reference's refusal); ``full_density_from_kept`` rebuilds a whole field
+from a file without it, placing each kept element by its centroid and giving
+the others a fill value (the culled values are not in such a file).
 """
 from __future__ import annotations
 
@@ -38,6 +49,8 @@ from .solver import get_context
 MAGIC = 0x48455841
 VERSION = 2
 _HEAD = struct.Struct("<IIiffIfff")
+RS_MAGIC = 0x53525848
+_RS_HEAD = struct.Struct("<IIIII")
 
 
 def _f32(v):
@@ -64,8 +77,10 @@ def _meta(config, iteration, radius, threshold):
     return meta
 
 
-def write_bin(filename, iteration, radius, threshold, dx, dy, dz, centroids, density, signed_l1, meta):
-    """Write the byte layout of Postprocessing.jl:236-258."""
+def write_bin(filename, iteration, radius, threshold, dx, dy, dz, centroids, density, signed_l1, meta,
+              full_density=None, grid=None):
+    """Write the byte layout of Postprocessing.jl:236-258, then (with
+    full_density and grid = (nx, ny, nz)) the restart section."""
     count = int(len(density))
     js = json.dumps(meta, separators=(",", ":")).encode("utf-8")
     with open(filename, "wb") as fh:
@@ -76,6 +91,13 @@ def write_bin(filename, iteration, radius, threshold, dx, dy, dz, centroids, den
         fh.write(np.ascontiguousarray(signed_l1, dtype="<f4").tobytes())
         fh.write(struct.pack("<I", len(js)))
         fh.write(js)
+        if full_density is not None:
+            nx, ny, nz = (int(v) for v in grid)
+            fd = np.ascontiguousarray(full_density, dtype="<f4").ravel()
+            if fd.size != nx * ny * nz:
+                raise ValueError("restart section: the density must cover the grid")
+            fh.write(_RS_HEAD.pack(RS_MAGIC, 1, nx, ny, nz))
+            fh.write(fd.tobytes())
 
 
 def export_binary_for_web(filename, nodes, elements, density, l1_stress, principal_field, geom, threshold,
@@ -91,8 +113,8 @@ def export_binary_for_web(filename, nodes, elements, density, l1_stress, princip
     dx, dy, dz = float(gd("dx")), float(gd("dy")), float(gd("dz"))
     if ctx is None:
         ctx = get_context()
-    if ctx.grid is None or tuple(ctx.grid[:6]) != tuple(grid):  # the packing does not depend on nu
-        ctx.set_grid(*grid, 0.3)
+    if ctx.grid is None or tuple(ctx.grid[:6]) != tuple(grid):  # the packing does not depend on nu: keep it
+        ctx.set_grid(*grid, ctx.grid[6] if ctx.grid else 0.3)
     pr = np.asarray(principal_field, dtype=np.float32)
     if pr.shape == (3, ctx.ne):
         pr = pr.T
@@ -100,7 +122,7 @@ def export_binary_for_web(filename, nodes, elements, density, l1_stress, princip
     if dens.size == 0:
         return 0
     write_bin(filename, iteration, current_radius, threshold, dx, dy, dz, cent, dens, sl1,
-              _meta(config, iteration, current_radius, threshold))
+              _meta(config, iteration, current_radius, threshold), full_density=density, grid=grid[:3])
     return int(dens.size)
 
 
@@ -128,17 +150,40 @@ def read_bin(filename):
     if len(buf) < off + jlen:
         raise ValueError("truncated checkpoint: JSON block cut short")
     cfg = json.loads(buf[off:off + jlen].decode("utf-8"))
-    return {"version": version, "iteration": it, "radius": radius, "threshold": thr, "dx": dx, "dy": dy, "dz": dz,
-            "centroids": cent, "density": dens, "signed_l1": sl1, "config": cfg}
+    off += jlen
+    out = {"version": version, "iteration": it, "radius": radius, "threshold": thr, "dx": dx, "dy": dy, "dz": dz,
+           "centroids": cent, "density": dens, "signed_l1": sl1, "config": cfg, "full_density": None, "grid": None}
+    if len(buf) >= off + _RS_HEAD.size:
+        rs, rv, nx, ny, nz = _RS_HEAD.unpack_from(buf, off)
+        n = nx * ny * nz
+        if rs == RS_MAGIC and rv == 1 and len(buf) >= off + _RS_HEAD.size + 4 * n:
+            out["full_density"] = np.frombuffer(buf, "<f4", n, off + _RS_HEAD.size).copy()
+            out["grid"] = (nx, ny, nz)
+    return out
+
+
+def full_density_from_kept(d, nx, ny, nz, fill):
+    """A whole density field from a file without the restart section: each
+    kept element at the cell its centroid falls in (i = floor(x / dx), ...),
+    every other element `fill`."""
+    rho = np.full(nx * ny * nz, np.float32(fill), dtype=np.float32)
+    c = d["centroids"].astype(np.float64)
+    i = np.clip(np.floor(c[:, 0] / d["dx"]).astype(int), 0, nx - 1)
+    j = np.clip(np.floor(c[:, 1] / d["dy"]).astype(int), 0, ny - 1)
+    k = np.clip(np.floor(c[:, 2] / d["dz"]).astype(int), 0, nz - 1)
+    rho[i + nx * (j + ny * k)] = d["density"]
+    return rho
 
 
 def load_checkpoint(filename):
     """Configuration.load_checkpoint (Configuration.jl:31-77):
-    (config, density, restart_iter, restart_radius, restart_threshold)."""
+    (config, density, restart_iter, restart_radius, restart_threshold); the
+    density is the whole field when the file has the restart section."""
     if not os.path.isfile(filename):
         raise FileNotFoundError("Checkpoint file not found.")
     d = read_bin(filename)
-    return d["config"], d["density"], int(d["iteration"]), float(d["radius"]), float(d["threshold"])
+    rho = d["full_density"] if d["full_density"] is not None else d["density"]
+    return d["config"], rho, int(d["iteration"]), float(d["radius"]), float(d["threshold"])
 
 
 def restore_density(density, n_elements):

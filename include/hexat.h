@@ -223,6 +223,24 @@ int hx_web_pack(hx_ctx* ctx, const float* density, const float* l1, const float*
 int hx_web_pack_device(hx_ctx* ctx, const float* density, const float* l1, const float* principal,
                        float threshold, float* centroids, float* density_out, float* signed_l1,
                        uint32_t* count);
+/* The data blocks of the legacy binary VTK file of ExportVTK.export_solution
+ * (ExportVTK.jl:20-172), every word an FP32 or Int32 in big-endian byte order
+ * as the file stores it; the caller writes the ASCII section headers between
+ * them.  Inputs: u, f (ndof, nullable: zeros), fixed (3 nn bytes, non-zero =
+ * fixed; nullable: none), density, l1, vm (Ne FP32; l1, vm nullable: zeros),
+ * threshold.  Outputs:
+ *   points: 13 nn words -- coordinates (3 nn), displacement (3 nn), external
+ *     force (3 nn), boundary-condition indicator (3 nn), displacement
+ *     magnitude (nn); u, f and l1 pass through the reference's
+ *     sanitize_data (non-finite -> 0, clamp to +-1e10);
+ *   cells: 9 words per kept element (density >= threshold, element order):
+ *     8 and its 0-based node ids in the reference's node order;
+ *   cell_scalars: l1, von Mises, density of the kept elements, count words each;
+ *   *count: the number kept.  cells and cell_scalars sized for the worst
+ *   case (9 Ne and 3 Ne words). */
+int hx_vtk_pack(hx_ctx* ctx, const double* u, const double* f, const uint8_t* fixed, const float* density,
+                const float* l1, const float* vm, float threshold, uint32_t* points, uint32_t* cells,
+                uint32_t* cell_scalars, uint32_t* count);
 /* ce_e = u_e^T Ke(E=1) u_e and SIMP sensitivity
  * dc_e = -penal xphys^(penal-1) (E0-Emin) ce_e; compliance = f^T u.
  * No reference counterpart (the reference has no sensitivities). */

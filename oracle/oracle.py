@@ -841,3 +841,59 @@ def web_pack(nx, ny, nz, dx, dy, dz, density, l1, principal, threshold):
     sg = np.where(t, np.sign(s3), sg).astype(f32)
     sl1 = (np.asarray(l1, dtype=f32).ravel()[keep] * sg).astype(f32)
     return cent, rho[keep].copy(), sl1
+
+
+# ----------------------------------------------------------------------------
+# legacy VTK file (ExportVTK.export_solution, ExportVTK.jl:20-172), restated on
+# explicit node and element arrays
+# ----------------------------------------------------------------------------
+
+def _sanitize32(a):
+    """ExportVTK.jl:35-42 on Float32 data"""
+    a = np.asarray(a, dtype=np.float32).copy()
+    a[~np.isfinite(a)] = np.float32(0.0)
+    if a.size and np.abs(a).max() > np.float32(1.0e10):
+        a = np.clip(a, np.float32(-1.0e10), np.float32(1.0e10))
+    return a
+
+
+def vtk_bytes(nodes, elements, U, F, bc_indicator, vonmises, l1, density=None, threshold=0.01):
+    """The bytes ExportVTK.export_solution writes.  nodes: (nNodes, 3) Float32,
+    elements: (nElem, 8) 1-based node ids, U and F: 3 nNodes, bc_indicator:
+    (nNodes, 3); every value is taken as Float32 as the reference holds it."""
+    U = _sanitize32(U)
+    Fs = _sanitize32(F)
+    bc = _sanitize32(bc_indicator)
+    nodes = _sanitize32(nodes)
+    l1 = _sanitize32(l1)
+    nN, nE = nodes.shape[0], elements.shape[0]
+    if density is not None:
+        valid = [e for e in range(nE) if density[e] >= np.float32(threshold)]
+    else:
+        valid = list(range(nE))
+    n = len(valid)
+    disp = U.reshape(nN, 3)
+    forces = Fs.reshape(nN, 3)
+    bcs = bc.reshape(nN, 3)
+    mag = np.sqrt((disp[:, 0] * disp[:, 0] + disp[:, 1] * disp[:, 1]) + disp[:, 2] * disp[:, 2]).astype(np.float32)
+    cells = np.empty((n, 9), dtype=np.int32)
+    for t, e in enumerate(valid):
+        cells[t, 0] = 8
+        cells[t, 1:] = np.asarray(elements[e], dtype=np.int64) - 1
+    be32 = lambda a: np.asarray(a, dtype=np.float32).astype(">f4").tobytes()
+    bei = lambda a: np.asarray(a, dtype=np.int32).astype(">i4").tobytes()
+    out = bytearray()
+    out += b"# vtk DataFile Version 3.0\nHEXA FEM Solution (Undeformed)\nBINARY\nDATASET UNSTRUCTURED_GRID\n"
+    out += f"POINTS {nN} float\n".encode() + be32(nodes.reshape(-1))
+    out += f"\nCELLS {n} {n * 9}\n".encode() + bei(cells.reshape(-1))
+    out += f"\nCELL_TYPES {n}\n".encode() + bei(np.full(n, 12))
+    out += f"\nPOINT_DATA {nN}\n".encode() + b"VECTORS Displacement float\n" + be32(disp.reshape(-1))
+    out += b"\nVECTORS External_Forces float\n" + be32(forces.reshape(-1))
+    out += b"\nVECTORS Boundary_Conditions float\n" + be32(bcs.reshape(-1))
+    out += b"\nSCALARS Displacement_Magnitude float 1\nLOOKUP_TABLE default\n" + be32(mag)
+    out += f"\nCELL_DATA {n}\n".encode() + b"SCALARS l1_stress_norm float 1\nLOOKUP_TABLE default\n" + be32(l1[valid])
+    out += b"\nSCALARS Von_Mises float 1\nLOOKUP_TABLE default\n" + be32(np.asarray(vonmises, np.float32)[valid])
+    if density is not None:
+        out += b"\nSCALARS Element_Density float 1\nLOOKUP_TABLE default\n" + be32(
+            np.asarray(density, np.float32)[valid])
+    return bytes(out)

@@ -119,3 +119,102 @@ def test_meta_aliases_and_action():
     assert m["loads"] == cfg["external_forces"] and m["bcs"] == cfg["boundary_conditions"]
     assert m["settings"] == {"n": 3} and m["threshold"] == 0.8 and m["iteration"] == 4
     assert "action" not in cfg["geometry"]["s1"]  # the caller's configuration is not modified
+
+
+def test_restart_section_round_trip(tmp_path, ox):
+    """The whole density field after the JSON block: the bytes the viewer and
+    the reference's reader consume are unchanged, the restart reader gets the
+    full field, so a restart on the full mesh passes Main.jl:259-261."""
+    nx, ny, nz = 6, 4, 3
+    ne = nx * ny * nz
+    rng = np.random.default_rng(2)
+    full = rng.uniform(0, 1, ne).astype(np.float32)
+    cent, dens, sl1 = _pack(ox, nx, ny, nz, 0.5, 1.0, 2.0, full, thr=0.6)
+    meta = {"iteration": 3}
+    fn = tmp_path / "r.bin"
+    pp.write_bin(fn, 3, 1.1, 0.6, 0.5, 1.0, 2.0, cent, dens, sl1, meta, full_density=full, grid=(nx, ny, nz))
+    js = b'{"iteration":3}'
+    prefix = _hand_file(dens.size, cent, dens, sl1, js, it=3, radius=1.1, thr=0.6, d=(0.5, 1.0, 2.0))
+    raw = fn.read_bytes()
+    assert raw[:len(prefix)] == prefix
+    assert struct.unpack_from("<IIIII", raw, len(prefix)) == (0x53525848, 1, nx, ny, nz)
+    assert len(raw) == len(prefix) + 20 + 4 * ne
+    cfg, rho, it, r, thr = pp.load_checkpoint(fn)
+    assert it == 3 and rho.size == ne and np.array_equal(rho, full)
+    assert np.array_equal(pp.restore_density(rho, ne), full)
+    # a file without the section (the reference's): the kept densities only, and the rebuild by centroid
+    fr = tmp_path / "ref.bin"
+    pp.write_bin(fr, 3, 1.1, 0.6, 0.5, 1.0, 2.0, cent, dens, sl1, meta)
+    _, rho_k, *_ = pp.load_checkpoint(fr)
+    assert rho_k.size == dens.size < ne
+    d = pp.read_bin(fr)
+    assert d["full_density"] is None
+    rebuilt = pp.full_density_from_kept(d, nx, ny, nz, 1e-3)
+    kept = full >= np.float32(0.6)
+    assert np.array_equal(rebuilt[kept], full[kept]) and np.all(rebuilt[~kept] == np.float32(1e-3))
+
+
+def test_oracle_vtk_writer_structure(ox):
+    """The oracle's restatement of the reference's VTK writer, read back by an
+    independent parser of the legacy format: section counts, big-endian
+    values, 0-based connectivity of the kept cells, and the sanitising rules
+    (non-finite -> 0; clamp to 1e10 only when something exceeds it)."""
+    mesh = importlib.import_module("2025_12_09_hexat_v069_amd.mesh")
+    nodes, elements, _ = mesh.generate_mesh(3, 2, 2, np.float32(0.5), np.float32(1), np.float32(2))
+    nn, ne = nodes.shape[0], elements.shape[0]
+    U = np.arange(3 * nn, dtype=np.float64) * 1e-3
+    U[4] = np.nan
+    F = np.zeros(3 * nn)
+    F[7] = 5e10
+    bc = np.zeros((nn, 3))
+    bc[0] = 1
+    rho = np.float32([0.1, 0.6, 0.5, 0.9, 0.2, 0.7, 0.5, 0.49, 1.0, 0.0, 0.55, 0.3])
+    l1 = np.arange(ne, dtype=np.float32)
+    l1[1] = np.inf
+    vm = np.arange(ne, dtype=np.float32) * 2
+    raw = ox.vtk_bytes(nodes, elements, U, F, bc, vm, l1, rho, 0.5)
+    pos = 0
+
+    def line():
+        nonlocal pos
+        e = raw.index(b"\n", pos)
+        s = raw[pos:e].decode()
+        pos = e + 1
+        return s
+
+    def block(n, dt):
+        nonlocal pos
+        a = np.frombuffer(raw, dt, n, pos)
+        pos += 4 * n
+        return a
+
+    assert [line() for _ in range(4)] == ["# vtk DataFile Version 3.0", "HEXA FEM Solution (Undeformed)", "BINARY",
+                                          "DATASET UNSTRUCTURED_GRID"]
+    assert line() == f"POINTS {nn} float"
+    assert np.array_equal(block(3 * nn, ">f4").reshape(nn, 3), nodes)
+    kept = np.nonzero(rho >= 0.5)[0]
+    assert line() == "" and line() == f"CELLS {kept.size} {9 * kept.size}"
+    cells = block(9 * kept.size, ">i4").reshape(-1, 9)
+    assert np.all(cells[:, 0] == 8) and np.array_equal(cells[:, 1:], elements[kept] - 1)
+    assert line() == "" and line() == f"CELL_TYPES {kept.size}"
+    assert np.all(block(kept.size, ">i4") == 12)
+    assert line() == "" and line() == f"POINT_DATA {nn}" and line() == "VECTORS Displacement float"
+    d = block(3 * nn, ">f4")
+    assert d[4] == 0 and d[5] == np.float32(5e-3)
+    assert line() == "" and line() == "VECTORS External_Forces float"
+    fr = block(3 * nn, ">f4")
+    assert fr[7] == np.float32(1e10)
+    assert line() == "" and line() == "VECTORS Boundary_Conditions float"
+    assert np.array_equal(block(3 * nn, ">f4").reshape(nn, 3), bc.astype(np.float32))
+    assert line() == "" and line() == "SCALARS Displacement_Magnitude float 1" and line() == "LOOKUP_TABLE default"
+    mag = block(nn, ">f4")
+    dd = d.reshape(nn, 3).astype(np.float64)
+    assert np.allclose(mag, np.sqrt((dd ** 2).sum(1)), rtol=1e-6)
+    assert line() == "" and line() == f"CELL_DATA {kept.size}" and line() == "SCALARS l1_stress_norm float 1"
+    assert line() == "LOOKUP_TABLE default"
+    l1k = block(kept.size, ">f4")
+    assert np.array_equal(l1k, np.where(np.isfinite(l1[kept]), l1[kept], 0))
+    assert line() == "" and line() == "SCALARS Von_Mises float 1" and line() == "LOOKUP_TABLE default"
+    assert np.array_equal(block(kept.size, ">f4"), vm[kept])
+    assert line() == "" and line() == "SCALARS Element_Density float 1" and line() == "LOOKUP_TABLE default"
+    assert np.array_equal(block(kept.size, ">f4"), rho[kept]) and pos == len(raw)
