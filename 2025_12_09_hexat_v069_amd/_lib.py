@@ -47,6 +47,7 @@ EXPORTS = [
     "hx_to_init", "hx_to_step", "hx_to_read", "hx_learned_train", "hx_learned_weights", "hx_timing_enable",
     "hx_timing_read", "hx_device_info", "hx_autotune", "hx_tune_set", "hx_time_operator", "hx_time_operator_pc",
     "hx_time_smoothing", "hx_device_memory", "hx_web_pack_device", "hx_vtk_pack",
+    "hx_stl_pack",
 ]
 #: test hooks declared in include/hexat_testing.h
 TEST_EXPORTS = ["hx_test_unet_nparam", "hx_test_unet_init", "hx_test_unet_forward", "hx_test_unet_backward",
@@ -160,6 +161,7 @@ def load():
         L.hx_compliance_sens.argtypes = [vp, pd, d, d, d, pd, pd, pd, pd, pd]
         L.hx_web_pack_device.argtypes = [vp, vp, vp, vp, C.c_float, vp, vp, vp, vp]
         pu32 = P(C.c_uint32)
+        L.hx_stl_pack.argtypes = [vp, P(C.c_float), i, C.c_float, P(C.c_float), C.c_long, P(C.c_long), P(C.c_float)]
         L.hx_vtk_pack.argtypes = [vp, pd, pd, P(C.c_uint8), P(C.c_float), P(C.c_float), P(C.c_float), C.c_float, pu32,
                                   pu32, pu32, pu32]
         L.hx_oc_update.argtypes = [vp, pd, pd, pd, d, d, d, d, pd, pd, P(C.c_int), pd]
@@ -412,6 +414,23 @@ class Context:
         k = int(n.value)
         return {"points": pts, "cells": cells[:9 * k].copy(), "l1": cs[:k].copy(), "vm": cs[k:2 * k].copy(),
                 "density": cs[2 * k:3 * k].copy(), "count": k}
+
+    def stl_pack(self, density, threshold, subdivision=1, field=False):
+        """Isosurface triangles (count x 12 FP32: normal, v1, v2, v3) of
+        hx_stl_pack, and the sampling grid when field is True."""
+        rho = np.ascontiguousarray(density, dtype=np.float32).reshape(self.ne)
+        nx, ny, nz = self.grid[:3]
+        s = int(subdivision)
+        fld = np.empty((s * nz + 3, s * ny + 3, s * nx + 3), np.float32) if field else None
+        n = C.c_long(0)
+        pf = lambda a: None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+        self.check(self.L.hx_stl_pack(self.h, pf(rho), s, C.c_float(np.float32(threshold)), None, 0, C.byref(n),
+                                      pf(fld)), "hx_stl_pack")
+        tris = np.empty((n.value, 12), np.float32)
+        if n.value:
+            self.check(self.L.hx_stl_pack(self.h, pf(rho), s, C.c_float(np.float32(threshold)), pf(tris), n.value,
+                                          C.byref(n), None), "hx_stl_pack")
+        return (tris, fld) if field else tris
 
     def compliance_sens(self, u, xphys=None, penal=3.0, E0=1.0, Emin=1e-9, f=None):
         u = f64(u, self.ndof)

@@ -149,6 +149,10 @@ struct hx_ctx {
     DevBuf<float> v_l1, v_vm;
     DevBuf<double> v_u, v_f;
     DevBuf<uint8_t> v_fixed;
+    DevBuf<float> s_nod, s_fine, s_tris;  // hx_stl_pack
+    DevBuf<unsigned> s_blk, s_offs;
+    DevBuf<int8_t> s_tri;
+    DevBuf<uint8_t> s_ntri;
     // SIMP session
     bool to_ready = false;
     hx_to_params tp{};
@@ -1694,6 +1698,49 @@ int hx_vtk_pack(hx_ctx* c, const double* u, const double* f, const uint8_t* fixe
         for (int q = 0; q < 3; ++q)
             if ((rc = download(c, cell_scalars + (long)q * kept, Cw + (9 + q) * ne, (long)kept * sizeof(unsigned))))
                 return rc;
+    }
+    return HX_OK;
+}
+
+int hx_stl_pack(hx_ctx* c, const float* density, int subdivision, float threshold, float* triangles,
+                long max_triangles, long* count, float* field) {
+    if (!c || !density || !count || subdivision < 1 || max_triangles < 0 || (max_triangles > 0 && !triangles))
+        return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    const GridArgs& g = c->g;
+    const long ne = g.ne, nn = g.nn;
+    const int FX = subdivision * g.nx + 3, FY = subdivision * g.ny + 3, FZ = subdivision * g.nz + 3;
+    const long np = (long)FX * FY * FZ, nb = mc_blocks(FX, FY, FZ);
+    if (nb > 64L * 1024 * 1024) return fail(c, HX_ERR_BAD_ARG, "hx_stl_pack: sampling grid too large");
+    HXA(c->w_rho, ne);
+    HXA(c->s_nod, nn);
+    HXA(c->s_fine, np);
+    HXA(c->s_blk, nb);
+    HXA(c->s_offs, nb + 1);
+    int rc;
+    if (!c->s_tri.p) {
+        HXA(c->s_tri, 256 * 16);
+        HXA(c->s_ntri, 256);
+        if ((rc = upload(c, c->s_tri.p, mc_table_host(0), 256 * 16)) || (rc = upload(c, c->s_ntri.p, mc_table_host(1), 256)))
+            return rc;
+    }
+    if ((rc = upload(c, c->w_rho.p, density, ne * sizeof(float)))) return rc;
+    launch_stl_field(g, subdivision, c->w_rho.p, c->s_nod.p, c->s_fine.p, c->stream);
+    launch_mc_count(c->s_fine.p, FX, FY, FZ, threshold, c->s_ntri.p, c->s_blk.p, c->s_offs.p, c->stream);
+    HXK();
+    unsigned total = 0;
+    if ((rc = download(c, &total, c->s_offs.p + nb, sizeof(unsigned)))) return rc;
+    *count = total;
+    if (field && (rc = download(c, field, c->s_fine.p, np * sizeof(float)))) return rc;
+    const long nw = std::min<long>(total, max_triangles);
+    if (nw > 0) {
+        HXA(c->s_tris, 12 * nw);
+        const float sx = (float)g.dx / (float)subdivision, sy = (float)g.dy / (float)subdivision,
+                    sz = (float)g.dz / (float)subdivision;
+        launch_mc_write(c->s_fine.p, FX, FY, FZ, threshold, -sx, -sy, -sz, sx, sy, sz, c->s_tri.p, c->s_ntri.p,
+                        c->s_offs.p, (unsigned)nw, c->s_tris.p, c->stream);
+        HXK();
+        if ((rc = download(c, triangles, c->s_tris.p, 12 * nw * sizeof(float)))) return rc;
     }
     return HX_OK;
 }

@@ -100,6 +100,20 @@ void launch_vtk_pack(const GridArgs& g, const double* u, const double* f, const 
                      const float* l1, const float* vm, float thr, unsigned* tile_count, unsigned* offs,
                      unsigned* total, unsigned* pts, unsigned* disp, unsigned* frc, unsigned* bcs, unsigned* dmag,
                      unsigned* cells, unsigned* l1_out, unsigned* vm_out, unsigned* dens_out, hipStream_t s);
+// ---- STL isosurface (hx_stl.hip) ----------------------------------------------
+// the marching-cubes case table (part 0: int8 [256][16] edge ids, part 1:
+// uint8 [256] triangle counts), generated once per process
+const void* mc_table_host(int part);
+// nodal mean density (nn) and the sampling grid of (sub nx + 3)(sub ny + 3)(sub nz + 3) points
+void launch_stl_field(const GridArgs& g, int sub, const float* rho, float* nod, float* fine, hipStream_t s);
+long mc_blocks(int FX, int FY, int FZ);
+// per-block triangle counts and their exclusive scan (offs: mc_blocks + 1, the last = total)
+void launch_mc_count(const float* fine, int FX, int FY, int FZ, float iso, const uint8_t* ntri, unsigned* blk,
+                     unsigned* offs, hipStream_t s);
+// triangles (12 floats each: unit normal, 3 vertices), at most cap
+void launch_mc_write(const float* fine, int FX, int FY, int FZ, float iso, float x0, float y0, float z0, float hx,
+                     float hy, float hz, const int8_t* tri, const uint8_t* ntri, const unsigned* offs, unsigned cap,
+                     float* out, hipStream_t s);
 // the packing's scratch for hx_web_pack_device: kWebScratchTiles tile counts
 // then kWebScratchTiles + 1 offsets (at most kWebScratchTiles * 2048 elements)
 constexpr int kWebScratchTiles = 65536;

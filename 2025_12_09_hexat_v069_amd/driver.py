@@ -5,10 +5,11 @@ iteration log of Diagnostics.jl:11-83).
 
 ``optimization_parameters.method: stress_ratio`` (default) runs the
 reference's loop: solve -> compliance f'u -> stress field -> stress-ratio
-update.  ``simp_oc`` runs the device-resident SIMP/OC loop.  Mesh refinement,
-gravity/thermal loads, checkpoints and VTK/STL export are outside this
-build's hot path and are not performed (a config that asks for gravity or
-thermal load is rejected rather than silently ignored).
+update, with the viewer/checkpoint file, the VTK solution and the isosurface
+STL at output_settings' frequencies and restarts from a checkpoint.
+``simp_oc`` runs the device-resident SIMP/OC loop.  Mesh refinement and
+gravity/thermal loads are outside this build's hot path (a config that asks
+for gravity or thermal load is rejected rather than silently ignored).
 """
 from __future__ import annotations
 
@@ -19,7 +20,7 @@ import time
 
 import numpy as np
 
-from . import boundary, configuration, mesh, postprocessing, stress, topopt
+from . import boundary, configuration, export_stl, export_vtk, mesh, postprocessing, stress, topopt
 from .solver import solve_system
 
 LOG_HEADER = ("| Iter | Mesh Size | Total El | Active El | Radius | Cutoff | Compliance | Strain Energy | "
@@ -78,14 +79,26 @@ def build_problem(config):
     return geo, nodes, elements, bc, F
 
 
-def _export(results_dir, base, it, nodes, elements, geo, density, l1, principal, R, config, iso):
+def _export(results_dir, base, it, nodes, elements, geo, density, l1, principal, R, config, iso, what, U=None, F=None,
+            bc=None, vm=None):
     """Main.jl:433-466, Postprocessing.export_iteration_results (:264-312):
-    the viewer/checkpoint file of this iteration."""
+    the viewer/checkpoint file, the VTK solution and the isosurface STL of
+    this iteration, as `what` (a set of "bin", "vtk", "stl") asks."""
     os.makedirs(results_dir, exist_ok=True)
-    fn = os.path.join(results_dir, f"iter_{it}_{base}_webdata.bin")
-    postprocessing.export_binary_for_web(fn, nodes, elements, density, l1, principal, geo, np.float32(iso), it,
-                                         np.float32(R), config)
-    return fn
+    pre = os.path.join(results_dir, f"iter_{it}_{base}")
+    out = {}
+    if "bin" in what:
+        out["bin"] = pre + "_webdata.bin"
+        postprocessing.export_binary_for_web(out["bin"], nodes, elements, density, l1, principal, geo,
+                                             np.float32(iso), it, np.float32(R), config)
+    if "vtk" in what:
+        out["vtk"] = export_vtk.export_solution(nodes, elements, U, F, bc, principal, vm, None, l1, None,
+                                                density=density, threshold=np.float32(iso),
+                                                filename=pre + "_solution")
+    if "stl" in what:
+        out["stl"] = pre + "_isosurface.stl"
+        export_stl.export_smooth_watertight_stl(density, geo, np.float32(iso), out["stl"], subdivision_level=1)
+    return out
 
 
 def _restart(config, n_elements):
@@ -131,7 +144,9 @@ def run_stress_ratio(config, n_iter=None, log_path=None, verbose=True, keep_dens
     nominal = int(config.get("number_of_iterations", 30))
     n_iter = n_iter or nominal
     out = config.get("output_settings", {}) or {}
-    bin_freq = int(out.get("save_bin_frequency", out.get("export_frequency", 5)))
+    dfreq = int(out.get("export_frequency", 5))  # Main.jl:235-238
+    freq = {"bin": int(out.get("save_bin_frequency", dfreq)), "stl": int(out.get("save_STL_frequency", dfreq)),
+            "vtk": int(out.get("save_VTK_frequency", dfreq))}
     iso = float(out.get("iso_surface_threshold", 0.8))
     U = np.zeros(3 * nodes.shape[0])
     dims = f"{geo['nElem_x']}x{geo['nElem_y']}x{geo['nElem_z']}"
@@ -144,10 +159,13 @@ def run_stress_ratio(config, n_iter=None, log_path=None, verbose=True, keep_dens
         U, res, prec = solve_system(nodes, elements, E, nu, bc, F, density=density, config=config,
                                     min_stiffness_threshold=min_density, prune_voids=True, u_prev=U)
         comp = float(F @ U)
-        principal, _, _, l1, _ = stress.compute_stress_field(nodes, elements, U, E, nu, density)
+        principal, vm, _, l1, _ = stress.compute_stress_field(nodes, elements, U, E, nu, density)
         l1 = l1.astype(np.float32)
-        if results_dir and it == 1 and bin_freq > 0:  # the initial state as iteration 0 (Main.jl:433-439)
-            _export(results_dir, base_name, 0, nodes, elements, geo, density, l1, principal, R, config, iso)
+        if results_dir and it == 1:  # the initial state as iteration 0 (Main.jl:433-439)
+            what = {k for k, v in freq.items() if v > 0}
+            if what:
+                _export(results_dir, base_name, 0, nodes, elements, geo, density, l1, principal, R, config, iso, what,
+                        U, F, bc, vm)
         act = density > np.float32(0.1)
         avg_l1 = float(l1[act].mean()) if act.any() else 0.0
         vf = float(density.astype(np.float64).sum() / density.size)
@@ -169,9 +187,10 @@ def run_stress_ratio(config, n_iter=None, log_path=None, verbose=True, keep_dens
         if log_path:
             with open(log_path, "a") as fh:
                 fh.write(line + "\n")
-        if results_dir and bin_freq > 0 and it % bin_freq == 0:
-            rec["bin"] = _export(results_dir, base_name, it, nodes, elements, geo, density, l1, principal, R, config,
-                                 iso)
+        what = {k for k, v in freq.items() if v > 0 and it % v == 0}
+        if results_dir and what:
+            rec.update(_export(results_dir, base_name, it, nodes, elements, geo, density, l1, principal, R, config,
+                               iso, what, U, F, bc, vm))
     return hist, density, U
 
 

@@ -241,6 +241,18 @@ int hx_web_pack_device(hx_ctx* ctx, const float* density, const float* l1, const
 int hx_vtk_pack(hx_ctx* ctx, const double* u, const double* f, const uint8_t* fixed, const float* density,
                 const float* l1, const float* vm, float threshold, uint32_t* points, uint32_t* cells,
                 uint32_t* cell_scalars, uint32_t* count);
+/* Isosurface of the density for the STL file of
+ * Postprocessing.export_smooth_watertight_stl (Postprocessing.jl:53-148): the
+ * nodal mean density, refined `subdivision` times per element by trilinear
+ * interpolation with a layer of zeros around it, then marching cubes at
+ * `threshold` (a generated, crack-free case table; the reference uses
+ * MarchingCubes.jl, whose choice of triangles differs).  Writes up to
+ * max_triangles triangles of 12 floats (unit normal or zeros, three vertices,
+ * outward from the region >= threshold) and the total in *count; with
+ * max_triangles 0 only the count is formed.  field (nullable): the sampling
+ * grid, (s nx + 3)(s ny + 3)(s nz + 3) FP32 values, x fastest. */
+int hx_stl_pack(hx_ctx* ctx, const float* density, int subdivision, float threshold, float* triangles,
+                long max_triangles, long* count, float* field);
 /* ce_e = u_e^T Ke(E=1) u_e and SIMP sensitivity
  * dc_e = -penal xphys^(penal-1) (E0-Emin) ce_e; compliance = f^T u.
  * No reference counterpart (the reference has no sensitivities). */

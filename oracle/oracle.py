@@ -897,3 +897,73 @@ def vtk_bytes(nodes, elements, U, F, bc_indicator, vonmises, l1, density=None, t
         out += b"\nSCALARS Element_Density float 1\nLOOKUP_TABLE default\n" + be32(
             np.asarray(density, np.float32)[valid])
     return bytes(out)
+
+
+# ----------------------------------------------------------------------------
+# STL sampling grid (Postprocessing.jl:19-92): nodal mean density, trilinear
+# refinement, one layer of zeros; FP32 in the reference's operation order
+# ----------------------------------------------------------------------------
+
+def stl_field(density, nx, ny, nz, sub):
+    rho = np.asarray(density, dtype=np.float32).reshape(nz, ny, nx)
+    NX, NY, NZ = nx + 1, ny + 1, nz + 1
+    sums = np.zeros((NZ, NY, NX), np.float32)
+    cnt = np.zeros((NZ, NY, NX), np.int64)
+    # the reference loops over elements in order (x fastest), adding each
+    # density to its 8 nodes: a node receives its elements (i-1..i, j-1..j,
+    # k-1..k) in that order
+    for dk in (-1, 0):
+        for dj in (-1, 0):
+            for di in (-1, 0):
+                kk, jj, ii = np.meshgrid(np.arange(NZ) + dk, np.arange(NY) + dj, np.arange(NX) + di, indexing="ij")
+                ok = (ii >= 0) & (jj >= 0) & (kk >= 0) & (ii < nx) & (jj < ny) & (kk < nz)
+                v = np.where(ok, rho[np.clip(kk, 0, nz - 1), np.clip(jj, 0, ny - 1), np.clip(ii, 0, nx - 1)],
+                             np.float32(0))
+                sums = np.where(ok, (sums + v).astype(np.float32), sums)
+                cnt += ok
+    nod = np.where(cnt > 0, (sums / cnt.astype(np.float32)).astype(np.float32), np.float32(0))
+    FX, FY, FZ = sub * nx + 3, sub * ny + 3, sub * nz + 3
+    fine = np.zeros((FZ, FY, FX), np.float32)
+    one = np.float32(1)
+    for c in range(1, FZ - 1):
+        iz = c - 1
+        cz = min(iz // sub, nz - 1)
+        zd = np.float32(iz - cz * sub) / np.float32(sub)
+        for b in range(1, FY - 1):
+            iy = b - 1
+            cy = min(iy // sub, ny - 1)
+            yd = np.float32(iy - cy * sub) / np.float32(sub)
+            ix = np.arange(FX - 2)
+            cx = np.minimum(ix // sub, nx - 1)
+            xd = (ix - cx * sub).astype(np.float32) / np.float32(sub)
+            v1, v2 = nod[cz, cy, cx], nod[cz, cy, cx + 1]
+            v3, v4 = nod[cz, cy + 1, cx + 1], nod[cz, cy + 1, cx]
+            v5, v6 = nod[cz + 1, cy, cx], nod[cz + 1, cy, cx + 1]
+            v7, v8 = nod[cz + 1, cy + 1, cx + 1], nod[cz + 1, cy + 1, cx]
+            c00 = v1 * (one - xd) + v2 * xd
+            c01 = v4 * (one - xd) + v3 * xd
+            c10 = v5 * (one - xd) + v6 * xd
+            c11 = v8 * (one - xd) + v7 * xd
+            c0 = c00 * (one - yd) + c01 * yd
+            c1 = c10 * (one - yd) + c11 * yd
+            fine[c, b, 1:FX - 1] = c0 * (one - zd) + c1 * zd
+    return fine
+
+
+def trilinear_volume(fine, iso, h, samples=4):
+    """Volume of {field >= iso} for the trilinear interpolant of a sampling
+    grid (spacing h), by midpoint sampling at `samples` points per cell edge."""
+    FZ, FY, FX = fine.shape
+    t = (np.arange(samples) + 0.5) / samples
+    f = fine.astype(np.float64)
+    tot = 0
+    for tz in t:
+        for ty in t:
+            for tx in t:
+                a = f[:-1, :-1, :-1] * (1 - tx) + f[:-1, :-1, 1:] * tx
+                b = f[:-1, 1:, :-1] * (1 - tx) + f[:-1, 1:, 1:] * tx
+                c = f[1:, :-1, :-1] * (1 - tx) + f[1:, :-1, 1:] * tx
+                d = f[1:, 1:, :-1] * (1 - tx) + f[1:, 1:, 1:] * tx
+                v = (a * (1 - ty) + b * ty) * (1 - tz) + (c * (1 - ty) + d * ty) * tz
+                tot += int((v >= iso).sum())
+    return tot / samples ** 3 * h[0] * h[1] * h[2]

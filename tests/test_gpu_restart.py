@@ -17,7 +17,8 @@ def _cfg(tmp=None):
                                                           "cantilever_4loads.yaml")))
     cfg["geometry"]["target_elem_count"] = 12000
     cfg["solver_parameters"].update(mode="northstar", tolerance=1e-10)
-    cfg["output_settings"] = {"save_bin_frequency": 1, "iso_surface_threshold": 0.3}
+    cfg["output_settings"] = {"save_bin_frequency": 1, "save_VTK_frequency": 2, "save_STL_frequency": 2,
+                              "iso_surface_threshold": 0.3}
     return cfg
 
 
@@ -29,6 +30,9 @@ def test_restart_continues_the_run(hx, tmp_path):
     full, _, _ = drv.run_stress_ratio(conf.apply_hardware_profile(_cfg()), n_iter=3, verbose=False,
                                       keep_density=True, results_dir=str(tmp_path / "a"), base_name="c")
     f2 = full[1]["bin"]
+    import os
+    assert os.path.getsize(full[1]["vtk"]) > 0 and os.path.getsize(full[1]["stl"]) > 84  # iteration 2 of 3
+    assert "vtk" not in full[0] and "stl" not in full[2]
     d = pp.read_bin(f2)
     assert d["iteration"] == 2 and d["full_density"] is not None
     assert np.array_equal(d["full_density"], full[1]["density"])

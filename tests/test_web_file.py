@@ -218,3 +218,48 @@ def test_oracle_vtk_writer_structure(ox):
     assert np.array_equal(block(kept.size, ">f4"), vm[kept])
     assert line() == "" and line() == "SCALARS Element_Density float 1" and line() == "LOOKUP_TABLE default"
     assert np.array_equal(block(kept.size, ">f4"), rho[kept]) and pos == len(raw)
+
+
+def test_oracle_stl_field_against_literal_loops(ox):
+    """The oracle's sampling grid against a literal restatement of the
+    reference's two loops (Postprocessing.jl:19-37 and 74-90) on a small grid,
+    bit for bit, and the closed form of a constant density."""
+    nx, ny, nz, sub = 3, 2, 2, 2
+    rng = np.random.default_rng(9)
+    rho = rng.uniform(0, 1, nx * ny * nz).astype(np.float32)
+    NX, NY, NZ = nx + 1, ny + 1, nz + 1
+    sums = np.zeros(NX * NY * NZ, np.float32)
+    cnt = np.zeros(NX * NY * NZ, int)
+    for e in range(nx * ny * nz):
+        i, j, k = e % nx, (e // nx) % ny, e // (nx * ny)
+        for di, dj, dk in ((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)):
+            n = (i + di) + NX * ((j + dj) + NY * (k + dk))
+            sums[n] = np.float32(sums[n] + rho[e])
+            cnt[n] += 1
+    nod = np.array([np.float32(sums[n] / np.float32(cnt[n])) for n in range(sums.size)], np.float32)
+    g = lambda a, b, c: nod[a + NX * (b + NY * c)]
+    FX, FY, FZ = sub * nx + 3, sub * ny + 3, sub * nz + 3
+    want = np.zeros((FZ, FY, FX), np.float32)
+    one = np.float32(1)
+    for c in range(1, FZ - 1):
+        for b in range(1, FY - 1):
+            for a in range(1, FX - 1):
+                ix, iy, iz = a - 1, b - 1, c - 1
+                cx, cy, cz = min(ix // sub, nx - 1), min(iy // sub, ny - 1), min(iz // sub, nz - 1)
+                xd, yd, zd = (np.float32(ix - cx * sub) / np.float32(sub), np.float32(iy - cy * sub) / np.float32(sub),
+                              np.float32(iz - cz * sub) / np.float32(sub))
+                v = [g(cx, cy, cz), g(cx + 1, cy, cz), g(cx + 1, cy + 1, cz), g(cx, cy + 1, cz),
+                     g(cx, cy, cz + 1), g(cx + 1, cy, cz + 1), g(cx + 1, cy + 1, cz + 1), g(cx, cy + 1, cz + 1)]
+                c00 = v[0] * (one - xd) + v[1] * xd
+                c01 = v[3] * (one - xd) + v[2] * xd
+                c10 = v[4] * (one - xd) + v[5] * xd
+                c11 = v[7] * (one - xd) + v[6] * xd
+                c0 = c00 * (one - yd) + c01 * yd
+                c1 = c10 * (one - yd) + c11 * yd
+                want[c, b, a] = c0 * (one - zd) + c1 * zd
+    assert np.array_equal(ox.stl_field(rho, nx, ny, nz, sub), want)
+    f = ox.stl_field(np.full(24, 0.375, np.float32), 4, 3, 2, 3)
+    assert np.all(f[1:-1, 1:-1, 1:-1] == np.float32(0.375)) and f[0].max() == 0 and f[:, :, -1].max() == 0
+    # the sampled volume of a field linear in x: {f >= 0.3} is 70 % of the box
+    ramp = np.broadcast_to(np.linspace(0, 1, 12)[None, None, :], (5, 6, 12))
+    assert ox.trilinear_volume(ramp, 0.3, (0.5, 2.0, 1.0), samples=8) == pytest.approx(4 * 5 * 11 * 0.7, rel=1e-2)
