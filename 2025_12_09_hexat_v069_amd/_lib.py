@@ -47,7 +47,7 @@ EXPORTS = [
     "hx_to_init", "hx_to_step", "hx_to_read", "hx_learned_train", "hx_learned_weights", "hx_timing_enable",
     "hx_timing_read", "hx_device_info", "hx_autotune", "hx_tune_set", "hx_time_operator", "hx_time_operator_pc",
     "hx_time_smoothing", "hx_device_memory", "hx_web_pack_device", "hx_vtk_pack",
-    "hx_stl_pack",
+    "hx_stl_pack", "hx_bc_indicator", "hx_external_forces", "hx_regrid",
 ]
 #: test hooks declared in include/hexat_testing.h
 TEST_EXPORTS = ["hx_test_unet_nparam", "hx_test_unet_init", "hx_test_unet_forward", "hx_test_unet_backward",
@@ -84,6 +84,10 @@ class SolveStats(C.Structure):
     _fields_ = [("status", C.c_int), ("outcome", C.c_int), ("attempts", C.c_int), ("iterations", C.c_int),
                 ("rel_res", C.c_double), ("true_rel", C.c_double), ("best_rel", C.c_double), ("shift", C.c_double),
                 ("n_active", C.c_long), ("n_kept", C.c_long), ("pruned", C.c_int), ("seconds", C.c_double)]
+
+
+class Location(C.Structure):
+    _fields_ = [("all", C.c_int * 3), ("value", C.c_float * 3)]
 
 
 class FilterOpts(C.Structure):
@@ -161,6 +165,10 @@ def load():
         L.hx_compliance_sens.argtypes = [vp, pd, d, d, d, pd, pd, pd, pd, pd]
         L.hx_web_pack_device.argtypes = [vp, vp, vp, vp, C.c_float, vp, vp, vp, vp]
         pu32 = P(C.c_uint32)
+        L.hx_bc_indicator.argtypes = [vp, i, P(Location), P(C.c_int), P(C.c_uint8)]
+        L.hx_external_forces.argtypes = [vp, i, P(Location), pd, P(C.c_int), pd, P(C.c_uint32)]
+        L.hx_regrid.argtypes = [vp, i, i, i, i, i, i, C.c_float, C.c_float, C.c_float, i, P(C.c_float),
+                                P(C.c_float)]
         L.hx_stl_pack.argtypes = [vp, P(C.c_float), i, C.c_float, P(C.c_float), C.c_long, P(C.c_long), P(C.c_float)]
         L.hx_vtk_pack.argtypes = [vp, pd, pd, P(C.c_uint8), P(C.c_float), P(C.c_float), P(C.c_float), C.c_float, pu32,
                                   pu32, pu32, pu32]
@@ -414,6 +422,39 @@ class Context:
         k = int(n.value)
         return {"points": pts, "cells": cells[:9 * k].copy(), "l1": cs[:k].copy(), "vm": cs[k:2 * k].copy(),
                 "density": cs[2 * k:3 * k].copy(), "count": k}
+
+    def bc_indicator(self, locs, dof_bits):
+        """hx_bc_indicator: locs = [(all3, value3)], dof_bits per spec -> fixed mask (3 nn uint8)."""
+        n = len(locs)
+        L = (Location * max(n, 1))(*[Location((C.c_int * 3)(*a), (C.c_float * 3)(*v)) for a, v in locs])
+        bits = (C.c_int * max(n, 1))(*dof_bits)
+        out = np.empty(self.ndof, np.uint8)
+        self.check(self.L.hx_bc_indicator(self.h, n, L, bits, out.ctypes.data_as(C.POINTER(C.c_uint8))),
+                   "hx_bc_indicator")
+        return out
+
+    def external_forces(self, locs, forces, point_fallback, F):
+        """hx_external_forces: adds into F (ndof float64, returned) -> (F, matched)."""
+        n = len(locs)
+        L = (Location * max(n, 1))(*[Location((C.c_int * 3)(*a), (C.c_float * 3)(*v)) for a, v in locs])
+        fv = np.ascontiguousarray(forces, dtype=np.float64).reshape(max(n, 0) * 3)
+        pf = (C.c_int * max(n, 1))(*[1 if p else 0 for p in point_fallback])
+        F = f64(F, self.ndof).copy()
+        m = np.zeros(max(n, 1), np.uint32)
+        self.check(self.L.hx_external_forces(self.h, n, L, dptr(fv), pf, dptr(F),
+                                             m.ctypes.data_as(C.POINTER(C.c_uint32))), "hx_external_forces")
+        return F, m[:n]
+
+    def regrid(self, old_dims, new_dims, lengths, fields):
+        """hx_regrid: fields (nf x old ne, float32) -> (nf x new ne)."""
+        fo = np.ascontiguousarray(fields, dtype=np.float32)
+        nf = fo.shape[0] if fo.ndim == 2 else 1
+        nne = int(np.prod(new_dims))
+        out = np.empty((nf, nne), np.float32)
+        pf = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        self.check(self.L.hx_regrid(self.h, *[int(v) for v in old_dims], *[int(v) for v in new_dims],
+                                    *[C.c_float(np.float32(v)) for v in lengths], nf, pf(fo), pf(out)), "hx_regrid")
+        return out
 
     def stl_pack(self, density, threshold, subdivision=1, field=False):
         """Isosurface triangles (count x 12 FP32: normal, v1, v2, v3) of

@@ -114,3 +114,67 @@ def apply_external_forces(F, forces_data, nodes, elements=None):
 def fixed_dof_mask(bc_indicator):
     """GPUSolver.jl:112-122 as a per-DOF mask (node-major)."""
     return (np.asarray(bc_indicator) > 0).astype(np.uint8).ravel()
+
+
+# ---------------------------------------------------------------------------
+# the same mapping on the device for a structured grid (hx_bc_indicator,
+# hx_external_forces): the host resolves each location to FP32 values as
+# above, the library matches nodes, splits loads and finds nearest nodes
+# ---------------------------------------------------------------------------
+
+def _location(loc, nodes):
+    lo, hi = nodes.min(axis=0), nodes.max(axis=0)
+    all3, val3 = [], []
+    for d in range(3):
+        spec = loc[d]
+        if isinstance(spec, str) and spec == ":":
+            all3.append(1)
+            val3.append(0.0)
+        elif isinstance(spec, (int, float, np.floating, np.integer)) and not isinstance(spec, bool) or (
+                isinstance(spec, str) and spec.endswith("%")):
+            all3.append(0)
+            val3.append(float(resolve_coordinate_value(spec, lo[d], hi[d])))
+        else:
+            raise ValueError(f"Invalid location component: {spec}")
+    return all3, val3
+
+
+def _all_located(specs):
+    return all("location" in s and "node" not in s and len(s["location"]) >= 3 for s in specs)
+
+
+def get_bc_indicator_device(ctx, nodes, bc_data):
+    """get_bc_indicator on the device (the context's grid is the mesh's);
+    the mask is also installed in the context.  Specs by explicit node ids
+    take the host path."""
+    if not _all_located(bc_data):
+        ind = get_bc_indicator(nodes.shape[0], nodes, bc_data)
+        ctx.set_fixed(fixed_dof_mask(ind))
+        return ind
+    locs, bits = [], []
+    for bc in bc_data:
+        b = 0
+        for d in bc["DoFs"]:
+            if d < 1 or d > 3:
+                raise ValueError(f"Invalid DoF index: {d} (must be 1..3).")
+            b |= 1 << (d - 1)
+        locs.append(_location(bc["location"], nodes))
+        bits.append(b)
+    fixed = ctx.bc_indicator(locs, bits)
+    return fixed.reshape(-1, 3).astype(np.float32)
+
+
+def apply_external_forces_device(ctx, F, forces_data, nodes):
+    """apply_external_forces on the device; returns the new F."""
+    if not _all_located(forces_data):
+        return apply_external_forces(F, forces_data, nodes)
+    locs, fv, point = [], [], []
+    for f in forces_data:
+        locs.append(_location(f["location"], nodes))
+        v = np.zeros(3)
+        raw = list(f["F"])[:3]
+        v[:len(raw)] = raw
+        fv.append(v)
+        point.append(all(not (isinstance(c, str) and c == ":") for c in f["location"]))
+    out, _ = ctx.external_forces(locs, np.array(fv), point, F)
+    return out

@@ -150,6 +150,11 @@ struct hx_ctx {
     DevBuf<double> v_u, v_f;
     DevBuf<uint8_t> v_fixed;
     DevBuf<float> s_nod, s_fine, s_tris;  // hx_stl_pack
+    DevBuf<unsigned> b_part, b_cnt;       // boundary mapping
+    DevBuf<double> b_F;
+    DevBuf<float> b_dpart, b_dmin;
+    DevBuf<unsigned long long> b_kpart, b_key;
+    DevBuf<float> r_old, r_new;           // regrid
     DevBuf<unsigned> s_blk, s_offs;
     DevBuf<int8_t> s_tri;
     DevBuf<uint8_t> s_ntri;
@@ -1743,6 +1748,94 @@ int hx_stl_pack(hx_ctx* c, const float* density, int subdivision, float threshol
         if ((rc = download(c, triangles, c->s_tris.p, 12 * nw * sizeof(float)))) return rc;
     }
     return HX_OK;
+}
+
+namespace {
+// the FP32 location of Boundary.interpret_location_component (Boundary.jl:79-115)
+// on this grid: tolerance 1e-4 * max(1, |cmax - cmin|) with cmin = 0 and cmax
+// the last node's coordinate Float32(n) * h
+LocSpec loc_spec(const hx_ctx* c, const hx_location& l) {
+    LocSpec s{};
+    const int n[3] = {c->g.nx, c->g.ny, c->g.nz};
+    const float h[3] = {(float)c->g.dx, (float)c->g.dy, (float)c->g.dz};
+    for (int d = 0; d < 3; ++d) {
+        s.all[d] = l.all[d] ? 1 : 0;
+        s.val[d] = l.value[d];
+        const float cmax = (float)n[d] * h[d];
+        s.tol[d] = 1.0e-4f * std::max(1.0f, std::fabs(cmax - 0.0f));
+    }
+    return s;
+}
+}  // namespace
+
+int hx_bc_indicator(hx_ctx* c, int nspec, const hx_location* locs, const int* dof_bits, uint8_t* fixed_out) {
+    if (!c || nspec < 0 || (nspec > 0 && (!locs || !dof_bits))) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    const long n = ndof(c);
+    HXC(hipMemsetAsync(c->fixed.p, 0, n, c->stream));
+    for (int q = 0; q < nspec; ++q) {
+        if (dof_bits[q] & ~7) return fail(c, HX_ERR_BAD_ARG, "Invalid DoF index (must be 1..3)");
+        launch_bc_mark(c->g, loc_spec(c, locs[q]), dof_bits[q], c->fixed.p, c->stream);
+    }
+    HXK();
+    c->has_fixed = true;
+    if (fixed_out) return download(c, fixed_out, c->fixed.p, n);
+    HXC(hipStreamSynchronize(c->stream));
+    return HX_OK;
+}
+
+int hx_external_forces(hx_ctx* c, int nspec, const hx_location* locs, const double* force, const int* point_fallback,
+                       double* F, uint32_t* matched) {
+    if (!c || !F || nspec < 0 || (nspec > 0 && (!locs || !force))) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    const long n = ndof(c), nb = boundary_blocks(c->g);
+    HXA(c->b_part, nb);
+    HXA(c->b_cnt, nspec > 0 ? nspec : 1);
+    HXA(c->b_dpart, nb);
+    HXA(c->b_dmin, 1);
+    HXA(c->b_kpart, nb);
+    HXA(c->b_key, 1);
+    HXA(c->b_F, n);
+    int rc;
+    if ((rc = upload(c, c->b_F.p, F, n * sizeof(double)))) return rc;
+    std::vector<unsigned> cnt(nspec > 0 ? nspec : 1, 0);
+    for (int q = 0; q < nspec; ++q) {  // the specs in order, as the reference adds them
+        const LocSpec s = loc_spec(c, locs[q]);
+        launch_loc_count(c->g, s, c->b_part.p, c->b_cnt.p + q, c->stream);
+        HXK();
+        if ((rc = download(c, &cnt[q], c->b_cnt.p + q, sizeof(unsigned)))) return rc;
+        if (cnt[q] > 0) {
+            launch_force_add(c->g, s, c->b_cnt.p + q, force + 3 * q, c->b_F.p, c->stream);
+        } else if (point_fallback && point_fallback[q] && !s.all[0] && !s.all[1] && !s.all[2]) {
+            // no node on the point: its nearest node takes the whole force (Boundary.jl:254-265)
+            launch_nearest(c->g, s.val, c->b_dpart.p, c->b_dmin.p, c->b_kpart.p, c->b_key.p, c->stream);
+            launch_force_point(c->g, c->b_key.p, force + 3 * q, c->b_F.p, c->stream);
+            cnt[q] = 1;
+        }
+        HXK();
+    }
+    if (matched)
+        for (int q = 0; q < nspec; ++q) matched[q] = cnt[q];
+    return download(c, F, c->b_F.p, n * sizeof(double));
+}
+
+int hx_regrid(hx_ctx* c, int onx, int ony, int onz, int nnx, int nny, int nnz, float len_x, float len_y, float len_z,
+              int nfields, const float* old_fields, float* new_fields) {
+    if (!c || onx < 1 || ony < 1 || onz < 1 || nnx < 1 || nny < 1 || nnz < 1 || nfields < 1 || !old_fields ||
+        !new_fields || !(len_x > 0) || !(len_y > 0) || !(len_z > 0))
+        return HX_ERR_BAD_ARG;
+    const long ono = (long)onx * ony * onz, nno = (long)nnx * nny * nnz;
+    HXA(c->r_old, ono * nfields);
+    HXA(c->r_new, nno * nfields);
+    int rc;
+    if ((rc = upload(c, c->r_old.p, old_fields, ono * nfields * sizeof(float)))) return rc;
+    // spacings as the reference forms them: Float32 length / Int count (MeshRefiner.jl:122-129,
+    // Helpers.calculate_element_distribution)
+    const float nh[3] = {len_x / (float)nnx, len_y / (float)nny, len_z / (float)nnz};
+    const float oh[3] = {len_x / (float)onx, len_y / (float)ony, len_z / (float)onz};
+    launch_regrid(onx, ony, onz, nnx, nny, nnz, nh, oh, nfields, c->r_old.p, c->r_new.p, c->stream);
+    HXK();
+    return download(c, new_fields, c->r_new.p, nno * nfields * sizeof(float));
 }
 
 int hx_compliance_sens(hx_ctx* c, const double* xphys, double penal, double E0, double Emin, const double* u,

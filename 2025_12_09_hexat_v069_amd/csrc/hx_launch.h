@@ -114,6 +114,29 @@ void launch_mc_count(const float* fine, int FX, int FY, int FZ, float iso, const
 void launch_mc_write(const float* fine, int FX, int FY, int FZ, float iso, float x0, float y0, float z0, float hx,
                      float hy, float hz, const int8_t* tri, const uint8_t* ntri, const unsigned* offs, unsigned cap,
                      float* out, hipStream_t s);
+// ---- boundary conditions, loads and regrid (hx_boundary.hip) ------------------
+// one location: per component "every node" (all = 1) or a value and tolerance
+struct LocSpec {
+    int all[3];
+    float val[3];
+    float tol[3];
+};
+long boundary_blocks(const GridArgs& g);
+// fixed[3n + d] = 1 for the matched nodes and the DoF bits (bit d = DoF d + 1)
+void launch_bc_mark(const GridArgs& g, const LocSpec& s, int dof_bits, uint8_t* fixed, hipStream_t st);
+// *count = matched nodes (part: boundary_blocks words)
+void launch_loc_count(const GridArgs& g, const LocSpec& s, unsigned* part, unsigned* count, hipStream_t st);
+// F += f / *count on the matched nodes
+void launch_force_add(const GridArgs& g, const LocSpec& s, const unsigned* count, const double f[3], double* F,
+                      hipStream_t st);
+// the nearest node to t as a key (bit 63 set, then i, j, k in 21 bits each)
+void launch_nearest(const GridArgs& g, const float t[3], float* dpart, float* dmin, unsigned long long* kpart,
+                    unsigned long long* key, hipStream_t st);
+void launch_force_point(const GridArgs& g, const unsigned long long* key, const double f[3], double* F,
+                        hipStream_t st);
+// nf stacked element fields of the old grid onto the new grid (nearest cell)
+void launch_regrid(int onx, int ony, int onz, int nnx, int nny, int nnz, const float nh[3], const float oh[3], int nf,
+                   const float* oldf, float* newf, hipStream_t st);
 // the packing's scratch for hx_web_pack_device: kWebScratchTiles tile counts
 // then kWebScratchTiles + 1 offsets (at most kWebScratchTiles * 2048 elements)
 constexpr int kWebScratchTiles = 65536;

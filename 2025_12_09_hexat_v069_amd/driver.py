@@ -68,9 +68,15 @@ def build_problem(config):
     g.update(nElem_x_computed=geo["nElem_x"], nElem_y_computed=geo["nElem_y"], nElem_z_computed=geo["nElem_z"],
              dx_computed=float(geo["dx"]), dy_computed=float(geo["dy"]), dz_computed=float(geo["dz"]),
              max_domain_dim=float(geo["max_domain_dim"]))
-    bc = boundary.get_bc_indicator(nodes.shape[0], nodes, list(config["boundary_conditions"]))
-    F = np.zeros(3 * nodes.shape[0], dtype=np.float64)
-    boundary.apply_external_forces(F, list(config["external_forces"]), nodes, elements)
+    # boundary conditions and loads mapped on the device (the structured grid of the run)
+    from .solver import get_context
+    ctx = get_context()
+    key = (geo["nElem_x"], geo["nElem_y"], geo["nElem_z"], float(geo["dx"]), float(geo["dy"]), float(geo["dz"]))
+    if ctx.grid is None or tuple(ctx.grid[:6]) != key:
+        ctx.set_grid(*key, nu=float(config["material"]["nu"]))
+    bc = boundary.get_bc_indicator_device(ctx, nodes, list(config["boundary_conditions"]))
+    F = boundary.apply_external_forces_device(ctx, np.zeros(3 * nodes.shape[0], dtype=np.float64),
+                                              list(config["external_forces"]), nodes)
     mat = config["material"]
     if float(mat.get("gravity_acceleration", 0.0)) != 0.0 and float(mat.get("material_density", 0.0)) > 1e-9:
         raise NotImplementedError("self-weight loads are outside this build")

@@ -96,3 +96,34 @@ def apply_geometric_modifiers(density, alpha_field, nodes, elements, shapes, min
         else:
             density[inside] = ratio
             alpha_field[inside] = 0
+
+
+def refine_mesh_and_fields(ctx, dims, density, alpha, target_active, lengths, max_growth_rate=1.2,
+                           hard_element_limit=800_000_000, free_bytes=None):
+    """MeshRefiner.refine_mesh_and_fields (MeshRefiner.jl:52-156) on a
+    structured grid: the new element count from the active ratio, capped by
+    the growth rate, the memory left (free_bytes; device memory by default)
+    and the hard limit; no change below 5 % growth.  The field map runs on the
+    device (hx_regrid).  Returns (new_dims, density, alpha) -- dims as element
+    counts (nx, ny, nz)."""
+    from .configuration import calculate_element_distribution
+    nx, ny, nz = dims
+    n_old = nx * ny * nz
+    density = np.asarray(density, dtype=np.float32)
+    ratio = max(0.01, int((density > np.float32(0.05)).sum()) / n_old)
+    ideal = int(round(target_active / ratio))
+    rate = int(round(n_old * max_growth_rate))
+    if free_bytes is None:
+        used = ctx.memory_used_gb()
+        free_bytes = max(0.0, ctx.device_info()["mem_gb"] - used) * 1024 ** 3
+    per = 180 if hard_element_limit > 500_000_000 else 250
+    buf = max(free_bytes * (0.05 if hard_element_limit > 500_000_000 else 0.20), (2.0 if hard_element_limit >
+                                                                                 500_000_000 else 4.0) * 1024 ** 3)
+    ram = int((free_bytes - buf) // per) if free_bytes > buf else 1_000_000
+    new_total = min(ideal, rate, ram, hard_element_limit)
+    if new_total < n_old * 1.05:
+        return dims, density, np.asarray(alpha, dtype=np.float32)
+    lx, ly, lz = (np.float32(v) for v in lengths)
+    nnx, nny, nnz, *_ = calculate_element_distribution(lx, ly, lz, new_total)
+    out = ctx.regrid((nx, ny, nz), (nnx, nny, nnz), (lx, ly, lz), np.stack([density, np.asarray(alpha, np.float32)]))
+    return (nnx, nny, nnz), out[0], out[1]

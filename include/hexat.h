@@ -198,6 +198,35 @@ int hx_helmholtz_filter(hx_ctx* ctx, double R, const hx_filter_opts* opts, const
  * stress (sign arbitrary).  Replaces Stress.compute_stress_field (Stress.jl:57-100). */
 int hx_element_stress(hx_ctx* ctx, const double* Ee, const double* u, double* principal,
                       double* vm, double* voigt, double* l1, double* dir);
+/* ---- boundary conditions, loads, regrid on the device --------------------- */
+/* A location of the YAML grammar resolved to this grid (Boundary.jl:79-130):
+ * per component, every node (all = 1) or the FP32 coordinate value the
+ * reference resolves ("50%", fractions in [0,1], absolute numbers); nodes
+ * match within 1e-4 * max(1, extent) per component, in FP32. */
+typedef struct {
+    int all[3];
+    float value[3];
+} hx_location;
+/* Boundary.get_bc_indicator (Boundary.jl:135-160): the fixed-DOF mask of the
+ * specs (dof_bits[q]: bit d = DoF d+1), installed in the context as
+ * hx_set_fixed would, and copied to fixed_out (3 nn bytes, nullable). */
+int hx_bc_indicator(hx_ctx* ctx, int nspec, const hx_location* locs, const int* dof_bits, uint8_t* fixed_out);
+/* Boundary.apply_external_forces! (Boundary.jl:232-291): F (ndof, in/out)
+ * += each spec's force (3 doubles) split evenly over its matched nodes, the
+ * specs in order; a spec with no match whose three components are values and
+ * point_fallback[q] set puts the whole force on the nearest node (least FP32
+ * squared distance; within 1e-9 of it the largest x, then y, then z,
+ * Boundary.jl:195-224).  matched (nullable): nodes that took each spec. */
+int hx_external_forces(hx_ctx* ctx, int nspec, const hx_location* locs, const double* force,
+                       const int* point_fallback, double* F, uint32_t* matched);
+/* MeshRefiner.refine_mesh_and_fields' field map (MeshRefiner.jl:131-150): each
+ * element of the new nnx*nny*nnz grid of the same domain takes the values of
+ * the old element its FP32 centroid falls in.  nfields stacked element fields
+ * (old: nfields x onx*ony*onz, new: nfields x nnx*nny*nnz).  The grid of the
+ * context is not changed. */
+int hx_regrid(hx_ctx* ctx, int onx, int ony, int onz, int nnx, int nny, int nnz, float len_x, float len_y,
+              float len_z, int nfields, const float* old_fields, float* new_fields);
+
 /* Payload of the viewer/checkpoint file: the elements with
  * density[e] >= threshold, in element order; for each, its centroid (FP32 mean
  * of the 8 node coordinates, MeshUtilities.jl:18-32), its density and its L1

@@ -967,3 +967,28 @@ def trilinear_volume(fine, iso, h, samples=4):
                 v = (a * (1 - ty) + b * ty) * (1 - tz) + (c * (1 - ty) + d * ty) * tz
                 tot += int((v >= iso).sum())
     return tot / samples ** 3 * h[0] * h[1] * h[2]
+
+
+# ----------------------------------------------------------------------------
+# regrid field map (MeshRefiner.jl:131-150), FP32 as the reference
+# ----------------------------------------------------------------------------
+
+def regrid_map(old_dims, new_dims, lengths, field):
+    onx, ony, onz = old_dims
+    nnx, nny, nnz = new_dims
+    lx, ly, lz = (np.float32(v) for v in lengths)
+    ndx, ndy, ndz = lx / np.float32(nnx), ly / np.float32(nny), lz / np.float32(nnz)
+    odx, ody, odz = lx / np.float32(onx), ly / np.float32(ony), lz / np.float32(onz)
+    e = np.arange(nnx * nny * nnz)
+    iz = e // (nnx * nny) + 1
+    rem = e % (nnx * nny)
+    iy = rem // nnx + 1
+    ix = rem % nnx + 1
+    cx = (ix.astype(np.float32) - np.float32(0.5)) * ndx
+    cy = (iy.astype(np.float32) - np.float32(0.5)) * ndy
+    cz = (iz.astype(np.float32) - np.float32(0.5)) * ndz
+    ox_ = np.clip(np.floor(cx / odx).astype(np.int64) + 1, 1, onx)
+    oy_ = np.clip(np.floor(cy / ody).astype(np.int64) + 1, 1, ony)
+    oz_ = np.clip(np.floor(cz / odz).astype(np.int64) + 1, 1, onz)
+    lin = ox_ + (oy_ - 1) * onx + (oz_ - 1) * onx * ony
+    return np.asarray(field)[lin - 1]

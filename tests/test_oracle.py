@@ -334,3 +334,18 @@ def test_oc_volume_and_bounds(ox):
     xe = np.full(4, 0.5)
     xn2, _, _ = ox.oc(xe, np.array([-1.0, -2.0, -3.0, -4.0]), np.ones(4), 0.5, move=0.2, rel_tol=1e-12)
     assert np.all(np.diff(xn2) >= 0)
+
+
+def test_regrid_map_closed_forms(ox):
+    """MeshRefiner.jl:131-150: same grid -> same field; each dimension doubled
+    -> every old value on its 2x2x2 children; a coarser grid samples the old
+    element under each new centroid."""
+    rng = np.random.default_rng(0)
+    f = rng.uniform(0, 1, 5 * 4 * 3).astype(np.float32)
+    assert np.array_equal(ox.regrid_map((5, 4, 3), (5, 4, 3), (2.5, 4.0, 0.9), f), f)
+    g = ox.regrid_map((5, 4, 3), (10, 8, 6), (2.5, 4.0, 0.9), f).reshape(6, 8, 10)
+    assert np.array_equal(g, np.repeat(np.repeat(np.repeat(f.reshape(3, 4, 5), 2, 0), 2, 1), 2, 2))
+    h = ox.regrid_map((6, 4, 2), (3, 2, 1), (6.0, 4.0, 2.0), np.arange(48, dtype=np.float32)).astype(int)
+    # new centroid (2i + 1) * 1.0 in x lies in old element 2i + 1 (0-based), likewise y, z: (2i+1, 2j+1, 1)
+    want = [(2 * i + 1) + 6 * ((2 * j + 1) + 4 * 1) for j in range(2) for i in range(3)]
+    assert list(h) == want
