@@ -348,14 +348,15 @@ class Context:
         self.check(rc, "hx_solve_system", (HX_OK, HX_ERR_NOT_CONVERGED))
         return u, st
 
-    def block_pcg(self, Ee, F, tol=1e-8, max_iter=100000, shift=0.0, precision=HX_FP64, matvec=HX_MV_AUTO):
+    def block_pcg(self, Ee, F, tol=1e-8, max_iter=100000, shift=0.0, precision=HX_FP64, matvec=HX_MV_AUTO,
+                  precond=HX_PC_JACOBI):
         Ee = f64(Ee, self.ne)
         F = np.ascontiguousarray(F, dtype=np.float64)
         nrhs = F.shape[0]
         if F.shape != (nrhs, self.ndof):
             raise ValueError("F must be nrhs x ndof")
         U = np.zeros_like(F)
-        o = PcgOpts(float(tol), int(max_iter), float(shift), int(precision), HX_PC_JACOBI, 0, int(matvec), 0)
+        o = PcgOpts(float(tol), int(max_iter), float(shift), int(precision), int(precond), 0, int(matvec), 0)
         S = (PcgStats * nrhs)()
         rc = self.L.hx_block_pcg_solve(self.h, nrhs, dptr(Ee), dptr(F), dptr(U), C.byref(o), S)
         self.check(rc, "hx_block_pcg_solve", (HX_OK, HX_ERR_NOT_CONVERGED))

@@ -187,6 +187,8 @@ struct hx_ctx {
     double ir_delta = kIrDelta;
     DevBuf<double> vc_s[kMlMaxLevels + 1], vc_res[kMlMaxLevels + 1], vc_E[kMlMaxLevels + 1];
     DevBuf<float> vc_s32[kMlMaxLevels + 1], vc_res32[kMlMaxLevels + 1], vc_E32[kMlMaxLevels + 1];
+    DevBuf<double> vc_cstage;  // the one-workgroup pass's level data, contiguous (MlHierarchy::cstage)
+    DevBuf<float> vc_cstage32;
     // captured iteration batches (hipGraph), keyed by kernel arguments
     bool graphs_on = true;
     std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
@@ -351,10 +353,12 @@ long ndof(const hx_ctx* c) { return 3 * c->g.nn; }
 // y = K(E) x (+ shift x); MV_PCG mode also reduces p'q into the state.
 long ndof(const hx_ctx* c);
 
-// The operator's p'q partials live after the vector kernels' (3 per block of
-// kVecBlocks, kMaxRhs columns), so a PCG update that reduces them itself
-// (defer_alpha) never overwrites them while other blocks still read.
-constexpr long kOpPartials = 3L * kVecBlocks * kMaxRhs;
+// The operator's p'q partials live after the vector kernels' and the
+// V-cycle's (per column 2 per block of kVecBlocks, then 2 per fine
+// post-smoothing workgroup, up to 4096 of them), so a PCG update that reduces
+// them itself (defer_alpha) never overwrites them while other blocks still
+// read.
+constexpr long kOpPartials = 2L * kMaxRhs * (kVecBlocks + 4096);
 
 // returns the operator's workgroups per column (its p'q partials) in *npq
 int mv64(hx_ctx* c, const double* x, double* y, const double* E, double shift, int mode, int variant,
@@ -451,7 +455,8 @@ int worst_code(const hx_ctx* c, int nrhs) {
 // V-cycle form also the smoothed levels' smoother diagonals, operator moduli
 // and vectors.  The fine level's moduli (E, E32) and inverse diagonal (minv,
 // minv32) are the solve's own.
-int ml_setup(hx_ctx* c, bool f32, bool vcycle = false) {
+// the hierarchy of the current moduli; its vectors hold nrhs columns
+int ml_setup(hx_ctx* c, bool f32, bool vcycle = false, int nrhs = 1) {
     ml_prepare_kernels();
     int L = ml_levels_for(c->g, std::min(c->ml_max_levels, kMlMaxLevels));
     bool exact = false;
@@ -476,8 +481,8 @@ int ml_setup(hx_ctx* c, bool f32, bool vcycle = false) {
     h.g[0] = h32.g[0] = c->g;
     h.minv[0] = c->minv.p;
     h32.minv[0] = c->minv32.p;
-    if (f32) HXA(c->ml_z32, n);
-    else HXA(c->ml_z, n);
+    if (f32) HXA(c->ml_z32, n * nrhs);
+    else HXA(c->ml_z, n * nrhs);
     GridArgs gl = c->g;
     const double* Ef = c->E.p;
     for (int l = 1; l <= L; ++l) {
@@ -489,8 +494,8 @@ int ml_setup(hx_ctx* c, bool f32, bool vcycle = false) {
         const double kdl[3] = {c->kd[0] * (1 << l), c->kd[1] * (1 << l), c->kd[2] * (1 << l)};
         const uint8_t* fx = c->has_fixed ? c->fixed.p : nullptr;
         if (f32) {
-            HXA(c->ml_r32[l], nc);
-            HXA(c->ml_y32[l], nc);
+            HXA(c->ml_r32[l], nc * nrhs);
+            HXA(c->ml_y32[l], nc * nrhs);
             HXA(c->ml_minv32[l], nc);
             launch_ml_minv<float>(gc, kdl, c->ml_E[l].p, fx, 1 << l, c->g, c->ml_minv32[l].p, c->stream);
             h32.r[l] = c->ml_r32[l].p;
@@ -498,8 +503,8 @@ int ml_setup(hx_ctx* c, bool f32, bool vcycle = false) {
             h32.minv[l] = c->ml_minv32[l].p;
             h32.w[l] = 1.0;
         } else {
-            HXA(c->ml_r[l], nc);
-            HXA(c->ml_y[l], nc);
+            HXA(c->ml_r[l], nc * nrhs);
+            HXA(c->ml_y[l], nc * nrhs);
             HXA(c->ml_minv[l], nc);
             launch_ml_minv<double>(gc, kdl, c->ml_E[l].p, fx, 1 << l, c->g, c->ml_minv[l].p, c->stream);
             h.r[l] = c->ml_r[l].p;
@@ -535,7 +540,7 @@ int ml_setup(hx_ctx* c, bool f32, bool vcycle = false) {
             const double hl = (double)(1 << l);
             if (f32) {
                 HXA(c->vc_s32[l], nl);
-                HXA(c->vc_res32[l], nl);
+                HXA(c->vc_res32[l], nl * nrhs);
                 launch_vc_smoother<float>(gl, El, hl, c->Kbin_d.p, fx, 1 << l, c->g, c->vc_omega, c->vc_s32[l].p,
                                           c->stream);
                 if (l > 0) {
@@ -547,7 +552,7 @@ int ml_setup(hx_ctx* c, bool f32, bool vcycle = false) {
                 h32.Eop[l] = l == 0 ? c->E32.p : c->vc_E32[l].p;
             } else {
                 HXA(c->vc_s[l], nl);
-                HXA(c->vc_res[l], nl);
+                HXA(c->vc_res[l], nl * nrhs);
                 launch_vc_smoother<double>(gl, El, hl, c->Kbin_d.p, fx, 1 << l, c->g, c->vc_omega, c->vc_s[l].p,
                                            c->stream);
                 if (l > 0) {
@@ -559,6 +564,15 @@ int ml_setup(hx_ctx* c, bool f32, bool vcycle = false) {
                 h.Eop[l] = l == 0 ? c->E.p : c->vc_E[l].p;
             }
         }
+        int rc;
+        if (f32) {
+            HXA(c->vc_cstage32, std::max(1L, vc_coarse_stage_values(h32)));
+            rc = launch_vc_coarse_stage(h32, c->vc_cstage32.p, c->stream);
+        } else {
+            HXA(c->vc_cstage, std::max(1L, vc_coarse_stage_values(h)));
+            rc = launch_vc_coarse_stage(h, c->vc_cstage.p, c->stream);
+        }
+        if (rc) return fail(c, rc, "staging the V-cycle's coarse level data");
     }
     HXK();
     return HX_OK;
@@ -577,24 +591,26 @@ bool vc_free_update(const hx_ctx* c, bool vcycle, bool f32) {
 }
 
 // z = M^-1 r of the explicit preconditioner into ml_z / ml_z32 (fin_mode as
-// launch_ml_apply's)
-int pc_apply64(hx_ctx* c, bool vcycle, const double* r, int fin_mode) {
+// launch_ml_apply's); nrhs columns (the V-cycle only)
+int pc_apply64(hx_ctx* c, bool vcycle, const double* r, int fin_mode, int nrhs = 1) {
     if (vcycle) {
-        if (launch_vc_apply<double>(c->ml64, c->kp64, c->Kbin_d.p, vc_shape(c, false, false),
-                                    vc_shape(c, false, true), r, c->ml_z.p, c->st, c->partials.p, kOpPartials,
-                                    fin_mode, c->stream, vc_free_update(c, true, false)) != HX_OK)
-            return fail(c, HX_ERR_BAD_ARG, "grid too large for the V-cycle's partial sums");
+        if (launch_vc_apply<double>(c->ml64, c->kp64, vc_shape(c, false, false), vc_shape(c, false, true), r,
+                                    c->ml_z.p, c->st, c->partials.p, kOpPartials, fin_mode, c->stream,
+                                    vc_free_update(c, true, false), nrhs) != HX_OK)
+            return fail(c, HX_ERR_BAD_ARG, "grid too large for the V-cycle's partial sums, or columns without a "
+                                           "coarse level");
     } else {
         launch_ml_apply<double>(c->ml64, r, c->minv.p, c->ml_z.p, c->st, c->partials.p, fin_mode, c->stream);
     }
     return HX_OK;
 }
-int pc_apply32(hx_ctx* c, bool vcycle, const float* r, int fin_mode) {
+int pc_apply32(hx_ctx* c, bool vcycle, const float* r, int fin_mode, int nrhs = 1) {
     if (vcycle) {
-        if (launch_vc_apply<float>(c->ml32, c->kp32, c->Kbin32_d.p, vc_shape(c, true, false),
-                                   vc_shape(c, true, true), r, c->ml_z32.p, c->st, c->partials.p, kOpPartials,
-                                   fin_mode, c->stream, vc_free_update(c, true, true)) != HX_OK)
-            return fail(c, HX_ERR_BAD_ARG, "grid too large for the V-cycle's partial sums");
+        if (launch_vc_apply<float>(c->ml32, c->kp32, vc_shape(c, true, false), vc_shape(c, true, true), r,
+                                   c->ml_z32.p, c->st, c->partials.p, kOpPartials, fin_mode, c->stream,
+                                   vc_free_update(c, true, true), nrhs) != HX_OK)
+            return fail(c, HX_ERR_BAD_ARG, "grid too large for the V-cycle's partial sums, or columns without a "
+                                           "coarse level");
     } else {
         launch_ml_apply<float>(c->ml32, r, c->minv32.p, c->ml_z32.p, c->st, c->partials.p, fin_mode, c->stream);
     }
@@ -613,7 +629,7 @@ int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& 
     // p = z + beta p_old (parity contraction, which has that fused form)
     const bool vc = o.precond == HX_PC_VCYCLE;
     const bool ml = o.precond == HX_PC_MULTILEVEL || vc;
-    int rc = ml ? ml_setup(c, false, vc) : HX_OK;
+    int rc = ml ? ml_setup(c, false, vc, nr) : HX_OK;
     if (rc) return rc;
     const bool free_upd = vc_free_update(c, vc, false);
     launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream, nr, v.xbest ? 1 : 0);
@@ -622,7 +638,7 @@ int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& 
     if (v.xbest) HXC(hipMemcpyAsync(v.xbest, v.x, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     mv64(c, v.x, v.q, c->E.p, 0.0, MV_PLAIN, o.matvec, nr);
     launch_pcg_init<double>(n, v.f, v.x, v.q, c->minv.p, v.r, v.p, o.shift, c->st, c->partials.p, c->stream, nr);
-    if (ml && (rc = pc_apply64(c, vc, v.r, 1))) return rc;
+    if (ml && (rc = pc_apply64(c, vc, v.r, 1, nr))) return rc;
     HXK();
     if ((rc = sync_state(c, nr))) return rc;
     const int mvv = ml ? HX_MV_PARITY : o.matvec;
@@ -643,14 +659,14 @@ int pcg_fp64(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs& 
                     TimeScope ts(c, HX_K_UPDATE);
                     if (free_upd)
                         launch_pcg_update_free<double>(n, v.x, pout, v.r, v.q, c->st, c->partials.p + kOpPartials,
-                                                       npq, c->stream, v.xbest);
+                                                       npq, c->stream, v.xbest, nr);
                     else
                         launch_pcg_update<double>(n, v.x, pout, v.r, v.q, c->minv.p, c->st, c->partials.p,
                                                   c->stream, nr, !ml, c->partials.p + kOpPartials, npq, v.xbest);
                 }
                 if (ml) {
                     TimeScope ts(c, HX_K_PRECOND);
-                    (void)pc_apply64(c, vc, v.r, 0);  // its one failure is caught at the start call
+                    (void)pc_apply64(c, vc, v.r, 0, nr);  // its one failure is caught at the start call
                 }
             }
         });
@@ -713,7 +729,7 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
     launch_convert<double, float>(n, c->minv.p, c->minv32.p, c->stream);
     const bool vc = o.precond == HX_PC_VCYCLE;
     const bool ml = o.precond == HX_PC_MULTILEVEL || vc;  // explicit z = M^-1 r (see pcg_fp64)
-    if (ml && (rc = ml_setup(c, true, vc))) return rc;
+    if (ml && (rc = ml_setup(c, true, vc, nr))) return rc;
     const bool free_upd = vc_free_update(c, vc, true);
     launch_pcg_reset(c->st, o.tol, o.shift, o.max_iter, 1, c->stream, nr, v.xbest ? 2 : 0);
     if (o.warm_start) launch_mask<double>(n, c->minv.p, v.x, c->stream, nr);
@@ -724,7 +740,7 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
     mv64(c, v.x, v.q, c->E.p, 0.0, MV_PLAIN, HX_MV_PARITY, nr);
     launch_true_residual(n, v.f, v.q, v.x, c->minv.p, o.shift, nullptr, v.r32, c->st, c->partials.p, 3, c->stream,
                          nr);
-    if (ml && (rc = pc_apply32(c, vc, v.r32, 2))) return rc;
+    if (ml && (rc = pc_apply32(c, vc, v.r32, 2, nr))) return rc;
     launch_ir_resume(c->st, o.tol, c->ir_delta, 1, c->stream, nr);
     HXK();
     if ((rc = sync_state(c, nr))) return rc;
@@ -747,7 +763,8 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
                         TimeScope ts(c, HX_K_UPDATE);
                         if (free_upd)
                             launch_pcg_update_free<float>(n, v.x32, pout, v.r32, v.q32, c->st,
-                                                          c->partials.p + kOpPartials, npq, c->stream);
+                                                          c->partials.p + kOpPartials, npq, c->stream, nullptr,
+                                                          nr);
                         else
                             launch_pcg_update<float>(n, v.x32, pout, v.r32, v.q32, c->minv32.p, c->st,
                                                      c->partials.p, c->stream, nr, !ml, c->partials.p + kOpPartials,
@@ -755,7 +772,7 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
                     }
                     if (ml) {
                         TimeScope ts(c, HX_K_PRECOND);
-                        (void)pc_apply32(c, vc, v.r32, 0);  // its one failure is caught at the start call
+                        (void)pc_apply32(c, vc, v.r32, 0, nr);  // its one failure is caught at the start call
                     }
                 }
             });
@@ -772,7 +789,7 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
         // the phase failed (its folded correction is then not an iterate)
         if (v.xbest && c->st_host[0].status != 4 && c->st_host[0].status != 5)
             launch_snapshot(n, v.x, v.xbest, c->st, c->stream);
-        if (ml && (rc = pc_apply32(c, vc, v.r32, 2))) return rc;
+        if (ml && (rc = pc_apply32(c, vc, v.r32, 2, nr))) return rc;
         launch_ir_resume(c->st, o.tol, c->ir_delta, 0, c->stream, nr);
         HXK();
         ++refinements;
@@ -1509,8 +1526,10 @@ int hx_block_pcg_solve(hx_ctx* c, int nrhs, const double* Ee, const double* F, d
     if (int rc = ensure_grid(c)) return rc;
     if (nrhs < 1 || nrhs > kMaxRhs) return fail(c, HX_ERR_BAD_ARG, "nrhs must be 1..8");
     if (int rc = check_opts(c, o)) return rc;
-    if ((o->precond == HX_PC_MULTILEVEL || o->precond == HX_PC_VCYCLE) && nrhs > 1)
-        return fail(c, HX_ERR_BAD_ARG, "the multilevel preconditioner takes one right-hand side");
+    if (o->precond == HX_PC_MULTILEVEL && nrhs > 1)
+        return fail(c, HX_ERR_BAD_ARG, "the additive multilevel preconditioner takes one right-hand side");
+    if (o->precond == HX_PC_VCYCLE && nrhs > 1 && ml_levels_for(c->g, std::min(c->ml_max_levels, kMlMaxLevels)) < 1)
+        return fail(c, HX_ERR_BAD_ARG, "a V-cycle block solve needs a coarse level (even element counts)");
     const long n = ndof(c), nt = n * nrhs;
     HXA(c->b_f, nt);
     HXA(c->b_x, nt);

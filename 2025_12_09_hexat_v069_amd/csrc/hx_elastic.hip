@@ -330,7 +330,7 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
     __shared__ T so[POUT ? 2 : 1][OVALS];  // results of the owned nodes, node-major, for row-contiguous stores
     __shared__ double red[NT / 64];
 
-    a.x += (long)blockIdx.y * a.cs;
+    if constexpr (!SMOOTH) a.x += (long)blockIdx.y * a.cs;  // the smoothing forms read no x
     a.y += (long)blockIdx.y * a.cs;
     if constexpr (FUSE) a.r += (long)blockIdx.y * a.cs;
     if constexpr (POUT) a.p_out += (long)blockIdx.y * a.cs;
@@ -339,7 +339,12 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
         a.partials += (long)blockIdx.y * gridDim.x;
     }
     if (a.mode == MV_PCG && a.st->done) return;
-    if constexpr (SMOOTH) {
+    if constexpr (SMOOTH) {  // column blockIdx.y: its state, coarse correction and r'y, r'r partials
+        if (a.st != nullptr) a.st += blockIdx.y;
+        if constexpr (PRO) {
+            a.ec += (long)blockIdx.y * a.ccs;
+            if (a.partials != nullptr) a.partials += 2L * gridDim.x * blockIdx.y;
+        }
         if (a.st != nullptr && a.st->done) return;
     }
     const T beta = (FUSE == 1 || FUSE == 2) ? T(a.st->beta) : T(0);

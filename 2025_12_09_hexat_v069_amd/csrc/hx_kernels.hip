@@ -244,9 +244,15 @@ __global__ __launch_bounds__(NT) void k_pcg_update_free(long n, T* __restrict__ 
                                                         T* __restrict__ xbest) {
     __shared__ double red[NT / 64];
     __shared__ double s_alpha;
+    {  // column blockIdx.y
+        const long off = (long)blockIdx.y * n;
+        x += off; p += off; r += off; q += off;
+        if (xbest) xbest += off;
+        st += blockIdx.y;
+    }
     if (st->done) return;
     const bool snap = xbest != nullptr && st->snap_pending;  // as in k_pcg_update
-    const double alpha = update_alpha(st, pq_partials, npq, red, s_alpha);
+    const double alpha = update_alpha(st, pq_partials + (size_t)blockIdx.y * npq, npq, red, s_alpha);
     if (alpha != alpha) return;
     auto one = [&](T& xi, T pi, T& ri, T qi) {
         xi = T((double)xi + alpha * (double)pi);
@@ -1040,8 +1046,8 @@ void launch_pcg_update(long n, T* x, const T* p, T* r, const T* q, const T* minv
 }
 template <typename T>
 void launch_pcg_update_free(long n, T* x, const T* p, T* r, const T* q, PcgState* st, const double* pq_partials,
-                            int npq, hipStream_t s, T* xbest) {
-    hipLaunchKernelGGL((k_pcg_update_free<T>), dim3(kVecBlocks), dim3(NT), 0, s, n, x, p, r, q, st, pq_partials,
+                            int npq, hipStream_t s, T* xbest, int nrhs) {
+    hipLaunchKernelGGL((k_pcg_update_free<T>), dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, x, p, r, q, st, pq_partials,
                        (unsigned)(pq_partials ? npq : 0), xbest);
 }
 template <typename T>
@@ -1212,9 +1218,9 @@ template void launch_pcg_update<double>(long, double*, const double*, double*, c
 template void launch_pcg_update<float>(long, float*, const float*, float*, const float*, const float*, PcgState*,
                                        double*, hipStream_t, int, bool, const double*, int, float*);
 template void launch_pcg_update_free<double>(long, double*, const double*, double*, const double*, PcgState*,
-                                             const double*, int, hipStream_t, double*);
+                                             const double*, int, hipStream_t, double*, int);
 template void launch_pcg_update_free<float>(long, float*, const float*, float*, const float*, PcgState*,
-                                            const double*, int, hipStream_t, float*);
+                                            const double*, int, hipStream_t, float*, int);
 template void launch_pcg_pupdate<double>(long, const double*, const double*, double*, const PcgState*, hipStream_t,
                                          int);
 template void launch_pcg_pupdate<float>(long, const float*, const float*, float*, const PcgState*, hipStream_t, int);

@@ -44,6 +44,7 @@ struct MvArgs {
     int form = 0;
     const T* ec = nullptr;         // form 4: coarse correction on the next level's node grid
     int cNX = 0, cNY = 0, cNZ = 0;  // its node counts
+    long ccs = 0;                   // form 4: column stride of ec (multi-RHS)
 };
 
 // tile: 0 -> 16x16 element columns (default), 1 -> 16x32, 2 -> 32x8.
@@ -160,10 +161,11 @@ void launch_pcg_update(long n, T* x, const T* p, T* r, const T* q, const T* minv
                        double* partials, hipStream_t s, int nrhs = 1, bool set_beta = true,
                        const double* pq_partials = nullptr, int npq = 0, T* xbest = nullptr);
 // x += alpha p ; r -= alpha q without the mask or the r'z, r'r partials (the
-// V-cycle's post-smoothing operator forms r'r); one right-hand side
+// V-cycle's post-smoothing operator forms r'r); nrhs columns of n values, the
+// operator's npq p'q partials per column
 template <typename T>
 void launch_pcg_update_free(long n, T* x, const T* p, T* r, const T* q, PcgState* st, const double* pq_partials,
-                            int npq, hipStream_t s, T* xbest = nullptr);
+                            int npq, hipStream_t s, T* xbest = nullptr, int nrhs = 1);
 template <typename T>
 void launch_pcg_pupdate(long n, const T* r, const T* minv, T* p, const PcgState* st, hipStream_t s, int nrhs = 1);
 // snap_mode: PcgState::snap_mode of every column
@@ -246,6 +248,12 @@ struct MlHierarchy {
     // workgroups a coarse level's launch aims for: the V-cycle's level
     // operators and every restriction take shallower z chunks until reached
     int min_blocks = 512;
+    // the V-cycle's one-workgroup pass: moduli and smoother diagonal of each
+    // level it smooths (E_l then S_l, level by level), one contiguous copy
+    // made per hierarchy (launch_vc_coarse_stage), so the pass stages them
+    // with coalesced loads from one address
+    const T* cstage = nullptr;
+    long cstage_n = 0;
 };
 // number of 2:1 coarsenings of g (all element counts even), at most max_levels
 int ml_levels_for(const GridArgs& g, int max_levels);
@@ -283,19 +291,28 @@ void launch_vc_smoother(const GridArgs& gc, const double* Ec, double h, const do
                         int stride, const GridArgs& gfine, double omega, T* sinv, hipStream_t s);
 template <typename T>
 void launch_vc_scale_E(long ne, const double* Ec, double h, T* Eop, hipStream_t s);
+// values of h.cstage for the hierarchy's one-workgroup pass (0: none)
+template <typename T>
+long vc_coarse_stage_values(const MlHierarchy<T>& h);
+// fill buf (vc_coarse_stage_values(h) values) and point h.cstage at it;
+// after the levels' Eop and sinv are set, outside graph capture
+template <typename T>
+int launch_vc_coarse_stage(MlHierarchy<T>& h, T* buf, hipStream_t s);
 // z = V(r) with the parity operator K, its pre- and post-smoothing forms at
 // shapes sh_pre / sh_post on the multi-block levels (coarse ones with
-// shallower z chunks), the small levels in one workgroup with the element
-// stiffness Kb (24x24, binary node order); the r'z partials (one per level-0
-// post-smoothing block) go after the update's and the finaliser runs as in
-// launch_ml_apply.  HX_ERR_BAD_ARG when those partials would pass max_partials.
-template <typename T>
+// shallower z chunks), the small levels in one workgroup per column; the r'z
+// partials (one per level-0 post-smoothing block) go after the update's and
+// the finaliser runs as in launch_ml_apply.  HX_ERR_BAD_ARG when those
+// partials would pass max_partials.
 // rr_post: the finaliser takes r'r from the fine post-smoothing operator
 // (after a launch_pcg_update_free) instead of the update's partials; needs a
-// coarse level
-int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const T* Kb, const MvShape& sh_pre,
-                    const MvShape& sh_post, const T* r, T* z, PcgState* st, double* partials, long max_partials,
-                    int fin_mode, hipStream_t s, bool rr_post = false);
+// coarse level.  nrhs columns (blockIdx.y) share the hierarchy: r, z and the
+// level vectors hold nrhs columns each, one state per column; nrhs > 1 needs
+// a coarse level.
+template <typename T>
+int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const MvShape& sh_pre, const MvShape& sh_post,
+                    const T* r, T* z, PcgState* st, double* partials, long max_partials, int fin_mode, hipStream_t s,
+                    bool rr_post = false, int nrhs = 1);
 
 // ---- Helmholtz filter --------------------------------------------------------
 struct HelmArgs {

@@ -107,6 +107,12 @@ __global__ __launch_bounds__(CX * CY) void k_ml_restrict_zm(GridArgs gf, const T
                                                             int nty, int kc) {
     constexpr int NTH = CX * CY, ROW = 3 * (2 * CX + 1), PLANE = (2 * CY + 1) * ROW, PER = (PLANE + NTH - 1) / NTH;
     __shared__ T sf[2][PLANE];
+    {  // column blockIdx.y
+        rf += (long)blockIdx.y * 3 * gf.nn;
+        rc += (long)blockIdx.y * 3 * gc.nn;
+        if constexpr (SCALE) yc += (long)blockIdx.y * 3 * gc.nn;
+        if (st != nullptr) st += blockIdx.y;
+    }
     if (st != nullptr && st->done) return;
     const int t = threadIdx.x;
     const int bx = blockIdx.x % ntx, by = (blockIdx.x / ntx) % nty, bz = blockIdx.x / (ntx * nty);
@@ -199,13 +205,14 @@ inline int restrict_zm_kc(const GridArgs& gc, int ntx, int nty, int min_blocks) 
     return kc;
 }
 
-// rc = R rf onto gc (and, with minvc, yc = w Dc^-1 rc)
+// rc = R rf onto gc (and, with minvc, yc = w Dc^-1 rc); nrhs columns
+// (blockIdx.y) of 3 nn values on each grid, one state per column
 template <typename T>
 void launch_restrict(const GridArgs& gf, const T* rf, const GridArgs& gc, T* rc, const T* minvc, T* yc, T w,
-                     const PcgState* st, int min_blocks, hipStream_t s) {
+                     const PcgState* st, int min_blocks, hipStream_t s, int nrhs = 1) {
     const int ntx = (gc.NX + kRzX - 1) / kRzX, nty = (gc.NY + kRzY - 1) / kRzY;
-    const int kc = restrict_zm_kc(gc, ntx, nty, min_blocks);
-    const dim3 grid(ntx * nty * ((gc.NZ + kc - 1) / kc));
+    const int kc = restrict_zm_kc(gc, ntx, nty, (min_blocks + nrhs - 1) / nrhs);
+    const dim3 grid(ntx * nty * ((gc.NZ + kc - 1) / kc), nrhs);
     if (minvc != nullptr)
         hipLaunchKernelGGL((k_ml_restrict_zm<T, true>), grid, dim3(kRzX * kRzY), 0, s, gf, rf, gc, rc, minvc, yc, w,
                            st, ntx, nty, kc);
@@ -396,112 +403,213 @@ __global__ __launch_bounds__(kCoarseThreads) void k_ml_coarse(MlHierarchy<T> h, 
     for (long i = threadIdx.x; i < 3 * h.g[s0].nn; i += kCoarseThreads) h.y[s0][i] = Y(s0)[i];
 }
 
-// V-cycle levels s..L in one workgroup (the grids too small for a launch per
-// level): from the residual of level s-1 (memory) down to the coarsest level
-// and back, every level's r, x and a work vector in LDS (dynamic, after the
-// element stiffness: level by level r_l, x_l, t_l), passes separated by
-// barriers; only e_s = x_s is written back.  Level l < L: x = S r;
-// t = mask(r - A x); r_{l+1} = R t; ...; x += mask(P x_{l+1}); t = A x;
-// x += S (r - t).  Level L: x = A_L^-1 r (exact: one wave per row, lanes
-// over the columns) or D_L^-1 r.  The level operator gathers, per node, the 8
-// elements around it.  Staged in LDS first, while the first restriction
-// reads memory: the fine element stiffness Kb (binary node order), the
-// smoothed levels' moduli h.Eop[l] and smoother diagonals h.sinv[l], and,
-// when it fits the budget (stage_ainv), the exact inverse.
+// V-cycle levels s..L in one workgroup per column (the grids too small for a
+// launch per level): from the residual of level s-1 (memory) down to the
+// coarsest level and back, every level's r, x and a work vector in LDS
+// (level by level r_l, x_l, t_l), passes separated by barriers; only
+// e_s = x_s is written back.  Level l < L: x = S r; t = mask(r - A x);
+// r_{l+1} = R t; ...; x += mask(P x_{l+1}); t = A x; x += S (r - t).  Level L:
+// x = A_L^-1 r (exact: one wave per row, lanes over the columns) or D_L^-1 r.
+// The level operator runs in two passes: every element's 24 outputs E_e Ke x_e
+// in the parity basis (one thread per element, the coefficients in registers)
+// into a work buffer W, component-major, then every node sums the outputs of
+// its elements in a fixed order (deterministic).  The first loads -- the
+// exact inverse (when it fits the budget, stage_ainv), the smoothed levels'
+// moduli h.Eop[l] and smoother diagonals h.sinv[l], and level s-1's residual
+// (restricted from LDS: a gather of its taps from memory by one workgroup is
+// bound by the workgroup's address throughput) -- are all issued, coalesced,
+// before any is used.
+template <typename T>
+__host__ __device__ long vc_coarse_work_values(const MlHierarchy<T>& h, int s0) {
+    long w = 3 * h.g[s0 - 1].nn;  // level s0-1's residual, staged for the first restriction
+    for (int l = s0; l < h.levels; ++l) w = 24 * h.g[l].ne > w ? 24 * h.g[l].ne : w;
+    return w;
+}
 template <typename T>
 long vc_coarse_lds_values(const MlHierarchy<T>& h, int s0, bool stage_ainv = false) {
-    long v = 576;
+    long v = vc_coarse_work_values(h, s0);
     for (int l = s0; l < h.levels; ++l) v += h.g[l].ne + 3 * h.g[l].nn;
     for (int l = s0; l <= h.levels; ++l) v += 9 * h.g[l].nn;
     if (stage_ainv) v += 9 * h.g[h.levels].nn * h.g[h.levels].nn;
     return v;
 }
 
+// node (i, j, k) of node index n < 2^31 on grid g, unsigned arithmetic
+__device__ __forceinline__ void node_ijk_u(const GridArgs& g, unsigned n, int& i, int& j, int& k) {
+    const unsigned q = n / (unsigned)g.NX;
+    i = (int)(n - q * (unsigned)g.NX);
+    k = (int)(q / (unsigned)g.NY);
+    j = (int)(q - (unsigned)k * (unsigned)g.NY);
+}
+
+// component c of the full-weighting restriction at node n of gc from src on
+// gf (LDS; 32-bit offsets): separable weights, clamped taps with weight 0
+// outside the grid
 template <typename T>
-__global__ __launch_bounds__(kCoarseThreads) void k_vc_coarse(MlHierarchy<T> h, int s0, const T* __restrict__ Kb,
-                                                              int stage_ainv, const PcgState* st) {
+__device__ __forceinline__ T restrict_comp(const GridArgs& gf, const T* src, const GridArgs& gc, unsigned n, int c) {
+    int I, J, K;
+    node_ijk_u(gc, n, I, J, K);
+    int ia[3], ja[3], ka[3];
+    T wi[3], wj[3], wk[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int i = 2 * I + d - 1, j = 2 * J + d - 1, k = 2 * K + d - 1;
+        const T wd = d == 1 ? T(1) : T(0.5);
+        wi[d] = (i >= 0 && i < gf.NX) ? wd : T(0);
+        wj[d] = (j >= 0 && j < gf.NY) ? wd : T(0);
+        wk[d] = (k >= 0 && k < gf.NZ) ? wd : T(0);
+        ia[d] = 3 * min(max(i, 0), gf.NX - 1) + c;
+        ja[d] = 3 * gf.NX * min(max(j, 0), gf.NY - 1);
+        ka[d] = 3 * gf.NX * gf.NY * min(max(k, 0), gf.NZ - 1);
+    }
+    T acc = T(0);
+#pragma unroll
+    for (int dk = 0; dk < 3; ++dk) {
+        T pk = T(0);
+#pragma unroll
+        for (int dj = 0; dj < 3; ++dj) {
+            const T* row = src + ka[dk] + ja[dj];
+            pk += wj[dj] * (wi[0] * row[ia[0]] + wi[1] * row[ia[1]] + wi[2] * row[ia[2]]);
+        }
+        acc += wk[dk] * pk;
+    }
+    return acc;
+}
+// component c of the trilinear interpolation at node n of gf from src on gc
+// (LDS; 32-bit offsets)
+template <typename T>
+__device__ __forceinline__ T prolong_comp(const GridArgs& gc, const T* src, const GridArgs& gf, unsigned n, int c) {
+    int i, j, k;
+    node_ijk_u(gf, n, i, j, k);
+    int xa[2], ya[2], za[2];
+    float xw[2], yw[2], zw[2];
+    lin_corners(i, gc.NX, xa[0], xa[1], xw[0], xw[1]);
+    lin_corners(j, gc.NY, ya[0], ya[1], yw[0], yw[1]);
+    lin_corners(k, gc.NZ, za[0], za[1], zw[0], zw[1]);
+    T acc = T(0);
+#pragma unroll
+    for (int c3 = 0; c3 < 2; ++c3)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+                acc += T(xw[a] * yw[b] * zw[c3]) * src[3 * ((za[c3] * gc.NY + ya[b]) * gc.NX + xa[a]) + c];
+    return acc;
+}
+
+constexpr int kCoarsePre = 18;       // exact-inverse values per thread fetched with the first loads
+constexpr int kCoarseStagePre = 4;   // level-data values per thread fetched with them
+constexpr int kCoarseResPre = 8;     // values of the finer level's residual per thread fetched with them
+
+template <typename T>
+__global__ __launch_bounds__(kCoarseThreads) void k_vc_coarse(MlHierarchy<T> h, int s0, ParityK<T> K, int stage_ainv,
+                                                              const PcgState* st) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     T* lds = reinterpret_cast<T*>(lds_raw);
-    if (st != nullptr && st->done) return;
+    const int col = blockIdx.y;
+    if (st != nullptr && st[col].done) return;
     const int L = h.levels, t = threadIdx.x;
-    T* const kb = lds;
     auto Ev = [&](int l) {  // moduli and smoother diagonal of smoothed level l
-        long o = 576;
+        long o = 0;
         for (int m = s0; m < l; ++m) o += h.g[m].ne + 3 * h.g[m].nn;
         return lds + o;
     };
     auto Sv = [&](int l) { return Ev(l) + h.g[l].ne; };
     auto Rv = [&](int l) {
-        long o = 576;
+        long o = 0;
         for (int m = s0; m < L; ++m) o += h.g[m].ne + 3 * h.g[m].nn;
         for (int m = s0; m < l; ++m) o += 9 * h.g[m].nn;
         return lds + o;
     };
-    T* const Av = Rv(L) + 9 * h.g[L].nn;  // the exact inverse, when staged
     auto Xv = [&](int l) { return Rv(l) + 3 * h.g[l].nn; };
     auto Tv = [&](int l) { return Rv(l) + 6 * h.g[l].nn; };
-    // copies into LDS, kStageBatch loads per thread in flight before any
-    // store; indices clamped on both sides (surplus lanes rewrite the last
-    // value with itself), so no load sits under a branch of its own -- a
-    // plain copy loop waits for every load before the next
-    constexpr int kStageBatch = 8;
-    auto stage = [&](T* dst, const T* src, long cnt) __attribute__((always_inline)) {
-        for (long base = 0; base < cnt; base += (long)kStageBatch * kCoarseThreads) {
-            T v[kStageBatch];
+    T* const W = Rv(L) + 9 * h.g[L].nn;
+    T* const Av = W + vc_coarse_work_values(h, s0);  // the exact inverse, when staged
+
+    // ---- first loads, all unconditional (clamped 32-bit indices, a valid
+    // stand-in pointer), all issued before any is parked in LDS: level s0-1's
+    // residual, the smoothed levels' moduli and smoother diagonals (E_l, S_l
+    // level by level, in LDS order from the start, from their contiguous
+    // copy) and the exact inverse
+    const GridArgs gr = h.g[s0 - 1];
+    const unsigned nres = 3u * (unsigned)gr.nn;
+    const T* rin = h.r[s0 - 1] + (long)col * nres;
+    const unsigned nstage = (unsigned)h.cstage_n;
+    const T* ssrc = nstage > 0 ? h.cstage : rin;
+    const int N = 3 * h.g[L].nn;
+    const unsigned na = stage_ainv ? (unsigned)(N * N) : 0u;
+    const T* asrc = na > 0 ? h.ainv : rin;
+    auto clampi = [](unsigned i, unsigned n) { return n > 0 ? min(i, n - 1) : 0u; };
+    T rpre[kCoarseResPre], spre[kCoarseStagePre], apre[kCoarsePre];
 #pragma unroll
-            for (int b = 0; b < kStageBatch; ++b) v[b] = src[min(base + b * kCoarseThreads + t, cnt - 1)];
+    for (int b = 0; b < kCoarseResPre; ++b) rpre[b] = rin[clampi(b * kCoarseThreads + t, nres)];
 #pragma unroll
-            for (int b = 0; b < kStageBatch; ++b) dst[min(base + b * kCoarseThreads + t, cnt - 1)] = v[b];
-        }
-    };
-    stage(kb, Kb, 576);
-    for (int l = s0; l < L; ++l) {
-        stage(Ev(l), h.Eop[l], h.g[l].ne);
-        stage(Sv(l), h.sinv[l], 3 * h.g[l].nn);
+    for (int b = 0; b < kCoarseStagePre; ++b) spre[b] = ssrc[clampi(b * kCoarseThreads + t, nstage)];
+#pragma unroll
+    for (int b = 0; b < kCoarsePre; ++b) apre[b] = asrc[clampi(b * kCoarseThreads + t, na)];
+#pragma unroll
+    for (int b = 0; b < kCoarseResPre; ++b) {
+        const unsigned i = b * kCoarseThreads + t;
+        if (i < nres) W[i] = rpre[b];
     }
-    if (stage_ainv) stage(Av, h.ainv, 9 * h.g[L].nn * h.g[L].nn);
-    {  // restriction of level s0-1's residual (memory) onto level s0
-        const GridArgs gf = h.g[s0 - 1], gc = h.g[s0];
-        for (long n = t; n < gc.nn; n += kCoarseThreads) {
-            T acc[3];
-            restrict_node(gf, (const T*)h.r[s0 - 1], gc, n, acc);
-            for (int c = 0; c < 3; ++c) Rv(s0)[3 * n + c] = acc[c];
-        }
+#pragma unroll
+    for (int b = 0; b < kCoarseStagePre; ++b) {
+        const unsigned i = b * kCoarseThreads + t;
+        if (i < nstage) lds[i] = spre[b];
     }
+#pragma unroll
+    for (int b = 0; b < kCoarsePre; ++b) {
+        const unsigned i = b * kCoarseThreads + t;
+        if (i < na) Av[i] = apre[b];
+    }
+    for (unsigned i = kCoarseResPre * kCoarseThreads + t; i < nres; i += kCoarseThreads) W[i] = rin[i];
+    for (unsigned i = kCoarseStagePre * kCoarseThreads + t; i < nstage; i += kCoarseThreads) lds[i] = h.cstage[i];
+    for (unsigned i = kCoarsePre * kCoarseThreads + t; i < na; i += kCoarseThreads) Av[i] = h.ainv[i];
     __syncthreads();
+    for (int u = t; u < 3 * (int)h.g[s0].nn; u += kCoarseThreads)  // restriction onto level s0
+        Rv(s0)[u] = restrict_comp(gr, (const T*)W, h.g[s0], (unsigned)u / 3u, (int)((unsigned)u % 3u));
+    __syncthreads();
+
     // y = A_l x over the level's nodes; with res, y = mask(r - A_l x)
     auto apply = [&](int l, const T* x, T* y, bool res) {
         const GridArgs g = h.g[l];
         const T* E = Ev(l);
+        const int ne = (int)g.ne, nx = g.nx, ny = g.ny, NX = g.NX, NXY = g.NX * g.NY;
+        for (int e = t; e < ne; e += kCoarseThreads) {  // element outputs, component-major in W
+            const unsigned q = (unsigned)e / (unsigned)nx;
+            const int ie = e - (int)q * nx, ke = (int)(q / (unsigned)ny), je = (int)q - ke * ny;
+            const int n0 = ke * NXY + je * NX + ie;
+            T u[24], yv[24];
+#pragma unroll
+            for (int Lb = 0; Lb < 8; ++Lb) {
+                const int nd = n0 + (Lb >> 2) * NXY + ((Lb >> 1) & 1) * NX + (Lb & 1);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) u[3 * Lb + c] = x[3 * nd + c];
+            }
+            element_apply_parity(u, yv, K, E[e]);
+#pragma unroll
+            for (int i = 0; i < 24; ++i) W[i * ne + e] = yv[i];
+        }
+        __syncthreads();
         const T* S = Sv(l);
         const T* r = Rv(l);
-        for (long n = t; n < g.nn; n += kCoarseThreads) {
+        for (int n = t; n < (int)g.nn; n += kCoarseThreads) {  // node sums, elements in a fixed order
             int i, j, k;
-            node_ijk(g, n, i, j, k);
+            node_ijk_u(g, (unsigned)n, i, j, k);
             T acc[3] = {T(0), T(0), T(0)};
-#pragma unroll 1
-            for (int e = 0; e < 8; ++e) {  // rolled: the element's 24x24 block is read from LDS per element
-                const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
+#pragma unroll
+            for (int e8 = 0; e8 < 8; ++e8) {
+                const int ex = e8 & 1, ey = (e8 >> 1) & 1, ez = e8 >> 2;
                 const int ie = i - 1 + ex, je = j - 1 + ey, ke = k - 1 + ez;
-                const bool in = ie >= 0 && je >= 0 && ke >= 0 && ie < g.nx && je < g.ny && ke < g.nz;
-                const int iec = min(max(ie, 0), g.nx - 1), jec = min(max(je, 0), g.ny - 1),
-                          kec = min(max(ke, 0), g.nz - 1);
-                const T Ee = in ? E[(long)kec * g.nx * g.ny + (long)jec * g.nx + iec] : T(0);
+                if (ie < 0 || je < 0 || ke < 0 || ie >= nx || je >= ny || ke >= g.nz) continue;
+                const int eidx = (ke * ny + je) * nx + ie;
                 const int La = (1 - ex) + 2 * (1 - ey) + 4 * (1 - ez);
 #pragma unroll
-                for (int Lb = 0; Lb < 8; ++Lb) {
-                    const long m = (long)(kec + (Lb >> 2)) * g.NY * g.NX + (long)(jec + ((Lb >> 1) & 1)) * g.NX +
-                                   (iec + (Lb & 1));
-                    const T xm0 = x[3 * m], xm1 = x[3 * m + 1], xm2 = x[3 * m + 2];
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        const T* kr = kb + (3 * La + a) * 24 + 3 * Lb;
-                        acc[a] += Ee * (kr[0] * xm0 + kr[1] * xm1 + kr[2] * xm2);
-                    }
-                }
+                for (int c = 0; c < 3; ++c) acc[c] += W[(3 * La + c) * ne + eidx];
             }
+#pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const long d = 3 * n + c;
+                const int d = 3 * n + c;
                 y[d] = res ? (S[d] != T(0) ? r[d] - acc[c] : T(0)) : acc[c];
             }
         }
@@ -513,25 +621,24 @@ __global__ __launch_bounds__(kCoarseThreads) void k_vc_coarse(MlHierarchy<T> h, 
         apply(l, Xv(l), Tv(l), true);
         __syncthreads();
         const GridArgs gf = h.g[l], gc = h.g[l + 1];
-        for (long n = t; n < gc.nn; n += kCoarseThreads) {
-            T acc[3];
-            restrict_node(gf, (const T*)Tv(l), gc, n, acc);
-            for (int c = 0; c < 3; ++c) Rv(l + 1)[3 * n + c] = acc[c];
-        }
+        for (int u = t; u < 3 * (int)gc.nn; u += kCoarseThreads)
+            Rv(l + 1)[u] = restrict_comp(gf, (const T*)Tv(l), gc, (unsigned)u / 3u, (int)((unsigned)u % 3u));
         __syncthreads();
     }
     {  // the coarsest level
         const long nL = 3 * h.g[L].nn;
         const T* rl = Rv(L);
-        if (h.ainv != nullptr) {  // one wave per row: coalesced columns, then a wave sum
-            const int N = (int)nL, lane = t & 63;
+        if (h.ainv != nullptr) {  // 8 lanes per row (columns q, q + 8, ...), then an 8-lane sum
+            const int q = t & 7;
             const T* A = stage_ainv ? Av : h.ainv;
-            for (int d = t >> 6; d < N; d += kCoarseThreads / 64) {
+            for (int d0 = 0; d0 < N; d0 += kCoarseThreads / 8) {  // uniform trip count: every lane shuffles
+                const int d = d0 + (t >> 3);
                 T acc = T(0);
-                for (int j = lane; j < N; j += 64) acc += A[(long)d * N + j] * rl[j];
+                if (d < N)
+                    for (int j = q; j < N; j += 8) acc += A[(long)d * N + j] * rl[j];
 #pragma unroll
-                for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-                if (lane == 0) Xv(L)[d] = acc;
+                for (int off = 4; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+                if (q == 0 && d < N) Xv(L)[d] = acc;
             }
         } else {
             for (long d = t; d < nL; d += kCoarseThreads) Xv(L)[d] = h.minv[L][d] * rl[d];
@@ -540,14 +647,9 @@ __global__ __launch_bounds__(kCoarseThreads) void k_vc_coarse(MlHierarchy<T> h, 
     }
     for (int l = L - 1; l >= s0; --l) {  // up: correction, post-smoothing
         const GridArgs gf = h.g[l], gc = h.g[l + 1];
-        for (long n = t; n < gf.nn; n += kCoarseThreads) {
-            T acc[3];
-            prolong_node(gc, (const T*)Xv(l + 1), gf, n, acc);
-            for (int c = 0; c < 3; ++c) {
-                const long d = 3 * n + c;
-                if (Sv(l)[d] != T(0)) Xv(l)[d] += acc[c];
-            }
-        }
+        for (int u = t; u < 3 * (int)gf.nn; u += kCoarseThreads)
+            if (Sv(l)[u] != T(0))
+                Xv(l)[u] += prolong_comp(gc, (const T*)Xv(l + 1), gf, (unsigned)u / 3u, (int)((unsigned)u % 3u));
         __syncthreads();
         apply(l, Xv(l), Tv(l), false);
         __syncthreads();
@@ -555,7 +657,8 @@ __global__ __launch_bounds__(kCoarseThreads) void k_vc_coarse(MlHierarchy<T> h, 
         for (long d = t; d < nl; d += kCoarseThreads) Xv(l)[d] += Sv(l)[d] * (Rv(l)[d] - Tv(l)[d]);
         __syncthreads();
     }
-    for (long i = t; i < 3 * h.g[s0].nn; i += kCoarseThreads) h.y[s0][i] = Xv(s0)[i];
+    T* const yout = h.y[s0] + (long)col * 3 * h.g[s0].nn;
+    for (long i = t; i < 3 * h.g[s0].nn; i += kCoarseThreads) yout[i] = Xv(s0)[i];
 }
 
 // r'z of the new residual.  mode 0 (an iteration): record ||r|| from the
@@ -563,9 +666,15 @@ __global__ __launch_bounds__(kCoarseThreads) void k_vc_coarse(MlHierarchy<T> h, 
 // rz = rz, then the convergence decision; 1 (start): rz = rz, beta = 0;
 // 2: park rz in scal[5] (reliable update)
 // r'r (mode 0): rr_partials[j * rr_stride], j < nb_rr
+// Column blockIdx.x: state st + column, partials at column * pcol and
+// rr_partials at column * rrcol.
 __global__ __launch_bounds__(NT) void k_fin_rz(PcgState* st, const double* partials, unsigned nb,
-                                               const double* rr_partials, int rr_stride, unsigned nb_rr, int mode) {
+                                               const double* rr_partials, int rr_stride, unsigned nb_rr, int mode,
+                                               long pcol, long rrcol) {
     __shared__ double red[NT / 64];
+    st += blockIdx.x;
+    partials += (long)blockIdx.x * pcol;
+    rr_partials += (long)blockIdx.x * rrcol;
     if (mode != 2 && st->done) return;  // parking follows a finished phase
     const double s = reduce_partials<NT>(partials, 1, 0, nb, red);
     double rr = 0.0;
@@ -838,7 +947,7 @@ void launch_ml_apply(const MlHierarchy<T>& h, const T* r, const T* minv0, T* z, 
                            minv0, T(1), z, st, partials);
     }
     hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(NT), 0, s, st_, partials, (unsigned)kProlongBlocks,
-                       (const double*)upd_partials + 1, 2, (unsigned)kVecBlocks, fin_mode);
+                       (const double*)upd_partials + 1, 2, (unsigned)kVecBlocks, fin_mode, 0L, 0L);
 }
 
 int ml_partial_blocks() { return kProlongBlocks; }
@@ -864,6 +973,30 @@ int vc_coarse_start(const MlHierarchy<T>& h) {
 }
 
 template <typename T>
+long vc_coarse_stage_values(const MlHierarchy<T>& h) {
+    if (h.levels == 0) return 0;
+    long n = 0;
+    for (int l = vc_coarse_start(h); l < h.levels; ++l) n += h.g[l].ne + 3 * h.g[l].nn;
+    return n;
+}
+template <typename T>
+int launch_vc_coarse_stage(MlHierarchy<T>& h, T* buf, hipStream_t s) {
+    h.cstage = nullptr;
+    h.cstage_n = vc_coarse_stage_values(h);
+    if (h.cstage_n == 0) return HX_OK;
+    long o = 0;
+    for (int l = vc_coarse_start(h); l < h.levels; ++l) {
+        if (hipMemcpyAsync(buf + o, h.Eop[l], h.g[l].ne * sizeof(T), hipMemcpyDeviceToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(buf + o + h.g[l].ne, h.sinv[l], 3 * h.g[l].nn * sizeof(T), hipMemcpyDeviceToDevice, s) !=
+                hipSuccess)
+            return HX_ERR_HIP;
+        o += h.g[l].ne + 3 * h.g[l].nn;
+    }
+    h.cstage = buf;
+    return HX_OK;
+}
+
+template <typename T>
 void launch_vc_smoother(const GridArgs& gc, const double* Ec, double h, const double* Kbin, const uint8_t* fixed_f,
                         int stride, const GridArgs& gfine, double omega, T* sinv, hipStream_t s) {
     hipLaunchKernelGGL((k_vc_l1<T>), dim3(blocks_for(gc.nn)), dim3(NT), 0, s, gc, Ec, h, Kbin, fixed_f, stride,
@@ -876,49 +1009,55 @@ void launch_vc_scale_E(long ne, const double* Ec, double h, T* Eop, hipStream_t 
 }
 
 template <typename T>
-int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const T* Kb, const MvShape& sh_pre,
-                    const MvShape& sh_post, const T* r, T* z, PcgState* st_, double* partials, long max_partials,
-                    int fin_mode, hipStream_t s, bool rr_post) {
+int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const MvShape& sh_pre, const MvShape& sh_post,
+                    const T* r, T* z, PcgState* st_, double* partials, long max_partials, int fin_mode, hipStream_t s,
+                    bool rr_post, int nrhs) {
     const int L = h.levels;
     if (rr_post && L == 0) return HX_ERR_BAD_ARG;  // no post-smoothing operator to form r'r
     if (L == 0) {  // no coarse level: plain Jacobi (the additive form's path)
+        if (nrhs != 1) return HX_ERR_BAD_ARG;
         launch_ml_apply<T>(h, r, h.minv[0], z, st_, partials, fin_mode, s);
         return HX_OK;
     }
     const int s0 = vc_coarse_start(h);
     const int c = std::min(s0, L);  // levels smoothed by multi-block launches
     PcgState* const st = fin_mode == 0 ? st_ : nullptr;
+    // the update's partials (2 kVecBlocks per column), then per column the
+    // fine post-smoothing operator's r'z and r'r partials (nb0 each)
     double* const upd_partials = partials;
-    double* const rz_partials = partials + 2 * kVecBlocks;
+    double* const rz_partials = partials + 2L * kVecBlocks * nrhs;
     const int kc0 = smoothing_post_kchunk(sh_post.tile, sh_post.kchunk, (int)sizeof(T));  // as the launch runs it
-    if (2L * kVecBlocks + 2L * matvec_parity_blocks(h.g[0], sh_post.tile, kc0) > max_partials) return HX_ERR_BAD_ARG;
+    if (2L * nrhs * (kVecBlocks + matvec_parity_blocks(h.g[0], sh_post.tile, kc0)) > max_partials)
+        return HX_ERR_BAD_ARG;
     auto level_r = [&](int l) { return l == 0 ? r : (const T*)h.r[l]; };
     // coarse levels: shallower z chunks until the launch has h.min_blocks
-    // workgroups (or one plane each), so the small grids are not latency-bound
+    // workgroups over all columns (or one plane each), so the small grids are
+    // not latency-bound
+    const int min_blocks = (h.min_blocks + nrhs - 1) / nrhs;
     auto level_shape = [&](int l, const MvShape& sh) {
         MvShape s2 = sh;
         if (l > 0)
-            while (s2.kchunk > 1 && matvec_parity_blocks(h.g[l], s2.tile, s2.kchunk) < h.min_blocks) s2.kchunk /= 2;
+            while (s2.kchunk > 1 && matvec_parity_blocks(h.g[l], s2.tile, s2.kchunk) < min_blocks) s2.kchunk /= 2;
         return s2;
     };
     auto restrict_to = [&](int l, bool scale) {  // R vres[l-1] -> r[l] (and y[l] = D_l^-1 r[l])
         launch_restrict<T>(h.g[l - 1], (const T*)h.vres[l - 1], h.g[l], h.r[l], scale ? (const T*)h.minv[l] : nullptr,
-                           scale ? h.y[l] : nullptr, T(scale ? h.w[l] : 1), st, h.min_blocks, s);
+                           scale ? h.y[l] : nullptr, T(scale ? h.w[l] : 1), st, h.min_blocks, s, nrhs);
     };
     for (int l = 0; l < c; ++l) {  // pre-smoothing x = S r, residual, restriction
         MvArgs<T> a{nullptr, h.vres[l], h.Eop[l], T(0), MV_PLAIN, st, nullptr, 3 * h.g[l].nn, level_r(l), h.sinv[l],
                     nullptr};
         a.form = 3;
-        launch_matvec_parity<T>(h.g[l], K, a, level_shape(l, sh_pre), s, 1);
+        launch_matvec_parity<T>(h.g[l], K, a, level_shape(l, sh_pre), s, nrhs);
         if (l + 1 < c) restrict_to(l + 1, false);
     }
-    if (s0 <= L) {  // levels s0..L in one workgroup, from vres[s0-1]
+    if (s0 <= L) {  // levels s0..L in one workgroup per column, from vres[s0-1]
         MlHierarchy<T> hc = h;
         hc.r[s0 - 1] = h.vres[s0 - 1];
         const bool stage = h.ainv != nullptr &&
                            vc_coarse_lds_values(h, s0, true) * (long)sizeof(T) <= kCoarseLdsBytes;
-        hipLaunchKernelGGL((k_vc_coarse<T>), dim3(1), dim3(kCoarseThreads),
-                           vc_coarse_lds_values(h, s0, stage) * sizeof(T), s, hc, s0, Kb, stage ? 1 : 0, st);
+        hipLaunchKernelGGL((k_vc_coarse<T>), dim3(1, nrhs), dim3(kCoarseThreads),
+                           vc_coarse_lds_values(h, s0, stage) * sizeof(T), s, hc, s0, K, stage ? 1 : 0, st);
     } else {  // no level fits it: the coarsest level by its diagonal
         restrict_to(L, true);
     }
@@ -931,17 +1070,19 @@ int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const T* Kb, c
         a.cNX = h.g[l + 1].NX;
         a.cNY = h.g[l + 1].NY;
         a.cNZ = h.g[l + 1].NZ;
-        const int nb = launch_matvec_parity<T>(h.g[l], K, a, level_shape(l, sh_post), s, 1);
+        a.ccs = 3 * h.g[l + 1].nn;
+        const int nb = launch_matvec_parity<T>(h.g[l], K, a, level_shape(l, sh_post), s, nrhs);
         if (l == 0) nb0 = nb;
     }
     // r'r from the update's partials (unmasked update) or from the fine
     // post-smoothing operator's, after its r'z (free update)
     if (rr_post)
-        hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(NT), 0, s, st_, rz_partials, (unsigned)nb0,
-                           (const double*)rz_partials + nb0, 1, (unsigned)nb0, fin_mode);
+        hipLaunchKernelGGL(k_fin_rz, dim3(nrhs), dim3(NT), 0, s, st_, rz_partials, (unsigned)nb0,
+                           (const double*)rz_partials + nb0, 1, (unsigned)nb0, fin_mode, 2L * nb0, 2L * nb0);
     else
-        hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(NT), 0, s, st_, rz_partials, (unsigned)nb0,
-                           (const double*)upd_partials + 1, 2, (unsigned)kVecBlocks, fin_mode);
+        hipLaunchKernelGGL(k_fin_rz, dim3(nrhs), dim3(NT), 0, s, st_, rz_partials, (unsigned)nb0,
+                           (const double*)upd_partials + 1, 2, (unsigned)kVecBlocks, fin_mode, 2L * nb0,
+                           2L * kVecBlocks);
     return HX_OK;
 }
 
@@ -993,11 +1134,14 @@ template void launch_vc_smoother<float>(const GridArgs&, const double*, double, 
                                         const GridArgs&, double, float*, hipStream_t);
 template void launch_vc_scale_E<double>(long, const double*, double, double*, hipStream_t);
 template void launch_vc_scale_E<float>(long, const double*, double, float*, hipStream_t);
-template int launch_vc_apply<double>(const MlHierarchy<double>&, const ParityK<double>&, const double*,
-                                     const MvShape&, const MvShape&, const double*, double*, PcgState*, double*,
-                                     long, int, hipStream_t, bool);
-template int launch_vc_apply<float>(const MlHierarchy<float>&, const ParityK<float>&, const float*, const MvShape&,
-                                    const MvShape&, const float*, float*, PcgState*, double*, long, int,
-                                    hipStream_t, bool);
+template long vc_coarse_stage_values<double>(const MlHierarchy<double>&);
+template long vc_coarse_stage_values<float>(const MlHierarchy<float>&);
+template int launch_vc_coarse_stage<double>(MlHierarchy<double>&, double*, hipStream_t);
+template int launch_vc_coarse_stage<float>(MlHierarchy<float>&, float*, hipStream_t);
+template int launch_vc_apply<double>(const MlHierarchy<double>&, const ParityK<double>&, const MvShape&,
+                                     const MvShape&, const double*, double*, PcgState*, double*, long, int,
+                                     hipStream_t, bool, int);
+template int launch_vc_apply<float>(const MlHierarchy<float>&, const ParityK<float>&, const MvShape&, const MvShape&,
+                                    const float*, float*, PcgState*, double*, long, int, hipStream_t, bool, int);
 
 }  // namespace hx

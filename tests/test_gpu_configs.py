@@ -10,8 +10,9 @@
 * 256x128x128 cantilever (12.8M DOFs), two TO iterations of the main form
   against the plain FP64 form on the device: compliance, physical densities
   and von Mises stress to 1e-6, the volume constraint held;
-* 128x64x64, eight load cases as one block solve: every column equal to its
-  single solve, the 1/8-weighted compliance, every column's true residual.
+* 128x64x64, eight load cases as one block solve (Jacobi and V-cycle): every
+  column equal to its single solve, the 1/8-weighted compliance, every
+  column's true residual.
 """
 import math
 import sys
@@ -126,20 +127,22 @@ def test_cfg2_main_form_against_plain_fp64(hx, ctx, wl):
         assert abs(x1.mean() - 0.15) <= 1e-8
 
 
-def test_cfg3_block8(ox, hx, ctx, wl):
+@pytest.mark.parametrize("precond", ["jacobi", "vcycle"])
+def test_cfg3_block8(ox, hx, ctx, wl, precond):
     L = hx._lib
+    pc = L.HX_PC_JACOBI if precond == "jacobi" else L.HX_PC_VCYCLE
     nx, ny, nz = 128, 64, 64
     fixed, _ = wl.cantilever(nx, ny, nz)
     ctx.set_grid(nx, ny, nz, 1.0, 1.0, 1.0, 0.3)
     ctx.set_fixed(fixed)
     F = wl.block_loads(nx, ny, nz, 8, seed=8)
     Ee = 1e-9 + wl.smoothed_density(ctx, nx * ny * nz, 3, 2.0) ** 3
-    U, S = ctx.block_pcg(Ee, F, tol=1e-8, precision=L.HX_FP32_IR)
+    U, S = ctx.block_pcg(Ee, F, tol=1e-8, precision=L.HX_FP32_IR, precond=pc)
     g = ox.Grid(nx, ny, nz)
     Ke = ox.ke_hex(1, 1, 1, 0.3)
     singles = []
     for c in range(8):
-        u, st = ctx.pcg(Ee, F[c], tol=1e-8, precision=L.HX_FP32_IR, raise_on_fail=True)
+        u, st = ctx.pcg(Ee, F[c], tol=1e-8, precision=L.HX_FP32_IR, precond=pc, raise_on_fail=True)
         singles.append(float(F[c] @ u))
         r = np.where(fixed == 0, F[c] - ox.matvec(g, Ke, Ee, U[c]), 0.0)
         true_rel = np.linalg.norm(r) / np.linalg.norm(F[c])

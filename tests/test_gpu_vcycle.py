@@ -155,16 +155,43 @@ def test_vc_simp_loop_matches_oracle(ox, hx, ctx, precision):
     assert np.allclose(ctx.to_read(L.HX_FIELD_XPHYS), xpo, rtol=1e-6, atol=1e-7)
 
 
-def test_vc_rejects_block_solves(ox, hx, ctx):
-    g, fixed, f = _setup(ox, ctx, (8, 4, 4))
+# Block solves: the columns share the hierarchy and run every V-cycle launch
+# with blockIdx.y = column (the one-workgroup pass as one workgroup per
+# column); each column must take its single solve's iterations and bits.
+# 16x8x8: levels in the one-workgroup pass only; 64x32x32: multi-block levels
+# first (whose coarse z chunks follow the column count).
+@pytest.mark.parametrize("dims", [(16, 8, 8), (64, 32, 32)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_vc_block_columns_equal_single_solves(ox, hx, ctx, dims, precision):
+    _say("vc block", dims, precision)
+    g, fixed, f = _setup(ox, ctx, dims)
+    rng = np.random.default_rng(31)
+    Ee = 1e-9 + rng.uniform(0.05, 1.0, g.ne) ** 3
+    F = np.stack([f, -2.0 * f, np.where(fixed == 0, rng.standard_normal(g.ndof), 0.0)])
+    VC = hx._lib.HX_PC_VCYCLE
+    U, S = ctx.block_pcg(Ee, F, tol=1e-10, precision=precision, precond=VC)
+    Ke = ox.ke_hex(1, 1, 1, 0.3)
+    for c in range(F.shape[0]):
+        u, s = ctx.pcg(Ee, F[c], tol=1e-10, precision=precision, precond=VC)
+        r = np.where(fixed == 0, F[c] - ox.matvec(g, Ke, Ee, U[c]), 0.0)
+        true_rel = np.linalg.norm(r) / np.linalg.norm(F[c])
+        _say("  column", c, "its", S[c].iterations, s.iterations, "true rel", true_rel)
+        assert S[c].status == 0 and S[c].iterations == s.iterations
+        assert np.array_equal(U[c], u)
+        assert true_rel < 1.05e-10
+
+
+def test_vc_block_needs_a_coarse_level(ox, hx, ctx):
+    g, fixed, f = _setup(ox, ctx, (7, 4, 4))  # odd element count: no coarse level
     Ee = np.ones(g.ne)
     F = np.stack([f, f])
-    o = hx._lib.PcgOpts(1e-8, 100, 0.0, 0, hx._lib.HX_PC_VCYCLE, 0, 0, 0)
-    S = (hx._lib.PcgStats * 2)()
-    U = np.zeros_like(F)
-    rc = ctx.L.hx_block_pcg_solve(ctx.h, 2, hx._lib.dptr(Ee), hx._lib.dptr(F), hx._lib.dptr(U),
-                                  hx._lib.C.byref(o), S)
-    assert rc == hx._lib.HX_ERR_BAD_ARG
+    L = hx._lib
+    for pc in (L.HX_PC_VCYCLE, L.HX_PC_MULTILEVEL):
+        o = L.PcgOpts(1e-8, 100, 0.0, 0, pc, 0, 0, 0)
+        S = (L.PcgStats * 2)()
+        U = np.zeros_like(F)
+        rc = ctx.L.hx_block_pcg_solve(ctx.h, 2, L.dptr(Ee), L.dptr(F), L.dptr(U), L.C.byref(o), S)
+        assert rc == L.HX_ERR_BAD_ARG
 
 
 def test_vc_smoothing_timing_hook(ox, hx, ctx):
