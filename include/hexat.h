@@ -164,9 +164,12 @@ int hx_solve_system(hx_ctx* ctx, const double* density, const double* f, double*
                     const hx_solve_opts* opts, hx_solve_stats* stats);
 
 /* nrhs independent systems with one shared operator, solved together
- * (fused multi-RHS kernels).  F, U are nrhs x ndof row-major (RHS-major).
- * stats: nrhs entries.  No reference counterpart (the reference solves one
- * load case per call, Main.jl:414-418). */
+ * (fused multi-RHS kernels; blockIdx.y = column).  F, U are nrhs x ndof
+ * row-major (RHS-major).  stats: nrhs entries.  Preconditioners: Jacobi, or
+ * the V-cycle on grids with a coarse level (every V-cycle launch carries the
+ * columns; each column takes its single solve's iterations and bits); the
+ * additive multilevel and learned ones take one column.  No reference
+ * counterpart (the reference solves one load case per call, Main.jl:414-418). */
 int hx_block_pcg_solve(hx_ctx* ctx, int nrhs, const double* Ee, const double* F, double* U,
                        const hx_pcg_opts* opts, hx_pcg_stats* stats);
 

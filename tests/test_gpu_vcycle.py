@@ -181,12 +181,14 @@ def test_vc_block_columns_equal_single_solves(ox, hx, ctx, dims, precision):
         assert true_rel < 1.05e-10
 
 
-def test_vc_block_needs_a_coarse_level(ox, hx, ctx):
+def test_block_solve_rejects_what_it_cannot_run(ox, hx, ctx):
+    """V-cycle on a grid without a coarse level; additive multilevel and
+    learned preconditioners with more than one column."""
     g, fixed, f = _setup(ox, ctx, (7, 4, 4))  # odd element count: no coarse level
     Ee = np.ones(g.ne)
     F = np.stack([f, f])
     L = hx._lib
-    for pc in (L.HX_PC_VCYCLE, L.HX_PC_MULTILEVEL):
+    for pc in (L.HX_PC_VCYCLE, L.HX_PC_MULTILEVEL, L.HX_PC_LEARNED):
         o = L.PcgOpts(1e-8, 100, 0.0, 0, pc, 0, 0, 0)
         S = (L.PcgStats * 2)()
         U = np.zeros_like(F)
