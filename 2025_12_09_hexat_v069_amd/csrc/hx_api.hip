@@ -1525,9 +1525,9 @@ int hx_block_pcg_solve(hx_ctx* c, int nrhs, const double* Ee, const double* F, d
     if (!c || !Ee || !F || !U) return HX_ERR_BAD_ARG;
     if (int rc = ensure_grid(c)) return rc;
     if (nrhs < 1 || nrhs > kMaxRhs) return fail(c, HX_ERR_BAD_ARG, "nrhs must be 1..8");
-    if (int rc = check_opts(c, o)) return rc;
-    if ((o->precond == HX_PC_MULTILEVEL || o->precond == HX_PC_LEARNED) && nrhs > 1)
+    if (o && (o->precond == HX_PC_MULTILEVEL || o->precond == HX_PC_LEARNED) && nrhs > 1)
         return fail(c, HX_ERR_BAD_ARG, "the additive multilevel and learned preconditioners take one right-hand side");
+    if (int rc = check_opts(c, o)) return rc;
     if (o->precond == HX_PC_VCYCLE && nrhs > 1 && ml_levels_for(c->g, std::min(c->ml_max_levels, kMlMaxLevels)) < 1)
         return fail(c, HX_ERR_BAD_ARG, "a V-cycle block solve needs a coarse level (even element counts)");
     const long n = ndof(c), nt = n * nrhs;

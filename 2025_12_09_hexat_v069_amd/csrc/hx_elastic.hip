@@ -463,11 +463,14 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
             }
             q.e = a.E[(long)min(max(k - 1, 0), g.nz - 1) * g.nx * g.ny + eoff];
         };
-        // park a prefetched plane in LDS; sets this thread's node value and
-        // the modulus of its element in layer q.k - 1
-        T xnew[3], ee;
-        T rnew[SMOOTH ? 3 : 1], mnew[SMOOTH ? 3 : 1];  // r and S of this thread's node (smoothing forms)
-        auto plane_store = [&](const Slot& q) __attribute__((always_inline)) {
+        // park a prefetched plane in LDS; sets this thread's node value (and,
+        // smoothing forms, its r and S) in xnew, rnew, mnew and the modulus of
+        // its element in layer q.k - 1.  The node values of two planes live
+        // in the two register sets of Xn, Rn, Mn, whose roles alternate with
+        // the layer parity (no copies between layers).
+        T ee;
+        T Xn[2][3], Rn[2][SMOOTH ? 3 : 1], Mn[2][SMOOTH ? 3 : 1];
+        auto plane_store = [&](const Slot& q, T* xnew, T* rnew, T* mnew) __attribute__((always_inline)) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 T v;
@@ -510,8 +513,7 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
             }
         };
 
-        T F0[12], F1[12], carry[12], xo[3];
-        T ro[SMOOTH ? 3 : 1], mo[SMOOTH ? 3 : 1];  // r and S of the plane being finished
+        T F0[12], F1[12], carry[12];
         // Node results of plane ko are parked in LDS (so) at the end of a
         // layer and written out after the next layer's first barrier, ahead of
         // its prefetch, as row-contiguous runs of 3*(TX-1) values: a lane
@@ -533,9 +535,15 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
             }
         };
         // element layer ke: Flo holds the face modes of plane ke, Fhi receives
-        // plane ke+1's, which sits in slot q; plane ke+1+PD is requested into q
-        auto step = [&](int ke, const T* Flo, T* Fhi, Slot& q) __attribute__((always_inline)) {
-            plane_store(q);  // plane ke+1, E of layer ke
+        // plane ke+1's, which sits in slot q; plane ke+1+PD is requested into
+        // q.  Register set `cur` holds plane ke's node values, `nxt` receives
+        // plane ke+1's.
+        auto step = [&](int ke, const T* Flo, T* Fhi, Slot& q, int cur) __attribute__((always_inline)) {
+            const int nxt = cur ^ 1;
+            const T* xo = Xn[cur];
+            [[maybe_unused]] const T* ro = Rn[cur];
+            [[maybe_unused]] const T* mo = Mn[cur];
+            plane_store(q, Xn[nxt], Rn[nxt], Mn[nxt]);  // plane ke+1, E of layer ke
             __syncthreads();
             flush();
             asm volatile("" ::: "memory");
@@ -543,12 +551,14 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
             asm volatile("" ::: "memory");  // keep the prefetch ahead of this layer's arithmetic
             face_modes(Fhi);
 
-            // element (gi, gj, ke): modes, contraction (E folded into xh), z-inverse
+            // element (gi, gj, ke): modes, contraction, z-inverse; E scales the
+            // face totals (12 products and 12 fused ones) rather than the
+            // modes (21 products)
             T xh[24];
 #pragma unroll
             for (int i = 0; i < 12; ++i) {
-                xh[i] = ee * (Flo[i] + Fhi[i]);
-                xh[12 + i] = ee * (Fhi[i] - Flo[i]);
+                xh[i] = Flo[i] + Fhi[i];
+                xh[12 + i] = Fhi[i] - Flo[i];
             }
             T yh[24];
 #pragma unroll
@@ -560,8 +570,8 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
             T H[12];
 #pragma unroll
             for (int i = 0; i < 12; ++i) {
-                H[i] = carry[i] + (yh[i] - yh[12 + i]);
-                carry[i] = yh[i] + yh[12 + i];
+                H[i] = fma(ee, yh[i] - yh[12 + i], carry[i]);
+                carry[i] = ee * (yh[i] + yh[12 + i]);
             }
             // y-inverse: row ty (A0,B0) stays, row ty+1 (A1,B1) goes to the face above in y
             T eA[3], eB[3];
@@ -584,7 +594,7 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
             const int so_i = (ty - 1) * OROW + 3 * (tx - 1);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const T yv = node[c] + a.shift * xo[c];
+                const T yv = SMOOTH ? node[c] : node[c] + a.shift * xo[c];  // the level operators carry no shift
                 T out = yv;
                 if constexpr (FUSE == 3) out = mo[c] != T(0) ? ro[c] - yv : T(0);  // residual after x = S r
                 else if constexpr (FUSE == 4) out = xo[c] + mo[c] * (ro[c] - yv);  // x + S (r - K x)
@@ -600,11 +610,6 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
                 } else if constexpr (FUSE != 3) {  // pre-smoothing reduces nothing
                     if (ke >= k0 && own) pq += (double)xo[c] * (double)yv;
                 }
-                xo[c] = xnew[c];
-                if constexpr (SMOOTH) {
-                    ro[c] = rnew[c];
-                    mo[c] = mnew[c];
-                }
             }
         };
 
@@ -613,15 +618,7 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
             coarse_fill();
             __syncthreads();
         }
-        plane_store(sl[0]);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            xo[c] = xnew[c];
-            if constexpr (SMOOTH) {
-                ro[c] = rnew[c];
-                mo[c] = mnew[c];
-            }
-        }
+        plane_store(sl[0], Xn[0], Rn[0], Mn[0]);  // plane k0-1: the first layer's `cur`
         __syncthreads();
         face_modes(F0);
         // plane k0+j sits in slot j % PD
@@ -633,8 +630,8 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
         // two layers per trip (face-mode ping-pong); with PD = 2 the slot
         // index then follows the layer parity as well
         for (int ke = k0 - 1; ke < k1; ke += 2) {
-            step(ke, F0, F1, sl[0]);
-            if (ke + 1 < k1) step(ke + 1, F1, F0, sl[PD - 1]);
+            step(ke, F0, F1, sl[0], 0);
+            if (ke + 1 < k1) step(ke + 1, F1, F0, sl[PD - 1], 1);
         }
         __syncthreads();
         flush();
