@@ -73,6 +73,36 @@ def other_workload(args, hx, L, ctx):
         out["compliance_rel_diff"] = abs(out["plain"]["compliance"] - out["precision"]["compliance"]) / abs(
             out["plain"]["compliance"])
         out["config"] = "half MBB 192x96x96 (5,447,811 DOFs), smoothed random density, SIMP p=3, tol 1e-8"
+    elif args.workload == "cfg4":
+        # the deployment size of the learned-preconditioner configuration,
+        # solver only: the main form against plain FP64 Jacobi
+        nx, ny, nz = 320, 160, 160
+        fixed, f = cantilever(nx, ny, nz)
+        ctx.set_grid(nx, ny, nz, 1.0, 1.0, 1.0, 0.3)
+        ctx.set_fixed(fixed)
+        ctx.tune_set(ctx.autotune(1))
+        Ee = 1e-9 + smoothed_density(ctx, nx * ny * nz, 4, 2.0) ** 3
+        res = {}
+        for name, prec, pc in (("precision_vc", L.HX_FP32_IR, L.HX_PC_VCYCLE), ("plain", L.HX_FP64, L.HX_PC_JACOBI)):
+            ctx.pcg(Ee, f, tol=1e-8, precision=prec, precond=pc)  # warm-up (graphs)
+            t0 = time.perf_counter()
+            u, st = ctx.pcg(Ee, f, tol=1e-8, precision=prec, precond=pc)
+            res[name] = (time.perf_counter() - t0, st, u)
+            log(f"  cfg4 {name}: {res[name][0]:.4f} s, {st.iterations} iterations, rel {st.rel_res:.3e}")
+        (tm, sm, um), (tp, sp, up) = res["precision_vc"], res["plain"]
+        cm, cp = float(f @ um), float(f @ up)
+        out = {"metric": "s per solve at 24.96M DOFs (320x160x160, tol 1e-8)", "value": round(tm, 5), "unit": "s",
+               "n_gpus": 1, "higher_is_better": False, "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+               "config": {"workload": "320x160x160 cantilever (8,192,000 elements, 24,961,923 DOFs), smoothed random "
+                                      "density (seed 4, r 2), SIMP p=3, solver only, tol 1e-8; main form FP32 "
+                                      "operator and vectors with FP64 reliable updates, V-cycle"},
+               "iterations": sm.iterations, "rel_res": sm.rel_res,
+               "plain_fp64": {"s": round(tp, 5), "iterations": sp.iterations, "rel_res": sp.rel_res},
+               "speedup_vs_plain_fp64": round(tp / tm, 2),
+               "compliance_rel_diff": abs(cm - cp) / cp,
+               "displacement_rel_diff": float(np.linalg.norm(um - up) / np.linalg.norm(up)),
+               "learned_preconditioner": "not run at this size: the learned flexible CG takes more iterations "
+                                         "than Jacobi on the problems it was trained on"}
     elif args.workload == "block8":
         # 8 load cases in one block solve (every launch carries the 8 columns;
         # the V-cycle's one-workgroup pass runs one workgroup per column)
@@ -462,9 +492,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-pcg-iters", type=int, default=20)
     ap.add_argument("--cpu-filter-iters", type=int, default=20)
-    ap.add_argument("--workload", choices=("cantilever", "mbb", "block8"), default="cantilever",
-                    help="cantilever: the headline TO-iteration bench; mbb / block8: solver-only measurements of "
-                         "the precision and batching configurations (their own JSON line)")
+    ap.add_argument("--workload", choices=("cantilever", "mbb", "block8", "cfg4"), default="cantilever",
+                    help="cantilever: the headline TO-iteration bench; mbb / block8 / cfg4: solver-only "
+                         "measurements of the precision (192x96x96), batching (8 load cases, 128x64x64) and "
+                         "large-grid (320x160x160) configurations, each its own JSON line")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes (roofline.traffic = null)")
     ap.add_argument("--kernel-sample", choices=("tune",) + tuple(FORMS), default=None,

@@ -12,7 +12,11 @@
   and von Mises stress to 1e-6, the volume constraint held;
 * 128x64x64, eight load cases as one block solve (Jacobi and V-cycle): every
   column equal to its single solve, the 1/8-weighted compliance, every
-  column's true residual.
+  column's true residual;
+* 320x160x160 (24.96M DOFs), the deployment size of the learned-preconditioner
+  configuration, solver only at tolerance 1e-8 on a seeded smoothed SIMP
+  density: the main form (FP32 contraction, V-cycle) against plain FP64
+  Jacobi, both FP64 true residuals, compliance and displacements.
 """
 import math
 import sys
@@ -151,3 +155,24 @@ def test_cfg3_block8(ox, hx, ctx, wl, precond):
         assert np.array_equal(U[c], u)
     wc = sum(float(F[c] @ U[c]) for c in range(8)) / 8
     assert abs(wc - sum(singles) / 8) <= 1e-12 * abs(wc)
+
+
+def test_cfg4_320x160x160_main_form_against_plain_fp64(hx, ctx, wl):
+    L = hx._lib
+    nx, ny, nz = 320, 160, 160
+    fixed, f = wl.cantilever(nx, ny, nz)
+    ctx.set_grid(nx, ny, nz, 1.0, 1.0, 1.0, 0.3)
+    ctx.set_fixed(fixed)
+    Ee = 1e-9 + wl.smoothed_density(ctx, nx * ny * nz, 4, 2.0) ** 3
+    _say("cfg4 main form")
+    u1, s1 = ctx.pcg(Ee, f, tol=1e-8, precision=L.HX_FP32_IR, precond=L.HX_PC_VCYCLE, raise_on_fail=True)
+    _say("  its", s1.iterations, "rel", s1.rel_res, "t", s1.seconds)
+    _say("cfg4 plain FP64 Jacobi")
+    u0, s0 = ctx.pcg(Ee, f, tol=1e-8, precision=L.HX_FP64, precond=L.HX_PC_JACOBI, raise_on_fail=True)
+    _say("  its", s0.iterations, "rel", s0.rel_res, "t", s0.seconds)
+    c0, c1 = float(f @ u0), float(f @ u1)
+    du = np.linalg.norm(u1 - u0) / np.linalg.norm(u0)
+    _say("  dc", abs(c1 - c0) / c0, "du", du)
+    assert s1.rel_res < 1e-8 and s0.rel_res < 1e-8  # FP64 true residuals
+    assert abs(c1 - c0) <= 1e-6 * c0
+    assert du <= 1e-4
