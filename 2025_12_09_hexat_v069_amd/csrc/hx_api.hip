@@ -140,7 +140,7 @@ struct hx_ctx {
     // block (multi-RHS) buffers, nrhs * ndof each
     DevBuf<double> b_f, b_x, b_r, b_p, b_q, b_p2;
     DevBuf<float> b_x32, b_r32, b_p32, b_q32, b_p32b;
-    DevBuf<double> h_b, h_x, h_r, h_p, h_q, h_minv, h_in, h_out;
+    DevBuf<double> h_b, h_x, h_r, h_p, h_q, h_minv, h_in, h_out, h_p2;
     DevBuf<double> e_pr, e_vm, e_sv, e_l1, e_dir, e_ce, e_dc, e_aux, small;
     DevBuf<float> w_rho, w_l1, w_pr, w_cent, w_rho_out, w_sl1;  // hx_web_pack
     DevBuf<unsigned> w_cnt, w_offs;
@@ -1000,6 +1000,10 @@ int helm_solve(hx_ctx* c, double R, const hx_filter_opts& o, const double* in_de
     scalar_matrices(c->g.dx, c->g.dy, c->g.dz, o.quirk, Ks, Ms);
     HelmArgs A;
     for (int t = 0; t < 64; ++t) A.Ae[t] = R * R * Ks[t] + Ms[t];  // GPUHelmholtz.jl:128-129
+    A.h[0] = c->g.dx;  // the box scalar_matrices integrated over (the quirk's is 1.5 dz tall)
+    A.h[1] = c->g.dy;
+    A.h[2] = o.quirk ? 1.5 * c->g.dz : c->g.dz;
+    A.R2 = R * R;
     const double vol8 = c->g.dx * c->g.dy * c->g.dz / 8.0;
     launch_helm_rhs(c->g, in_dev, vol8, c->h_b.p, c->stream);
     launch_helm_minv(c->g, A, c->h_minv.p, c->stream);
@@ -1030,21 +1034,32 @@ int helm_solve(hx_ctx* c, double R, const hx_filter_opts& o, const double* in_de
         std::memcpy(&bits, &A.Ae[t], sizeof bits);
         for (int q = 0; q < 8; ++q) ah = (ah ^ ((bits >> (8 * q)) & 0xff)) * 1099511628211ull;
     }
-    char key[192];
-    std::snprintf(key, sizeof key, "helm %d %d %d %llx %p %p %p %p %p", c->g.nx, c->g.ny, c->g.nz, ah,
-                  (void*)c->h_x.p, (void*)c->h_r.p, (void*)c->h_p.p, (void*)c->h_q.p, (void*)c->h_minv.p);
+    // the sum-factorised operator forms p = minv r + beta p_old itself from a
+    // second p buffer (batches even: p returns to h_p); otherwise the gather
+    // form and a separate p-update
+    const bool fused = helm_sf_available(A);
+    if (fused) HXA(c->h_p2, nn);
+    char key[224];
+    std::snprintf(key, sizeof key, "helm %d %d %d %d %llx %p %p %p %p %p %p", c->g.nx, c->g.ny, c->g.nz, fused ? 1 : 0,
+                  ah, (void*)c->h_x.p, (void*)c->h_r.p, (void*)c->h_p.p, (void*)c->h_q.p, (void*)c->h_minv.p,
+                  fused ? (void*)c->h_p2.p : nullptr);
     double* const op_partials = c->partials.p + kOpPartials;
     while (!c->st_host->done) {
         rc = run_batch(c, key, [&] {
             for (int b = 0; b < batch; ++b) {
                 int npq;
+                double* pin = (fused && (b & 1)) ? c->h_p2.p : c->h_p.p;
+                double* pout = (fused && !(b & 1)) ? c->h_p2.p : c->h_p.p;
                 {
                     TimeScope ts(c, HX_K_HELM);
-                    npq = launch_helm_matvec(c->g, A, c->h_p.p, c->h_q.p, MV_PCG, c->st, op_partials, c->stream);
+                    npq = fused ? launch_helm_matvec_fused(c->g, A, pin, c->h_r.p, c->h_minv.p, pout, c->h_q.p, c->st,
+                                                           op_partials, c->stream)
+                                : launch_helm_matvec(c->g, A, c->h_p.p, c->h_q.p, MV_PCG, c->st, op_partials,
+                                                     c->stream);
                 }
-                launch_pcg_update<double>(nn, c->h_x.p, c->h_p.p, c->h_r.p, c->h_q.p, c->h_minv.p, c->st,
-                                          c->partials.p, c->stream, 1, true, op_partials, npq);
-                launch_pcg_pupdate<double>(nn, c->h_r.p, c->h_minv.p, c->h_p.p, c->st, c->stream);
+                launch_pcg_update<double>(nn, c->h_x.p, pout, c->h_r.p, c->h_q.p, c->h_minv.p, c->st, c->partials.p,
+                                          c->stream, 1, true, op_partials, npq);
+                if (!fused) launch_pcg_pupdate<double>(nn, c->h_r.p, c->h_minv.p, c->h_p.p, c->st, c->stream);
             }
         });
         if (rc) return rc;
@@ -1178,7 +1193,7 @@ void hx_ctx_destroy(hx_ctx* c) {
     if (c->unet) unet_destroy(c->unet);
     for (auto e : c->free_events) (void)hipEventDestroy(e);
     DevBuf<double>* d[] = {&c->E, &c->f, &c->x, &c->r, &c->p, &c->q, &c->minv, &c->Kbin_d, &c->partials,
-                           &c->h_b, &c->h_x, &c->h_r, &c->h_p, &c->h_q, &c->h_minv, &c->h_in, &c->h_out,
+                           &c->h_b, &c->h_x, &c->h_r, &c->h_p, &c->h_q, &c->h_minv, &c->h_in, &c->h_out, &c->h_p2,
                            &c->e_pr, &c->e_vm, &c->e_sv, &c->e_l1, &c->e_dir, &c->e_ce, &c->e_dc, &c->e_aux,
                            &c->small, &c->t_x, &c->t_xphys, &c->t_xnew, &c->t_dc, &c->t_dcf, &c->t_dv, &c->t_vm,
                            &c->t_ones, &c->b_f, &c->b_x, &c->b_r, &c->b_p, &c->b_q, &c->b_p2, &c->p2, &c->l_z, &c->l_zold,

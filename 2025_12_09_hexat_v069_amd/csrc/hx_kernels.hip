@@ -597,6 +597,127 @@ __global__ __launch_bounds__(NT) void k_helm_matvec(GridArgs g, HelmArgs A, Helm
     }
 }
 
+// Sum-factorised form of the same operator.  Every element of the filter
+// has the same box matrix R^2 Ks + Ms, and on a box Ks = Kx(x)My(x)Mz +
+// Mx(x)Ky(x)Mz + Mx(x)My(x)Kz, Ms = Mx(x)My(x)Mz with the 1-D element
+// matrices K = k [1 -1; -1 1], M = m [2 1; 1 2] (k = 1/h, m = h/6), so the
+// assembled operator factorises per axis:
+//   y = Kx U + Mx W,  U = R^2 My a,  W = (R^2 Ky + My) a + R^2 My b,
+//   a = Mz x,  b = Kz x,
+// each factor a 3-point stencil whose end entries vanish where the element
+// between two nodes lies outside the grid.  A thread owns a node column and
+// marches z (a, b from its own column's three planes); row neighbours'
+// contributions P = a and Q = (m_y - R^2 k_y) a + R^2 m_y b come through LDS
+// (double-buffered: one barrier per plane), the x neighbours' S = -k_x U +
+// m_x W through a DPP lane shift (a tile row is one wave).  ~22 FMAs per node
+// and plane instead of 64, one coalesced load per plane instead of 9.
+// FUSE: the PCG's p = minv r + beta p_old is formed from the planes staged
+// anyway and written to p_out (p_old in x), as the elasticity operator does.
+struct HelmSF {
+    double kx, mx, ky, my, kz, mz, R2;
+};
+constexpr int kHsfX = 64, kHsfY = 16, kHsfKc = 16;  // tile: one wave per row; 62 x 14 owned node columns
+__device__ __forceinline__ double wave_from_left(double v) {  // lane l <- lane l-1 (lane 0 gets 0)
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_from_right(double v) {  // lane l <- lane l+1 (lane 63 gets 0)
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+template <bool FUSE>
+__global__ __launch_bounds__(kHsfX * kHsfY) void k_helm_sf(GridArgs g, HelmSF c, const double* __restrict__ x,
+                                                          const double* __restrict__ r,
+                                                          const double* __restrict__ minv, double* __restrict__ p_out,
+                                                          double* __restrict__ y, int mode, PcgState* st,
+                                                          double* partials, int ntx, int nty) {
+    constexpr int NT2 = kHsfX * kHsfY;
+    __shared__ double PQ[2][2][kHsfY][kHsfX];  // [buffer][P, Q][row][column]
+    __shared__ double red[NT2 / 64];
+    if (mode == MV_PCG && st->done) return;
+    const double beta = FUSE ? st->beta : 0.0;
+    const int tx = threadIdx.x % kHsfX, ty = threadIdx.x / kHsfX;
+    const int bx = blockIdx.x % ntx, by = (blockIdx.x / ntx) % nty, bz = blockIdx.x / (ntx * nty);
+    const int i = bx * (kHsfX - 2) - 1 + tx, j = by * (kHsfY - 2) - 1 + ty;
+    const int k0 = bz * kHsfKc, k1 = min(k0 + kHsfKc, g.NZ);
+    const bool own = tx >= 1 && tx <= kHsfX - 2 && ty >= 1 && ty <= kHsfY - 2 && i < g.NX && j < g.NY;
+    // per-axis stencil flags: the element on the low / high side of the node exists
+    const double nlx = (i >= 1 && i <= g.NX - 1) ? 1.0 : 0.0, nrx = (i >= 0 && i <= g.NX - 2) ? 1.0 : 0.0;
+    const double nly = (j >= 1 && j <= g.NY - 1) ? 1.0 : 0.0, nry = (j >= 0 && j <= g.NY - 2) ? 1.0 : 0.0;
+    const double kxc = c.kx * (nlx + nrx), mxc = 2.0 * c.mx * (nlx + nrx);
+    const double kyc = c.ky * (nly + nry), myc = 2.0 * c.my * (nly + nry);
+    const double qa = c.my - c.R2 * c.ky, qb = c.R2 * c.my;  // Q = qa a + qb b
+    const double wa = c.R2 * kyc + myc, wb = c.R2 * myc;     // centre of W
+    const double ua = c.R2 * myc, un = c.R2 * c.my;          // U = ua a + un (neighbours' a)
+    const long col = (long)min(max(j, 0), g.NY - 1) * g.NX + min(max(i, 0), g.NX - 1);
+    const long plane = (long)g.NX * g.NY;
+    // the staged value of plane k at this column: x, or the fused p; the raw
+    // values of a plane are requested two planes ahead of their use
+    struct Raw {
+        double x, r, m;
+    };
+    auto request = [&](int k) __attribute__((always_inline)) -> Raw {
+        const long o = (long)min(max(k, 0), g.NZ - 1) * plane + col;
+        Raw q;
+        q.x = x[o];
+        if constexpr (FUSE) {
+            q.r = r[o];
+            q.m = minv[o];
+        } else {
+            q.r = q.m = 0.0;
+        }
+        return q;
+    };
+    auto staged = [&](const Raw& q) __attribute__((always_inline)) -> double {
+        if constexpr (FUSE) return q.m * q.r + beta * q.x;
+        else return q.x;
+    };
+    double v0 = staged(request(k0 - 1)), v1 = staged(request(k0));
+    Raw n1 = request(k0 + 1), n2 = request(k0 + 2);
+    double pq = 0.0;
+    int buf = 0;
+    for (int k = k0; k < k1; ++k) {
+        const double v2 = staged(n1);  // plane k+1, requested two planes ago
+        n1 = n2;
+        n2 = request(k + 3);  // past the grid: re-reads the last plane (a cache hit)
+        if constexpr (FUSE) {
+            if (own) p_out[(long)k * plane + col] = v1;
+        }
+        // z-pass at plane k (wave-uniform coefficients)
+        const double nlz = k >= 1 ? 1.0 : 0.0, nrz = k <= g.NZ - 2 ? 1.0 : 0.0;
+        const double a = c.mz * (nlz * v0 + nrz * v2) + 2.0 * c.mz * (nlz + nrz) * v1;
+        const double b = -c.kz * (nlz * v0 + nrz * v2) + c.kz * (nlz + nrz) * v1;
+        PQ[buf][0][ty][tx] = a;
+        PQ[buf][1][ty][tx] = qa * a + qb * b;
+        __syncthreads();
+        const int ylo = ty >= 1 ? ty - 1 : ty, yhi = ty + 1 < kHsfY ? ty + 1 : ty;  // halo rows: unused results
+        const double U = ua * a + un * (nly * PQ[buf][0][ylo][tx] + nry * PQ[buf][0][yhi][tx]);
+        const double W = wa * a + wb * b + nly * PQ[buf][1][ylo][tx] + nry * PQ[buf][1][yhi][tx];
+        // x-pass: this node's share of its x neighbours, moved by one lane
+        const double S = -c.kx * U + c.mx * W;
+        const double yv = kxc * U + mxc * W + nlx * wave_from_left(S) + nrx * wave_from_right(S);
+        if (own) {
+            y[(long)k * plane + col] = yv;
+            pq += v1 * yv;
+        }
+        v0 = v1;
+        v1 = v2;
+        buf ^= 1;
+    }
+    if (mode == MV_PCG) {
+        const double vsum = block_sum<NT2>(pq, red);
+        if (threadIdx.x == 0) partials[blockIdx.x] = vsum;
+    }
+}
+
+inline int helm_sf_tiles(const GridArgs& g, int* ntx, int* nty) {
+    *ntx = (g.NX + kHsfX - 3) / (kHsfX - 2);
+    *nty = (g.NY + kHsfY - 3) / (kHsfY - 2);
+    return *ntx * *nty * ((g.NZ + kHsfKc - 1) / kHsfKc);
+}
+
 inline int helm_tiles(const GridArgs& g, int* ntx, int* nty) {
     *ntx = (g.NX + kHx - 1) / kHx;
     *nty = (g.NY + kHy - 1) / kHy;
@@ -634,51 +755,73 @@ __global__ __launch_bounds__(NT) void k_helm_extract(GridArgs g, const double* _
 }
 
 // ------------------------------------------------------------ element post
-// cyclic Jacobi for a symmetric 3x3 (A row-major); vals descending, V columns
-__device__ void eig3_jacobi(double A[3][3], double vals[3], double vec[3]) {
+// cyclic Jacobi for a symmetric 3x3 (A row-major); vals descending, V columns.
+// The rotations of one sweep, (0,1), (0,2), (1,2), are written out with
+// constant indices, so A and V stay in registers.
+template <int P, int Q>
+__device__ __forceinline__ void jacobi_rot(double (&A)[3][3], double (&V)[3][3]) {
+    if (A[P][Q] == 0.0) return;
+    const double th = (A[Q][Q] - A[P][P]) / (2.0 * A[P][Q]);
+    const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+    const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double akp = A[k][P], akq = A[k][Q];
+        A[k][P] = c * akp - sn * akq;
+        A[k][Q] = sn * akp + c * akq;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double apk = A[P][k], aqk = A[Q][k];
+        A[P][k] = c * apk - sn * aqk;
+        A[Q][k] = sn * apk + c * aqk;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double vkp = V[k][P], vkq = V[k][Q];
+        V[k][P] = c * vkp - sn * vkq;
+        V[k][Q] = sn * vkp + c * vkq;
+    }
+}
+__device__ void eig3_jacobi(double (&A)[3][3], double vals[3], double vec[3]) {
     double V[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
     for (int sweep = 0; sweep < 50; ++sweep) {
         const double off = fabs(A[0][1]) + fabs(A[0][2]) + fabs(A[1][2]);
         const double scale = fabs(A[0][0]) + fabs(A[1][1]) + fabs(A[2][2]) + off;
         if (off <= 1e-300 || off <= 1e-17 * scale) break;
-        for (int p = 0; p < 2; ++p)
-            for (int q = p + 1; q < 3; ++q) {
-                if (A[p][q] == 0.0) continue;
-                const double th = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
-                const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-                for (int k = 0; k < 3; ++k) {
-                    const double akp = A[k][p], akq = A[k][q];
-                    A[k][p] = c * akp - s * akq;
-                    A[k][q] = s * akp + c * akq;
-                }
-                for (int k = 0; k < 3; ++k) {
-                    const double apk = A[p][k], aqk = A[q][k];
-                    A[p][k] = c * apk - s * aqk;
-                    A[q][k] = s * apk + c * aqk;
-                }
-                for (int k = 0; k < 3; ++k) {
-                    const double vkp = V[k][p], vkq = V[k][q];
-                    V[k][p] = c * vkp - s * vkq;
-                    V[k][q] = s * vkp + c * vkq;
-                }
-            }
+        jacobi_rot<0, 1>(A, V);
+        jacobi_rot<0, 2>(A, V);
+        jacobi_rot<1, 2>(A, V);
     }
+    // descending order of the diagonal: a 3-element sorting network on
+    // (value, column) pairs, the earlier column first on ties
     double ev[3] = {A[0][0], A[1][1], A[2][2]};
     int idx[3] = {0, 1, 2};
-    for (int a = 0; a < 3; ++a)
-        for (int b = a + 1; b < 3; ++b)
-            if (ev[idx[b]] > ev[idx[a]]) {
-                const int t = idx[a];
-                idx[a] = idx[b];
-                idx[b] = t;
-            }
-    for (int a = 0; a < 3; ++a) vals[a] = ev[idx[a]];
-    for (int k = 0; k < 3; ++k) vec[k] = V[k][idx[0]];
+    auto cswap = [&](int a, int b) {
+        if (ev[b] > ev[a]) {
+            const double tv = ev[a];
+            ev[a] = ev[b];
+            ev[b] = tv;
+            const int ti = idx[a];
+            idx[a] = idx[b];
+            idx[b] = ti;
+        }
+    };
+    cswap(0, 1);
+    cswap(0, 2);
+    cswap(1, 2);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) vals[a] = ev[a];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vec[k] = idx[0] == 0 ? V[k][0] : idx[0] == 1 ? V[k][1] : V[k][2];
 }
 
 // Stress.jl:10-100 (centroid stress, von Mises, principal, L1, direction) and
 // the SIMP energy / sensitivity, all from one read of the element's 24 DOFs.
+// EIG: principal stresses, L1 or direction requested (the eigen solver's
+// registers are then part of the kernel; without it, as in the TO loop, the
+// kernel keeps more waves resident).
+template <bool EIG>
 __global__ __launch_bounds__(NT) void k_element_post(GridArgs g, ParityK<double> K, ElemArgs a) {
     __shared__ double red[NT / 64];
     const long e = gtid();
@@ -709,8 +852,10 @@ __global__ __launch_bounds__(NT) void k_element_post(GridArgs g, ParityK<double>
             for (int r = 3; r < 24; ++r) ce += xh[r] * yh[r];
             if (a.ce) a.ce[e] = ce;
             if (a.dc) {
-                const double xp = a.xphys[e];
-                a.dc[e] = -a.penal * pow(xp, a.penal - 1.0) * (a.E0 - a.Emin) * ce;
+                const double xp = a.xphys[e], pm1 = a.penal - 1.0;
+                // integer exponents 1 and 2 (penal 2, 3) by products: FP64 pow is a long routine
+                const double xw = pm1 == 2.0 ? xp * xp : pm1 == 1.0 ? xp : pow(xp, pm1);
+                a.dc[e] = -a.penal * xw * (a.E0 - a.Emin) * ce;
             }
         }
         if (a.Emod) {
@@ -734,7 +879,7 @@ __global__ __launch_bounds__(NT) void k_element_post(GridArgs g, ParityK<double>
                 vmv = sqrt(0.5 * ((sv[0] - sv[1]) * (sv[0] - sv[1]) + (sv[1] - sv[2]) * (sv[1] - sv[2]) +
                                   (sv[2] - sv[0]) * (sv[2] - sv[0])) +
                            3.0 * (sv[3] * sv[3] + sv[4] * sv[4] + sv[5] * sv[5]));
-                if (a.principal || a.l1 || a.dir) {
+                if constexpr (EIG) {
                     double S[3][3] = {{sv[0], sv[3], sv[5]}, {sv[3], sv[1], sv[4]}, {sv[5], sv[4], sv[2]}};
                     eig3_jacobi(S, pr, dv);
                     l1 = fabs(pr[0]) + fabs(pr[1]) + fabs(pr[2]);
@@ -1128,13 +1273,61 @@ void launch_helm_rhs(const GridArgs& g, const double* in_e, double vol8, double*
 void launch_helm_minv(const GridArgs& g, const HelmArgs& A, double* minv, hipStream_t s) {
     hipLaunchKernelGGL(k_helm_minv, dim3(blocks_for(g.nn)), dim3(NT), 0, s, g, A, minv);
 }
+// the 1-D factors of A's box (HelmSF) when they reproduce all 64 entries of
+// A.Ae to 1e-13 of the largest: entry (p, q) = R^2 (Kx My Mz + Mx Ky Mz +
+// Mx My Kz) + Mx My Mz with the per-axis factor of "same coordinate" (K: k,
+// M: 2m) or "different" (K: -k, M: m)
+static bool helm_sf_params(const HelmArgs& A, HelmSF* c) {
+    if (!(A.h[0] > 0.0 && A.h[1] > 0.0 && A.h[2] > 0.0)) return false;
+    c->kx = 1.0 / A.h[0];
+    c->ky = 1.0 / A.h[1];
+    c->kz = 1.0 / A.h[2];
+    c->mx = A.h[0] / 6.0;
+    c->my = A.h[1] / 6.0;
+    c->mz = A.h[2] / 6.0;
+    c->R2 = A.R2;
+    double amax = 0.0;
+    for (int t = 0; t < 64; ++t) amax = std::max(amax, std::fabs(A.Ae[t]));
+    for (int p = 0; p < 8; ++p)
+        for (int q = 0; q < 8; ++q) {
+            const bool sx = (p & 1) == (q & 1), sy = ((p >> 1) & 1) == ((q >> 1) & 1), sz = (p >> 2) == (q >> 2);
+            const double Kx = sx ? c->kx : -c->kx, Ky = sy ? c->ky : -c->ky, Kz = sz ? c->kz : -c->kz;
+            const double Mx = sx ? 2.0 * c->mx : c->mx, My = sy ? 2.0 * c->my : c->my, Mz = sz ? 2.0 * c->mz : c->mz;
+            const double v = c->R2 * (Kx * My * Mz + Mx * Ky * Mz + Mx * My * Kz) + Mx * My * Mz;
+            if (std::fabs(A.Ae[8 * kBinToRef[p] + kBinToRef[q]] - v) > 1e-13 * amax) return false;
+        }
+    return true;
+}
+
+bool helm_sf_available(const HelmArgs& A) {
+    HelmSF c;
+    return helm_sf_params(A, &c);
+}
 int helm_matvec_blocks(const GridArgs& g) {
     int a, b;
-    return helm_tiles(g, &a, &b);
+    return std::max(helm_tiles(g, &a, &b), helm_sf_tiles(g, &a, &b));
+}
+int launch_helm_matvec_fused(const GridArgs& g, const HelmArgs& A, const double* p_old, const double* r,
+                             const double* minv, double* p_out, double* q, PcgState* st, double* partials,
+                             hipStream_t s) {
+    HelmSF c;
+    if (!helm_sf_params(A, &c)) return -1;
+    int ntx, nty;
+    const int nb = helm_sf_tiles(g, &ntx, &nty);
+    hipLaunchKernelGGL((k_helm_sf<true>), dim3(nb), dim3(kHsfX * kHsfY), 0, s, g, c, p_old, r, minv, p_out, q,
+                       (int)MV_PCG, st, partials, ntx, nty);
+    return nb;
 }
 int launch_helm_matvec(const GridArgs& g, const HelmArgs& A, const double* x, double* y, int mode, PcgState* st,
                        double* partials, hipStream_t s) {
     int ntx, nty;
+    HelmSF c;
+    if (helm_sf_params(A, &c)) {
+        const int nb = helm_sf_tiles(g, &ntx, &nty);
+        hipLaunchKernelGGL((k_helm_sf<false>), dim3(nb), dim3(kHsfX * kHsfY), 0, s, g, c, x, nullptr, nullptr,
+                           nullptr, y, mode, st, partials, ntx, nty);
+        return nb;
+    }
     const int nb = helm_tiles(g, &ntx, &nty);
     // the eight box values, taken from node 0's row and checked against every entry
     HelmBox hb{};
@@ -1166,7 +1359,10 @@ void launch_helm_extract(const GridArgs& g, const double* xn, double* out_e, hip
 }
 int element_post_blocks(const GridArgs& g) { return (int)blocks_for(g.ne); }
 void launch_element_post(const GridArgs& g, const ParityK<double>& K, const ElemArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_element_post, dim3(blocks_for(g.ne)), dim3(NT), 0, s, g, K, a);
+    if (a.principal || a.l1 || a.dir)
+        hipLaunchKernelGGL(k_element_post<true>, dim3(blocks_for(g.ne)), dim3(NT), 0, s, g, K, a);
+    else
+        hipLaunchKernelGGL(k_element_post<false>, dim3(blocks_for(g.ne)), dim3(NT), 0, s, g, K, a);
 }
 void launch_max_reduce(long n, const double* partials_in, int nin, double* out, hipStream_t s) {
     (void)n;

@@ -317,6 +317,12 @@ int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const MvShape&
 // ---- Helmholtz filter --------------------------------------------------------
 struct HelmArgs {
     double Ae[64];  // reference node order (R^2 Ks + Ms)
+    // the box it was formed on (element sides, the quirk's taller z
+    // included) and R^2: with them the operator runs sum-factorised, after
+    // a check that the factors reproduce every entry of Ae (h[0] = 0: not
+    // given, the gather form runs)
+    double h[3] = {0.0, 0.0, 0.0};
+    double R2 = 0.0;
 };
 void launch_helm_rhs(const GridArgs& g, const double* in_e, double vol8, double* b, hipStream_t s);
 void launch_helm_minv(const GridArgs& g, const HelmArgs& A, double* minv, hipStream_t s);
@@ -325,6 +331,15 @@ void launch_helm_minv(const GridArgs& g, const HelmArgs& A, double* minv, hipStr
 int launch_helm_matvec(const GridArgs& g, const HelmArgs& A, const double* x, double* y, int mode, PcgState* st,
                        double* partials, hipStream_t s);
 int helm_matvec_blocks(const GridArgs& g);
+// true when A's sum-factorised form reproduces its element matrix (the
+// operator then runs in that form, and the fused PCG operator below exists)
+bool helm_sf_available(const HelmArgs& A);
+// the PCG operator with the p-update fused: p_out = minv r + beta p_old and
+// q = A p_out, with the per-block p'q partials; -1 (nothing launched) when A
+// has no checked sum-factorised form
+int launch_helm_matvec_fused(const GridArgs& g, const HelmArgs& A, const double* p_old, const double* r,
+                             const double* minv, double* p_out, double* q, PcgState* st, double* partials,
+                             hipStream_t s);
 void launch_helm_extract(const GridArgs& g, const double* xn, double* out_e, hipStream_t s);
 
 // ---- element kernels -----------------------------------------------------------
