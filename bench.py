@@ -160,14 +160,26 @@ def level_dims(nx, ny, nz):
 
 
 def smoothed_levels(nx, ny, nz, word):
-    """Levels the V-cycle smooths: those above the first level (>= 1) with at most 2048 nodes whose
-    levels' r and y (6 values per node) fit 128 KiB, i.e. above the one-workgroup coarse pass."""
-    nn = [(a + 1) * (b + 1) * (c + 1) for a, b, c in level_dims(nx, ny, nz)]
+    """Levels the V-cycle smooths with multi-block launches: those below the
+    first level (>= 1) of the one-workgroup pass, which holds at most 512
+    nodes and whose LDS (the finer level's residual or the element outputs,
+    each pass level's moduli and smoother diagonal, r, x, t of every level
+    from it down) fits 128 KiB -- the library's rule."""
+    dims = level_dims(nx, ny, nz)
+    nn = [(a + 1) * (b + 1) * (c + 1) for a, b, c in dims]
+    ne = [a * b * c for a, b, c in dims]
     L = len(nn) - 1
+    if L == 0:
+        return 0
+
+    def lds_values(s0):
+        w = max([3 * nn[s0 - 1]] + [24 * ne[l] for l in range(s0, L)])
+        return w + sum(ne[l] + 3 * nn[l] for l in range(s0, L)) + sum(9 * nn[l] for l in range(s0, L + 1))
+
     s0 = 1
-    while s0 <= L and (nn[s0] > 2048 or 6 * sum(nn[s0:]) * word > 128 * 1024):
+    while s0 <= L and (nn[s0] > 512 or lds_values(s0) * word > 128 * 1024):
         s0 += 1
-    return 0 if L == 0 else min(s0, L)
+    return min(s0, L)
 
 
 def ml_apply_bytes(nx, ny, nz, word):
@@ -595,7 +607,7 @@ def main():
         if main_form.endswith("_vc"):
             sl = smoothed_levels(nx, ny, nz, 8 if fp64_main else 4)
             by = vc_apply_bytes(nx, ny, nz, 8 if fp64_main else 4, sl)
-            kn = ("k_matvec_sf forms 3 and 4, k_ml_restrict_zm, k_ml_coarse, k_fin_rz "
+            kn = ("k_matvec_sf forms 3 and 4, k_ml_restrict_zm, k_vc_coarse, k_fin_rz "
                   f"({sl} smoothed levels, one application)")
         else:
             by = ml_apply_bytes(nx, ny, nz, 8 if fp64_main else 4)
