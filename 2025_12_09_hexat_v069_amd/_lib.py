@@ -123,7 +123,8 @@ class TuneResult(C.Structure):
 
 class LearnStats(C.Structure):
     _fields_ = [("samples", C.c_int), ("epochs", C.c_int), ("lr", C.c_double), ("seed", C.c_int),
-                ("final_loss", C.c_double), ("seconds", C.c_double), ("jacobi_loss", C.c_double)]
+                ("final_loss", C.c_double), ("seconds", C.c_double), ("jacobi_loss", C.c_double),
+                ("manufactured", C.c_int)]
 
 
 _lib = None
@@ -512,10 +513,10 @@ class Context:
         return out
 
     # ---- learned preconditioner -----------------------------------------------------
-    def learned_train(self, Ee=None, f=None, samples=64, epochs=4, lr=1e-3, seed=0):
+    def learned_train(self, Ee=None, f=None, samples=64, epochs=4, lr=1e-3, seed=0, manufactured=0):
         Ee = None if Ee is None else f64(Ee, self.ne)
         f = None if f is None else f64(f, self.ndof)
-        s = LearnStats(int(samples), int(epochs), float(lr), int(seed), 0.0, 0.0, 0.0)
+        s = LearnStats(int(samples), int(epochs), float(lr), int(seed), 0.0, 0.0, 0.0, int(manufactured))
         self.check(self.L.hx_learned_train(self.h, dptr(Ee), dptr(f), C.byref(s)), "hx_learned_train")
         return s
 

@@ -166,7 +166,7 @@ struct hx_ctx {
     // learned preconditioner
     UNet* unet = nullptr;
     bool unet_trained = false;
-    DevBuf<double> l_z, l_zold, l_P, l_AP, l_dots, l_ustar, l_samples, l_rho;
+    DevBuf<double> l_z, l_zold, l_P, l_AP, l_dots, l_ustar, l_samples, l_rho, l_man;
     int l_nsamples = 0;
     long l_sample_stride = 0;
     // multilevel preconditioner: level moduli, inverse diagonals, work vectors
@@ -2069,6 +2069,117 @@ int hx_to_read(hx_ctx* c, int field, double* out) {
     }
 }
 
+// Random training moduli: uniform densities smoothed by the PDE filter of a
+// random radius in [1, 3], stretched to [0, 1], solid above a random
+// threshold in [0.2, 0.7] and graded to 0.01 below it, SIMP p = 3.
+static int random_moduli(hx_ctx* c, std::mt19937_64& rng, std::vector<double>& hrho, std::vector<double>& hE) {
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    const long ne = c->g.ne;
+    int rc;
+    for (long e = 0; e < ne; ++e) hrho[e] = U(rng);
+    if ((rc = upload(c, c->l_rho.p, hrho.data(), ne * sizeof(double)))) return rc;
+    hx_filter_opts fo{0, 1e-6, 500, 1};
+    if ((rc = helm_solve(c, 1.0 + 2.0 * U(rng), fo, c->l_rho.p, c->l_rho.p, nullptr))) return rc;
+    HXC(hipMemcpy(hrho.data(), c->l_rho.p, ne * sizeof(double), hipMemcpyDeviceToHost));
+    double lo = 1e300, hi = -1e300;
+    for (long e = 0; e < ne; ++e) lo = std::min(lo, hrho[e]), hi = std::max(hi, hrho[e]);
+    const double thr = 0.2 + 0.5 * U(rng);
+    for (long e = 0; e < ne; ++e) {
+        double t = hi > lo ? (hrho[e] - lo) / (hi - lo) : 0.5;
+        t = t > thr ? 1.0 : std::max(0.01, t / thr);
+        hE[e] = 1e-9 + t * t * t;
+    }
+    return upload(c, c->E.p, hE.data(), ne * sizeof(double));
+}
+
+// Streaming training on manufactured solutions (hx_learn_stats::manufactured
+// = m): per density field kManSolutions smooth random displacements e*, load
+// b = A e* on the solved DOFs, Jacobi-PCG from zero, and at the first m of the
+// iterations k = 0, 2, 8, 32 the pair (r_k = b - A x_k, e_k = e* - x_k) --
+// A e_k = r_k exactly, so no reference solve is needed -- takes one Adam step
+// on the subspace objective right away.
+static int learned_train_manufactured(hx_ctx* c, hx_learn_stats* s) {
+    constexpr int kManSolutions = 4;
+    const int ks[] = {0, 2, 8, 32};
+    const int nk = std::min(4, std::max(1, s->manufactured));
+    const long n = ndof(c), ne = c->g.ne;
+    HXA(c->l_man, 7 * n);
+    HXA(c->l_rho, ne);
+    HXA(c->l_dots, 8 + 2 * kFcgKeep);
+    double* estar = c->l_man.p;
+    double *ek = estar + n, *rk = ek + n, *u = rk + n, *Au = u + n, *y = Au + n, *Ay = y + n;
+    std::vector<double> hE(ne), hrho(ne);
+    int levels = 1;
+    while ((2 << levels) <= std::max({c->g.nx, c->g.ny, c->g.nz})) ++levels;  // coarsest lattice ~ half the grid
+    levels = std::max(1, levels - 1);
+    const int per_field = kManSolutions * nk;
+    const int nfield = std::max(1, (s->samples + per_field - 1) / per_field);
+    int rc;
+    double last = 0.0, lj = 0.0;
+    long used = 0;
+    for (int ep = 0; ep < s->epochs; ++ep) {
+        std::mt19937_64 rng((uint64_t)s->seed * 7919u + 29u);  // the same stream every pass
+        double acc = 0.0, accj = 0.0;
+        long cnt = 0;
+        for (int fi = 0; fi < nfield; ++fi) {
+            if ((rc = random_moduli(c, rng, hrho, hE))) return rc;
+            launch_elastic_minv<double>(c->g, c->kd, c->E.p, c->has_fixed ? c->fixed.p : nullptr, 0.0, c->minv.p,
+                                        c->stream);
+            for (int si = 0; si < kManSolutions; ++si) {
+                const unsigned long long seed = rng();
+                launch_smooth_field(c->g, seed, levels, c->minv.p, estar, c->stream);
+                mv64(c, estar, c->f.p, c->E.p, 0.0, MV_PLAIN, HX_MV_AUTO, 1);
+                launch_mask<double>(n, c->minv.p, c->f.p, c->stream, 1);  // b = A e* on the solved DOFs
+                for (int kk = 0; kk < nk; ++kk) {
+                    if (ks[kk] == 0) {
+                        launch_fill(n, 0.0, c->x.p, c->stream);
+                    } else {
+                        hx_pcg_opts it{1e-30, ks[kk], 0.0, HX_FP64, HX_PC_JACOBI, 0, HX_MV_AUTO, 0};
+                        rc = pcg_fp64(c, it, nullptr, single_vecs(c));
+                        if (rc == HX_ERR_HIP || rc == HX_ERR_OOM) return rc;
+                    }
+                    mv64(c, c->x.p, c->q.p, c->E.p, 0.0, MV_PLAIN, HX_MV_AUTO, 1);
+                    launch_true_residual(n, c->f.p, c->q.p, c->x.p, c->minv.p, 0.0, rk, nullptr, c->st, c->partials.p,
+                                         1, c->stream, 1);
+                    launch_masked_diff(n, estar, c->x.p, c->minv.p, ek, c->stream);
+                    launch_mul(n, c->minv.p, rk, u, c->stream);  // u = D^-1 r
+                    mv64(c, u, Au, c->E.p, 0.0, MV_PLAIN, HX_MV_AUTO, 1);
+                    double h[4];
+                    if ((rc = dots_host(c, n, {{rk, rk}, {ek, rk}, {u, Au}, {u, rk}}, h))) return rc;
+                    const double rr = h[0], er = h[1], uAu = h[2], ur = h[3];
+                    if (!(rr > 0.0) || !(er > 0.0) || !(uAu > 0.0)) continue;
+                    unet_forward_sample(c->unet, rk, c->E.p, 1.0 / std::sqrt(rr / (double)n), c->stream);
+                    unet_direction(c->unet, c->minv.p, 1.0, y, c->stream);
+                    mv64(c, y, Ay, c->E.p, 0.0, MV_PLAIN, HX_MV_AUTO, 1);
+                    double g[3];
+                    if ((rc = dots_host(c, n, {{u, Ay}, {y, Ay}, {y, rk}}, g))) return rc;
+                    const double uAy = g[0], yAy = g[1], yr = g[2];
+                    const double jl = 1.0 - ur * ur / (uAu * er);
+                    accj += jl;
+                    ++cnt;
+                    const double det = uAu * yAy - uAy * uAy;
+                    if (!(det > 1e-12 * uAu * yAy)) {
+                        acc += jl;
+                        continue;
+                    }
+                    const double a = (ur * yAy - yr * uAy) / det, b = (yr * uAu - ur * uAy) / det;
+                    acc += 1.0 - (a * ur + b * yr) / er;
+                    unet_subspace_grad(c->unet, rk, Au, Ay, c->minv.p, a, b, -2.0 * b / er, c->stream);
+                    unet_backward_step(c->unet, (float)s->lr, c->stream);
+                }
+            }
+        }
+        if (cnt == 0) return fail(c, HX_ERR_STATE, "no usable training samples");
+        last = acc / (double)cnt;
+        lj = accj / (double)cnt;
+        used = cnt;
+    }
+    s->samples = (int)used;
+    s->final_loss = last;
+    s->jacobi_loss = lj;
+    return HX_OK;
+}
+
 int hx_learned_train(hx_ctx* c, const double* Ee, const double* f, hx_learn_stats* s) {
     if (!c || !s) return HX_ERR_BAD_ARG;
     if (int rc = ensure_grid(c)) return rc;
@@ -2077,6 +2188,19 @@ int hx_learned_train(hx_ctx* c, const double* Ee, const double* f, hx_learn_stat
     (void)hipEventRecord(ev0, c->stream);
     if (!c->unet) c->unet = unet_create();
     if (!unet_bind(c->unet, c->g, (unsigned)s->seed, c->stream)) return fail(c, HX_ERR_OOM, "U-Net buffers");
+    if (s->manufactured) {
+        const int rc = learned_train_manufactured(c, s);
+        (void)hipEventRecord(ev1, c->stream);
+        (void)hipEventSynchronize(ev1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ev0, ev1);
+        c->free_events.push_back(ev0);
+        c->free_events.push_back(ev1);
+        if (rc) return rc;
+        c->unet_trained = true;
+        s->seconds = ms * 1e-3;
+        return HX_OK;
+    }
     const long n = ndof(c), ne = c->g.ne;
     // residual/error pairs at these Jacobi-PCG iteration counts of each problem
     const int ks[] = {0, 2, 8, 32, 128, 512};
@@ -2100,22 +2224,8 @@ int hx_learned_train(hx_ctx* c, const double* Ee, const double* f, hx_learn_stat
         if (pi == 0 && Ee) {
             std::memcpy(hE.data(), Ee, ne * sizeof(double));
             if ((rc = upload(c, c->E.p, hE.data(), ne * sizeof(double)))) return rc;
-        } else {
-            for (long e = 0; e < ne; ++e) hrho[e] = U(rng);
-            if ((rc = upload(c, c->l_rho.p, hrho.data(), ne * sizeof(double)))) return rc;
-            hx_filter_opts fo{0, 1e-6, 500, 1};
-            if ((rc = helm_solve(c, 1.0 + 2.0 * U(rng), fo, c->l_rho.p, c->l_rho.p, nullptr))) return rc;
-            HXC(hipMemcpy(hrho.data(), c->l_rho.p, ne * sizeof(double), hipMemcpyDeviceToHost));
-            // stretch the smoothed field to [0.01, 1], SIMP p = 3
-            double lo = 1e300, hi = -1e300;
-            for (long e = 0; e < ne; ++e) lo = std::min(lo, hrho[e]), hi = std::max(hi, hrho[e]);
-            const double thr = 0.2 + 0.5 * U(rng);
-            for (long e = 0; e < ne; ++e) {
-                double t = hi > lo ? (hrho[e] - lo) / (hi - lo) : 0.5;
-                t = t > thr ? 1.0 : std::max(0.01, t / thr);  // solid above the threshold, graded below
-                hE[e] = 1e-9 + t * t * t;
-            }
-            if ((rc = upload(c, c->E.p, hE.data(), ne * sizeof(double)))) return rc;
+        } else if ((rc = random_moduli(c, rng, hrho, hE))) {
+            return rc;
         }
         if (pi == 0 && f) {
             std::memcpy(hf.data(), f, n * sizeof(double));

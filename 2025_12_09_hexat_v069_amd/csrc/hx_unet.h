@@ -63,6 +63,10 @@ void launch_adam(long n, float* w, float* g, float* m, float* v, uint16_t* wbf, 
 void launch_fillf(long n, float v, float* a, hipStream_t s);
 void launch_pack_input(const GridArgs& g, const double* r, const double* E, double inv_scale, uint16_t* out,
                        hipStream_t s);
+// e = sum_{l=1..levels} 2^(l-levels) * trilinear interpolation of hashed standard
+// normal values on the lattice of spacing 2^l; 0 where minv = 0
+void launch_smooth_field(const GridArgs& g, unsigned long long seed, int levels, const double* minv, double* e,
+                         hipStream_t s);
 void launch_unpack_output(long nn, const float* y, int cy, const double* r, const double* minv, double scale,
                           double* z, hipStream_t s);
 void launch_loss_grad(long nn, const float* y, int cy, const double* tgt, const double* minv, double tscale,

@@ -284,6 +284,50 @@ __global__ __launch_bounds__(256) void k_fillf(long n, float v, float* a) {
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) a[i] = v;
 }
 
+// splitmix64 finaliser (Steele, Lea & Flood 2014): a counter-based hash that
+// gives every lattice value its own stream without state
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+// standard normal value of lattice key q (Box-Muller on two hashed uniforms in (0, 1))
+__device__ __forceinline__ double lattice_normal(unsigned long long seed, unsigned long long q) {
+    const unsigned long long h1 = mix64(seed ^ mix64(q)), h2 = mix64(h1 ^ 0xD1B54A32D192ED03ull);
+    const double u1 = ((double)(h1 >> 11) + 0.5) * 0x1p-53, u2 = (double)(h2 >> 11) * 0x1p-53;
+    return sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
+}
+
+// Manufactured displacement for training: on every level l = 1..levels the
+// standard normal values of the lattice of spacing 2^l (one per lattice point
+// and component) interpolated trilinearly to the nodes, weighted 2^(l-levels)
+// (smoother parts larger), summed; 0 where minv = 0 (not solved for).
+__global__ __launch_bounds__(256) void k_smooth_field(GridArgs g, unsigned long long seed, int levels,
+                                                      const double* __restrict__ minv, double* __restrict__ e) {
+    const long n = (long)blockIdx.x * 256 + threadIdx.x;
+    if (n >= g.nn) return;
+    const int i = (int)(n % g.NX), j = (int)((n / g.NX) % g.NY), k = (int)(n / ((long)g.NX * g.NY));
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int l = 1; l <= levels; ++l) {
+        const int h = 1 << l;
+        const long LX = (g.NX + h - 1) / h + 1, LY = (g.NY + h - 1) / h + 1;
+        const int I = i >> l, J = j >> l, K = k >> l;
+        const double fx = (double)(i & (h - 1)) / h, fy = (double)(j & (h - 1)) / h, fz = (double)(k & (h - 1)) / h;
+        const double w = ldexp(1.0, l - levels);
+        for (int c8 = 0; c8 < 8; ++c8) {
+            const int ox = c8 & 1, oy = (c8 >> 1) & 1, oz = c8 >> 2;
+            const double wt = w * (ox ? fx : 1.0 - fx) * (oy ? fy : 1.0 - fy) * (oz ? fz : 1.0 - fz);
+            const unsigned long long pt =
+                ((unsigned long long)l << 56) ^ (unsigned long long)(((long)(K + oz) * LY + (J + oy)) * LX + (I + ox));
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c] += wt * lattice_normal(seed, 3ull * pt + c);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) e[3 * n + c] = minv[3 * n + c] != 0.0 ? v[c] : 0.0;
+}
+
 // Network input from a residual: ch 0..2 = r/scale at the node, ch 3 = the
 // node's mean element modulus on a log scale, ch 4..7 = 0.
 __global__ __launch_bounds__(256) void k_pack_input(GridArgs g, const double* __restrict__ r, const double* __restrict__ E,
@@ -374,6 +418,11 @@ void launch_adam(long n, float* w, float* g, float* m, float* v, uint16_t* wbf, 
 }
 void launch_fillf(long n, float v, float* a, hipStream_t s) {
     hipLaunchKernelGGL(k_fillf, dim3(nblk(n)), dim3(256), 0, s, n, v, a);
+}
+void launch_smooth_field(const GridArgs& g, unsigned long long seed, int levels, const double* minv, double* e,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_smooth_field, dim3((unsigned)((g.nn + 255) / 256)), dim3(256), 0, s, g, seed, levels, minv,
+                       e);
 }
 void launch_pack_input(const GridArgs& g, const double* r, const double* E, double inv_scale, uint16_t* out,
                        hipStream_t s) {

@@ -74,23 +74,66 @@ def other_workload(args, hx, L, ctx):
             out["plain"]["compliance"])
         out["config"] = "half MBB 192x96x96 (5,447,811 DOFs), smoothed random density, SIMP p=3, tol 1e-8"
     elif args.workload == "cfg4":
-        # the deployment size of the learned-preconditioner configuration,
-        # solver only: the main form against plain FP64 Jacobi
+        # the learned-preconditioner configuration: the network trained on
+        # manufactured-solution pairs streamed on a G^3 grid, then deployed in
+        # the flexible CG at 320x160x160, beside the main form and plain FP64
+        # Jacobi; solver only, then one TO iteration per form
         nx, ny, nz = 320, 160, 160
+        learned = None
+        if args.learn_samples > 0:
+            G = args.learn_grid
+            tfix, _ = cantilever(G, G, G)
+            ctx.set_grid(G, G, G, 1.0, 1.0, 1.0, 0.3)
+            ctx.set_fixed(tfix)
+            chunk, done, tt, loss, jl = 2000, 0, 0.0, [], []
+            while done < args.learn_samples:
+                ls = ctx.learned_train(samples=min(chunk, args.learn_samples - done), epochs=1, lr=1e-3,
+                                       seed=done // chunk, manufactured=1)
+                done += ls.samples
+                tt += ls.seconds
+                loss.append(ls.final_loss)
+                jl.append(ls.jacobi_loss)
+                log(f"  cfg4 training on {G}^3: {done} pairs, loss {ls.final_loss:.4f} (D^-1 r alone "
+                    f"{ls.jacobi_loss:.4f}), {tt:.1f} s")
+            learned = {"train_grid": f"{G}x{G}x{G}", "pairs": done, "train_s": round(tt, 2),
+                       "loss_per_chunk": [round(v, 4) for v in loss], "jacobi_loss": round(float(np.mean(jl)), 4),
+                       "source": "manufactured solutions: smooth random displacement e*, load A e* on random "
+                                 "thresholded densities; (r, e) = (A e*, e*) streamed into training"}
         fixed, f = cantilever(nx, ny, nz)
         ctx.set_grid(nx, ny, nz, 1.0, 1.0, 1.0, 0.3)
         ctx.set_fixed(fixed)
         ctx.tune_set(ctx.autotune(1))
         Ee = 1e-9 + smoothed_density(ctx, nx * ny * nz, 4, 2.0) ** 3
         res = {}
-        for name, prec, pc in (("precision_vc", L.HX_FP32_IR, L.HX_PC_VCYCLE), ("plain", L.HX_FP64, L.HX_PC_JACOBI)):
-            ctx.pcg(Ee, f, tol=1e-8, precision=prec, precond=pc)  # warm-up (graphs)
+        forms = [("precision_vc", L.HX_FP32_IR, L.HX_PC_VCYCLE), ("plain", L.HX_FP64, L.HX_PC_JACOBI)]
+        if learned is not None:
+            forms.append(("learned", L.HX_FP64, L.HX_PC_LEARNED))
+        for name, prec, pc in forms:
+            if pc != L.HX_PC_LEARNED:
+                ctx.pcg(Ee, f, tol=1e-8, precision=prec, precond=pc)  # warm-up (graphs)
             t0 = time.perf_counter()
             u, st = ctx.pcg(Ee, f, tol=1e-8, precision=prec, precond=pc)
             res[name] = (time.perf_counter() - t0, st, u)
             log(f"  cfg4 {name}: {res[name][0]:.4f} s, {st.iterations} iterations, rel {st.rel_res:.3e}")
         (tm, sm, um), (tp, sp, up) = res["precision_vc"], res["plain"]
         cm, cp = float(f @ um), float(f @ up)
+        # one TO iteration per form from uniform density (the main configuration's
+        # volume fraction and filter radius), its results against plain FP64
+        to = {}
+        for name, prec, pc in forms:
+            ctx.to_init(to_params(L, args, prec, pc), f, np.full(nx * ny * nz, args.volfrac))
+            s1 = ctx.to_step()
+            to[name] = {"s_per_iter": round(s1.seconds, 4), "pcg_iterations": s1.pcg_iterations,
+                        "pcg_rel_res": s1.pcg_rel_res, "compliance": s1.compliance,
+                        "xphys": ctx.to_read(L.HX_FIELD_XPHYS)}
+            log(f"  cfg4 TO iteration {name}: {s1.seconds:.3f} s, {s1.pcg_iterations} PCG iterations, "
+                f"c={s1.compliance:.6e}")
+        for name in to:
+            to[name]["compliance_rel_diff_vs_plain"] = abs(to[name]["compliance"] - to["plain"]["compliance"]) / abs(
+                to["plain"]["compliance"])
+            to[name]["xphys_max_abs_diff_vs_plain"] = float(np.abs(to[name]["xphys"] - to["plain"]["xphys"]).max())
+        for name in to:
+            del to[name]["xphys"]
         out = {"metric": "s per solve at 24.96M DOFs (320x160x160, tol 1e-8)", "value": round(tm, 5), "unit": "s",
                "n_gpus": 1, "higher_is_better": False, "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
                "config": {"workload": "320x160x160 cantilever (8,192,000 elements, 24,961,923 DOFs), smoothed random "
@@ -101,8 +144,16 @@ def other_workload(args, hx, L, ctx):
                "speedup_vs_plain_fp64": round(tp / tm, 2),
                "compliance_rel_diff": abs(cm - cp) / cp,
                "displacement_rel_diff": float(np.linalg.norm(um - up) / np.linalg.norm(up)),
-               "learned_preconditioner": "not run at this size: the learned flexible CG takes more iterations "
-                                         "than Jacobi on the problems it was trained on"}
+               "to_iteration": to}
+        if learned is not None:
+            tl, sl, ul = res["learned"]
+            learned.update({"s": round(tl, 5), "iterations": sl.iterations, "rel_res": sl.rel_res,
+                            "fallbacks": sl.fallbacks, "status": sl.status,
+                            "iterations_vs_plain_jacobi": round(sl.iterations / sp.iterations, 3),
+                            "time_vs_plain_jacobi": round(tl / tp, 3),
+                            "compliance_rel_diff_vs_plain": abs(float(f @ ul) - cp) / cp,
+                            "displacement_rel_diff_vs_plain": float(np.linalg.norm(ul - up) / np.linalg.norm(up))})
+            out["learned"] = learned
     elif args.workload == "block8":
         # 8 load cases in one block solve (every launch carries the 8 columns;
         # the V-cycle's one-workgroup pass runs one workgroup per column)
@@ -504,6 +555,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-pcg-iters", type=int, default=20)
     ap.add_argument("--cpu-filter-iters", type=int, default=20)
+    ap.add_argument("--learn-samples", type=int, default=20000,
+                    help="cfg4: manufactured-solution pairs to train the learned preconditioner on (0: skip it)")
+    ap.add_argument("--learn-grid", type=int, default=96, help="cfg4: edge of the cubic training grid")
     ap.add_argument("--workload", choices=("cantilever", "mbb", "block8", "cfg4"), default="cantilever",
                     help="cantilever: the headline TO-iteration bench; mbb / block8 / cfg4: solver-only "
                          "measurements of the precision (192x96x96), batching (8 load cases, 128x64x64) and "

@@ -348,6 +348,15 @@ typedef struct {
     double final_loss;   /* mean loss of the last epoch */
     double seconds;
     double jacobi_loss;  /* mean loss of D^-1 r alone */
+    /* 0: triples from reference solves of random-load problems (stored,
+     * `epochs` passes; Ee/f give the first problem).  m = 1..4: triples
+     * streamed from manufactured solutions: per random density field smooth
+     * random displacements e* (multi-scale lattice noise), loads b = A e*,
+     * Jacobi-PCG from zero; (r_k, e* - x_k) at the first m of the iterations
+     * k = 0, 2, 8, 32 each take one training step at once (no storage, any
+     * grid size); `samples` triples per pass, `epochs` passes over the same
+     * seeded stream. */
+    int manufactured;
 } hx_learn_stats;
 int hx_learned_train(hx_ctx* ctx, const double* Ee, const double* f, hx_learn_stats* stats);
 /* copy of the current weights (BF16 bit patterns) and their count */

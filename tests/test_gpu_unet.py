@@ -154,16 +154,23 @@ def _problem(ox, g, seed):
 
 def test_learned_preconditioner_ablation(ox, hx):
     """Train on solver-generated data, then solve an unseen problem with the
-    learned flexible CG and with plain Jacobi-PCG.
+    learned flexible CG and with plain Jacobi-PCG (the method without the
+    network).
 
-    What the network gains is measured on its training objective: the share
-    of the error's energy |e|_A^2 left outside span{D^-1 r, net(r)} must fall
-    below the share left by D^-1 r alone on the same samples.  In the solve
-    the learned flexible CG must reach the FP64 tolerance on the true
-    residual without the Jacobi fallback and agree with the Jacobi solution;
-    its iteration count is reported beside Jacobi's (currently above it: a
-    preconditioner that changes nonlinearly from step to step gives up CG's
-    Krylov optimality, so the per-step gain does not carry over).
+    Two training sources:
+    * reference solves of random-load problems, pairs taken along Jacobi-PCG
+      (stored; a few passes): the share of the error's energy |e|_A^2 left
+      outside span{D^-1 r, net(r)} must fall below the share left by D^-1 r
+      alone on the same samples;
+    * manufactured solutions (smooth random displacements e*, loads A e*,
+      pairs (r, e) = (A e*, e*) streamed straight into training): the
+      network trained this way must cut the flexible CG's iterations on the
+      unseen cantilever well below Jacobi-PCG's (measured on MI355X: 247-279
+      against 494), reach the FP64 tolerance on the true residual without the
+      Jacobi fallback and agree with the Jacobi solution.
+    Pairs harvested later along Jacobi-PCG (k = 2, 8, 32) teach the network
+    the residuals Jacobi leaves, which the flexible CG never meets: mixed in,
+    they cost the solve its gain (845 against 302 iterations at 48x24x24).
     """
     g = ox.Grid(32, 16, 16)
     with hx.Context(0) as c:
@@ -175,6 +182,13 @@ def test_learned_preconditioner_ablation(ox, hx):
         print("subspace loss", s1.final_loss, "->", s2.final_loss, "(D^-1 r alone:", s2.jacobi_loss, ") time",
               s1.seconds + s2.seconds, flush=True)
         assert 0.0 <= s2.final_loss < 0.9 * s2.jacobi_loss <= 1.0
+    with hx.Context(0) as c:
+        c.set_grid(32, 16, 16)
+        c.set_fixed(fixed)
+        sm = c.learned_train(samples=2000, epochs=1, lr=1e-3, seed=0, manufactured=1)
+        print("manufactured: loss", sm.final_loss, "(D^-1 r alone:", sm.jacobi_loss, ") samples", sm.samples,
+              "time", sm.seconds, flush=True)
+        assert sm.samples >= 2000 and 0.0 <= sm.final_loss < 0.5 * sm.jacobi_loss
         uj, sj = c.pcg(Ee, f, tol=1e-8)
         ul, sl = c.pcg(Ee, f, tol=1e-8, precond=hx._lib.HX_PC_LEARNED)
         print("jacobi its", sj.iterations, sj.seconds, "learned its", sl.iterations, sl.seconds, "fallbacks",
@@ -186,3 +200,4 @@ def test_learned_preconditioner_ablation(ox, hx):
         assert sl.status == 0 and np.linalg.norm(r) / np.linalg.norm(b) < 1.5e-8
         assert abs(f @ ul - f @ uj) < 1e-6 * abs(f @ uj)
         assert sl.fallbacks == 0
+        assert sl.iterations < 0.75 * sj.iterations
