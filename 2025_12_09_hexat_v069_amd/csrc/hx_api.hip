@@ -604,11 +604,13 @@ int pc_apply64(hx_ctx* c, bool vcycle, const double* r, int fin_mode, int nrhs =
     }
     return HX_OK;
 }
-int pc_apply32(hx_ctx* c, bool vcycle, const float* r, int fin_mode, int nrhs = 1) {
+// xup, pup: the V-cycle's one-workgroup pass also does xup += alpha pup
+int pc_apply32(hx_ctx* c, bool vcycle, const float* r, int fin_mode, int nrhs = 1, float* xup = nullptr,
+               const float* pup = nullptr) {
     if (vcycle) {
         if (launch_vc_apply<float>(c->ml32, c->kp32, vc_shape(c, true, false), vc_shape(c, true, true), r,
                                    c->ml_z32.p, c->st, c->partials.p, kOpPartials, fin_mode, c->stream,
-                                   vc_free_update(c, true, true), nrhs) != HX_OK)
+                                   vc_free_update(c, true, true), nrhs, xup, pup) != HX_OK)
             return fail(c, HX_ERR_BAD_ARG, "grid too large for the V-cycle's partial sums, or columns without a "
                                            "coarse level");
     } else {
@@ -745,8 +747,13 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
     HXK();
     if ((rc = sync_state(c, nr))) return rc;
     int refinements = 0;
-    const std::string key = batch_key(c, vc ? (free_upd ? "f32vcf" : "f32vc") : ml ? "f32ml" : "f32", nr, batch,
-                                      ml ? HX_MV_PARITY : v32, o.shift, v.x32, v.p32, v.q32, v.r32, v.p32b);
+    // with a one-workgroup pass in the V-cycle, its spare workgroups take x += alpha p
+    // and the update kernel does r -= alpha q alone
+    const bool x_in_pass = free_upd && vc_coarse_pass(c->ml32);
+    const std::string key = batch_key(c, vc ? (x_in_pass ? "f32vcx" : free_upd ? "f32vcf" : "f32vc") : ml ? "f32ml"
+                                                                                                       : "f32",
+                                      nr, batch, ml ? HX_MV_PARITY : v32, o.shift, v.x32, v.p32, v.q32, v.r32,
+                                      v.p32b);
     while (!all_done(c, nr)) {
         while (!all_done(c, nr)) {
             rc = run_batch(c, key, [&] {
@@ -761,7 +768,10 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
                         mv32(c, pin, v.q32, c->E32.p, o.shift, MV_PCG, v32, nr, v.r32, c->minv32.p, pout, true, &npq);
                     {
                         TimeScope ts(c, HX_K_UPDATE);
-                        if (free_upd)
+                        if (x_in_pass)
+                            launch_pcg_update_r<float>(n, v.r32, v.q32, c->st, c->partials.p + kOpPartials, npq,
+                                                       c->stream, nr);
+                        else if (free_upd)
                             launch_pcg_update_free<float>(n, v.x32, pout, v.r32, v.q32, c->st,
                                                           c->partials.p + kOpPartials, npq, c->stream, nullptr,
                                                           nr);
@@ -772,7 +782,8 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
                     }
                     if (ml) {
                         TimeScope ts(c, HX_K_PRECOND);
-                        (void)pc_apply32(c, vc, v.r32, 0, nr);  // its one failure is caught at the start call
+                        (void)pc_apply32(c, vc, v.r32, 0, nr, x_in_pass ? v.x32 : nullptr,
+                                         x_in_pass ? pout : nullptr);  // its one failure is caught at the start call
                     }
                 }
             });

@@ -288,6 +288,40 @@ __global__ __launch_bounds__(NT) void k_pcg_update_free(long n, T* __restrict__ 
     }
 }
 
+// r -= alpha q alone (k_pcg_update_free's r half, the same bits): the V-cycle
+// PCG leaves x += alpha p to the spare workgroups of its one-workgroup
+// coarse pass, which runs latency-bound on one compute unit
+template <typename T>
+__global__ __launch_bounds__(NT) void k_pcg_update_r(long n, T* __restrict__ r, const T* __restrict__ q,
+                                                     PcgState* st, const double* __restrict__ pq_partials,
+                                                     unsigned npq) {
+    __shared__ double red[NT / 64];
+    __shared__ double s_alpha;
+    {  // column blockIdx.y
+        const long off = (long)blockIdx.y * n;
+        r += off; q += off;
+        st += blockIdx.y;
+    }
+    if (st->done) return;
+    const double alpha = update_alpha(st, pq_partials + (size_t)blockIdx.y * npq, npq, red, s_alpha);
+    if (alpha != alpha) return;
+    typedef typename Vec16<T>::type VT;
+    constexpr int V = Vec16<T>::n;
+    const long nv = n / V;
+    if (aligned16(r) && aligned16(q)) {
+        for (long v = gtid(); v < nv; v += gstride()) {
+            VT rv = ((const VT*)r)[v];
+            const VT qv = ((const VT*)q)[v];
+#pragma unroll
+            for (int l = 0; l < V; ++l) rv[l] = T((double)rv[l] - alpha * (double)qv[l]);
+            ((VT*)r)[v] = rv;
+        }
+    } else {
+        for (long i = gtid(); i < nv * V; i += gstride()) r[i] = T((double)r[i] - alpha * (double)q[i]);
+    }
+    for (long i = nv * V + gtid(); i < n; i += gstride()) r[i] = T((double)r[i] - alpha * (double)q[i]);
+}
+
 // p = minv r + beta p
 template <typename T>
 __global__ __launch_bounds__(NT) void k_pcg_pupdate(long n, const T* __restrict__ r, const T* __restrict__ minv,
@@ -1196,6 +1230,12 @@ void launch_pcg_update_free(long n, T* x, const T* p, T* r, const T* q, PcgState
                        (unsigned)(pq_partials ? npq : 0), xbest);
 }
 template <typename T>
+void launch_pcg_update_r(long n, T* r, const T* q, PcgState* st, const double* pq_partials, int npq, hipStream_t s,
+                         int nrhs) {
+    hipLaunchKernelGGL((k_pcg_update_r<T>), dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, r, q, st, pq_partials,
+                       (unsigned)(pq_partials ? npq : 0));
+}
+template <typename T>
 void launch_pcg_pupdate(long n, const T* r, const T* minv, T* p, const PcgState* st, hipStream_t s, int nrhs) {
     hipLaunchKernelGGL((k_pcg_pupdate<T>), dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, r, minv, p, st);
 }
@@ -1417,6 +1457,8 @@ template void launch_pcg_update_free<double>(long, double*, const double*, doubl
                                              const double*, int, hipStream_t, double*, int);
 template void launch_pcg_update_free<float>(long, float*, const float*, float*, const float*, PcgState*,
                                             const double*, int, hipStream_t, float*, int);
+template void launch_pcg_update_r<float>(long, float*, const float*, PcgState*, const double*, int, hipStream_t,
+                                         int);
 template void launch_pcg_pupdate<double>(long, const double*, const double*, double*, const PcgState*, hipStream_t,
                                          int);
 template void launch_pcg_pupdate<float>(long, const float*, const float*, float*, const PcgState*, hipStream_t, int);

@@ -166,6 +166,11 @@ void launch_pcg_update(long n, T* x, const T* p, T* r, const T* q, const T* minv
 template <typename T>
 void launch_pcg_update_free(long n, T* x, const T* p, T* r, const T* q, PcgState* st, const double* pq_partials,
                             int npq, hipStream_t s, T* xbest = nullptr, int nrhs = 1);
+// r -= alpha q alone (the r half of launch_pcg_update_free, the same bits), for
+// a V-cycle whose one-workgroup pass takes x += alpha p (launch_vc_apply's xup)
+template <typename T>
+void launch_pcg_update_r(long n, T* r, const T* q, PcgState* st, const double* pq_partials, int npq, hipStream_t s,
+                         int nrhs = 1);
 template <typename T>
 void launch_pcg_pupdate(long n, const T* r, const T* minv, T* p, const PcgState* st, hipStream_t s, int nrhs = 1);
 // snap_mode: PcgState::snap_mode of every column
@@ -309,10 +314,17 @@ int launch_vc_coarse_stage(MlHierarchy<T>& h, T* buf, hipStream_t s);
 // coarse level.  nrhs columns (blockIdx.y) share the hierarchy: r, z and the
 // level vectors hold nrhs columns each, one state per column; nrhs > 1 needs
 // a coarse level.
+// xup, pup (fin_mode 0, a one-workgroup pass present: vc_coarse_pass): the
+// pass's launch carries spare workgroups that do xup += alpha pup per column
+// (alpha from the state, as the update kernel before it left it), beside the
+// pass.
 template <typename T>
 int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const MvShape& sh_pre, const MvShape& sh_post,
                     const T* r, T* z, PcgState* st, double* partials, long max_partials, int fin_mode, hipStream_t s,
-                    bool rr_post = false, int nrhs = 1);
+                    bool rr_post = false, int nrhs = 1, T* xup = nullptr, const T* pup = nullptr);
+// the V-cycle of h runs a one-workgroup pass (levels small enough for it exist)
+template <typename T>
+bool vc_coarse_pass(const MlHierarchy<T>& h);
 
 // ---- Helmholtz filter --------------------------------------------------------
 struct HelmArgs {

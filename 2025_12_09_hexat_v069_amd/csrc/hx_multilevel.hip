@@ -501,13 +501,42 @@ constexpr int kCoarsePre = 18;       // exact-inverse values per thread fetched 
 constexpr int kCoarseStagePre = 4;   // level-data values per thread fetched with them
 constexpr int kCoarseResPre = 8;     // values of the finer level's residual per thread fetched with them
 
+// spare workgroups of the one-workgroup pass's launch (blockIdx.x >= 1): the
+// PCG update's x half, x += alpha p over column col, alpha as the update
+// kernel left it in the state (the bits of k_pcg_update_free)
+constexpr int kCoarseXBlocks = 255;
+
 template <typename T>
 __global__ __launch_bounds__(kCoarseThreads) void k_vc_coarse(MlHierarchy<T> h, int s0, ParityK<T> K, int stage_ainv,
-                                                              const PcgState* st) {
+                                                              const PcgState* st, T* xu, const T* pu, long xn) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     T* lds = reinterpret_cast<T*>(lds_raw);
     const int col = blockIdx.y;
     if (st != nullptr && st[col].done) return;
+    if (blockIdx.x > 0) {
+        if (xu == nullptr || st == nullptr) return;
+        const double alpha = st[col].alpha;
+        T* x = xu + (long)col * xn;
+        const T* p = pu + (long)col * xn;
+        const long nthr = (long)(gridDim.x - 1) * kCoarseThreads;
+        const long tid = (long)(blockIdx.x - 1) * kCoarseThreads + threadIdx.x;
+        constexpr int V = 16 / (int)sizeof(T);
+        typedef T VT __attribute__((ext_vector_type(16 / sizeof(T))));
+        const long nv = xn / V;
+        if ((((uintptr_t)x | (uintptr_t)p) & 15u) == 0) {
+            for (long v = tid; v < nv; v += nthr) {
+                VT xv = ((const VT*)x)[v];
+                const VT pv = ((const VT*)p)[v];
+#pragma unroll
+                for (int l = 0; l < V; ++l) xv[l] = T((double)xv[l] + alpha * (double)pv[l]);
+                ((VT*)x)[v] = xv;
+            }
+        } else {
+            for (long i = tid; i < nv * V; i += nthr) x[i] = T((double)x[i] + alpha * (double)p[i]);
+        }
+        for (long i = nv * V + tid; i < xn; i += nthr) x[i] = T((double)x[i] + alpha * (double)p[i]);
+        return;
+    }
     const int L = h.levels, t = threadIdx.x;
     auto Ev = [&](int l) {  // moduli and smoother diagonal of smoothed level l
         long o = 0;
@@ -1009,9 +1038,14 @@ void launch_vc_scale_E(long ne, const double* Ec, double h, T* Eop, hipStream_t 
 }
 
 template <typename T>
+bool vc_coarse_pass(const MlHierarchy<T>& h) {
+    return h.levels >= 1 && vc_coarse_start(h) <= h.levels;
+}
+
+template <typename T>
 int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const MvShape& sh_pre, const MvShape& sh_post,
                     const T* r, T* z, PcgState* st_, double* partials, long max_partials, int fin_mode, hipStream_t s,
-                    bool rr_post, int nrhs) {
+                    bool rr_post, int nrhs, T* xup, const T* pup) {
     const int L = h.levels;
     if (rr_post && L == 0) return HX_ERR_BAD_ARG;  // no post-smoothing operator to form r'r
     if (L == 0) {  // no coarse level: plain Jacobi (the additive form's path)
@@ -1056,8 +1090,10 @@ int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const MvShape&
         hc.r[s0 - 1] = h.vres[s0 - 1];
         const bool stage = h.ainv != nullptr &&
                            vc_coarse_lds_values(h, s0, true) * (long)sizeof(T) <= kCoarseLdsBytes;
-        hipLaunchKernelGGL((k_vc_coarse<T>), dim3(1, nrhs), dim3(kCoarseThreads),
-                           vc_coarse_lds_values(h, s0, stage) * sizeof(T), s, hc, s0, K, stage ? 1 : 0, st);
+        const bool xupd = xup != nullptr && pup != nullptr && st != nullptr;
+        hipLaunchKernelGGL((k_vc_coarse<T>), dim3(xupd ? 1 + kCoarseXBlocks : 1, nrhs), dim3(kCoarseThreads),
+                           vc_coarse_lds_values(h, s0, stage) * sizeof(T), s, hc, s0, K, stage ? 1 : 0, st,
+                           xupd ? xup : nullptr, xupd ? pup : nullptr, 3 * h.g[0].nn);
     } else {  // no level fits it: the coarsest level by its diagonal
         restrict_to(L, true);
     }
@@ -1140,8 +1176,11 @@ template int launch_vc_coarse_stage<double>(MlHierarchy<double>&, double*, hipSt
 template int launch_vc_coarse_stage<float>(MlHierarchy<float>&, float*, hipStream_t);
 template int launch_vc_apply<double>(const MlHierarchy<double>&, const ParityK<double>&, const MvShape&,
                                      const MvShape&, const double*, double*, PcgState*, double*, long, int,
-                                     hipStream_t, bool, int);
+                                     hipStream_t, bool, int, double*, const double*);
 template int launch_vc_apply<float>(const MlHierarchy<float>&, const ParityK<float>&, const MvShape&, const MvShape&,
-                                    const float*, float*, PcgState*, double*, long, int, hipStream_t, bool, int);
+                                    const float*, float*, PcgState*, double*, long, int, hipStream_t, bool, int,
+                                    float*, const float*);
+template bool vc_coarse_pass<double>(const MlHierarchy<double>&);
+template bool vc_coarse_pass<float>(const MlHierarchy<float>&);
 
 }  // namespace hx
