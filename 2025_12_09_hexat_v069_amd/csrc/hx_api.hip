@@ -162,6 +162,7 @@ struct hx_ctx {
     bool to_ready = false;
     hx_to_params tp{};
     int to_iter = 0;
+    double oc_lambda = 0.0;  // the last TO iteration's OC multiplier (0: none yet)
     DevBuf<double> t_x, t_xphys, t_xnew, t_dc, t_dcf, t_dv, t_vm, t_ones;
     // learned preconditioner
     UNet* unet = nullptr;
@@ -1105,9 +1106,10 @@ void material_consts(double nu, double* l1, double* l2, double* mu) {
 // OC update with the multiplier found by bisection (top88, hx_oc_update) or,
 // with kary, by the 8-point search (9x bracket reduction per pass, the same
 // stopping rule; the device-resident loop)
+// lguess > 0 (8-point search only): start from a bracket around this multiplier
 int oc_run(hx_ctx* c, long ne, const double* x, const double* dc, const double* dv, double volfrac, double move,
-           double xmin, double rel_tol, double* xnew, bool kary = false) {
-    launch_oc_init(c->st, c->stream);
+           double xmin, double rel_tol, double* xnew, bool kary = false, double lguess = 0.0) {
+    launch_oc_init(c->st, c->stream, kary ? lguess : 0.0);
     int rc;
     // chunks of passes replayed as one graph per field set and parameters
     // (a finished search returns at once from the remaining passes)
@@ -1965,6 +1967,7 @@ int hx_to_init(hx_ctx* c, const hx_to_params* p, const double* f, const double* 
     if ((rc = helm_solve(c, R, fo, c->t_x.p, c->t_xphys.p, nullptr))) return rc;
     if ((rc = helm_solve(c, R, fo, c->t_ones.p, c->t_dv.p, nullptr))) return rc;  // dv = H 1
     c->to_iter = 0;
+    c->oc_lambda = 0.0;
     c->to_ready = true;
     return HX_OK;
 }
@@ -2036,9 +2039,11 @@ int hx_to_step(hx_ctx* c, hx_to_stats* s) {
     hx_filter_stats f1{}, f2{};
     if ((rc = helm_solve(c, R, fo, c->t_dc.p, c->t_dcf.p, &f1))) return rc;
     // 5. OC (volume of the mass-preserving filter equals that of x_new)
+    // the multiplier moves little between TO iterations: the search starts around the last one
     if ((rc = oc_run(c, ne, c->t_x.p, c->t_dcf.p, c->t_dv.p, p.volfrac, p.move, p.xmin, p.oc_rel_tol, c->t_xnew.p,
-                     true)))
+                     true, c->oc_lambda)))
         return rc;
+    c->oc_lambda = c->st_host->scal[2];
     const double change = c->st_host->scal[3];
     // 6. density filter
     if ((rc = helm_solve(c, R, fo, c->t_xnew.p, c->t_xphys.p, &f2))) return rc;

@@ -150,7 +150,7 @@ struct PcgState {
     // anyway; mode 2 = copies at the FP32 form's check points only; 0 = none
     int snap_pending;
     int snap_mode;
-    int pad;
+    int oc_widen;    // 8-point OC search: its first pass starts from a guessed bracket
     unsigned ticket[4];
 };
 

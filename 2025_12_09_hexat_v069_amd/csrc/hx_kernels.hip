@@ -391,7 +391,8 @@ __global__ __launch_bounds__(NT) void k_true_residual(long n, const double* __re
         const double m = minv[i];
         const bool fr = m != 0.0;
         const double bi = b[i];
-        const double ri = fr ? bi - (q[i] + shift * x[i]) : 0.0;
+        const double sx = shift != 0.0 ? shift * x[i] : 0.0;  // x is not read without a shift
+        const double ri = fr ? bi - (q[i] + sx) : 0.0;
         if (r64) r64[i] = ri;
         const float rf = (float)ri;
         if (r32) r32[i] = rf;
@@ -956,14 +957,19 @@ __device__ __forceinline__ double oc_candidate(double x, double dc, double dv, d
     return fmin(fmax(cand, lo), hi);
 }
 
-__global__ void k_oc_init(PcgState* st) {
+// [l1, l2] = [0, 1e9] (top88), or with lguess > 0 the guess [lguess/16,
+// 16 lguess] for the 8-point search, whose first pass widens the bracket to
+// 0 or 1e9 on the side where the volume does not cross inside it (st->oc_widen)
+__global__ void k_oc_init(PcgState* st, double lguess) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        st->scal[0] = 0.0;  // l1
-        st->scal[1] = 1e9;  // l2
+        const bool warm = lguess > 0.0;
+        st->scal[0] = warm ? lguess / 16.0 : 0.0;  // l1
+        st->scal[1] = warm ? lguess * 16.0 : 1e9;  // l2
         st->scal[2] = 0.0;  // lmid
         st->scal[3] = 0.0;  // change
         st->iter = 0;
         st->done = 0;
+        st->oc_widen = warm ? 1 : 0;
         for (int i = 0; i < 4; ++i) st->ticket[i] = 0;
     }
 }
@@ -1044,7 +1050,12 @@ __global__ __launch_bounds__(NT) void k_fin_oc_eval_k(PcgState* st, const double
         const double l1 = st->scal[0], l2 = st->scal[1], target = volfrac * (double)ne;
         int j = 0;  // the first candidate whose volume is within the target
         while (j < kOcK && t[j] > target) ++j;
-        const double lo = oc_point(l1, l2, j), hi = j == kOcK ? l2 : oc_point(l1, l2, j + 1);
+        double lo = oc_point(l1, l2, j), hi = j == kOcK ? l2 : oc_point(l1, l2, j + 1);
+        if (st->oc_widen) {  // first pass from a guessed bracket: the crossing may lie outside it
+            if (j == 0) lo = 0.0;
+            if (j == kOcK) hi = 1e9;
+            st->oc_widen = 0;
+        }
         st->scal[0] = lo;
         st->scal[1] = hi;
         st->scal[2] = 0.5 * (lo + hi);
@@ -1426,7 +1437,9 @@ void launch_mul(long n, const double* a, const double* b, double* out, hipStream
 void launch_masked_diff(long n, const double* a, const double* b, const double* minv, double* e, hipStream_t s) {
     hipLaunchKernelGGL(k_masked_diff, dim3(kVecBlocks), dim3(NT), 0, s, n, a, b, minv, e);
 }
-void launch_oc_init(PcgState* st, hipStream_t s) { hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(64), 0, s, st); }
+void launch_oc_init(PcgState* st, hipStream_t s, double lguess) {
+    hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(64), 0, s, st, lguess);
+}
 void launch_oc_eval_k(long ne, const double* x, const double* dc, const double* dv, double volfrac, double move,
                       double xmin, double rel_tol, PcgState* st, double* partials, hipStream_t s) {
     hipLaunchKernelGGL(k_oc_eval_k, dim3(kVecBlocks), dim3(NT), 0, s, ne, x, dc, dv, move, xmin, st, partials);

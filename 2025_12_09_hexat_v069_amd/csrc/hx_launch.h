@@ -373,7 +373,9 @@ struct ElemArgs {
 void launch_element_post(const GridArgs& g, const ParityK<double>& Kunit, const ElemArgs& a, hipStream_t s);
 int element_post_blocks(const GridArgs& g);
 void launch_simp_modulus(long ne, const double* xphys, double penal, double E0, double Emin, double* E, hipStream_t s);
-void launch_oc_init(PcgState* st, hipStream_t s);
+// lguess > 0: the 8-point search starts from [lguess/16, 16 lguess] (widened on its first pass when the
+// volume does not cross inside); otherwise [0, 1e9]
+void launch_oc_init(PcgState* st, hipStream_t s, double lguess = 0.0);
 // one pass of the 8-point search for the same multiplier (the device-resident
 // loop's OC): bracket shrunk 9x per pass; partials: 8 per block
 void launch_oc_eval_k(long ne, const double* x, const double* dc, const double* dv, double volfrac, double move,
