@@ -801,7 +801,10 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
         // the phase failed (its folded correction is then not an iterate)
         if (v.xbest && c->st_host[0].status != 4 && c->st_host[0].status != 5)
             launch_snapshot(n, v.x, v.xbest, c->st, c->stream);
-        if (ml && (rc = pc_apply32(c, vc, v.r32, 2, nr))) return rc;
+        // the phase's last preconditioner application left z = M^-1 r of the
+        // recursive residual, which the true one matches to FP32 rounding:
+        // the resumed phase keeps it and takes r'z with the true r
+        if (ml) launch_rz_park<float>(n, v.r32, c->ml_z32.p, c->st, c->partials.p, c->stream, nr);
         launch_ir_resume(c->st, o.tol, c->ir_delta, 0, c->stream, nr);
         HXK();
         ++refinements;

@@ -350,6 +350,27 @@ __global__ __launch_bounds__(NT) void k_dot(long n, const double* __restrict__ a
     if (threadIdx.x == 0) partials[blockIdx.x] = v;
 }
 
+// r'z per column (blockIdx.y) of FP32 vectors with FP64 sums, parked in
+// scal[5] for k_ir_resume: a reliable update of an explicitly preconditioned
+// FP32 phase keeps the last z (of the recursive residual, which the true one
+// matches to FP32 rounding) instead of applying the preconditioner again
+template <typename T>
+__global__ __launch_bounds__(NT) void k_rz_part(long n, const T* __restrict__ r, const T* __restrict__ z,
+                                                double* partials) {
+    __shared__ double red[NT / 64];
+    r += (long)blockIdx.y * n;
+    z += (long)blockIdx.y * n;
+    double s = 0.0;
+    for (long i = gtid(); i < n; i += gstride()) s += (double)r[i] * (double)z[i];
+    const double v = block_sum<NT>(s, red);
+    if (threadIdx.x == 0) partials[(long)blockIdx.y * gridDim.x + blockIdx.x] = v;
+}
+__global__ __launch_bounds__(NT) void k_fin_rz_park(PcgState* st0, const double* partials, unsigned nb) {
+    __shared__ double red[NT / 64];
+    const double t = reduce_partials<NT>(partials + (size_t)blockIdx.x * nb, 1, 0, nb, red);
+    if (threadIdx.x == 0) st0[blockIdx.x].scal[5] = t;
+}
+
 __global__ __launch_bounds__(NT) void k_fin_dot(double* out, const double* partials, unsigned nb) {
     __shared__ double red[NT / 64];
     const double t = reduce_partials<NT>(partials, 1, 0, nb, red);
@@ -1262,6 +1283,13 @@ void launch_dot(long n, const double* a, const double* b, double* out, unsigned*
 void launch_fill(long n, double v, double* a, hipStream_t s) {
     hipLaunchKernelGGL(k_fill, dim3(kVecBlocks), dim3(NT), 0, s, n, v, a);
 }
+template <typename T>
+void launch_rz_park(long n, const T* r, const T* z, PcgState* st, double* partials, hipStream_t s, int nrhs) {
+    hipLaunchKernelGGL((k_rz_part<T>), dim3(kVecBlocks, nrhs), dim3(NT), 0, s, n, r, z, partials);
+    hipLaunchKernelGGL(k_fin_rz_park, dim3(nrhs), dim3(NT), 0, s, st, partials, (unsigned)kVecBlocks);
+}
+template void launch_rz_park<float>(long, const float*, const float*, PcgState*, double*, hipStream_t, int);
+template void launch_rz_park<double>(long, const double*, const double*, PcgState*, double*, hipStream_t, int);
 void launch_fill32(long n, float v, float* a, hipStream_t s) {
     hipLaunchKernelGGL(k_fill32, dim3(kVecBlocks), dim3(NT), 0, s, n, v, a);
 }

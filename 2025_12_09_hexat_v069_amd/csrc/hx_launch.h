@@ -181,6 +181,9 @@ void launch_pcg_reset(PcgState* st, double tol, double shift, int max_iter, int 
 void launch_dot(long n, const double* a, const double* b, double* out, unsigned* ticket, double* partials,
                 hipStream_t s);
 void launch_fill(long n, double v, double* a, hipStream_t s);
+// per column: st->scal[5] = r'z (FP64 sums), for k_ir_resume after a reliable update that keeps z
+template <typename T>
+void launch_rz_park(long n, const T* r, const T* z, PcgState* st, double* partials, hipStream_t s, int nrhs = 1);
 void launch_fill32(long n, float v, float* a, hipStream_t s);
 template <typename Tin, typename Tout>
 void launch_convert(long n, const Tin* a, Tout* b, hipStream_t s);
