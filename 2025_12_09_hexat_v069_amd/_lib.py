@@ -52,7 +52,7 @@ EXPORTS = [
 #: test hooks declared in include/hexat_testing.h
 TEST_EXPORTS = ["hx_test_unet_nparam", "hx_test_unet_init", "hx_test_unet_forward", "hx_test_unet_backward",
                 "hx_test_copy", "hx_test_ml_apply", "hx_test_vc_apply", "hx_test_device_alloc",
-                "hx_test_device_free", "hx_test_memcpy"]
+                "hx_test_device_free", "hx_test_memcpy", "hx_test_oc_search"]
 
 
 class HexatError(RuntimeError):
@@ -193,6 +193,7 @@ def load():
         L.hx_test_device_free.argtypes = [vp, vp]
         L.hx_test_memcpy.argtypes = [vp, vp, vp, C.c_long, i]
         L.hx_test_ml_apply.argtypes = [vp, pd, pd, pd, i, P(C.c_int)]
+        L.hx_test_oc_search.argtypes = [vp, pd, pd, pd, d, d, d, d, d, pd, pd, P(C.c_int)]
         L.hx_test_vc_apply.argtypes = [vp, pd, pd, pd, i, P(C.c_int), P(C.c_int)]
         pf = P(C.c_float)
         L.hx_test_unet_nparam.argtypes = [vp, P(C.c_long)]
@@ -495,6 +496,16 @@ class Context:
                                        float(xmin), float(rel_tol), dptr(xn), C.byref(lm), C.byref(st),
                                        C.byref(ch)), "hx_oc_update")
         return xn, lm.value, st.value, ch.value
+
+    def test_oc_search(self, x, dc, dv, volfrac, move=0.2, xmin=0.0, rel_tol=1e-9, lguess=0.0):
+        """hx_test_oc_search: the SIMP loop's 8-point OC search -> (x_new, multiplier, passes)"""
+        x, dc, dv = f64(x, self.ne), f64(dc, self.ne), f64(dv, self.ne)
+        xn = np.empty(self.ne)
+        lm, st = C.c_double(0), C.c_int(0)
+        self.check(self.L.hx_test_oc_search(self.h, dptr(x), dptr(dc), dptr(dv), float(volfrac), float(move),
+                                            float(xmin), float(rel_tol), float(lguess), dptr(xn), C.byref(lm),
+                                            C.byref(st)), "hx_test_oc_search")
+        return xn, lm.value, st.value
 
     # ---- device-resident SIMP loop ------------------------------------------------
     def to_init(self, params: TOParams, f, x0):

@@ -1937,6 +1937,29 @@ int hx_oc_update(hx_ctx* c, const double* x, const double* dc, const double* dv,
     return download(c, xnew, c->e_vm.p, ne * sizeof(double));
 }
 
+// the device-resident loop's OC search (8-point, optionally from a guessed
+// bracket around lguess) on caller arrays
+int hx_test_oc_search(hx_ctx* c, const double* x, const double* dc, const double* dv, double volfrac, double move,
+                      double xmin, double rel_tol, double lguess, double* xnew, double* lmid, int* passes) {
+    if (!c || !x || !dc || !dv || !xnew || !(rel_tol > 0) || !(move > 0)) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    const long ne = c->g.ne;
+    HXA(c->e_ce, ne);
+    HXA(c->e_dc, ne);
+    HXA(c->e_aux, ne);
+    HXA(c->e_vm, ne);
+    int rc;
+    if ((rc = upload(c, c->e_aux.p, x, ne * sizeof(double))) || (rc = upload(c, c->e_dc.p, dc, ne * sizeof(double))) ||
+        (rc = upload(c, c->e_ce.p, dv, ne * sizeof(double))))
+        return rc;
+    if ((rc = oc_run(c, ne, c->e_aux.p, c->e_dc.p, c->e_ce.p, volfrac, move, xmin, rel_tol, c->e_vm.p, true,
+                     lguess)))
+        return rc;
+    if (lmid) *lmid = c->st_host->scal[2];
+    if (passes) *passes = c->st_host->iter;
+    return download(c, xnew, c->e_vm.p, ne * sizeof(double));
+}
+
 // ------------------------------------------------------- SIMP/OC session
 int hx_to_init(hx_ctx* c, const hx_to_params* p, const double* f, const double* x0) {
     if (!c || !p || !f || !x0) return HX_ERR_BAD_ARG;

@@ -38,6 +38,14 @@ int hx_test_ml_apply(hx_ctx* ctx, const double* Ee, const double* r, double* z, 
 int hx_test_vc_apply(hx_ctx* ctx, const double* Ee, const double* r, double* z, int precision, int* levels,
                      int* vlevels);
 
+/* The SIMP loop's OC multiplier search (8 points per pass, hx_to_step's) on
+ * caller arrays (Ne each): from [0, 1e9], or with lguess > 0 from
+ * [lguess/16, 16 lguess], which its first pass widens to 0 or 1e9 on the side
+ * where the volume does not cross inside; x_new, the multiplier and the
+ * number of passes.  No reference counterpart (top88's OC, lines 55-64). */
+int hx_test_oc_search(hx_ctx* ctx, const double* x, const double* dc, const double* dv, double volfrac, double move,
+                      double xmin, double rel_tol, double lguess, double* xnew, double* lmid, int* passes);
+
 /* number of network parameters (conv weights then bias, layer by layer) */
 int hx_test_unet_nparam(hx_ctx* ctx, long* nparam);
 /* (re)initialise the network weights with a seed; copy the FP32 master

@@ -207,6 +207,28 @@ def test_oc_matches_oracle(ox, ctx):
     assert abs(ch - np.abs(xo - x).max()) < 1e-14
 
 
+def test_oc_search_from_a_guessed_bracket(ox, ctx):
+    """The SIMP loop's 8-point search finds top88's multiplier from [0, 1e9]
+    and from a guessed bracket [g/16, 16 g] around any guess g: inside it,
+    and with the crossing below or above it, where the first pass has to
+    widen the bracket to 0 or to 1e9.  A close guess needs fewer passes."""
+    g = _grid_setup(ox, ctx, (20, 10, 10))
+    rng = np.random.default_rng(9)
+    x = rng.uniform(0.05, 0.9, g.ne)
+    dc = -rng.uniform(0.01, 5.0, g.ne)
+    dv = rng.uniform(0.9, 1.1, g.ne)
+    xo, lo, _ = ox.oc(x, dc, dv, 0.3, move=0.2, xmin=0.0, rel_tol=1e-12)
+    _, _, cold = ctx.test_oc_search(x, dc, dv, 0.3, rel_tol=1e-9)
+    for guess in (0.0, 1.3 * lo, lo / 7.0, 300.0 * lo, lo / 300.0):
+        xn, lm, passes = ctx.test_oc_search(x, dc, dv, 0.3, rel_tol=1e-9, lguess=guess)
+        _say("oc search guess", guess / lo, "passes", passes, "cold", cold)
+        assert abs(lm - lo) <= 2e-9 * lo, (guess, lm, lo)
+        assert np.allclose(xn, xo, rtol=2e-9, atol=1e-12)
+        assert abs(xn.mean() - 0.3) < 1e-8
+        if 0.1 < guess / lo < 10.0:
+            assert passes < cold
+
+
 @pytest.mark.parametrize("dims,precision,n_iter", [((24, 8, 8), 0, 4), ((24, 8, 8), 1, 4), ((60, 20, 20), 0, 3)])
 def test_simp_loop_matches_oracle(ox, hx, ctx, dims, precision, n_iter):
     _say("simp loop", dims, precision, n_iter)
