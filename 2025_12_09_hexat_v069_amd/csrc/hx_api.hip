@@ -32,7 +32,8 @@ void unet_destroy(UNet* u);
 long unet_nparam(const UNet* u);
 bool unet_ready(const UNet* u);
 bool unet_bind(UNet* u, const GridArgs& g, unsigned seed, hipStream_t s);
-void unet_forward_sample(UNet* u, const double* r, const double* E, double inv_scale, hipStream_t s);
+void unet_forward_sample(UNet* u, const double* r, const double* E, double inv_scale, hipStream_t s,
+                         const double* rr_dev = nullptr);
 void unet_direction(UNet* u, const double* minv, double scale, double* y, hipStream_t s);
 void unet_subspace_grad(UNet* u, const double* r, const double* Au, const double* Ay, const double* minv, double a,
                         double b, double scale, hipStream_t s);
@@ -167,7 +168,7 @@ struct hx_ctx {
     // learned preconditioner
     UNet* unet = nullptr;
     bool unet_trained = false;
-    DevBuf<double> l_z, l_zold, l_P, l_AP, l_dots, l_ustar, l_samples, l_rho, l_man;
+    DevBuf<double> l_z, l_zold, l_P, l_AP, l_dots, l_ustar, l_samples, l_rho, l_man, l_part;
     int l_nsamples = 0;
     long l_sample_stride = 0;
     // multilevel preconditioner: level moduli, inverse diagonals, work vectors
@@ -186,6 +187,7 @@ struct hx_ctx {
     int vc_min_blocks = 512;  // coarse levels' launch width in workgroups (two per compute unit)
     // FP32 form: reliable update each time a column's recursive residual falls by this factor
     double ir_delta = kIrDelta;
+    int fcg_keep = kFcgKeep;  // search directions the learned flexible CG keeps (the newest keep-1 orthogonalise)
     DevBuf<double> vc_s[kMlMaxLevels + 1], vc_res[kMlMaxLevels + 1], vc_E[kMlMaxLevels + 1];
     DevBuf<float> vc_s32[kMlMaxLevels + 1], vc_res32[kMlMaxLevels + 1], vc_E32[kMlMaxLevels + 1];
     DevBuf<double> vc_cstage;  // the one-workgroup pass's level data, contiguous (MlHierarchy::cstage)
@@ -837,10 +839,14 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
 // one two-column launch and takes z = a u + b y with (a, b) minimising the
 // energy error over span{u, y} -- which needs no knowledge of the error since
 // A e = r:  [u'Au u'Ay; u'Ay y'Ay] (a, b) = (u'r, y'r).  The search direction
-// is z A-orthogonalised against the last kFcgKeep-1 ones (truncated flexible
-// CG), whose images under A are kept, so A p costs no operator application.
-// Jacobi-PCG takes over from the current iterate when the flexible iteration
-// stalls or breaks down, and the true residual judges convergence.
+// is z A-orthogonalised against the last keep-1 ones (truncated flexible CG;
+// keep = 8, HEXAT_FCG_KEEP: 4 kept took 1.6x the iterations, 12 and 16 no
+// fewer), whose images under A are kept, so A p costs no operator
+// application.  The vector work is three fused passes (the dot products; the
+// direction with p'Ap and p'r; the step with r'r), alpha stays on the device,
+// and the host synchronises once per iteration.  Jacobi-PCG takes over from
+// the current iterate when the flexible iteration stalls or breaks down, and
+// the true residual judges convergence.
 // stall: no 1% drop of the best residual for max(kLearnedStall, it/2)
 // iterations (CG residuals plateau for ~100 iterations on healthy runs)
 constexpr int kLearnedStall = 200;
@@ -862,7 +868,7 @@ int pcg_learned(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
     const long n = ndof(c);
     HXA(c->l_z, 2 * n);     // W = [u | y]
     HXA(c->l_zold, 2 * n);  // A W
-    HXA(c->l_dots, 8 + 2 * kFcgKeep);
+    HXA(c->l_dots, kFcgDots + 8);
     hipEvent_t ev0 = get_event(c), ev1 = get_event(c);
     (void)hipEventRecord(ev0, c->stream);
     launch_elastic_minv<double>(c->g, c->kd, c->E.p, c->has_fixed ? c->fixed.p : nullptr, o.shift, c->minv.p,
@@ -884,9 +890,47 @@ int pcg_learned(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
     double* AW = c->l_zold.p;
     HXA(c->l_P, (long)kFcgKeep * n);   // stored search directions
     HXA(c->l_AP, (long)kFcgKeep * n);  // and their images under A
+    HXA(c->l_part, (long)kVecBlocks * kFcgDots);
+    double* dots = c->l_dots.p;              // [0, kFcgDots): the iteration's dot products
+    double* scal = c->l_dots.p + kFcgDots;   // p'Ap, p'r, alpha, breakdown flag, r'r of the last step
     double pAp_j[kFcgKeep];
-    int nstored = 0, next = 0;  // ring of kFcgKeep slots; the newest kFcgKeep-1 orthogonalise
+    // ring of `keep` slots (<= kFcgKeep); the newest keep-1 directions orthogonalise
+    const int keep = c->fcg_keep;
+    int nstored = 0, next = 0, last = -1;
+    // Per iteration one host synchronisation: the dot products of this
+    // iteration arrive with the last step's p'Ap, breakdown flag and r'r, so
+    // its convergence is judged one iteration late (the last network
+    // evaluation of a converged solve is discarded)
+    HXC(hipMemcpyAsync(scal + 4, &rr, sizeof(double), hipMemcpyHostToDevice, c->stream));  // r'r of the start
     while (status < 0) {
+        launch_mul(n, c->minv.p, c->r.p, W, c->stream);                       // u
+        // the network sees r / rms(r) with r'r of the last step (on the device);
+        // y keeps the network's scale (the energy-optimal pair (a, b) absorbs it)
+        unet_forward_sample(c->unet, c->r.p, c->E.p, 0.0, c->stream, scal + 4);
+        unet_direction(c->unet, c->minv.p, 1.0, W + n, c->stream);            // y
+        mv64(c, W, AW, c->E.p, o.shift, MV_PLAIN, HX_MV_AUTO, 2);             // [Au | Ay]
+        FcgCoef cf{};
+        cf.m = std::min(nstored, keep - 1);
+        for (int j = 0; j < cf.m; ++j) cf.slot[j] = (next - 1 - j + 2 * keep) % keep;
+        launch_fcg_dots(n, W, AW, c->r.p, cf, c->l_P.p, c->l_part.p, dots, c->stream);
+        double h[kFcgDots + 5];
+        HXC(hipMemcpyAsync(h, dots, (kFcgDots + 5) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HXC(hipStreamSynchronize(c->stream));
+        if (last >= 0) {  // the last step: its direction's p'Ap, breakdown, residual
+            if (h[kFcgDots + 3] != 0.0) {
+                status = 4;
+                break;
+            }
+            pAp_j[last] = h[kFcgDots + 0];
+            rr = h[kFcgDots + 4];
+            rel = std::sqrt(rr) / normb;
+            ++it;
+            if (!(rel == rel) || rel > 1e3) {
+                status = 5;
+                break;
+            }
+            if (rel < 0.99 * best) best = rel, best_it = it;
+        }
         if (rel < o.tol) {
             status = 0;
             break;
@@ -899,29 +943,6 @@ int pcg_learned(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
             stalled = true;
             break;
         }
-        const double rms = std::sqrt(rr / (double)n);
-        launch_mul(n, c->minv.p, c->r.p, W, c->stream);                       // u
-        unet_forward_sample(c->unet, c->r.p, c->E.p, 1.0 / rms, c->stream);
-        unet_direction(c->unet, c->minv.p, rms, W + n, c->stream);            // y
-        mv64(c, W, AW, c->E.p, o.shift, MV_PLAIN, HX_MV_AUTO, 2);             // [Au | Ay]
-        FcgCoef cf{};
-        cf.m = std::min(nstored, kFcgKeep - 1);
-        for (int j = 0; j < cf.m; ++j) cf.slot[j] = (next - 1 - j + 2 * kFcgKeep) % kFcgKeep;
-        double h[5 + 2 * kFcgKeep];
-        {
-            int k = 0;
-            const std::pair<const double*, const double*> base[5] = {
-                {W, AW}, {W, AW + n}, {W + n, AW + n}, {W, c->r.p}, {W + n, c->r.p}};
-            for (const auto& q : base)
-                launch_dot(n, q.first, q.second, c->l_dots.p + k++, nullptr, c->partials.p, c->stream);
-            for (int j = 0; j < cf.m; ++j) {
-                const double* Pj = c->l_P.p + (long)cf.slot[j] * n;
-                launch_dot(n, AW, Pj, c->l_dots.p + k++, nullptr, c->partials.p, c->stream);
-                launch_dot(n, AW + n, Pj, c->l_dots.p + k++, nullptr, c->partials.p, c->stream);
-            }
-            HXC(hipMemcpyAsync(h, c->l_dots.p, k * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-            HXC(hipStreamSynchronize(c->stream));
-        }
         const double uAu = h[0], uAy = h[1], yAy = h[2], ur = h[3], yr = h[4];
         const double det = uAu * yAy - uAy * uAy;
         double a = ur / uAu, b = 0.0;  // Jacobi alone when y adds nothing usable
@@ -930,25 +951,13 @@ int pcg_learned(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
             cf.c[j] = (a * h[5 + 2 * j] + b * h[6 + 2 * j]) / pAp_j[cf.slot[j]];
         double* pn = c->l_P.p + (long)next * n;
         double* apn = c->l_AP.p + (long)next * n;
-        launch_fcg_direction(n, a, b, W, AW, cf, c->l_P.p, c->l_AP.p, pn, apn, c->stream);
-        double g[2];
-        if ((rc = dots_host(c, n, {{pn, apn}, {pn, c->r.p}}, g))) return rc;
-        if (!(g[0] > 0.0) || !(g[1] > 0.0)) {
-            status = 4;
-            break;
-        }
-        pAp_j[next] = g[0];
-        next = (next + 1) % kFcgKeep;
-        nstored = std::min(nstored + 1, kFcgKeep);
-        launch_fcg_step(n, g[1] / g[0], c->x.p, pn, c->r.p, apn, c->minv.p, c->stream);
-        if ((rc = dots_host(c, n, {{c->r.p, c->r.p}}, &rr))) return rc;
-        rel = std::sqrt(rr) / normb;
-        ++it;
-        if (!(rel == rel) || rel > 1e3) {
-            status = 5;
-            break;
-        }
-        if (rel < 0.99 * best) best = rel, best_it = it;
+        launch_fcg_direction2(n, a, b, W, AW, cf, c->l_P.p, c->l_AP.p, c->r.p, pn, apn, c->l_part.p, scal,
+                              c->stream);
+        launch_fcg_step2(n, scal, c->x.p, pn, c->r.p, apn, c->minv.p, c->l_part.p, scal, c->stream);
+        HXK();
+        last = next;
+        next = (next + 1) % keep;
+        nstored = std::min(nstored + 1, keep);
     }
     (void)hipEventRecord(ev1, c->stream);
     (void)hipEventSynchronize(ev1);
@@ -1172,6 +1181,10 @@ int hx_ctx_create(int device, hx_ctx** out) {
     if (const char* ev = std::getenv("HEXAT_NO_GRAPH")) c->graphs_on = ev[0] == '0';
     // cap on the multilevel preconditioner's coarse levels (default: every 2:1 coarsening)
     if (const char* ev = std::getenv("HEXAT_ML_LEVELS")) c->ml_max_levels = std::max(0, std::atoi(ev));
+    if (const char* ev = std::getenv("HEXAT_FCG_KEEP")) {  // learned flexible CG: directions kept, 2..8
+        const int k = std::atoi(ev);
+        if (k >= 2 && k <= kFcgKeep) c->fcg_keep = k;
+    }
     if (const char* ev = std::getenv("HEXAT_ML_EXACT")) c->ml_exact = ev[0] != '0';
     if (const char* ev = std::getenv("HEXAT_IR_DELTA")) {  // reliable-update factor of the FP32 form, 0 < d < 1
         const double d = std::atof(ev);

@@ -1108,15 +1108,65 @@ __global__ __launch_bounds__(NT) void k_fin_max(double* out, const double* parti
 }
 
 // ---- flexible PCG (variable preconditioner z = P(r)) -----------------------
-// x += alpha p ; r = free ? r - alpha q : 0 ; rr -> convergence (no beta)
 // Flexible CG over the two-direction preconditioner of the learned solve
 // (truncated FCG): z = a u + b y, then p = z - sum_j c_j P_j and
 // Ap = Az - sum_j c_j AP_j over the last m stored directions (A p from the
 // stored images, no operator application)
-__global__ __launch_bounds__(NT) void k_fcg_direction(long n, double a, double b, const double* __restrict__ W,
-                                                      const double* __restrict__ AW, FcgCoef cf,
-                                                      const double* __restrict__ P, const double* __restrict__ AP,
-                                                      double* __restrict__ p, double* __restrict__ Ap) {
+// Fused forms of the learned flexible CG's vector work (one pass each):
+// k_fcg_dots: the iteration's dot products, out[0..4] = u'Au, u'Ay, y'Ay, u'r,
+// y'r (W = [u | y], AW = [Au | Ay]) and out[5 + 2j], out[6 + 2j] = Au'P_j,
+// Ay'P_j over the cf.m stored directions, as per-block partials
+// [block][kFcgDots] that k_fin_fcg_dots reduces (one block per dot).
+__global__ __launch_bounds__(NT) void k_fcg_dots(long n, const double* __restrict__ W, const double* __restrict__ AW,
+                                                 const double* __restrict__ r, FcgCoef cf,
+                                                 const double* __restrict__ P, double* __restrict__ partials) {
+    __shared__ double red[NT / 64];
+    double s[kFcgDots];
+#pragma unroll
+    for (int d = 0; d < kFcgDots; ++d) s[d] = 0.0;
+    const double* Pj[kFcgKeep - 1];
+#pragma unroll
+    for (int j = 0; j < kFcgKeep - 1; ++j) Pj[j] = P + (long)cf.slot[j < cf.m ? j : 0] * n;
+    for (long i = gtid(); i < n; i += gstride()) {
+        const double u = W[i], y = W[n + i], au = AW[i], ay = AW[n + i], ri = r[i];
+        s[0] += u * au;
+        s[1] += u * ay;
+        s[2] += y * ay;
+        s[3] += u * ri;
+        s[4] += y * ri;
+#pragma unroll
+        for (int j = 0; j < kFcgKeep - 1; ++j)
+            if (j < cf.m) {
+                const double pj = Pj[j][i];
+                s[5 + 2 * j] += au * pj;
+                s[6 + 2 * j] += ay * pj;
+            }
+    }
+#pragma unroll
+    for (int d = 0; d < kFcgDots; ++d) {
+        const double v = block_sum<NT>(s[d], red);
+        if (threadIdx.x == 0) partials[(long)blockIdx.x * kFcgDots + d] = v;
+    }
+}
+__global__ __launch_bounds__(NT) void k_fin_fcg_dots(const double* partials, unsigned nb, int ndot, double* out) {
+    __shared__ double red[NT / 64];
+    const int d = blockIdx.x;
+    if (d >= ndot) return;
+    const double t = reduce_partials<NT>(partials, kFcgDots, d, nb, red);
+    if (threadIdx.x == 0) out[d] = t;
+}
+
+// k_fcg_direction2: the new direction p and Ap, with p'Ap and p'r; its
+// finaliser leaves scal[0] = p'Ap, scal[1] = p'r, scal[2] = alpha = p'r / p'Ap
+// (0 and scal[3] = 1 when either is not positive: breakdown, the step does
+// nothing) for k_fcg_step2, which then forms r'r (scal[4])
+__global__ __launch_bounds__(NT) void k_fcg_direction2(long n, double a, double b, const double* __restrict__ W,
+                                                       const double* __restrict__ AW, FcgCoef cf,
+                                                       const double* __restrict__ P, const double* __restrict__ AP,
+                                                       const double* __restrict__ r, double* __restrict__ p,
+                                                       double* __restrict__ Ap, double* __restrict__ partials) {
+    __shared__ double red[NT / 64];
+    double pap = 0.0, pr = 0.0;
     for (long i = gtid(); i < n; i += gstride()) {
         double pi = a * W[i] + b * W[n + i];
         double api = a * AW[i] + b * AW[n + i];
@@ -1126,17 +1176,49 @@ __global__ __launch_bounds__(NT) void k_fcg_direction(long n, double a, double b
         }
         p[i] = pi;
         Ap[i] = api;
+        pap += pi * api;
+        pr += pi * r[i];
+    }
+    const double v0 = block_sum<NT>(pap, red);
+    const double v1 = block_sum<NT>(pr, red);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = v0;
+        partials[2 * blockIdx.x + 1] = v1;
     }
 }
-
-// x += alpha p ; r = free ? r - alpha Ap : 0
-__global__ __launch_bounds__(NT) void k_fcg_step(long n, double alpha, double* __restrict__ x,
-                                                 const double* __restrict__ p, double* __restrict__ r,
-                                                 const double* __restrict__ Ap, const double* __restrict__ minv) {
+__global__ __launch_bounds__(NT) void k_fin_fcg_alpha(const double* partials, unsigned nb, double* scal) {
+    __shared__ double red[NT / 64];
+    double v[2];
+    reduce_partials_n<NT, 2>(partials, 2, nb, red, v);
+    if (threadIdx.x == 0) {
+        const bool ok = v[0] > 0.0 && v[1] > 0.0;  // also false for NaN
+        scal[0] = v[0];
+        scal[1] = v[1];
+        scal[2] = ok ? v[1] / v[0] : 0.0;
+        scal[3] = ok ? 0.0 : 1.0;
+    }
+}
+// x += alpha p ; r = free ? r - alpha Ap : 0 with alpha = scal[2]; r'r partials
+__global__ __launch_bounds__(NT) void k_fcg_step2(long n, const double* __restrict__ scal, double* __restrict__ x,
+                                                  const double* __restrict__ p, double* __restrict__ r,
+                                                  const double* __restrict__ Ap, const double* __restrict__ minv,
+                                                  double* __restrict__ partials) {
+    __shared__ double red[NT / 64];
+    const double alpha = scal[2];
+    double rr = 0.0;
     for (long i = gtid(); i < n; i += gstride()) {
         x[i] += alpha * p[i];
-        r[i] = minv[i] != 0.0 ? r[i] - alpha * Ap[i] : 0.0;
+        const double ri = minv[i] != 0.0 ? r[i] - alpha * Ap[i] : 0.0;
+        r[i] = ri;
+        rr += ri * ri;
     }
+    const double v = block_sum<NT>(rr, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = v;
+}
+__global__ __launch_bounds__(NT) void k_fin_fcg_rr(const double* partials, unsigned nb, double* scal) {
+    __shared__ double red[NT / 64];
+    const double t = reduce_partials<NT>(partials, 1, 0, nb, red);
+    if (threadIdx.x == 0) scal[4] = t;
 }
 
 // e = a - b (masked by minv != 0)
@@ -1451,13 +1533,23 @@ void launch_simp_modulus(long ne, const double* xphys, double penal, double E0, 
                          hipStream_t s) {
     hipLaunchKernelGGL(k_simp_modulus, dim3(kVecBlocks), dim3(NT), 0, s, ne, xphys, penal, E0, Emin, E);
 }
-void launch_fcg_direction(long n, double a, double b, const double* W, const double* AW, const FcgCoef& cf,
-                          const double* P, const double* AP, double* p, double* Ap, hipStream_t s) {
-    hipLaunchKernelGGL(k_fcg_direction, dim3(kVecBlocks), dim3(NT), 0, s, n, a, b, W, AW, cf, P, AP, p, Ap);
+void launch_fcg_dots(long n, const double* W, const double* AW, const double* r, const FcgCoef& cf, const double* P,
+                     double* partials, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_fcg_dots, dim3(kVecBlocks), dim3(NT), 0, s, n, W, AW, r, cf, P, partials);
+    hipLaunchKernelGGL(k_fin_fcg_dots, dim3(5 + 2 * cf.m), dim3(NT), 0, s, partials, (unsigned)kVecBlocks,
+                       5 + 2 * cf.m, out);
 }
-void launch_fcg_step(long n, double alpha, double* x, const double* p, double* r, const double* Ap, const double* minv,
-                     hipStream_t s) {
-    hipLaunchKernelGGL(k_fcg_step, dim3(kVecBlocks), dim3(NT), 0, s, n, alpha, x, p, r, Ap, minv);
+void launch_fcg_direction2(long n, double a, double b, const double* W, const double* AW, const FcgCoef& cf,
+                           const double* P, const double* AP, const double* r, double* p, double* Ap, double* partials,
+                           double* scal, hipStream_t s) {
+    hipLaunchKernelGGL(k_fcg_direction2, dim3(kVecBlocks), dim3(NT), 0, s, n, a, b, W, AW, cf, P, AP, r, p, Ap,
+                       partials);
+    hipLaunchKernelGGL(k_fin_fcg_alpha, dim3(1), dim3(NT), 0, s, partials, (unsigned)kVecBlocks, scal);
+}
+void launch_fcg_step2(long n, const double* scal_alpha, double* x, const double* p, double* r, const double* Ap,
+                      const double* minv, double* partials, double* scal, hipStream_t s) {
+    hipLaunchKernelGGL(k_fcg_step2, dim3(kVecBlocks), dim3(NT), 0, s, n, scal_alpha, x, p, r, Ap, minv, partials);
+    hipLaunchKernelGGL(k_fin_fcg_rr, dim3(1), dim3(NT), 0, s, partials, (unsigned)kVecBlocks, scal);
 }
 void launch_mul(long n, const double* a, const double* b, double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_mul, dim3(kVecBlocks), dim3(NT), 0, s, n, a, b, out);

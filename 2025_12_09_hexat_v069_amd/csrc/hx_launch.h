@@ -225,13 +225,20 @@ struct FcgCoef {
     int slot[kFcgKeep];
     double c[kFcgKeep];
 };
+// one pass for the iteration's dot products (u'Au, u'Ay, y'Ay, u'r, y'r, then
+// Au'P_j, Ay'P_j for the cf.m stored directions) into out[0 .. 5 + 2 cf.m)
+constexpr int kFcgDots = 5 + 2 * (kFcgKeep - 1);
+void launch_fcg_dots(long n, const double* W, const double* AW, const double* r, const FcgCoef& cf, const double* P,
+                     double* partials, double* out, hipStream_t s);
 // W = [u | y], AW = [Au | Ay] (two columns of n): p = a u + b y - sum c_j P_j,
-// Ap = a Au + b Ay - sum c_j AP_j
-void launch_fcg_direction(long n, double a, double b, const double* W, const double* AW, const FcgCoef& cf,
-                          const double* P, const double* AP, double* p, double* Ap, hipStream_t s);
-// x += alpha p ; r = free ? r - alpha Ap : 0
-void launch_fcg_step(long n, double alpha, double* x, const double* p, double* r, const double* Ap, const double* minv,
-                     hipStream_t s);
+// Ap = a Au + b Ay - sum c_j AP_j; leaves scal[0] = p'Ap, scal[1] = p'r,
+// scal[2] = alpha (0 and scal[3] = 1 on breakdown) on the device
+void launch_fcg_direction2(long n, double a, double b, const double* W, const double* AW, const FcgCoef& cf,
+                           const double* P, const double* AP, const double* r, double* p, double* Ap, double* partials,
+                           double* scal, hipStream_t s);
+// x += alpha p ; r = free ? r - alpha Ap : 0 with alpha = scal_alpha[2] read on the device; scal[4] = r'r after it
+void launch_fcg_step2(long n, const double* scal_alpha, double* x, const double* p, double* r, const double* Ap,
+                      const double* minv, double* partials, double* scal, hipStream_t s);
 void launch_masked_diff(long n, const double* a, const double* b, const double* minv, double* e, hipStream_t s);
 // out = a * b (elementwise)
 void launch_mul(long n, const double* a, const double* b, double* out, hipStream_t s);

@@ -61,8 +61,9 @@ void launch_weight_transpose(const uint16_t* w, int cout, int cin, uint16_t* wt,
 void launch_adam(long n, float* w, float* g, float* m, float* v, uint16_t* wbf, float lr, float b1, float b2,
                  float eps, float c1, float c2, float clip, hipStream_t s);
 void launch_fillf(long n, float v, float* a, hipStream_t s);
+// rr_dev non-null: the scale is 1 / rms(r) from r'r held on the device there (inv_scale unused)
 void launch_pack_input(const GridArgs& g, const double* r, const double* E, double inv_scale, uint16_t* out,
-                       hipStream_t s);
+                       hipStream_t s, const double* rr_dev = nullptr);
 // e = sum_{l=1..levels} 2^(l-levels) * trilinear interpolation of hashed standard
 // normal values on the lattice of spacing 2^l; 0 where minv = 0
 void launch_smooth_field(const GridArgs& g, unsigned long long seed, int levels, const double* minv, double* e,

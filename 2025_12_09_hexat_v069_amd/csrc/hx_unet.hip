@@ -389,8 +389,9 @@ bool unet_bind(UNet* u, const GridArgs& g, unsigned seed, hipStream_t s) {
 // Training pieces for the subspace objective: the caller forms the network's
 // direction y, applies the operator and solves the 2x2 energy problem over
 // span{D^-1 r, y}, then hands the output gradient back.
-void unet_forward_sample(UNet* u, const double* r, const double* E, double inv_scale, hipStream_t s) {
-    launch_pack_input(u->g, r, E, inv_scale, u->x0.p, s);
+void unet_forward_sample(UNet* u, const double* r, const double* E, double inv_scale, hipStream_t s,
+                         const double* rr_dev) {
+    launch_pack_input(u->g, r, E, inv_scale, u->x0.p, s, rr_dev);
     u->forward(s);
 }
 // y = scale * net output on free DOFs

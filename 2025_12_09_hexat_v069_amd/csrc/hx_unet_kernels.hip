@@ -331,9 +331,11 @@ __global__ __launch_bounds__(256) void k_smooth_field(GridArgs g, unsigned long 
 // Network input from a residual: ch 0..2 = r/scale at the node, ch 3 = the
 // node's mean element modulus on a log scale, ch 4..7 = 0.
 __global__ __launch_bounds__(256) void k_pack_input(GridArgs g, const double* __restrict__ r, const double* __restrict__ E,
-                                                    double inv_scale, uint16_t* __restrict__ out) {
+                                                    double inv_scale, const double* __restrict__ rr_dev,
+                                                    uint16_t* __restrict__ out) {
     const long n = (long)blockIdx.x * 256 + threadIdx.x;
     if (n >= g.nn) return;
+    if (rr_dev) inv_scale = 1.0 / sqrt(*rr_dev / (3.0 * (double)g.nn));  // 1 / rms(r) from r'r on the device
     const int i = (int)(n % g.NX), j = (int)((n / g.NX) % g.NY), k = (int)(n / ((long)g.NX * g.NY));
     double s = 0.0;
     int cnt = 0;
@@ -425,8 +427,9 @@ void launch_smooth_field(const GridArgs& g, unsigned long long seed, int levels,
                        e);
 }
 void launch_pack_input(const GridArgs& g, const double* r, const double* E, double inv_scale, uint16_t* out,
-                       hipStream_t s) {
-    hipLaunchKernelGGL(k_pack_input, dim3((unsigned)((g.nn + 255) / 256)), dim3(256), 0, s, g, r, E, inv_scale, out);
+                       hipStream_t s, const double* rr_dev) {
+    hipLaunchKernelGGL(k_pack_input, dim3((unsigned)((g.nn + 255) / 256)), dim3(256), 0, s, g, r, E, inv_scale,
+                       rr_dev, out);
 }
 void launch_unpack_output(long nn, const float* y, int cy, const double* r, const double* minv, double scale,
                           double* z, hipStream_t s) {
