@@ -52,7 +52,7 @@ EXPORTS = [
 #: test hooks declared in include/hexat_testing.h
 TEST_EXPORTS = ["hx_test_unet_nparam", "hx_test_unet_init", "hx_test_unet_forward", "hx_test_unet_backward",
                 "hx_test_copy", "hx_test_ml_apply", "hx_test_vc_apply", "hx_test_device_alloc",
-                "hx_test_device_free", "hx_test_memcpy", "hx_test_oc_search"]
+                "hx_test_device_free", "hx_test_memcpy", "hx_test_oc_search", "hx_test_smooth_field"]
 
 
 class HexatError(RuntimeError):
@@ -194,6 +194,7 @@ def load():
         L.hx_test_memcpy.argtypes = [vp, vp, vp, C.c_long, i]
         L.hx_test_ml_apply.argtypes = [vp, pd, pd, pd, i, P(C.c_int)]
         L.hx_test_oc_search.argtypes = [vp, pd, pd, pd, d, d, d, d, d, pd, pd, P(C.c_int)]
+        L.hx_test_smooth_field.argtypes = [vp, C.c_ulonglong, i, pd]
         L.hx_test_vc_apply.argtypes = [vp, pd, pd, pd, i, P(C.c_int), P(C.c_int)]
         pf = P(C.c_float)
         L.hx_test_unet_nparam.argtypes = [vp, P(C.c_long)]
@@ -506,6 +507,13 @@ class Context:
                                             float(xmin), float(rel_tol), float(lguess), dptr(xn), C.byref(lm),
                                             C.byref(st)), "hx_test_oc_search")
         return xn, lm.value, st.value
+
+    def test_smooth_field(self, seed, levels):
+        """hx_test_smooth_field: the training's manufactured displacement (ndof values)"""
+        e = np.empty(self.ndof)
+        self.check(self.L.hx_test_smooth_field(self.h, C.c_ulonglong(int(seed)), int(levels), dptr(e)),
+                   "hx_test_smooth_field")
+        return e
 
     # ---- device-resident SIMP loop ------------------------------------------------
     def to_init(self, params: TOParams, f, x0):

@@ -1973,6 +1973,19 @@ int hx_test_oc_search(hx_ctx* c, const double* x, const double* dc, const double
     return download(c, xnew, c->e_vm.p, ne * sizeof(double));
 }
 
+// the manufactured displacement of the learned preconditioner's training on
+// the current grid, every DOF solved (no mask)
+int hx_test_smooth_field(hx_ctx* c, unsigned long long seed, int levels, double* e) {
+    if (!c || !e || levels < 1 || levels > 20) return HX_ERR_BAD_ARG;
+    if (int rc = ensure_grid(c)) return rc;
+    const long n = ndof(c);
+    HXA(c->l_z, 2 * n);
+    launch_fill(n, 1.0, c->l_z.p + n, c->stream);
+    launch_smooth_field(c->g, seed, levels, c->l_z.p + n, c->l_z.p, c->stream);
+    HXK();
+    return download(c, e, c->l_z.p, n * sizeof(double));
+}
+
 // ------------------------------------------------------- SIMP/OC session
 int hx_to_init(hx_ctx* c, const hx_to_params* p, const double* f, const double* x0) {
     if (!c || !p || !f || !x0) return HX_ERR_BAD_ARG;

@@ -46,6 +46,14 @@ int hx_test_vc_apply(hx_ctx* ctx, const double* Ee, const double* r, double* z, 
 int hx_test_oc_search(hx_ctx* ctx, const double* x, const double* dc, const double* dv, double volfrac, double move,
                       double xmin, double rel_tol, double lguess, double* xnew, double* lmid, int* passes);
 
+/* The manufactured displacement the learned preconditioner's training uses
+ * (hx_learn_stats::manufactured) on the current grid, every DOF solved: on
+ * levels l = 1..levels the standard normal values of the lattice of spacing
+ * 2^l (splitmix64 of the seed and the lattice point, Box-Muller),
+ * interpolated trilinearly and weighted 2^(l - levels), summed; ndof values.
+ * No reference counterpart. */
+int hx_test_smooth_field(hx_ctx* ctx, unsigned long long seed, int levels, double* e);
+
 /* number of network parameters (conv weights then bias, layer by layer) */
 int hx_test_unet_nparam(hx_ctx* ctx, long* nparam);
 /* (re)initialise the network weights with a seed; copy the FP32 master

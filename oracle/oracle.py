@@ -641,6 +641,52 @@ def ml_pcg(g: Grid, Ke, Ee, fixed, b, tol=1e-8, max_iter=100000, max_levels=12, 
 
 
 # ---------------------------------------------------------------------------------------------
+# manufactured displacement of the learned preconditioner's training (no reference counterpart)
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+
+
+def mix64(z):
+    """splitmix64's output function of state z + golden gamma (Steele, Lea & Flood 2014), on uint64 arrays"""
+    z = np.asarray(z, dtype=np.uint64) + _GOLDEN
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def lattice_normal(seed, q):
+    """standard normal value of lattice key q: Box-Muller on two hashed uniforms"""
+    h1 = mix64(np.uint64(seed) ^ mix64(q))
+    h2 = mix64(h1 ^ np.uint64(0xD1B54A32D192ED03))
+    u1 = ((h1 >> np.uint64(11)).astype(np.float64) + 0.5) * 2.0 ** -53
+    u2 = (h2 >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def smooth_field(g: Grid, seed, levels):
+    """sum over l = 1..levels of 2^(l-levels) times the trilinear interpolation of lattice_normal on
+    the lattice of spacing 2^l (key 3*((l << 56) ^ lattice index) + component), node-major (ndof,)"""
+    NX, NY, NZ = g.nx + 1, g.ny + 1, g.nz + 1
+    k, j, i = np.meshgrid(np.arange(NZ), np.arange(NY), np.arange(NX), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    out = np.zeros((i.size, 3))
+    with np.errstate(over="ignore"):
+        for l in range(1, levels + 1):
+            h = 1 << l
+            LX, LY = (NX + h - 1) // h + 1, (NY + h - 1) // h + 1
+            I, J, K = i >> l, j >> l, k >> l
+            fx, fy, fz = (i & (h - 1)) / h, (j & (h - 1)) / h, (k & (h - 1)) / h
+            w = 2.0 ** (l - levels)
+            for c8 in range(8):
+                ox, oy, oz = c8 & 1, (c8 >> 1) & 1, c8 >> 2
+                wt = w * (fx if ox else 1 - fx) * (fy if oy else 1 - fy) * (fz if oz else 1 - fz)
+                pt = (np.uint64(l) << np.uint64(56)) ^ (((K + oz) * LY + (J + oy)) * LX + (I + ox)).astype(np.uint64)
+                for c in range(3):
+                    out[:, c] += wt * lattice_normal(seed, np.uint64(3) * pt + np.uint64(c))
+    return out.ravel()
+
+
+# ---------------------------------------------------------------------------------------------
 # V-cycle form (the library's HX_PC_VCYCLE): V-cycle on levels 0..L-1, the coarsest level L exactly
 # (or by its diagonal when the hierarchy has no exact level)
 

@@ -349,3 +349,27 @@ def test_regrid_map_closed_forms(ox):
     # new centroid (2i + 1) * 1.0 in x lies in old element 2i + 1 (0-based), likewise y, z: (2i+1, 2j+1, 1)
     want = [(2 * i + 1) + 6 * ((2 * j + 1) + 4 * 1) for j in range(2) for i in range(3)]
     assert list(h) == want
+
+
+def test_smooth_field_generator(ox):
+    """The manufactured displacement of the learned preconditioner's training:
+    splitmix64 against its published first outputs from seed 0, the
+    Box-Muller values' moments over many lattice keys, and the structure of
+    the sum: with one level the field equals the lattice values at even nodes
+    and their mean at odd ones (linear interpolation along each axis)."""
+    z = np.array([0, 0x9E3779B97F4A7C15, (2 * 0x9E3779B97F4A7C15) % 2 ** 64], dtype=np.uint64)
+    assert [int(v) for v in ox.mix64(z)] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    v = ox.lattice_normal(7, np.arange(200000, dtype=np.uint64))
+    assert abs(v.mean()) < 0.01 and abs(v.std() - 1.0) < 0.01
+    assert abs((v ** 3).mean()) < 0.03 and abs((v ** 4).mean() - 3.0) < 0.06
+    g = ox.Grid(8, 6, 4)
+    e = ox.smooth_field(g, 11, 1).reshape(5, 7, 9, 3)  # (k, j, i, c)
+    even = e[::2, ::2, ::2]
+    LX, LY = (9 + 1) // 2 + 1, (7 + 1) // 2 + 1
+    K, J, I = np.meshgrid(np.arange(3), np.arange(4), np.arange(5), indexing="ij")
+    pt = (np.uint64(1) << np.uint64(56)) ^ ((K * LY + J) * LX + I).astype(np.uint64)
+    for c in range(3):
+        assert np.allclose(even[..., c], ox.lattice_normal(11, np.uint64(3) * pt + np.uint64(c)), rtol=0, atol=1e-15)
+    assert np.allclose(e[:, :, 1::2], 0.5 * (e[:, :, 0:-1:2] + e[:, :, 2::2]), atol=1e-14)
+    assert np.allclose(e[:, 1::2], 0.5 * (e[:, 0:-1:2] + e[:, 2::2]), atol=1e-14)
+    assert np.allclose(e[1::2], 0.5 * (e[0:-1:2] + e[2::2]), atol=1e-14)
