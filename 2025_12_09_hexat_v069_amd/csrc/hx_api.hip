@@ -848,8 +848,14 @@ int pcg_fp32ir(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s, const SolveVecs
 // the current iterate when the flexible iteration stalls or breaks down, and
 // the true residual judges convergence.
 // stall: no 1% drop of the best residual for max(kLearnedStall, it/2)
-// iterations (CG residuals plateau for ~100 iterations on healthy runs)
+// iterations (CG residuals plateau for ~100 iterations on healthy runs).  The
+// first kLearnedRestarts stalls restart the flexible iteration from the true
+// residual with an empty direction ring (truncated flexible CG loses the
+// A-orthogonality it relies on over thousands of iterations); the next one
+// hands over to Jacobi-PCG.  HEXAT_LEARNED_STALL, read at each solve, fixes the
+// window instead (iterations, >= 1), so a short window forces the restarts.
 constexpr int kLearnedStall = 200;
+constexpr int kLearnedRestarts = 2;
 
 extern "C++" {
 // dot products of n-vectors, one sync for the lot
@@ -884,7 +890,7 @@ int pcg_learned(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
     if (rc) return rc;
     const double normb = c->st_host->normb;
     double rr = c->st_host->rr, rel = c->st_host->rel, best = rel;
-    int it = 0, best_it = 0, status = c->st_host->status == 10 ? 10 : -1;
+    int it = 0, best_it = 0, status = c->st_host->status == 10 ? 10 : -1, restarts = 0;
     bool stalled = false;
     double* W = c->l_z.p;
     double* AW = c->l_zold.p;
@@ -896,7 +902,8 @@ int pcg_learned(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
     double pAp_j[kFcgKeep];
     // ring of `keep` slots (<= kFcgKeep); the newest keep-1 directions orthogonalise
     const int keep = c->fcg_keep;
-    int nstored = 0, next = 0, last = -1;
+    int nstored = 0, next = 0, last = -1, restart_it = 0, window = 0;
+    if (const char* ev = std::getenv("HEXAT_LEARNED_STALL")) window = std::max(0, std::atoi(ev));
     // Per iteration one host synchronisation: the dot products of this
     // iteration arrive with the last step's p'Ap, breakdown flag and r'r, so
     // its convergence is judged one iteration late (the last network
@@ -939,9 +946,27 @@ int pcg_learned(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
             status = 3;
             break;
         }
-        if (it - best_it >= std::max(kLearnedStall, it / 2)) {
-            stalled = true;
-            break;
+        if (it - best_it >= (window > 0 ? window : std::max(kLearnedStall, (it - restart_it) / 2))) {
+            if (restarts == kLearnedRestarts) {
+                stalled = true;
+                break;
+            }
+            // restart: r = b - A x in FP64, an empty ring, the stall window and best from here
+            ++restarts;
+            mv64(c, c->x.p, c->q.p, c->E.p, 0.0, MV_PLAIN, HX_MV_AUTO, 1);
+            launch_pcg_init<double>(n, c->f.p, c->x.p, c->q.p, c->minv.p, c->r.p, c->p.p, o.shift, c->st,
+                                    c->partials.p, c->stream, 1);
+            HXK();
+            if ((rc = sync_state(c, 1))) return rc;
+            rr = c->st_host->rr;
+            rel = std::sqrt(rr) / normb;
+            best = rel;
+            best_it = restart_it = it;
+            nstored = 0;
+            next = 0;
+            last = -1;
+            HXC(hipMemcpyAsync(scal + 4, &rr, sizeof(double), hipMemcpyHostToDevice, c->stream));
+            continue;
         }
         const double uAu = h[0], uAy = h[1], yAy = h[2], ur = h[3], yr = h[4];
         const double det = uAu * yAy - uAy * uAy;
@@ -982,7 +1007,7 @@ int pcg_learned(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
                 s->status = HX_OK;
                 s->rel_res = c->st_host->rel;
                 s->seconds = ms * 1e-3;
-                s->refinements = 0;
+                s->refinements = restarts;
                 s->fallbacks = 0;
             }
             return HX_OK;
@@ -1000,6 +1025,7 @@ int pcg_learned(hx_ctx* c, const hx_pcg_opts& o, hx_pcg_stats* s) {
     if (s) {
         s->iterations += it;
         s->seconds += ms * 1e-3;
+        s->refinements = restarts;
         s->fallbacks = 1;
     }
     return rc;

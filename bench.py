@@ -148,7 +148,7 @@ def other_workload(args, hx, L, ctx):
         if learned is not None:
             tl, sl, ul = res["learned"]
             learned.update({"s": round(tl, 5), "iterations": sl.iterations, "rel_res": sl.rel_res,
-                            "fallbacks": sl.fallbacks, "status": sl.status,
+                            "fallbacks": sl.fallbacks, "restarts": sl.refinements, "status": sl.status,
                             "iterations_vs_plain_jacobi": round(sl.iterations / sp.iterations, 3),
                             "time_vs_plain_jacobi": round(tl / tp, 3),
                             "compliance_rel_diff_vs_plain": abs(float(f @ ul) - cp) / cp,

@@ -95,7 +95,7 @@ typedef struct {
     int status;       /* hx_status of the solve */
     double rel_res;   /* final ||b - K u||/||b||, recomputed in FP64 */
     double seconds;   /* device time of the solve */
-    int refinements;  /* FP64 residual replacements (HX_FP32_IR) */
+    int refinements;  /* FP64 residual replacements (HX_FP32_IR; HX_PC_LEARNED: its restarts) */
     int fallbacks;    /* learned -> Jacobi fallbacks */
 } hx_pcg_stats;
 

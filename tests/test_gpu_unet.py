@@ -6,6 +6,8 @@ stored activations), differentiated by torch autograd.  The library runs
 BF16 MFMA convolutions with FP32 accumulation and BF16 activations, so the
 comparison is norm-wise at 3e-2.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -214,3 +216,17 @@ def test_learned_preconditioner_ablation(ox, hx):
         assert abs(f @ ul - f @ uj) < 1e-6 * abs(f @ uj)
         assert sl.fallbacks == 0
         assert sl.iterations < 0.75 * sj.iterations
+        # a one-iteration stall window forces the restarts from the true residual
+        # (at most two) and then the Jacobi fallback; the solve still converges
+        os.environ["HEXAT_LEARNED_STALL"] = "1"
+        try:
+            ur, sr = c.pcg(Ee, f, tol=1e-8, precond=hx._lib.HX_PC_LEARNED)
+        finally:
+            del os.environ["HEXAT_LEARNED_STALL"]
+        print("forced stalls: its", sr.iterations, "restarts", sr.refinements, "fallbacks", sr.fallbacks, flush=True)
+        r = f - ox.matvec(g, Ke, Ee, ur)
+        r[fixed.astype(bool)] = 0
+        assert sr.status == 0 and np.linalg.norm(r) / np.linalg.norm(b) < 1.5e-8
+        assert abs(f @ ur - f @ uj) < 1e-6 * abs(f @ uj)
+        assert 0 <= sr.refinements <= 2 and sr.fallbacks in (0, 1)
+        assert sr.fallbacks == 0 or sr.refinements == 2
