@@ -8,7 +8,7 @@ reference's Julia entry points on top of it:
 * ``gpu_helmholtz.apply_gpu_filter``                       -- GPUHelmholtz.jl:344
 * ``stress.compute_stress_field``                          -- Stress.jl:57
 * ``topopt.update_density`` (stress-ratio, parity mode)    -- TopOpt.jl:139
-* ``topopt.SimpOptimizer`` (SIMP/OC, north-star mode)
+* ``topopt.SimpOptimizer`` (SIMP/OC, device-resident loop)
 * ``driver.run_main`` (YAML-driven TO loop)               -- Main.jl:175-476
 """
 from . import _lib

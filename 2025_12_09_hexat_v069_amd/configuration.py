@@ -1,7 +1,7 @@
 """YAML configuration in the reference grammar (mirror of Configuration.jl:17-173,
 Helpers.calculate_element_distribution and Main.apply_hardware_profile!).
 
-Keys added by this build (all optional, old configs keep their meaning):
+Keys added by this package (all optional, old configs keep their meaning):
 
     optimization_parameters:
       method: stress_ratio | simp_oc      # default stress_ratio (reference)
@@ -79,7 +79,7 @@ def initialize_density_field(nodes, elements, shapes, config):
 
 def apply_hardware_profile(config):
     """Main.jl:118-155: the profile overrides tolerance/shift/solver type.
-    "MI355X" is this build's profile: FP64 tolerance 1e-8, no shift cap."""
+    "MI355X" is the MI355X profile: FP64 tolerance 1e-8, no shift cap."""
     gpu = str(config.get("gpu_profile", "RTX")).upper()
     growth = config.setdefault("growth_settings", {})
     solver = config.setdefault("solver_parameters", {})

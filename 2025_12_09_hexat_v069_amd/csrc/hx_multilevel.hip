@@ -28,7 +28,7 @@
 // V-cycle symmetric positive definite whatever the moduli contrast.
 //
 // The reference has no counterpart; its strong-preconditioner slot is the
-// learned one (north-star path) and its CPU AMG branch (GPUSolver.jl:168-268).
+// learned U-Net preconditioner and its CPU AMG branch (GPUSolver.jl:168-268).
 #include "hx_launch.h"
 
 #include <algorithm>

@@ -8,7 +8,7 @@ reference's loop: solve -> compliance f'u -> stress field -> stress-ratio
 update, with the viewer/checkpoint file, the VTK solution and the isosurface
 STL at output_settings' frequencies and restarts from a checkpoint.
 ``simp_oc`` runs the device-resident SIMP/OC loop.  Mesh refinement and
-gravity/thermal loads are outside this build's hot path (a config that asks
+gravity/thermal loads are outside this package's hot path (a config that asks
 for gravity or thermal load is rejected rather than silently ignored).
 """
 from __future__ import annotations
@@ -79,9 +79,9 @@ def build_problem(config):
                                               list(config["external_forces"]), nodes)
     mat = config["material"]
     if float(mat.get("gravity_acceleration", 0.0)) != 0.0 and float(mat.get("material_density", 0.0)) > 1e-9:
-        raise NotImplementedError("self-weight loads are outside this build")
+        raise NotImplementedError("self-weight loads are not supported by libhexat")
     if abs(float(mat.get("delta_temperature", 0.0))) > 1e-6:
-        raise NotImplementedError("thermal loads are outside this build")
+        raise NotImplementedError("thermal loads are not supported by libhexat")
     return geo, nodes, elements, bc, F
 
 

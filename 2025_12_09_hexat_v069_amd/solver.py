@@ -122,7 +122,7 @@ def adaptive_tolerance(config, solver_type="gpu"):
 
 
 def choose_solver(nNodes, nElem, config):
-    """Solver.jl:13-36 (only the GPU path exists in this build)."""
+    """Solver.jl:13-36 (only the GPU path exists in this package)."""
     t = str(config["solver_parameters"].get("solver_type", "direct")).lower()
     if t == "gpu":
         if _lib.device_count() > 0:

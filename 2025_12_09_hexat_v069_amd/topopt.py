@@ -11,7 +11,7 @@ restore (:398-402).  The two filter solves run on the GPU; the element-wise
 steps are cheap host loops in the reference too and stay vectorised numpy on
 the caller's arrays, which the function mutates in place like the original.
 
-``SimpOptimizer`` is the north-star path: SIMP E(rho) = Emin + rho^p (E0-Emin),
+``SimpOptimizer`` is the SIMP/OC path: SIMP E(rho) = Emin + rho^p (E0-Emin),
 compliance sensitivities, PDE sensitivity filter, OC update and PDE density
 filter, all device-resident in libhexat (one ``hx_to_step`` per iteration).
 """

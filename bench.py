@@ -574,8 +574,8 @@ def main():
         kernel_sample(args)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus N>1 is not supported (the path is designed "
-                                   "to shard by x-slabs with halo exchange, see DESIGN.md)"}))
+        print(json.dumps({"error": "one GPU only: --gpus N>1 is not supported (the path is designed to shard by "
+                                   "x-slabs with a halo exchange of one node plane per neighbour)"}))
         sys.exit(2)
     if args.workload != "cantilever":
         hx = importlib.import_module("2025_12_09_hexat_v069_amd")
