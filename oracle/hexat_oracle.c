@@ -1,7 +1,7 @@
 /*
  * hexat_oracle.c -- CPU FP64 restatement of the HEXAT hot path.
  *
- * TEST INFRASTRUCTURE: this is the checker for the HIP kernels (used only by
+ * TEST INFRASTRUCTURE: the HIP kernels are compared against it (used only by
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg).  It
  * follows the reference Julia files cited per function, in FP64 (the
  * reference computes in FP32).  OpenMP parallelism uses static partitioning

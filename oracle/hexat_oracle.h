@@ -1,7 +1,7 @@
 /*
  * hexat_oracle.h -- CPU FP64 restatement of the HEXAT topology-optimisation
- * hot path.  TEST INFRASTRUCTURE ONLY: this library is the checker that the
- * HIP kernels are compared against (tests/, __graft_entry__.smoke() and the
+ * hot path.  TEST INFRASTRUCTURE ONLY: this library is what the HIP kernels
+ * are compared against (tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py).  The product never links or calls it.
  *
  * Conventions (0-based; the reference is 1-based Julia):

@@ -1,6 +1,6 @@
 """CPU oracle for the HEXAT topology-optimisation hot path.
 
-TEST INFRASTRUCTURE ONLY.  This module is the checker that the HIP path is
+TEST INFRASTRUCTURE ONLY.  This module is the reference that the HIP path is
 compared against; it may be imported only by ``tests/``, by
 ``__graft_entry__.smoke()`` and by the ``cpu_baseline`` leg of ``bench.py``.
 The product package never imports it.
