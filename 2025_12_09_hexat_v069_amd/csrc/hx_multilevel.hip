@@ -643,48 +643,76 @@ __global__ __launch_bounds__(kCoarseThreads) void k_vc_coarse(MlHierarchy<T> h, 
             }
         }
     };
-    for (int l = s0; l < L; ++l) {  // down: pre-smoothing, residual, restriction
-        const long nl = 3 * h.g[l].nn;
-        for (long d = t; d < nl; d += kCoarseThreads) Xv(l)[d] = Sv(l)[d] * Rv(l)[d];
-        __syncthreads();
-        apply(l, Xv(l), Tv(l), true);
-        __syncthreads();
-        const GridArgs gf = h.g[l], gc = h.g[l + 1];
-        for (int u = t; u < 3 * (int)gc.nn; u += kCoarseThreads)
-            Rv(l + 1)[u] = restrict_comp(gf, (const T*)Tv(l), gc, (unsigned)u / 3u, (int)((unsigned)u % 3u));
-        __syncthreads();
-    }
-    {  // the coarsest level
-        const long nL = 3 * h.g[L].nn;
-        const T* rl = Rv(L);
-        if (h.ainv != nullptr) {  // 8 lanes per row (columns q, q + 8, ...), then an 8-lane sum
-            const int q = t & 7;
-            const T* A = stage_ainv ? Av : h.ainv;
-            for (int d0 = 0; d0 < N; d0 += kCoarseThreads / 8) {  // uniform trip count: every lane shuffles
-                const int d = d0 + (t >> 3);
-                T acc = T(0);
-                if (d < N)
-                    for (int j = q; j < N; j += 8) acc += A[(long)d * N + j] * rl[j];
-#pragma unroll
-                for (int off = 4; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-                if (q == 0 && d < N) Xv(L)[d] = acc;
+    // down: pre-smoothing, residual, restriction; the coarsest level; up: correction,
+    // post-smoothing.  The level above an exact coarsest level visits it twice
+    // (smoothing, correction, smoothing, correction, smoothing: symmetric): after its
+    // first post-smoothing it takes the down step again from its current iterate.
+    // One loop with each operation at one place (a second copy of the level
+    // operator's code spills the per-level arrays of h to scratch).
+    const int wl = h.ainv != nullptr ? L - 1 : -1;
+    int phase = s0 < L ? 0 : 1;  // 0: down step at level l, 1: the coarsest level, 2: up step at level l
+    bool again = false;
+    for (int l = s0;;) {
+        if (phase == 0) {
+            const long nl = 3 * h.g[l].nn;
+            if (!again)
+                for (long d = t; d < nl; d += kCoarseThreads) Xv(l)[d] = Sv(l)[d] * Rv(l)[d];
+            __syncthreads();
+            apply(l, Xv(l), Tv(l), true);
+            __syncthreads();
+            const GridArgs gf = h.g[l], gc = h.g[l + 1];
+            for (int u = t; u < 3 * (int)gc.nn; u += kCoarseThreads)
+                Rv(l + 1)[u] = restrict_comp(gf, (const T*)Tv(l), gc, (unsigned)u / 3u, (int)((unsigned)u % 3u));
+            __syncthreads();
+            if (l + 1 < L) {
+                ++l;
+                continue;
             }
-        } else {
-            for (long d = t; d < nL; d += kCoarseThreads) Xv(L)[d] = h.minv[L][d] * rl[d];
+            phase = 1;
         }
-        __syncthreads();
-    }
-    for (int l = L - 1; l >= s0; --l) {  // up: correction, post-smoothing
-        const GridArgs gf = h.g[l], gc = h.g[l + 1];
-        for (int u = t; u < 3 * (int)gf.nn; u += kCoarseThreads)
-            if (Sv(l)[u] != T(0))
-                Xv(l)[u] += prolong_comp(gc, (const T*)Xv(l + 1), gf, (unsigned)u / 3u, (int)((unsigned)u % 3u));
-        __syncthreads();
-        apply(l, Xv(l), Tv(l), false);
-        __syncthreads();
-        const long nl = 3 * h.g[l].nn;
-        for (long d = t; d < nl; d += kCoarseThreads) Xv(l)[d] += Sv(l)[d] * (Rv(l)[d] - Tv(l)[d]);
-        __syncthreads();
+        if (phase == 1) {
+            {  // the coarsest level
+                const long nL = 3 * h.g[L].nn;
+                const T* rl = Rv(L);
+                if (h.ainv != nullptr) {  // 8 lanes per row (columns q, q + 8, ...), then an 8-lane sum
+                    const int q = t & 7;
+                    const T* A = stage_ainv ? Av : h.ainv;
+                    for (int d0 = 0; d0 < N; d0 += kCoarseThreads / 8) {  // uniform trip count: every lane shuffles
+                        const int d = d0 + (t >> 3);
+                        T acc = T(0);
+                        if (d < N)
+                            for (int j = q; j < N; j += 8) acc += A[(long)d * N + j] * rl[j];
+#pragma unroll
+                        for (int off = 4; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+                        if (q == 0 && d < N) Xv(L)[d] = acc;
+                    }
+                } else {
+                    for (long d = t; d < nL; d += kCoarseThreads) Xv(L)[d] = h.minv[L][d] * rl[d];
+                }
+                __syncthreads();
+            }
+            if (L - 1 < s0) break;
+            l = L - 1;
+            phase = 2;
+        }
+        {  // up step at level l: correction, post-smoothing
+            const GridArgs gf = h.g[l], gc = h.g[l + 1];
+            for (int u = t; u < 3 * (int)gf.nn; u += kCoarseThreads)
+                if (Sv(l)[u] != T(0))
+                    Xv(l)[u] += prolong_comp(gc, (const T*)Xv(l + 1), gf, (unsigned)u / 3u, (int)((unsigned)u % 3u));
+            __syncthreads();
+            apply(l, Xv(l), Tv(l), false);
+            __syncthreads();
+            const long nl = 3 * h.g[l].nn;
+            for (long d = t; d < nl; d += kCoarseThreads) Xv(l)[d] += Sv(l)[d] * (Rv(l)[d] - Tv(l)[d]);
+            __syncthreads();
+        }
+        if (l == wl && !again) {  // the second visit: down from the current iterate
+            again = true;
+            phase = 0;
+            continue;
+        }
+        if (--l < s0) break;
     }
     T* const yout = h.y[s0] + (long)col * 3 * h.g[s0].nn;
     for (long i = t; i < 3 * h.g[s0].nn; i += kCoarseThreads) yout[i] = Xv(s0)[i];

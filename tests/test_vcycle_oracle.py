@@ -58,16 +58,22 @@ def _vcycle_sparse(lv, hier, r):
         P = lv[l]["P"]
         return lv[l]["Dinv"] * rl + P @ additive(l + 1, P.T @ rl)
 
+    # the level above an exact coarsest level, when small, cycles twice
+    nn = [(n[0] + 1) * (n[1] + 1) * (n[2] + 1) for n in (v["n"] for v in lv)]
+    w = L - 1 if L >= 2 and hier[L][2] is not None and nn[L - 1] <= 512 else -1
+
     def V(l, rl):
         if l == c:
             return additive(l, rl)
         A, S, m, P = lv[l]["A"], lv[l]["S"], lv[l]["solved"], lv[l]["P"]
         x = S * rl
-        res = np.where(m, rl - A @ x, 0.0)
-        x = x + np.where(m, P @ V(l + 1, P.T @ res), 0.0)
-        return x + S * (rl - A @ x)
+        for _ in range(2 if l == w else 1):
+            res = np.where(m, rl - A @ x, 0.0)
+            x = x + np.where(m, P @ V(l + 1, P.T @ res), 0.0)
+            x = x + S * (rl - A @ x)
+        return x
 
-    return V(0, r), c
+    return V(0, r), c, w
 
 
 SHAPES = [(8, 4, 4, 1.0, 1.0, 1.0), (16, 8, 8, 1.0, 0.5, 2.0), (48, 24, 24, 1.0, 1.0, 1.0), (7, 4, 4, 1.0, 1.0, 1.0)]
@@ -106,8 +112,8 @@ def test_vc_apply_matches_sparse_cycle(shape):
     for s in range(2):
         r = np.where(lv[0]["solved"], rng.standard_normal(g.ndof), 0.0)
         z = ox.vc_apply(g, Ke, E, fixed, r, state=state)
-        zr, c = _vcycle_sparse(lv, hier, r)
-        assert c == state["c"]
+        zr, c, w = _vcycle_sparse(lv, hier, r)
+        assert c == state["c"] and w == ox.vc_w_level(state["levels"])
         assert np.allclose(z, zr, rtol=1e-10, atol=1e-10 * np.abs(zr).max())
     if shape[0] == 48:
         assert state["c"] == 3  # three smoothed levels above a coarsest level solved by its diagonal
@@ -140,6 +146,23 @@ def test_vc_symmetric_with_two_smoothed_levels():
     u, v = (np.where(mask, rng.standard_normal(g.ndof), 0.0) for _ in range(2))
     Mu, Mv = ox.vc_apply(g, Ke, E, fixed, u, state=state), ox.vc_apply(g, Ke, E, fixed, v, state=state)
     assert state["c"] == 3
+    assert abs(v @ Mu - u @ Mv) < 1e-10 * abs(v @ Mu)
+    assert u @ Mu > 0 and v @ Mv > 0
+
+
+def test_vc_symmetric_with_the_w_visit():
+    """16x8x8: levels 16x8x8, 8x4x4 (225 nodes, visits the exact 4x2x2 level twice), 4x2x2."""
+    nx, ny, nz = 16, 8, 8
+    g = ox.Grid(nx, ny, nz)
+    rng = np.random.default_rng(6)
+    E = 1e-9 + rng.uniform(0, 1, g.ne) ** 3
+    fixed = _fixed(g)
+    Ke = ox.ke_hex(1, 1, 1, 0.3)
+    state = {}
+    mask = ox.ml_hierarchy(g, Ke, E, fixed, 0)[0][1].ravel() != 0
+    u, v = (np.where(mask, rng.standard_normal(g.ndof), 0.0) for _ in range(2))
+    Mu, Mv = ox.vc_apply(g, Ke, E, fixed, u, state=state), ox.vc_apply(g, Ke, E, fixed, v, state=state)
+    assert state["c"] == 2 and ox.vc_w_level(state["levels"]) == 1
     assert abs(v @ Mu - u @ Mv) < 1e-10 * abs(v @ Mu)
     assert u @ Mu > 0 and v @ Mv > 0
 
