@@ -304,7 +304,9 @@ __device__ __forceinline__ double lane_from_left(double v) {
 //   z onto every fine plane of its chunk over the tile's coarse footprint, in
 //   dynamic LDS ((kchunk + 2) planes, smoothing_post_kchunk); the march then
 //   reads the four xy corners of each staged node.
-// Forms 3 and 4 return at once when a.st is set and its solve is done.
+// 6 (residual of a given iterate) y = r - K x where S != 0 (0 elsewhere), x = a.x;
+// 7 (post-smoothing from a given iterate) x = a.x + mask(P e), then y as in 4.
+// Forms 3, 4, 6 and 7 return at once when a.st is set and its solve is done.
 // (the FP32 post-smoothing form with one plane in flight is held to 128 VGPRs:
 // four waves per SIMD instead of three, without spills)
 template <typename T, int TX, int TY, int FUSE, int PD>
@@ -321,7 +323,8 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
     __shared__ T ex[6][NT];      // y-exchange: row ty+1 half of each face's y-inverse
     constexpr bool POUT = FUSE == 1 || FUSE == 2;  // forms that write the staged vector to p_out
     constexpr bool SMOOTH = FUSE >= 3;
-    constexpr bool PRO = FUSE == 4;  // fused prolongation of a coarse correction
+    constexpr bool PRO = FUSE == 4 || FUSE == 7;  // fused prolongation of a coarse correction
+    constexpr bool XB = FUSE == 6 || FUSE == 7;   // smoothing forms from a given iterate a.x
     // coarse footprint of the tile's node columns (i0-1 .. i0-1+TX, j0-1 .. j0-1+TY)
     constexpr int CPX = TX / 2 + 2, CPY = TY / 2 + 2, CVALS = 3 * CPX * CPY;
     // form 4: the z-interpolated coarse correction, fine plane f of the chunk at [f CVALS + (c CPY + ly) CPX + lx]
@@ -330,7 +333,7 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
     __shared__ T so[POUT ? 2 : 1][OVALS];  // results of the owned nodes, node-major, for row-contiguous stores
     __shared__ double red[NT / 64];
 
-    if constexpr (!SMOOTH) a.x += (long)blockIdx.y * a.cs;  // the smoothing forms read no x
+    if constexpr (!SMOOTH || XB) a.x += (long)blockIdx.y * a.cs;  // forms 3 and 4 read no x
     a.y += (long)blockIdx.y * a.cs;
     if constexpr (FUSE) a.r += (long)blockIdx.y * a.cs;
     if constexpr (POUT) a.p_out += (long)blockIdx.y * a.cs;
@@ -479,6 +482,8 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
                 else if constexpr (FUSE == 3) v = q.m[c] * q.r[c];              // x = S r
                 else if constexpr (FUSE == 4)                                     // x = S r + mask(P e)
                     v = q.m[c] * q.r[c] + (q.m[c] != T(0) ? coarse_at(cn_own, c, q.k) : T(0));
+                else if constexpr (FUSE == 7)                                     // x = x0 + mask(P e)
+                    v = q.x[c] + (q.m[c] != T(0) ? coarse_at(cn_own, c, q.k) : T(0));
                 else v = q.x[c];
                 xs[c][ty][tx] = v;
                 xnew[c] = v;
@@ -493,6 +498,9 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
             else if constexpr (FUSE == 3) v = q.qm * q.qr;
             else if constexpr (FUSE == 4) {
                 v = q.qm * q.qr;
+                if (xstore && q.qm != T(0)) v += coarse_at(cn_x, xc, q.k);
+            } else if constexpr (FUSE == 7) {
+                v = q.qx;
                 if (xstore && q.qm != T(0)) v += coarse_at(cn_x, xc, q.k);
             }
             else v = q.qx;
@@ -596,18 +604,18 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
             for (int c = 0; c < 3; ++c) {
                 const T yv = SMOOTH ? node[c] : node[c] + a.shift * xo[c];  // the level operators carry no shift
                 T out = yv;
-                if constexpr (FUSE == 3) out = mo[c] != T(0) ? ro[c] - yv : T(0);  // residual after x = S r
-                else if constexpr (FUSE == 4) out = xo[c] + mo[c] * (ro[c] - yv);  // x + S (r - K x)
+                if constexpr (FUSE == 3 || FUSE == 6) out = mo[c] != T(0) ? ro[c] - yv : T(0);  // residual
+                else if constexpr (FUSE == 4 || FUSE == 7) out = xo[c] + mo[c] * (ro[c] - yv);  // x + S (r - K x)
                 if (tx >= 1 && ty >= 1) {
                     so[0][so_i + c] = out;
                     if constexpr (POUT) so[POUT ? 1 : 0][so_i + c] = xo[c];
                 }
-                if constexpr (FUSE == 4) {
+                if constexpr (FUSE == 4 || FUSE == 7) {
                     if (ke >= k0 && own) {
                         pq += (double)ro[c] * (double)out;
                         if (mo[c] != T(0)) rr += (double)ro[c] * (double)ro[c];
                     }
-                } else if constexpr (FUSE != 3) {  // pre-smoothing reduces nothing
+                } else if constexpr (FUSE != 3 && FUSE != 6) {  // the residual forms reduce nothing
                     if (ke >= k0 && own) pq += (double)xo[c] * (double)yv;
                 }
             }
@@ -637,10 +645,10 @@ __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ?
         flush();
     }
 
-    if (FUSE != 3 && (a.mode == MV_PCG || (FUSE == 4 && a.partials != nullptr))) {  // per-block p'q, or r'y
+    if (FUSE != 3 && FUSE != 6 && (a.mode == MV_PCG || (PRO && a.partials != nullptr))) {  // p'q, or r'y
         const double v = block_sum<NT>(pq, red);
         if (threadIdx.x == 0) a.partials[blockIdx.x] = v;
-        if constexpr (FUSE == 4) {  // and r'r after all blocks' r'y
+        if constexpr (PRO) {  // and r'r after all blocks' r'y
             const double w = block_sum<NT>(rr, red);
             if (threadIdx.x == 0) a.partials[gridDim.x + blockIdx.x] = w;
         }
@@ -802,12 +810,14 @@ int launch_sf(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, const 
               int nrhs) {
     MvShape sh = sh_in;
     size_t dyn = 0;
-    if (a.form == 4) {  // the z-interpolated coarse correction of the chunk, (kchunk + 2) footprints
+    if (a.form == 4 || a.form == 7) {  // the z-interpolated coarse correction of the chunk, (kchunk + 2) footprints
         sh.kchunk = smoothing_post_kchunk(sh.tile, sh.kchunk, (int)sizeof(T));
         dyn = (size_t)(sh.kchunk + 2) * post_footprint<TX, TY>() * sizeof(T);
         static bool lds_set = false;  // first use: the start of a solve, outside graph capture
         if (!lds_set) {
             (void)hipFuncSetAttribute((const void*)k_matvec_sf<T, TX, TY, 4, PD>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPostLdsBytes);
+            (void)hipFuncSetAttribute((const void*)k_matvec_sf<T, TX, TY, 7, PD>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPostLdsBytes);
             lds_set = true;
         }
@@ -820,6 +830,12 @@ int launch_sf(const GridArgs& g, const ParityK<T>& K, const MvArgs<T>& a, const 
                            ntx, nty, nt);
     else if (a.form == 4)
         hipLaunchKernelGGL((k_matvec_sf<T, TX, TY, 4, PD>), dim3(nb, nrhs), dim3(TX * TY), dyn, s, g, K, a, sh.kchunk,
+                           ntx, nty, nt);
+    else if (a.form == 6)
+        hipLaunchKernelGGL((k_matvec_sf<T, TX, TY, 6, PD>), dim3(nb, nrhs), dim3(TX * TY), 0, s, g, K, a, sh.kchunk,
+                           ntx, nty, nt);
+    else if (a.form == 7)
+        hipLaunchKernelGGL((k_matvec_sf<T, TX, TY, 7, PD>), dim3(nb, nrhs), dim3(TX * TY), dyn, s, g, K, a, sh.kchunk,
                            ntx, nty, nt);
     else if (a.p_out != nullptr && a.minv == nullptr)
         hipLaunchKernelGGL((k_matvec_sf<T, TX, TY, 2, PD>), dim3(nb, nrhs), dim3(TX * TY), 0, s, g, K, a, sh.kchunk,

@@ -39,12 +39,14 @@ struct MvArgs {
     int defer_alpha = 0;
     // parity contraction, MV_PLAIN: 3 = pre-smoothing (x = S r, y = masked
     // r - K x), 4 = post-smoothing (x = S r + mask(P ec), y = x + S (r - K x),
-    // r'y partials when partials is set); S in minv; st (if set) skips a
-    // finished solve
+    // r'y partials when partials is set); 6 = residual of the iterate in x
+    // (y = masked r - K x), 7 = post-smoothing from the iterate in x
+    // (x + mask(P ec), then as 4); S in minv; st (if set) skips a finished
+    // solve
     int form = 0;
-    const T* ec = nullptr;         // form 4: coarse correction on the next level's node grid
+    const T* ec = nullptr;         // forms 4, 7: coarse correction on the next level's node grid
     int cNX = 0, cNY = 0, cNZ = 0;  // its node counts
-    long ccs = 0;                   // form 4: column stride of ec (multi-RHS)
+    long ccs = 0;                   // forms 4, 7: column stride of ec (multi-RHS)
 };
 
 // tile: 0 -> 16x16 element columns (default), 1 -> 16x32, 2 -> 32x8.

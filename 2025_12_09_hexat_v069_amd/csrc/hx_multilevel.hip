@@ -1113,30 +1113,53 @@ int launch_vc_apply(const MlHierarchy<T>& h, const ParityK<T>& K, const MvShape&
         launch_matvec_parity<T>(h.g[l], K, a, level_shape(l, sh_pre), s, nrhs);
         if (l + 1 < c) restrict_to(l + 1, false);
     }
-    if (s0 <= L) {  // levels s0..L in one workgroup per column, from vres[s0-1]
+    // levels s0..L in one workgroup per column, from vres[s0-1] (the first launch also does x += alpha p)
+    auto coarse_pass = [&](bool with_x) {
         MlHierarchy<T> hc = h;
         hc.r[s0 - 1] = h.vres[s0 - 1];
         const bool stage = h.ainv != nullptr &&
                            vc_coarse_lds_values(h, s0, true) * (long)sizeof(T) <= kCoarseLdsBytes;
-        const bool xupd = xup != nullptr && pup != nullptr && st != nullptr;
+        const bool xupd = with_x && xup != nullptr && pup != nullptr && st != nullptr;
         hipLaunchKernelGGL((k_vc_coarse<T>), dim3(xupd ? 1 + kCoarseXBlocks : 1, nrhs), dim3(kCoarseThreads),
                            vc_coarse_lds_values(h, s0, stage) * sizeof(T), s, hc, s0, K, stage ? 1 : 0, st,
                            xupd ? xup : nullptr, xupd ? pup : nullptr, 3 * h.g[0].nn);
-    } else {  // no level fits it: the coarsest level by its diagonal
-        restrict_to(L, true);
-    }
+    };
+    if (s0 <= L) coarse_pass(true);
+    else restrict_to(L, true);  // no level fits it: the coarsest level by its diagonal
+    // The last multi-block level visits the pass twice when the pass holds the
+    // level above an exact coarsest level (which visits that one twice itself)
+    // and it is not the fine level: after its post-smoothing, its residual
+    // again (form 6), the pass, and a post-smoothing from its iterate (form 7)
+    // into vres, which the level below then prolongates.
+    const int w2 = (h.ainv != nullptr && s0 == L - 1 && c - 1 >= 1) ? c - 1 : -1;
     int nb0 = 0;
     for (int l = c - 1; l >= 0; --l) {  // post-smoothing with the prolongation of the correction fused
         MvArgs<T> a{nullptr, l == 0 ? z : h.y[l], h.Eop[l], T(0), MV_PLAIN, st, l == 0 ? rz_partials : nullptr,
                     3 * h.g[l].nn, level_r(l), h.sinv[l], nullptr};
         a.form = 4;
-        a.ec = h.y[l + 1];
+        a.ec = l + 1 == w2 ? h.vres[w2] : h.y[l + 1];
         a.cNX = h.g[l + 1].NX;
         a.cNY = h.g[l + 1].NY;
         a.cNZ = h.g[l + 1].NZ;
         a.ccs = 3 * h.g[l + 1].nn;
         const int nb = launch_matvec_parity<T>(h.g[l], K, a, level_shape(l, sh_post), s, nrhs);
         if (l == 0) nb0 = nb;
+        if (l == w2) {
+            MvArgs<T> b{h.y[l], h.vres[l], h.Eop[l], T(0), MV_PLAIN, st, nullptr, 3 * h.g[l].nn, level_r(l),
+                        h.sinv[l], nullptr};
+            b.form = 6;  // vres = mask(r - A y)
+            launch_matvec_parity<T>(h.g[l], K, b, level_shape(l, sh_pre), s, nrhs);
+            coarse_pass(false);  // y[l+1]: the second correction
+            MvArgs<T> d{h.y[l], h.vres[l], h.Eop[l], T(0), MV_PLAIN, st, nullptr, 3 * h.g[l].nn, level_r(l),
+                        h.sinv[l], nullptr};
+            d.form = 7;  // vres = y + mask(P y[l+1]) + S (r - A (...))
+            d.ec = h.y[l + 1];
+            d.cNX = h.g[l + 1].NX;
+            d.cNY = h.g[l + 1].NY;
+            d.cNZ = h.g[l + 1].NZ;
+            d.ccs = 3 * h.g[l + 1].nn;
+            launch_matvec_parity<T>(h.g[l], K, d, level_shape(l, sh_post), s, nrhs);
+        }
     }
     // r'r from the update's partials (unmasked update) or from the fine
     // post-smoothing operator's, after its r'z (free update)

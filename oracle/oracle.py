@@ -766,25 +766,28 @@ def _additive_from(levels, c, rc):
     return y
 
 
-VC_W_NODES = 512  # the level above an exact coarsest level cycles twice when it has at most this many nodes
+VC_W_NODES = 512  # levels of at most this many nodes: the library's one-workgroup pass
 
 
-def vc_w_level(levels):
-    """The level that visits the coarsest twice: L-1 when the coarsest level L is solved exactly, L-1 >= 1
-    and level L-1 has at most VC_W_NODES nodes (the library's one-workgroup pass holds it); -1 otherwise."""
+def vc_w_levels(levels):
+    """The levels that visit the next coarser level twice: L-1 when the coarsest level L is solved
+    exactly, L-1 >= 1 and level L-1 has at most VC_W_NODES nodes (the library's one-workgroup pass holds
+    it); with it L-2 when L-2 >= 1 and it has more (the library's last multi-block level)."""
     L = len(levels) - 1
     if L < 2 or levels[L][2] is None:
-        return -1
-    n = levels[L - 1][0]
-    return L - 1 if (n[0] + 1) * (n[1] + 1) * (n[2] + 1) <= VC_W_NODES else -1
+        return set()
+    nn = [(n[0] + 1) * (n[1] + 1) * (n[2] + 1) for n, *_ in levels]
+    if nn[L - 1] > VC_W_NODES:
+        return set()
+    return {L - 1} | ({L - 2} if L - 2 >= 1 and nn[L - 2] > VC_W_NODES else set())
 
 
 def vc_apply(g: Grid, Ke, Ee, fixed, r, omega=1.95, max_levels=12, exact=True, state=None):
     """z = M^-1 r of the V-cycle form: on levels l < L (vc_levels), with S_l = vc_smoother and A_l
     the level operator (moduli 2^l E_l averaged over children, the fine Ke),
       x = S r;  res = mask(r - A x);  e = V_{l+1}(R res);  x += mask(P e);  z = x + S (r - A x)
-    and the coarsest level L by its inverse (exact) or its diagonal (_additive_from at L).  The level
-    vc_w_level names repeats the correction and the smoothing after it once (smoothing, correction,
+    and the coarsest level L by its inverse (exact) or its diagonal (_additive_from at L).  The levels
+    vc_w_levels names repeat the correction and the smoothing after it once (smoothing, correction,
     smoothing, correction, smoothing: still symmetric).  `state` caches the levels between calls."""
     Ke = _f64(Ke)
     if state is None or "levels" not in state:
@@ -813,14 +816,14 @@ def vc_apply(g: Grid, Ke, Ee, fixed, r, omega=1.95, max_levels=12, exact=True, s
         n = levels[l][0]
         return matvec(Grid(*n), Ke, st["E"][l], x.ravel()).reshape(x.shape)
 
-    w = vc_w_level(levels)
+    w = vc_w_levels(levels)
 
     def V(l, rl):
         if l == c:
             return _additive_from(levels, c, rl)
         S = st["S"][l]
         x = S * rl
-        for _ in range(2 if l == w else 1):
+        for _ in range(2 if l in w else 1):
             res = np.where(S != 0.0, rl - A(l, x), 0.0)
             e = V(l + 1, _restrict3(res))
             x = x + np.where(S != 0.0, _prolong3(e), 0.0)

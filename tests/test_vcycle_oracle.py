@@ -58,16 +58,19 @@ def _vcycle_sparse(lv, hier, r):
         P = lv[l]["P"]
         return lv[l]["Dinv"] * rl + P @ additive(l + 1, P.T @ rl)
 
-    # the level above an exact coarsest level, when small, cycles twice
+    # the level above an exact coarsest level, when small, visits it twice, and so does the level
+    # above that when it is not small (and not the fine level)
     nn = [(n[0] + 1) * (n[1] + 1) * (n[2] + 1) for n in (v["n"] for v in lv)]
-    w = L - 1 if L >= 2 and hier[L][2] is not None and nn[L - 1] <= 512 else -1
+    w = set()
+    if L >= 2 and hier[L][2] is not None and nn[L - 1] <= 512:
+        w = {L - 1} | ({L - 2} if L - 2 >= 1 and nn[L - 2] > 512 else set())
 
     def V(l, rl):
         if l == c:
             return additive(l, rl)
         A, S, m, P = lv[l]["A"], lv[l]["S"], lv[l]["solved"], lv[l]["P"]
         x = S * rl
-        for _ in range(2 if l == w else 1):
+        for _ in range(2 if l in w else 1):
             res = np.where(m, rl - A @ x, 0.0)
             x = x + np.where(m, P @ V(l + 1, P.T @ res), 0.0)
             x = x + S * (rl - A @ x)
@@ -76,7 +79,8 @@ def _vcycle_sparse(lv, hier, r):
     return V(0, r), c, w
 
 
-SHAPES = [(8, 4, 4, 1.0, 1.0, 1.0), (16, 8, 8, 1.0, 0.5, 2.0), (48, 24, 24, 1.0, 1.0, 1.0), (7, 4, 4, 1.0, 1.0, 1.0)]
+SHAPES = [(8, 4, 4, 1.0, 1.0, 1.0), (16, 8, 8, 1.0, 0.5, 2.0), (48, 24, 24, 1.0, 1.0, 1.0), (7, 4, 4, 1.0, 1.0, 1.0),
+          (32, 16, 16, 1.0, 1.0, 1.0)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
@@ -113,10 +117,12 @@ def test_vc_apply_matches_sparse_cycle(shape):
         r = np.where(lv[0]["solved"], rng.standard_normal(g.ndof), 0.0)
         z = ox.vc_apply(g, Ke, E, fixed, r, state=state)
         zr, c, w = _vcycle_sparse(lv, hier, r)
-        assert c == state["c"] and w == ox.vc_w_level(state["levels"])
+        assert c == state["c"] and w == ox.vc_w_levels(state["levels"])
         assert np.allclose(z, zr, rtol=1e-10, atol=1e-10 * np.abs(zr).max())
     if shape[0] == 48:
         assert state["c"] == 3  # three smoothed levels above a coarsest level solved by its diagonal
+    if shape[0] == 32:  # levels 32x16x16, 16x8x8 (1377 nodes), 8x4x4 (225), 4x2x2 exact
+        assert state["c"] == 3 and ox.vc_w_levels(state["levels"]) == {1, 2}
 
 
 def test_vc_dense_form_is_spd():
@@ -162,7 +168,7 @@ def test_vc_symmetric_with_the_w_visit():
     mask = ox.ml_hierarchy(g, Ke, E, fixed, 0)[0][1].ravel() != 0
     u, v = (np.where(mask, rng.standard_normal(g.ndof), 0.0) for _ in range(2))
     Mu, Mv = ox.vc_apply(g, Ke, E, fixed, u, state=state), ox.vc_apply(g, Ke, E, fixed, v, state=state)
-    assert state["c"] == 2 and ox.vc_w_level(state["levels"]) == 1
+    assert state["c"] == 2 and ox.vc_w_levels(state["levels"]) == {1}
     assert abs(v @ Mu - u @ Mv) < 1e-10 * abs(v @ Mu)
     assert u @ Mu > 0 and v @ Mv > 0
 
