@@ -308,7 +308,8 @@ __device__ __forceinline__ double lane_from_left(double v) {
 // 7 (post-smoothing from a given iterate) x = a.x + mask(P e), then y as in 4.
 // Forms 3, 4, 6 and 7 return at once when a.st is set and its solve is done.
 // (the FP32 post-smoothing form with one plane in flight is held to 128 VGPRs:
-// four waves per SIMD instead of three, without spills)
+// four waves per SIMD instead of three; the compiler spills two of them, 12 B
+// of scratch per lane)
 template <typename T, int TX, int TY, int FUSE, int PD>
 __global__ __launch_bounds__(TX * TY, (FUSE == 4 && sizeof(T) == 4 && PD == 1) ? 4 : 1) void k_matvec_sf(
     GridArgs g, ParityK<T> K, MvArgs<T> a, int kchunk, int ntx, int nty, int ntiles) {
