@@ -661,8 +661,9 @@ def main():
         if main_form.endswith("_vc"):
             sl = smoothed_levels(nx, ny, nz, 8 if fp64_main else 4)
             by = vc_apply_bytes(nx, ny, nz, 8 if fp64_main else 4, sl)
-            kn = ("k_matvec_sf forms 3 and 4, k_ml_restrict_zm, k_vc_coarse, k_fin_rz "
-                  f"({sl} smoothed levels, one application)")
+            kn = ("k_matvec_sf forms 3, 4, 6 and 7, k_ml_restrict_zm, k_vc_coarse, k_fin_rz "
+                  f"({sl} smoothed levels, W visits on the two above an exact coarsest level; one application; "
+                  "bytes of a single visit per level)")
         else:
             by = ml_apply_bytes(nx, ny, nz, 8 if fp64_main else 4)
             kn = "k_ml_restrict_zm, k_ml_coarse, k_ml_prolong, k_fin_rz (one application)"
